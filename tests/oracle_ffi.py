@@ -1,0 +1,308 @@
+"""ctypes bindings to the oracle (the CPU restatement of the reference
+prover under ``oracle/``). Test infrastructure: only tests, ``smoke()`` and the
+benchmark's CPU-baseline leg load it, always as the checker."""
+import ctypes
+import os
+import pathlib
+
+ROOT = pathlib.Path(__file__).resolve().parent.parent
+LIB_PATH = ROOT / "oracle" / "build" / "liboracle.so"
+
+u8p = ctypes.POINTER(ctypes.c_uint8)
+u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class AirParams(ctypes.Structure):
+    """Layout shared with ``z0p_air_params`` (include/z0p.h)."""
+
+    _fields_ = [
+        ("n_steps", ctypes.c_uint64),
+        ("blowup", ctypes.c_uint64),
+        ("num_queries", ctypes.c_uint64),
+        ("final_size", ctypes.c_uint64),
+        ("grinding_bits", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
+        ("cap_height", ctypes.c_uint64),
+        ("a0", ctypes.c_uint32),
+        ("b0", ctypes.c_uint32),
+        ("b_last", ctypes.c_uint32),
+        ("reserved1", ctypes.c_uint32),
+    ]
+
+    def shape(self):
+        return "B%dQ%dG%dC%dF%d" % (self.blowup, self.num_queries, self.grinding_bits, self.cap_height,
+                                     self.final_size)
+
+
+class FriParams(ctypes.Structure):
+    """Layout shared with ``z0p_fri_params`` (include/z0p.h)."""
+
+    _fields_ = [
+        ("blowup", ctypes.c_uint64),
+        ("num_queries", ctypes.c_uint64),
+        ("final_size", ctypes.c_uint64),
+        ("grinding_bits", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
+        ("cap_height", ctypes.c_uint64),
+    ]
+
+    def shape(self):
+        return "B%dQ%dG%dC%dF%d" % (self.blowup, self.num_queries, self.grinding_bits, self.cap_height,
+                                     self.final_size)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError("oracle library missing: build with `make -C oracle` (or __graft_entry__.build())")
+    L = ctypes.CDLL(str(LIB_PATH))
+    sz = ctypes.c_size_t
+    vp = ctypes.c_void_p
+    sig = {
+        "oracle_set_threads": (None, [sz]),
+        "oracle_get_threads": (sz, []),
+        "oracle_free": (None, [vp]),
+        "oracle_blake3": (None, [u8p, sz, u8p]),
+        "oracle_blake3_small": (None, [u8p, sz, u8p]),
+        "oracle_blake3_leaf64": (None, [u8p, u8p]),
+        "oracle_blake3_node": (None, [u8p, u8p, u8p]),
+        "oracle_blake3_split": (None, [u8p, sz, ctypes.POINTER(sz), sz, u8p]),
+        "oracle_field_mul": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
+        "oracle_field_inv": (ctypes.c_uint32, [ctypes.c_uint32]),
+        "oracle_field_pow": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint64]),
+        "oracle_field_halve": (ctypes.c_uint32, [ctypes.c_uint32]),
+        "oracle_two_adic_generator": (ctypes.c_int, [ctypes.c_uint32, u32p]),
+        "oracle_ext4_mul": (None, [u32p, u32p, u32p]),
+        "oracle_ext4_inv": (None, [u32p, u32p]),
+        "oracle_ext4_pow": (None, [u32p, ctypes.c_uint64, ctypes.c_uint64, u32p]),
+        "oracle_ntt_forward": (ctypes.c_int, [u32p, sz]),
+        "oracle_ntt_inverse": (ctypes.c_int, [u32p, sz]),
+        "oracle_ntt_evaluate": (ctypes.c_int, [u32p, sz, ctypes.c_uint32, u32p]),
+        "oracle_evaluate_on_coset": (ctypes.c_int, [u32p, sz, sz, u32p]),
+        "oracle_merkle_flat64": (ctypes.c_int, [u8p, sz, u8p]),
+        "oracle_fold_values": (None, [u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p]),
+        "oracle_point_at": (ctypes.c_int, [sz, sz, sz, u32p]),
+        "oracle_linked_lde": (ctypes.c_int, [u32p, sz, sz, ctypes.c_uint32, ctypes.c_uint32, u32p]),
+        "oracle_compose": (ctypes.c_int, [u32p, u32p, sz, sz, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                          u32p, ctypes.c_int, u32p]),
+        "oracle_transcript_new": (vp, [u8p, sz]),
+        "oracle_transcript_clone": (vp, [vp]),
+        "oracle_transcript_free": (None, [vp]),
+        "oracle_transcript_absorb": (None, [vp, ctypes.c_char_p, u8p, sz]),
+        "oracle_transcript_challenge_bytes": (None, [vp, ctypes.c_char_p, u8p]),
+        "oracle_transcript_challenge_ext": (None, [vp, ctypes.c_char_p, u32p]),
+        "oracle_transcript_challenge_index": (ctypes.c_int, [vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)]),
+        "oracle_transcript_grind": (ctypes.c_uint64, [vp, ctypes.c_char_p, ctypes.c_uint32]),
+        "oracle_transcript_verify_grind": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32]),
+        "oracle_transcript_history_digest": (None, [vp, u8p]),
+        "oracle_grind_head": (ctypes.c_uint64, [u8p, sz, ctypes.c_uint32]),
+        "oracle_air_params_for_security": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                                          ctypes.c_uint32, ctypes.c_uint32,
+                                                          ctypes.POINTER(AirParams)]),
+        "oracle_air_params_new": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                                 ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                 ctypes.POINTER(AirParams)]),
+        "oracle_fri_params_for_security": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32,
+                                                          ctypes.POINTER(FriParams)]),
+        "oracle_fib_b_last": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64]),
+        "oracle_build_trace": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, u32p, u32p]),
+        "oracle_air_prove": (ctypes.c_int, [ctypes.POINTER(AirParams), vp, ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+        "oracle_air_prove_with_trace": (ctypes.c_int, [ctypes.POINTER(AirParams), u32p, u32p, vp,
+                                                       ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+        "oracle_air_verify": (ctypes.c_int, [ctypes.POINTER(AirParams), vp, u8p, sz]),
+        "oracle_fri_commit_open": (ctypes.c_int, [u32p, sz, ctypes.POINTER(FriParams), vp, ctypes.POINTER(u8p),
+                                                  ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+        "oracle_fri_verify": (ctypes.c_int, [u8p, sz, u8p, sz, ctypes.POINTER(FriParams), vp]),
+        "oracle_stage_timers_clear": (None, []),
+        "oracle_stage_timers": (sz, [ctypes.c_char_p, sz]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _buf(data: bytes):
+    return (ctypes.c_uint8 * max(len(data), 1)).from_buffer_copy(data if data else b"\0")
+
+
+def _take(ptr, n):
+    out = ctypes.string_at(ptr, n)
+    lib().oracle_free(ptr)
+    return out
+
+
+def u32_array(values):
+    arr = (ctypes.c_uint32 * max(len(values), 1))()
+    for i, v in enumerate(values):
+        arr[i] = v
+    return arr
+
+
+def set_threads(n):
+    lib().oracle_set_threads(n)
+
+
+def blake3(data: bytes) -> bytes:
+    out = (ctypes.c_uint8 * 32)()
+    lib().oracle_blake3(_buf(data), len(data), out)
+    return bytes(out)
+
+
+def blake3_split(data: bytes, splits) -> bytes:
+    out = (ctypes.c_uint8 * 32)()
+    arr = (ctypes.c_size_t * max(len(splits), 1))(*splits)
+    lib().oracle_blake3_split(_buf(data), len(data), arr, len(splits), out)
+    return bytes(out)
+
+
+def blake3_small(data: bytes) -> bytes:
+    out = (ctypes.c_uint8 * 32)()
+    lib().oracle_blake3_small(_buf(data), len(data), out)
+    return bytes(out)
+
+
+def leaf64(block: bytes) -> bytes:
+    assert len(block) == 64
+    out = (ctypes.c_uint8 * 32)()
+    lib().oracle_blake3_leaf64(_buf(block), out)
+    return bytes(out)
+
+
+def node(left: bytes, right: bytes) -> bytes:
+    out = (ctypes.c_uint8 * 32)()
+    lib().oracle_blake3_node(_buf(left), _buf(right), out)
+    return bytes(out)
+
+
+class Transcript:
+    """Oracle transcript (transcript.rs:54-171)."""
+
+    def __init__(self, label: bytes, _handle=None):
+        self.h = _handle if _handle is not None else lib().oracle_transcript_new(_buf(label), len(label))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_transcript_free(self.h)
+            self.h = None
+
+    def clone(self):
+        return Transcript(b"", lib().oracle_transcript_clone(self.h))
+
+    def absorb(self, label: bytes, data: bytes):
+        lib().oracle_transcript_absorb(self.h, label, _buf(data), len(data))
+
+    def challenge_bytes(self, label: bytes) -> bytes:
+        out = (ctypes.c_uint8 * 32)()
+        lib().oracle_transcript_challenge_bytes(self.h, label, out)
+        return bytes(out)
+
+    def challenge_ext(self, label: bytes):
+        out = (ctypes.c_uint32 * 4)()
+        lib().oracle_transcript_challenge_ext(self.h, label, out)
+        return list(out)
+
+    def challenge_index(self, label: bytes, domain: int):
+        out = ctypes.c_size_t()
+        rc = lib().oracle_transcript_challenge_index(self.h, label, domain, ctypes.byref(out))
+        return None if rc else out.value
+
+    def grind(self, label: bytes, bits: int) -> int:
+        return lib().oracle_transcript_grind(self.h, label, bits)
+
+    def verify_grind(self, label: bytes, nonce: int, bits: int) -> bool:
+        return bool(lib().oracle_transcript_verify_grind(self.h, label, nonce, bits))
+
+    def history_digest(self) -> bytes:
+        out = (ctypes.c_uint8 * 32)()
+        lib().oracle_transcript_history_digest(self.h, out)
+        return bytes(out)
+
+
+def air_params_for_security(n, a0, b0, b_last, bits=100):
+    p = AirParams()
+    rc = lib().oracle_air_params_for_security(n, a0, b0, b_last, bits, ctypes.byref(p))
+    if rc:
+        raise ValueError("oracle params error %d" % rc)
+    return p
+
+
+def air_params_new(n, blowup, nq, final_size, a0, b0, b_last):
+    p = AirParams()
+    rc = lib().oracle_air_params_new(n, blowup, nq, final_size, a0, b0, b_last, ctypes.byref(p))
+    if rc:
+        raise ValueError("oracle params error %d" % rc)
+    return p
+
+
+def fri_params_for_security(poly_len, bits=100):
+    p = FriParams()
+    rc = lib().oracle_fri_params_for_security(poly_len, bits, ctypes.byref(p))
+    if rc:
+        raise ValueError("oracle params error %d" % rc)
+    return p
+
+
+def fib_b_last(a0, b0, n):
+    return lib().oracle_fib_b_last(a0, b0, n)
+
+
+def air_prove(params: AirParams, transcript: Transcript) -> bytes:
+    out = u8p()
+    n = ctypes.c_size_t()
+    rc = lib().oracle_air_prove(ctypes.byref(params), transcript.h, ctypes.byref(out), ctypes.byref(n))
+    if rc:
+        raise ValueError("oracle prove error %d" % rc)
+    return _take(out, n.value)
+
+
+def air_prove_with_trace(params: AirParams, col_a, col_b, transcript: Transcript) -> bytes:
+    out = u8p()
+    n = ctypes.c_size_t()
+    rc = lib().oracle_air_prove_with_trace(ctypes.byref(params), u32_array(col_a), u32_array(col_b), transcript.h,
+                                           ctypes.byref(out), ctypes.byref(n))
+    if rc:
+        raise ValueError("oracle prove error %d" % rc)
+    return _take(out, n.value)
+
+
+def air_verify(params: AirParams, transcript: Transcript, proof: bytes) -> int:
+    return lib().oracle_air_verify(ctypes.byref(params), transcript.h, _buf(proof), len(proof))
+
+
+def fri_commit_open(coeffs, params: FriParams, transcript: Transcript):
+    cm, pf = u8p(), u8p()
+    cn, pn = ctypes.c_size_t(), ctypes.c_size_t()
+    rc = lib().oracle_fri_commit_open(u32_array(coeffs), len(coeffs), ctypes.byref(params), transcript.h,
+                                      ctypes.byref(cm), ctypes.byref(cn), ctypes.byref(pf), ctypes.byref(pn))
+    if rc:
+        raise ValueError("oracle fri error %d" % rc)
+    return _take(cm, cn.value), _take(pf, pn.value)
+
+
+def fri_verify(commitment: bytes, proof: bytes, params: FriParams, transcript: Transcript) -> int:
+    return lib().oracle_fri_verify(_buf(commitment), len(commitment), _buf(proof), len(proof), ctypes.byref(params),
+                                   transcript.h)
+
+
+def stage_timers():
+    n = lib().oracle_stage_timers(None, 0)
+    buf = ctypes.create_string_buffer(n)
+    lib().oracle_stage_timers(buf, n)
+    out = {}
+    for part in buf.value.decode().split(";"):
+        if part:
+            k, v = part.split("=")
+            out[k] = float(v)
+    return out
+
+
+def clear_stage_timers():
+    lib().oracle_stage_timers_clear()
