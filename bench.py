@@ -1,0 +1,151 @@
+"""Benchmark: 2^20-step Fibonacci AIR prove, production schedule
+(B16 Q40 G20 C4 F16), on one MI355X; prints one JSON line.
+
+A step is one complete `air.prove` call (trace generation, INTT, coset LDE,
+linked column, trace Merkle commit, composition, FRI commit/folds, 20-bit
+proof-of-work, openings and wire serialisation) for the canonical reference
+instance a0 = b0 = 1 under the transcript label b"bench/air-scale"
+(benches/speed_size.rs:719-754). Inputs are the 64-byte parameters; all bulk
+data is produced and kept in HBM, so no host buffers enter the timed region.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+N_STEPS = 1 << 20
+LABEL = b"bench/air-scale"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(n, m, layers):
+    """Per-kernel-family algorithmic HBM bytes for one proof (1 read + 1
+    write of every materialised array)."""
+    fri_leaf = sum(16 * L + 8 * L + 12 * L for L in layers)  # values read, leaf digests, node traffic
+    fold = sum(16 * L + 4 * L for L in layers)
+    return {
+        "intt": 8 * n,
+        "lde": 4 * n + 4 * m,
+        "linked_b": 8 * m,
+        "trace_tree": 8 * m + 4 * m + 4 * m + 2 * m,
+        "compose": 8 * m + 16 * m,
+        "fri_tree": fri_leaf,
+        "fold": fold,
+    }
+
+
+FAMILY = {
+    "intt_small": "intt", "intt_cols": "intt", "intt_rows": "intt",
+    "lde_small": "lde", "lde_rows": "lde", "lde_cols": "lde",
+    "linked_b": "linked_b", "trace_leaves": "trace_tree", "compose": "compose",
+    "fri_leaves": "fri_tree", "fri_leaves_base": "fri_tree", "fold": "fold", "fold_base": "fold",
+}
+
+
+def cpu_baseline(n, repeats):
+    import oracle_ffi as o
+    threads = os.cpu_count() or 1
+    o.set_threads(threads)
+    bl = o.fib_b_last(1, 1, n)
+    p = o.air_params_for_security(n, 1, 1, bl)
+    o.air_prove(p, o.Transcript(LABEL))  # warm-up
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        o.air_prove(p, o.Transcript(LABEL))
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    med = times[len(times) // 2]
+    return {
+        "value": round(med * 1e6 / n, 4),
+        "unit": "us/step",
+        "cores": threads,
+        "kind": "port",
+        "sample": "oracle (C++ restatement of the reference prover) AIR N=%d prod, %d threads, median of %d after 1 warm-up, %.3f s/proof"
+        % (n, threads, repeats, med),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=N_STEPS)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the oracle CPU baseline")
+    ap.add_argument("--cpu-repeats", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: proofs are independent units and this build runs on a single "
+              "MI355X; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import z0p_amd as z
+
+    n = args.n
+    bl = z.air.b_last(1, 1, n)
+    params = z.air.for_security(n, 1, 1, bl)
+    ctx = z.Context(0)
+    proof = None
+    for _ in range(args.warmup):
+        proof = z.air.prove(params, z.Transcript(LABEL), ctx)
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        proof = z.air.prove(params, z.Transcript(LABEL), ctx)
+    t1 = time.perf_counter()
+    ctx.set_profiling(False)
+    stats = ctx.kernel_stats()
+    z.air.verify(params, proof, z.Transcript(LABEL))  # outside the timed region, as the reference bench
+
+    ms = (t1 - t0) * 1000.0 / args.steps
+    m = n * params.blowup
+    layers = []
+    L = m
+    while L > params.final_size:
+        layers.append(L)
+        L //= 4
+    alg = algorithmic_bytes(n, m, layers)
+    fam_time = {}
+    for k, (cnt, tot) in stats.items():
+        f = FAMILY.get(k)
+        if f:
+            fam_time[f] = fam_time.get(f, 0.0) + tot / args.steps
+    dom = max(fam_time, key=fam_time.get) if fam_time else None
+    roofline = None
+    if dom:
+        t_ms = fam_time[dom]
+        achieved = alg[dom] / (t_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "ms_per_proof": round(t_ms, 4), "alg_bytes": alg[dom]}
+    out = {
+        "metric": "us/step, 2^20-step Fib AIR prove",
+        "value": round(ms * 1000.0 / n, 5),
+        "unit": "us/step",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (a0 = b0 = 1, label bench/air-scale)",
+        "config": {"workload": "AIR Fibonacci N=%d production schedule %s (non-ZK)" % (n, params.shape()),
+                   "proof_bytes": len(proof), "proofs_per_s": round(1000.0 / ms, 2)},
+        "kernels_ms_per_proof": {k: round(v[1] / args.steps, 4) for k, v in sorted(stats.items())},
+        "roofline": roofline,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(n, args.cpu_repeats),
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

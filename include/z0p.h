@@ -1,0 +1,139 @@
+/*
+ * z0p-mi355x: C ABI of the MI355X (gfx950) STARK prover for the z0p
+ * Fibonacci AIR and FRI polynomial commitment.
+ *
+ * Drop-in boundary for the reference crate's public proving API
+ * (FutureProofsTech/z0p v0.3.0). Each entry point names the reference item it
+ * replaces. Proofs and commitments are the reference's canonical VERSION = 6
+ * wire bytes (src/codec.rs:21), byte-identical to the reference CPU prover's
+ * output for the same parameters and transcript.
+ *
+ * Conventions
+ *  - Status: 0 = Ok; 1..9 = the reference `Error` variants in declaration
+ *    order (src/error.rs:9-39); >= 100 = device/library errors.
+ *  - Ownership: inputs are borrowed for the call. Byte outputs are allocated
+ *    by the library and released with z0p_free.
+ *  - Transcripts are advanced exactly as the reference advances its
+ *    `&mut Transcript` (src/transcript.rs:37-40): callers may pre-absorb and
+ *    keep using the transcript after a call.
+ *  - A z0p_ctx owns one HIP stream on one device plus its device memory.
+ *    Calls on one ctx are synchronous and must not overlap; use one ctx per
+ *    host thread.
+ *  - Field elements cross the boundary as canonical u32 (< p = 2013265921).
+ */
+#ifndef Z0P_H
+#define Z0P_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    Z0P_OK = 0,
+    Z0P_ERR_EMPTY_INPUT = 1,               /* Error::EmptyInput            error.rs:9  */
+    Z0P_ERR_INDEX_OUT_OF_BOUNDS = 2,       /* Error::IndexOutOfBounds      error.rs:11 */
+    Z0P_ERR_INVALID_DOMAIN_SIZE = 3,       /* Error::InvalidDomainSize     error.rs:18 */
+    Z0P_ERR_INVALID_BLOWUP = 4,            /* Error::InvalidBlowup         error.rs:23 */
+    Z0P_ERR_NO_QUERIES = 5,                /* Error::NoQueries             error.rs:28 */
+    Z0P_ERR_BAD_MERKLE_PATH = 6,           /* Error::BadMerklePath         error.rs:30 */
+    Z0P_ERR_FRI_VERIFICATION_FAILED = 7,   /* Error::FriVerificationFailed error.rs:32 */
+    Z0P_ERR_MALFORMED_INPUT = 8,           /* Error::MalformedInput        error.rs:34 */
+    Z0P_ERR_RNG_UNAVAILABLE = 9,           /* Error::RngUnavailable        error.rs:39 */
+    Z0P_ERR_DEVICE = 100,                  /* HIP runtime failure */
+    Z0P_ERR_OUT_OF_MEMORY = 101,           /* device allocation failed or over budget */
+    Z0P_ERR_INVALID_ARGUMENT = 102,        /* null pointer or bad handle */
+    Z0P_ERR_NO_DEVICE = 103                /* no usable gfx950 device */
+};
+
+/* air::Params (src/air.rs:31-54). a0, b0, b_last are canonical field values. */
+typedef struct z0p_air_params {
+    uint64_t n_steps;
+    uint64_t blowup;
+    uint64_t num_queries;
+    uint64_t final_size;
+    uint32_t grinding_bits;
+    uint32_t reserved0;
+    uint64_t cap_height;
+    uint32_t a0;
+    uint32_t b0;
+    uint32_t b_last;
+    uint32_t reserved1;
+} z0p_air_params;
+
+/* fri::Params (src/fri.rs:105-133). */
+typedef struct z0p_fri_params {
+    uint64_t blowup;
+    uint64_t num_queries;
+    uint64_t final_size;
+    uint32_t grinding_bits;
+    uint32_t reserved0;
+    uint64_t cap_height;
+} z0p_fri_params;
+
+/* Opaque Fiat-Shamir transcript (src/transcript.rs:54-57). */
+typedef struct z0p_transcript z0p_transcript;
+
+/* Opaque device context: one HIP stream, device buffers, cached tables. */
+typedef struct z0p_ctx z0p_ctx;
+
+/* ---- library / context (no reference counterpart: the reference has no device) ---- */
+int z0p_ctx_create(int device, z0p_ctx** out);
+void z0p_ctx_destroy(z0p_ctx* ctx);
+/* Per-kernel HIP-event timing on the ctx stream (off by default). */
+int z0p_ctx_set_profiling(z0p_ctx* ctx, int enabled);
+/* "name=launches,total_ms;..." for the launches since the last reset; returns
+ * the length needed including the terminating NUL. */
+size_t z0p_ctx_kernel_stats(z0p_ctx* ctx, char* buf, size_t cap);
+void z0p_ctx_reset_stats(z0p_ctx* ctx);
+const char* z0p_status_string(int status);
+void z0p_free(void* p);
+
+/* ---- transcript: replaces transcript::Transcript (src/transcript.rs) ---- */
+/* Transcript::new (transcript.rs:62-69). */
+z0p_transcript* z0p_transcript_new(const uint8_t* label, size_t label_len);
+/* Transcript: Clone (transcript.rs:53). */
+z0p_transcript* z0p_transcript_clone(const z0p_transcript* t);
+void z0p_transcript_free(z0p_transcript* t);
+/* Transcript::absorb (transcript.rs:72-77). */
+void z0p_transcript_absorb(z0p_transcript* t, const char* label, const uint8_t* data, size_t len);
+/* Transcript::challenge_bytes (transcript.rs:107-109). */
+void z0p_transcript_challenge_bytes(z0p_transcript* t, const char* label, uint8_t out[32]);
+
+/* ---- parameters ---- */
+/* air::Params::new (air.rs:63-101) with grinding_bits = cap_height = 0. */
+int z0p_air_params_new(uint64_t n_steps, uint64_t blowup, uint64_t num_queries, uint64_t final_size, uint32_t a0,
+                       uint32_t b0, uint32_t b_last, z0p_air_params* out);
+/* air::Params::for_security (air.rs:129-162). */
+int z0p_air_params_for_security(uint64_t n_steps, uint32_t a0, uint32_t b0, uint32_t b_last, uint32_t target_bits,
+                                z0p_air_params* out);
+/* fri::Params::new (fri.rs:141-158) with grinding_bits = cap_height = 0. */
+int z0p_fri_params_new(uint64_t blowup, uint64_t num_queries, uint64_t final_size, z0p_fri_params* out);
+/* fri::Params::for_security (fri.rs:232-255). */
+int z0p_fri_params_for_security(uint64_t poly_len, uint32_t target_bits, z0p_fri_params* out);
+/* air::build_trace(a0, b0, n).1[n-1] (air.rs:341-354): the honest output claim. */
+uint32_t z0p_air_b_last(uint32_t a0, uint32_t b0, uint64_t n_steps);
+
+/* ---- proving: air.rs / fri.rs ---- */
+/* air::prove (air.rs:365-372) followed by Proof::to_bytes (air.rs:262-281). */
+int z0p_air_prove(z0p_ctx* ctx, const z0p_air_params* params, z0p_transcript* transcript, uint8_t** proof,
+                  size_t* proof_len);
+/* Proof::from_bytes + air::verify (air.rs:289-327, 607-752). Host only. */
+int z0p_air_verify(const z0p_air_params* params, z0p_transcript* transcript, const uint8_t* proof, size_t proof_len);
+/* fri::commit (fri.rs:408-433) + fri::public_commitment (fri.rs:863-870) +
+ * fri::open (fri.rs:645-666); outputs Commitment::to_bytes and
+ * Proof::to_bytes (fri.rs:875-880, 331-336). */
+int z0p_fri_commit_open(z0p_ctx* ctx, const uint32_t* coeffs, size_t n, const z0p_fri_params* params,
+                        z0p_transcript* transcript, uint8_t** commitment, size_t* commitment_len, uint8_t** proof,
+                        size_t* proof_len);
+/* Commitment/Proof::from_bytes + fri::verify (fri.rs:686-859). Host only. */
+int z0p_fri_verify(const uint8_t* commitment, size_t commitment_len, const uint8_t* proof, size_t proof_len,
+                   const z0p_fri_params* params, z0p_transcript* transcript);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* Z0P_H */

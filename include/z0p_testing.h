@@ -1,0 +1,56 @@
+/*
+ * Kernel-level diagnostic entry points of libz0p_amd.so. They have no
+ * counterpart in the reference crate: each runs one device stage of the
+ * prover on host-supplied data so parity tests can localise a mismatch to a
+ * single kernel. Same status conventions as z0p.h; all buffers are host
+ * memory, copied in and out by the call.
+ */
+#ifndef Z0P_TESTING_H
+#define Z0P_TESTING_H
+
+#include "z0p.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Inverse NTT of n natural-order evaluations; coefficients returned in
+ * bit-reversed order (position q holds coefficient brev(q)). */
+int z0p_test_intt(z0p_ctx* ctx, const uint32_t* evals, size_t n, uint32_t* coeffs_br);
+
+/* Coset LDE (shift 31, size n*blowup) of n natural-order coefficients,
+ * returned in bit-reversed order over the whole domain. */
+int z0p_test_lde(z0p_ctx* ctx, const uint32_t* coeffs, size_t n, size_t blowup, uint32_t* lde_br);
+
+/* Merkle tree over 64-byte leaves (leaves a power of two): all levels leaf
+ * to root, concatenated, 32 bytes per digest. */
+int z0p_test_merkle64(z0p_ctx* ctx, const uint8_t* flat, size_t leaves, uint8_t* levels);
+
+/* Trace tree leaves+levels from natural-order LDE columns a, b (m = n *
+ * blowup each): all levels leaf to root, 32 bytes per digest. */
+int z0p_test_trace_tree(z0p_ctx* ctx, const uint32_t* a, const uint32_t* b, size_t n, size_t blowup,
+                        uint8_t* levels);
+
+/* Linked LDE column B from natural-order A (air.rs:806-844). */
+int z0p_test_linked(z0p_ctx* ctx, const uint32_t* a, size_t n, size_t blowup, uint32_t actual_b_last, uint32_t a0,
+                    uint32_t* b);
+
+/* Composition over natural-order columns; out is m Ext4 (4 words each) in
+ * natural order (air.rs:847-929). alphas: 5 Ext4 = 20 words. */
+int z0p_test_compose(z0p_ctx* ctx, const uint32_t* a, const uint32_t* b, size_t n, size_t blowup, uint32_t a0,
+                     uint32_t b0, uint32_t b_last, const uint32_t* alphas, uint32_t* out);
+
+/* One arity-4 fold of a bit-reversed Ext4 layer of length len at fold depth
+ * `depth` of an LDE of size lde_len; challenges: 3 Ext4 = 12 words. */
+int z0p_test_fold(z0p_ctx* ctx, const uint32_t* layer, size_t len, size_t lde_len, size_t depth,
+                  const uint32_t* challenges, uint32_t* out);
+
+/* Proof-of-work: smallest nonce whose BLAKE3(head || u64 nonce) has at least
+ * `bits` leading zero bits; head must be 47 bytes. */
+int z0p_test_grind(z0p_ctx* ctx, const uint8_t* head, size_t head_len, uint32_t bits, uint64_t* nonce);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* Z0P_TESTING_H */

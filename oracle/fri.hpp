@@ -68,7 +68,9 @@ inline size_t estimate_fri_bytes(const std::vector<size_t>& layers, size_t num_q
     for (size_t i = 0; i < layers.size(); ++i) {
         size_t values = layers[i];
         size_t tree_leaves = values % 4 == 0 ? values / 4 : values;
-        size_t depth = size_t(__builtin_ctzll(tree_leaves));
+        // usize::trailing_zeros(0) = 64: the final-size enumeration of
+        // tune_shape can reach an empty layer on odd log2 sizes.
+        size_t depth = tree_leaves == 0 ? 64 : size_t(__builtin_ctzll(tree_leaves));
         size_t drop = std::min(cap_height, depth);
         total += 32 * (size_t(1) << std::min<size_t>(drop, 16));
         if (i + 1 < layers.size()) {

@@ -306,3 +306,136 @@ def stage_timers():
 
 def clear_stage_timers():
     lib().oracle_stage_timers_clear()
+
+
+# ---- numpy conveniences over the primitive entry points ----
+def _np():
+    import numpy as np
+    return np
+
+
+def _u32ptr(arr):
+    return arr.ctypes.data_as(u32p)
+
+
+def ntt_inverse(values):
+    np = _np()
+    a = np.ascontiguousarray(np.asarray(values, dtype=np.uint32)).copy()
+    rc = lib().oracle_ntt_inverse(_u32ptr(a), a.size)
+    if rc:
+        raise ValueError("oracle ntt error %d" % rc)
+    return a
+
+
+def ntt_forward(values):
+    np = _np()
+    a = np.ascontiguousarray(np.asarray(values, dtype=np.uint32)).copy()
+    rc = lib().oracle_ntt_forward(_u32ptr(a), a.size)
+    if rc:
+        raise ValueError("oracle ntt error %d" % rc)
+    return a
+
+
+def ntt_evaluate(coeffs, point):
+    np = _np()
+    a = np.ascontiguousarray(np.asarray(coeffs, dtype=np.uint32))
+    out = ctypes.c_uint32()
+    rc = lib().oracle_ntt_evaluate(_u32ptr(a), a.size, point, ctypes.byref(out))
+    if rc:
+        raise ValueError("oracle error %d" % rc)
+    return out.value
+
+
+def evaluate_on_coset(coeffs, blowup):
+    np = _np()
+    a = np.ascontiguousarray(np.asarray(coeffs, dtype=np.uint32))
+    out = np.zeros(a.size * blowup, dtype=np.uint32)
+    rc = lib().oracle_evaluate_on_coset(_u32ptr(a), a.size, blowup, _u32ptr(out))
+    if rc:
+        raise ValueError("oracle lde error %d" % rc)
+    return out
+
+
+def merkle_flat64(flat: bytes):
+    """All levels leaf to root (32 bytes per digest), concatenated."""
+    leaves = len(flat) // 64
+    p2 = 1
+    while p2 < leaves:
+        p2 <<= 1
+    out = (ctypes.c_uint8 * (32 * (2 * p2 - 1)))()
+    rc = lib().oracle_merkle_flat64(_buf(flat), len(flat), out)
+    if rc:
+        raise ValueError("oracle merkle error %d" % rc)
+    return bytes(out)
+
+
+def linked_lde(lde_a, n, actual_b_last, a0):
+    np = _np()
+    a = np.ascontiguousarray(np.asarray(lde_a, dtype=np.uint32))
+    out = np.zeros_like(a)
+    rc = lib().oracle_linked_lde(_u32ptr(a), a.size, n, actual_b_last, a0, _u32ptr(out))
+    if rc:
+        raise ValueError("oracle linked error %d" % rc)
+    return out
+
+
+def compose(lde_a, lde_b, n, a0, b0, b_last, alphas, scalar=False):
+    np = _np()
+    a = np.ascontiguousarray(np.asarray(lde_a, dtype=np.uint32))
+    b = np.ascontiguousarray(np.asarray(lde_b, dtype=np.uint32))
+    al = np.ascontiguousarray(np.asarray(alphas, dtype=np.uint32).reshape(20))
+    out = np.zeros(4 * a.size, dtype=np.uint32)
+    rc = lib().oracle_compose(_u32ptr(a), _u32ptr(b), a.size, n, a0, b0, b_last, _u32ptr(al), 1 if scalar else 0,
+                              _u32ptr(out))
+    if rc:
+        raise ValueError("oracle compose error %d" % rc)
+    return out.reshape(-1, 4)
+
+
+def fold_values(quad, inv_point, challenges, unit):
+    np = _np()
+    q = np.ascontiguousarray(np.asarray(quad, dtype=np.uint32).reshape(16))
+    c = np.ascontiguousarray(np.asarray(challenges, dtype=np.uint32).reshape(12))
+    out = np.zeros(4, dtype=np.uint32)
+    lib().oracle_fold_values(_u32ptr(q), inv_point, _u32ptr(c), unit, _u32ptr(out))
+    return out
+
+
+def point_at(lde_len, depth, j):
+    out = ctypes.c_uint32()
+    rc = lib().oracle_point_at(lde_len, depth, j, ctypes.byref(out))
+    if rc:
+        raise ValueError("oracle error %d" % rc)
+    return out.value
+
+
+def field_inv(a):
+    return lib().oracle_field_inv(a)
+
+
+def field_mul(a, b):
+    return lib().oracle_field_mul(a, b)
+
+
+def field_pow(a, e):
+    return lib().oracle_field_pow(a, e)
+
+
+def two_adic_generator(bits):
+    out = ctypes.c_uint32()
+    rc = lib().oracle_two_adic_generator(bits, ctypes.byref(out))
+    if rc:
+        raise ValueError("oracle error %d" % rc)
+    return out.value
+
+
+def grind_head(head: bytes, bits: int) -> int:
+    return lib().oracle_grind_head(_buf(head), len(head), bits)
+
+
+def build_trace(a0, b0, n):
+    np = _np()
+    a = np.zeros(n, dtype=np.uint32)
+    b = np.zeros(n, dtype=np.uint32)
+    lib().oracle_build_trace(a0, b0, n, _u32ptr(a), _u32ptr(b))
+    return a, b

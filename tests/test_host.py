@@ -1,0 +1,167 @@
+"""Host-side parts of the product library on the CPU: the C ABI exports,
+the parameter tuners, the Fiat-Shamir transcript, the fast-doubling trace
+claim and the verifier (which must accept the oracle's proofs and reject
+every tampered one)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_ffi as o
+from helpers import P, fri_bench_coeffs, seeded_claims
+
+import z0p_amd as z
+from z0p_amd import _ffi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in ["z0p.h", "z0p_testing.h"]:
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(z0p_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_ffi.LIB_PATH)], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    declared = declared_symbols()
+    assert declared, "no declarations parsed"
+    assert declared <= exported, sorted(declared - exported)
+    assert declared == set(_ffi.SIGNATURES), sorted(declared ^ set(_ffi.SIGNATURES))
+    _ffi.lib()  # loads and binds every declared symbol
+
+
+@pytest.mark.parametrize("logn", list(range(1, 24)))
+def test_air_tuner_matches_oracle(logn):
+    n = 1 << logn
+    for bits in (40, 100, 128):
+        a = z.air.for_security(n, 1, 2, 3, bits)
+        b = o.air_params_for_security(n, 1, 2, 3, bits)
+        assert bytes(a) == bytes(b), (n, bits)
+
+
+@pytest.mark.parametrize("logn", list(range(0, 24)))
+def test_fri_tuner_matches_oracle(logn):
+    n = 1 << logn
+    for bits in (30, 100):
+        assert bytes(z.fri.for_security(n, bits)) == bytes(o.fri_params_for_security(n, bits))
+
+
+def test_param_validation_codes():
+    with pytest.raises(z.Z0pError) as e:
+        z.air.for_security(3, 1, 1, 1)
+    assert e.value.name == "InvalidDomainSize"
+    with pytest.raises(z.Z0pError) as e:
+        z.air.params_new(64, 2, 8, 8, 1, 1, 1)
+    assert e.value.name == "InvalidBlowup"
+    with pytest.raises(z.Z0pError) as e:
+        z.air.params_new(64, 8, 0, 8, 1, 1, 1)
+    assert e.value.name == "NoQueries"
+    with pytest.raises(z.Z0pError) as e:
+        z.air.for_security(1 << 24, 1, 1, 1)
+    assert e.value.name == "InvalidDomainSize"
+    with pytest.raises(z.Z0pError) as e:
+        z.fri.for_security(0)
+    assert e.value.name == "EmptyInput"
+
+
+def test_b_last_fast_doubling():
+    for n in [1, 2, 3, 64, 1000, 1 << 16]:
+        for seed in range(3):
+            a0, b0 = seeded_claims(seed)
+            assert z.air.b_last(a0, b0, n) == o.fib_b_last(a0, b0, n)
+
+
+def test_transcript_matches_oracle_on_random_sequences():
+    rng = np.random.default_rng(11)
+    for trial in range(5):
+        label = rng.integers(0, 256, int(rng.integers(0, 40)), dtype=np.uint8).tobytes()
+        zt, ot = z.Transcript(label), o.Transcript(label)
+        for _ in range(40):
+            if rng.random() < 0.6:
+                lab = b"l%d" % int(rng.integers(0, 9))
+                data = rng.integers(0, 256, int(rng.integers(0, 3000)), dtype=np.uint8).tobytes()
+                zt.absorb(lab, data)
+                ot.absorb(lab, data)
+            else:
+                assert zt.challenge_bytes(b"c") == ot.challenge_bytes(b"c")
+        zc, oc = zt.clone(), ot.clone()
+        assert zc.challenge_bytes(b"x") == oc.challenge_bytes(b"x")
+
+
+@pytest.fixture(scope="module")
+def oracle_air_proof():
+    n, label = 1024, b"bench/air-scale"
+    bl = o.fib_b_last(1, 1, n)
+    return n, label, bl, o.air_prove(o.air_params_for_security(n, 1, 1, bl), o.Transcript(label))
+
+
+def test_verifier_accepts_oracle_proofs(oracle_air_proof):
+    n, label, bl, proof = oracle_air_proof
+    z.air.verify(z.air.for_security(n, 1, 1, bl), proof, z.Transcript(label))
+    for seed in range(2):
+        a0, b0 = seeded_claims(seed)
+        b = o.fib_b_last(a0, b0, 256)
+        pr = o.air_prove(o.air_params_for_security(256, a0, b0, b), o.Transcript(b"s"))
+        z.air.verify(z.air.for_security(256, a0, b0, b), pr, z.Transcript(b"s"))
+
+
+def _expect_reject(params, proof, label, oracle_params):
+    with pytest.raises(z.Z0pError):
+        z.air.verify(params, proof, z.Transcript(label))
+    assert o.air_verify(oracle_params, o.Transcript(label), proof) != 0
+
+
+def test_verifier_rejects_tampering(oracle_air_proof):
+    # tests/security.rs:60-354 style: byte flips across every proof region.
+    n, label, bl, proof = oracle_air_proof
+    zp = z.air.for_security(n, 1, 1, bl)
+    op = o.air_params_for_security(n, 1, 1, bl)
+    rng = np.random.default_rng(3)
+    positions = sorted(set([1, 5, 40, 600, 1000, 2000, len(proof) // 2, len(proof) - 1, len(proof) - 130]
+                           + list(rng.integers(1, len(proof), 24))))
+    for pos in positions:
+        bad = bytearray(proof)
+        bad[int(pos)] ^= 1 << int(rng.integers(0, 8))
+        _expect_reject(zp, bytes(bad), label, op)
+    # wrong statement, wrong label, truncation, trailing bytes, wrong version
+    _expect_reject(z.air.for_security(n, 1, 1, (bl + 1) % P), proof, label,
+                   o.air_params_for_security(n, 1, 1, (bl + 1) % P))
+    _expect_reject(zp, proof, b"other-label", op)
+    _expect_reject(zp, proof[:-1], label, op)
+    _expect_reject(zp, proof + b"\0", label, op)
+    _expect_reject(zp, b"\x05" + proof[1:], label, op)
+
+
+def test_fri_verifier_accepts_and_rejects():
+    n, label = 256, b"bench/fri-prod"
+    op = o.fri_params_for_security(n)
+    c, pf = o.fri_commit_open(fri_bench_coeffs(n), op, o.Transcript(label))
+    zp = z.fri.for_security(n)
+    z.fri.verify(c, pf, zp, z.Transcript(label))
+    for pos in [10, 30, len(pf) // 3, len(pf) - 3]:
+        bad = bytearray(pf)
+        bad[pos] ^= 4
+        with pytest.raises(z.Z0pError):
+            z.fri.verify(c, bytes(bad), zp, z.Transcript(label))
+    for pos in [3, 40, len(c) - 1]:
+        bad = bytearray(c)
+        bad[pos] ^= 4
+        with pytest.raises(z.Z0pError):
+            z.fri.verify(bytes(bad), pf, zp, z.Transcript(label))
+
+
+def test_no_gpu_context_fails_loudly():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    with pytest.raises(z.Z0pError) as e:
+        z.Context(0)
+    assert e.value.name == "NoDevice"

@@ -1,0 +1,132 @@
+"""ctypes binding of libz0p_amd.so (the C ABI in include/z0p.h).
+
+The library is built in-tree (``__graft_entry__.build()`` or
+``make -C z0p_amd/csrc``); importing this package without it fails loudly, so
+no code path can silently fall back to a CPU implementation.
+"""
+import ctypes
+import pathlib
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "libz0p_amd.so"
+
+u8p = ctypes.POINTER(ctypes.c_uint8)
+u32p = ctypes.POINTER(ctypes.c_uint32)
+u64 = ctypes.c_uint64
+u32 = ctypes.c_uint32
+sz = ctypes.c_size_t
+vp = ctypes.c_void_p
+
+ERROR_NAMES = {
+    1: "EmptyInput",
+    2: "IndexOutOfBounds",
+    3: "InvalidDomainSize",
+    4: "InvalidBlowup",
+    5: "NoQueries",
+    6: "BadMerklePath",
+    7: "FriVerificationFailed",
+    8: "MalformedInput",
+    9: "RngUnavailable",
+    100: "Device",
+    101: "OutOfMemory",
+    102: "InvalidArgument",
+    103: "NoDevice",
+}
+
+
+class Z0pError(Exception):
+    """A non-zero status from the library; ``code`` follows src/error.rs order."""
+
+    def __init__(self, code):
+        self.code = code
+        self.name = ERROR_NAMES.get(code, "Unknown")
+        super().__init__("%s (%d)" % (self.name, code))
+
+
+class AirParamsC(ctypes.Structure):
+    _fields_ = [
+        ("n_steps", u64), ("blowup", u64), ("num_queries", u64), ("final_size", u64),
+        ("grinding_bits", u32), ("reserved0", u32), ("cap_height", u64),
+        ("a0", u32), ("b0", u32), ("b_last", u32), ("reserved1", u32),
+    ]
+
+
+class FriParamsC(ctypes.Structure):
+    _fields_ = [
+        ("blowup", u64), ("num_queries", u64), ("final_size", u64),
+        ("grinding_bits", u32), ("reserved0", u32), ("cap_height", u64),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/z0p.h and
+# include/z0p_testing.h.
+SIGNATURES = {
+    "z0p_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(vp)]),
+    "z0p_ctx_destroy": (None, [vp]),
+    "z0p_ctx_set_profiling": (ctypes.c_int, [vp, ctypes.c_int]),
+    "z0p_ctx_kernel_stats": (sz, [vp, ctypes.c_char_p, sz]),
+    "z0p_ctx_reset_stats": (None, [vp]),
+    "z0p_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "z0p_free": (None, [vp]),
+    "z0p_transcript_new": (vp, [u8p, sz]),
+    "z0p_transcript_clone": (vp, [vp]),
+    "z0p_transcript_free": (None, [vp]),
+    "z0p_transcript_absorb": (None, [vp, ctypes.c_char_p, u8p, sz]),
+    "z0p_transcript_challenge_bytes": (None, [vp, ctypes.c_char_p, u8p]),
+    "z0p_air_params_new": (ctypes.c_int, [u64, u64, u64, u64, u32, u32, u32, ctypes.POINTER(AirParamsC)]),
+    "z0p_air_params_for_security": (ctypes.c_int, [u64, u32, u32, u32, u32, ctypes.POINTER(AirParamsC)]),
+    "z0p_fri_params_new": (ctypes.c_int, [u64, u64, u64, ctypes.POINTER(FriParamsC)]),
+    "z0p_fri_params_for_security": (ctypes.c_int, [u64, u32, ctypes.POINTER(FriParamsC)]),
+    "z0p_air_b_last": (u32, [u32, u32, u64]),
+    "z0p_air_prove": (ctypes.c_int, [vp, ctypes.POINTER(AirParamsC), vp, ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+    "z0p_air_verify": (ctypes.c_int, [ctypes.POINTER(AirParamsC), vp, u8p, sz]),
+    "z0p_fri_commit_open": (ctypes.c_int, [vp, u32p, sz, ctypes.POINTER(FriParamsC), vp, ctypes.POINTER(u8p),
+                                           ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+    "z0p_fri_verify": (ctypes.c_int, [u8p, sz, u8p, sz, ctypes.POINTER(FriParamsC), vp]),
+    # include/z0p_testing.h
+    "z0p_test_intt": (ctypes.c_int, [vp, u32p, sz, u32p]),
+    "z0p_test_lde": (ctypes.c_int, [vp, u32p, sz, sz, u32p]),
+    "z0p_test_merkle64": (ctypes.c_int, [vp, u8p, sz, u8p]),
+    "z0p_test_trace_tree": (ctypes.c_int, [vp, u32p, u32p, sz, sz, u8p]),
+    "z0p_test_linked": (ctypes.c_int, [vp, u32p, sz, sz, u32, u32, u32p]),
+    "z0p_test_compose": (ctypes.c_int, [vp, u32p, u32p, sz, sz, u32, u32, u32, u32p, u32p]),
+    "z0p_test_fold": (ctypes.c_int, [vp, u32p, sz, sz, sz, u32p, u32p]),
+    "z0p_test_grind": (ctypes.c_int, [vp, u8p, sz, u32, ctypes.POINTER(u64)]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError("libz0p_amd.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    L = ctypes.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise Z0pError(rc)
+
+
+def buf(data: bytes):
+    return (ctypes.c_uint8 * max(len(data), 1)).from_buffer_copy(data if data else b"\0")
+
+
+def u32_buf(values):
+    import numpy as np
+    arr = np.ascontiguousarray(np.asarray(values, dtype=np.uint32))
+    return arr, arr.ctypes.data_as(u32p)
+
+
+def take(ptr, n):
+    out = ctypes.string_at(ptr, n)
+    lib().z0p_free(ptr)
+    return out

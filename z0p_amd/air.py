@@ -1,0 +1,62 @@
+"""Fibonacci AIR prover/verifier (reference: src/air.rs)."""
+import ctypes
+
+from ._ffi import AirParamsC, Z0pError, buf, check, lib, take, u8p
+
+Params = AirParamsC
+
+
+def _params_repr(self):
+    return "air.Params(n_steps=%d, B%dQ%dG%dC%dF%d, a0=%d, b0=%d, b_last=%d)" % (
+        self.n_steps, self.blowup, self.num_queries, self.grinding_bits, self.cap_height, self.final_size,
+        self.a0, self.b0, self.b_last)
+
+
+def _shape(self):
+    return "B%dQ%dG%dC%dF%d" % (self.blowup, self.num_queries, self.grinding_bits, self.cap_height, self.final_size)
+
+
+Params.__repr__ = _params_repr
+Params.shape = _shape
+
+
+def params_new(n_steps, blowup, num_queries, final_size, a0, b0, b_last) -> Params:
+    """air::Params::new (air.rs:63-101)."""
+    p = Params()
+    check(lib().z0p_air_params_new(n_steps, blowup, num_queries, final_size, a0, b0, b_last, ctypes.byref(p)))
+    return p
+
+
+def for_security(n_steps, a0, b0, b_last, target_bits=100) -> Params:
+    """air::Params::for_security (air.rs:129-162)."""
+    p = Params()
+    check(lib().z0p_air_params_for_security(n_steps, a0, b0, b_last, target_bits, ctypes.byref(p)))
+    return p
+
+
+Params.new = staticmethod(params_new)
+Params.for_security = staticmethod(for_security)
+
+
+def b_last(a0, b0, n_steps) -> int:
+    """build_trace(a0, b0, n).1[n - 1] (air.rs:341-354)."""
+    return lib().z0p_air_b_last(a0, b0, n_steps)
+
+
+def prove(params: Params, transcript, ctx=None) -> bytes:
+    """air::prove + Proof::to_bytes (air.rs:365-372, 262-281) on the GPU."""
+    from . import default_context
+    ctx = ctx or default_context()
+    out = u8p()
+    n = ctypes.c_size_t()
+    check(lib().z0p_air_prove(ctx.handle, ctypes.byref(params), transcript.handle, ctypes.byref(out),
+                              ctypes.byref(n)))
+    return take(out, n.value)
+
+
+def verify(params: Params, proof: bytes, transcript) -> None:
+    """Proof::from_bytes + air::verify (air.rs:289-327, 607-752); raises Z0pError."""
+    check(lib().z0p_air_verify(ctypes.byref(params), transcript.handle, buf(proof), len(proof)))
+
+
+__all__ = ["Params", "params_new", "for_security", "b_last", "prove", "verify", "Z0pError"]

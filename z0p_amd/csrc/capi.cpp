@@ -1,0 +1,243 @@
+// extern "C" boundary (include/z0p.h): translate statuses, own outputs.
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "params.hpp"
+#include "prover.hpp"
+
+
+namespace {
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return Z0P_OK;
+    } catch (const z0p::Status& s) {
+        return s.code;
+    } catch (const std::bad_alloc&) {
+        return Z0P_ERR_OUT_OF_MEMORY;
+    } catch (...) {
+        return Z0P_ERR_DEVICE;
+    }
+}
+
+int emit(const std::vector<uint8_t>& v, uint8_t** out, size_t* len) {
+    uint8_t* p = static_cast<uint8_t*>(std::malloc(v.empty() ? 1 : v.size()));
+    if (!p) return Z0P_ERR_OUT_OF_MEMORY;
+    if (!v.empty()) std::memcpy(p, v.data(), v.size());
+    *out = p;
+    *len = v.size();
+    return Z0P_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int z0p_ctx_create(int device, z0p_ctx** out) {
+    if (!out) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] {
+        z0p::Ctx* c = z0p::ctx_create(device);
+        *out = new z0p_ctx{c};
+    });
+}
+
+void z0p_ctx_destroy(z0p_ctx* ctx) {
+    if (!ctx) return;
+    delete ctx->c;
+    delete ctx;
+}
+
+int z0p_ctx_set_profiling(z0p_ctx* ctx, int enabled) {
+    if (!ctx) return Z0P_ERR_INVALID_ARGUMENT;
+    ctx->c->prof.enabled = enabled != 0;
+    return Z0P_OK;
+}
+
+size_t z0p_ctx_kernel_stats(z0p_ctx* ctx, char* buf, size_t cap) {
+    if (!ctx) return 0;
+    std::string s;
+    for (auto& kv : ctx->c->prof.stats)
+        s += kv.first + "=" + std::to_string(kv.second.launches) + "," + std::to_string(kv.second.total_ms) + ";";
+    if (buf && cap) {
+        size_t n = std::min(cap - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = 0;
+    }
+    return s.size() + 1;
+}
+
+void z0p_ctx_reset_stats(z0p_ctx* ctx) {
+    if (ctx) ctx->c->prof.stats.clear();
+}
+
+const char* z0p_status_string(int s) {
+    switch (s) {
+        case Z0P_OK: return "ok";
+        case Z0P_ERR_EMPTY_INPUT: return "empty input: need at least one element";
+        case Z0P_ERR_INDEX_OUT_OF_BOUNDS: return "index out of bounds";
+        case Z0P_ERR_INVALID_DOMAIN_SIZE: return "invalid domain size: must be a power of two";
+        case Z0P_ERR_INVALID_BLOWUP: return "invalid blowup: must be a power of two";
+        case Z0P_ERR_NO_QUERIES: return "query count must be non-zero";
+        case Z0P_ERR_BAD_MERKLE_PATH: return "merkle authentication path failed";
+        case Z0P_ERR_FRI_VERIFICATION_FAILED: return "FRI verification failed";
+        case Z0P_ERR_MALFORMED_INPUT: return "malformed input";
+        case Z0P_ERR_RNG_UNAVAILABLE: return "operating-system RNG unavailable";
+        case Z0P_ERR_DEVICE: return "HIP device error";
+        case Z0P_ERR_OUT_OF_MEMORY: return "device out of memory";
+        case Z0P_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case Z0P_ERR_NO_DEVICE: return "no usable GPU device";
+        default: return "unknown status";
+    }
+}
+
+void z0p_free(void* p) { std::free(p); }
+
+z0p_transcript* z0p_transcript_new(const uint8_t* label, size_t label_len) {
+    auto* t = new (std::nothrow) z0p_transcript;
+    if (t) t->t = z0p::Transcript(label, label_len);
+    return t;
+}
+
+z0p_transcript* z0p_transcript_clone(const z0p_transcript* t) {
+    if (!t) return nullptr;
+    return new (std::nothrow) z0p_transcript(*t);
+}
+
+void z0p_transcript_free(z0p_transcript* t) { delete t; }
+
+void z0p_transcript_absorb(z0p_transcript* t, const char* label, const uint8_t* data, size_t len) {
+    if (t && label) t->t.absorb(label, data, len);
+}
+
+void z0p_transcript_challenge_bytes(z0p_transcript* t, const char* label, uint8_t out[32]) {
+    if (t && label && out) t->t.squeeze(label, out);
+}
+
+int z0p_air_params_new(uint64_t n_steps, uint64_t blowup, uint64_t num_queries, uint64_t final_size, uint32_t a0,
+                       uint32_t b0, uint32_t b_last, z0p_air_params* out) {
+    if (!out) return Z0P_ERR_INVALID_ARGUMENT;
+    using namespace z0p;
+    if (n_steps < 2 || !pow2(n_steps)) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    if (blowup < 4 || !pow2(blowup)) return Z0P_ERR_INVALID_BLOWUP;
+    if (num_queries == 0) return Z0P_ERR_NO_QUERIES;
+    if (final_size < 2 || !pow2(final_size)) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    if (n_steps > MAX_DOMAIN || blowup > MAX_DOMAIN || n_steps * blowup > MAX_DOMAIN) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    std::memset(out, 0, sizeof(*out));
+    out->n_steps = n_steps;
+    out->blowup = blowup;
+    out->num_queries = num_queries;
+    out->final_size = final_size;
+    out->a0 = a0 % P;
+    out->b0 = b0 % P;
+    out->b_last = b_last % P;
+    return Z0P_OK;
+}
+
+int z0p_air_params_for_security(uint64_t n_steps, uint32_t a0, uint32_t b0, uint32_t b_last, uint32_t target_bits,
+                                z0p_air_params* out) {
+    if (!out) return Z0P_ERR_INVALID_ARGUMENT;
+    using namespace z0p;
+    if (n_steps < 2 || !pow2(n_steps)) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    const uint64_t blowup = 16;
+    if (n_steps > MAX_DOMAIN / blowup) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    const uint64_t lde = n_steps * blowup;
+    auto strength = tune_strength(target_bits, composition_millibits(lde, n_steps - 1));
+    auto shape = tune_shape(lde, strength.first, true, lde, 24);
+    std::memset(out, 0, sizeof(*out));
+    out->n_steps = n_steps;
+    out->blowup = blowup;
+    out->num_queries = strength.first;
+    out->final_size = shape.first;
+    out->grinding_bits = strength.second;
+    out->cap_height = shape.second;
+    out->a0 = a0 % P;
+    out->b0 = b0 % P;
+    out->b_last = b_last % P;
+    return Z0P_OK;
+}
+
+int z0p_fri_params_new(uint64_t blowup, uint64_t num_queries, uint64_t final_size, z0p_fri_params* out) {
+    if (!out) return Z0P_ERR_INVALID_ARGUMENT;
+    using namespace z0p;
+    if (blowup < 2 || !pow2(blowup)) return Z0P_ERR_INVALID_BLOWUP;
+    if (num_queries == 0) return Z0P_ERR_NO_QUERIES;
+    if (final_size < 2 || !pow2(final_size) || final_size > MAX_FINAL_SIZE) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    std::memset(out, 0, sizeof(*out));
+    out->blowup = blowup;
+    out->num_queries = num_queries;
+    out->final_size = final_size;
+    return Z0P_OK;
+}
+
+int z0p_fri_params_for_security(uint64_t poly_len, uint32_t target_bits, z0p_fri_params* out) {
+    if (!out) return Z0P_ERR_INVALID_ARGUMENT;
+    using namespace z0p;
+    if (poly_len == 0) return Z0P_ERR_EMPTY_INPUT;
+    if (!pow2(poly_len)) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    const uint64_t blowup = 16;
+    if (poly_len > MAX_DOMAIN / blowup) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    const uint64_t lde = poly_len * blowup;
+    auto strength = tune_strength(target_bits, 2000);
+    auto shape = tune_shape(lde, strength.first, false, 0, 0);
+    std::memset(out, 0, sizeof(*out));
+    out->blowup = blowup;
+    out->num_queries = strength.first;
+    out->final_size = shape.first;
+    out->grinding_bits = strength.second;
+    out->cap_height = shape.second;
+    return Z0P_OK;
+}
+
+uint32_t z0p_air_b_last(uint32_t a0, uint32_t b0, uint64_t n_steps) {
+    if (n_steps == 0) return 0;
+    return z0p::fib_b_last(a0 % z0p::P, b0 % z0p::P, n_steps);
+}
+
+int z0p_air_prove(z0p_ctx* ctx, const z0p_air_params* params, z0p_transcript* transcript, uint8_t** proof,
+                  size_t* proof_len) {
+    if (!ctx || !params || !transcript || !proof || !proof_len) return Z0P_ERR_INVALID_ARGUMENT;
+    int rc = Z0P_OK;
+    int g = guarded([&] {
+        std::vector<uint8_t> bytes = z0p::air_prove(*ctx->c, *params, transcript->t);
+        rc = emit(bytes, proof, proof_len);
+    });
+    return g != Z0P_OK ? g : rc;
+}
+
+int z0p_air_verify(const z0p_air_params* params, z0p_transcript* transcript, const uint8_t* proof,
+                   size_t proof_len) {
+    if (!params || !transcript || (!proof && proof_len)) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] { z0p::air_verify(*params, transcript->t, proof, proof_len); });
+}
+
+int z0p_fri_commit_open(z0p_ctx* ctx, const uint32_t* coeffs, size_t n, const z0p_fri_params* params,
+                        z0p_transcript* transcript, uint8_t** commitment, size_t* commitment_len, uint8_t** proof,
+                        size_t* proof_len) {
+    if (!ctx || !params || !transcript || !commitment || !commitment_len || !proof || !proof_len ||
+        (!coeffs && n))
+        return Z0P_ERR_INVALID_ARGUMENT;
+    int rc = Z0P_OK;
+    int g = guarded([&] {
+        std::vector<uint8_t> c, p;
+        z0p::fri_commit_open(*ctx->c, coeffs, n, *params, transcript->t, c, p);
+        rc = emit(c, commitment, commitment_len);
+        if (rc == Z0P_OK) {
+            rc = emit(p, proof, proof_len);
+            if (rc != Z0P_OK) std::free(*commitment);
+        }
+    });
+    return g != Z0P_OK ? g : rc;
+}
+
+int z0p_fri_verify(const uint8_t* commitment, size_t commitment_len, const uint8_t* proof, size_t proof_len,
+                   const z0p_fri_params* params, z0p_transcript* transcript) {
+    if (!params || !transcript || (!commitment && commitment_len) || (!proof && proof_len))
+        return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] { z0p::fri_verify(commitment, commitment_len, proof, proof_len, *params, transcript->t); });
+}
+
+}  // extern "C"

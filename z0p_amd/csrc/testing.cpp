@@ -1,0 +1,250 @@
+// Kernel-level diagnostic entry points (include/z0p_testing.h): each runs
+// one device stage on host data so a parity failure can be pinned to one
+// kernel. Natural-order inputs are permuted to the device's bit-reversed
+// layout on the host and back.
+#include <cstring>
+#include <vector>
+
+#include "../../include/z0p_testing.h"
+#include "params.hpp"
+#include "prover.hpp"
+
+
+namespace z0p {
+namespace {
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return Z0P_OK;
+    } catch (const Status& s) {
+        return s.code;
+    } catch (...) {
+        return Z0P_ERR_DEVICE;
+    }
+}
+
+template <typename T>
+T* to_dev(Ctx& c, const T* host, size_t count) {
+    T* d = c.alloc<T>(count);
+    hip_check(hipMemcpyAsync(d, host, count * sizeof(T), hipMemcpyHostToDevice, c.stream));
+    return d;
+}
+template <typename T>
+void to_host(Ctx& c, T* host, const T* dev, size_t count) {
+    hip_check(hipMemcpyAsync(host, dev, count * sizeof(T), hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+}
+
+// natural index i = j + B t  <->  position brev(j) n + brev(t)
+uint64_t br_pos(uint64_t i, int logn, int logb) {
+    uint64_t B = uint64_t(1) << logb;
+    return (uint64_t(brev_host(i & (B - 1), logb)) << logn) | brev_host(i >> logb, logn);
+}
+
+void check_geometry(size_t n, size_t blowup) {
+    require(pow2(n) && n >= 2 && pow2(blowup) && blowup >= 2 && n * blowup <= MAX_DOMAIN, Z0P_ERR_INVALID_DOMAIN_SIZE);
+}
+
+}  // namespace
+}  // namespace z0p
+
+using namespace z0p;
+
+extern "C" {
+
+int z0p_test_intt(z0p_ctx* ctx, const uint32_t* evals, size_t n, uint32_t* coeffs_br) {
+    if (!ctx || !evals || !coeffs_br) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] {
+        check_geometry(n, 16);
+        Ctx& c = *ctx->c;
+        int logn = int(ilog2(n));
+        const GeometryTables& gt = c.geometry(logn, 4);
+        c.reserve_arena(n * 12 + (1 << 20));
+        c.reset_arena();
+        uint32_t* din = to_dev(c, evals, n);
+        uint32_t* dout = c.alloc<uint32_t>(n);
+        uint32_t* scratch = c.alloc<uint32_t>(n);
+        launch_intt_to_br(c.launch(), din, dout, scratch, logn, c.tw4096_inv, gt.winv, 4,
+                          to_mont(inv_c(uint32_t(n % P))));
+        to_host(c, coeffs_br, dout, n);
+    });
+}
+
+int z0p_test_lde(z0p_ctx* ctx, const uint32_t* coeffs, size_t n, size_t blowup, uint32_t* lde_br) {
+    if (!ctx || !coeffs || !lde_br) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] {
+        check_geometry(n, blowup);
+        Ctx& c = *ctx->c;
+        int logn = int(ilog2(n)), logb = int(ilog2(blowup));
+        size_t m = n * blowup;
+        const GeometryTables& gt = c.geometry(logn, logb);
+        c.reserve_arena(n * 8 + m * 8 + (1 << 20));
+        c.reset_arena();
+        uint32_t* din = to_dev(c, coeffs, n);
+        uint32_t* dbr = c.alloc<uint32_t>(n);
+        uint32_t* scratch = c.alloc<uint32_t>(m);
+        uint32_t* dout = c.alloc<uint32_t>(m);
+        launch_bitrev(c.launch(), din, dbr, logn);
+        launch_lde_br(c.launch(), dbr, dout, scratch, logn, logb, c.tw4096, gt.w, gt.coset);
+        to_host(c, lde_br, dout, m);
+    });
+}
+
+int z0p_test_merkle64(z0p_ctx* ctx, const uint8_t* flat, size_t leaves, uint8_t* levels) {
+    if (!ctx || !flat || !levels) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] {
+        require(pow2(leaves), Z0P_ERR_INVALID_DOMAIN_SIZE);
+        Ctx& c = *ctx->c;
+        TreeGeom g = tree_geom(leaves, 0);
+        c.reserve_arena(leaves * 64 + g.words() * 4 + (1 << 20));
+        c.reset_arena();
+        const uint32_t* din = to_dev(c, reinterpret_cast<const uint32_t*>(flat), leaves * 16);
+        uint32_t* tree = c.alloc<uint32_t>(g.words());
+        int first = std::min(9, g.target);
+        launch_fri_leaves(c.launch(), din, false, tree, leaves, first);
+        build_levels(c, tree, g, first);
+        to_host(c, reinterpret_cast<uint32_t*>(levels), tree, g.words());
+    });
+}
+
+int z0p_test_trace_tree(z0p_ctx* ctx, const uint32_t* a, const uint32_t* b, size_t n, size_t blowup,
+                        uint8_t* levels) {
+    if (!ctx || !a || !b || !levels) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] {
+        check_geometry(n, blowup);
+        require(blowup >= 4, Z0P_ERR_INVALID_BLOWUP);
+        Ctx& c = *ctx->c;
+        int logn = int(ilog2(n)), logb = int(ilog2(blowup));
+        size_t m = n * blowup;
+        std::vector<uint32_t> abr(m), bbr(m);
+        for (size_t i = 0; i < m; ++i) {
+            uint64_t p = br_pos(i, logn, logb);
+            abr[p] = a[i];
+            bbr[p] = b[i];
+        }
+        TreeGeom g = tree_geom(m / 8, 0);
+        c.reserve_arena(m * 8 + g.words() * 4 + (1 << 20));
+        c.reset_arena();
+        uint32_t* da = to_dev(c, abr.data(), m);
+        uint32_t* db = to_dev(c, bbr.data(), m);
+        uint32_t* tree = c.alloc<uint32_t>(g.words());
+        int have = launch_trace_leaves(c.launch(), da, db, tree, logn, logb, g.target);
+        build_levels(c, tree, g, have);
+        to_host(c, reinterpret_cast<uint32_t*>(levels), tree, g.words());
+    });
+}
+
+int z0p_test_linked(z0p_ctx* ctx, const uint32_t* a, size_t n, size_t blowup, uint32_t actual_b_last, uint32_t a0,
+                    uint32_t* b) {
+    if (!ctx || !a || !b) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] {
+        check_geometry(n, blowup);
+        Ctx& c = *ctx->c;
+        int logn = int(ilog2(n)), logb = int(ilog2(blowup));
+        size_t m = n * blowup, B = blowup;
+        const GeometryTables& gt = c.geometry(logn, logb);
+        std::vector<uint32_t> abr(m), bbr(m);
+        for (size_t i = 0; i < m; ++i) abr[br_pos(i, logn, logb)] = a[i];
+        const uint32_t w_last = pow_c(root_of_unity(uint32_t(logn)), n - 1);
+        const uint32_t coeff = mul(mul(sub(actual_b_last % P, a0 % P), w_last), inv_c(uint32_t(n % P)));
+        const uint32_t wm = root_of_unity(uint32_t(logn + logb));
+        std::vector<uint32_t> cv(B);
+        for (size_t j = 0; j < B; ++j) cv[j] = mul(coeff, sub(pow_c(mul(GENERATOR, pow_c(wm, j)), n), 1));
+        c.reserve_arena(m * 8 + (1 << 20));
+        c.reset_arena();
+        uint32_t* da = to_dev(c, abr.data(), m);
+        uint32_t* dcv = to_dev(c, cv.data(), B);
+        uint32_t* db = c.alloc<uint32_t>(m);
+        launch_linked_b(c.launch(), da, db, logn, logb, gt.w, gt.coset.shift, dcv, to_mont(w_last));
+        to_host(c, bbr.data(), db, m);
+        for (size_t i = 0; i < m; ++i) b[i] = bbr[br_pos(i, logn, logb)];
+    });
+}
+
+int z0p_test_compose(z0p_ctx* ctx, const uint32_t* a, const uint32_t* b, size_t n, size_t blowup, uint32_t a0,
+                     uint32_t b0, uint32_t b_last, const uint32_t* alphas, uint32_t* out) {
+    if (!ctx || !a || !b || !alphas || !out) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] {
+        check_geometry(n, blowup);
+        Ctx& c = *ctx->c;
+        int logn = int(ilog2(n)), logb = int(ilog2(blowup));
+        size_t m = n * blowup;
+        const GeometryTables& gt = c.geometry(logn, logb);
+        std::vector<uint32_t> abr(m), bbr(m), o(4 * m);
+        for (size_t i = 0; i < m; ++i) {
+            uint64_t p = br_pos(i, logn, logb);
+            abr[p] = a[i];
+            bbr[p] = b[i];
+        }
+        ComposeConsts cc{};
+        for (int k = 0; k < 5; ++k)
+            for (int q = 0; q < 4; ++q) cc.alpha_m[k][q] = to_mont(alphas[4 * k + q] % P);
+        cc.w_last_m = to_mont(pow_c(root_of_unity(uint32_t(logn)), n - 1));
+        cc.a0 = a0 % P;
+        cc.b0 = b0 % P;
+        cc.b_last = b_last % P;
+        c.reserve_arena(m * 24 + (1 << 20));
+        c.reset_arena();
+        uint32_t* da = to_dev(c, abr.data(), m);
+        uint32_t* db = to_dev(c, bbr.data(), m);
+        uint32_t* dl = c.alloc<uint32_t>(4 * m);
+        launch_compose(c.launch(), da, db, dl, logn, logb, gt.w, gt.coset.shift, gt.zinv, cc);
+        to_host(c, o.data(), dl, 4 * m);
+        for (size_t i = 0; i < m; ++i) std::memcpy(out + 4 * i, &o[4 * br_pos(i, logn, logb)], 16);
+    });
+}
+
+int z0p_test_fold(z0p_ctx* ctx, const uint32_t* layer, size_t len, size_t lde_len, size_t depth,
+                  const uint32_t* challenges, uint32_t* out) {
+    if (!ctx || !layer || !challenges || !out) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] {
+        require(pow2(len) && len >= 4 && pow2(lde_len) && lde_len <= MAX_DOMAIN && (len << (2 * depth)) == lde_len,
+                Z0P_ERR_INVALID_DOMAIN_SIZE);
+        require(lde_len >= 16, Z0P_ERR_INVALID_DOMAIN_SIZE);
+        Ctx& c = *ctx->c;
+        int logm = int(ilog2(lde_len));
+        const GeometryTables& gt = c.geometry(logm - 4, 4);
+        FoldConsts fc{};
+        for (int k = 0; k < 3; ++k)
+            for (int q = 0; q < 4; ++q) fc.ch_m[k][q] = to_mont(challenges[4 * k + q] % P);
+        fc.unit_m = to_mont(root_of_unity(2));
+        fc.ginv_m = to_mont(pow_c(inv_c(GENERATOR), uint64_t(1) << (2 * depth)));
+        fc.log_len = int(ilog2(len));
+        fc.depth = int(depth);
+        c.reserve_arena(len * 20 + (1 << 20));
+        c.reset_arena();
+        uint32_t* dl = to_dev(c, layer, 4 * len);
+        uint32_t* dn = c.alloc<uint32_t>(len);
+        launch_fold(c.launch(), dl, false, dn, gt.winv, fc);
+        to_host(c, out, dn, len);
+    });
+}
+
+int z0p_test_grind(z0p_ctx* ctx, const uint8_t* head, size_t head_len, uint32_t bits, uint64_t* nonce) {
+    if (!ctx || !head || !nonce) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] {
+        require(head_len == 47 && bits <= 32, Z0P_ERR_MALFORMED_INPUT);
+        Ctx& c = *ctx->c;
+        c.reserve_arena(1 << 20);
+        c.reset_arena();
+        uint32_t hw[16] = {0};
+        for (size_t i = 0; i < head_len; ++i) hw[i >> 2] |= uint32_t(head[i]) << (8 * (i & 3));
+        unsigned long long* res = c.alloc<unsigned long long>(1);
+        uint64_t found = ~uint64_t(0);
+        for (uint64_t window = 0; found == ~uint64_t(0); ++window) {
+            uint64_t limit = uint64_t(1) << std::min<uint64_t>(63, 24 + 4 * window);
+            uint64_t init = ~uint64_t(0);
+            std::memcpy(c.pinned, &init, 8);
+            hip_check(hipMemcpyAsync(res, c.pinned, 8, hipMemcpyHostToDevice, c.stream));
+            launch_grind(c.launch(), hw, 47, bits, limit, res);
+            hip_check(hipMemcpyAsync(c.pinned + 8, res, 8, hipMemcpyDeviceToHost, c.stream));
+            c.sync();
+            std::memcpy(&found, c.pinned + 8, 8);
+        }
+        *nonce = found;
+    });
+}
+
+}  // extern "C"

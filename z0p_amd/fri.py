@@ -1,0 +1,55 @@
+"""FRI polynomial commitment (reference: src/fri.rs)."""
+import ctypes
+
+from ._ffi import FriParamsC, Z0pError, buf, check, lib, take, u32_buf, u8p
+
+Params = FriParamsC
+
+
+def _shape(self):
+    return "B%dQ%dG%dC%dF%d" % (self.blowup, self.num_queries, self.grinding_bits, self.cap_height, self.final_size)
+
+
+Params.shape = _shape
+
+
+def params_new(blowup, num_queries, final_size, grinding_bits=0, cap_height=0) -> Params:
+    """fri::Params::new (fri.rs:141-158) plus the builder setters."""
+    p = Params()
+    check(lib().z0p_fri_params_new(blowup, num_queries, final_size, ctypes.byref(p)))
+    p.grinding_bits = grinding_bits
+    p.cap_height = cap_height
+    return p
+
+
+def for_security(poly_len, target_bits=100) -> Params:
+    """fri::Params::for_security (fri.rs:232-255)."""
+    p = Params()
+    check(lib().z0p_fri_params_for_security(poly_len, target_bits, ctypes.byref(p)))
+    return p
+
+
+Params.new = staticmethod(params_new)
+Params.for_security = staticmethod(for_security)
+
+
+def commit_open(coeffs, params: Params, transcript, ctx=None):
+    """fri::commit + public_commitment + open (fri.rs:408-433, 863-870,
+    645-666) on the GPU; returns (Commitment::to_bytes, Proof::to_bytes)."""
+    from . import default_context
+    ctx = ctx or default_context()
+    arr, ptr = u32_buf(coeffs)
+    cm, pf = u8p(), u8p()
+    cn, pn = ctypes.c_size_t(), ctypes.c_size_t()
+    check(lib().z0p_fri_commit_open(ctx.handle, ptr, arr.size, ctypes.byref(params), transcript.handle,
+                                    ctypes.byref(cm), ctypes.byref(cn), ctypes.byref(pf), ctypes.byref(pn)))
+    return take(cm, cn.value), take(pf, pn.value)
+
+
+def verify(commitment: bytes, proof: bytes, params: Params, transcript) -> None:
+    """Commitment/Proof::from_bytes + fri::verify (fri.rs:686-859); raises Z0pError."""
+    check(lib().z0p_fri_verify(buf(commitment), len(commitment), buf(proof), len(proof), ctypes.byref(params),
+                               transcript.handle))
+
+
+__all__ = ["Params", "params_new", "for_security", "commit_open", "verify", "Z0pError"]
