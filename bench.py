@@ -79,6 +79,9 @@ def main():
     ap.add_argument("--n", type=int, default=N_STEPS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle CPU baseline")
     ap.add_argument("--cpu-repeats", type=int, default=3)
+    ap.add_argument("--copy-mib", type=int, default=256,
+                    help="size of the plain copy kernel run after the proofs (counter calibration and "
+                         "achievable-bandwidth reference); 0 disables")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: proofs are independent units and this build runs on a single "
@@ -103,6 +106,17 @@ def main():
     ctx.set_profiling(False)
     stats = ctx.kernel_stats()
     z.air.verify(params, proof, z.Transcript(LABEL))  # outside the timed region, as the reference bench
+
+    copy = None
+    if args.copy_mib:
+        import ctypes
+        from z0p_amd import _ffi
+        cms = ctypes.c_float()
+        nbytes = args.copy_mib << 20
+        _ffi.check(_ffi.lib().z0p_test_copy(ctx.handle, nbytes, ctypes.byref(cms)))
+        _ffi.check(_ffi.lib().z0p_test_copy(ctx.handle, nbytes, ctypes.byref(cms)))
+        copy = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": round(cms.value, 4),
+                "gbs_rw": round(2 * nbytes / (cms.value * 1e-3) / 1e9, 1)}
 
     ms = (t1 - t0) * 1000.0 / args.steps
     m = n * params.blowup
@@ -142,6 +156,7 @@ def main():
                    "proof_bytes": len(proof), "proofs_per_s": round(1000.0 / ms, 2)},
         "kernels_ms_per_proof": {k: round(v[1] / args.steps, 4) for k, v in sorted(stats.items())},
         "roofline": roofline,
+        "copy_reference": copy,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(n, args.cpu_repeats),
     }
     print(json.dumps(out))

@@ -49,6 +49,11 @@ int z0p_test_fold(z0p_ctx* ctx, const uint32_t* layer, size_t len, size_t lde_le
  * `bits` leading zero bits; head must be 47 bytes. */
 int z0p_test_grind(z0p_ctx* ctx, const uint8_t* head, size_t head_len, uint32_t bits, uint64_t* nonce);
 
+/* Plain device copy of `bytes` (multiple of 16) between two arena buffers;
+ * *ms receives its event-timed duration. The calibration kernel for the
+ * FETCH_SIZE / WRITE_SIZE counters. */
+int z0p_test_copy(z0p_ctx* ctx, size_t bytes, float* ms);
+
 #ifdef __cplusplus
 }
 #endif

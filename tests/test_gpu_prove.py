@@ -89,3 +89,47 @@ def test_fri_bench_shape(ctx):
     oc, opf = o.fri_commit_open(coeffs, op, o.Transcript(b"bench/fri"))
     assert c == oc and p == opf
     assert len(c) + len(p) == 11522  # tools/baseline.json:16-20
+
+
+def _seeded_coeffs(n, seed):
+    """c_i = Field::from_u64(splitmix64(&mut st)) (the reference's test idiom)."""
+    import numpy as np
+    from helpers import splitmix64, P
+    out = np.empty(n, dtype=np.uint32)
+    st = seed
+    for i in range(n):
+        st, z_ = splitmix64(st)
+        out[i] = z_ % P
+    return out
+
+
+@pytest.mark.parametrize("logp", [12, 16])
+def test_fri_seeded_byte_identical(ctx, logp):
+    n = 1 << logp
+    coeffs = _seeded_coeffs(n, logp)
+    zp, op = z.fri.for_security(n), o.fri_params_for_security(n)
+    print("fri seeded n", n, zp.shape(), flush=True)
+    c, p = z.fri.commit_open(coeffs, zp, z.Transcript(b"bench/fri-prod-big"), ctx)
+    oc, opf = o.fri_commit_open(coeffs, op, o.Transcript(b"bench/fri-prod-big"))
+    assert c == oc and p == opf
+    z.fri.verify(c, p, zp, z.Transcript(b"bench/fri-prod-big"))
+
+
+def test_fri_2_22_config3(ctx):
+    """Config 3: 2^22 coefficients, 2^26-point LDE, byte-identical to the oracle."""
+    n = 1 << 22
+    coeffs = _seeded_coeffs(n, 0)
+    zp, op = z.fri.for_security(n), o.fri_params_for_security(n)
+    assert zp.shape() == "B16Q40G20C4F16"
+    print("fri config3 n", n, flush=True)
+    c, p = z.fri.commit_open(coeffs, zp, z.Transcript(b"bench/fri-prod-big"), ctx)
+    assert len(c) + len(p) == 181390  # size formula of fri.rs:963-1074 (BASELINE.md)
+    oc, opf = o.fri_commit_open(coeffs, op, o.Transcript(b"bench/fri-prod-big"))
+    assert c == oc and p == opf
+    z.fri.verify(c, p, zp, z.Transcript(b"bench/fri-prod-big"))
+
+
+def test_air_2_20_byte_identical(ctx):
+    """The headline instance (a0 = b0 = 1, N = 2^20): 203,553 bytes, identical to the oracle."""
+    proof = _air_case(1 << 20, 1, 1, b"bench/air-scale", ctx)
+    assert len(proof) == 203553

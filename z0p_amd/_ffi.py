@@ -91,6 +91,7 @@ SIGNATURES = {
     "z0p_test_compose": (ctypes.c_int, [vp, u32p, u32p, sz, sz, u32, u32, u32, u32p, u32p]),
     "z0p_test_fold": (ctypes.c_int, [vp, u32p, sz, sz, sz, u32p, u32p]),
     "z0p_test_grind": (ctypes.c_int, [vp, u8p, sz, u32, ctypes.POINTER(u64)]),
+    "z0p_test_copy": (ctypes.c_int, [vp, sz, ctypes.POINTER(ctypes.c_float)]),
 }
 
 _lib = None

@@ -1,0 +1,198 @@
+// Device helpers shared by the kernels: bit reversal, power-table lookups,
+// Montgomery inversion, LDS transforms, batched inversion, layout helpers,
+// the LDS Merkle subtree reduction and the job-table lookup.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "blake3.hpp"
+#include "field.hpp"
+#include "kernels.hpp"
+
+namespace z0p {
+namespace dev {
+
+constexpr int WG = 256;
+
+__device__ __forceinline__ uint32_t brev_bits(uint32_t x, int bits) {
+    return bits == 0 ? 0u : (__builtin_bitreverse32(x) >> (32 - bits));
+}
+
+__device__ __forceinline__ uint32_t pow_lookup(PowTable t, uint32_t e) {
+    return mont_mul(t.lo[e & 4095u], t.hi[e >> 12]);
+}
+
+// Montgomery-domain inverse: (a^-1)_m from a_m (zero maps to zero).
+// Exponent p-2 = 0b111_0_(27 ones): window-3 chain, 30 squarings + 11 mults.
+__device__ __forceinline__ uint32_t inv_m(uint32_t a) {
+    uint32_t a2 = mont_mul(a, a);
+    uint32_t a3 = mont_mul(a2, a);
+    uint32_t a6 = mont_mul(a3, a3);
+    uint32_t a7 = mont_mul(a6, a);
+    uint32_t acc = mont_mul(a7, a7);
+#pragma unroll
+    for (int w = 0; w < 9; ++w) {
+        acc = mont_mul(acc, acc);
+        acc = mont_mul(acc, acc);
+        acc = mont_mul(acc, acc);
+        acc = mont_mul(acc, a7);
+    }
+    return acc;
+}
+
+// Radix-2 decimation-in-frequency in LDS over `cnt` transforms of size
+// 2^logn at s + k*stride: natural order in, bit-reversed out.
+__device__ __forceinline__ void lds_dif(uint32_t* s, int logn, int cnt, int stride, const uint32_t* tw) {
+    if (logn == 0) return;
+    const int half = 1 << (logn - 1);
+    const int total = half * cnt;
+    for (int lh = logn - 1; lh >= 0; --lh) {
+        const int h = 1 << lh;
+        const int tws = logn - 1 - lh;
+        for (int b = threadIdx.x; b < total; b += blockDim.x) {
+            int k = b >> (logn - 1);
+            int bi = b & (half - 1);
+            int jl = bi & (h - 1);
+            int i = ((bi >> lh) << (lh + 1)) | jl;
+            uint32_t* x = s + k * stride;
+            uint32_t u = x[i], v = x[i + h];
+            x[i] = add(u, v);
+            x[i + h] = mont_mul(sub(u, v), tw[jl << tws]);
+        }
+        __syncthreads();
+    }
+}
+
+// Radix-2 decimation-in-time in LDS: bit-reversed order in, natural out.
+__device__ __forceinline__ void lds_dit(uint32_t* s, int logn, int cnt, int stride, const uint32_t* tw) {
+    if (logn == 0) return;
+    const int half = 1 << (logn - 1);
+    const int total = half * cnt;
+    for (int lh = 0; lh < logn; ++lh) {
+        const int h = 1 << lh;
+        const int tws = logn - 1 - lh;
+        for (int b = threadIdx.x; b < total; b += blockDim.x) {
+            int k = b >> (logn - 1);
+            int bi = b & (half - 1);
+            int jl = bi & (h - 1);
+            int i = ((bi >> lh) << (lh + 1)) | jl;
+            uint32_t* x = s + k * stride;
+            uint32_t u = x[i], v = mont_mul(x[i + h], tw[jl << tws]);
+            x[i] = add(u, v);
+            x[i + h] = sub(u, v);
+        }
+        __syncthreads();
+    }
+}
+
+// tw[e] = w_n^e for e < n/2 from the 4096-point table.
+__device__ __forceinline__ void load_tw(uint32_t* tw, const uint32_t* tw4096, int logn) {
+    const int half = logn == 0 ? 0 : 1 << (logn - 1);
+    for (int e = threadIdx.x; e < half; e += blockDim.x) tw[e] = tw4096[e << (12 - logn)];
+}
+
+// Per-thread Montgomery batch inversion of K register values (nonzero on
+// every proving domain: the coset 31*<Omega> avoids all 2-power roots of
+// unity because 31 generates the full multiplicative group).
+template <int K>
+__device__ __forceinline__ void batch_inv_m(uint32_t (&v)[K]) {
+    uint32_t pre[K];
+    pre[0] = v[0];
+#pragma unroll
+    for (int e = 1; e < K; ++e) pre[e] = mont_mul(pre[e - 1], v[e]);
+    uint32_t inv = inv_m(pre[K - 1]);
+#pragma unroll
+    for (int e = K - 1; e > 0; --e) {
+        uint32_t ve = v[e];
+        v[e] = mont_mul(inv, pre[e - 1]);
+        inv = mont_mul(inv, ve);
+    }
+    v[0] = inv;
+}
+
+// Position p = block*n + q of the bit-reversed LDE -> coset j, step t and
+// the position of the next trace row (t + 1 mod n, same coset).
+struct PosInfo {
+    uint32_t j, t, pnext;
+};
+__device__ __forceinline__ PosInfo pos_info(uint32_t p, int logn, int logb) {
+    uint32_t block = p >> logn;
+    uint32_t q = p & ((1u << logn) - 1);
+    uint32_t t = brev_bits(q, logn);
+    uint32_t t1 = (t + 1) & ((1u << logn) - 1);
+    PosInfo r;
+    r.j = brev_bits(block, logb);
+    r.t = t;
+    r.pnext = (block << logn) | brev_bits(t1, logn);
+    return r;
+}
+
+// x = s_j * omega_n^t (Montgomery form); omega_n = Omega_m^B.
+__device__ __forceinline__ uint32_t point_m(PowTable w_m, const uint32_t* shift_m, uint32_t j, uint32_t t, int logb) {
+    return mont_mul(pow_lookup(w_m, t << logb), shift_m[j]);
+}
+
+__device__ __forceinline__ void store_digest(uint32_t* dst, const uint32_t d[8]) {
+    uint4* g = reinterpret_cast<uint4*>(dst);
+    g[0] = make_uint4(d[0], d[1], d[2], d[3]);
+    g[1] = make_uint4(d[4], d[5], d[6], d[7]);
+}
+
+// Merkle subtree reduction in LDS. Digests word-major (SoA) so a thread
+// reads both children of word k with one 8-byte access.
+constexpr int SUB_IN = 2 * WG;
+
+// cur holds `count` digests of the level at level_ptr (count_total digests,
+// this group starting at index first); writes up to nlevels parent levels,
+// which follow each other contiguously in global memory.
+__device__ __forceinline__ void subtree_levels(uint32_t (*cur)[SUB_IN], uint32_t (*nxt)[SUB_IN], int count,
+                                               int nlevels, uint32_t* level_ptr, uint64_t count_total,
+                                               uint64_t first) {
+    uint32_t* lvl = level_ptr;
+    uint64_t lc = count_total;
+    for (int l = 1; l <= nlevels && count > 1; ++l) {
+        uint32_t* next_lvl = lvl + lc * 8;
+        int pc = count >> 1;
+        for (int t = threadIdx.x; t < pc; t += blockDim.x) {
+            uint32_t L[8], R[8], o[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                L[k] = cur[k][2 * t];
+                R[k] = cur[k][2 * t + 1];
+            }
+            node_words(L, R, o);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) nxt[k][t] = o[k];
+            store_digest(next_lvl + (first / 2 + t) * 8, o);
+        }
+        __syncthreads();
+        auto tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+        count = pc;
+        lvl = next_lvl;
+        lc >>= 1;
+        first >>= 1;
+    }
+}
+
+// This workgroup's job: table sorted by block_begin (njobs > 0), or the
+// inline descriptor when njobs == 0. `local` is the block index inside the job.
+template <typename D>
+__device__ __forceinline__ D job_of(const D& inline_d, const D* table, uint32_t njobs, uint32_t& local) {
+    const uint32_t b = blockIdx.x;
+    if (njobs == 0) {
+        local = b;
+        return inline_d;
+    }
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (table[mid].block_begin <= b) lo = mid;
+        else hi = mid;
+    }
+    local = b - table[lo].block_begin;
+    return table[lo];
+}
+
+}  // namespace dev
+}  // namespace z0p

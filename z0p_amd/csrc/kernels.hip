@@ -649,6 +649,14 @@ __global__ void k_gather(const uint32_t* base, const uint64_t* src, const uint32
     for (uint32_t k = 0; k < len[i]; ++k) d[k] = s[k];
 }
 
+// Plain streaming copy of `n16` 16-byte words: the calibration reference for
+// the FETCH_SIZE / WRITE_SIZE hardware counters.
+__global__ void k_copy(const uint4* __restrict__ in, uint4* __restrict__ out, uint64_t n16) {
+    uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x;
+    const uint64_t stride = uint64_t(gridDim.x) * WG;
+    for (; i < n16; i += stride) out[i] = in[i];
+}
+
 inline unsigned grid_for(uint64_t work, uint64_t per_block) { return unsigned((work + per_block - 1) / per_block); }
 
 }  // namespace
@@ -793,6 +801,12 @@ void launch_grind(const Launch& L, const uint32_t hw[16], uint32_t head_len, uin
     uint4 h2 = make_uint4(hw[8], hw[9], hw[10], hw[11]), h3 = make_uint4(hw[12], hw[13], hw[14], hw[15]);
     (void)head_len;  // fixed 47-byte head, checked by the caller
     hipLaunchKernelGGL(k_grind, dim3(2048), dim3(WG), 0, L.stream, h0, h1, h2, h3, bits, limit, result);
+}
+
+void launch_copy(const Launch& L, const uint32_t* in, uint32_t* out, uint64_t words) {
+    ProfScope ps(L.prof, "copy", L.stream);
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(WG), 0, L.stream, reinterpret_cast<const uint4*>(in),
+                       reinterpret_cast<uint4*>(out), words / 4);
 }
 
 void launch_gather(const Launch& L, const uint32_t* base, const uint64_t* src, const uint32_t* dst,

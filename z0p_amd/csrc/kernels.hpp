@@ -100,6 +100,9 @@ void launch_fold(const Launch& L, const uint32_t* layer, bool base_field, uint32
 void launch_grind(const Launch& L, const uint32_t head_words[16], uint32_t head_len, uint32_t bits, uint64_t limit,
                   unsigned long long* result);
 
+// Streaming copy of `words` (a multiple of 4) u32: counter calibration.
+void launch_copy(const Launch& L, const uint32_t* in, uint32_t* out, uint64_t words);
+
 // Gather words: for item i copy len[i] words from base + src[i] to out + dst[i].
 void launch_gather(const Launch& L, const uint32_t* base, const uint64_t* src, const uint32_t* dst,
                    const uint32_t* len, uint32_t items, uint32_t* out);

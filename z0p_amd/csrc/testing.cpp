@@ -222,6 +222,29 @@ int z0p_test_fold(z0p_ctx* ctx, const uint32_t* layer, size_t len, size_t lde_le
     });
 }
 
+int z0p_test_copy(z0p_ctx* ctx, size_t bytes, float* ms) {
+    if (!ctx || !ms) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] {
+        require(bytes % 16 == 0 && bytes > 0, Z0P_ERR_MALFORMED_INPUT);
+        Ctx& c = *ctx->c;
+        c.reserve_arena(2 * bytes + (1 << 20));
+        c.reset_arena();
+        uint32_t* a = c.alloc<uint32_t>(bytes / 4);
+        uint32_t* b = c.alloc<uint32_t>(bytes / 4);
+        hip_check(hipMemsetAsync(a, 1, bytes, c.stream));
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0));
+        hip_check(hipEventCreate(&e1));
+        hip_check(hipEventRecord(e0, c.stream));
+        launch_copy(c.launch(), a, b, bytes / 4);
+        hip_check(hipEventRecord(e1, c.stream));
+        c.sync();
+        hip_check(hipEventElapsedTime(ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+
 int z0p_test_grind(z0p_ctx* ctx, const uint8_t* head, size_t head_len, uint32_t bits, uint64_t* nonce) {
     if (!ctx || !head || !nonce) return Z0P_ERR_INVALID_ARGUMENT;
     return guarded([&] {
