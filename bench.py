@@ -24,26 +24,30 @@ HBM_PEAK_GBS = 8000.0
 
 
 def algorithmic_bytes(n, m, layers):
-    """Per-kernel-family algorithmic HBM bytes for one proof (1 read + 1
-    write of every materialised array)."""
-    fri_leaf = sum(16 * L + 8 * L + 12 * L for L in layers)  # values read, leaf digests, node traffic
-    fold = sum(16 * L + 4 * L for L in layers)
+    """Algorithmic HBM bytes per proof for each kernel family: every array a
+    family must read or produce, counted once (u32 words, Ext4 = 16 bytes,
+    digests = 32 bytes). Tree families count the leaf input once and every
+    digest written once."""
+    def tree_digests(leaves):
+        return 2 * leaves - 1
+    fri_tree = sum(16 * L + 32 * tree_digests(L // 4) for L in layers)
     return {
-        "intt": 8 * n,
-        "lde": 4 * n + 4 * m,
-        "linked_b": 8 * m,
-        "trace_tree": 8 * m + 4 * m + 4 * m + 2 * m,
-        "compose": 8 * m + 16 * m,
-        "fri_tree": fri_leaf,
-        "fold": fold,
+        "intt": 4 * n + 4 * n,                       # evaluations in, coefficients out
+        "lde": 4 * n + 4 * m,                        # coefficients in, m evaluations out
+        "linked_b": 4 * m + 4 * m,                   # A in, B out
+        "compose": 8 * m + 16 * m,                   # A, B in, Ext4 layer 0 out
+        "trace_tree": 8 * m + 32 * tree_digests(m // 8),
+        "fri_tree": fri_tree,
+        "fold": sum(16 * L + 4 * L for L in layers),  # layer in, layer/4 Ext4 out
     }
 
 
+# Kernel launch names -> families. Tree level kernels are shared by the
+# trace and FRI trees and are reported as one "merkle" family beside them.
 FAMILY = {
-    "intt_small": "intt", "intt_cols": "intt", "intt_rows": "intt",
-    "lde_small": "lde", "lde_rows": "lde", "lde_cols": "lde",
-    "linked_b": "linked_b", "trace_leaves": "trace_tree", "compose": "compose",
-    "fri_leaves": "fri_tree", "fri_leaves_base": "fri_tree", "fold": "fold", "fold_base": "fold",
+    "intt": "intt", "lde": "lde", "linked_b": "linked_b", "compose": "compose",
+    "trace_leaves": "merkle", "fri_leaves": "merkle", "tree_levels": "merkle", "tree_tail": "merkle",
+    "fold": "fold",
 }
 
 
@@ -77,6 +81,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=N_STEPS)
+    ap.add_argument("--prof-steps", type=int, default=3, help="extra proofs timed per kernel (HIP events)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle CPU baseline")
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=256,
@@ -97,12 +102,17 @@ def main():
     proof = None
     for _ in range(args.warmup):
         proof = z.air.prove(params, z.Transcript(LABEL), ctx)
-    ctx.set_profiling(True)
-    ctx.reset_stats()
+    # Timed region: plain proofs, no per-kernel events (each call returns
+    # the serialized proof, so the device is idle at both ends).
     t0 = time.perf_counter()
     for _ in range(args.steps):
         proof = z.air.prove(params, z.Transcript(LABEL), ctx)
     t1 = time.perf_counter()
+    # Per-kernel breakdown: separate proofs with HIP events around every launch.
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    for _ in range(args.prof_steps):
+        z.air.prove(params, z.Transcript(LABEL), ctx)
     ctx.set_profiling(False)
     stats = ctx.kernel_stats()
     z.air.verify(params, proof, z.Transcript(LABEL))  # outside the timed region, as the reference bench
@@ -126,11 +136,23 @@ def main():
         layers.append(L)
         L //= 4
     alg = algorithmic_bytes(n, m, layers)
-    fam_time = {}
+    alg["merkle"] = alg.pop("trace_tree") + alg.pop("fri_tree")
+    per = max(args.prof_steps, 1)
+    fam_time, fam_launches = {}, {}
     for k, (cnt, tot) in stats.items():
         f = FAMILY.get(k)
         if f:
-            fam_time[f] = fam_time.get(f, 0.0) + tot / args.steps
+            fam_time[f] = fam_time.get(f, 0.0) + tot / per
+            fam_launches[f] = fam_launches.get(f, 0) + cnt / per
+    families = {f: {"ms_per_proof": round(t, 4), "launches_per_proof": fam_launches[f],
+                    "alg_bytes": alg[f], "achieved_gbs": round(alg[f] / (t * 1e-3) / 1e9, 1)}
+                for f, t in sorted(fam_time.items())}
+    # Merkle hashing is integer-VALU work: report BLAKE3 compressions/s too.
+    cap = 1 << params.cap_height
+    comp = (m // 8) + 2 * (m // 8 - cap) + sum(L // 4 + 2 * (L // 4 - cap) for L in layers)
+    if "merkle" in families:
+        families["merkle"]["compressions"] = comp
+        families["merkle"]["gcompress_per_s"] = round(comp / (fam_time["merkle"] * 1e-3) / 1e9, 2)
     dom = max(fam_time, key=fam_time.get) if fam_time else None
     roofline = None
     if dom:
@@ -154,7 +176,8 @@ def main():
         "data": "synthetic (a0 = b0 = 1, label bench/air-scale)",
         "config": {"workload": "AIR Fibonacci N=%d production schedule %s (non-ZK)" % (n, params.shape()),
                    "proof_bytes": len(proof), "proofs_per_s": round(1000.0 / ms, 2)},
-        "kernels_ms_per_proof": {k: round(v[1] / args.steps, 4) for k, v in sorted(stats.items())},
+        "kernels_ms_per_proof": {k: round(v[1] / per, 4) for k, v in sorted(stats.items())},
+        "families": families,
         "roofline": roofline,
         "copy_reference": copy,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(n, args.cpu_repeats),

@@ -1,9 +1,11 @@
-// Device context: one HIP stream, a bump-allocated device arena reused across
-// proofs, pinned host staging, and cached per-geometry constant tables.
+// Device context: one HIP stream, a device memory pool for per-proof
+// buffers, a descriptor ring for batched kernel tables, a staging area for
+// device-to-host collection, and cached per-geometry constant tables.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <vector>
@@ -25,64 +27,200 @@ inline void hip_check(hipError_t e) {
 struct GeometryTables {
     int logn = 0, logb = 0;
     std::vector<void*> owned;
-    PowTable w{}, winv{};           // omega_m^{+e}, omega_m^{-e}
+    PowTable w{}, winv{};            // omega_m^{+e}, omega_m^{-e}
     CosetTables coset{};
     const uint32_t* zinv = nullptr;  // [B] 1/(s_j^n - 1), Montgomery
+    const uint32_t* zvan = nullptr;  // [B] s_j^n - 1, Montgomery
     ~GeometryTables() {
         for (void* p : owned) (void)hipFree(p);
     }
+};
+
+// First-fit allocator over one device allocation (host-side bookkeeping):
+// proofs of different sizes come and go without device allocation calls.
+class DevicePool {
+   public:
+    ~DevicePool() {
+        if (base_) (void)hipFree(base_);
+    }
+    size_t capacity() const { return cap_; }
+    size_t in_use() const { return in_use_; }
+    // Resize; only valid while nothing is allocated.
+    void reset_capacity(size_t bytes) {
+        require(in_use_ == 0, Z0P_ERR_INVALID_ARGUMENT);
+        if (base_) hip_check(hipFree(base_));
+        base_ = nullptr;
+        cap_ = 0;
+        free_.clear();
+        if (bytes == 0) return;
+        hip_check(hipMalloc(&base_, bytes));
+        cap_ = bytes;
+        free_[0] = bytes;
+    }
+    bool alloc(size_t bytes, uint8_t*& out) {
+        bytes = (bytes + 255) & ~size_t(255);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            size_t off = it->first, sz = it->second;
+            free_.erase(it);
+            if (sz > bytes) free_[off + bytes] = sz - bytes;
+            in_use_ += bytes;
+            out = base_ + off;
+            return true;
+        }
+        return false;
+    }
+    void release(uint8_t* p, size_t bytes) {
+        bytes = (bytes + 255) & ~size_t(255);
+        size_t off = size_t(p - base_);
+        in_use_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto next = std::next(it);
+        if (next != free_.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_.erase(next);
+        }
+        if (it != free_.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    uint8_t* base() const { return base_; }
+
+   private:
+    uint8_t* base_ = nullptr;
+    size_t cap_ = 0, in_use_ = 0;
+    std::map<size_t, size_t> free_;
+};
+
+// Pinned host ring mirrored in device memory: the descriptor tables and
+// small uploads of one scheduler iteration. Pushes fill the host side and
+// return the matching device address; flush() uploads everything pushed so
+// far with one copy. In replay mode pushes only recompute addresses, which
+// match the recorded ones when the same pushes repeat in the same order.
+// Reset after the stream has been synchronised.
+class DescRing final : public TableSink {
+   public:
+    ~DescRing() override {
+        if (host_) (void)hipHostFree(host_);
+        if (dev_) (void)hipFree(dev_);
+    }
+    void init(size_t bytes) {
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_), bytes, hipHostMallocDefault));
+        hip_check(hipMalloc(reinterpret_cast<void**>(&dev_), bytes));
+        cap_ = bytes;
+    }
+    const void* push(const void* p, size_t bytes) override {
+        size_t at = (used_ + 255) & ~size_t(255);
+        if (replay_) {
+            require(at + bytes <= flushed_, Z0P_ERR_DEVICE);
+        } else {
+            require(at + bytes <= cap_, Z0P_ERR_OUT_OF_MEMORY);
+            std::memcpy(host_ + at, p, bytes);
+        }
+        used_ = at + bytes;
+        return dev_ + at;
+    }
+    size_t mark() const { return used_; }
+    void flush(hipStream_t s) {
+        if (used_ > flushed_)
+            hip_check(hipMemcpyAsync(dev_ + flushed_, host_ + flushed_, used_ - flushed_, hipMemcpyHostToDevice, s));
+        flushed_ = used_;
+    }
+    void replay_from(size_t mark) {
+        used_ = mark;
+        replay_ = true;
+    }
+    void end_replay() { replay_ = false; }
+    void reset() {
+        used_ = flushed_ = 0;
+        replay_ = false;
+    }
+
+   private:
+    uint8_t* host_ = nullptr;
+    uint8_t* dev_ = nullptr;
+    size_t cap_ = 0, used_ = 0, flushed_ = 0;
+    bool replay_ = false;
 };
 
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     Profiler prof;
+    DevicePool pool;
+    DescRing ring;
+    size_t hbm_budget = 0;  // 0: the pool grows to fit (between batches)
 
-    uint8_t* arena = nullptr;
-    size_t arena_cap = 0;
-    size_t arena_used = 0;
+    // Staging for device-to-host collection (device + pinned mirror).
+    uint32_t* stage_dev = nullptr;
+    uint32_t* stage_host = nullptr;
+    size_t stage_words = 0;
 
-    uint8_t* pinned = nullptr;
-    size_t pinned_cap = 0;
+    // Scratch arena for the kernel-level diagnostic hooks.
+    uint8_t* scratch = nullptr;
+    size_t scratch_cap = 0, scratch_used = 0;
 
     uint32_t* tw4096 = nullptr;      // [2048] omega_4096^e, Montgomery
     uint32_t* tw4096_inv = nullptr;  // [2048] omega_4096^{-e}, Montgomery
     std::map<std::pair<int, int>, std::unique_ptr<GeometryTables>> tables;
 
-    Launch launch() { return Launch{stream, &prof}; }
+    Launch launch() { return Launch{stream, &prof, &ring, false}; }
 
-    // Ensure the arena holds `bytes`; only valid between proofs.
-    void reserve_arena(size_t bytes) {
-        if (bytes <= arena_cap) return;
-        hip_check(hipStreamSynchronize(stream));
-        if (arena) hip_check(hipFree(arena));
-        arena = nullptr;
-        arena_cap = 0;
-        size_t cap = bytes + (bytes >> 3);
-        hip_check(hipMalloc(&arena, cap));
-        arena_cap = cap;
+    // Runs a launch sequence dry (tables only), uploads the tables with one
+    // copy, then replays it for real.
+    template <typename F>
+    void enqueue(F&& launches) {
+        const size_t mark = ring.mark();
+        Launch dry = launch();
+        dry.dry = true;
+        launches(static_cast<const Launch&>(dry));
+        ring.flush(stream);
+        ring.replay_from(mark);
+        launches(static_cast<const Launch&>(launch()));
+        ring.end_replay();
     }
-    void reset_arena() { arena_used = 0; }
+
+    void reserve_stage(size_t words) {
+        if (words <= stage_words) return;
+        hip_check(hipStreamSynchronize(stream));
+        if (stage_dev) hip_check(hipFree(stage_dev));
+        if (stage_host) hip_check(hipHostFree(stage_host));
+        stage_dev = nullptr;
+        stage_host = nullptr;
+        size_t w = words + (words >> 2) + 1024;
+        hip_check(hipMalloc(reinterpret_cast<void**>(&stage_dev), w * 4));
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&stage_host), w * 4, hipHostMallocDefault));
+        stage_words = w;
+    }
+
+    void reserve_scratch(size_t bytes) {
+        if (bytes > scratch_cap) {
+            hip_check(hipStreamSynchronize(stream));
+            if (scratch) hip_check(hipFree(scratch));
+            scratch = nullptr;
+            scratch_cap = 0;
+            hip_check(hipMalloc(&scratch, bytes));
+            scratch_cap = bytes;
+        }
+        scratch_used = 0;
+    }
     template <typename T>
-    T* alloc(size_t count) {
+    T* scratch_alloc(size_t count) {
         size_t bytes = (count * sizeof(T) + 255) & ~size_t(255);
-        require(arena_used + bytes <= arena_cap, Z0P_ERR_OUT_OF_MEMORY);
-        T* p = reinterpret_cast<T*>(arena + arena_used);
-        arena_used += bytes;
+        require(scratch_used + bytes <= scratch_cap, Z0P_ERR_OUT_OF_MEMORY);
+        T* p = reinterpret_cast<T*>(scratch + scratch_used);
+        scratch_used += bytes;
         return p;
     }
-    void reserve_pinned(size_t bytes) {
-        if (bytes <= pinned_cap) return;
-        hip_check(hipStreamSynchronize(stream));
-        if (pinned) hip_check(hipHostFree(pinned));
-        pinned = nullptr;
-        pinned_cap = 0;
-        hip_check(hipHostMalloc(reinterpret_cast<void**>(&pinned), bytes, hipHostMallocDefault));
-        pinned_cap = bytes;
-    }
+
     void sync() {
         hip_check(hipStreamSynchronize(stream));
         if (prof.enabled) prof.resolve();
+        ring.reset();
     }
 
     const GeometryTables& geometry(int logn, int logb);

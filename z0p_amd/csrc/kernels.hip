@@ -7,58 +7,35 @@
 //    contiguous block, FRI quads are adjacent, and the next trace row
 //    (i + B) stays inside its block.
 //  - Digests are 8 little-endian u32 words; tree levels are stored
-//    contiguously leaf level first.
+//    contiguously, leaf level first.
+//  - Every kernel takes (inline descriptor, table, njobs): njobs == 0 runs
+//    the inline job over the whole grid, otherwise each workgroup looks its
+//    job up by block index (device_common.hpp: job_of).
 #include <hip/hip_runtime.h>
 
-#include "blake3.hpp"
-#include "field.hpp"
-#include "kernels.hpp"
+#include <algorithm>
+
+#include "device_common.hpp"
 #include "profiler.hpp"
 
 namespace z0p {
 
+using namespace dev;
+
 namespace {
-
-constexpr int WG = 256;
-
-__device__ __forceinline__ uint32_t brev_bits(uint32_t x, int bits) {
-    return bits == 0 ? 0u : (__builtin_bitreverse32(x) >> (32 - bits));
-}
-
-__device__ __forceinline__ uint32_t pow_lookup(PowTable t, uint32_t e) {
-    return mont_mul(t.lo[e & 4095u], t.hi[e >> 12]);
-}
-
-// Montgomery-domain inverse: (a^-1)_m from a_m, a != 0 (a = 0 maps to 0).
-// Exponent p-2 = 0b111_0_(27 ones): window-3 chain, 30 squarings + 11 mults.
-__device__ __forceinline__ uint32_t inv_m(uint32_t a) {
-    uint32_t a2 = mont_mul(a, a);
-    uint32_t a3 = mont_mul(a2, a);
-    uint32_t a6 = mont_mul(a3, a3);
-    uint32_t a7 = mont_mul(a6, a);  // t3 = a^(2^3-1)
-    uint32_t acc = mont_mul(a7, a7);  // bits 30..27 = 1110
-#pragma unroll
-    for (int w = 0; w < 9; ++w) {
-        acc = mont_mul(acc, acc);
-        acc = mont_mul(acc, acc);
-        acc = mont_mul(acc, acc);
-        acc = mont_mul(acc, a7);
-    }
-    return acc;
-}
 
 // ---------------------------------------------------------------------------
 // Fibonacci trace column A: A[i] = F(i-1) a0 + F(i) b0 (F(-1) = 1, F(0) = 0).
 // Each thread jumps to its chunk start with fast doubling, then iterates.
 constexpr int FIB_CHUNK = 64;
 
-__global__ void k_fib_trace(uint32_t* col, uint64_t n, uint32_t a0, uint32_t b0) {
-    uint64_t start = (uint64_t(blockIdx.x) * WG + threadIdx.x) * FIB_CHUNK;
-    if (start >= n) return;
-    // (f, g) = (F(k), F(k+1)) in Montgomery form by fast doubling over bits of start.
-    uint32_t f = 0, g = R1;
+__global__ __launch_bounds__(WG) void k_fib(FibJob inl, const FibJob* tab, uint32_t njobs) {
+    uint32_t local;
+    const FibJob J = job_of(inl, tab, njobs, local);
+    uint64_t start = (uint64_t(local) * WG + threadIdx.x) * FIB_CHUNK;
+    if (start >= J.n) return;
+    uint32_t f = 0, g = R1;  // (F(k), F(k+1)) in Montgomery form
     for (int b = 63; b >= 0; --b) {
-        // F(2k) = F(k) (2F(k+1) - F(k)); F(2k+1) = F(k)^2 + F(k+1)^2
         uint32_t f2 = mont_mul(f, sub(dbl(g), f));
         uint32_t g2 = add(mont_mul(f, f), mont_mul(g, g));
         if ((start >> b) & 1) {
@@ -69,13 +46,12 @@ __global__ void k_fib_trace(uint32_t* col, uint64_t n, uint32_t a0, uint32_t b0)
             g = g2;
         }
     }
-    // F(s-1) = F(s+1) - F(s)
     uint32_t fm1 = sub(g, f);
-    uint32_t a = add(mont_mul(a0, fm1), mont_mul(b0, f));       // A[s]
-    uint32_t bb = add(mont_mul(a0, f), mont_mul(b0, g));        // B[s]
-    uint64_t end = start + FIB_CHUNK < n ? start + FIB_CHUNK : n;
+    uint32_t a = add(mont_mul(J.a0, fm1), mont_mul(J.b0, f));
+    uint32_t bb = add(mont_mul(J.a0, f), mont_mul(J.b0, g));
+    uint64_t end = start + FIB_CHUNK < J.n ? start + FIB_CHUNK : J.n;
     for (uint64_t i = start; i < end; ++i) {
-        col[i] = a;
+        J.col[i] = a;
         uint32_t nb = add(a, bb);
         a = bb;
         bb = nb;
@@ -83,263 +59,186 @@ __global__ void k_fib_trace(uint32_t* col, uint64_t n, uint32_t a0, uint32_t b0)
 }
 
 // ---------------------------------------------------------------------------
-// In-LDS radix-2 decimation-in-frequency over `cnt` transforms of size
-// 2^logn at s + k*stride: natural order in, bit-reversed order out.
-// tw[e] = w^e (Montgomery), e < n/2, w the n-th root for this direction.
-__device__ void lds_dif(uint32_t* s, int logn, int cnt, int stride, const uint32_t* tw) {
-    if (logn == 0) return;
-    const int half = 1 << (logn - 1);
-    const int total = half * cnt;
-    for (int lh = logn - 1; lh >= 0; --lh) {
-        const int h = 1 << lh;
-        const int tws = logn - 1 - lh;
-        for (int b = threadIdx.x; b < total; b += blockDim.x) {
-            int k = b >> (logn - 1);
-            int bi = b & (half - 1);
-            int jl = bi & (h - 1);
-            int i = ((bi >> lh) << (lh + 1)) | jl;
-            uint32_t* x = s + k * stride;
-            uint32_t u = x[i], v = x[i + h];
-            x[i] = add(u, v);
-            x[i + h] = mont_mul(sub(u, v), tw[jl << tws]);
-        }
-        __syncthreads();
-    }
-}
+// NTT passes. n = n1 * n2; "rows" are contiguous length-n2 segments,
+// "columns" strided length-n1 transforms handled `cols` at a time.
+//
+// INTT (natural x[r*n2 + c]):
+//   cols (over r): Y[u*n2 + c] = w_n^{-uc} * sum_r x[r n2 + c] w_n1^{-ur}
+//   rows (over c): X[u + n1 v] at brev(u)*n2 + brev(v), times 1/n
+// LDE (bit-reversed coeffs, k = r*n1 + c sits at brev(c)*n2 + brev(r)):
+//   rows (over r): Y[c*n2 + u] = s_j^c w_n^{uc} * sum_r c_k (s_j^{n1})^r w_n2^{ur}
+//   cols (over c): X[u + n2 v] at block_j*n + brev(u)*n1 + brev(v)
+constexpr int COL_LDS_WORDS = 8192;
 
-// In-LDS radix-2 decimation-in-time: bit-reversed order in, natural out.
-__device__ void lds_dit(uint32_t* s, int logn, int cnt, int stride, const uint32_t* tw) {
-    if (logn == 0) return;
-    const int half = 1 << (logn - 1);
-    const int total = half * cnt;
-    for (int lh = 0; lh < logn; ++lh) {
-        const int h = 1 << lh;
-        const int tws = logn - 1 - lh;
-        for (int b = threadIdx.x; b < total; b += blockDim.x) {
-            int k = b >> (logn - 1);
-            int bi = b & (half - 1);
-            int jl = bi & (h - 1);
-            int i = ((bi >> lh) << (lh + 1)) | jl;
-            uint32_t* x = s + k * stride;
-            uint32_t u = x[i], v = mont_mul(x[i + h], tw[jl << tws]);
-            x[i] = add(u, v);
-            x[i + h] = sub(u, v);
-        }
-        __syncthreads();
-    }
-}
-
-__device__ __forceinline__ void load_tw(uint32_t* tw, const uint32_t* tw4096, int logn) {
-    const int half = logn == 0 ? 0 : 1 << (logn - 1);
-    for (int e = threadIdx.x; e < half; e += blockDim.x) tw[e] = tw4096[e << (12 - logn)];
-}
-
-// Single-workgroup NTT for n <= 4096.
-// INTT: natural evals -> bit-reversed coeffs * 1/n.
-__global__ void k_intt_small(const uint32_t* in, uint32_t* out, int logn, const uint32_t* tw4096, uint32_t ninv_m) {
-    extern __shared__ uint32_t lds[];
-    const int n = 1 << logn;
+__device__ void ntt_intt_small(const NttJob& J, uint32_t* lds) {
+    const int n = 1 << J.logn;
     uint32_t* s = lds;
     uint32_t* tw = lds + n;
-    load_tw(tw, tw4096, logn);
-    for (int k = threadIdx.x; k < n; k += blockDim.x) s[k] = in[k];
+    load_tw(tw, J.tw4096, J.logn);
+    for (int k = threadIdx.x; k < n; k += blockDim.x) s[k] = J.in[k];
     __syncthreads();
-    lds_dif(s, logn, 1, n, tw);
-    for (int q = threadIdx.x; q < n; q += blockDim.x) out[q] = mont_mul(s[q], ninv_m);
+    lds_dif(s, J.logn, 1, n, tw);
+    for (int q = threadIdx.x; q < n; q += blockDim.x) J.out[q] = mont_mul(s[q], J.ninv_m);
 }
 
-// LDE coset j = blockIdx.x: bit-reversed coeffs -> coset evals (bit-reversed).
-__global__ void k_lde_small(const uint32_t* in, uint32_t* out, int logn, int logb, const uint32_t* tw4096,
-                            const uint32_t* scale) {
-    extern __shared__ uint32_t lds[];
-    const int n = 1 << logn;
-    const int j = blockIdx.x;
+__device__ void ntt_lde_small(const NttJob& J, uint32_t* lds, uint32_t j) {
+    const int n = 1 << J.logn;
     uint32_t* s = lds;
     uint32_t* tw = lds + n;
-    load_tw(tw, tw4096, logn);
-    const uint32_t* sc = scale + size_t(j) * n;
+    load_tw(tw, J.tw4096, J.logn);
+    const uint32_t* sc = J.tab_a + size_t(j) * n;
     for (int P = threadIdx.x; P < n; P += blockDim.x) {
-        uint32_t k = brev_bits(P, logn);
-        s[k] = mont_mul(in[P], sc[k]);
+        uint32_t k = brev_bits(P, J.logn);
+        s[k] = mont_mul(J.in[P], sc[k]);
     }
     __syncthreads();
-    lds_dif(s, logn, 1, n, tw);
-    uint32_t* o = out + (size_t(brev_bits(j, logb)) << logn);
+    lds_dif(s, J.logn, 1, n, tw);
+    uint32_t* o = J.out + (size_t(brev_bits(j, J.logb)) << J.logn);
     for (int q = threadIdx.x; q < n; q += blockDim.x) o[q] = s[q];
 }
 
-// Two-pass NTT, n = n1 * n2. "Rows" are contiguous length-n2 segments;
-// "columns" are strided length-n1 transforms handled COLS at a time.
-//
-// INTT (natural x[r*n2 + c]):
-//   pass A (columns, over r):  Y[u*n2 + c] = w_n^{-uc} * sum_r x[r n2 + c] w_n1^{-ur}
-//   pass B (rows, over c):     X[u + n1 v] at brev(u)*n2 + brev(v), times 1/n
-// LDE (bit-reversed coeffs, k = r*n1 + c sits at brev(c)*n2 + brev(r)):
-//   pass A (rows, over r):     Y[c*n2 + u] = s_j^c w_n^{uc} * sum_r c_k (s_j^{n1})^r w_n2^{ur}
-//   pass B (columns, over c):  X[u + n2 v] at block_j*n + brev(u)*n1 + brev(v)
-constexpr int COL_LDS_WORDS = 8192;
-
-// INTT pass A. Column transforms use a padded LDS stride (n1 + 1) so the
-// transposing loads and stores hit distinct banks.
-__global__ void k_intt_cols(const uint32_t* x, uint32_t* y, int log1, int log2, int cols, const uint32_t* tw4096,
-                            PowTable winv_m, int logb) {
-    extern __shared__ uint32_t lds[];
-    const int n1 = 1 << log1, n2 = 1 << log2;
+// Column transforms use a padded LDS stride (n1 + 1) so the transposing
+// loads and stores hit distinct banks.
+__device__ void ntt_intt_cols(const NttJob& J, uint32_t* lds, uint32_t bx) {
+    const int n1 = 1 << J.log1, n2 = 1 << J.log2, cols = J.cols;
     const int st = n1 + 1;
-    uint32_t* s = lds;  // [cols][n1 + 1]
+    uint32_t* s = lds;
     uint32_t* tw = lds + cols * st;
-    load_tw(tw, tw4096, log1);
-    const int c0 = blockIdx.x * cols;
+    load_tw(tw, J.tw4096, J.log1);
+    const int c0 = bx * cols;
     for (int e = threadIdx.x; e < cols * n1; e += blockDim.x) {
         int r = e / cols, c = e % cols;
-        s[c * st + r] = x[size_t(r) * n2 + c0 + c];
+        s[c * st + r] = J.in[size_t(r) * n2 + c0 + c];
     }
     __syncthreads();
-    lds_dif(s, log1, cols, st, tw);
+    lds_dif(s, J.log1, cols, st, tw);
     for (int e = threadIdx.x; e < cols * n1; e += blockDim.x) {
         int q = e / cols, c = e % cols;
-        uint32_t u = brev_bits(q, log1);
+        uint32_t u = brev_bits(q, J.log1);
         uint32_t cc = uint32_t(c0 + c);
-        uint32_t tw_uc = pow_lookup(winv_m, (u * cc) << logb);
-        y[size_t(u) * n2 + cc] = mont_mul(s[c * st + q], tw_uc);
+        uint32_t tw_uc = pow_lookup(J.pt, (u * cc) << J.logb);
+        J.out[size_t(u) * n2 + cc] = mont_mul(s[c * st + q], tw_uc);
     }
 }
 
-// INTT pass B: one row u per workgroup.
-__global__ void k_intt_rows(const uint32_t* y, uint32_t* out, int log1, int log2, const uint32_t* tw4096,
-                            uint32_t ninv_m) {
-    extern __shared__ uint32_t lds[];
-    const int n2 = 1 << log2;
+__device__ void ntt_intt_rows(const NttJob& J, uint32_t* lds, uint32_t u) {
+    const int n2 = 1 << J.log2;
     uint32_t* s = lds;
     uint32_t* tw = lds + n2;
-    load_tw(tw, tw4096, log2);
-    const uint32_t u = blockIdx.x;
-    const uint32_t* row = y + size_t(u) * n2;
+    load_tw(tw, J.tw4096, J.log2);
+    const uint32_t* row = J.in + size_t(u) * n2;
     for (int c = threadIdx.x; c < n2; c += blockDim.x) s[c] = row[c];
     __syncthreads();
-    lds_dif(s, log2, 1, n2, tw);
-    uint32_t* o = out + size_t(brev_bits(u, log1)) * n2;
-    for (int q = threadIdx.x; q < n2; q += blockDim.x) o[q] = mont_mul(s[q], ninv_m);
+    lds_dif(s, J.log2, 1, n2, tw);
+    uint32_t* o = J.out + size_t(brev_bits(u, J.log1)) * n2;
+    for (int q = threadIdx.x; q < n2; q += blockDim.x) o[q] = mont_mul(s[q], J.ninv_m);
 }
 
-// LDE pass A: one bit-reversed coefficient row R per workgroup, coset
-// j = blockIdx.y. The row is already in bit-reversed order, so a DIT
-// transform consumes it in place; rowtab is stored bit-reversed to match.
-__global__ void k_lde_rows(const uint32_t* coeffs, uint32_t* y, int log1, int log2, int logb, const uint32_t* tw4096,
-                           PowTable w_m, const uint32_t* rowtab_br, const uint32_t* coltab) {
-    extern __shared__ uint32_t lds[];
-    const int n1 = 1 << log1, n2 = 1 << log2;
-    const int logn = log1 + log2;
+// The coefficient row is already bit-reversed, so a DIT transform consumes
+// it in place; the row table is stored bit-reversed to match.
+__device__ void ntt_lde_rows(const NttJob& J, uint32_t* lds, uint32_t R, uint32_t j) {
+    const int n1 = 1 << J.log1, n2 = 1 << J.log2;
+    const int logn = J.log1 + J.log2;
     uint32_t* s = lds;
     uint32_t* tw = lds + n2;
-    load_tw(tw, tw4096, log2);
-    const uint32_t R = blockIdx.x;
-    const int j = blockIdx.y;
-    const uint32_t c = brev_bits(R, log1);
-    const uint32_t* rt = rowtab_br + size_t(j) * n2;
-    const uint32_t* in = coeffs + size_t(R) * n2;
+    load_tw(tw, J.tw4096, J.log2);
+    const uint32_t c = brev_bits(R, J.log1);
+    const uint32_t* rt = J.tab_a + size_t(j) * n2;
+    const uint32_t* in = J.in + size_t(R) * n2;
     for (int Cc = threadIdx.x; Cc < n2; Cc += blockDim.x) s[Cc] = mont_mul(in[Cc], rt[Cc]);
     __syncthreads();
-    lds_dit(s, log2, 1, n2, tw);
-    const uint32_t colscale = coltab[size_t(j) * n1 + c];
-    uint32_t* yo = y + (size_t(j) << logn) + size_t(c) * n2;
+    lds_dit(s, J.log2, 1, n2, tw);
+    const uint32_t colscale = J.tab_b[size_t(j) * n1 + c];
+    uint32_t* yo = J.out + (size_t(j) << logn) + size_t(c) * n2;
     for (int u = threadIdx.x; u < n2; u += blockDim.x) {
-        uint32_t f = mont_mul(pow_lookup(w_m, (uint32_t(u) * c) << logb), colscale);
+        uint32_t f = mont_mul(pow_lookup(J.pt, (uint32_t(u) * c) << J.logb), colscale);
         yo[u] = mont_mul(s[u], f);
     }
 }
 
-// LDE pass B: `cols` adjacent columns u per workgroup, coset j = blockIdx.y.
-__global__ void k_lde_cols(const uint32_t* y, uint32_t* out, int log1, int log2, int logb, int cols,
-                           const uint32_t* tw4096) {
-    extern __shared__ uint32_t lds[];
-    const int n1 = 1 << log1, n2 = 1 << log2;
-    const int logn = log1 + log2;
+__device__ void ntt_lde_cols(const NttJob& J, uint32_t* lds, uint32_t bx, uint32_t j) {
+    const int n1 = 1 << J.log1, n2 = 1 << J.log2, cols = J.cols;
+    const int logn = J.log1 + J.log2;
     const int st = n1 + 1;
-    uint32_t* s = lds;  // [cols][n1 + 1]
+    uint32_t* s = lds;
     uint32_t* tw = lds + cols * st;
-    load_tw(tw, tw4096, log1);
-    const int j = blockIdx.y;
-    const int u0 = blockIdx.x * cols;
-    const uint32_t* yj = y + (size_t(j) << logn);
+    load_tw(tw, J.tw4096, J.log1);
+    const int u0 = bx * cols;
+    const uint32_t* yj = J.in + (size_t(j) << logn);
     for (int e = threadIdx.x; e < cols * n1; e += blockDim.x) {
         int c = e / cols, uu = e % cols;
         s[uu * st + c] = yj[size_t(c) * n2 + u0 + uu];
     }
     __syncthreads();
-    lds_dif(s, log1, cols, st, tw);
-    uint32_t* oj = out + (size_t(brev_bits(j, logb)) << logn);
+    lds_dif(s, J.log1, cols, st, tw);
+    uint32_t* oj = J.out + (size_t(brev_bits(j, J.logb)) << logn);
     for (int e = threadIdx.x; e < cols * n1; e += blockDim.x) {
         int uu = e / n1, q = e % n1;
         uint32_t u = uint32_t(u0 + uu);
-        oj[size_t(brev_bits(u, log2)) * n1 + q] = s[uu * st + q];
+        oj[size_t(brev_bits(u, J.log2)) * n1 + q] = s[uu * st + q];
     }
 }
 
-__global__ void k_bitrev(const uint32_t* in, uint32_t* out, int logn) {
-    uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x;
-    if (i >> logn) return;
-    out[i] = in[brev_bits(uint32_t(i), logn)];
+__global__ __launch_bounds__(WG) void k_ntt(NttJob inl, const NttJob* tab, uint32_t njobs) {
+    extern __shared__ uint32_t lds[];
+    uint32_t local;
+    const NttJob J = job_of(inl, tab, njobs, local);
+    switch (J.kind) {
+        case NTT_INTT_SMALL:
+            ntt_intt_small(J, lds);
+            break;
+        case NTT_INTT_COLS:
+            ntt_intt_cols(J, lds, local);
+            break;
+        case NTT_INTT_ROWS:
+            ntt_intt_rows(J, lds, local);
+            break;
+        case NTT_LDE_SMALL:
+            ntt_lde_small(J, lds, local);
+            break;
+        case NTT_LDE_ROWS: {
+            const uint32_t n1 = 1u << J.log1;
+            ntt_lde_rows(J, lds, local % n1, local / n1);
+            break;
+        }
+        case NTT_LDE_COLS: {
+            const uint32_t per = (1u << J.log2) / uint32_t(J.cols);
+            ntt_lde_cols(J, lds, local % per, local / per);
+            break;
+        }
+        default:
+            break;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_bitrev(BitrevJob inl, const BitrevJob* tab, uint32_t njobs) {
+    uint32_t local;
+    const BitrevJob J = job_of(inl, tab, njobs, local);
+    uint64_t i = uint64_t(local) * WG + threadIdx.x;
+    if (i >> J.logn) return;
+    J.out[i] = J.in[brev_bits(uint32_t(i), J.logn)] % P;
 }
 
 // ---------------------------------------------------------------------------
-// Per-thread Montgomery batch inversion over K register values (all nonzero
-// on the proving domains: the LDE coset 31*<Omega> avoids every 2-power
-// root of unity because 31 generates the full multiplicative group).
-template <int K>
-__device__ __forceinline__ void batch_inv_m(uint32_t (&v)[K]) {
-    uint32_t pre[K];
-    pre[0] = v[0];
-#pragma unroll
-    for (int e = 1; e < K; ++e) pre[e] = mont_mul(pre[e - 1], v[e]);
-    uint32_t inv = inv_m(pre[K - 1]);
-#pragma unroll
-    for (int e = K - 1; e > 0; --e) {
-        uint32_t ve = v[e];
-        v[e] = mont_mul(inv, pre[e - 1]);
-        inv = mont_mul(inv, ve);
-    }
-    v[0] = inv;
-}
+// Pointwise kernels over the bit-reversed LDE: ELT elements per thread,
+// strided by the workgroup size so every access is coalesced.
+constexpr int ELT = 16;
 
-constexpr int ELT = 16;  // elements per thread in the pointwise kernels
-
-// Position p = block*n + q of the bit-reversed LDE -> (coset j, step t) and
-// the position of the next trace row (t + 1 mod n, same coset).
-struct PosInfo {
-    uint32_t j, t, pnext;
-};
-__device__ __forceinline__ PosInfo pos_info(uint32_t p, int logn, int logb) {
-    uint32_t block = p >> logn;
-    uint32_t q = p & ((1u << logn) - 1);
-    uint32_t t = brev_bits(q, logn);
-    uint32_t t1 = (t + 1) & ((1u << logn) - 1);
-    PosInfo r;
-    r.j = brev_bits(block, logb);
-    r.t = t;
-    r.pnext = (block << logn) | brev_bits(t1, logn);
-    return r;
-}
-
-// Point x = s_j * omega_n^t in Montgomery form; omega_n = Omega_m^B.
-__device__ __forceinline__ uint32_t point_m(PowTable w_m, const uint32_t* shift_m, uint32_t j, uint32_t t, int logb) {
-    return mont_mul(pow_lookup(w_m, t << logb), shift_m[j]);
-}
-
-// B[i] = A[i + B] + coeff*(x^n - 1)/(x - w_last); coeff_van[j] =
-// coeff*(s_j^n - 1) (canonical) is constant per coset since x^n = s_j^n there.
-__global__ void k_linked_b(const uint32_t* a, uint32_t* b, int logn, int logb, PowTable w_m,
-                           const uint32_t* shift_m, const uint32_t* coeff_van, uint32_t w_last_m) {
-    const uint32_t m = 1u << (logn + logb);
-    const uint32_t base = blockIdx.x * (WG * ELT) + threadIdx.x;
+// B[i] = A[i + B] + coeff*(x^n - 1)/(x - w_last) (air.rs:806-844);
+// x^n - 1 = s_j^n - 1 is constant per coset (table zvan_m).
+__global__ __launch_bounds__(WG) void k_linked(LinkedJob inl, const LinkedJob* tab, uint32_t njobs) {
+    uint32_t local;
+    const LinkedJob J = job_of(inl, tab, njobs, local);
+    const uint32_t m = 1u << (J.logn + J.logb);
+    const uint32_t base = local * (WG * ELT) + threadIdx.x;
     uint32_t den[ELT];
     PosInfo pi[ELT];
 #pragma unroll
     for (int e = 0; e < ELT; ++e) {
         uint32_t p = base + e * WG;
         if (p < m) {
-            pi[e] = pos_info(p, logn, logb);
-            den[e] = sub(point_m(w_m, shift_m, pi[e].j, pi[e].t, logb), w_last_m);
+            pi[e] = pos_info(p, J.logn, J.logb);
+            den[e] = sub(point_m(J.w, J.shift_m, pi[e].j, pi[e].t, J.logb), J.w_last_m);
         } else {
             den[e] = R1;
         }
@@ -348,237 +247,218 @@ __global__ void k_linked_b(const uint32_t* a, uint32_t* b, int logn, int logb, P
 #pragma unroll
     for (int e = 0; e < ELT; ++e) {
         uint32_t p = base + e * WG;
-        if (p < m) {
-            // den is (1/(x - w_last)) in Montgomery form, coeff_van canonical.
-            b[p] = add(a[pi[e].pnext], mont_mul(den[e], coeff_van[pi[e].j]));
-        }
+        if (p < m) J.b[p] = add(J.a[pi[e].pnext], mont_mul(den[e], mont_mul(J.coeff, J.zvan_m[pi[e].j])));
     }
 }
 
-// Composition value for one point (all scalars canonical, alpha Montgomery):
+// Composition value per point (air.rs:847-929), written as FRI layer 0:
 //   a0*(A'-B)/Z + a1*(B'-A-B)/Z + a2*(A-a0)/(x-1) + a3*(B-b0)/(x-1) + a4*(B-bl)/(x-w)
-__global__ void k_compose(const uint32_t* a, const uint32_t* b, uint4* layer0, int logn, int logb, PowTable w_m,
-                          const uint32_t* shift_m, const uint32_t* zinv_m, ComposeConsts c) {
-    const uint32_t m = 1u << (logn + logb);
-    const uint32_t base = blockIdx.x * (WG * ELT) + threadIdx.x;
+// with 1/Z = (x-w)/(x^n-1) and x^n - 1 constant per coset.
+__global__ __launch_bounds__(WG) void k_compose(ComposeJob inl, const ComposeJob* tab, uint32_t njobs) {
+    uint32_t local;
+    const ComposeJob J = job_of(inl, tab, njobs, local);
+    const uint32_t m = 1u << (J.logn + J.logb);
+    const uint32_t base = local * (WG * ELT) + threadIdx.x;
     uint32_t d1[ELT], dw[ELT], pr[ELT];
 #pragma unroll
     for (int e = 0; e < ELT; ++e) {
         uint32_t p = base + e * WG;
         if (p < m) {
-            PosInfo pi = pos_info(p, logn, logb);
-            uint32_t x = point_m(w_m, shift_m, pi.j, pi.t, logb);
+            PosInfo pi = pos_info(p, J.logn, J.logb);
+            uint32_t x = point_m(J.w, J.shift_m, pi.j, pi.t, J.logb);
             d1[e] = sub(x, R1);
-            dw[e] = sub(x, c.w_last_m);
+            dw[e] = sub(x, J.c.w_last_m);
             pr[e] = mont_mul(d1[e], dw[e]);
         } else {
             d1[e] = dw[e] = pr[e] = R1;
         }
     }
     batch_inv_m<ELT>(pr);
+    uint4* out = reinterpret_cast<uint4*>(J.layer0);
 #pragma unroll
     for (int e = 0; e < ELT; ++e) {
         uint32_t p = base + e * WG;
         if (p >= m) continue;
-        PosInfo pi = pos_info(p, logn, logb);
-        uint32_t i1 = mont_mul(dw[e], pr[e]);                 // 1/(x-1), Montgomery
-        uint32_t iw = mont_mul(d1[e], pr[e]);                 // 1/(x-w), Montgomery
-        uint32_t iz = mont_mul(dw[e], zinv_m[pi.j]);          // (x-w)/(x^n-1), Montgomery
-        uint32_t va = a[p], vb = b[p], na = a[pi.pnext], nb = b[pi.pnext];
+        PosInfo pi = pos_info(p, J.logn, J.logb);
+        uint32_t i1 = mont_mul(dw[e], pr[e]);
+        uint32_t iw = mont_mul(d1[e], pr[e]);
+        uint32_t iz = mont_mul(dw[e], J.zinv_m[pi.j]);
+        uint32_t va = J.a[p], vb = J.b[p], na = J.a[pi.pnext], nb = J.b[pi.pnext];
         uint32_t s0 = mont_mul(sub(na, vb), iz);
         uint32_t s1 = mont_mul(sub(nb, add(va, vb)), iz);
-        uint32_t s2 = mont_mul(sub(va, c.a0), i1);
-        uint32_t s3 = mont_mul(sub(vb, c.b0), i1);
-        uint32_t s4 = mont_mul(sub(vb, c.b_last), iw);
+        uint32_t s2 = mont_mul(sub(va, J.c.a0), i1);
+        uint32_t s3 = mont_mul(sub(vb, J.c.b0), i1);
+        uint32_t s4 = mont_mul(sub(vb, J.c.b_last), iw);
         uint32_t o[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            uint32_t acc = mont_mul(s0, c.alpha_m[0][k]);
-            acc = add(acc, mont_mul(s1, c.alpha_m[1][k]));
-            acc = add(acc, mont_mul(s2, c.alpha_m[2][k]));
-            acc = add(acc, mont_mul(s3, c.alpha_m[3][k]));
-            acc = add(acc, mont_mul(s4, c.alpha_m[4][k]));
+            uint32_t acc = mont_mul(s0, J.c.alpha_m[0][k]);
+            acc = add(acc, mont_mul(s1, J.c.alpha_m[1][k]));
+            acc = add(acc, mont_mul(s2, J.c.alpha_m[2][k]));
+            acc = add(acc, mont_mul(s3, J.c.alpha_m[3][k]));
+            acc = add(acc, mont_mul(s4, J.c.alpha_m[4][k]));
             o[k] = acc;
         }
-        layer0[p] = make_uint4(o[0], o[1], o[2], o[3]);
+        out[p] = make_uint4(o[0], o[1], o[2], o[3]);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Merkle subtree reduction in LDS. Digests held word-major (SoA) so a
-// thread reads both children of word k with one 8-byte access.
-constexpr int SUB_IN = 2 * WG;  // digests entering the LDS reduction per workgroup
-
-// cur: [8][SUB_IN] current level in LDS (count valid entries), writes
-// parent levels 1..nlevels to global at level_ptr(l) + first_index(l).
-__device__ void subtree_levels(uint32_t (*cur)[SUB_IN], uint32_t (*nxt)[SUB_IN], int count, int nlevels,
-                               uint32_t* out_level_base, uint64_t level_count, uint64_t group_first) {
-    // out_level_base points at the level that `cur` belongs to; subsequent
-    // levels follow contiguously (level sizes level_count >> l).
-    uint32_t* lvl = out_level_base;
-    uint64_t lc = level_count;
-    uint64_t first = group_first;
-    for (int l = 1; l <= nlevels && count > 1; ++l) {
-        uint32_t* next_lvl = lvl + lc * 8;
-        int pc = count >> 1;
-        for (int t = threadIdx.x; t < pc; t += blockDim.x) {
-            uint32_t L[8], Rr[8], o[8];
+// Merkle leaves: thread g hashes leaves 2g and 2g+1 and, when asked, their
+// parent (one node of level 1). No LDS: wide levels run at full occupancy.
+__device__ __forceinline__ void leaf_message(const LeavesJob& J, uint64_t leaf, uint32_t m[16]) {
+    if (J.kind == LEAF_BASE) {
+        uint4 v = reinterpret_cast<const uint4*>(J.src)[leaf];
+        m[0] = v.x; m[4] = v.y; m[8] = v.z; m[12] = v.w;
+        m[1] = m[2] = m[3] = m[5] = m[6] = m[7] = 0;
+        m[9] = m[10] = m[11] = m[13] = m[14] = m[15] = 0;
+        return;
+    }
+    // LEAF_EXT, LEAF_RAW: 64 contiguous bytes
+    const uint4* src = reinterpret_cast<const uint4*>(J.src) + leaf * 4;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                L[k] = cur[k][2 * t];
-                Rr[k] = cur[k][2 * t + 1];
-            }
-            node_words(L, Rr, o);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) nxt[k][t] = o[k];
-            uint4* g = reinterpret_cast<uint4*>(next_lvl + (first / 2 + t) * 8);
-            g[0] = make_uint4(o[0], o[1], o[2], o[3]);
-            g[1] = make_uint4(o[4], o[5], o[6], o[7]);
-        }
-        __syncthreads();
-        auto tmp = cur;
-        cur = nxt;
-        nxt = tmp;
-        count = pc;
-        lvl = next_lvl;
-        lc >>= 1;
-        first >>= 1;
+    for (int k = 0; k < 4; ++k) {
+        uint4 v = src[k];
+        m[4 * k] = v.x; m[4 * k + 1] = v.y; m[4 * k + 2] = v.z; m[4 * k + 3] = v.w;
     }
 }
 
-// Generic: digests of one level -> up to nlevels parent levels.
-__global__ void k_merkle_levels(uint32_t* level_base, uint64_t count, int nlevels) {
-    __shared__ uint32_t bufA[8][SUB_IN];
-    __shared__ uint32_t bufB[8][SUB_IN];
-    uint64_t first = uint64_t(blockIdx.x) * SUB_IN;
-    int here = int(count - first < uint64_t(SUB_IN) ? count - first : uint64_t(SUB_IN));
-    for (int t = threadIdx.x; t < here; t += blockDim.x) {
-        const uint4* g = reinterpret_cast<const uint4*>(level_base + (first + t) * 8);
-        uint4 x = g[0], y = g[1];
-        bufA[0][t] = x.x; bufA[1][t] = x.y; bufA[2][t] = x.z; bufA[3][t] = x.w;
-        bufA[4][t] = y.x; bufA[5][t] = y.y; bufA[6][t] = y.z; bufA[7][t] = y.w;
+// Trace rows: leaf L packs natural rows 8L..8L+7 as A||B LE words; row
+// i = j + B t lives at brev(j)*n + brev(t). Thread tau is mapped to its leaf
+// pair so that lanes read consecutive positions of each coset block.
+__device__ __forceinline__ uint64_t trace_pair_of(const LeavesJob& J, uint64_t tau) {
+    const int logn = J.logn, logb = J.logb;
+    if (logb >= 4) {
+        const int logh = logb - 4;  // pairs per step
+        uint64_t h = tau >> logn, q = tau & ((uint64_t(1) << logn) - 1);
+        return (uint64_t(brev_bits(uint32_t(q), logn)) << logh) | h;
     }
-    __syncthreads();
-    subtree_levels(bufA, bufB, here, nlevels, level_base, count, first);
+    const int logs = 4 - logb;  // steps per pair
+    return brev_bits(uint32_t(tau), logn - logs);
 }
 
-// FRI leaves: 4 consecutive values per leaf (Ext4 or base-field values with
-// zero upper coordinates), then fused parent levels.
-__global__ void k_fri_leaves(const uint32_t* layer, int base_field, uint32_t* levels, uint64_t leaves, int nlevels) {
-    __shared__ uint32_t bufA[8][SUB_IN];
-    __shared__ uint32_t bufB[8][SUB_IN];
-    uint64_t first = uint64_t(blockIdx.x) * SUB_IN;
-    int here = int(leaves - first < uint64_t(SUB_IN) ? leaves - first : uint64_t(SUB_IN));
-    for (int t = threadIdx.x; t < here; t += blockDim.x) {
-        uint64_t leaf = first + t;
-        uint32_t m[16];
-        if (base_field) {
-            uint4 v = reinterpret_cast<const uint4*>(layer)[leaf];
-            m[0] = v.x; m[4] = v.y; m[8] = v.z; m[12] = v.w;
-            m[1] = m[2] = m[3] = m[5] = m[6] = m[7] = 0;
-            m[9] = m[10] = m[11] = m[13] = m[14] = m[15] = 0;
-        } else {
-            const uint4* src = reinterpret_cast<const uint4*>(layer) + leaf * 4;
+__device__ __forceinline__ void trace_leaf_message(const LeavesJob& J, uint64_t leaf, uint32_t m[16]) {
+    const int logn = J.logn, logb = J.logb;
+    const uint32_t bmask = (1u << logb) - 1;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                uint4 v = src[k];
-                m[4 * k] = v.x; m[4 * k + 1] = v.y; m[4 * k + 2] = v.z; m[4 * k + 3] = v.w;
-            }
-        }
-        uint32_t d[8];
+    for (int r = 0; r < 8; ++r) {
+        uint64_t i = leaf * 8 + r;
+        uint32_t j = uint32_t(i) & bmask;
+        uint32_t t = uint32_t(i >> logb);
+        uint64_t pos = (uint64_t(brev_bits(j, logb)) << logn) | brev_bits(t, logn);
+        m[2 * r] = J.src[pos];
+        m[2 * r + 1] = J.src_b[pos];
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_leaves(LeavesJob inl, const LeavesJob* tab, uint32_t njobs) {
+    uint32_t local;
+    const LeavesJob J = job_of(inl, tab, njobs, local);
+    const uint64_t tau = uint64_t(local) * WG + threadIdx.x;
+    const bool trace = J.kind == LEAF_TRACE;
+    if (J.leaves == 1) {
+        if (tau != 0) return;
+        uint32_t m[16], d[8];
+        if (trace) trace_leaf_message(J, 0, m);
+        else leaf_message(J, 0, m);
         leaf64_words(m, d);
-        uint4* g = reinterpret_cast<uint4*>(levels + leaf * 8);
-        g[0] = make_uint4(d[0], d[1], d[2], d[3]);
-        g[1] = make_uint4(d[4], d[5], d[6], d[7]);
+        store_digest(J.levels, d);
+        return;
+    }
+    const uint64_t pairs = J.leaves >> 1;
+    if (tau >= pairs) return;
+    const uint64_t g = trace ? trace_pair_of(J, tau) : tau;
+    uint32_t m[16], d0[8], d1[8];
+    if (trace) trace_leaf_message(J, 2 * g, m);
+    else leaf_message(J, 2 * g, m);
+    leaf64_words(m, d0);
+    if (trace) trace_leaf_message(J, 2 * g + 1, m);
+    else leaf_message(J, 2 * g + 1, m);
+    leaf64_words(m, d1);
+    store_digest(J.levels + (2 * g) * 8, d0);
+    store_digest(J.levels + (2 * g + 1) * 8, d1);
+    if (J.with_parent) {
+        uint32_t p[8];
+        node_words(d0, d1, p);
+        store_digest(J.levels + (J.leaves + g) * 8, p);
+    }
+}
+
+__device__ __forceinline__ void load_digest(const uint32_t* src, uint32_t d[8]) {
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    uint4 x = s[0], y = s[1];
+    d[0] = x.x; d[1] = x.y; d[2] = x.z; d[3] = x.w;
+    d[4] = y.x; d[5] = y.y; d[6] = y.z; d[7] = y.w;
+}
+
+// Wide levels: thread g reduces digests 2g, 2g+1 (fuse = 1) or 4g..4g+3
+// (fuse = 2: two nodes and their parent).
+__global__ __launch_bounds__(WG) void k_levels(LevelsJob inl, const LevelsJob* tab, uint32_t njobs) {
+    uint32_t local;
+    const LevelsJob J = job_of(inl, tab, njobs, local);
+    const uint64_t g = uint64_t(local) * WG + threadIdx.x;
+    const uint64_t count = J.count;
+    uint32_t* next = J.level + count * 8;
+    uint32_t a[8], b[8];
+    if (J.fuse == 2) {
+        if (g >= count / 4) return;
+        uint32_t p0[8], p1[8], top[8];
+        load_digest(J.level + (4 * g) * 8, a);
+        load_digest(J.level + (4 * g + 1) * 8, b);
+        node_words(a, b, p0);
+        load_digest(J.level + (4 * g + 2) * 8, a);
+        load_digest(J.level + (4 * g + 3) * 8, b);
+        node_words(a, b, p1);
+        store_digest(next + (2 * g) * 8, p0);
+        store_digest(next + (2 * g + 1) * 8, p1);
+        node_words(p0, p1, top);
+        store_digest(next + (count / 2 + g) * 8, top);
+        return;
+    }
+    if (g >= count / 2) return;
+    uint32_t p[8];
+    load_digest(J.level + (2 * g) * 8, a);
+    load_digest(J.level + (2 * g + 1) * 8, b);
+    node_words(a, b, p);
+    store_digest(next + g * 8, p);
+}
+
+// Narrow tail: up to 9 levels per 512-digest group in LDS.
+__global__ __launch_bounds__(WG) void k_tail(TailJob inl, const TailJob* tab, uint32_t njobs) {
+    __shared__ uint32_t bufA[8][SUB_IN];
+    __shared__ uint32_t bufB[8][SUB_IN];
+    uint32_t local;
+    const TailJob J = job_of(inl, tab, njobs, local);
+    const uint64_t first = uint64_t(local) * SUB_IN;
+    const int here = int(J.count - first < uint64_t(SUB_IN) ? J.count - first : uint64_t(SUB_IN));
+    for (int t = threadIdx.x; t < here; t += blockDim.x) {
+        uint32_t d[8];
+        load_digest(J.level + (first + t) * 8, d);
 #pragma unroll
         for (int k = 0; k < 8; ++k) bufA[k][t] = d[k];
     }
     __syncthreads();
-    subtree_levels(bufA, bufB, here, nlevels, levels, leaves, first);
-}
-
-// Trace leaves: leaf L packs natural rows 8L..8L+7 as A||B (LE words).
-// Thread g handles the rows of trace steps t sharing one position q of the
-// bit-reversed blocks: for B >= 8 one step t = brev(q) (B/8 leaves, reduced
-// in registers to one node), for B = 4 the step pair (t, t+1) with t even.
-__global__ void k_trace_leaves(const uint32_t* a, const uint32_t* b, uint32_t* levels, int logn, int logb,
-                               int max_level) {
-    const uint32_t n = 1u << logn;
-    const int B = 1 << logb;
-    uint32_t g = blockIdx.x * WG + threadIdx.x;
-    const uint64_t leaves = (uint64_t(n) << logb) >> 3;
-    if (logb >= 3) {
-        if (g >= n) return;
-        const uint32_t q = g;
-        const uint32_t t = brev_bits(q, logn);
-        const int per = B >> 3;  // leaves per step
-        // Leaves L = per*t + h, h < per; rows j = 8h + r.
-        uint32_t node[2][8];  // B = 16: both leaves of step t stay in registers
-        for (int h = 0; h < per; ++h) {
-            uint32_t msg[16];
-#pragma unroll
-            for (int r = 0; r < 8; ++r) {
-                uint32_t j = uint32_t(8 * h + r);
-                uint32_t pos = (brev_bits(j, logb) << logn) | q;
-                msg[2 * r] = a[pos];
-                msg[2 * r + 1] = b[pos];
-            }
-            uint32_t d[8];
-            leaf64_words(msg, d);
-            uint64_t L = uint64_t(per) * t + h;
-            uint4* gp = reinterpret_cast<uint4*>(levels + L * 8);
-            gp[0] = make_uint4(d[0], d[1], d[2], d[3]);
-            gp[1] = make_uint4(d[4], d[5], d[6], d[7]);
-            if (per == 2) {
-#pragma unroll
-                for (int k = 0; k < 8; ++k) node[h & 1][k] = d[k];
-            }
-        }
-        if (per == 2 && max_level >= 1) {
-            uint32_t o[8];
-            node_words(node[0], node[1], o);
-            uint4* gp = reinterpret_cast<uint4*>(levels + (leaves + t) * 8);
-            gp[0] = make_uint4(o[0], o[1], o[2], o[3]);
-            gp[1] = make_uint4(o[4], o[5], o[6], o[7]);
-        }
-    } else {
-        // B = 4: leaf L = t/2 covers steps t (even) and t+1, cosets 0..3 each.
-        if (g >= n / 2) return;
-        const uint32_t q = g;                 // top bit of q clear -> t even
-        const uint32_t t = brev_bits(q, logn);
-        const uint32_t q1 = q | (n >> 1);     // position of t + 1
-        uint32_t msg[16];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            uint32_t j = uint32_t(r & 3);
-            uint32_t pos = (brev_bits(j, 2) << logn) | (r < 4 ? q : q1);
-            msg[2 * r] = a[pos];
-            msg[2 * r + 1] = b[pos];
-        }
-        uint32_t d[8];
-        leaf64_words(msg, d);
-        uint64_t L = t >> 1;
-        uint4* gp = reinterpret_cast<uint4*>(levels + L * 8);
-        gp[0] = make_uint4(d[0], d[1], d[2], d[3]);
-        gp[1] = make_uint4(d[4], d[5], d[6], d[7]);
-    }
+    subtree_levels(bufA, bufB, here, J.nlevels, J.level, J.count, first);
 }
 
 // ---------------------------------------------------------------------------
-// Arity-4 fold of a bit-reversed layer. Quad i sits at (x, -x, j0 x, -j0 x)
-// with x = point of position 4i; x^{-1} = 31^{-4^d} * Omega_m^{-(brev(i) << 2d)}.
-__global__ void k_fold(const uint32_t* layer, int base_field, uint4* next, PowTable winv_m, FoldConsts c) {
-    const uint64_t quads = uint64_t(1) << (c.log_len - 2);
-    uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x;
+// Arity-4 fold of a bit-reversed layer (fri.rs:1273-1331). Quad i sits at
+// (x, -x, j0 x, -j0 x), x the point of position 4i;
+// x^{-1} = 31^{-4^d} * Omega_m^{-(brev(i) << 2d)}.
+__global__ __launch_bounds__(WG) void k_fold(FoldJob inl, const FoldJob* tab, uint32_t njobs) {
+    uint32_t local;
+    const FoldJob J = job_of(inl, tab, njobs, local);
+    const uint64_t quads = uint64_t(1) << (J.c.log_len - 2);
+    const uint64_t i = uint64_t(local) * WG + threadIdx.x;
     if (i >= quads) return;
-    uint32_t e = brev_bits(uint32_t(i), c.log_len - 2);
-    uint32_t inv_x = mont_mul(pow_lookup(winv_m, e << (2 * c.depth)), c.ginv_m);  // Montgomery
+    uint32_t e = brev_bits(uint32_t(i), J.c.log_len - 2);
+    uint32_t inv_x = mont_mul(pow_lookup(J.winv, e << (2 * J.c.depth)), J.c.ginv_m);
     E4 q[4];
-    if (base_field) {
-        uint4 v = reinterpret_cast<const uint4*>(layer)[i];
+    if (J.base_field) {
+        uint4 v = reinterpret_cast<const uint4*>(J.layer)[i];
         q[0] = e4_base(v.x); q[1] = e4_base(v.y); q[2] = e4_base(v.z); q[3] = e4_base(v.w);
     } else {
-        const uint4* src = reinterpret_cast<const uint4*>(layer) + i * 4;
+        const uint4* src = reinterpret_cast<const uint4*>(J.layer) + i * 4;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             uint4 v = src[k];
@@ -587,43 +467,52 @@ __global__ void k_fold(const uint32_t* layer, int base_field, uint4* next, PowTa
     }
     E4 s01 = e4_add(q[0], q[1]), s23 = e4_add(q[2], q[3]);
     E4 odd_diff = e4_sub(q[0], q[1]);
-    E4 skew_diff = e4_scale_m(e4_sub(q[2], q[3]), c.unit_m);
+    E4 skew_diff = e4_scale_m(e4_sub(q[2], q[3]), J.c.unit_m);
     E4 mean = e4_halve(e4_halve(e4_add(s01, s23)));
     E4 odd = e4_halve(e4_halve(e4_sub(odd_diff, skew_diff)));
     E4 even = e4_halve(e4_halve(e4_sub(s01, s23)));
     E4 skew = e4_halve(e4_halve(e4_add(odd_diff, skew_diff)));
-    E4 ch0{{c.ch_m[0][0], c.ch_m[0][1], c.ch_m[0][2], c.ch_m[0][3]}};
-    E4 ch1{{c.ch_m[1][0], c.ch_m[1][1], c.ch_m[1][2], c.ch_m[1][3]}};
-    E4 ch2{{c.ch_m[2][0], c.ch_m[2][1], c.ch_m[2][2], c.ch_m[2][3]}};
+    E4 ch0{{J.c.ch_m[0][0], J.c.ch_m[0][1], J.c.ch_m[0][2], J.c.ch_m[0][3]}};
+    E4 ch1{{J.c.ch_m[1][0], J.c.ch_m[1][1], J.c.ch_m[1][2], J.c.ch_m[1][3]}};
+    E4 ch2{{J.c.ch_m[2][0], J.c.ch_m[2][1], J.c.ch_m[2][2], J.c.ch_m[2][3]}};
     E4 acc = e4_scale_m(e4_mul_m(skew, ch2), inv_x);
     acc = e4_scale_m(e4_add(acc, e4_mul_m(even, ch1)), inv_x);
     acc = e4_add(acc, e4_mul_m(odd, ch0));
     E4 out = e4_add(mean, e4_scale_m(acc, inv_x));
-    next[i] = make_uint4(out.v[0], out.v[1], out.v[2], out.v[3]);
+    reinterpret_cast<uint4*>(J.next)[i] = make_uint4(out.v[0], out.v[1], out.v[2], out.v[3]);
 }
 
 // ---------------------------------------------------------------------------
 // Proof-of-work scan. The 55-byte preimage head||nonce is one BLAKE3 block
-// (START|END|ROOT). Workgroups walk chunks in increasing order and stop once
-// a passing nonce below their next chunk is known, so the reported minimum
-// is the exact smallest passing nonce (every chunk at or below it is scanned).
+// (START|END|ROOT). A job's workgroups walk its chunks in increasing order
+// and stop once a passing nonce below their next chunk is known, so the
+// reported minimum is the exact smallest passing nonce in [0, limit): every
+// chunk at or below it is scanned.
 constexpr int GRIND_PER_THREAD = 4;
 constexpr uint32_t POW_HEAD_LEN = 47;  // 32-byte history digest + u64 len + "fri-pow"
 
-__global__ void k_grind(uint4 h0, uint4 h1, uint4 h2, uint4 h3, uint32_t bits, uint64_t limit,
-                        unsigned long long* result) {
-    const uint32_t hw[16] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w,
-                             h2.x, h2.y, h2.z, h2.w, h3.x, h3.y, h3.z, h3.w};
+__global__ void k_grind_init(GrindJob inl, const GrindJob* tab, uint32_t njobs) {
+    uint32_t i = blockIdx.x * WG + threadIdx.x;
+    if (njobs == 0) {
+        if (i == 0) *inl.result = ~0ull;
+    } else if (i < njobs) {
+        *tab[i].result = ~0ull;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_grind(GrindJob inl, const GrindJob* tab, uint32_t njobs) {
+    uint32_t local;
+    const GrindJob J = job_of(inl, tab, njobs, local);
     const uint64_t chunk = uint64_t(WG) * GRIND_PER_THREAD;
-    const uint64_t stride = uint64_t(gridDim.x) * chunk;
-    for (uint64_t start = uint64_t(blockIdx.x) * chunk; start < limit; start += stride) {
-        if (__hip_atomic_load(result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < start) return;
+    const uint64_t stride = uint64_t(J.nblocks) * chunk;
+    for (uint64_t start = uint64_t(local) * chunk; start < J.limit; start += stride) {
+        if (__hip_atomic_load(J.result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < start) return;
         for (int k = 0; k < GRIND_PER_THREAD; ++k) {
             uint64_t nonce = start + uint64_t(k) * WG + threadIdx.x;
-            if (nonce >= limit) break;
+            if (nonce >= J.limit) break;
             uint32_t m[16];
 #pragma unroll
-            for (int w = 0; w < 16; ++w) m[w] = hw[w];
+            for (int w = 0; w < 16; ++w) m[w] = J.hw[w];
             // nonce bytes occupy preimage bytes 47..54: word 11 byte 3, words 12-13
             uint32_t lo = uint32_t(nonce), hi = uint32_t(nonce >> 32);
             m[11] |= lo << 24;
@@ -634,23 +523,19 @@ __global__ void k_grind(uint4 h0, uint4 h1, uint4 h2, uint4 h3, uint32_t bits, u
             compress8(iv, m, 0, 0, POW_HEAD_LEN + 8, b3::CHUNK_START | b3::CHUNK_END | b3::ROOT, d);
             uint32_t be = __builtin_bswap32(d[0]);
             uint32_t lz = be == 0 ? 32u : uint32_t(__builtin_clz(be));
-            if (lz >= bits) atomicMin(result, (unsigned long long)nonce);
+            if (lz >= J.bits) atomicMin(J.result, (unsigned long long)nonce);
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-__global__ void k_gather(const uint32_t* base, const uint64_t* src, const uint32_t* dst, const uint32_t* len,
-                         uint32_t items, uint32_t* out) {
-    uint32_t i = blockIdx.x * WG + threadIdx.x;
-    if (i >= items) return;
-    const uint32_t* s = base + src[i];
-    uint32_t* d = out + dst[i];
-    for (uint32_t k = 0; k < len[i]; ++k) d[k] = s[k];
+__global__ __launch_bounds__(WG) void k_collect(CollectItem inl, const CollectItem* tab, uint32_t njobs) {
+    uint32_t local;
+    const CollectItem J = job_of(inl, tab, njobs, local);
+    for (uint32_t k = threadIdx.x; k < J.words; k += WG) J.dst[k] = J.src[k];
 }
 
-// Plain streaming copy of `n16` 16-byte words: the calibration reference for
-// the FETCH_SIZE / WRITE_SIZE hardware counters.
+// Plain streaming copy: the calibration reference for FETCH/WRITE_SIZE.
 __global__ void k_copy(const uint4* __restrict__ in, uint4* __restrict__ out, uint64_t n16) {
     uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x;
     const uint64_t stride = uint64_t(gridDim.x) * WG;
@@ -659,13 +544,70 @@ __global__ void k_copy(const uint4* __restrict__ in, uint4* __restrict__ out, ui
 
 inline unsigned grid_for(uint64_t work, uint64_t per_block) { return unsigned((work + per_block - 1) / per_block); }
 
+// Assign block_begin, return the total grid.
+template <typename J, typename F>
+unsigned assign_blocks(std::vector<J>& jobs, F blocks_of) {
+    uint64_t total = 0;
+    for (J& j : jobs) {
+        j.block_begin = uint32_t(total);
+        total += blocks_of(j);
+    }
+    return unsigned(total);
+}
+
+template <typename J>
+const J* table_of(const Launch& L, const std::vector<J>& jobs) {
+    if (jobs.size() <= 1) return nullptr;
+    return static_cast<const J*>(L.sink->push(jobs.data(), jobs.size() * sizeof(J)));
+}
+
+template <typename J, typename K>
+void run(const Launch& L, const char* name, K kernel, std::vector<J>& jobs, unsigned grid, size_t lds) {
+    if (jobs.empty() || grid == 0) return;
+    const J* table = table_of(L, jobs);
+    if (L.dry) return;
+    const uint32_t n = jobs.size() <= 1 ? 0u : uint32_t(jobs.size());
+    ProfScope ps(L.prof, name, L.stream);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(WG), lds, L.stream, jobs[0], table, n);
+}
+
+int ntt_cols_for(int log1, int log2) {
+    int cols = COL_LDS_WORDS >> log1;
+    return std::min(cols, 1 << log2);
+}
+
+uint64_t ntt_blocks(const NttJob& j) {
+    const uint64_t B = uint64_t(1) << j.logb;
+    switch (j.kind) {
+        case NTT_INTT_SMALL: return 1;
+        case NTT_INTT_COLS: return (uint64_t(1) << j.log2) / j.cols;
+        case NTT_INTT_ROWS: return uint64_t(1) << j.log1;
+        case NTT_LDE_SMALL: return B;
+        case NTT_LDE_ROWS: return (uint64_t(1) << j.log1) * B;
+        case NTT_LDE_COLS: return ((uint64_t(1) << j.log2) / j.cols) * B;
+        default: return 0;
+    }
+}
+
+size_t ntt_lds(const NttJob& j) {
+    switch (j.kind) {
+        case NTT_INTT_SMALL:
+        case NTT_LDE_SMALL: return (size_t(1) << j.logn) * 6;
+        case NTT_INTT_ROWS:
+        case NTT_LDE_ROWS: return (size_t(1) << j.log2) * 6;
+        case NTT_INTT_COLS:
+        case NTT_LDE_COLS: return size_t(j.cols) * ((size_t(1) << j.log1) + 1) * 4 + (size_t(1) << j.log1) * 2;
+        default: return 0;
+    }
+}
+
 }  // namespace
 
 NttShape ntt_shape(int logn) {
+    // Rows (contiguous, DIT/DIF in LDS) take up to 4096 points; the strided
+    // column pass takes the remainder, COL_LDS_WORDS / n1 columns at a time.
     NttShape s;
     s.logn = logn;
-    // Rows (contiguous, DIT/DIF in LDS) take up to 4096 points; the strided
-    // column pass takes the remainder, COLS_WORDS / n1 columns at a time.
     if (logn <= 12) {
         s.log1 = 0;
         s.log2 = logn;
@@ -676,143 +618,239 @@ NttShape ntt_shape(int logn) {
     return s;
 }
 
-
-void launch_fib_trace(const Launch& L, uint32_t* col, uint64_t n, uint32_t a0, uint32_t b0) {
-    unsigned g = grid_for((n + FIB_CHUNK - 1) / FIB_CHUNK, WG);
-    ProfScope ps(L.prof, "fib_trace", L.stream);
-    hipLaunchKernelGGL(k_fib_trace, dim3(g), dim3(WG), 0, L.stream, col, n, a0, b0);
-}
-
-void launch_intt_to_br(const Launch& L, const uint32_t* evals, uint32_t* coeffs_br, uint32_t* scratch, int logn,
-                       const uint32_t* tw4096_inv, PowTable winv_m, int log_blowup, uint32_t ninv_m) {
+NttPlan plan_intt(const uint32_t* evals, uint32_t* coeffs_br, uint32_t* scratch, int logn, const uint32_t* tw4096_inv,
+                  PowTable winv_m, int logb, uint32_t ninv_m) {
     NttShape s = ntt_shape(logn);
+    NttJob j{};
+    j.tw4096 = tw4096_inv;
+    j.pt = winv_m;
+    j.ninv_m = ninv_m;
+    j.logn = logn;
+    j.log1 = s.log1;
+    j.log2 = s.log2;
+    j.logb = logb;
+    NttPlan plan;
     if (s.log1 == 0) {
-        size_t lds = (size_t(1) << logn) * 4 + (size_t(1) << logn) * 2;
-        ProfScope ps(L.prof, "intt_small", L.stream);
-        hipLaunchKernelGGL(k_intt_small, dim3(1), dim3(WG), lds, L.stream, evals, coeffs_br, logn, tw4096_inv,
-                           ninv_m);
-        return;
+        j.kind = NTT_INTT_SMALL;
+        j.in = evals;
+        j.out = coeffs_br;
+        plan.pass1.push_back(j);
+        return plan;
     }
-    const int n1 = 1 << s.log1, n2 = 1 << s.log2;
-    int cols = COL_LDS_WORDS >> s.log1;
-    if (cols > n2) cols = n2;
-    {
-        size_t lds = size_t(cols) * (n1 + 1) * 4 + size_t(n1) * 2;
-        ProfScope ps(L.prof, "intt_cols", L.stream);
-        hipLaunchKernelGGL(k_intt_cols, dim3(n2 / cols), dim3(WG), lds, L.stream, evals, scratch, s.log1, s.log2,
-                           cols, tw4096_inv, winv_m, log_blowup);
-    }
-    {
-        size_t lds = size_t(n2) * 4 + size_t(n2) * 2;
-        ProfScope ps(L.prof, "intt_rows", L.stream);
-        hipLaunchKernelGGL(k_intt_rows, dim3(n1), dim3(WG), lds, L.stream, scratch, coeffs_br, s.log1, s.log2,
-                           tw4096_inv, ninv_m);
-    }
+    j.cols = ntt_cols_for(s.log1, s.log2);
+    j.kind = NTT_INTT_COLS;
+    j.in = evals;
+    j.out = scratch;
+    plan.pass1.push_back(j);
+    j.kind = NTT_INTT_ROWS;
+    j.in = scratch;
+    j.out = coeffs_br;
+    plan.pass2.push_back(j);
+    return plan;
 }
 
-void launch_lde_br(const Launch& L, const uint32_t* coeffs_br, uint32_t* lde_br, uint32_t* scratch, int logn,
-                   int log_blowup, const uint32_t* tw4096, PowTable w_m, CosetTables ct) {
+NttPlan plan_lde(const uint32_t* coeffs_br, uint32_t* lde_br, uint32_t* scratch, int logn, int logb,
+                 const uint32_t* tw4096, PowTable w_m, CosetTables ct) {
     NttShape s = ntt_shape(logn);
-    const int B = 1 << log_blowup;
+    NttJob j{};
+    j.tw4096 = tw4096;
+    j.pt = w_m;
+    j.logn = logn;
+    j.log1 = s.log1;
+    j.log2 = s.log2;
+    j.logb = logb;
+    NttPlan plan;
     if (s.log1 == 0) {
-        size_t lds = (size_t(1) << logn) * 4 + (size_t(1) << logn) * 2;
-        ProfScope ps(L.prof, "lde_small", L.stream);
-        hipLaunchKernelGGL(k_lde_small, dim3(B), dim3(WG), lds, L.stream, coeffs_br, lde_br, logn, log_blowup,
-                           tw4096, ct.small);
-        return;
+        j.kind = NTT_LDE_SMALL;
+        j.in = coeffs_br;
+        j.out = lde_br;
+        j.tab_a = ct.small;
+        plan.pass1.push_back(j);
+        return plan;
     }
-    const int n1 = 1 << s.log1, n2 = 1 << s.log2;
+    j.cols = ntt_cols_for(s.log1, s.log2);
+    j.kind = NTT_LDE_ROWS;
+    j.in = coeffs_br;
+    j.out = scratch;
+    j.tab_a = ct.row;
+    j.tab_b = ct.col;
+    plan.pass1.push_back(j);
+    j.kind = NTT_LDE_COLS;
+    j.in = scratch;
+    j.out = lde_br;
+    j.tab_a = j.tab_b = nullptr;
+    plan.pass2.push_back(j);
+    return plan;
+}
+
+void launch_fib(const Launch& L, std::vector<FibJob> jobs) {
+    unsigned g = assign_blocks(jobs, [](const FibJob& j) { return grid_for((j.n + FIB_CHUNK - 1) / FIB_CHUNK, WG); });
+    run(L, "fib_trace", k_fib, jobs, g, 0);
+}
+
+void launch_ntt(const Launch& L, std::vector<NttJob> jobs) {
+    size_t lds = 0;
+    bool intt = false;
+    for (const NttJob& j : jobs) {
+        lds = std::max(lds, ntt_lds(j));
+        intt = intt || j.kind <= NTT_INTT_ROWS;
+    }
+    unsigned g = assign_blocks(jobs, [](const NttJob& j) { return ntt_blocks(j); });
+    run(L, intt ? "intt" : "lde", k_ntt, jobs, g, lds);
+}
+
+void launch_linked(const Launch& L, std::vector<LinkedJob> jobs) {
+    unsigned g = assign_blocks(jobs, [](const LinkedJob& j) {
+        return grid_for(uint64_t(1) << (j.logn + j.logb), WG * ELT);
+    });
+    run(L, "linked_b", k_linked, jobs, g, 0);
+}
+
+void launch_compose(const Launch& L, std::vector<ComposeJob> jobs) {
+    unsigned g = assign_blocks(jobs, [](const ComposeJob& j) {
+        return grid_for(uint64_t(1) << (j.logn + j.logb), WG * ELT);
+    });
+    run(L, "compose", k_compose, jobs, g, 0);
+}
+
+void launch_leaves(const Launch& L, std::vector<LeavesJob> jobs) {
+    bool trace = false;
+    for (const LeavesJob& j : jobs) trace = trace || j.kind == LEAF_TRACE;
+    unsigned g = assign_blocks(jobs, [](const LeavesJob& j) {
+        return j.leaves == 1 ? 1u : grid_for(j.leaves / 2, WG);
+    });
+    run(L, trace ? "trace_leaves" : "fri_leaves", k_leaves, jobs, g, 0);
+}
+
+void launch_levels(const Launch& L, std::vector<LevelsJob> jobs) {
+    unsigned g = assign_blocks(jobs, [](const LevelsJob& j) { return grid_for(j.count / (2 * j.fuse), WG); });
+    run(L, "tree_levels", k_levels, jobs, g, 0);
+}
+
+void launch_tail(const Launch& L, std::vector<TailJob> jobs) {
+    unsigned g = assign_blocks(jobs, [](const TailJob& j) { return grid_for(j.count, SUB_IN); });
+    run(L, "tree_tail", k_tail, jobs, g, 0);
+}
+
+void launch_fold(const Launch& L, std::vector<FoldJob> jobs) {
+    unsigned g = assign_blocks(jobs, [](const FoldJob& j) { return grid_for(uint64_t(1) << (j.c.log_len - 2), WG); });
+    run(L, "fold", k_fold, jobs, g, 0);
+}
+
+void launch_grind(const Launch& L, std::vector<GrindJob> jobs) {
+    if (jobs.empty()) return;
+    unsigned g = assign_blocks(jobs, [](const GrindJob& j) { return uint64_t(j.nblocks); });
+    const GrindJob* table = table_of(L, jobs);
+    if (L.dry) return;
+    const uint32_t n = jobs.size() <= 1 ? 0u : uint32_t(jobs.size());
     {
-        size_t lds = size_t(n2) * 4 + size_t(n2) * 2;
-        ProfScope ps(L.prof, "lde_rows", L.stream);
-        hipLaunchKernelGGL(k_lde_rows, dim3(n1, B), dim3(WG), lds, L.stream, coeffs_br, scratch, s.log1, s.log2,
-                           log_blowup, tw4096, w_m, ct.row, ct.col);
+        ProfScope ps(L.prof, "grind_init", L.stream);
+        hipLaunchKernelGGL(k_grind_init, dim3(grid_for(std::max<size_t>(jobs.size(), 1), WG)), dim3(WG), 0, L.stream,
+                           jobs[0], table, n);
     }
-    int cols = COL_LDS_WORDS >> s.log1;
-    if (cols > n2) cols = n2;
-    {
-        size_t lds = size_t(cols) * (n1 + 1) * 4 + size_t(n1) * 2;
-        ProfScope ps(L.prof, "lde_cols", L.stream);
-        hipLaunchKernelGGL(k_lde_cols, dim3(n2 / cols, B), dim3(WG), lds, L.stream, scratch, lde_br, s.log1, s.log2,
-                           log_blowup, cols, tw4096);
-    }
-}
-
-void launch_bitrev(const Launch& L, const uint32_t* in, uint32_t* out, int logn) {
-    ProfScope ps(L.prof, "bitrev", L.stream);
-    hipLaunchKernelGGL(k_bitrev, dim3(grid_for(uint64_t(1) << logn, WG)), dim3(WG), 0, L.stream, in, out, logn);
-}
-
-void launch_linked_b(const Launch& L, const uint32_t* a_br, uint32_t* b_br, int logn, int log_blowup, PowTable w_m,
-                     const uint32_t* shift_m, const uint32_t* coeff_van, uint32_t w_last_m) {
-    uint64_t m = uint64_t(1) << (logn + log_blowup);
-    ProfScope ps(L.prof, "linked_b", L.stream);
-    hipLaunchKernelGGL(k_linked_b, dim3(grid_for(m, WG * ELT)), dim3(WG), 0, L.stream, a_br, b_br, logn, log_blowup,
-                       w_m, shift_m, coeff_van, w_last_m);
-}
-
-void launch_compose(const Launch& L, const uint32_t* a_br, const uint32_t* b_br, uint32_t* layer0, int logn,
-                    int log_blowup, PowTable w_m, const uint32_t* shift_m, const uint32_t* zinv_m,
-                    const ComposeConsts& c) {
-    uint64_t m = uint64_t(1) << (logn + log_blowup);
-    ProfScope ps(L.prof, "compose", L.stream);
-    hipLaunchKernelGGL(k_compose, dim3(grid_for(m, WG * ELT)), dim3(WG), 0, L.stream, a_br, b_br,
-                       reinterpret_cast<uint4*>(layer0), logn, log_blowup, w_m, shift_m, zinv_m, c);
-}
-
-int launch_trace_leaves(const Launch& L, const uint32_t* a_br, const uint32_t* b_br, uint32_t* levels, int logn,
-                        int log_blowup, int max_level) {
-    uint64_t n = uint64_t(1) << logn;
-    uint64_t threads = log_blowup >= 3 ? n : n / 2;
-    {
-        ProfScope ps(L.prof, "trace_leaves", L.stream);
-        hipLaunchKernelGGL(k_trace_leaves, dim3(grid_for(threads, WG)), dim3(WG), 0, L.stream, a_br, b_br, levels,
-                           logn, log_blowup, max_level);
-    }
-    // levels produced in-thread beyond the leaf level
-    return (log_blowup == 4 && max_level >= 1) ? 1 : 0;
-}
-
-void launch_merkle_levels(const Launch& L, uint32_t* level_base, uint64_t count, int nlevels) {
-    ProfScope ps(L.prof, "merkle_levels", L.stream);
-    hipLaunchKernelGGL(k_merkle_levels, dim3(grid_for(count, SUB_IN)), dim3(WG), 0, L.stream, level_base, count,
-                       nlevels);
-}
-
-void launch_fri_leaves(const Launch& L, const uint32_t* layer, bool base_field, uint32_t* levels, uint64_t leaves,
-                       int nlevels) {
-    ProfScope ps(L.prof, base_field ? "fri_leaves_base" : "fri_leaves", L.stream);
-    hipLaunchKernelGGL(k_fri_leaves, dim3(grid_for(leaves, SUB_IN)), dim3(WG), 0, L.stream, layer,
-                       base_field ? 1 : 0, levels, leaves, nlevels);
-}
-
-void launch_fold(const Launch& L, const uint32_t* layer, bool base_field, uint32_t* next, PowTable winv_m,
-                 const FoldConsts& c) {
-    uint64_t quads = uint64_t(1) << (c.log_len - 2);
-    ProfScope ps(L.prof, base_field ? "fold_base" : "fold", L.stream);
-    hipLaunchKernelGGL(k_fold, dim3(grid_for(quads, WG)), dim3(WG), 0, L.stream, layer, base_field ? 1 : 0,
-                       reinterpret_cast<uint4*>(next), winv_m, c);
-}
-
-void launch_grind(const Launch& L, const uint32_t hw[16], uint32_t head_len, uint32_t bits, uint64_t limit,
-                  unsigned long long* result) {
     ProfScope ps(L.prof, "grind", L.stream);
-    uint4 h0 = make_uint4(hw[0], hw[1], hw[2], hw[3]), h1 = make_uint4(hw[4], hw[5], hw[6], hw[7]);
-    uint4 h2 = make_uint4(hw[8], hw[9], hw[10], hw[11]), h3 = make_uint4(hw[12], hw[13], hw[14], hw[15]);
-    (void)head_len;  // fixed 47-byte head, checked by the caller
-    hipLaunchKernelGGL(k_grind, dim3(2048), dim3(WG), 0, L.stream, h0, h1, h2, h3, bits, limit, result);
+    hipLaunchKernelGGL(k_grind, dim3(g), dim3(WG), 0, L.stream, jobs[0], table, n);
+}
+
+void launch_bitrev(const Launch& L, std::vector<BitrevJob> jobs) {
+    unsigned g = assign_blocks(jobs, [](const BitrevJob& j) { return grid_for(uint64_t(1) << j.logn, WG); });
+    run(L, "bitrev", k_bitrev, jobs, g, 0);
+}
+
+void launch_collect(const Launch& L, std::vector<CollectItem> items) {
+    unsigned g = assign_blocks(items, [](const CollectItem&) { return uint64_t(1); });
+    run(L, "collect", k_collect, items, g, 0);
 }
 
 void launch_copy(const Launch& L, const uint32_t* in, uint32_t* out, uint64_t words) {
+    if (L.dry) return;
     ProfScope ps(L.prof, "copy", L.stream);
     hipLaunchKernelGGL(k_copy, dim3(4096), dim3(WG), 0, L.stream, reinterpret_cast<const uint4*>(in),
                        reinterpret_cast<uint4*>(out), words / 4);
 }
 
-void launch_gather(const Launch& L, const uint32_t* base, const uint64_t* src, const uint32_t* dst,
-                   const uint32_t* len, uint32_t items, uint32_t* out) {
-    ProfScope ps(L.prof, "gather", L.stream);
-    hipLaunchKernelGGL(k_gather, dim3(grid_for(items, WG)), dim3(WG), 0, L.stream, base, src, dst, len, items, out);
+}  // namespace z0p
+
+namespace z0p {
+namespace {
+
+__device__ __forceinline__ void copy_words(uint32_t* dst, const uint32_t* src, uint32_t words) {
+    for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) dst[k] = src[k];
+}
+
+__device__ __forceinline__ uint64_t level_offset_words(uint64_t leaves, int l) {
+    uint64_t d = 0;
+    for (int k = 0; k < l; ++k) d += leaves >> k;
+    return d * 8;
+}
+
+// The descriptor table always lives in device memory here (it is large and
+// indexed by layer), so the workgroup reads its entry in place.
+__global__ __launch_bounds__(WG) void k_open(const OpenJob* tab, uint32_t njobs) {
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (tab[mid].block_begin <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    const OpenJob& J = tab[lo];
+    const uint32_t q = blockIdx.x - J.block_begin;
+    if (q >= J.nq) return;
+    const uint64_t leaf = J.queries[q];
+    uint32_t* o = J.out + uint64_t(q) * J.words_per_query;
+    uint64_t index = leaf;
+    for (int d = 0; d < J.nlayers; ++d) {
+        const uint64_t base = index / 4 * 4;
+        if (J.base[d]) {
+            copy_words(o, J.layer[d] + base, 4);
+            o += 4;
+        } else {
+            copy_words(o, J.layer[d] + base * 4, 16);
+            o += 16;
+        }
+        const uint64_t qi = base / 4;
+        for (int l = 0; l < J.target[d]; ++l) {
+            const uint64_t sib = (qi >> l) ^ 1;
+            copy_words(o, J.tree[d] + level_offset_words(J.tree_leaves[d], l) + sib * 8, 8);
+            o += 8;
+        }
+        index /= 4;
+    }
+    if (J.a == nullptr) return;
+    const int logm = J.logn + J.logb;
+    const uint64_t m = uint64_t(1) << logm, B = uint64_t(1) << J.logb;
+    const uint64_t idx = brev_bits(uint32_t(leaf), logm);
+    const uint64_t shifted = (idx + B) & (m - 1);
+    for (int side = 0; side < 2; ++side) {
+        const uint64_t row0 = (side == 0 ? idx : shifted) & ~uint64_t(7);
+        for (uint32_t k = threadIdx.x; k < 16; k += blockDim.x) {
+            const uint64_t i = row0 + (k >> 1);
+            const uint32_t j = uint32_t(i & (B - 1)), t = uint32_t(i >> J.logb);
+            const uint64_t pos = (uint64_t(brev_bits(j, J.logb)) << J.logn) | brev_bits(t, J.logn);
+            o[k] = (k & 1) ? J.b[pos] : J.a[pos];
+        }
+        o += 16;
+    }
+    for (int side = 0; side < 2; ++side) {
+        const uint64_t leaf8 = (side == 0 ? idx : shifted) / 8;
+        for (int l = 0; l < J.trace_target; ++l) {
+            const uint64_t sib = (leaf8 >> l) ^ 1;
+            copy_words(o, J.trace_tree + level_offset_words(J.trace_leaves, l) + sib * 8, 8);
+            o += 8;
+        }
+    }
+}
+
+}  // namespace
+
+void launch_open(const Launch& L, std::vector<OpenJob> jobs) {
+    if (jobs.empty()) return;
+    unsigned g = assign_blocks(jobs, [](const OpenJob& j) { return uint64_t(j.nq); });
+    if (g == 0) return;
+    const OpenJob* table = static_cast<const OpenJob*>(L.sink->push(jobs.data(), jobs.size() * sizeof(OpenJob)));
+    if (L.dry) return;
+    ProfScope ps(L.prof, "open", L.stream);
+    hipLaunchKernelGGL(k_open, dim3(g), dim3(WG), 0, L.stream, table, uint32_t(jobs.size()));
 }
 
 }  // namespace z0p

@@ -1,33 +1,51 @@
-// Kernel launchers (host-callable). Every launcher enqueues on `st` and
-// performs no synchronisation; device pointers come from the context arena.
+// Kernel launchers (host-callable).
+//
+// Every kernel is job-table driven: one launch covers one or many proofs
+// ("jobs") of possibly different sizes. Each job descriptor records the
+// first workgroup it owns (block_begin); a workgroup finds its job by binary
+// search over the table. A single job travels as a by-value kernel argument,
+// so the one-proof path needs no descriptor upload. Launchers enqueue on the
+// given stream and never synchronise.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace z0p {
 
-struct Profiler;  // per-kernel event timing, see prover.cpp
+struct Profiler;
 
-struct Launch {
-    hipStream_t stream;
-    Profiler* prof;  // may be null
+// Copies a host descriptor table into device memory for one launch.
+struct TableSink {
+    virtual const void* push(const void* host, size_t bytes) = 0;
+    virtual ~TableSink() = default;
 };
 
-// Two-level power tables for omega_m^{+-e}, e < m (m = lde size):
+// dry: compute and push the descriptor tables only, launch nothing. A
+// scheduler runs its launch sequence once dry, uploads every table with one
+// copy, then replays the sequence, which pushes the same tables in the same
+// order and so receives the same device addresses.
+struct Launch {
+    hipStream_t stream;
+    Profiler* prof;   // may be null
+    TableSink* sink;  // required for launches with more than one job
+    bool dry = false;
+};
+
+// Two-level power tables for omega_m^{+-e}, e < m:
 // w^e = lo[e & 4095] * hi[e >> 12], entries in Montgomery form.
 struct PowTable {
     const uint32_t* lo;
     const uint32_t* hi;
 };
 
-// Per-coset constants of an LDE with blowup B (all Montgomery form):
-//   shift[j] = s_j = 31 * Omega^j, used for the point x = s_j * omega_n^t.
+// Per-coset constants of an LDE with blowup B (Montgomery form).
 struct CosetTables {
-    const uint32_t* shift;     // [B]
-    const uint32_t* small;     // single-pass NTT: [B][n] s_j^k
-    const uint32_t* row;       // two-pass: [B][n2] (s_j^{n1})^r
-    const uint32_t* col;       // two-pass: [B][n1] s_j^c
+    const uint32_t* shift;  // [B] s_j = 31 * Omega^j
+    const uint32_t* small;  // single-pass NTT: [B][n] s_j^k
+    const uint32_t* row;    // two-pass: [B][n2] (s_j^{n1})^brev(r)
+    const uint32_t* col;    // two-pass: [B][n1] s_j^c
 };
 
 // NTT geometry: n = 2^logn; two-pass split n = n1 * n2 (n1 strided length).
@@ -36,75 +54,194 @@ struct NttShape {
 };
 NttShape ntt_shape(int logn);
 
-void launch_fib_trace(const Launch& L, uint32_t* col, uint64_t n, uint32_t a0, uint32_t b0);
+// ---- descriptors ----------------------------------------------------------
+struct FibJob {
+    uint32_t* col;
+    uint64_t n;
+    uint32_t a0, b0;
+    uint32_t block_begin;
+};
 
-// Inverse NTT: natural-order evaluations -> coefficients in bit-reversed
-// order, scaled by 1/n. scratch must hold n words (two-pass only).
-void launch_intt_to_br(const Launch& L, const uint32_t* evals, uint32_t* coeffs_br, uint32_t* scratch, int logn,
-                       const uint32_t* tw4096_inv, PowTable winv_m, int log_blowup, uint32_t ninv_m);
+enum NttKind : int {
+    NTT_INTT_SMALL = 0,  // natural evals -> bit-reversed coeffs * 1/n (n <= 4096)
+    NTT_INTT_COLS = 1,   // pass A of the two-pass inverse
+    NTT_INTT_ROWS = 2,   // pass B of the two-pass inverse
+    NTT_LDE_SMALL = 3,   // bit-reversed coeffs -> coset evals (n <= 4096)
+    NTT_LDE_ROWS = 4,    // pass A of the two-pass coset LDE
+    NTT_LDE_COLS = 5,    // pass B of the two-pass coset LDE
+};
+struct NttJob {
+    const uint32_t* in;
+    uint32_t* out;
+    const uint32_t* tw4096;  // omega_4096^{+-e}, e < 2048
+    PowTable pt;             // omega_m^{+-e}
+    const uint32_t* tab_a;   // LDE_SMALL: scale; LDE_ROWS: rowtab
+    const uint32_t* tab_b;   // LDE_ROWS: coltab
+    uint32_t ninv_m;
+    int kind, logn, log1, log2, logb, cols;
+    uint32_t block_begin;
+};
 
-// Coset LDE: bit-reversed coefficients (n) -> m = B*n evaluations on
-// 31*<Omega_m> in bit-reversed order (coset j lands in block brev(j)).
-// scratch must hold m words (two-pass only).
-void launch_lde_br(const Launch& L, const uint32_t* coeffs_br, uint32_t* lde_br, uint32_t* scratch, int logn,
-                   int log_blowup, const uint32_t* tw4096, PowTable w_m, CosetTables ct);
+struct LinkedJob {
+    const uint32_t* a;
+    uint32_t* b;
+    PowTable w;
+    const uint32_t* shift_m;
+    const uint32_t* zvan_m;  // [B] s_j^n - 1, Montgomery
+    uint32_t coeff;          // (b_last - a0) w_last / n, canonical
+    uint32_t w_last_m;
+    int logn, logb;
+    uint32_t block_begin;
+};
 
-// In-place bit-reversal permutation of n words into out (out != in).
-void launch_bitrev(const Launch& L, const uint32_t* in, uint32_t* out, int logn);
-
-// Linked second column (air.rs:806-844) in bit-reversed layout.
-void launch_linked_b(const Launch& L, const uint32_t* a_br, uint32_t* b_br, int logn, int log_blowup,
-                     PowTable w_m, const uint32_t* shift_m, const uint32_t* coeff_van, uint32_t w_last_m);
-
-// Composition (air.rs:847-929) written as FRI layer 0 (bit-reversed Ext4).
 struct ComposeConsts {
     uint32_t w_last_m;
     uint32_t a0, b0, b_last;
     uint32_t alpha_m[5][4];
 };
-void launch_compose(const Launch& L, const uint32_t* a_br, const uint32_t* b_br, uint32_t* layer0, int logn,
-                    int log_blowup, PowTable w_m, const uint32_t* shift_m, const uint32_t* zinv_m,
-                    const ComposeConsts& c);
+struct ComposeJob {
+    const uint32_t* a;
+    const uint32_t* b;
+    uint32_t* layer0;
+    PowTable w;
+    const uint32_t* shift_m;
+    const uint32_t* zinv_m;  // [B] 1/(s_j^n - 1)
+    ComposeConsts c;
+    int logn, logb;
+    uint32_t block_begin;
+};
 
-// Trace Merkle leaves (8 joint rows A||B per 64-byte leaf) from the
-// bit-reversed columns, written in natural leaf order into `levels`
-// (contiguous level arrays). With B = 16 the two leaves of a trace step are
-// also reduced to their parent (level 1) when max_level >= 1. Returns the
-// highest level written.
-int launch_trace_leaves(const Launch& L, const uint32_t* a_br, const uint32_t* b_br, uint32_t* levels, int logn,
-                        int log_blowup, int max_level);
+// Merkle leaves. Kinds: trace rows (8 joint A||B rows per leaf from the
+// bit-reversed columns), FRI Ext4 quads, FRI base-field quads (upper
+// coordinates zero), raw 64-byte leaves.
+enum LeafKind : int { LEAF_TRACE = 0, LEAF_EXT = 1, LEAF_BASE = 2, LEAF_RAW = 3 };
+struct LeavesJob {
+    const uint32_t* src;
+    const uint32_t* src_b;  // trace column B
+    uint32_t* levels;       // level 0 then level 1, contiguous
+    uint64_t leaves;
+    int kind;
+    int with_parent;        // also write level 1 (needs leaves >= 2)
+    int logn, logb;         // trace geometry
+    uint32_t block_begin;
+};
 
-// Generic fused subtree levels: from `count` digests at level_base produce
-// up to `nlevels` (<= 9) parent levels, stored contiguously after it.
-void launch_merkle_levels(const Launch& L, uint32_t* level_base, uint64_t count, int nlevels);
+// One or two parent levels from a level of `count` digests, one node (or
+// one node pair plus their parent) per thread: the wide levels.
+struct LevelsJob {
+    uint32_t* level;  // this level; parents follow contiguously
+    uint64_t count;
+    int fuse;         // 1 or 2 levels
+    uint32_t block_begin;
+};
 
-// FRI layer leaves: each leaf is 4 consecutive Ext4 values (64 bytes); base
-// layers hold one u32 per value with the upper coordinates zero. Writes the
-// leaf level and up to `nlevels` more fused levels contiguously.
-void launch_fri_leaves(const Launch& L, const uint32_t* layer, bool base_field, uint32_t* levels, uint64_t leaves,
-                       int nlevels);
+// Up to 9 levels in LDS per 512-digest group: the narrow tail.
+struct TailJob {
+    uint32_t* level;
+    uint64_t count;
+    int nlevels;
+    uint32_t block_begin;
+};
 
-// Arity-4 fold (fri.rs:1273-1331) of a bit-reversed layer of length len.
 struct FoldConsts {
     uint32_t ch_m[3][4];
     uint32_t unit_m;
-    uint32_t ginv_m;      // 31^{-4^d}, Montgomery
-    int log_len;          // log2 of this layer's length
-    int depth;            // fold depth d
+    uint32_t ginv_m;  // 31^{-4^d}, Montgomery
+    int log_len;      // log2 of this layer's length
+    int depth;        // fold depth d
 };
-void launch_fold(const Launch& L, const uint32_t* layer, bool base_field, uint32_t* next, PowTable winv_m,
-                 const FoldConsts& c);
+struct FoldJob {
+    const uint32_t* layer;
+    uint32_t* next;
+    PowTable winv;
+    FoldConsts c;
+    int base_field;
+    uint32_t block_begin;
+};
 
-// Proof-of-work: smallest nonce in [0, limit) whose digest of head||nonce has
-// >= bits leading zero bits; result (u64, ~0 when none) written to *result.
-void launch_grind(const Launch& L, const uint32_t head_words[16], uint32_t head_len, uint32_t bits, uint64_t limit,
-                  unsigned long long* result);
+struct GrindJob {
+    uint32_t hw[16];  // 47-byte head as LE words, zero-padded
+    uint32_t bits;
+    uint32_t nblocks;
+    uint64_t limit;
+    unsigned long long* result;
+    uint32_t block_begin;
+};
 
-// Streaming copy of `words` (a multiple of 4) u32: counter calibration.
+// ---- launchers ---------------------------------------------------------------
+void launch_fib(const Launch& L, std::vector<FibJob> jobs);
+void launch_ntt(const Launch& L, std::vector<NttJob> jobs);
+void launch_linked(const Launch& L, std::vector<LinkedJob> jobs);
+void launch_compose(const Launch& L, std::vector<ComposeJob> jobs);
+void launch_leaves(const Launch& L, std::vector<LeavesJob> jobs);
+void launch_levels(const Launch& L, std::vector<LevelsJob> jobs);
+void launch_tail(const Launch& L, std::vector<TailJob> jobs);
+void launch_fold(const Launch& L, std::vector<FoldJob> jobs);
+// Initialises every job's result to ~0, then searches.
+void launch_grind(const Launch& L, std::vector<GrindJob> jobs);
+// out[i] = in[brev(i)] mod p: raw coefficients to the bit-reversed,
+// canonical order the coset LDE consumes.
+struct BitrevJob {
+    const uint32_t* in;
+    uint32_t* out;
+    int logn;
+    uint32_t block_begin;
+};
+void launch_bitrev(const Launch& L, std::vector<BitrevJob> jobs);
+// Device-to-device copies of small regions into one staging buffer, one
+// workgroup per item (caps, final layers, nonces of many proofs at once).
+struct CollectItem {
+    const uint32_t* src;
+    uint32_t* dst;
+    uint32_t words;
+    uint32_t block_begin;
+};
+void launch_collect(const Launch& L, std::vector<CollectItem> items);
 void launch_copy(const Launch& L, const uint32_t* in, uint32_t* out, uint64_t words);
 
-// Gather words: for item i copy len[i] words from base + src[i] to out + dst[i].
-void launch_gather(const Launch& L, const uint32_t* base, const uint64_t* src, const uint32_t* dst,
-                   const uint32_t* len, uint32_t items, uint32_t* out);
+// NTT job builders: the passes of one inverse / coset LDE. pass2 is empty
+// for single-pass sizes; otherwise pass1 must complete before pass2.
+struct NttPlan {
+    std::vector<NttJob> pass1, pass2;
+};
+NttPlan plan_intt(const uint32_t* evals, uint32_t* coeffs_br, uint32_t* scratch, int logn, const uint32_t* tw4096_inv,
+                  PowTable winv_m, int logb, uint32_t ninv_m);
+NttPlan plan_lde(const uint32_t* coeffs_br, uint32_t* lde_br, uint32_t* scratch, int logn, int logb,
+                 const uint32_t* tw4096, PowTable w_m, CosetTables ct);
+
+// A level of at most this many digests is reduced by the LDS tail kernel;
+// wider levels use the one-node-per-thread kernel.
+constexpr uint64_t TAIL_MAX_COUNT = uint64_t(1) << 13;
+
+}  // namespace z0p
+
+namespace z0p {
+
+// Openings of one proof: a workgroup per query copies the quad values and
+// path siblings of every opened FRI layer and, for AIR proofs, the two
+// trace leaves and their paths, to out + q * words_per_query in the fixed
+// order the host parses (see prover.cpp: open_words).
+constexpr int MAX_OPEN_LAYERS = 14;
+struct OpenJob {
+    const uint32_t* layer[MAX_OPEN_LAYERS];
+    const uint32_t* tree[MAX_OPEN_LAYERS];
+    uint64_t tree_leaves[MAX_OPEN_LAYERS];
+    int target[MAX_OPEN_LAYERS];
+    int base[MAX_OPEN_LAYERS];
+    int nlayers;
+    // AIR trace (a == nullptr for a FRI-only proof)
+    const uint32_t* a;
+    const uint32_t* b;
+    const uint32_t* trace_tree;
+    uint64_t trace_leaves;
+    int trace_target;
+    int logn, logb;
+    const uint64_t* queries;
+    uint32_t nq;
+    uint32_t words_per_query;
+    uint32_t* out;
+    uint32_t block_begin;
+};
+void launch_open(const Launch& L, std::vector<OpenJob> jobs);
 
 }  // namespace z0p

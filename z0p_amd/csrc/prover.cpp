@@ -1,21 +1,34 @@
 // Host orchestration of the GPU prover: Fibonacci AIR (air.rs:365-599) and
-// the FRI commitment (fri.rs:408-666), driving the kernels on the context
-// stream and the Fiat-Shamir transcript on the host.
+// the FRI commitment (fri.rs:408-666).
 //
-// The transcript is the only serial dependency: each FRI layer's cap must be
-// absorbed before its three fold challenges exist (fri.rs:594-612), the
-// proof-of-work gates the query indices (fri.rs:645-656). Each such point is
-// one small device-to-host copy plus a stream synchronisation; all bulk data
-// stays resident in device memory.
+// Every proof is a small state machine whose phases are cut by the
+// Fiat-Shamir transcript: the trace cap gates the composition challenges
+// (air.rs:525-540), each FRI layer cap gates that layer's fold challenges
+// (fri.rs:594-612), and the proof-of-work gates the query indices
+// (fri.rs:645-656). One scheduler loop advances every admitted proof by one
+// phase per iteration. It launches each kernel type once for all proofs
+// (descriptor tables), gathers the few words the transcript needs (caps,
+// final layers, nonces, openings) with one collect kernel and one
+// device-to-host copy, synchronises once, then runs the host transcript
+// steps. Bulk data never leaves device memory. Proofs are admitted as pool
+// memory frees up, so proofs of different sizes and phases share every
+// launch; a single proof is a batch of one.
 #include "prover.hpp"
 
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <cstring>
+#include <deque>
+#include <variant>
 
 #include "blake3.hpp"
 #include "params.hpp"
 
 namespace z0p {
+
+// The field modulus under a name that the proof variables (P) do not shadow.
+constexpr uint32_t P_MOD = P;
 
 uint32_t brev_host(uint64_t x, int bits) {
     uint64_t r = 0;
@@ -65,6 +78,11 @@ std::vector<uint32_t> geometric_m(uint32_t base, size_t len) {
     return out;
 }
 
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
 }  // namespace
 
 uint32_t fib_b_last(uint32_t a0, uint32_t b0, uint64_t n) {
@@ -90,14 +108,17 @@ const GeometryTables& Ctx::geometry(int logn, int logb) {
     t->w.hi = upload(t->owned, geometric_m(pow_c(wm, 4096), hi_len));
     t->winv.lo = upload(t->owned, geometric_m(wm_inv, 4096));
     t->winv.hi = upload(t->owned, geometric_m(pow_c(wm_inv, 4096), hi_len));
-    std::vector<uint32_t> shift(B), zinv(B);
+    std::vector<uint32_t> shift(B), zinv(B), zvan(B);
     for (uint64_t j = 0; j < B; ++j) {
         uint32_t s = mul(GENERATOR, pow_c(wm, j));
         shift[j] = to_mont(s);
-        zinv[j] = to_mont(inv_c(sub(pow_c(s, n), 1)));
+        uint32_t v = sub(pow_c(s, n), 1);
+        zvan[j] = to_mont(v);
+        zinv[j] = to_mont(inv_c(v));
     }
     t->coset.shift = upload(t->owned, shift);
     t->zinv = upload(t->owned, zinv);
+    t->zvan = upload(t->owned, zvan);
     NttShape sh = ntt_shape(logn);
     if (sh.log1 == 0) {
         std::vector<uint32_t> small;
@@ -128,8 +149,9 @@ const GeometryTables& Ctx::geometry(int logn, int logb) {
 
 Ctx::~Ctx() {
     tables.clear();
-    if (arena) (void)hipFree(arena);
-    if (pinned) (void)hipHostFree(pinned);
+    if (stage_dev) (void)hipFree(stage_dev);
+    if (stage_host) (void)hipHostFree(stage_host);
+    if (scratch) (void)hipFree(scratch);
     if (tw4096) (void)hipFree(tw4096);
     if (tw4096_inv) (void)hipFree(tw4096_inv);
     if (stream) (void)hipStreamDestroy(stream);
@@ -149,12 +171,10 @@ Ctx* ctx_create(int device) {
     hip_check(hipMalloc(&c->tw4096_inv, 2048 * 4));
     hip_check(hipMemcpy(c->tw4096, fwd.data(), 2048 * 4, hipMemcpyHostToDevice));
     hip_check(hipMemcpy(c->tw4096_inv, inv.data(), 2048 * 4, hipMemcpyHostToDevice));
-    c->reserve_pinned(size_t(4) << 20);
+    c->ring.init(size_t(32) << 20);
+    c->reserve_stage(size_t(1) << 16);
     return c.release();
 }
-
-// ---------------------------------------------------------------------------
-// FRI over a device-resident bit-reversed layer 0.
 
 TreeGeom tree_geom(uint64_t leaves, uint64_t cap_height) {
     TreeGeom g;
@@ -165,16 +185,91 @@ TreeGeom tree_geom(uint64_t leaves, uint64_t cap_height) {
     return g;
 }
 
-// Build the parent levels of a tree whose levels [0, have] exist.
-void build_levels(Ctx& ctx, uint32_t* tree, const TreeGeom& g, int have) {
-    Launch L = ctx.launch();
-    int cur = have;
-    while (cur < g.target) {
-        int k = std::min(9, g.target - cur);
-        launch_merkle_levels(L, tree + g.level_offset_words(cur), g.leaves >> cur, k);
-        cur += k;
+void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves) {
+    leaves.levels = tree;
+    leaves.leaves = g.leaves;
+    leaves.with_parent = g.target >= 1 ? 1 : 0;
+    chain.push_back(leaves);
+    int have = leaves.with_parent;
+    while (have < g.target) {
+        const uint64_t count = g.leaves >> have;
+        const int remaining = g.target - have;
+        uint32_t* lvl = tree + g.level_offset_words(have);
+        if (count > TAIL_MAX_COUNT) {
+            const int fuse = remaining >= 2 ? 2 : 1;
+            chain.push_back(LevelsJob{lvl, count, fuse, 0});
+            have += fuse;
+        } else {
+            const int k = std::min(9, remaining);
+            chain.push_back(TailJob{lvl, count, k, 0});
+            have += k;
+        }
     }
 }
+
+namespace {
+
+// Per-round buckets: step r of every chain, grouped by kernel type.
+struct Buckets {
+    std::vector<FibJob> fib;
+    std::vector<BitrevJob> bitrev;
+    std::vector<NttJob> ntt;
+    std::vector<LinkedJob> linked;
+    std::vector<ComposeJob> compose;
+    std::vector<LeavesJob> leaves;
+    std::vector<LevelsJob> levels;
+    std::vector<TailJob> tail;
+    std::vector<FoldJob> fold;
+    std::vector<GrindJob> grind;
+    std::vector<OpenJob> open;
+
+    std::vector<FibJob>& of(const FibJob&) { return fib; }
+    std::vector<BitrevJob>& of(const BitrevJob&) { return bitrev; }
+    std::vector<NttJob>& of(const NttJob&) { return ntt; }
+    std::vector<LinkedJob>& of(const LinkedJob&) { return linked; }
+    std::vector<ComposeJob>& of(const ComposeJob&) { return compose; }
+    std::vector<LeavesJob>& of(const LeavesJob&) { return leaves; }
+    std::vector<LevelsJob>& of(const LevelsJob&) { return levels; }
+    std::vector<TailJob>& of(const TailJob&) { return tail; }
+    std::vector<FoldJob>& of(const FoldJob&) { return fold; }
+    std::vector<GrindJob>& of(const GrindJob&) { return grind; }
+    std::vector<OpenJob>& of(const OpenJob&) { return open; }
+
+    void add(const Step& s) {
+        std::visit([this](const auto& j) { of(j).push_back(j); }, s);
+    }
+    void launch(const Launch& L) const {
+        if (!fib.empty()) launch_fib(L, fib);
+        if (!bitrev.empty()) launch_bitrev(L, bitrev);
+        if (!ntt.empty()) launch_ntt(L, ntt);
+        if (!linked.empty()) launch_linked(L, linked);
+        if (!compose.empty()) launch_compose(L, compose);
+        if (!leaves.empty()) launch_leaves(L, leaves);
+        if (!levels.empty()) launch_levels(L, levels);
+        if (!tail.empty()) launch_tail(L, tail);
+        if (!fold.empty()) launch_fold(L, fold);
+        if (!grind.empty()) launch_grind(L, grind);
+        if (!open.empty()) launch_open(L, open);
+    }
+};
+
+}  // namespace
+
+void launch_chains(const Launch& L, const std::vector<const std::vector<Step>*>& chains,
+                   const std::vector<CollectItem>& collect) {
+    size_t rounds = 0;
+    for (auto* c : chains) rounds = std::max(rounds, c->size());
+    for (size_t r = 0; r < rounds; ++r) {
+        Buckets b;
+        for (auto* c : chains)
+            if (r < c->size()) b.add((*c)[r]);
+        b.launch(L);
+    }
+    if (!collect.empty()) launch_collect(L, collect);
+}
+
+// ---------------------------------------------------------------------------
+// Proof state machine.
 
 namespace {
 
@@ -182,6 +277,12 @@ using Digest = std::array<uint8_t, 32>;
 
 void digest_words_to_bytes(const uint32_t* w, uint8_t* out) {
     for (int k = 0; k < 8; ++k) store_le32(out + 4 * k, w[k]);
+}
+
+std::vector<Digest> read_digests(const uint32_t* w, uint64_t count) {
+    std::vector<Digest> out(count);
+    for (uint64_t i = 0; i < count; ++i) digest_words_to_bytes(w + 8 * i, out[i].data());
+    return out;
 }
 
 // Host Merkle cap for the (small) final layer: commit_layer (fri.rs:1401-1428).
@@ -230,452 +331,672 @@ std::vector<Digest> host_layer_cap(const std::vector<E4>& vals, uint64_t cap_hei
     return level;
 }
 
-struct FriLayerDev {
-    uint32_t* values = nullptr;  // Ext4 (4 words) or base (1 word) per value
-    bool base = false;
-    uint64_t len = 0;
-    uint32_t* tree = nullptr;
-    TreeGeom geom;
-};
-
-struct FriResult {
-    std::vector<FriLayerDev> layers;  // opened layers (all but the final)
-    std::vector<std::vector<Digest>> caps;
-    std::vector<E4> final_values;
-    uint64_t nonce = 0;
-    std::vector<uint64_t> queries;
-};
-
-uint64_t fri_arena_words(uint64_t m, bool base0, uint64_t final_size, uint64_t cap_height) {
-    uint64_t w = 0;
-    for (uint64_t len = m; len > final_size; len /= 4) {
-        bool base = base0 && len == m;
-        if (len != m) w += len * 4;
-        (void)base;
-        w += tree_geom(len / 4, cap_height).words();
-        w += 64;  // alignment slack
-    }
-    w += final_size * 4 + 64;
-    return w;
-}
-
-// fri.rs:551-633 from layer 0 already on the device, then fri.rs:645-656
-// (grind and query sampling). Leaves tree data resident for openings.
-FriResult fri_commit_and_sample(Ctx& ctx, uint32_t* layer0, bool base0, uint64_t m, uint64_t poly_len,
-                                uint64_t degree_bound, const z0p_fri_params& fp, PowTable winv, Transcript& tr) {
-    FriResult R;
-    Launch L = ctx.launch();
-    uint8_t dims[24];
-    store_le64(dims, m);
-    store_le64(dims + 8, poly_len);
-    store_le64(dims + 16, degree_bound);
-    tr.absorb("fri-dims", dims, 24);
-    const uint32_t unit_m = to_mont(root_of_unity(2));
-    const uint32_t ginv = inv_c(GENERATOR);
-
-    uint32_t* cur = layer0;
-    bool cur_base = base0;
-    uint64_t len = m;
-    int depth = 0;
-    while (true) {
-        if (len == fp.final_size) {
-            // Final layer: values to the host, cap computed there.
-            size_t words = cur_base ? len : len * 4;
-            hip_check(hipMemcpyAsync(ctx.pinned, cur, words * 4, hipMemcpyDeviceToHost, ctx.stream));
-            ctx.sync();
-            const uint32_t* w = reinterpret_cast<const uint32_t*>(ctx.pinned);
-            R.final_values.resize(len);
-            for (uint64_t i = 0; i < len; ++i)
-                R.final_values[i] = cur_base ? e4_base(w[i]) : E4{{w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]}};
-            auto cap = host_layer_cap(R.final_values, fp.cap_height);
-            for (auto& d : cap) tr.absorb("fri-layer-root", d.data(), 32);
-            R.caps.push_back(cap);
-            break;
-        }
-        FriLayerDev lay;
-        lay.values = cur;
-        lay.base = cur_base;
-        lay.len = len;
-        lay.geom = tree_geom(len / 4, fp.cap_height);
-        lay.tree = ctx.alloc<uint32_t>(lay.geom.words());
-        int first_levels = std::min(9, lay.geom.target);
-        launch_fri_leaves(L, cur, cur_base, lay.tree, lay.geom.leaves, first_levels);
-        build_levels(ctx, lay.tree, lay.geom, first_levels);
-        uint64_t cap_n = uint64_t(1) << lay.geom.drop;
-        hip_check(hipMemcpyAsync(ctx.pinned, lay.tree + lay.geom.level_offset_words(lay.geom.target), cap_n * 32,
-                                 hipMemcpyDeviceToHost, ctx.stream));
-        ctx.sync();
-        std::vector<Digest> cap(cap_n);
-        for (uint64_t i = 0; i < cap_n; ++i) {
-            std::memcpy(cap[i].data(), ctx.pinned + 32 * i, 32);
-            tr.absorb("fri-layer-root", cap[i].data(), 32);
-        }
-        R.caps.push_back(cap);
-        R.layers.push_back(lay);
-        FoldConsts fc{};
-        for (int k = 0; k < 3; ++k) {
-            E4 ch = tr.challenge_ext("fri-fold");
-            for (int c = 0; c < 4; ++c) fc.ch_m[k][c] = to_mont(ch.v[c]);
-        }
-        fc.unit_m = unit_m;
-        fc.ginv_m = to_mont(pow_c(ginv, uint64_t(1) << (2 * depth)));
-        fc.log_len = int(ilog2(len));
-        fc.depth = depth;
-        uint32_t* next = ctx.alloc<uint32_t>((len / 4) * 4);
-        launch_fold(L, cur, cur_base, next, winv, fc);
-        cur = next;
-        cur_base = false;
-        len /= 4;
-        ++depth;
-    }
-    std::vector<uint8_t> fin(R.final_values.size() * 16);
-    for (size_t i = 0; i < R.final_values.size(); ++i)
-        for (int c = 0; c < 4; ++c) store_le32(fin.data() + 16 * i + 4 * c, R.final_values[i].v[c]);
-    tr.absorb("fri-final", fin);
-
-    // Proof-of-work (transcript.rs:206-228) on the device.
-    if (fp.grinding_bits > 0) {
-        std::vector<uint8_t> head = tr.pow_head("fri-pow");
-        require(head.size() == 47, Z0P_ERR_DEVICE);
-        uint32_t hw[16] = {0};
-        for (size_t i = 0; i < head.size(); ++i) hw[i >> 2] |= uint32_t(head[i]) << (8 * (i & 3));
-        unsigned long long* res = ctx.alloc<unsigned long long>(1);
-        uint64_t found = ~uint64_t(0);
-        for (uint64_t window = 0; found == ~uint64_t(0); ++window) {
-            // Each launch searches [0, limit); later windows only widen the
-            // bound, and the kernel returns the smallest passing nonce below it.
-            uint64_t limit = uint64_t(1) << std::min<uint64_t>(63, 24 + 4 * window);
-            uint64_t init = ~uint64_t(0);
-            std::memcpy(ctx.pinned, &init, 8);
-            hip_check(hipMemcpyAsync(res, ctx.pinned, 8, hipMemcpyHostToDevice, ctx.stream));
-            launch_grind(L, hw, 47, fp.grinding_bits, limit, res);
-            hip_check(hipMemcpyAsync(ctx.pinned + 8, res, 8, hipMemcpyDeviceToHost, ctx.stream));
-            ctx.sync();
-            std::memcpy(&found, ctx.pinned + 8, 8);
-        }
-        R.nonce = found;
-        uint8_t nb[8];
-        store_le64(nb, found);
-        tr.absorb("fri-pow", nb, 8);
-    }
-    for (uint64_t q = 0; q < fp.num_queries; ++q) R.queries.push_back(tr.challenge_index("fri-query", m));
-    return R;
-}
-
-// Opening gather plan: (src word offset from arena base, words, dst word).
-struct Gather {
-    std::vector<uint64_t> src;
-    std::vector<uint32_t> dst, len;
-    uint32_t total = 0;
-    uint32_t add(const Ctx& ctx, const uint32_t* p, uint32_t words) {
-        src.push_back(uint64_t(reinterpret_cast<const uint8_t*>(p) - ctx.arena) / 4);
-        dst.push_back(total);
-        len.push_back(words);
-        uint32_t at = total;
-        total += words;
-        return at;
-    }
-};
-
-struct FriOpenPlan {
-    // per query, per opened layer: offset of the 4 values and of each path step
-    std::vector<std::vector<uint32_t>> values_at;
-    std::vector<std::vector<std::vector<uint32_t>>> path_at;
-};
-
-FriOpenPlan plan_fri_openings(Ctx& ctx, const FriResult& R, Gather& g) {
-    FriOpenPlan P;
-    for (uint64_t leaf : R.queries) {
-        std::vector<uint32_t> vals;
-        std::vector<std::vector<uint32_t>> paths;
-        uint64_t index = leaf;
-        for (const FriLayerDev& lay : R.layers) {
-            uint64_t base = index / 4 * 4;
-            vals.push_back(lay.base ? g.add(ctx, lay.values + base, 4) : g.add(ctx, lay.values + base * 4, 16));
-            std::vector<uint32_t> steps;
-            uint64_t qi = base / 4;
-            for (int l = 0; l < lay.geom.target; ++l) {
-                uint64_t sib = (qi >> l) ^ 1;
-                steps.push_back(g.add(ctx, lay.tree + lay.geom.level_offset_words(l) + sib * 8, 8));
-            }
-            paths.push_back(steps);
-            index /= 4;
-        }
-        P.values_at.push_back(vals);
-        P.path_at.push_back(paths);
-    }
-    return P;
-}
-
-void run_gather(Ctx& ctx, const Gather& g, std::vector<uint32_t>& out) {
-    const uint32_t items = uint32_t(g.src.size());
-    size_t desc_bytes = size_t(items) * 16;
-    size_t out_bytes = size_t(g.total) * 4;
-    ctx.reserve_pinned(desc_bytes + out_bytes + 4096);
-    uint64_t* d_src = ctx.alloc<uint64_t>(items);
-    uint32_t* d_dst = ctx.alloc<uint32_t>(items);
-    uint32_t* d_len = ctx.alloc<uint32_t>(items);
-    uint32_t* d_out = ctx.alloc<uint32_t>(g.total);
-    uint8_t* hp = ctx.pinned;
-    std::memcpy(hp, g.src.data(), size_t(items) * 8);
-    std::memcpy(hp + size_t(items) * 8, g.dst.data(), size_t(items) * 4);
-    std::memcpy(hp + size_t(items) * 12, g.len.data(), size_t(items) * 4);
-    hip_check(hipMemcpyAsync(d_src, hp, size_t(items) * 8, hipMemcpyHostToDevice, ctx.stream));
-    hip_check(hipMemcpyAsync(d_dst, hp + size_t(items) * 8, size_t(items) * 4, hipMemcpyHostToDevice, ctx.stream));
-    hip_check(hipMemcpyAsync(d_len, hp + size_t(items) * 12, size_t(items) * 4, hipMemcpyHostToDevice, ctx.stream));
-    launch_gather(ctx.launch(), reinterpret_cast<const uint32_t*>(ctx.arena), d_src, d_dst, d_len, items, d_out);
-    hip_check(hipMemcpyAsync(hp + desc_bytes, d_out, out_bytes, hipMemcpyDeviceToHost, ctx.stream));
-    ctx.sync();
-    out.resize(g.total);
-    std::memcpy(out.data(), hp + desc_bytes, out_bytes);
-}
-
 void write_cap(Writer& w, const std::vector<Digest>& cap) {
     w.u32(uint32_t(cap.size()));
     for (auto& d : cap) w.raw(d.data(), 32);
 }
 
-void write_path(Writer& w, const std::vector<uint32_t>& at, const std::vector<uint32_t>& words, uint64_t index) {
-    w.u32(uint32_t(at.size()));
-    for (size_t l = 0; l < at.size(); ++l) {
+// Authentication path from `count` digests of 8 words; one direction byte
+// per step, 1 when the opened node is the left child (merkle.rs path order).
+const uint32_t* write_path(Writer& w, const uint32_t* p, int count, uint64_t index) {
+    w.u32(uint32_t(count));
+    for (int l = 0; l < count; ++l) {
         uint8_t d[32];
-        digest_words_to_bytes(&words[at[l]], d);
+        digest_words_to_bytes(p + 8 * l, d);
         w.raw(d, 32);
         w.byte(((index >> l) & 1) == 0 ? 1 : 0);
+    }
+    return p + 8 * count;
+}
+
+}  // namespace
+
+enum Phase : int { PH_FRONT = 0, PH_COMPOSE, PH_FOLD, PH_GRIND, PH_OPEN, PH_DONE };
+
+struct Proof {
+    // request
+    bool air = true;
+    z0p_air_params ap{};
+    z0p_fri_params fp{};
+    Transcript* tr = nullptr;
+    const uint32_t* coeff_src = nullptr;  // FRI: host coefficients
+    uint64_t tag = 0;
+    // geometry
+    uint64_t n = 0, B = 0, m = 0, poly_len = 0, degree_bound = 0;
+    int logn = 0, logb = 0;
+    size_t folds = 0;
+    const GeometryTables* gt = nullptr;
+    TreeGeom tg;                     // AIR trace tree
+    std::vector<uint64_t> lens;      // FRI layer d length, d = 0..folds
+    std::vector<TreeGeom> geoms;     // FRI layer d tree, d < folds
+    // device memory: one pool block
+    uint8_t* mem = nullptr;
+    size_t mem_bytes = 0;
+    size_t stage_need = 0;  // words of staging any one iteration may use
+    uint32_t *col = nullptr, *coeff = nullptr, *scratch = nullptr, *A = nullptr, *Bc = nullptr, *trace_tree = nullptr;
+    std::vector<uint32_t*> vals, trees;
+    unsigned long long* grind_res = nullptr;
+    uint32_t wpq = 0, fri_wpq = 0;  // opening words per query (all / FRI part)
+    // AIR constants
+    uint32_t a0 = 0, b0 = 0, b_last = 0, coeff_c = 0, w_last = 0;
+    ComposeConsts cc{};
+    // transcript-derived state
+    std::vector<Digest> column_cap;
+    std::vector<std::vector<Digest>> caps;
+    std::vector<E4> final_values;
+    FoldConsts fold{};
+    uint32_t pow_hw[16] = {0};
+    int window = 0;
+    uint64_t nonce = 0;
+    std::vector<uint64_t> queries;
+    // progress
+    Phase phase = PH_FRONT;
+    size_t depth = 0;      // layer folded in the current PH_FOLD iteration
+    size_t stage_off = 0;  // words, this iteration
+    // result
+    int status = Z0P_OK;
+    std::vector<uint8_t> proof, commitment;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+};
+
+namespace {
+
+size_t layout(Proof& P, uint8_t* base) {
+    size_t off = 0;
+    auto take = [&](uint64_t words) -> uint32_t* {
+        uint32_t* p = base ? reinterpret_cast<uint32_t*>(base + off) : nullptr;
+        off += (size_t(words) * 4 + 255) & ~size_t(255);
+        return p;
+    };
+    P.col = take(P.n);
+    P.coeff = take(P.n);
+    P.scratch = take(P.m);
+    if (P.air) {
+        P.A = take(P.m);
+        P.Bc = take(P.m);
+        P.trace_tree = take(P.tg.words());
+    }
+    P.vals.assign(P.folds + 1, nullptr);
+    P.trees.assign(P.folds, nullptr);
+    for (size_t d = 0; d <= P.folds; ++d) {
+        P.vals[d] = take(d == 0 && !P.air ? P.m : P.lens[d] * 4);
+        if (d < P.folds) P.trees[d] = take(P.geoms[d].words());
+    }
+    P.grind_res = reinterpret_cast<unsigned long long*>(take(2));
+    return off;
+}
+
+void init_fri_shape(Proof& P, const z0p_fri_params& fp) {
+    P.fp = fp;
+    P.folds = fri_num_folds(P.m, fp.final_size, P.degree_bound);
+    require(P.folds <= size_t(MAX_OPEN_LAYERS), Z0P_ERR_INVALID_DOMAIN_SIZE);
+    P.lens.clear();
+    P.geoms.clear();
+    for (uint64_t len = P.m, d = 0; d <= P.folds; ++d, len /= 4) {
+        P.lens.push_back(len);
+        if (d < P.folds) P.geoms.push_back(tree_geom(len / 4, fp.cap_height));
+    }
+    uint32_t w = 0;
+    for (size_t d = 0; d < P.folds; ++d) w += (d == 0 && !P.air ? 4 : 16) + 8 * uint32_t(P.geoms[d].target);
+    P.fri_wpq = w;
+    if (P.air) w += 32 + 16 * uint32_t(P.tg.target);
+    P.wpq = w;
+    P.stage_need = size_t(fp.num_queries) * P.wpq + 256 * 8 + P.lens[P.folds] * 4 + 256;
+}
+
+void init_air(Ctx& c, Proof& P, const z0p_air_params& p, Transcript* tr) {
+    validate_air(p);
+    const z0p_fri_params fp = air_fri_params(p);
+    validate_fri(fp);
+    P.air = true;
+    P.ap = p;
+    P.tr = tr;
+    P.n = p.n_steps;
+    P.B = p.blowup;
+    P.m = P.n * P.B;
+    P.logn = int(ilog2(P.n));
+    P.logb = int(ilog2(P.B));
+    P.poly_len = P.m;
+    P.degree_bound = P.n - 1;
+    P.tg = tree_geom(P.m / 8, p.cap_height);
+    init_fri_shape(P, fp);
+    P.a0 = p.a0 % P_MOD;
+    P.b0 = p.b0 % P_MOD;
+    P.b_last = p.b_last % P_MOD;
+    // Linked LDE constant (air.rs:815-820): coeff = (b_last - a0) w_last / n.
+    const uint32_t actual_b_last = fib_b_last(P.a0, P.b0, P.n);
+    P.w_last = pow_c(root_of_unity(uint32_t(P.logn)), P.n - 1);
+    P.coeff_c = mul(mul(sub(actual_b_last, P.a0), P.w_last), inv_c(uint32_t(P.n % P_MOD)));
+    P.gt = &c.geometry(P.logn, P.logb);
+    P.mem_bytes = layout(P, nullptr);
+}
+
+void init_fri(Ctx& c, Proof& P, const uint32_t* coeffs, uint64_t n, const z0p_fri_params& fp, Transcript* tr) {
+    validate_fri(fp);
+    require(n != 0, Z0P_ERR_EMPTY_INPUT);
+    require(pow2(n) && n <= MAX_DOMAIN, Z0P_ERR_INVALID_DOMAIN_SIZE);
+    require(n <= MAX_DOMAIN / fp.blowup, Z0P_ERR_INVALID_DOMAIN_SIZE);
+    P.air = false;
+    P.tr = tr;
+    P.coeff_src = coeffs;
+    P.n = n;
+    P.B = fp.blowup;
+    P.m = n * fp.blowup;
+    P.logn = int(ilog2(n));
+    P.logb = int(ilog2(P.B));
+    P.poly_len = n;
+    P.degree_bound = n - 1;
+    init_fri_shape(P, fp);
+    P.gt = &c.geometry(P.logn, P.logb);
+    P.mem_bytes = layout(P, nullptr);
+}
+
+// Host work when a proof is admitted (its memory assigned).
+void start(Ctx& c, Proof& P) {
+    layout(P, P.mem);
+    Transcript& tr = *P.tr;
+    if (P.air) {
+        // absorb_params (air.rs:754-766): 56 bytes, grinding_bits as u32.
+        const z0p_air_params& p = P.ap;
+        uint8_t pb[56];
+        store_le64(pb, p.n_steps);
+        store_le64(pb + 8, p.blowup);
+        store_le64(pb + 16, p.num_queries);
+        store_le64(pb + 24, p.final_size);
+        store_le32(pb + 32, p.grinding_bits);
+        store_le64(pb + 36, p.cap_height);
+        store_le32(pb + 44, P.a0);
+        store_le32(pb + 48, P.b0);
+        store_le32(pb + 52, P.b_last);
+        tr.absorb("fib/params", pb, 56);
+    } else {
+        uint8_t dims[24];
+        store_le64(dims, P.m);
+        store_le64(dims + 8, P.poly_len);
+        store_le64(dims + 16, P.degree_bound);
+        tr.absorb("fri-dims", dims, 24);
+        hip_check(hipMemcpyAsync(P.col, P.coeff_src, P.n * 4, hipMemcpyHostToDevice, c.stream));
+    }
+    P.phase = PH_FRONT;
+}
+
+uint32_t* stage_take(Ctx& c, size_t& used, size_t words, size_t& off) {
+    off = used;
+    used += (words + 63) & ~size_t(63);
+    require(used <= c.stage_words, Z0P_ERR_DEVICE);
+    return c.stage_dev + off;
+}
+
+void collect_cap(Ctx& c, Proof& P, uint32_t* tree, const TreeGeom& g, std::vector<CollectItem>& items,
+                 size_t& used) {
+    const uint32_t words = uint32_t(8u << g.drop);
+    uint32_t* dst = stage_take(c, used, words, P.stage_off);
+    items.push_back(CollectItem{tree + g.level_offset_words(g.target), dst, words, 0});
+}
+
+LeavesJob leaves_of(int kind, const uint32_t* src) {
+    LeavesJob j{};
+    j.kind = kind;
+    j.src = src;
+    return j;
+}
+
+// The device work of this proof's current phase.
+void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& items, size_t& used) {
+    const GeometryTables& gt = *P.gt;
+    switch (P.phase) {
+        case PH_FRONT: {
+            if (P.air) {
+                ch.push_back(FibJob{P.col, P.n, P.a0, P.b0, 0});
+                NttPlan ip = plan_intt(P.col, P.coeff, P.scratch, P.logn, c.tw4096_inv, gt.winv, P.logb,
+                                       to_mont(inv_c(uint32_t(P.n % P_MOD))));
+                for (auto& j : ip.pass1) ch.push_back(j);
+                for (auto& j : ip.pass2) ch.push_back(j);
+                NttPlan lp = plan_lde(P.coeff, P.A, P.scratch, P.logn, P.logb, c.tw4096, gt.w, gt.coset);
+                for (auto& j : lp.pass1) ch.push_back(j);
+                for (auto& j : lp.pass2) ch.push_back(j);
+                LinkedJob lj{};
+                lj.a = P.A;
+                lj.b = P.Bc;
+                lj.w = gt.w;
+                lj.shift_m = gt.coset.shift;
+                lj.zvan_m = gt.zvan;
+                lj.coeff = P.coeff_c;
+                lj.w_last_m = to_mont(P.w_last);
+                lj.logn = P.logn;
+                lj.logb = P.logb;
+                ch.push_back(lj);
+                LeavesJob t = leaves_of(LEAF_TRACE, P.A);
+                t.src_b = P.Bc;
+                t.logn = P.logn;
+                t.logb = P.logb;
+                tree_steps(ch, P.trace_tree, P.tg, t);
+                collect_cap(c, P, P.trace_tree, P.tg, items, used);
+            } else {
+                ch.push_back(BitrevJob{P.col, P.coeff, P.logn, 0});
+                NttPlan lp = plan_lde(P.coeff, P.vals[0], P.scratch, P.logn, P.logb, c.tw4096, gt.w, gt.coset);
+                for (auto& j : lp.pass1) ch.push_back(j);
+                for (auto& j : lp.pass2) ch.push_back(j);
+                tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_BASE, P.vals[0]));
+                collect_cap(c, P, P.trees[0], P.geoms[0], items, used);
+            }
+            break;
+        }
+        case PH_COMPOSE: {
+            ComposeJob cj{};
+            cj.a = P.A;
+            cj.b = P.Bc;
+            cj.layer0 = P.vals[0];
+            cj.w = gt.w;
+            cj.shift_m = gt.coset.shift;
+            cj.zinv_m = gt.zinv;
+            cj.c = P.cc;
+            cj.logn = P.logn;
+            cj.logb = P.logb;
+            ch.push_back(cj);
+            tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_EXT, P.vals[0]));
+            collect_cap(c, P, P.trees[0], P.geoms[0], items, used);
+            break;
+        }
+        case PH_FOLD: {
+            const size_t d = P.depth;
+            FoldJob fj{};
+            fj.layer = P.vals[d];
+            fj.next = P.vals[d + 1];
+            fj.winv = gt.winv;
+            fj.c = P.fold;
+            fj.base_field = d == 0 && !P.air ? 1 : 0;
+            ch.push_back(fj);
+            if (d + 1 == P.folds) {
+                const uint32_t words = uint32_t(P.lens[d + 1] * 4);
+                uint32_t* dst = stage_take(c, used, words, P.stage_off);
+                items.push_back(CollectItem{P.vals[d + 1], dst, words, 0});
+            } else {
+                tree_steps(ch, P.trees[d + 1], P.geoms[d + 1], leaves_of(LEAF_EXT, P.vals[d + 1]));
+                collect_cap(c, P, P.trees[d + 1], P.geoms[d + 1], items, used);
+            }
+            break;
+        }
+        case PH_GRIND: {
+            GrindJob g{};
+            std::memcpy(g.hw, P.pow_hw, sizeof(g.hw));
+            g.bits = P.fp.grinding_bits;
+            g.nblocks = 2048;
+            g.limit = uint64_t(1) << std::min<uint64_t>(63, 24 + 4 * uint64_t(P.window));
+            g.result = P.grind_res;
+            ch.push_back(g);
+            uint32_t* dst = stage_take(c, used, 2, P.stage_off);
+            items.push_back(CollectItem{reinterpret_cast<const uint32_t*>(P.grind_res), dst, 2, 0});
+            break;
+        }
+        case PH_OPEN: {
+            OpenJob o{};
+            for (size_t d = 0; d < P.folds; ++d) {
+                o.layer[d] = P.vals[d];
+                o.tree[d] = P.trees[d];
+                o.tree_leaves[d] = P.geoms[d].leaves;
+                o.target[d] = P.geoms[d].target;
+                o.base[d] = d == 0 && !P.air ? 1 : 0;
+            }
+            o.nlayers = int(P.folds);
+            if (P.air) {
+                o.a = P.A;
+                o.b = P.Bc;
+                o.trace_tree = P.trace_tree;
+                o.trace_leaves = P.tg.leaves;
+                o.trace_target = P.tg.target;
+            }
+            o.logn = P.logn;
+            o.logb = P.logb;
+            o.queries = static_cast<const uint64_t*>(c.ring.push(P.queries.data(), P.queries.size() * 8));
+            o.nq = uint32_t(P.queries.size());
+            o.words_per_query = P.wpq;
+            o.out = stage_take(c, used, size_t(P.wpq) * P.queries.size(), P.stage_off);
+            ch.push_back(o);
+            break;
+        }
+        case PH_DONE:
+            break;
+    }
+}
+
+// The layer just committed is layer d: absorb its cap, draw the three fold
+// challenges of fold d (fri.rs:594-612).
+void commit_layer(Proof& P, const uint32_t* w, size_t d) {
+    const TreeGeom& g = P.geoms[d];
+    auto cap = read_digests(w, uint64_t(1) << g.drop);
+    for (auto& dg : cap) P.tr->absorb("fri-layer-root", dg.data(), 32);
+    P.caps.push_back(std::move(cap));
+    FoldConsts fc{};
+    for (int k = 0; k < 3; ++k) {
+        E4 ch = P.tr->challenge_ext("fri-fold");
+        for (int q = 0; q < 4; ++q) fc.ch_m[k][q] = to_mont(ch.v[q]);
+    }
+    fc.unit_m = to_mont(root_of_unity(2));
+    fc.ginv_m = to_mont(pow_c(inv_c(GENERATOR), uint64_t(1) << (2 * d)));
+    fc.log_len = int(ilog2(P.lens[d]));
+    fc.depth = int(d);
+    P.fold = fc;
+    P.depth = d;
+    P.phase = PH_FOLD;
+}
+
+void sample_queries(Proof& P) {
+    P.queries.clear();
+    for (uint64_t q = 0; q < P.fp.num_queries; ++q) P.queries.push_back(P.tr->challenge_index("fri-query", P.m));
+    P.phase = PH_OPEN;
+}
+
+void finish_final(Proof& P, const uint32_t* w) {
+    const uint64_t len = P.lens[P.folds];
+    P.final_values.resize(len);
+    for (uint64_t i = 0; i < len; ++i) P.final_values[i] = E4{{w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]}};
+    auto cap = host_layer_cap(P.final_values, P.fp.cap_height);
+    for (auto& d : cap) P.tr->absorb("fri-layer-root", d.data(), 32);
+    P.caps.push_back(std::move(cap));
+    std::vector<uint8_t> fin(len * 16);
+    for (uint64_t i = 0; i < len; ++i)
+        for (int c = 0; c < 4; ++c) store_le32(fin.data() + 16 * i + 4 * c, P.final_values[i].v[c]);
+    P.tr->absorb("fri-final", fin);
+    if (P.fp.grinding_bits > 0) {
+        // Proof-of-work (transcript.rs:206-228) on the device.
+        std::vector<uint8_t> head = P.tr->pow_head("fri-pow");
+        require(head.size() == 47, Z0P_ERR_DEVICE);
+        std::memset(P.pow_hw, 0, sizeof(P.pow_hw));
+        for (size_t i = 0; i < head.size(); ++i) P.pow_hw[i >> 2] |= uint32_t(head[i]) << (8 * (i & 3));
+        P.window = 0;
+        P.phase = PH_GRIND;
+    } else {
+        sample_queries(P);
     }
 }
 
 // fri::Commitment::write_body (fri.rs:896-904).
-void write_fri_commitment_body(Writer& w, const FriResult& R, uint64_t m, uint64_t poly_len, uint64_t degree_bound) {
-    w.u64(m);
-    w.u64(poly_len);
-    w.u64(degree_bound);
-    w.u32(uint32_t(R.caps.size()));
-    for (auto& c : R.caps) write_cap(w, c);
+void write_fri_commitment_body(Writer& w, const Proof& P) {
+    w.u64(P.m);
+    w.u64(P.poly_len);
+    w.u64(P.degree_bound);
+    w.u32(uint32_t(P.caps.size()));
+    for (auto& c : P.caps) write_cap(w, c);
 }
 
-// fri::Proof::write_body (fri.rs:352-368).
-void write_fri_proof_body(Writer& w, const FriResult& R, const FriOpenPlan& P, const std::vector<uint32_t>& words) {
-    w.u32(uint32_t(R.final_values.size()));
-    for (auto& v : R.final_values) w.e4(v);
-    w.u64(R.nonce);
-    w.u32(uint32_t(R.queries.size()));
-    for (size_t q = 0; q < R.queries.size(); ++q) {
-        w.u32(uint32_t(R.layers.size()));
-        uint64_t index = R.queries[q];
-        for (size_t d = 0; d < R.layers.size(); ++d) {
-            const uint32_t* v = &words[P.values_at[q][d]];
-            if (R.layers[d].base) {
-                for (int k = 0; k < 4; ++k) w.e4(e4_base(v[k]));
+// fri::Proof::write_body (fri.rs:352-368) from the opened words.
+void write_fri_proof_body(Writer& w, const Proof& P, const uint32_t* words) {
+    w.u32(uint32_t(P.final_values.size()));
+    for (auto& v : P.final_values) w.e4(v);
+    w.u64(P.nonce);
+    w.u32(uint32_t(P.queries.size()));
+    for (size_t q = 0; q < P.queries.size(); ++q) {
+        const uint32_t* p = words + q * P.wpq;
+        w.u32(uint32_t(P.folds));
+        uint64_t index = P.queries[q];
+        for (size_t d = 0; d < P.folds; ++d) {
+            if (d == 0 && !P.air) {
+                for (int k = 0; k < 4; ++k) w.e4(e4_base(p[k]));
+                p += 4;
             } else {
-                for (int k = 0; k < 16; ++k) w.u32(v[k]);
+                for (int k = 0; k < 16; ++k) w.u32(p[k]);
+                p += 16;
             }
-            write_path(w, P.path_at[q][d], words, index / 4);
+            p = write_path(w, p, P.geoms[d].target, index / 4);
             index /= 4;
         }
     }
 }
 
-}  // namespace
-
-
-// ---------------------------------------------------------------------------
-// air::prove (air.rs:365-372 -> 408-422 -> 485-599), linked path.
-std::vector<uint8_t> air_prove(Ctx& ctx, const z0p_air_params& p, Transcript& tr) {
-    validate_air(p);
-    const uint64_t n = p.n_steps, B = p.blowup, m = n * B;
-    const int logn = int(ilog2(n)), logb = int(ilog2(B)), logm = logn + logb;
-    const uint32_t a0 = p.a0 % P, b0 = p.b0 % P, b_last = p.b_last % P;
-    const uint32_t actual_b_last = fib_b_last(a0, b0, n);
-
-    // absorb_params (air.rs:754-766): 56 bytes, grinding_bits as u32.
-    uint8_t pb[56];
-    store_le64(pb, p.n_steps);
-    store_le64(pb + 8, p.blowup);
-    store_le64(pb + 16, p.num_queries);
-    store_le64(pb + 24, p.final_size);
-    store_le32(pb + 32, p.grinding_bits);
-    store_le64(pb + 36, p.cap_height);
-    store_le32(pb + 44, a0);
-    store_le32(pb + 48, b0);
-    store_le32(pb + 52, b_last);
-    tr.absorb("fib/params", pb, 56);
-
-    const z0p_fri_params fp = air_fri_params(p);
-    validate_fri(fp);
-    fri_num_folds(m, fp.final_size, n - 1);
-
-    const GeometryTables& gt = ctx.geometry(logn, logb);
-    const TreeGeom tg = tree_geom(m / 8, p.cap_height);
-    const uint64_t words = 2 * n + std::max(n, m) + 2 * m + tg.words() + 4 * m +
-                           fri_arena_words(m, false, fp.final_size, fp.cap_height) + (1u << 16);
-    ctx.reserve_arena(words * 4 + (size_t(p.num_queries) * 4096 * 16) + (16u << 20));
-    ctx.reset_arena();
-    Launch L = ctx.launch();
-
-    uint32_t* col = ctx.alloc<uint32_t>(n);
-    uint32_t* coeff = ctx.alloc<uint32_t>(n);
-    uint32_t* scratch = ctx.alloc<uint32_t>(std::max(n, m));
-    uint32_t* A = ctx.alloc<uint32_t>(m);
-    uint32_t* Bc = ctx.alloc<uint32_t>(m);
-    uint32_t* trace_tree = ctx.alloc<uint32_t>(tg.words());
-    uint32_t* consts = ctx.alloc<uint32_t>(B);
-
-    // Linked LDE constant (air.rs:815-820): coeff = (b_last - a0) w_last / n,
-    // times (x^n - 1) = (s_j^n - 1) per coset.
-    const uint32_t w_n = root_of_unity(uint32_t(logn));
-    const uint32_t w_last = pow_c(w_n, n - 1);
-    const uint32_t coeff_c = mul(mul(sub(actual_b_last, a0), w_last), inv_c(uint32_t(n % P)));
-    {
-        const uint32_t wm = root_of_unity(uint32_t(logm));
-        uint32_t* hc = reinterpret_cast<uint32_t*>(ctx.pinned);
-        for (uint64_t j = 0; j < B; ++j) {
-            uint32_t s = mul(GENERATOR, pow_c(wm, j));
-            hc[j] = mul(coeff_c, sub(pow_c(s, n), 1));
+// Proof::to_bytes (air.rs:262-281) / the FRI commitment and proof.
+void serialize(Proof& P, const uint32_t* words) {
+    if (P.air) {
+        Writer w;
+        w.byte(WIRE_VERSION);
+        write_cap(w, P.column_cap);
+        write_fri_commitment_body(w, P);
+        write_fri_proof_body(w, P, words);
+        w.u32(uint32_t(P.queries.size()));
+        const uint64_t B = P.B, m = P.m;
+        const int logm = P.logn + P.logb;
+        for (size_t q = 0; q < P.queries.size(); ++q) {
+            const uint32_t* lw = words + q * P.wpq + P.fri_wpq;
+            const uint32_t* sw = lw + 16;
+            const uint32_t* paths = lw + 32;
+            const uint64_t index = brev_host(P.queries[q], logm);
+            const uint64_t shifted = (index + B) % m;
+            w.u64(index);
+            w.u32(lw[2 * (index & 7)]);
+            w.u32(lw[2 * (index & 7) + 1]);
+            w.u32(sw[2 * (shifted & 7)]);
+            w.u32(sw[2 * (shifted & 7) + 1]);
+            paths = write_path(w, paths, P.tg.target, index / 8);
+            write_path(w, paths, P.tg.target, shifted / 8);
+            for (int k = 0; k < 16; ++k) w.u32(lw[k]);
+            for (int k = 0; k < 16; ++k) w.u32(sw[k]);
         }
-        hip_check(hipMemcpyAsync(consts, hc, B * 4, hipMemcpyHostToDevice, ctx.stream));
+        P.proof = std::move(w.out);
+    } else {
+        Writer wc;
+        wc.byte(WIRE_VERSION);
+        write_fri_commitment_body(wc, P);
+        P.commitment = std::move(wc.out);
+        Writer wp;
+        wp.byte(WIRE_VERSION);
+        write_fri_proof_body(wp, P, words);
+        P.proof = std::move(wp.out);
     }
-
-    launch_fib_trace(L, col, n, a0, b0);
-    launch_intt_to_br(L, col, coeff, scratch, logn, ctx.tw4096_inv, gt.winv, logb, to_mont(inv_c(uint32_t(n % P))));
-    launch_lde_br(L, coeff, A, scratch, logn, logb, ctx.tw4096, gt.w, gt.coset);
-    launch_linked_b(L, A, Bc, logn, logb, gt.w, gt.coset.shift, consts, to_mont(w_last));
-    int have = launch_trace_leaves(L, A, Bc, trace_tree, logn, logb, tg.target);
-    build_levels(ctx, trace_tree, tg, have);
-
-    const uint64_t cap_n = uint64_t(1) << tg.drop;
-    hip_check(hipMemcpyAsync(ctx.pinned + 4096, trace_tree + tg.level_offset_words(tg.target), cap_n * 32,
-                             hipMemcpyDeviceToHost, ctx.stream));
-    ctx.sync();
-    std::vector<Digest> column_cap(cap_n);
-    for (uint64_t i = 0; i < cap_n; ++i) std::memcpy(column_cap[i].data(), ctx.pinned + 4096 + 32 * i, 32);
-    {
-        std::vector<uint8_t> cb(cap_n * 32);
-        for (uint64_t i = 0; i < cap_n; ++i) std::memcpy(cb.data() + 32 * i, column_cap[i].data(), 32);
-        tr.absorb("fib/trace", cb);
-    }
-    ComposeConsts cc{};
-    for (int k = 0; k < 5; ++k) {
-        E4 al = tr.challenge_ext("fib/alpha");
-        for (int c = 0; c < 4; ++c) cc.alpha_m[k][c] = to_mont(al.v[c]);
-    }
-    cc.w_last_m = to_mont(w_last);
-    cc.a0 = a0;
-    cc.b0 = b0;
-    cc.b_last = b_last;
-    uint32_t* layer0 = ctx.alloc<uint32_t>(4 * m);
-    launch_compose(L, A, Bc, layer0, logn, logb, gt.w, gt.coset.shift, gt.zinv, cc);
-
-    FriResult R = fri_commit_and_sample(ctx, layer0, false, m, m, n - 1, fp, gt.winv, tr);
-
-    // Openings: FRI quads/paths plus the trace rows (air.rs:567-591).
-    Gather g;
-    FriOpenPlan P = plan_fri_openings(ctx, R, g);
-    struct TraceOpen {
-        uint64_t index, shifted;
-        uint32_t leaf_at, shift_leaf_at;
-        std::vector<uint32_t> path, shift_path;
-    };
-    std::vector<TraceOpen> tops;
-    auto row_pos = [&](uint64_t i) -> uint64_t {
-        uint64_t j = i & (B - 1), t = i >> logb;
-        return (uint64_t(brev_host(j, logb)) << logn) | brev_host(t, logn);
-    };
-    auto add_leaf = [&](uint64_t base) {
-        uint32_t first = 0;
-        for (int r = 0; r < 8; ++r) {
-            uint64_t pos = row_pos(base + r);
-            uint32_t at = g.add(ctx, A + pos, 1);
-            g.add(ctx, Bc + pos, 1);
-            if (r == 0) first = at;
-        }
-        return first;
-    };
-    auto add_path = [&](uint64_t leaf) {
-        std::vector<uint32_t> at;
-        for (int l = 0; l < tg.target; ++l)
-            at.push_back(g.add(ctx, trace_tree + tg.level_offset_words(l) + ((leaf >> l) ^ 1) * 8, 8));
-        return at;
-    };
-    for (uint64_t leaf : R.queries) {
-        TraceOpen o;
-        o.index = brev_host(leaf, logm);
-        o.shifted = (o.index + B) % m;
-        uint64_t base = o.index & ~uint64_t(7), sbase = o.shifted & ~uint64_t(7);
-        o.leaf_at = add_leaf(base);
-        o.shift_leaf_at = add_leaf(sbase);
-        o.path = add_path(base / 8);
-        o.shift_path = add_path(sbase / 8);
-        tops.push_back(std::move(o));
-    }
-    std::vector<uint32_t> words_out;
-    run_gather(ctx, g, words_out);
-
-    // Proof::to_bytes (air.rs:262-281).
-    Writer w;
-    w.byte(WIRE_VERSION);
-    write_cap(w, column_cap);
-    write_fri_commitment_body(w, R, m, m, n - 1);
-    write_fri_proof_body(w, R, P, words_out);
-    w.u32(uint32_t(tops.size()));
-    for (const TraceOpen& o : tops) {
-        const uint32_t* lw = &words_out[o.leaf_at];
-        const uint32_t* sw = &words_out[o.shift_leaf_at];
-        w.u64(o.index);
-        w.u32(lw[2 * (o.index & 7)]);
-        w.u32(lw[2 * (o.index & 7) + 1]);
-        w.u32(sw[2 * (o.shifted & 7)]);
-        w.u32(sw[2 * (o.shifted & 7) + 1]);
-        write_path(w, o.path, words_out, o.index / 8);
-        write_path(w, o.shift_path, words_out, o.shifted / 8);
-        for (int k = 0; k < 16; ++k) w.u32(lw[k]);
-        for (int k = 0; k < 16; ++k) w.u32(sw[k]);
-    }
-    return std::move(w.out);
 }
 
-// fri::commit + public_commitment + open (fri.rs:408-433, 863-870, 645-666).
+// Host transcript step after the iteration's device work completed.
+void host_step(Ctx& c, Proof& P) {
+    const uint32_t* w = c.stage_host + P.stage_off;
+    switch (P.phase) {
+        case PH_FRONT:
+            if (P.air) {
+                P.column_cap = read_digests(w, uint64_t(1) << P.tg.drop);
+                std::vector<uint8_t> cb(P.column_cap.size() * 32);
+                for (size_t i = 0; i < P.column_cap.size(); ++i) std::memcpy(cb.data() + 32 * i, P.column_cap[i].data(), 32);
+                P.tr->absorb("fib/trace", cb);
+                for (int k = 0; k < 5; ++k) {
+                    E4 al = P.tr->challenge_ext("fib/alpha");
+                    for (int q = 0; q < 4; ++q) P.cc.alpha_m[k][q] = to_mont(al.v[q]);
+                }
+                P.cc.w_last_m = to_mont(P.w_last);
+                P.cc.a0 = P.a0;
+                P.cc.b0 = P.b0;
+                P.cc.b_last = P.b_last;
+                uint8_t dims[24];
+                store_le64(dims, P.m);
+                store_le64(dims + 8, P.poly_len);
+                store_le64(dims + 16, P.degree_bound);
+                P.tr->absorb("fri-dims", dims, 24);
+                P.phase = PH_COMPOSE;
+            } else {
+                commit_layer(P, w, 0);
+            }
+            break;
+        case PH_COMPOSE:
+            commit_layer(P, w, 0);
+            break;
+        case PH_FOLD:
+            if (P.depth + 1 == P.folds) finish_final(P, w);
+            else commit_layer(P, w, P.depth + 1);
+            break;
+        case PH_GRIND: {
+            const uint64_t found = uint64_t(w[0]) | (uint64_t(w[1]) << 32);
+            if (found != ~uint64_t(0)) {
+                P.nonce = found;
+                uint8_t nb[8];
+                store_le64(nb, found);
+                P.tr->absorb("fri-pow", nb, 8);
+                sample_queries(P);
+            } else {
+                // Each window searches [0, limit); later windows only widen
+                // the bound, and the kernel returns the smallest passing nonce.
+                ++P.window;
+            }
+            break;
+        }
+        case PH_OPEN:
+            serialize(P, w);
+            P.phase = PH_DONE;
+            break;
+        case PH_DONE:
+            break;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Scheduler.
+
+Engine::Engine(Ctx& c, size_t budget, size_t max_active) : c_(c), budget_(budget), max_active_(max_active) {}
+
+Engine::~Engine() {
+    for (Proof* p : active_)
+        if (p->mem) c_.pool.release(p->mem, p->mem_bytes);
+}
+
+void Engine::submit(Proof* p) {
+    p->t_submit = now_ms();
+    queue_.push_back(p);
+}
+
+void Engine::finish(Proof* p, int status) {
+    if (p->mem) c_.pool.release(p->mem, p->mem_bytes);
+    p->mem = nullptr;
+    p->status = status;
+    p->phase = PH_DONE;
+    p->t_done = now_ms();
+    if (on_done_) on_done_(p);
+}
+
+void Engine::admit() {
+    while (!queue_.empty() && active_.size() < max_active_) {
+        Proof* p = queue_.front();
+        uint8_t* mem = nullptr;
+        if (!c_.pool.alloc(p->mem_bytes, mem)) {
+            if (c_.pool.in_use() != 0) break;  // wait for memory to free up
+            // Idle pool: grow it to fit (bounded by the budget, if any).
+            size_t want = budget_ ? std::max(budget_, p->mem_bytes) : p->mem_bytes;
+            if (budget_ && p->mem_bytes > budget_) {
+                queue_.pop_front();
+                finish(p, Z0P_ERR_OUT_OF_MEMORY);
+                continue;
+            }
+            c_.sync();
+            c_.pool.reset_capacity(want);
+            if (!c_.pool.alloc(p->mem_bytes, mem)) {
+                queue_.pop_front();
+                finish(p, Z0P_ERR_OUT_OF_MEMORY);
+                continue;
+            }
+        }
+        queue_.pop_front();
+        p->mem = mem;
+        p->t_admit = now_ms();
+        stage_need_ += p->stage_need;
+        c_.reserve_stage(stage_need_);
+        start(c_, *p);
+        active_.push_back(p);
+    }
+}
+
+bool Engine::iterate() {
+    admit();
+    if (active_.empty()) return !queue_.empty();
+    std::vector<std::vector<Step>> chains(active_.size());
+    std::vector<CollectItem> items;
+    size_t used = 0;
+    for (size_t i = 0; i < active_.size(); ++i) build(c_, *active_[i], chains[i], items, used);
+    std::vector<const std::vector<Step>*> ptrs;
+    for (auto& ch : chains) ptrs.push_back(&ch);
+    c_.enqueue([&](const Launch& L) { launch_chains(L, ptrs, items); });
+    if (used)
+        hip_check(hipMemcpyAsync(c_.stage_host, c_.stage_dev, used * 4, hipMemcpyDeviceToHost, c_.stream));
+    c_.sync();
+    ++iterations_;
+    std::vector<Proof*> still;
+    for (Proof* p : active_) {
+        try {
+            host_step(c_, *p);
+        } catch (const Status& s) {
+            stage_need_ -= p->stage_need;
+            finish(p, s.code);
+            continue;
+        }
+        if (p->phase == PH_DONE) {
+            stage_need_ -= p->stage_need;
+            finish(p, Z0P_OK);
+        } else {
+            still.push_back(p);
+        }
+    }
+    active_.swap(still);
+    return !active_.empty() || !queue_.empty();
+}
+
+void Engine::run() {
+    try {
+        while (iterate()) {
+        }
+    } catch (const Status& s) {
+        // Device failure: every unfinished proof fails with it.
+        for (Proof* p : active_) finish(p, s.code);
+        active_.clear();
+        stage_need_ = 0;
+        while (!queue_.empty()) {
+            finish(queue_.front(), s.code);
+            queue_.pop_front();
+        }
+        throw;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Entry points.
+
+Proof* proof_new_air(Ctx& c, const z0p_air_params& p, Transcript* tr) {
+    auto P = std::make_unique<Proof>();
+    init_air(c, *P, p, tr);
+    return P.release();
+}
+
+Proof* proof_new_fri(Ctx& c, const uint32_t* coeffs, uint64_t n, const z0p_fri_params& fp, Transcript* tr) {
+    auto P = std::make_unique<Proof>();
+    init_fri(c, *P, coeffs, n, fp, tr);
+    return P.release();
+}
+
+void proof_free(Proof* p) { delete p; }
+int proof_status(const Proof* p) { return p->status; }
+std::vector<uint8_t>& proof_bytes(Proof* p) { return p->proof; }
+std::vector<uint8_t>& proof_commitment(Proof* p) { return p->commitment; }
+size_t proof_mem_bytes(const Proof* p) { return p->mem_bytes; }
+void proof_times(const Proof* p, double& submit, double& admit, double& done) {
+    submit = p->t_submit;
+    admit = p->t_admit;
+    done = p->t_done;
+}
+void proof_set_tag(Proof* p, uint64_t tag) { p->tag = tag; }
+uint64_t proof_tag(const Proof* p) { return p->tag; }
+
+std::vector<uint8_t> air_prove(Ctx& ctx, const z0p_air_params& p, Transcript& tr) {
+    std::unique_ptr<Proof> P(proof_new_air(ctx, p, &tr));
+    Engine e(ctx, 0, 1);
+    e.submit(P.get());
+    e.run();
+    if (P->status != Z0P_OK) throw Status(P->status);
+    return std::move(P->proof);
+}
+
 void fri_commit_open(Ctx& ctx, const uint32_t* coeffs, uint64_t n, const z0p_fri_params& fp, Transcript& tr,
                      std::vector<uint8_t>& commitment, std::vector<uint8_t>& proof) {
-    validate_fri(fp);
-    require(n != 0, Z0P_ERR_EMPTY_INPUT);
-    require(pow2(n) && n <= MAX_DOMAIN, Z0P_ERR_INVALID_DOMAIN_SIZE);
-    require(n <= MAX_DOMAIN / fp.blowup, Z0P_ERR_INVALID_DOMAIN_SIZE);
-    const uint64_t B = fp.blowup, m = n * B;
-    const int logn = int(ilog2(n)), logb = int(ilog2(B));
-    fri_num_folds(m, fp.final_size, n - 1);
-    const GeometryTables& gt = ctx.geometry(logn, logb);
-    const uint64_t words = 2 * n + m + m + fri_arena_words(m, true, fp.final_size, fp.cap_height) + (1u << 16);
-    ctx.reserve_arena(words * 4 + (size_t(fp.num_queries) * 4096 * 16) + (16u << 20));
-    ctx.reset_arena();
-    Launch L = ctx.launch();
-    uint32_t* raw = ctx.alloc<uint32_t>(n);
-    uint32_t* coeff_br = ctx.alloc<uint32_t>(n);
-    uint32_t* scratch = ctx.alloc<uint32_t>(m);
-    uint32_t* layer0 = ctx.alloc<uint32_t>(m);
-    // Coefficients are reduced mod p on entry (Field::new semantics).
-    ctx.reserve_pinned(size_t(n) * 4 + 4096);
-    uint32_t* hc = reinterpret_cast<uint32_t*>(ctx.pinned);
-    for (uint64_t i = 0; i < n; ++i) hc[i] = coeffs[i] % P;
-    hip_check(hipMemcpyAsync(raw, hc, n * 4, hipMemcpyHostToDevice, ctx.stream));
-    launch_bitrev(L, raw, coeff_br, logn);
-    launch_lde_br(L, coeff_br, layer0, scratch, logn, logb, ctx.tw4096, gt.w, gt.coset);
-    ctx.sync();  // pinned staging is reused below
-
-    FriResult R = fri_commit_and_sample(ctx, layer0, true, m, n, n - 1, fp, gt.winv, tr);
-    Gather g;
-    FriOpenPlan Pl = plan_fri_openings(ctx, R, g);
-    std::vector<uint32_t> words_out;
-    run_gather(ctx, g, words_out);
-
-    Writer wc;
-    wc.byte(WIRE_VERSION);
-    write_fri_commitment_body(wc, R, m, n, n - 1);
-    commitment = std::move(wc.out);
-    Writer wp;
-    wp.byte(WIRE_VERSION);
-    write_fri_proof_body(wp, R, Pl, words_out);
-    proof = std::move(wp.out);
+    std::unique_ptr<Proof> P(proof_new_fri(ctx, coeffs, n, fp, &tr));
+    Engine e(ctx, 0, 1);
+    e.submit(P.get());
+    e.run();
+    if (P->status != Z0P_OK) throw Status(P->status);
+    commitment = std::move(P->commitment);
+    proof = std::move(P->proof);
 }
 
 }  // namespace z0p

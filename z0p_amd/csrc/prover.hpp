@@ -1,6 +1,9 @@
 // Prover entry points behind the C ABI.
 #pragma once
 #include <cstdint>
+#include <deque>
+#include <functional>
+#include <variant>
 #include <vector>
 
 #include "context.hpp"
@@ -29,10 +32,63 @@ struct TreeGeom {
     }
 };
 TreeGeom tree_geom(uint64_t leaves, uint64_t cap_height);
-void build_levels(Ctx& ctx, uint32_t* tree, const TreeGeom& g, int have);
+
+// One kernel step of one proof; a proof's steps of one iteration form a
+// chain that must run in order.
+using Step = std::variant<FibJob, BitrevJob, NttJob, LinkedJob, ComposeJob, LeavesJob, LevelsJob, TailJob, FoldJob,
+                          GrindJob, OpenJob>;
+
+// Leaves, then wide levels, then LDS tail launches up to the cap level.
+void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves);
+
+// Launches step r of every chain for r = 0, 1, ... (one launch per kernel
+// type per r), then the collect items.
+void launch_chains(const Launch& L, const std::vector<const std::vector<Step>*>& chains,
+                   const std::vector<CollectItem>& collect);
 
 // air::build_trace(a0, b0, n).1[n-1] by fast doubling (air.rs:341-354).
 uint32_t fib_b_last(uint32_t a0, uint32_t b0, uint64_t n);
+
+// One proof request and its state (defined in prover.cpp).
+struct Proof;
+Proof* proof_new_air(Ctx& c, const z0p_air_params& p, Transcript* tr);
+Proof* proof_new_fri(Ctx& c, const uint32_t* coeffs, uint64_t n, const z0p_fri_params& fp, Transcript* tr);
+void proof_free(Proof* p);
+int proof_status(const Proof* p);
+std::vector<uint8_t>& proof_bytes(Proof* p);
+std::vector<uint8_t>& proof_commitment(Proof* p);
+size_t proof_mem_bytes(const Proof* p);
+void proof_times(const Proof* p, double& submit_ms, double& admit_ms, double& done_ms);
+void proof_set_tag(Proof* p, uint64_t tag);
+uint64_t proof_tag(const Proof* p);
+
+// The scheduler: admits queued proofs while pool memory allows (budget 0:
+// one proof at a time, the pool grown to fit it) and advances all admitted
+// proofs by one transcript phase per iteration.
+class Engine {
+   public:
+    Engine(Ctx& c, size_t budget, size_t max_active);
+    ~Engine();
+    void submit(Proof* p);
+    // One iteration; false once nothing is queued or active.
+    bool iterate();
+    void run();
+    void on_done(std::function<void(Proof*)> f) { on_done_ = std::move(f); }
+    size_t active() const { return active_.size(); }
+    size_t queued() const { return queue_.size(); }
+    uint64_t iterations() const { return iterations_; }
+
+   private:
+    void admit();
+    void finish(Proof* p, int status);
+    Ctx& c_;
+    size_t budget_, max_active_;
+    std::deque<Proof*> queue_;
+    std::vector<Proof*> active_;
+    size_t stage_need_ = 0;
+    uint64_t iterations_ = 0;
+    std::function<void(Proof*)> on_done_;
+};
 
 std::vector<uint8_t> air_prove(Ctx& ctx, const z0p_air_params& p, Transcript& tr);
 void fri_commit_open(Ctx& ctx, const uint32_t* coeffs, uint64_t n, const z0p_fri_params& fp, Transcript& tr,

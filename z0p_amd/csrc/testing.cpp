@@ -27,7 +27,7 @@ int guarded(F&& f) {
 
 template <typename T>
 T* to_dev(Ctx& c, const T* host, size_t count) {
-    T* d = c.alloc<T>(count);
+    T* d = c.scratch_alloc<T>(count);
     hip_check(hipMemcpyAsync(d, host, count * sizeof(T), hipMemcpyHostToDevice, c.stream));
     return d;
 }
@@ -35,6 +35,12 @@ template <typename T>
 void to_host(Ctx& c, T* host, const T* dev, size_t count) {
     hip_check(hipMemcpyAsync(host, dev, count * sizeof(T), hipMemcpyDeviceToHost, c.stream));
     c.sync();
+}
+
+// Runs one chain of steps (a single job per launch: no tables).
+void run_chain(Ctx& c, const std::vector<Step>& chain) {
+    std::vector<const std::vector<Step>*> chains{&chain};
+    c.enqueue([&](const Launch& L) { launch_chains(L, chains, {}); });
 }
 
 // natural index i = j + B t  <->  position brev(j) n + brev(t)
@@ -61,13 +67,14 @@ int z0p_test_intt(z0p_ctx* ctx, const uint32_t* evals, size_t n, uint32_t* coeff
         Ctx& c = *ctx->c;
         int logn = int(ilog2(n));
         const GeometryTables& gt = c.geometry(logn, 4);
-        c.reserve_arena(n * 12 + (1 << 20));
-        c.reset_arena();
+        c.reserve_scratch(n * 12 + (1 << 20));
         uint32_t* din = to_dev(c, evals, n);
-        uint32_t* dout = c.alloc<uint32_t>(n);
-        uint32_t* scratch = c.alloc<uint32_t>(n);
-        launch_intt_to_br(c.launch(), din, dout, scratch, logn, c.tw4096_inv, gt.winv, 4,
-                          to_mont(inv_c(uint32_t(n % P))));
+        uint32_t* dout = c.scratch_alloc<uint32_t>(n);
+        uint32_t* scratch = c.scratch_alloc<uint32_t>(n);
+        NttPlan plan = plan_intt(din, dout, scratch, logn, c.tw4096_inv, gt.winv, 4, to_mont(inv_c(uint32_t(n % P))));
+        std::vector<Step> chain(plan.pass1.begin(), plan.pass1.end());
+        chain.insert(chain.end(), plan.pass2.begin(), plan.pass2.end());
+        run_chain(c, chain);
         to_host(c, coeffs_br, dout, n);
     });
 }
@@ -80,14 +87,16 @@ int z0p_test_lde(z0p_ctx* ctx, const uint32_t* coeffs, size_t n, size_t blowup, 
         int logn = int(ilog2(n)), logb = int(ilog2(blowup));
         size_t m = n * blowup;
         const GeometryTables& gt = c.geometry(logn, logb);
-        c.reserve_arena(n * 8 + m * 8 + (1 << 20));
-        c.reset_arena();
+        c.reserve_scratch(n * 8 + m * 8 + (1 << 20));
         uint32_t* din = to_dev(c, coeffs, n);
-        uint32_t* dbr = c.alloc<uint32_t>(n);
-        uint32_t* scratch = c.alloc<uint32_t>(m);
-        uint32_t* dout = c.alloc<uint32_t>(m);
-        launch_bitrev(c.launch(), din, dbr, logn);
-        launch_lde_br(c.launch(), dbr, dout, scratch, logn, logb, c.tw4096, gt.w, gt.coset);
+        uint32_t* dbr = c.scratch_alloc<uint32_t>(n);
+        uint32_t* scratch = c.scratch_alloc<uint32_t>(m);
+        uint32_t* dout = c.scratch_alloc<uint32_t>(m);
+        NttPlan plan = plan_lde(dbr, dout, scratch, logn, logb, c.tw4096, gt.w, gt.coset);
+        std::vector<Step> chain{BitrevJob{din, dbr, logn, 0}};
+        chain.insert(chain.end(), plan.pass1.begin(), plan.pass1.end());
+        chain.insert(chain.end(), plan.pass2.begin(), plan.pass2.end());
+        run_chain(c, chain);
         to_host(c, lde_br, dout, m);
     });
 }
@@ -98,13 +107,15 @@ int z0p_test_merkle64(z0p_ctx* ctx, const uint8_t* flat, size_t leaves, uint8_t*
         require(pow2(leaves), Z0P_ERR_INVALID_DOMAIN_SIZE);
         Ctx& c = *ctx->c;
         TreeGeom g = tree_geom(leaves, 0);
-        c.reserve_arena(leaves * 64 + g.words() * 4 + (1 << 20));
-        c.reset_arena();
+        c.reserve_scratch(leaves * 64 + g.words() * 4 + (1 << 20));
         const uint32_t* din = to_dev(c, reinterpret_cast<const uint32_t*>(flat), leaves * 16);
-        uint32_t* tree = c.alloc<uint32_t>(g.words());
-        int first = std::min(9, g.target);
-        launch_fri_leaves(c.launch(), din, false, tree, leaves, first);
-        build_levels(c, tree, g, first);
+        uint32_t* tree = c.scratch_alloc<uint32_t>(g.words());
+        LeavesJob lj{};
+        lj.kind = LEAF_RAW;
+        lj.src = din;
+        std::vector<Step> chain;
+        tree_steps(chain, tree, g, lj);
+        run_chain(c, chain);
         to_host(c, reinterpret_cast<uint32_t*>(levels), tree, g.words());
     });
 }
@@ -125,13 +136,19 @@ int z0p_test_trace_tree(z0p_ctx* ctx, const uint32_t* a, const uint32_t* b, size
             bbr[p] = b[i];
         }
         TreeGeom g = tree_geom(m / 8, 0);
-        c.reserve_arena(m * 8 + g.words() * 4 + (1 << 20));
-        c.reset_arena();
+        c.reserve_scratch(m * 8 + g.words() * 4 + (1 << 20));
         uint32_t* da = to_dev(c, abr.data(), m);
         uint32_t* db = to_dev(c, bbr.data(), m);
-        uint32_t* tree = c.alloc<uint32_t>(g.words());
-        int have = launch_trace_leaves(c.launch(), da, db, tree, logn, logb, g.target);
-        build_levels(c, tree, g, have);
+        uint32_t* tree = c.scratch_alloc<uint32_t>(g.words());
+        LeavesJob lj{};
+        lj.kind = LEAF_TRACE;
+        lj.src = da;
+        lj.src_b = db;
+        lj.logn = logn;
+        lj.logb = logb;
+        std::vector<Step> chain;
+        tree_steps(chain, tree, g, lj);
+        run_chain(c, chain);
         to_host(c, reinterpret_cast<uint32_t*>(levels), tree, g.words());
     });
 }
@@ -143,21 +160,26 @@ int z0p_test_linked(z0p_ctx* ctx, const uint32_t* a, size_t n, size_t blowup, ui
         check_geometry(n, blowup);
         Ctx& c = *ctx->c;
         int logn = int(ilog2(n)), logb = int(ilog2(blowup));
-        size_t m = n * blowup, B = blowup;
+        size_t m = n * blowup;
         const GeometryTables& gt = c.geometry(logn, logb);
         std::vector<uint32_t> abr(m), bbr(m);
         for (size_t i = 0; i < m; ++i) abr[br_pos(i, logn, logb)] = a[i];
         const uint32_t w_last = pow_c(root_of_unity(uint32_t(logn)), n - 1);
         const uint32_t coeff = mul(mul(sub(actual_b_last % P, a0 % P), w_last), inv_c(uint32_t(n % P)));
-        const uint32_t wm = root_of_unity(uint32_t(logn + logb));
-        std::vector<uint32_t> cv(B);
-        for (size_t j = 0; j < B; ++j) cv[j] = mul(coeff, sub(pow_c(mul(GENERATOR, pow_c(wm, j)), n), 1));
-        c.reserve_arena(m * 8 + (1 << 20));
-        c.reset_arena();
+        c.reserve_scratch(m * 8 + (1 << 20));
         uint32_t* da = to_dev(c, abr.data(), m);
-        uint32_t* dcv = to_dev(c, cv.data(), B);
-        uint32_t* db = c.alloc<uint32_t>(m);
-        launch_linked_b(c.launch(), da, db, logn, logb, gt.w, gt.coset.shift, dcv, to_mont(w_last));
+        uint32_t* db = c.scratch_alloc<uint32_t>(m);
+        LinkedJob lj{};
+        lj.a = da;
+        lj.b = db;
+        lj.w = gt.w;
+        lj.shift_m = gt.coset.shift;
+        lj.zvan_m = gt.zvan;
+        lj.coeff = coeff;
+        lj.w_last_m = to_mont(w_last);
+        lj.logn = logn;
+        lj.logb = logb;
+        run_chain(c, {lj});
         to_host(c, bbr.data(), db, m);
         for (size_t i = 0; i < m; ++i) b[i] = bbr[br_pos(i, logn, logb)];
     });
@@ -185,12 +207,21 @@ int z0p_test_compose(z0p_ctx* ctx, const uint32_t* a, const uint32_t* b, size_t 
         cc.a0 = a0 % P;
         cc.b0 = b0 % P;
         cc.b_last = b_last % P;
-        c.reserve_arena(m * 24 + (1 << 20));
-        c.reset_arena();
+        c.reserve_scratch(m * 24 + (1 << 20));
         uint32_t* da = to_dev(c, abr.data(), m);
         uint32_t* db = to_dev(c, bbr.data(), m);
-        uint32_t* dl = c.alloc<uint32_t>(4 * m);
-        launch_compose(c.launch(), da, db, dl, logn, logb, gt.w, gt.coset.shift, gt.zinv, cc);
+        uint32_t* dl = c.scratch_alloc<uint32_t>(4 * m);
+        ComposeJob cj{};
+        cj.a = da;
+        cj.b = db;
+        cj.layer0 = dl;
+        cj.w = gt.w;
+        cj.shift_m = gt.coset.shift;
+        cj.zinv_m = gt.zinv;
+        cj.c = cc;
+        cj.logn = logn;
+        cj.logb = logb;
+        run_chain(c, {cj});
         to_host(c, o.data(), dl, 4 * m);
         for (size_t i = 0; i < m; ++i) std::memcpy(out + 4 * i, &o[4 * br_pos(i, logn, logb)], 16);
     });
@@ -213,11 +244,16 @@ int z0p_test_fold(z0p_ctx* ctx, const uint32_t* layer, size_t len, size_t lde_le
         fc.ginv_m = to_mont(pow_c(inv_c(GENERATOR), uint64_t(1) << (2 * depth)));
         fc.log_len = int(ilog2(len));
         fc.depth = int(depth);
-        c.reserve_arena(len * 20 + (1 << 20));
-        c.reset_arena();
+        c.reserve_scratch(len * 20 + (1 << 20));
         uint32_t* dl = to_dev(c, layer, 4 * len);
-        uint32_t* dn = c.alloc<uint32_t>(len);
-        launch_fold(c.launch(), dl, false, dn, gt.winv, fc);
+        uint32_t* dn = c.scratch_alloc<uint32_t>(len);
+        FoldJob fj{};
+        fj.layer = dl;
+        fj.next = dn;
+        fj.winv = gt.winv;
+        fj.c = fc;
+        fj.base_field = 0;
+        run_chain(c, {fj});
         to_host(c, out, dn, len);
     });
 }
@@ -227,10 +263,9 @@ int z0p_test_copy(z0p_ctx* ctx, size_t bytes, float* ms) {
     return guarded([&] {
         require(bytes % 16 == 0 && bytes > 0, Z0P_ERR_MALFORMED_INPUT);
         Ctx& c = *ctx->c;
-        c.reserve_arena(2 * bytes + (1 << 20));
-        c.reset_arena();
-        uint32_t* a = c.alloc<uint32_t>(bytes / 4);
-        uint32_t* b = c.alloc<uint32_t>(bytes / 4);
+        c.reserve_scratch(2 * bytes + (1 << 20));
+        uint32_t* a = c.scratch_alloc<uint32_t>(bytes / 4);
+        uint32_t* b = c.scratch_alloc<uint32_t>(bytes / 4);
         hip_check(hipMemsetAsync(a, 1, bytes, c.stream));
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0));
@@ -250,21 +285,17 @@ int z0p_test_grind(z0p_ctx* ctx, const uint8_t* head, size_t head_len, uint32_t 
     return guarded([&] {
         require(head_len == 47 && bits <= 32, Z0P_ERR_MALFORMED_INPUT);
         Ctx& c = *ctx->c;
-        c.reserve_arena(1 << 20);
-        c.reset_arena();
-        uint32_t hw[16] = {0};
-        for (size_t i = 0; i < head_len; ++i) hw[i >> 2] |= uint32_t(head[i]) << (8 * (i & 3));
-        unsigned long long* res = c.alloc<unsigned long long>(1);
+        c.reserve_scratch(1 << 20);
+        GrindJob g{};
+        for (size_t i = 0; i < head_len; ++i) g.hw[i >> 2] |= uint32_t(head[i]) << (8 * (i & 3));
+        g.bits = bits;
+        g.nblocks = 2048;
+        g.result = c.scratch_alloc<unsigned long long>(1);
         uint64_t found = ~uint64_t(0);
         for (uint64_t window = 0; found == ~uint64_t(0); ++window) {
-            uint64_t limit = uint64_t(1) << std::min<uint64_t>(63, 24 + 4 * window);
-            uint64_t init = ~uint64_t(0);
-            std::memcpy(c.pinned, &init, 8);
-            hip_check(hipMemcpyAsync(res, c.pinned, 8, hipMemcpyHostToDevice, c.stream));
-            launch_grind(c.launch(), hw, 47, bits, limit, res);
-            hip_check(hipMemcpyAsync(c.pinned + 8, res, 8, hipMemcpyDeviceToHost, c.stream));
-            c.sync();
-            std::memcpy(&found, c.pinned + 8, 8);
+            g.limit = uint64_t(1) << std::min<uint64_t>(63, 24 + 4 * window);
+            run_chain(c, {g});
+            to_host(c, &found, reinterpret_cast<const uint64_t*>(g.result), 1);
         }
         *nonce = found;
     });
