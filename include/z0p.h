@@ -18,7 +18,7 @@
  *    keep using the transcript after a call.
  *  - A z0p_ctx owns one HIP stream on one device plus its device memory.
  *    Calls on one ctx are synchronous and must not overlap; use one ctx per
- *    host thread.
+ *    host thread. A ctx held by a z0p_server is used by the server only.
  *  - Field elements cross the boundary as canonical u32 (< p = 2013265921).
  */
 #ifndef Z0P_H
@@ -131,6 +131,46 @@ int z0p_fri_commit_open(z0p_ctx* ctx, const uint32_t* coeffs, size_t n, const z0
 /* Commitment/Proof::from_bytes + fri::verify (fri.rs:686-859). Host only. */
 int z0p_fri_verify(const uint8_t* commitment, size_t commitment_len, const uint8_t* proof, size_t proof_len,
                    const z0p_fri_params* params, z0p_transcript* transcript);
+
+/* ---- batched proving (no reference counterpart: the reference proves one
+ * instance per air::prove call; this is `count` such calls at once) ---- */
+/* Device-memory budget of the batch and server schedulers on this ctx, in
+ * bytes; 0 (the default) lets the library choose from free device memory. */
+int z0p_ctx_set_hbm_budget(z0p_ctx* ctx, uint64_t bytes);
+/* Runs air::prove for every params[i] with transcripts[i] (each advanced as
+ * that single call would advance it). The instances share every kernel
+ * launch and are admitted as device memory allows. statuses[i] receives
+ * instance i's status; proofs[i] / proof_lens[i] are set when it is 0.
+ * Returns Z0P_OK when the batch ran (per-instance failures are only in
+ * statuses) or a device error that ended the whole batch. */
+int z0p_air_prove_batch(z0p_ctx* ctx, const z0p_air_params* params, z0p_transcript* const* transcripts,
+                        size_t count, uint8_t** proofs, size_t* proof_lens, int* statuses);
+
+/* ---- serving (no reference counterpart) ----
+ * An asynchronous request loop on its own host thread, which takes over the
+ * ctx until z0p_server_destroy. Requests arriving while others run are
+ * admitted into the running batch as device memory and slots free up, so
+ * their device work overlaps; every result is byte-identical to the
+ * proof z0p_air_prove returns for the same request. */
+typedef struct z0p_server z0p_server;
+/* hbm_budget 0: half of the free device memory; max_active 0: 256. */
+int z0p_server_create(z0p_ctx* ctx, uint64_t hbm_budget, uint32_t max_active, z0p_server** out);
+/* Queues one AIR proof. The transcript is copied: the request proves over
+ * its own copy and the caller's transcript is left unchanged. */
+int z0p_server_submit_air(z0p_server* s, const z0p_air_params* params, const z0p_transcript* transcript,
+                          uint64_t* request_id);
+/* Takes one finished request, waiting up to timeout_ms (0: no wait, < 0:
+ * until one finishes). Returns Z0P_OK with *request_id set, or
+ * Z0P_ERR_EMPTY_INPUT when none finished in time. The request's own status
+ * goes to *status; *proof (free with z0p_free) is set when it is 0.
+ * Latencies are milliseconds from submission to completion and to
+ * admission onto the device. */
+int z0p_server_poll(z0p_server* s, int timeout_ms, uint64_t* request_id, int* status, uint8_t** proof,
+                    size_t* proof_len, double* latency_ms, double* queue_ms);
+/* Requests submitted and not yet polled (queued, running or finished). */
+size_t z0p_server_pending(z0p_server* s);
+/* Finishes every running request, drops unpolled results, releases the ctx. */
+void z0p_server_destroy(z0p_server* s);
 
 #ifdef __cplusplus
 }

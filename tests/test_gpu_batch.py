@@ -1,0 +1,160 @@
+"""Batched and served proving: every proof must be byte-identical to the one
+the single-instance path (and the oracle) produces for the same request,
+whatever it ran beside, however the scheduler admitted it, and each
+transcript must end in the state the single call leaves."""
+import random
+import threading
+import time
+
+import pytest
+
+import oracle_ffi as o
+from helpers import seeded_claims, splitmix64
+
+import z0p_amd as z
+
+pytestmark = pytest.mark.gpu
+
+
+def _instances(count, kmin, kmax, seed):
+    """Log-uniform step counts 2^kmin..2^kmax and seeded (a0, b0), as the
+    batched workload draws them."""
+    out = []
+    st = seed
+    for i in range(count):
+        st, r = splitmix64(st)
+        k = kmin + r % (kmax - kmin + 1)
+        n = 1 << k
+        a0, b0 = seeded_claims(seed * 1000 + i)
+        bl = o.fib_b_last(a0, b0, n)
+        out.append((n, a0, b0, bl, b"batch/%d" % i))
+    return out
+
+
+def _oracle(n, a0, b0, bl, label):
+    return o.air_prove(o.air_params_for_security(n, a0, b0, bl), o.Transcript(label))
+
+
+def test_batch_mixed_sizes_byte_identical(ctx):
+    inst = _instances(24, 6, 14, 7)
+    print("batch sizes", [i[0] for i in inst], flush=True)
+    params = [z.air.for_security(n, a0, b0, bl) for n, a0, b0, bl, _ in inst]
+    trs = [z.Transcript(lab) for *_, lab in inst]
+    got = z.air.prove_batch(params, trs, ctx)
+    for (n, a0, b0, bl, lab), g, t in zip(inst, got, trs):
+        assert isinstance(g, bytes), (n, g)
+        ot = o.Transcript(lab)
+        want = o.air_prove(o.air_params_for_security(n, a0, b0, bl), ot)
+        assert g == want, n
+        assert t.challenge_bytes(b"after") == ot.challenge_bytes(b"after")
+
+
+def test_batch_large_and_small_together(ctx):
+    """A 2^20 proof beside small ones equals the single-instance proof."""
+    inst = [(1 << 20, 1, 1, o.fib_b_last(1, 1, 1 << 20), b"bench/air-scale")]
+    inst += _instances(6, 10, 16, 3)
+    params = [z.air.for_security(n, a0, b0, bl) for n, a0, b0, bl, _ in inst]
+    got = z.air.prove_batch(params, [z.Transcript(lab) for *_, lab in inst], ctx)
+    for (n, a0, b0, bl, lab), g, p in zip(inst, got, params):
+        print("compare n", n, flush=True)
+        assert g == z.air.prove(p, z.Transcript(lab), ctx), n
+    assert len(got[0]) == 203553
+
+
+def test_batch_under_tight_budget(ctx):
+    """A budget that holds only a few proofs at once: later instances are
+    admitted as earlier ones finish, with identical results."""
+    inst = _instances(12, 8, 12, 11)
+    params = [z.air.for_security(n, a0, b0, bl) for n, a0, b0, bl, _ in inst]
+    want = [z.air.prove(p, z.Transcript(lab), ctx) for p, (*_, lab) in zip(params, inst)]
+    ctx.set_hbm_budget(24 << 20)
+    try:
+        got = z.air.prove_batch(params, [z.Transcript(lab) for *_, lab in inst], ctx)
+    finally:
+        ctx.set_hbm_budget(0)
+    assert got == want
+
+
+def test_batch_reports_per_instance_errors(ctx):
+    inst = _instances(3, 6, 8, 5)
+    params = [z.air.for_security(n, a0, b0, bl) for n, a0, b0, bl, _ in inst]
+    bad = z.air.Params()
+    bad.n_steps, bad.blowup, bad.num_queries, bad.final_size = 3, 16, 4, 4  # not a power of two
+    got = z.air.prove_batch(params[:1] + [bad] + params[1:], [z.Transcript(b"e%d" % i) for i in range(4)], ctx)
+    assert isinstance(got[1], z.Z0pError) and got[1].name == "InvalidDomainSize"
+    for g, p, i in zip([got[0], got[2], got[3]], params, [0, 2, 3]):
+        assert g == z.air.prove(p, z.Transcript(b"e%d" % i), ctx)
+
+
+def test_batch_rejects_shared_transcript(ctx):
+    p = z.air.for_security(64, 1, 1, o.fib_b_last(1, 1, 64))
+    t = z.Transcript(b"shared")
+    with pytest.raises(z.Z0pError) as e:
+        z.air.prove_batch([p, p], [t, t], ctx)
+    assert e.value.name == "InvalidArgument"
+
+
+def _serve(requests, budget, gaps_s):
+    """Submit from a producer thread with the given inter-arrival gaps, poll
+    from this thread; returns {request index: (proof, latency, queue)}."""
+    sctx = z.Context(0)
+    out = {}
+    with z.Server(sctx, hbm_budget=budget) as srv:
+        ids = {}
+
+        def producer():
+            for i, (p, lab) in enumerate(requests):
+                time.sleep(gaps_s[i])
+                ids[srv.submit(p, z.Transcript(lab))] = i
+
+        th = threading.Thread(target=producer)
+        th.start()
+        deadline = time.time() + 120
+        while len(out) < len(requests) and time.time() < deadline:
+            r = srv.poll(200)
+            if r is None:
+                continue
+            rid, res, lat, q = r
+            while rid not in ids:  # submit() returned after the worker finished it
+                time.sleep(0.001)
+            out[ids[rid]] = (res, lat, q)
+        th.join()
+        assert srv.pending() == 0
+    sctx.close()
+    return out
+
+
+def test_server_same_alone_and_in_mix(ctx):
+    inst = _instances(20, 6, 16, 21)
+    params = [z.air.for_security(n, a0, b0, bl) for n, a0, b0, bl, _ in inst]
+    alone = [z.air.prove(p, z.Transcript(lab), ctx) for p, (*_, lab) in zip(params, inst)]
+    rng = random.Random(4)
+    gaps = [rng.expovariate(1000.0) for _ in inst]  # ~1 ms mean gap: arrivals overlap running work
+    print("serving", [i[0] for i in inst], flush=True)
+    out = _serve([(p, lab) for p, (*_, lab) in zip(params, inst)], 0, gaps)
+    assert sorted(out) == list(range(len(inst)))
+    for i, a in enumerate(alone):
+        res, lat, q = out[i]
+        assert res == a, i
+        assert lat >= q >= 0
+
+
+def test_server_tight_budget_queues_and_matches(ctx):
+    inst = _instances(10, 10, 12, 33)
+    params = [z.air.for_security(n, a0, b0, bl) for n, a0, b0, bl, _ in inst]
+    alone = [z.air.prove(p, z.Transcript(lab), ctx) for p, (*_, lab) in zip(params, inst)]
+    out = _serve([(p, lab) for p, (*_, lab) in zip(params, inst)], 24 << 20, [0.0] * len(inst))
+    assert [out[i][0] for i in range(len(inst))] == alone
+    assert max(out[i][2] for i in range(len(inst))) > 0.0  # some request waited for memory
+
+
+def test_server_rejects_invalid_request(ctx):
+    sctx = z.Context(0)
+    with z.Server(sctx) as srv:
+        bad = z.air.Params()
+        bad.n_steps, bad.blowup, bad.num_queries, bad.final_size = 64, 3, 4, 4
+        with pytest.raises(z.Z0pError) as e:
+            srv.submit(bad, z.Transcript(b"x"))
+        assert e.value.name == "InvalidBlowup"
+        assert srv.pending() == 0
+    sctx.close()

@@ -11,7 +11,7 @@ import ctypes
 
 from ._ffi import Z0pError, buf, check, lib, take
 
-__all__ = ["Z0pError", "Transcript", "Context", "air", "fri", "default_context"]
+__all__ = ["Z0pError", "Transcript", "Context", "Server", "air", "fri", "default_context"]
 
 
 class Transcript:
@@ -84,6 +84,62 @@ class Context:
 
     def reset_stats(self):
         lib().z0p_ctx_reset_stats(self._h)
+
+    def set_hbm_budget(self, nbytes: int):
+        """Device-memory budget of batched proving on this context (0: auto)."""
+        check(lib().z0p_ctx_set_hbm_budget(self._h, nbytes))
+
+
+class Server:
+    """Asynchronous proving loop (z0p_server) that takes over a Context:
+    requests arriving while others run join the running batch as device
+    memory and slots free up. ``submit`` returns a request id; ``poll``
+    returns ``(id, proof bytes or Z0pError, latency_ms, queue_ms)`` or None.
+    """
+
+    def __init__(self, ctx: "Context", hbm_budget: int = 0, max_active: int = 0):
+        h = ctypes.c_void_p()
+        check(lib().z0p_server_create(ctx.handle, hbm_budget, max_active, ctypes.byref(h)))
+        self._h = h
+        self._ctx = ctx  # keep the context alive while the server runs
+
+    def submit(self, params, transcript) -> int:
+        rid = ctypes.c_uint64()
+        check(lib().z0p_server_submit_air(self._h, ctypes.byref(params), transcript.handle, ctypes.byref(rid)))
+        return rid.value
+
+    def poll(self, timeout_ms: int = 0):
+        from ._ffi import u8p
+        rid = ctypes.c_uint64()
+        st = ctypes.c_int()
+        out = u8p()
+        n = ctypes.c_size_t()
+        lat = ctypes.c_double()
+        q = ctypes.c_double()
+        rc = lib().z0p_server_poll(self._h, timeout_ms, ctypes.byref(rid), ctypes.byref(st), ctypes.byref(out),
+                                   ctypes.byref(n), ctypes.byref(lat), ctypes.byref(q))
+        if rc == 1:
+            return None
+        check(rc)
+        res = take(out, n.value) if st.value == 0 else Z0pError(st.value)
+        return rid.value, res, lat.value, q.value
+
+    def pending(self) -> int:
+        return lib().z0p_server_pending(self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().z0p_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 _default = None

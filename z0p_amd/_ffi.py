@@ -82,6 +82,16 @@ SIGNATURES = {
     "z0p_fri_commit_open": (ctypes.c_int, [vp, u32p, sz, ctypes.POINTER(FriParamsC), vp, ctypes.POINTER(u8p),
                                            ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
     "z0p_fri_verify": (ctypes.c_int, [u8p, sz, u8p, sz, ctypes.POINTER(FriParamsC), vp]),
+    "z0p_ctx_set_hbm_budget": (ctypes.c_int, [vp, u64]),
+    "z0p_air_prove_batch": (ctypes.c_int, [vp, ctypes.POINTER(AirParamsC), ctypes.POINTER(vp), sz,
+                                           ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_int)]),
+    "z0p_server_create": (ctypes.c_int, [vp, u64, u32, ctypes.POINTER(vp)]),
+    "z0p_server_submit_air": (ctypes.c_int, [vp, ctypes.POINTER(AirParamsC), vp, ctypes.POINTER(u64)]),
+    "z0p_server_poll": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double)]),
+    "z0p_server_pending": (sz, [vp]),
+    "z0p_server_destroy": (None, [vp]),
     # include/z0p_testing.h
     "z0p_test_intt": (ctypes.c_int, [vp, u32p, sz, u32p]),
     "z0p_test_lde": (ctypes.c_int, [vp, u32p, sz, sz, u32p]),

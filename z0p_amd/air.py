@@ -54,9 +54,32 @@ def prove(params: Params, transcript, ctx=None) -> bytes:
     return take(out, n.value)
 
 
+def prove_batch(params_list, transcripts, ctx=None):
+    """``air.prove`` for every (params, transcript) pair in one batch: the
+    instances share every kernel launch and are admitted as device memory
+    allows. Returns a list of proof bytes, or of Z0pError for instances that
+    failed; each transcript is advanced as its single call would advance it.
+    """
+    from . import default_context
+    ctx = ctx or default_context()
+    count = len(params_list)
+    if len(transcripts) != count:
+        raise ValueError("one transcript per instance")
+    ps = (Params * max(count, 1))(*params_list)
+    ts = (ctypes.c_void_p * max(count, 1))(*[t.handle for t in transcripts])
+    outs = (u8p * max(count, 1))()
+    lens = (ctypes.c_size_t * max(count, 1))()
+    sts = (ctypes.c_int * max(count, 1))()
+    check(lib().z0p_air_prove_batch(ctx.handle, ps, ts, count, outs, lens, sts))
+    result = []
+    for i in range(count):
+        result.append(take(outs[i], lens[i]) if sts[i] == 0 else Z0pError(sts[i]))
+    return result
+
+
 def verify(params: Params, proof: bytes, transcript) -> None:
     """Proof::from_bytes + air::verify (air.rs:289-327, 607-752); raises Z0pError."""
     check(lib().z0p_air_verify(ctypes.byref(params), transcript.handle, buf(proof), len(proof)))
 
 
-__all__ = ["Params", "params_new", "for_security", "b_last", "prove", "verify", "Z0pError"]
+__all__ = ["Params", "params_new", "for_security", "b_last", "prove", "prove_batch", "verify", "Z0pError"]

@@ -848,7 +848,20 @@ void host_step(Ctx& c, Proof& P) {
 // ---------------------------------------------------------------------------
 // Scheduler.
 
-Engine::Engine(Ctx& c, size_t budget, size_t max_active) : c_(c), budget_(budget), max_active_(max_active) {}
+Engine::Engine(Ctx& c, size_t budget, size_t max_active) : c_(c), budget_(budget), max_active_(max_active) {
+    if (max_active_ == 0) max_active_ = 1;
+    if (budget_ && c_.pool.in_use() == 0 && c_.pool.capacity() < budget_) {
+        c_.sync();
+        c_.pool.reset_capacity(budget_);
+    }
+}
+
+size_t auto_budget(Ctx& c, size_t want) {
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b));
+    if (c.pool.in_use() == 0) free_b += c.pool.capacity();
+    return std::min(want, free_b / 5 * 4);
+}
 
 Engine::~Engine() {
     for (Proof* p : active_)
@@ -873,6 +886,7 @@ void Engine::admit() {
     while (!queue_.empty() && active_.size() < max_active_) {
         Proof* p = queue_.front();
         uint8_t* mem = nullptr;
+        if (budget_ && c_.pool.in_use() != 0 && c_.pool.in_use() + p->mem_bytes > budget_) break;
         if (!c_.pool.alloc(p->mem_bytes, mem)) {
             if (c_.pool.in_use() != 0) break;  // wait for memory to free up
             // Idle pool: grow it to fit (bounded by the budget, if any).
@@ -934,19 +948,23 @@ bool Engine::iterate() {
     return !active_.empty() || !queue_.empty();
 }
 
+void Engine::fail_all(int status) {
+    for (Proof* p : active_) finish(p, status);
+    active_.clear();
+    stage_need_ = 0;
+    while (!queue_.empty()) {
+        finish(queue_.front(), status);
+        queue_.pop_front();
+    }
+}
+
 void Engine::run() {
     try {
         while (iterate()) {
         }
     } catch (const Status& s) {
         // Device failure: every unfinished proof fails with it.
-        for (Proof* p : active_) finish(p, s.code);
-        active_.clear();
-        stage_need_ = 0;
-        while (!queue_.empty()) {
-            finish(queue_.front(), s.code);
-            queue_.pop_front();
-        }
+        fail_all(s.code);
         throw;
     }
 }

@@ -70,9 +70,12 @@ class Engine {
     Engine(Ctx& c, size_t budget, size_t max_active);
     ~Engine();
     void submit(Proof* p);
-    // One iteration; false once nothing is queued or active.
+    // One iteration; false once nothing is queued or active. Throws Status
+    // on a device failure (call fail_all, then drop the engine).
     bool iterate();
     void run();
+    // Ends every queued and active proof with `status`.
+    void fail_all(int status);
     void on_done(std::function<void(Proof*)> f) { on_done_ = std::move(f); }
     size_t active() const { return active_.size(); }
     size_t queued() const { return queue_.size(); }
@@ -89,6 +92,10 @@ class Engine {
     uint64_t iterations_ = 0;
     std::function<void(Proof*)> on_done_;
 };
+
+// Budget for proofs needing `want` bytes in total: at most 80% of the device
+// memory that is free (counting an idle pool as free).
+size_t auto_budget(Ctx& c, size_t want);
 
 std::vector<uint8_t> air_prove(Ctx& ctx, const z0p_air_params& p, Transcript& tr);
 void fri_commit_open(Ctx& ctx, const uint32_t* coeffs, uint64_t n, const z0p_fri_params& fp, Transcript& tr,

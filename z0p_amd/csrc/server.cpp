@@ -1,0 +1,292 @@
+// Batched and served proving on one device (include/z0p.h: z0p_air_prove_batch,
+// z0p_server_*). Both feed the proof scheduler (prover.cpp: Engine): proofs
+// share every kernel launch and are admitted as pool memory frees up. The
+// server runs the scheduler on its own host thread and admits requests that
+// arrive while others are on the device into the running batch (continuous
+// batching); a request's proof bytes do not depend on what it ran beside.
+#include <chrono>
+#include <condition_variable>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <set>
+#include <thread>
+
+#include "params.hpp"
+#include "prover.hpp"
+
+namespace {
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return Z0P_OK;
+    } catch (const z0p::Status& s) {
+        return s.code;
+    } catch (const std::bad_alloc&) {
+        return Z0P_ERR_OUT_OF_MEMORY;
+    } catch (...) {
+        return Z0P_ERR_DEVICE;
+    }
+}
+
+uint8_t* copy_out(const std::vector<uint8_t>& v) {
+    uint8_t* p = static_cast<uint8_t*>(std::malloc(v.empty() ? 1 : v.size()));
+    if (p && !v.empty()) std::memcpy(p, v.data(), v.size());
+    return p;
+}
+
+constexpr size_t DEFAULT_MAX_ACTIVE = 256;
+
+// Same clock as the scheduler's proof timestamps.
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+using z0p::Proof;
+
+struct ServerRequest {
+    uint64_t id = 0;
+    z0p::Transcript tr;
+    z0p_air_params params{};
+    Proof* proof = nullptr;
+    int status = Z0P_OK;
+    double submit_ms = 0, admit_ms = 0, done_ms = 0;
+    ~ServerRequest() {
+        if (proof) z0p::proof_free(proof);
+    }
+};
+
+struct z0p_server {
+    z0p::Ctx* c = nullptr;
+    size_t budget = 0, max_active = DEFAULT_MAX_ACTIVE;
+    std::unique_ptr<z0p::Engine> engine;
+    std::thread worker;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    bool stop = false;
+    uint64_t next_id = 1;
+    size_t pending = 0;  // submitted and not yet polled
+    std::deque<ServerRequest*> incoming;
+    std::map<uint64_t, ServerRequest*> running;
+    std::deque<ServerRequest*> finished;
+
+    void complete(ServerRequest* r) {
+        if (r->done_ms == 0) r->done_ms = r->admit_ms = now_ms();
+        std::lock_guard<std::mutex> lk(mu);
+        finished.push_back(r);
+        cv_done.notify_all();
+    }
+
+    void loop() {
+        (void)hipSetDevice(c->device);
+        z0p::Engine* e = engine.get();
+        auto on_done = [this](Proof* p) {
+            ServerRequest* r = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                auto it = running.find(z0p::proof_tag(p));
+                if (it == running.end()) return;
+                r = it->second;
+                running.erase(it);
+            }
+            r->status = z0p::proof_status(p);
+            double engine_submit = 0;
+            z0p::proof_times(p, engine_submit, r->admit_ms, r->done_ms);
+            if (r->admit_ms == 0) r->admit_ms = r->done_ms;  // never admitted (over budget)
+            complete(r);
+        };
+        e->on_done(on_done);
+        while (true) {
+            std::deque<ServerRequest*> arrived;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return stop || !incoming.empty() || e->active() + e->queued() > 0; });
+                if (stop && incoming.empty() && e->active() + e->queued() == 0) break;
+                arrived.swap(incoming);
+            }
+            for (ServerRequest* r : arrived) {
+                try {
+                    r->proof = z0p::proof_new_air(*c, r->params, &r->tr);
+                } catch (const z0p::Status& s) {
+                    r->status = s.code;
+                    complete(r);
+                    continue;
+                } catch (...) {
+                    r->status = Z0P_ERR_DEVICE;
+                    complete(r);
+                    continue;
+                }
+                z0p::proof_set_tag(r->proof, r->id);
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    running[r->id] = r;
+                }
+                e->submit(r->proof);
+            }
+            // A device failure ends the proofs on the device with its status;
+            // the scheduler is left empty and later requests run as usual.
+            try {
+                e->iterate();
+            } catch (const z0p::Status& s) {
+                e->fail_all(s.code);
+            } catch (...) {
+                e->fail_all(Z0P_ERR_DEVICE);
+            }
+        }
+    }
+};
+
+extern "C" {
+
+int z0p_ctx_set_hbm_budget(z0p_ctx* ctx, uint64_t bytes) {
+    if (!ctx) return Z0P_ERR_INVALID_ARGUMENT;
+    ctx->c->hbm_budget = size_t(bytes);
+    return Z0P_OK;
+}
+
+int z0p_air_prove_batch(z0p_ctx* ctx, const z0p_air_params* params, z0p_transcript* const* transcripts,
+                        size_t count, uint8_t** proofs, size_t* proof_lens, int* statuses) {
+    if (!ctx || (count && (!params || !transcripts || !proofs || !proof_lens || !statuses)))
+        return Z0P_ERR_INVALID_ARGUMENT;
+    std::set<const z0p_transcript*> seen;
+    for (size_t i = 0; i < count; ++i) {
+        // Each instance advances its own transcript: no null and no sharing.
+        if (!transcripts[i] || !seen.insert(transcripts[i]).second) return Z0P_ERR_INVALID_ARGUMENT;
+        proofs[i] = nullptr;
+        proof_lens[i] = 0;
+        statuses[i] = Z0P_OK;
+    }
+    z0p::Ctx& c = *ctx->c;
+    std::vector<std::unique_ptr<Proof, void (*)(Proof*)>> ps;
+    ps.reserve(count);
+    return guarded([&] {
+        size_t want = 0;
+        for (size_t i = 0; i < count; ++i) {
+            Proof* p = nullptr;
+            try {
+                p = z0p::proof_new_air(c, params[i], &transcripts[i]->t);
+            } catch (const z0p::Status& s) {
+                statuses[i] = s.code;
+            }
+            ps.emplace_back(p, z0p::proof_free);
+            if (p) want += z0p::proof_mem_bytes(p);
+        }
+        const size_t budget = c.hbm_budget ? c.hbm_budget : z0p::auto_budget(c, want);
+        z0p::Engine e(c, budget, DEFAULT_MAX_ACTIVE);
+        for (auto& p : ps)
+            if (p) e.submit(p.get());
+        e.run();
+        for (size_t i = 0; i < count; ++i) {
+            if (!ps[i]) continue;
+            statuses[i] = z0p::proof_status(ps[i].get());
+            if (statuses[i] != Z0P_OK) continue;
+            const std::vector<uint8_t>& b = z0p::proof_bytes(ps[i].get());
+            proofs[i] = copy_out(b);
+            if (!proofs[i]) {
+                statuses[i] = Z0P_ERR_OUT_OF_MEMORY;
+                continue;
+            }
+            proof_lens[i] = b.size();
+        }
+    });
+}
+
+int z0p_server_create(z0p_ctx* ctx, uint64_t hbm_budget, uint32_t max_active, z0p_server** out) {
+    if (!ctx || !out) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] {
+        auto s = std::make_unique<z0p_server>();
+        s->c = ctx->c;
+        if (hbm_budget == 0) {
+            size_t free_b = 0, total_b = 0;
+            z0p::hip_check(hipMemGetInfo(&free_b, &total_b));
+            if (s->c->pool.in_use() == 0) free_b += s->c->pool.capacity();
+            hbm_budget = free_b / 2;
+        }
+        s->budget = size_t(hbm_budget);
+        s->max_active = max_active ? max_active : DEFAULT_MAX_ACTIVE;
+        s->engine.reset(new z0p::Engine(*s->c, s->budget, s->max_active));
+        z0p_server* raw = s.get();
+        s->worker = std::thread([raw] { raw->loop(); });
+        *out = s.release();
+    });
+}
+
+int z0p_server_submit_air(z0p_server* s, const z0p_air_params* params, const z0p_transcript* transcript,
+                          uint64_t* request_id) {
+    if (!s || !params || !transcript || !request_id) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] {
+        // Parameter errors surface at submission, as from z0p_air_prove.
+        z0p::validate_air(*params);
+        z0p::validate_fri(z0p::air_fri_params(*params));
+        auto r = std::make_unique<ServerRequest>();
+        r->tr = transcript->t;
+        r->params = *params;
+        r->submit_ms = now_ms();
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->id = s->next_id++;
+        *request_id = r->id;
+        s->incoming.push_back(r.release());
+        ++s->pending;
+        s->cv_work.notify_one();
+    });
+}
+
+int z0p_server_poll(z0p_server* s, int timeout_ms, uint64_t* request_id, int* status, uint8_t** proof,
+                    size_t* proof_len, double* latency_ms, double* queue_ms) {
+    if (!s || !request_id || !status || !proof || !proof_len) return Z0P_ERR_INVALID_ARGUMENT;
+    std::unique_ptr<ServerRequest> r;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto ready = [&] { return !s->finished.empty(); };
+        if (timeout_ms < 0) s->cv_done.wait(lk, ready);
+        else if (timeout_ms > 0) s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready);
+        if (s->finished.empty()) return Z0P_ERR_EMPTY_INPUT;
+        r.reset(s->finished.front());
+        s->finished.pop_front();
+        --s->pending;
+    }
+    *request_id = r->id;
+    *status = r->status;
+    *proof = nullptr;
+    *proof_len = 0;
+    if (latency_ms) *latency_ms = r->done_ms - r->submit_ms;
+    if (queue_ms) *queue_ms = r->admit_ms - r->submit_ms;
+    if (r->status == Z0P_OK && r->proof) {
+        const std::vector<uint8_t>& b = z0p::proof_bytes(r->proof);
+        *proof = copy_out(b);
+        if (!*proof) {
+            *status = Z0P_ERR_OUT_OF_MEMORY;
+            return Z0P_OK;
+        }
+        *proof_len = b.size();
+    }
+    return Z0P_OK;
+}
+
+size_t z0p_server_pending(z0p_server* s) {
+    if (!s) return 0;
+    std::lock_guard<std::mutex> lk(s->mu);
+    return s->pending;
+}
+
+void z0p_server_destroy(z0p_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv_work.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    for (ServerRequest* r : s->finished) delete r;
+    for (ServerRequest* r : s->incoming) delete r;
+    delete s;
+}
+
+}  // extern "C"
