@@ -19,6 +19,9 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 N_STEPS = 1 << 20
+METRIC = "\u00b5s/step, 2^20-step Fib AIR prove; batched proofs/s + p50/p99 latency"
+P = 2013265921
+MASK64 = (1 << 64) - 1
 LABEL = b"bench/air-scale"
 HBM_PEAK_GBS = 8000.0
 
@@ -49,6 +52,121 @@ FAMILY = {
     "trace_leaves": "merkle", "fri_leaves": "merkle", "tree_levels": "merkle", "tree_tail": "merkle",
     "fold": "fold",
 }
+
+
+def splitmix64(state):
+    state = (state + 0x9E3779B97F4A7C15) & MASK64
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+    return state, z ^ (z >> 31)
+
+
+def seeded_instance(z, k, seed, label):
+    """(params, label) of a Fibonacci instance with 2^k steps and (a0, b0)
+    drawn from the seed (src/gf2.rs:425-431 idiom)."""
+    st, x = splitmix64(seed)
+    st, y = splitmix64(st)
+    a0, b0 = x % P, y % P
+    n = 1 << k
+    return z.air.for_security(n, a0, b0, z.air.b_last(a0, b0, n)), label
+
+
+def batched(z, ctx, count, seed=2024):
+    """256 independent proofs, log2 steps drawn uniformly from 10..18, proved
+    as one batch; beside it the same instances one at a time."""
+    st = seed
+    inst = []
+    for i in range(count):
+        st, r = splitmix64(st)
+        inst.append(seeded_instance(z, 10 + r % 9, seed * 7919 + i, b"batch/%d" % i))
+    steps = sum(p.n_steps for p, _ in inst)
+    z.air.prove_batch([p for p, _ in inst[:16]], [z.Transcript(l) for _, l in inst[:16]], ctx)  # warm-up
+    t0 = time.perf_counter()
+    got = z.air.prove_batch([p for p, _ in inst], [z.Transcript(l) for _, l in inst], ctx)
+    t_batch = time.perf_counter() - t0
+    assert all(isinstance(g, bytes) for g in got)
+    t0 = time.perf_counter()
+    for p, l in inst:
+        z.air.prove(p, z.Transcript(l), ctx)
+    t_single = time.perf_counter() - t0
+    return {
+        "workload": "%d AIR proofs, steps 2^10..2^18 log-uniform (seed %d), production schedule" % (count, seed),
+        "total_steps": steps,
+        "batch_s": round(t_batch, 4),
+        "proofs_per_s": round(count / t_batch, 1),
+        "steps_per_s": round(steps / t_batch, 1),
+        "one_at_a_time_s": round(t_single, 4),
+        "one_at_a_time_proofs_per_s": round(count / t_single, 1),
+        "speedup_vs_one_at_a_time": round(t_single / t_batch, 2),
+    }
+
+
+def served(z, budget_gib, duration_s, seed=77):
+    """Open-loop Poisson arrivals of AIR proofs with log2 steps uniform in
+    10..20 against a fixed HBM budget: saturated throughput first (all
+    requests at once), then three offered loads below it."""
+    import random
+    import threading
+    rng = random.Random(seed)
+
+    def make(count, tag):
+        out = []
+        for i in range(count):
+            out.append(seeded_instance(z, rng.randint(10, 20), rng.getrandbits(63), b"serve/%s/%d" % (tag.encode(), i)))
+        return out
+
+    def run(reqs, arrivals):
+        sctx = z.Context(0)
+        lat, res = [], []
+        with z.Server(sctx, hbm_budget=budget_gib << 30) as srv:
+            def producer():
+                t0 = time.perf_counter()
+                for (p, l), at in zip(reqs, arrivals):
+                    d = t0 + at - time.perf_counter()
+                    if d > 0:
+                        time.sleep(d)
+                    srv.submit(p, z.Transcript(l))
+            th = threading.Thread(target=producer)
+            t0 = time.perf_counter()
+            th.start()
+            while len(lat) < len(reqs):
+                r = srv.poll(1000)
+                if r is None:
+                    continue
+                _, proof, lt, q = r
+                assert isinstance(proof, bytes)
+                lat.append(lt)
+                res.append(q)
+            t1 = time.perf_counter()
+            th.join()
+        sctx.close()
+        lat.sort()
+        return t1 - t0, lat
+
+    def pct(v, q):
+        return round(v[min(len(v) - 1, int(q * len(v)))], 3)
+
+    sat_n = 400
+    wall, lat = run(make(sat_n, "sat"), [0.0] * sat_n)
+    sat = sat_n / wall
+    levels = []
+    for frac in (0.5, 0.8, 0.95):
+        qps = frac * sat
+        count = max(50, min(4000, int(qps * duration_s)))
+        t, arr = 0.0, []
+        for _ in range(count):
+            t += rng.expovariate(qps)
+            arr.append(t)
+        wall, lat = run(make(count, "q%d" % int(frac * 100)), arr)
+        levels.append({"offered_qps": round(qps, 1), "requests": count, "proofs_per_s": round(count / wall, 1),
+                       "p50_ms": pct(lat, 0.50), "p99_ms": pct(lat, 0.99), "max_ms": round(lat[-1], 3)})
+    return {
+        "workload": "Poisson AIR requests, steps 2^10..2^20 log-uniform, production schedule, HBM budget %d GiB"
+                    % budget_gib,
+        "saturated_proofs_per_s": round(sat, 1),
+        "levels": levels,
+    }
 
 
 def cpu_baseline(n, repeats):
@@ -84,6 +202,9 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=3, help="extra proofs timed per kernel (HIP events)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle CPU baseline")
     ap.add_argument("--cpu-repeats", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=256, help="instances in the batched workload; 0 skips it")
+    ap.add_argument("--serve-s", type=float, default=2.0, help="seconds of arrivals per served load; 0 skips it")
+    ap.add_argument("--serve-budget-gib", type=int, default=32)
     ap.add_argument("--copy-mib", type=int, default=256,
                     help="size of the plain copy kernel run after the proofs (counter calibration and "
                          "achievable-bandwidth reference); 0 disables")
@@ -161,10 +282,13 @@ def main():
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                     "ms_per_proof": round(t_ms, 4), "alg_bytes": alg[dom]}
+    batch = batched(z, ctx, args.batch) if args.batch else None
+    serve = served(z, args.serve_budget_gib, args.serve_s) if args.serve_s > 0 else None
+
     out = {
-        "metric": "us/step, 2^20-step Fib AIR prove",
+        "metric": METRIC,
         "value": round(ms * 1000.0 / n, 5),
-        "unit": "us/step",
+        "unit": "\u00b5s/step",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -180,6 +304,8 @@ def main():
         "families": families,
         "roofline": roofline,
         "copy_reference": copy,
+        "batched": batch,
+        "served": serve,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(n, args.cpu_repeats),
     }
     print(json.dumps(out))

@@ -155,10 +155,16 @@ struct Ctx {
     DescRing ring;
     size_t hbm_budget = 0;  // 0: the pool grows to fit (between batches)
 
-    // Staging for device-to-host collection (device + pinned mirror).
+    // Staging for the words the host transcript needs: pinned host memory
+    // the kernels write directly (stage_dev is its device address).
     uint32_t* stage_dev = nullptr;
     uint32_t* stage_host = nullptr;
     size_t stage_words = 0;
+    // Completion flag in pinned host memory: the iteration's last kernel
+    // stores the iteration number there after a system-scope fence.
+    uint64_t* flag_host = nullptr;
+    uint64_t* flag_dev = nullptr;
+    uint64_t seq = 0;
 
     // Scratch arena for the kernel-level diagnostic hooks.
     uint8_t* scratch = nullptr;
@@ -187,14 +193,36 @@ struct Ctx {
     void reserve_stage(size_t words) {
         if (words <= stage_words) return;
         hip_check(hipStreamSynchronize(stream));
-        if (stage_dev) hip_check(hipFree(stage_dev));
         if (stage_host) hip_check(hipHostFree(stage_host));
-        stage_dev = nullptr;
-        stage_host = nullptr;
+        stage_host = stage_dev = nullptr;
         size_t w = words + (words >> 2) + 1024;
-        hip_check(hipMalloc(reinterpret_cast<void**>(&stage_dev), w * 4));
-        hip_check(hipHostMalloc(reinterpret_cast<void**>(&stage_host), w * 4, hipHostMallocDefault));
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&stage_host), w * 4,
+                                hipHostMallocMapped | hipHostMallocCoherent));
+        hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&stage_dev), stage_host, 0));
         stage_words = w;
+    }
+
+    // Waits until the flag reaches `s` (spinning: the kernels are short and
+    // the host transcript step is on the critical path). The stream is
+    // queried now and then so a device failure surfaces as an error.
+    void wait_flag(uint64_t s) {
+        const volatile uint64_t* f = flag_host;
+        for (uint64_t spins = 1;; ++spins) {
+            if (__atomic_load_n(const_cast<const uint64_t*>(f), __ATOMIC_ACQUIRE) >= s) break;
+            if ((spins & 1023) == 0) {
+                hipError_t e = hipStreamQuery(stream);
+                if (e == hipSuccess) {
+                    if (__atomic_load_n(const_cast<const uint64_t*>(f), __ATOMIC_ACQUIRE) >= s) break;
+                    throw Status(Z0P_ERR_DEVICE);  // stream idle, flag never written
+                }
+                if (e != hipErrorNotReady) hip_check(e);
+            }
+        }
+        if (prof.enabled) {
+            hip_check(hipStreamSynchronize(stream));
+            prof.resolve();
+        }
+        ring.reset();
     }
 
     void reserve_scratch(size_t bytes) {

@@ -529,10 +529,17 @@ __global__ __launch_bounds__(WG) void k_grind(GrindJob inl, const GrindJob* tab,
 }
 
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(WG) void k_collect(CollectItem inl, const CollectItem* tab, uint32_t njobs) {
-    uint32_t local;
-    const CollectItem J = job_of(inl, tab, njobs, local);
-    for (uint32_t k = threadIdx.x; k < J.words; k += WG) J.dst[k] = J.src[k];
+constexpr int COLLECT_WG = 1024;
+
+__global__ __launch_bounds__(COLLECT_WG) void k_collect(CollectItem inl, const CollectItem* tab, uint32_t nitems,
+                                                        uint64_t* flag, uint64_t seq) {
+    for (uint32_t i = 0; i < nitems; ++i) {
+        const CollectItem J = nitems == 1 ? inl : tab[i];
+        for (uint32_t k = threadIdx.x; k < J.words; k += COLLECT_WG) J.dst[k] = J.src[k];
+    }
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0 && flag) __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Plain streaming copy: the calibration reference for FETCH/WRITE_SIZE.
@@ -757,9 +764,13 @@ void launch_bitrev(const Launch& L, std::vector<BitrevJob> jobs) {
     run(L, "bitrev", k_bitrev, jobs, g, 0);
 }
 
-void launch_collect(const Launch& L, std::vector<CollectItem> items) {
-    unsigned g = assign_blocks(items, [](const CollectItem&) { return uint64_t(1); });
-    run(L, "collect", k_collect, items, g, 0);
+void launch_collect(const Launch& L, std::vector<CollectItem> items, uint64_t* flag, uint64_t seq) {
+    const CollectItem* table = table_of(L, items);
+    if (L.dry) return;
+    const CollectItem inl = items.empty() ? CollectItem{nullptr, nullptr, 0, 0} : items[0];
+    ProfScope ps(L.prof, "collect", L.stream);
+    hipLaunchKernelGGL(k_collect, dim3(1), dim3(COLLECT_WG), 0, L.stream, inl, table, uint32_t(items.size()), flag,
+                       seq);
 }
 
 void launch_copy(const Launch& L, const uint32_t* in, uint32_t* out, uint64_t words) {
@@ -774,18 +785,51 @@ void launch_copy(const Launch& L, const uint32_t* in, uint32_t* out, uint64_t wo
 namespace z0p {
 namespace {
 
-__device__ __forceinline__ void copy_words(uint32_t* dst, const uint32_t* src, uint32_t words) {
-    for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) dst[k] = src[k];
+// Word offset of level l in a level-contiguous tree of L = 2^depth leaves
+// (l <= depth): 8 * (L + L/2 + ... + L/2^(l-1)).
+__device__ __forceinline__ uint64_t level_offset_words(uint64_t L, int l) { return 16 * (L - (L >> l)); }
+
+// Source of output word w of one query's opening (layout in kernels.hpp).
+__device__ const uint32_t* open_src(const OpenJob& J, uint64_t leaf, uint32_t w) {
+    uint64_t index = leaf;
+    for (int d = 0; d < J.nlayers; ++d) {
+        const uint64_t base = index / 4 * 4;
+        const uint32_t vals = J.base[d] ? 4u : 16u;
+        if (w < vals) return J.base[d] ? J.layer[d] + base + w : J.layer[d] + base * 4 + w;
+        w -= vals;
+        const uint32_t pw = uint32_t(J.target[d]) * 8u;
+        if (w < pw) {
+            const int l = int(w >> 3);
+            const uint64_t sib = ((base / 4) >> l) ^ 1;
+            return J.tree[d] + level_offset_words(J.tree_leaves[d], l) + sib * 8 + (w & 7);
+        }
+        w -= pw;
+        index /= 4;
+    }
+    // AIR trace rows: leaves of natural rows idx and idx + B, then their paths.
+    const int logm = J.logn + J.logb;
+    const uint64_t m = uint64_t(1) << logm, B = uint64_t(1) << J.logb;
+    const uint64_t idx = brev_bits(uint32_t(leaf), logm);
+    const uint64_t shifted = (idx + B) & (m - 1);
+    if (w < 32) {
+        const uint64_t row0 = ((w >> 4) ? shifted : idx) & ~uint64_t(7);
+        const uint32_t k = w & 15;
+        const uint64_t i = row0 + (k >> 1);
+        const uint32_t j = uint32_t(i & (B - 1)), t = uint32_t(i >> J.logb);
+        const uint64_t pos = (uint64_t(brev_bits(j, J.logb)) << J.logn) | brev_bits(t, J.logn);
+        return ((k & 1) ? J.b : J.a) + pos;
+    }
+    w -= 32;
+    const uint32_t pw = uint32_t(J.trace_target) * 8u;  // > 0 whenever words remain
+    const uint64_t leaf8 = (w >= pw ? shifted : idx) / 8;
+    if (w >= pw) w -= pw;
+    const int l = int(w >> 3);
+    const uint64_t sib = (leaf8 >> l) ^ 1;
+    return J.trace_tree + level_offset_words(J.trace_leaves, l) + sib * 8 + (w & 7);
 }
 
-__device__ __forceinline__ uint64_t level_offset_words(uint64_t leaves, int l) {
-    uint64_t d = 0;
-    for (int k = 0; k < l; ++k) d += leaves >> k;
-    return d * 8;
-}
-
-// The descriptor table always lives in device memory here (it is large and
-// indexed by layer), so the workgroup reads its entry in place.
+// One workgroup per query, one thread per output word (all loads
+// independent). The output is host memory: fence before finishing.
 __global__ __launch_bounds__(WG) void k_open(const OpenJob* tab, uint32_t njobs) {
     uint32_t lo = 0, hi = njobs;
     while (hi - lo > 1) {
@@ -795,50 +839,12 @@ __global__ __launch_bounds__(WG) void k_open(const OpenJob* tab, uint32_t njobs)
     }
     const OpenJob& J = tab[lo];
     const uint32_t q = blockIdx.x - J.block_begin;
-    if (q >= J.nq) return;
-    const uint64_t leaf = J.queries[q];
-    uint32_t* o = J.out + uint64_t(q) * J.words_per_query;
-    uint64_t index = leaf;
-    for (int d = 0; d < J.nlayers; ++d) {
-        const uint64_t base = index / 4 * 4;
-        if (J.base[d]) {
-            copy_words(o, J.layer[d] + base, 4);
-            o += 4;
-        } else {
-            copy_words(o, J.layer[d] + base * 4, 16);
-            o += 16;
-        }
-        const uint64_t qi = base / 4;
-        for (int l = 0; l < J.target[d]; ++l) {
-            const uint64_t sib = (qi >> l) ^ 1;
-            copy_words(o, J.tree[d] + level_offset_words(J.tree_leaves[d], l) + sib * 8, 8);
-            o += 8;
-        }
-        index /= 4;
+    if (q < J.nq) {
+        const uint64_t leaf = J.queries[q];
+        uint32_t* o = J.out + uint64_t(q) * J.words_per_query;
+        for (uint32_t w = threadIdx.x; w < J.words_per_query; w += WG) o[w] = *open_src(J, leaf, w);
     }
-    if (J.a == nullptr) return;
-    const int logm = J.logn + J.logb;
-    const uint64_t m = uint64_t(1) << logm, B = uint64_t(1) << J.logb;
-    const uint64_t idx = brev_bits(uint32_t(leaf), logm);
-    const uint64_t shifted = (idx + B) & (m - 1);
-    for (int side = 0; side < 2; ++side) {
-        const uint64_t row0 = (side == 0 ? idx : shifted) & ~uint64_t(7);
-        for (uint32_t k = threadIdx.x; k < 16; k += blockDim.x) {
-            const uint64_t i = row0 + (k >> 1);
-            const uint32_t j = uint32_t(i & (B - 1)), t = uint32_t(i >> J.logb);
-            const uint64_t pos = (uint64_t(brev_bits(j, J.logb)) << J.logn) | brev_bits(t, J.logn);
-            o[k] = (k & 1) ? J.b[pos] : J.a[pos];
-        }
-        o += 16;
-    }
-    for (int side = 0; side < 2; ++side) {
-        const uint64_t leaf8 = (side == 0 ? idx : shifted) / 8;
-        for (int l = 0; l < J.trace_target; ++l) {
-            const uint64_t sib = (leaf8 >> l) ^ 1;
-            copy_words(o, J.trace_tree + level_offset_words(J.trace_leaves, l) + sib * 8, 8);
-            o += 8;
-        }
-    }
+    __threadfence_system();
 }
 
 }  // namespace

@@ -188,15 +188,17 @@ struct BitrevJob {
     uint32_t block_begin;
 };
 void launch_bitrev(const Launch& L, std::vector<BitrevJob> jobs);
-// Device-to-device copies of small regions into one staging buffer, one
-// workgroup per item (caps, final layers, nonces of many proofs at once).
+// Copies of small regions (caps, final layers, nonces of many proofs) into
+// the host-visible staging area, by one workgroup, which then stores `seq`
+// to `flag` (if not null) after a system-scope fence: the host waits on the
+// flag instead of synchronising the stream.
 struct CollectItem {
     const uint32_t* src;
     uint32_t* dst;
     uint32_t words;
     uint32_t block_begin;
 };
-void launch_collect(const Launch& L, std::vector<CollectItem> items);
+void launch_collect(const Launch& L, std::vector<CollectItem> items, uint64_t* flag, uint64_t seq);
 void launch_copy(const Launch& L, const uint32_t* in, uint32_t* out, uint64_t words);
 
 // NTT job builders: the passes of one inverse / coset LDE. pass2 is empty
@@ -217,10 +219,12 @@ constexpr uint64_t TAIL_MAX_COUNT = uint64_t(1) << 13;
 
 namespace z0p {
 
-// Openings of one proof: a workgroup per query copies the quad values and
-// path siblings of every opened FRI layer and, for AIR proofs, the two
-// trace leaves and their paths, to out + q * words_per_query in the fixed
-// order the host parses (see prover.cpp: open_words).
+// Openings of one proof, query q written to out + q * words_per_query:
+// for each FRI layer d, the opened quad (16 words; 4 for a base-field layer)
+// then target[d] sibling digests (8 words each, leaf level first); for AIR
+// proofs then the two 16-word trace leaves of natural rows idx and idx + B
+// (8 rows of A||B each) and their trace_target-step paths, where idx is the
+// bit-reversed query index (air.rs:567-591).
 constexpr int MAX_OPEN_LAYERS = 14;
 struct OpenJob {
     const uint32_t* layer[MAX_OPEN_LAYERS];

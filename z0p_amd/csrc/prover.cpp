@@ -149,8 +149,9 @@ const GeometryTables& Ctx::geometry(int logn, int logb) {
 
 Ctx::~Ctx() {
     tables.clear();
-    if (stage_dev) (void)hipFree(stage_dev);
+    if (stream) (void)hipStreamSynchronize(stream);
     if (stage_host) (void)hipHostFree(stage_host);
+    if (flag_host) (void)hipHostFree(flag_host);
     if (scratch) (void)hipFree(scratch);
     if (tw4096) (void)hipFree(tw4096);
     if (tw4096_inv) (void)hipFree(tw4096_inv);
@@ -173,6 +174,9 @@ Ctx* ctx_create(int device) {
     hip_check(hipMemcpy(c->tw4096_inv, inv.data(), 2048 * 4, hipMemcpyHostToDevice));
     c->ring.init(size_t(32) << 20);
     c->reserve_stage(size_t(1) << 16);
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->flag_host), 256, hipHostMallocMapped | hipHostMallocCoherent));
+    hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->flag_dev), c->flag_host, 0));
+    *c->flag_host = 0;
     return c.release();
 }
 
@@ -256,7 +260,7 @@ struct Buckets {
 }  // namespace
 
 void launch_chains(const Launch& L, const std::vector<const std::vector<Step>*>& chains,
-                   const std::vector<CollectItem>& collect) {
+                   const std::vector<CollectItem>& collect, uint64_t* flag, uint64_t seq) {
     size_t rounds = 0;
     for (auto* c : chains) rounds = std::max(rounds, c->size());
     for (size_t r = 0; r < rounds; ++r) {
@@ -265,7 +269,7 @@ void launch_chains(const Launch& L, const std::vector<const std::vector<Step>*>&
             if (r < c->size()) b.add((*c)[r]);
         b.launch(L);
     }
-    if (!collect.empty()) launch_collect(L, collect);
+    if (flag || !collect.empty()) launch_collect(L, collect, flag, seq);
 }
 
 // ---------------------------------------------------------------------------
@@ -923,10 +927,9 @@ bool Engine::iterate() {
     for (size_t i = 0; i < active_.size(); ++i) build(c_, *active_[i], chains[i], items, used);
     std::vector<const std::vector<Step>*> ptrs;
     for (auto& ch : chains) ptrs.push_back(&ch);
-    c_.enqueue([&](const Launch& L) { launch_chains(L, ptrs, items); });
-    if (used)
-        hip_check(hipMemcpyAsync(c_.stage_host, c_.stage_dev, used * 4, hipMemcpyDeviceToHost, c_.stream));
-    c_.sync();
+    const uint64_t seq = ++c_.seq;
+    c_.enqueue([&](const Launch& L) { launch_chains(L, ptrs, items, c_.flag_dev, seq); });
+    c_.wait_flag(seq);
     ++iterations_;
     std::vector<Proof*> still;
     for (Proof* p : active_) {
