@@ -21,7 +21,7 @@ def _check(rc):
     assert rc == 0, _ffi.ERROR_NAMES.get(rc, rc)
 
 
-@pytest.mark.parametrize("logn", [1, 4, 10, 12, 13, 16])
+@pytest.mark.parametrize("logn", [1, 4, 10, 11, 12, 13, 15, 16, 17, 18, 20])
 def test_intt_bit_reversed(ctx, logn):
     rng = np.random.default_rng(100 + logn)
     n = 1 << logn
@@ -33,7 +33,7 @@ def test_intt_bit_reversed(ctx, logn):
     np.testing.assert_array_equal(out, coeffs[brev_perm(logn)])
 
 
-@pytest.mark.parametrize("logn,logb", [(1, 2), (3, 4), (10, 4), (12, 3), (13, 4), (14, 2), (16, 4)])
+@pytest.mark.parametrize("logn,logb", [(1, 2), (3, 4), (10, 4), (11, 4), (12, 3), (13, 4), (14, 2), (15, 1), (16, 4), (17, 3), (19, 2)])
 def test_lde_bit_reversed(ctx, logn, logb):
     rng = np.random.default_rng(200 + 16 * logn + logb)
     n, B = 1 << logn, 1 << logb

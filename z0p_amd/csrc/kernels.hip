@@ -70,7 +70,219 @@ __global__ __launch_bounds__(WG) void k_fib(FibJob inl, const FibJob* tab, uint3
 //   cols (over c): X[u + n2 v] at block_j*n + brev(u)*n1 + brev(v)
 constexpr int COL_LDS_WORDS = 8192;
 
+// ---- 4096-point transforms: 256 threads x 16 registers -------------------
+// Three radix-16 passes of four radix-2 stages each, with two LDS
+// exchanges instead of twelve barriers. Element index i of the transform:
+//   pass 0: thread q holds i = q + 256 t          (stage distances 256..2048)
+//   pass 1: thread (blk = q >> 4, b = q & 15) holds i = 256 blk + b + 16 t
+//   pass 2: thread q holds i = 16 q + t           (stage distances 1..8)
+// A stage of distance h pairs (i, i + h) with twiddle w_4096^(j * 2048 / h),
+// j = i mod h; DIF runs passes 0, 1, 2 with h falling (natural in,
+// bit-reversed out), DIT runs 2, 1, 0 with h rising (bit-reversed in,
+// natural out). LDS rows are padded by one word per 16.
+constexpr int N4K = 4096;
+constexpr int N4K_LDS = N4K + N4K / 16;
+
+__device__ __forceinline__ int pad16(int i) { return i + (i >> 4); }
+
+template <bool DIT>
+__device__ __forceinline__ void bfly(uint32_t& a, uint32_t& b, uint32_t w) {
+    if (DIT) {
+        const uint32_t v = mont_mul(b, w);
+        b = sub(a, v);
+        a = add(a, v);
+    } else {
+        const uint32_t u = a;
+        a = add(u, b);
+        b = mont_mul(sub(u, b), w);
+    }
+}
+
+// One stage of register distance D within pass P; q is the thread index.
+template <bool DIT, int P, int D>
+__device__ __forceinline__ void stage16(uint32_t (&x)[16], const uint32_t* tw, uint32_t q) {
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        if (t & D) continue;
+        const uint32_t r = uint32_t(t & (D - 1));
+        if (P == 2 && r == 0) {  // twiddle w^0 = 1
+            const uint32_t u = x[t];
+            x[t] = add(u, x[t + D]);
+            x[t + D] = sub(u, x[t + D]);
+            continue;
+        }
+        uint32_t e;
+        if (P == 0) e = (q + 256u * r) * (8u / D);
+        else if (P == 1) e = ((q & 15u) + 16u * r) * (128u / D);
+        else e = r * (2048u / D);
+        bfly<DIT>(x[t], x[t + D], tw[e]);
+    }
+}
+
+template <bool DIT, int P>
+__device__ __forceinline__ void pass16(uint32_t (&x)[16], const uint32_t* tw, uint32_t q) {
+    if (DIT) {
+        stage16<DIT, P, 1>(x, tw, q);
+        stage16<DIT, P, 2>(x, tw, q);
+        stage16<DIT, P, 4>(x, tw, q);
+        stage16<DIT, P, 8>(x, tw, q);
+    } else {
+        stage16<DIT, P, 8>(x, tw, q);
+        stage16<DIT, P, 4>(x, tw, q);
+        stage16<DIT, P, 2>(x, tw, q);
+        stage16<DIT, P, 1>(x, tw, q);
+    }
+}
+
+template <int P>
+__device__ __forceinline__ int idx16(uint32_t q, int t) {
+    if (P == 0) return int(q) + 256 * t;
+    if (P == 1) return int(q >> 4) * 256 + int(q & 15) + 16 * t;
+    return 16 * int(q) + t;
+}
+
+template <int P>
+__device__ __forceinline__ void lds_put(uint32_t* s, const uint32_t (&x)[16], uint32_t q) {
+#pragma unroll
+    for (int t = 0; t < 16; ++t) s[pad16(idx16<P>(q, t))] = x[t];
+}
+template <int P>
+__device__ __forceinline__ void lds_get(const uint32_t* s, uint32_t (&x)[16], uint32_t q) {
+#pragma unroll
+    for (int t = 0; t < 16; ++t) x[t] = s[pad16(idx16<P>(q, t))];
+}
+
+// DIF over x in pass-0 layout; returns with x in pass-2 layout (bit-reversed
+// positions 16 q + t).
+__device__ __forceinline__ void dif4096(uint32_t (&x)[16], uint32_t* s, const uint32_t* tw) {
+    const uint32_t q = threadIdx.x;
+    pass16<false, 0>(x, tw, q);
+    lds_put<0>(s, x, q);
+    __syncthreads();
+    lds_get<1>(s, x, q);
+    pass16<false, 1>(x, tw, q);
+    lds_put<1>(s, x, q);
+    __syncthreads();
+    lds_get<2>(s, x, q);
+    pass16<false, 2>(x, tw, q);
+}
+
+// DIT over x in pass-2 layout (bit-reversed input positions 16 q + t);
+// returns with x in pass-0 layout (natural i = q + 256 t).
+__device__ __forceinline__ void dit4096(uint32_t (&x)[16], uint32_t* s, const uint32_t* tw) {
+    const uint32_t q = threadIdx.x;
+    pass16<true, 2>(x, tw, q);
+    lds_put<2>(s, x, q);
+    __syncthreads();
+    lds_get<1>(s, x, q);
+    pass16<true, 1>(x, tw, q);
+    lds_put<1>(s, x, q);
+    __syncthreads();
+    lds_get<0>(s, x, q);
+    pass16<true, 0>(x, tw, q);
+}
+
+// Inverse of one 4096-point row (natural evaluations at `in`), scaled by
+// 1/n, bit-reversed coefficients to `out`.
+__device__ void intt4096_row(const uint32_t* in, uint32_t* out, const uint32_t* tw, uint32_t ninv_m, uint32_t* s) {
+    const uint32_t q = threadIdx.x;
+    uint32_t x[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) x[t] = in[q + 256 * t];
+    dif4096(x, s, tw);
+    __syncthreads();  // everyone has read pass-2 inputs before they are overwritten
+#pragma unroll
+    for (int t = 0; t < 16; ++t) s[pad16(16 * int(q) + t)] = mont_mul(x[t], ninv_m);
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 16; ++t) out[q + 256 * t] = s[pad16(int(q) + 256 * t)];
+}
+
+// ---- column transforms: C = 4096 / N1 columns of N1 = 2^L points --------
+// The tile sits in LDS row-major with pitch C + 1 (element i of column c at
+// i * (C + 1) + c): global rows load and store coalesced, and a read down a
+// column has odd stride. Thread (c = tid mod C, ti = tid / C) owns 16/G
+// groups of G = 2^R elements i = blk * S * G + base + S * t' for a pass of R
+// stages with distances h = S * 2^k; twiddle exponent j * 2048 / h in
+// w_4096 units with j = base + S * (t' mod 2^k).
+template <bool DIT, int R>
+__device__ __forceinline__ void cols_pass(uint32_t* s, int lS, int logC, const uint32_t* tw) {
+    constexpr int G = 1 << R, NG = 16 / G;
+    const int tid = threadIdx.x;
+    const int C = 1 << logC, pitch = C + 1;
+    const int c = tid & (C - 1), ti = tid >> logC;
+    const int S = 1 << lS;
+    uint32_t x[16];
+    int i0[NG];
+    uint32_t base[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        const int gid = ti * NG + g;
+        base[g] = uint32_t(gid & (S - 1));
+        i0[g] = ((gid >> lS) << (lS + R)) + int(base[g]);
+#pragma unroll
+        for (int t = 0; t < G; ++t) x[g * G + t] = s[(i0[g] + (t << lS)) * pitch + c];
+    }
+#pragma unroll
+    for (int kk = 0; kk < R; ++kk) {
+        const int k = DIT ? kk : R - 1 - kk;
+        const int D = 1 << k;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+#pragma unroll
+            for (int t = 0; t < G; ++t) {
+                if (t & D) continue;
+                const uint32_t j = base[g] + (uint32_t(t & (D - 1)) << lS);
+                bfly<DIT>(x[g * G + t], x[g * G + t + D], tw[j << (11 - lS - k)]);
+            }
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int t = 0; t < G; ++t) s[(i0[g] + (t << lS)) * pitch + c] = x[g * G + t];
+}
+
+template <bool DIT>
+__device__ __forceinline__ void cols_partial(uint32_t* s, int r0, int logC, const uint32_t* tw) {
+    if (r0 == 1) cols_pass<DIT, 1>(s, 0, logC, tw);
+    else if (r0 == 2) cols_pass<DIT, 2>(s, 0, logC, tw);
+    else cols_pass<DIT, 3>(s, 0, logC, tw);
+}
+
+// Full transform of every column of the tile (4 <= L <= 12); the tile must
+// be complete in LDS on entry (barrier done) and is complete on exit.
+template <bool DIT>
+__device__ __forceinline__ void cols_ntt(uint32_t* s, int L, int logC, const uint32_t* tw) {
+    const int r0 = L & 3;
+    if (DIT) {
+        if (r0) {
+            cols_partial<true>(s, r0, logC, tw);
+            __syncthreads();
+        }
+        for (int lS = r0; lS + 4 <= L; lS += 4) {
+            cols_pass<true, 4>(s, lS, logC, tw);
+            __syncthreads();
+        }
+    } else {
+        for (int lS = L - 4; lS >= r0; lS -= 4) {
+            cols_pass<false, 4>(s, lS, logC, tw);
+            __syncthreads();
+        }
+        if (r0) {
+            cols_partial<false>(s, r0, logC, tw);
+            __syncthreads();
+        }
+    }
+}
+
+__device__ __forceinline__ bool cols_in_regs(int log1) { return log1 >= 4 && log1 <= 12; }
+
 __device__ void ntt_intt_small(const NttJob& J, uint32_t* lds) {
+    if (J.logn == 12) {
+        intt4096_row(J.in, J.out, J.tw4096, J.ninv_m, lds);
+        return;
+    }
     const int n = 1 << J.logn;
     uint32_t* s = lds;
     uint32_t* tw = lds + n;
@@ -83,6 +295,26 @@ __device__ void ntt_intt_small(const NttJob& J, uint32_t* lds) {
 
 __device__ void ntt_lde_small(const NttJob& J, uint32_t* lds, uint32_t j) {
     const int n = 1 << J.logn;
+    if (J.logn == 12) {
+        // natural k = q + 256 t: x = c_k s_j^k (c_k at bit-reversed position)
+        const uint32_t q = threadIdx.x;
+        const uint32_t* sc = J.tab_a + size_t(j) * n;
+        uint32_t x[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const uint32_t k = q + 256u * t;
+            x[t] = mont_mul(J.in[brev_bits(k, 12)], sc[k]);
+        }
+        dif4096(x, lds, J.tw4096);
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 16; ++t) lds[pad16(16 * int(q) + t)] = x[t];
+        __syncthreads();
+        uint32_t* o = J.out + (size_t(brev_bits(j, J.logb)) << J.logn);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) o[q + 256 * t] = lds[pad16(int(q) + 256 * t)];
+        return;
+    }
     uint32_t* s = lds;
     uint32_t* tw = lds + n;
     load_tw(tw, J.tw4096, J.logn);
@@ -100,6 +332,27 @@ __device__ void ntt_lde_small(const NttJob& J, uint32_t* lds, uint32_t j) {
 // Column transforms use a padded LDS stride (n1 + 1) so the transposing
 // loads and stores hit distinct banks.
 __device__ void ntt_intt_cols(const NttJob& J, uint32_t* lds, uint32_t bx) {
+    if (cols_in_regs(J.log1)) {
+        const int L = J.log1, logC = 12 - L, C = 1 << logC, pitch = C + 1;
+        const int n2 = 1 << J.log2;
+        const int c0 = int(bx) * C;
+        const int tid = threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = tid + 256 * k, r = e >> logC, c = e & (C - 1);
+            lds[r * pitch + c] = J.in[size_t(r) * n2 + c0 + c];
+        }
+        __syncthreads();
+        cols_ntt<false>(lds, L, logC, J.tw4096);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = tid + 256 * k, qq = e >> logC, c = e & (C - 1);
+            const uint32_t u = brev_bits(uint32_t(qq), L);
+            const uint32_t cc = uint32_t(c0 + c);
+            J.out[size_t(u) * n2 + cc] = mont_mul(lds[qq * pitch + c], pow_lookup(J.pt, (u * cc) << J.logb));
+        }
+        return;
+    }
     const int n1 = 1 << J.log1, n2 = 1 << J.log2, cols = J.cols;
     const int st = n1 + 1;
     uint32_t* s = lds;
@@ -123,6 +376,10 @@ __device__ void ntt_intt_cols(const NttJob& J, uint32_t* lds, uint32_t bx) {
 
 __device__ void ntt_intt_rows(const NttJob& J, uint32_t* lds, uint32_t u) {
     const int n2 = 1 << J.log2;
+    if (J.log2 == 12) {
+        intt4096_row(J.in + size_t(u) * n2, J.out + size_t(brev_bits(u, J.log1)) * n2, J.tw4096, J.ninv_m, lds);
+        return;
+    }
     uint32_t* s = lds;
     uint32_t* tw = lds + n2;
     load_tw(tw, J.tw4096, J.log2);
@@ -139,6 +396,34 @@ __device__ void ntt_intt_rows(const NttJob& J, uint32_t* lds, uint32_t u) {
 __device__ void ntt_lde_rows(const NttJob& J, uint32_t* lds, uint32_t R, uint32_t j) {
     const int n1 = 1 << J.log1, n2 = 1 << J.log2;
     const int logn = J.log1 + J.log2;
+    if (J.log2 == 12) {
+        const uint32_t q = threadIdx.x;
+        const uint32_t c = brev_bits(R, J.log1);
+        const uint32_t* rt = J.tab_a + size_t(j) * n2;
+        const uint32_t* in = J.in + size_t(R) * n2;
+        // Coalesced load + row-table scale into LDS, then DIT from the
+        // bit-reversed positions.
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const int C = int(q) + 256 * t;
+            lds[pad16(C)] = mont_mul(in[C], rt[C]);
+        }
+        __syncthreads();
+        uint32_t x[16];
+        lds_get<2>(lds, x, q);
+        __syncthreads();  // pass-2 reads complete before dit4096 overwrites
+        dit4096(x, lds, J.tw4096);
+        // y[u] = x * s_j^c * w_n^(u c), u = q + 256 t: geometric in t.
+        uint32_t f = mont_mul(pow_lookup(J.pt, (q * c) << J.logb), J.tab_b[size_t(j) * n1 + c]);
+        const uint32_t step = pow_lookup(J.pt, (256u * c) << J.logb);
+        uint32_t* yo = J.out + (size_t(j) << logn) + size_t(c) * n2;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            yo[q + 256 * t] = mont_mul(x[t], f);
+            f = mont_mul(f, step);
+        }
+        return;
+    }
     uint32_t* s = lds;
     uint32_t* tw = lds + n2;
     load_tw(tw, J.tw4096, J.log2);
@@ -159,6 +444,26 @@ __device__ void ntt_lde_rows(const NttJob& J, uint32_t* lds, uint32_t R, uint32_
 __device__ void ntt_lde_cols(const NttJob& J, uint32_t* lds, uint32_t bx, uint32_t j) {
     const int n1 = 1 << J.log1, n2 = 1 << J.log2, cols = J.cols;
     const int logn = J.log1 + J.log2;
+    if (cols_in_regs(J.log1)) {
+        const int L = J.log1, logC = 12 - L, C = 1 << logC, pitch = C + 1;
+        const int u0 = int(bx) * C;
+        const int tid = threadIdx.x;
+        const uint32_t* yj = J.in + (size_t(j) << logn);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = tid + 256 * k, r = e >> logC, c = e & (C - 1);
+            lds[r * pitch + c] = yj[size_t(r) * n2 + u0 + c];
+        }
+        __syncthreads();
+        cols_ntt<false>(lds, L, logC, J.tw4096);
+        uint32_t* oj = J.out + (size_t(brev_bits(j, J.logb)) << logn);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = tid + 256 * k, qq = e & (n1 - 1), uu = e >> L;
+            oj[size_t(brev_bits(uint32_t(u0 + uu), J.log2)) * n1 + qq] = lds[qq * pitch + uu];
+        }
+        return;
+    }
     const int st = n1 + 1;
     uint32_t* s = lds;
     uint32_t* tw = lds + cols * st;
@@ -579,6 +884,7 @@ void run(const Launch& L, const char* name, K kernel, std::vector<J>& jobs, unsi
 }
 
 int ntt_cols_for(int log1, int log2) {
+    if (log1 >= 4 && log1 <= 12) return N4K >> log1;  // register path: 4096-element tiles
     int cols = COL_LDS_WORDS >> log1;
     return std::min(cols, 1 << log2);
 }
@@ -599,11 +905,13 @@ uint64_t ntt_blocks(const NttJob& j) {
 size_t ntt_lds(const NttJob& j) {
     switch (j.kind) {
         case NTT_INTT_SMALL:
-        case NTT_LDE_SMALL: return (size_t(1) << j.logn) * 6;
+        case NTT_LDE_SMALL: return j.logn == 12 ? size_t(N4K_LDS) * 4 : (size_t(1) << j.logn) * 6;
         case NTT_INTT_ROWS:
-        case NTT_LDE_ROWS: return (size_t(1) << j.log2) * 6;
+        case NTT_LDE_ROWS: return j.log2 == 12 ? size_t(N4K_LDS) * 4 : (size_t(1) << j.log2) * 6;
         case NTT_INTT_COLS:
-        case NTT_LDE_COLS: return size_t(j.cols) * ((size_t(1) << j.log1) + 1) * 4 + (size_t(1) << j.log1) * 2;
+        case NTT_LDE_COLS:
+            if (j.log1 >= 4 && j.log1 <= 12) return ((size_t(1) << j.log1) + N4K) * 4;  // pitch C + 1
+            return size_t(j.cols) * ((size_t(1) << j.log1) + 1) * 4 + (size_t(1) << j.log1) * 2;
         default: return 0;
     }
 }
@@ -618,9 +926,14 @@ NttShape ntt_shape(int logn) {
     if (logn <= 12) {
         s.log1 = 0;
         s.log2 = logn;
-    } else {
+    } else if (logn <= 25) {
         s.log2 = 12;
         s.log1 = logn - 12;
+    } else {
+        // 2^26 (FRI at blowup 2): 8192-point rows keep the strided length
+        // at 2^13, one column per workgroup in LDS.
+        s.log2 = 13;
+        s.log1 = logn - 13;
     }
     return s;
 }
