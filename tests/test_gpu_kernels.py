@@ -45,7 +45,7 @@ def test_lde_bit_reversed(ctx, logn, logb):
     np.testing.assert_array_equal(out, lde[brev_perm(logn + logb)])
 
 
-@pytest.mark.parametrize("leaves", [1, 2, 8, 512, 1024, 8192])
+@pytest.mark.parametrize("leaves", [1, 2, 8, 512, 1024, 8192, 1 << 15, 1 << 18])
 def test_merkle64_all_levels(ctx, leaves):
     rng = np.random.default_rng(300 + leaves)
     flat = rng.integers(0, 256, size=64 * leaves, dtype=np.uint8).tobytes()
@@ -62,7 +62,7 @@ def _packed_rows(a, b):
     return rows.tobytes()
 
 
-@pytest.mark.parametrize("logn,logb", [(1, 2), (2, 3), (8, 4), (12, 4), (13, 2), (14, 3)])
+@pytest.mark.parametrize("logn,logb", [(1, 2), (2, 3), (8, 4), (12, 4), (13, 2), (14, 3), (16, 4)])
 def test_trace_tree_all_levels(ctx, logn, logb):
     rng = np.random.default_rng(400 + 16 * logn + logb)
     n, B = 1 << logn, 1 << logb

@@ -13,12 +13,30 @@ namespace dev {
 
 constexpr int WG = 256;
 
+// Descriptor pointers are generic; these casts state that they address
+// global memory, so the compiler emits global (not flat) accesses and does
+// not have to order them against LDS traffic.
+#define Z0P_GLOBAL __attribute__((address_space(1)))
+typedef const Z0P_GLOBAL uint32_t* gcptr;
+typedef Z0P_GLOBAL uint32_t* gptr;
+__device__ __forceinline__ gcptr gc(const uint32_t* p) { return (gcptr)p; }
+__device__ __forceinline__ gptr gm(uint32_t* p) { return (gptr)p; }
+// 16-byte global accesses (native vector type: HIP's uint4 has no
+// address-space-qualified operators).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ u32x4 ld4(const uint32_t* p, uint64_t i) {
+    return reinterpret_cast<const Z0P_GLOBAL u32x4*>(gc(p))[i];
+}
+__device__ __forceinline__ void st4(uint32_t* p, uint64_t i, u32x4 v) {
+    reinterpret_cast<Z0P_GLOBAL u32x4*>(gm(p))[i] = v;
+}
+
 __device__ __forceinline__ uint32_t brev_bits(uint32_t x, int bits) {
     return bits == 0 ? 0u : (__builtin_bitreverse32(x) >> (32 - bits));
 }
 
 __device__ __forceinline__ uint32_t pow_lookup(PowTable t, uint32_t e) {
-    return mont_mul(t.lo[e & 4095u], t.hi[e >> 12]);
+    return mont_mul(gc(t.lo)[e & 4095u], gc(t.hi)[e >> 12]);
 }
 
 // Montgomery-domain inverse: (a^-1)_m from a_m (zero maps to zero).
@@ -87,7 +105,8 @@ __device__ __forceinline__ void lds_dit(uint32_t* s, int logn, int cnt, int stri
 // tw[e] = w_n^e for e < n/2 from the 4096-point table.
 __device__ __forceinline__ void load_tw(uint32_t* tw, const uint32_t* tw4096, int logn) {
     const int half = logn == 0 ? 0 : 1 << (logn - 1);
-    for (int e = threadIdx.x; e < half; e += blockDim.x) tw[e] = tw4096[e << (12 - logn)];
+    const gcptr src = gc(tw4096);
+    for (int e = threadIdx.x; e < half; e += blockDim.x) tw[e] = src[e << (12 - logn)];
 }
 
 // Per-thread Montgomery batch inversion of K register values (nonzero on
@@ -128,13 +147,12 @@ __device__ __forceinline__ PosInfo pos_info(uint32_t p, int logn, int logb) {
 
 // x = s_j * omega_n^t (Montgomery form); omega_n = Omega_m^B.
 __device__ __forceinline__ uint32_t point_m(PowTable w_m, const uint32_t* shift_m, uint32_t j, uint32_t t, int logb) {
-    return mont_mul(pow_lookup(w_m, t << logb), shift_m[j]);
+    return mont_mul(pow_lookup(w_m, t << logb), gc(shift_m)[j]);
 }
 
 __device__ __forceinline__ void store_digest(uint32_t* dst, const uint32_t d[8]) {
-    uint4* g = reinterpret_cast<uint4*>(dst);
-    g[0] = make_uint4(d[0], d[1], d[2], d[3]);
-    g[1] = make_uint4(d[4], d[5], d[6], d[7]);
+    st4(dst, 0, u32x4{d[0], d[1], d[2], d[3]});
+    st4(dst, 1, u32x4{d[4], d[5], d[6], d[7]});
 }
 
 // Merkle subtree reduction in LDS. Digests word-major (SoA) so a thread

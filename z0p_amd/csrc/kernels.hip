@@ -100,7 +100,7 @@ __device__ __forceinline__ void bfly(uint32_t& a, uint32_t& b, uint32_t w) {
 
 // One stage of register distance D within pass P; q is the thread index.
 template <bool DIT, int P, int D>
-__device__ __forceinline__ void stage16(uint32_t (&x)[16], const uint32_t* tw, uint32_t q) {
+__device__ __forceinline__ void stage16(uint32_t (&x)[16], gcptr tw, uint32_t q) {
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
         if (t & D) continue;
@@ -120,7 +120,7 @@ __device__ __forceinline__ void stage16(uint32_t (&x)[16], const uint32_t* tw, u
 }
 
 template <bool DIT, int P>
-__device__ __forceinline__ void pass16(uint32_t (&x)[16], const uint32_t* tw, uint32_t q) {
+__device__ __forceinline__ void pass16(uint32_t (&x)[16], gcptr tw, uint32_t q) {
     if (DIT) {
         stage16<DIT, P, 1>(x, tw, q);
         stage16<DIT, P, 2>(x, tw, q);
@@ -154,7 +154,7 @@ __device__ __forceinline__ void lds_get(const uint32_t* s, uint32_t (&x)[16], ui
 
 // DIF over x in pass-0 layout; returns with x in pass-2 layout (bit-reversed
 // positions 16 q + t).
-__device__ __forceinline__ void dif4096(uint32_t (&x)[16], uint32_t* s, const uint32_t* tw) {
+__device__ __forceinline__ void dif4096(uint32_t (&x)[16], uint32_t* s, gcptr tw) {
     const uint32_t q = threadIdx.x;
     pass16<false, 0>(x, tw, q);
     lds_put<0>(s, x, q);
@@ -169,7 +169,7 @@ __device__ __forceinline__ void dif4096(uint32_t (&x)[16], uint32_t* s, const ui
 
 // DIT over x in pass-2 layout (bit-reversed input positions 16 q + t);
 // returns with x in pass-0 layout (natural i = q + 256 t).
-__device__ __forceinline__ void dit4096(uint32_t (&x)[16], uint32_t* s, const uint32_t* tw) {
+__device__ __forceinline__ void dit4096(uint32_t (&x)[16], uint32_t* s, gcptr tw) {
     const uint32_t q = threadIdx.x;
     pass16<true, 2>(x, tw, q);
     lds_put<2>(s, x, q);
@@ -184,7 +184,7 @@ __device__ __forceinline__ void dit4096(uint32_t (&x)[16], uint32_t* s, const ui
 
 // Inverse of one 4096-point row (natural evaluations at `in`), scaled by
 // 1/n, bit-reversed coefficients to `out`.
-__device__ void intt4096_row(const uint32_t* in, uint32_t* out, const uint32_t* tw, uint32_t ninv_m, uint32_t* s) {
+__device__ void intt4096_row(gcptr in, gptr out, gcptr tw, uint32_t ninv_m, uint32_t* s) {
     const uint32_t q = threadIdx.x;
     uint32_t x[16];
 #pragma unroll
@@ -206,7 +206,7 @@ __device__ void intt4096_row(const uint32_t* in, uint32_t* out, const uint32_t* 
 // stages with distances h = S * 2^k; twiddle exponent j * 2048 / h in
 // w_4096 units with j = base + S * (t' mod 2^k).
 template <bool DIT, int R>
-__device__ __forceinline__ void cols_pass(uint32_t* s, int lS, int logC, const uint32_t* tw) {
+__device__ __forceinline__ void cols_pass(uint32_t* s, int lS, int logC, gcptr tw) {
     constexpr int G = 1 << R, NG = 16 / G;
     const int tid = threadIdx.x;
     const int C = 1 << logC, pitch = C + 1;
@@ -244,7 +244,7 @@ __device__ __forceinline__ void cols_pass(uint32_t* s, int lS, int logC, const u
 }
 
 template <bool DIT>
-__device__ __forceinline__ void cols_partial(uint32_t* s, int r0, int logC, const uint32_t* tw) {
+__device__ __forceinline__ void cols_partial(uint32_t* s, int r0, int logC, gcptr tw) {
     if (r0 == 1) cols_pass<DIT, 1>(s, 0, logC, tw);
     else if (r0 == 2) cols_pass<DIT, 2>(s, 0, logC, tw);
     else cols_pass<DIT, 3>(s, 0, logC, tw);
@@ -253,7 +253,7 @@ __device__ __forceinline__ void cols_partial(uint32_t* s, int r0, int logC, cons
 // Full transform of every column of the tile (4 <= L <= 12); the tile must
 // be complete in LDS on entry (barrier done) and is complete on exit.
 template <bool DIT>
-__device__ __forceinline__ void cols_ntt(uint32_t* s, int L, int logC, const uint32_t* tw) {
+__device__ __forceinline__ void cols_ntt(uint32_t* s, int L, int logC, gcptr tw) {
     const int r0 = L & 3;
     if (DIT) {
         if (r0) {
@@ -280,7 +280,7 @@ __device__ __forceinline__ bool cols_in_regs(int log1) { return log1 >= 4 && log
 
 __device__ void ntt_intt_small(const NttJob& J, uint32_t* lds) {
     if (J.logn == 12) {
-        intt4096_row(J.in, J.out, J.tw4096, J.ninv_m, lds);
+        intt4096_row(gc(J.in), gm(J.out), gc(J.tw4096), J.ninv_m, lds);
         return;
     }
     const int n = 1 << J.logn;
@@ -298,19 +298,20 @@ __device__ void ntt_lde_small(const NttJob& J, uint32_t* lds, uint32_t j) {
     if (J.logn == 12) {
         // natural k = q + 256 t: x = c_k s_j^k (c_k at bit-reversed position)
         const uint32_t q = threadIdx.x;
-        const uint32_t* sc = J.tab_a + size_t(j) * n;
+        const gcptr sc = gc(J.tab_a) + size_t(j) * n;
+        const gcptr in = gc(J.in);
         uint32_t x[16];
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
             const uint32_t k = q + 256u * t;
-            x[t] = mont_mul(J.in[brev_bits(k, 12)], sc[k]);
+            x[t] = mont_mul(in[brev_bits(k, 12)], sc[k]);
         }
-        dif4096(x, lds, J.tw4096);
+        dif4096(x, lds, gc(J.tw4096));
         __syncthreads();
 #pragma unroll
         for (int t = 0; t < 16; ++t) lds[pad16(16 * int(q) + t)] = x[t];
         __syncthreads();
-        uint32_t* o = J.out + (size_t(brev_bits(j, J.logb)) << J.logn);
+        const gptr o = gm(J.out) + (size_t(brev_bits(j, J.logb)) << J.logn);
 #pragma unroll
         for (int t = 0; t < 16; ++t) o[q + 256 * t] = lds[pad16(int(q) + 256 * t)];
         return;
@@ -337,19 +338,27 @@ __device__ void ntt_intt_cols(const NttJob& J, uint32_t* lds, uint32_t bx) {
         const int n2 = 1 << J.log2;
         const int c0 = int(bx) * C;
         const int tid = threadIdx.x;
+        const gcptr in = gc(J.in);
+        const gptr out = gm(J.out);
+        uint32_t v[16];
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const int e = tid + 256 * k, r = e >> logC, c = e & (C - 1);
-            lds[r * pitch + c] = J.in[size_t(r) * n2 + c0 + c];
+            v[k] = in[size_t(r) * n2 + c0 + c];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = tid + 256 * k, r = e >> logC, c = e & (C - 1);
+            lds[r * pitch + c] = v[k];
         }
         __syncthreads();
-        cols_ntt<false>(lds, L, logC, J.tw4096);
+        cols_ntt<false>(lds, L, logC, gc(J.tw4096));
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const int e = tid + 256 * k, qq = e >> logC, c = e & (C - 1);
             const uint32_t u = brev_bits(uint32_t(qq), L);
             const uint32_t cc = uint32_t(c0 + c);
-            J.out[size_t(u) * n2 + cc] = mont_mul(lds[qq * pitch + c], pow_lookup(J.pt, (u * cc) << J.logb));
+            out[size_t(u) * n2 + cc] = mont_mul(lds[qq * pitch + c], pow_lookup(J.pt, (u * cc) << J.logb));
         }
         return;
     }
@@ -377,7 +386,8 @@ __device__ void ntt_intt_cols(const NttJob& J, uint32_t* lds, uint32_t bx) {
 __device__ void ntt_intt_rows(const NttJob& J, uint32_t* lds, uint32_t u) {
     const int n2 = 1 << J.log2;
     if (J.log2 == 12) {
-        intt4096_row(J.in + size_t(u) * n2, J.out + size_t(brev_bits(u, J.log1)) * n2, J.tw4096, J.ninv_m, lds);
+        intt4096_row(gc(J.in) + size_t(u) * n2, gm(J.out) + size_t(brev_bits(u, J.log1)) * n2, gc(J.tw4096), J.ninv_m,
+                     lds);
         return;
     }
     uint32_t* s = lds;
@@ -399,24 +409,26 @@ __device__ void ntt_lde_rows(const NttJob& J, uint32_t* lds, uint32_t R, uint32_
     if (J.log2 == 12) {
         const uint32_t q = threadIdx.x;
         const uint32_t c = brev_bits(R, J.log1);
-        const uint32_t* rt = J.tab_a + size_t(j) * n2;
-        const uint32_t* in = J.in + size_t(R) * n2;
+        const gcptr rt = gc(J.tab_a) + size_t(j) * n2;
+        const gcptr in = gc(J.in) + size_t(R) * n2;
         // Coalesced load + row-table scale into LDS, then DIT from the
         // bit-reversed positions.
+        uint32_t x[16];
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
             const int C = int(q) + 256 * t;
-            lds[pad16(C)] = mont_mul(in[C], rt[C]);
+            x[t] = mont_mul(in[C], rt[C]);
         }
+#pragma unroll
+        for (int t = 0; t < 16; ++t) lds[pad16(int(q) + 256 * t)] = x[t];
         __syncthreads();
-        uint32_t x[16];
         lds_get<2>(lds, x, q);
         __syncthreads();  // pass-2 reads complete before dit4096 overwrites
-        dit4096(x, lds, J.tw4096);
+        dit4096(x, lds, gc(J.tw4096));
         // y[u] = x * s_j^c * w_n^(u c), u = q + 256 t: geometric in t.
-        uint32_t f = mont_mul(pow_lookup(J.pt, (q * c) << J.logb), J.tab_b[size_t(j) * n1 + c]);
+        uint32_t f = mont_mul(pow_lookup(J.pt, (q * c) << J.logb), gc(J.tab_b)[size_t(j) * n1 + c]);
         const uint32_t step = pow_lookup(J.pt, (256u * c) << J.logb);
-        uint32_t* yo = J.out + (size_t(j) << logn) + size_t(c) * n2;
+        const gptr yo = gm(J.out) + (size_t(j) << logn) + size_t(c) * n2;
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
             yo[q + 256 * t] = mont_mul(x[t], f);
@@ -448,15 +460,21 @@ __device__ void ntt_lde_cols(const NttJob& J, uint32_t* lds, uint32_t bx, uint32
         const int L = J.log1, logC = 12 - L, C = 1 << logC, pitch = C + 1;
         const int u0 = int(bx) * C;
         const int tid = threadIdx.x;
-        const uint32_t* yj = J.in + (size_t(j) << logn);
+        const gcptr yj = gc(J.in) + (size_t(j) << logn);
+        uint32_t v[16];
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const int e = tid + 256 * k, r = e >> logC, c = e & (C - 1);
-            lds[r * pitch + c] = yj[size_t(r) * n2 + u0 + c];
+            v[k] = yj[size_t(r) * n2 + u0 + c];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = tid + 256 * k, r = e >> logC, c = e & (C - 1);
+            lds[r * pitch + c] = v[k];
         }
         __syncthreads();
-        cols_ntt<false>(lds, L, logC, J.tw4096);
-        uint32_t* oj = J.out + (size_t(brev_bits(j, J.logb)) << logn);
+        cols_ntt<false>(lds, L, logC, gc(J.tw4096));
+        const gptr oj = gm(J.out) + (size_t(brev_bits(j, J.logb)) << logn);
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const int e = tid + 256 * k, qq = e & (n1 - 1), uu = e >> L;
@@ -552,7 +570,7 @@ __global__ __launch_bounds__(WG) void k_linked(LinkedJob inl, const LinkedJob* t
 #pragma unroll
     for (int e = 0; e < ELT; ++e) {
         uint32_t p = base + e * WG;
-        if (p < m) J.b[p] = add(J.a[pi[e].pnext], mont_mul(den[e], mont_mul(J.coeff, J.zvan_m[pi[e].j])));
+        if (p < m) gm(J.b)[p] = add(gc(J.a)[pi[e].pnext], mont_mul(den[e], mont_mul(J.coeff, gc(J.zvan_m)[pi[e].j])));
     }
 }
 
@@ -579,7 +597,7 @@ __global__ __launch_bounds__(WG) void k_compose(ComposeJob inl, const ComposeJob
         }
     }
     batch_inv_m<ELT>(pr);
-    uint4* out = reinterpret_cast<uint4*>(J.layer0);
+    const gcptr A = gc(J.a), Bv = gc(J.b);
 #pragma unroll
     for (int e = 0; e < ELT; ++e) {
         uint32_t p = base + e * WG;
@@ -587,8 +605,8 @@ __global__ __launch_bounds__(WG) void k_compose(ComposeJob inl, const ComposeJob
         PosInfo pi = pos_info(p, J.logn, J.logb);
         uint32_t i1 = mont_mul(dw[e], pr[e]);
         uint32_t iw = mont_mul(d1[e], pr[e]);
-        uint32_t iz = mont_mul(dw[e], J.zinv_m[pi.j]);
-        uint32_t va = J.a[p], vb = J.b[p], na = J.a[pi.pnext], nb = J.b[pi.pnext];
+        uint32_t iz = mont_mul(dw[e], gc(J.zinv_m)[pi.j]);
+        uint32_t va = A[p], vb = Bv[p], na = A[pi.pnext], nb = Bv[pi.pnext];
         uint32_t s0 = mont_mul(sub(na, vb), iz);
         uint32_t s1 = mont_mul(sub(nb, add(va, vb)), iz);
         uint32_t s2 = mont_mul(sub(va, J.c.a0), i1);
@@ -604,33 +622,52 @@ __global__ __launch_bounds__(WG) void k_compose(ComposeJob inl, const ComposeJob
             acc = add(acc, mont_mul(s4, J.c.alpha_m[4][k]));
             o[k] = acc;
         }
-        out[p] = make_uint4(o[0], o[1], o[2], o[3]);
+        st4(J.layer0, p, u32x4{o[0], o[1], o[2], o[3]});
     }
 }
 
 // ---------------------------------------------------------------------------
 // Merkle leaves: thread g hashes leaves 2g and 2g+1 and, when asked, their
 // parent (one node of level 1). No LDS: wide levels run at full occupancy.
-__device__ __forceinline__ void leaf_message(const LeavesJob& J, uint64_t leaf, uint32_t m[16]) {
-    if (J.kind == LEAF_BASE) {
-        uint4 v = reinterpret_cast<const uint4*>(J.src)[leaf];
-        m[0] = v.x; m[4] = v.y; m[8] = v.z; m[12] = v.w;
-        m[1] = m[2] = m[3] = m[5] = m[6] = m[7] = 0;
-        m[9] = m[10] = m[11] = m[13] = m[14] = m[15] = 0;
-        return;
-    }
-    // LEAF_EXT, LEAF_RAW: 64 contiguous bytes
-    const uint4* src = reinterpret_cast<const uint4*>(J.src) + leaf * 4;
+// Each leaf kind has its own fully unrolled body (static register indexing).
+template <int KIND>
+__device__ __forceinline__ void leaf_message(const LeavesJob& J, uint64_t leaf, uint32_t (&m)[16]) {
+    if (KIND == LEAF_BASE) {
+        const u32x4 v = ld4(J.src, leaf);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        uint4 v = src[k];
-        m[4 * k] = v.x; m[4 * k + 1] = v.y; m[4 * k + 2] = v.z; m[4 * k + 3] = v.w;
+        for (int k = 0; k < 16; ++k) m[k] = 0;
+        m[0] = v.x;
+        m[4] = v.y;
+        m[8] = v.z;
+        m[12] = v.w;
+    } else if (KIND == LEAF_TRACE) {
+        // Leaf L packs natural rows 8L..8L+7 as A||B LE words; row
+        // i = j + B t lives at brev(j)*n + brev(t).
+        const int logn = J.logn, logb = J.logb;
+        const uint32_t bmask = (1u << logb) - 1;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const uint64_t i = leaf * 8 + r;
+            const uint32_t j = uint32_t(i) & bmask;
+            const uint32_t t = uint32_t(i >> logb);
+            const uint64_t pos = (uint64_t(brev_bits(j, logb)) << logn) | brev_bits(t, logn);
+            m[2 * r] = gc(J.src)[pos];
+            m[2 * r + 1] = gc(J.src_b)[pos];
+        }
+    } else {  // LEAF_EXT, LEAF_RAW: 64 contiguous bytes
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const u32x4 v = ld4(J.src, leaf * 4 + k);
+            m[4 * k] = v.x;
+            m[4 * k + 1] = v.y;
+            m[4 * k + 2] = v.z;
+            m[4 * k + 3] = v.w;
+        }
     }
 }
 
-// Trace rows: leaf L packs natural rows 8L..8L+7 as A||B LE words; row
-// i = j + B t lives at brev(j)*n + brev(t). Thread tau is mapped to its leaf
-// pair so that lanes read consecutive positions of each coset block.
+// Trace leaves: thread tau is mapped to its leaf pair so that lanes read
+// consecutive positions of each coset block.
 __device__ __forceinline__ uint64_t trace_pair_of(const LeavesJob& J, uint64_t tau) {
     const int logn = J.logn, logb = J.logb;
     if (logb >= 4) {
@@ -642,43 +679,22 @@ __device__ __forceinline__ uint64_t trace_pair_of(const LeavesJob& J, uint64_t t
     return brev_bits(uint32_t(tau), logn - logs);
 }
 
-__device__ __forceinline__ void trace_leaf_message(const LeavesJob& J, uint64_t leaf, uint32_t m[16]) {
-    const int logn = J.logn, logb = J.logb;
-    const uint32_t bmask = (1u << logb) - 1;
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        uint64_t i = leaf * 8 + r;
-        uint32_t j = uint32_t(i) & bmask;
-        uint32_t t = uint32_t(i >> logb);
-        uint64_t pos = (uint64_t(brev_bits(j, logb)) << logn) | brev_bits(t, logn);
-        m[2 * r] = J.src[pos];
-        m[2 * r + 1] = J.src_b[pos];
-    }
-}
-
-__global__ __launch_bounds__(WG) void k_leaves(LeavesJob inl, const LeavesJob* tab, uint32_t njobs) {
-    uint32_t local;
-    const LeavesJob J = job_of(inl, tab, njobs, local);
-    const uint64_t tau = uint64_t(local) * WG + threadIdx.x;
-    const bool trace = J.kind == LEAF_TRACE;
+template <int KIND>
+__device__ __forceinline__ void leaves_body(const LeavesJob& J, uint64_t tau) {
+    uint32_t m[16], d0[8], d1[8];
     if (J.leaves == 1) {
         if (tau != 0) return;
-        uint32_t m[16], d[8];
-        if (trace) trace_leaf_message(J, 0, m);
-        else leaf_message(J, 0, m);
-        leaf64_words(m, d);
-        store_digest(J.levels, d);
+        leaf_message<KIND>(J, 0, m);
+        leaf64_words(m, d0);
+        store_digest(J.levels, d0);
         return;
     }
     const uint64_t pairs = J.leaves >> 1;
     if (tau >= pairs) return;
-    const uint64_t g = trace ? trace_pair_of(J, tau) : tau;
-    uint32_t m[16], d0[8], d1[8];
-    if (trace) trace_leaf_message(J, 2 * g, m);
-    else leaf_message(J, 2 * g, m);
+    const uint64_t g = KIND == LEAF_TRACE ? trace_pair_of(J, tau) : tau;
+    leaf_message<KIND>(J, 2 * g, m);
     leaf64_words(m, d0);
-    if (trace) trace_leaf_message(J, 2 * g + 1, m);
-    else leaf_message(J, 2 * g + 1, m);
+    leaf_message<KIND>(J, 2 * g + 1, m);
     leaf64_words(m, d1);
     store_digest(J.levels + (2 * g) * 8, d0);
     store_digest(J.levels + (2 * g + 1) * 8, d1);
@@ -689,9 +705,17 @@ __global__ __launch_bounds__(WG) void k_leaves(LeavesJob inl, const LeavesJob* t
     }
 }
 
+__global__ __launch_bounds__(WG) void k_leaves(LeavesJob inl, const LeavesJob* tab, uint32_t njobs) {
+    uint32_t local;
+    const LeavesJob J = job_of(inl, tab, njobs, local);
+    const uint64_t tau = uint64_t(local) * WG + threadIdx.x;
+    if (J.kind == LEAF_TRACE) leaves_body<LEAF_TRACE>(J, tau);
+    else if (J.kind == LEAF_BASE) leaves_body<LEAF_BASE>(J, tau);
+    else leaves_body<LEAF_EXT>(J, tau);
+}
+
 __device__ __forceinline__ void load_digest(const uint32_t* src, uint32_t d[8]) {
-    const uint4* s = reinterpret_cast<const uint4*>(src);
-    uint4 x = s[0], y = s[1];
+    const u32x4 x = ld4(src, 0), y = ld4(src, 1);
     d[0] = x.x; d[1] = x.y; d[2] = x.z; d[3] = x.w;
     d[4] = y.x; d[5] = y.y; d[6] = y.z; d[7] = y.w;
 }
@@ -728,22 +752,152 @@ __global__ __launch_bounds__(WG) void k_levels(LevelsJob inl, const LevelsJob* t
     store_digest(next + g * 8, p);
 }
 
-// Narrow tail: up to 9 levels per 512-digest group in LDS.
-__global__ __launch_bounds__(WG) void k_tail(TailJob inl, const TailJob* tab, uint32_t njobs) {
+// ---- quad-lane BLAKE3 for the narrow tree levels ------------------------
+// Four lanes of a quad compute one compression: lane q holds state column q
+// (v[q], v[4+q], v[8+q], v[12+q]) and runs the column G on it; for the
+// diagonal G it borrows v[4+(q+1)%4], v[8+(q+2)%4], v[12+(q+3)%4] from its
+// neighbours by DPP quad permutations and gives them back afterwards. The
+// chaining value enters and leaves as words (q, 4+q) of lane q. A narrow
+// level is latency-bound, and this cuts a compression's dependent
+// instruction chain to under half. Control flow must be uniform per quad.
+template <int CTRL>
+__device__ __forceinline__ uint32_t quad_dpp(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_mov_dpp(int(v), CTRL, 0xF, 0xF, true));
+}
+constexpr int QP_NEXT1 = 0x39;  // lane q reads lane (q + 1) & 3
+constexpr int QP_NEXT2 = 0x4E;  // lane q reads lane (q + 2) & 3
+constexpr int QP_NEXT3 = 0x93;  // lane q reads lane (q + 3) & 3
+
+__device__ __forceinline__ uint32_t sel4(bool q_odd, bool q_hi, uint32_t v0, uint32_t v1, uint32_t v2, uint32_t v3) {
+    const uint32_t lo = q_odd ? v1 : v0, hi = q_odd ? v3 : v2;
+    return q_hi ? hi : lo;
+}
+
+// Message word of slot (column: k = 0, 1 -> 2q + k; diagonal: k = 2, 3 ->
+// 8 + 2q + k - 2) in round r, picked from the 16 words every lane holds.
+template <int RND, int K>
+__device__ __forceinline__ uint32_t quad_msg(const uint32_t (&m)[16], bool q_odd, bool q_hi) {
+    constexpr B3Schedule S;
+    constexpr int s0 = K < 2 ? K : 8 + K - 2;
+    return sel4(q_odd, q_hi, m[S.idx[RND][s0]], m[S.idx[RND][s0 + 2]], m[S.idx[RND][s0 + 4]], m[S.idx[RND][s0 + 6]]);
+}
+
+#define Z0P_Q4_G(x, y)         \
+    a = a + b + (x);           \
+    d = rotr(d ^ a, 16);       \
+    c = c + d;                 \
+    b = rotr(b ^ c, 12);       \
+    a = a + b + (y);           \
+    d = rotr(d ^ a, 8);        \
+    c = c + d;                 \
+    b = rotr(b ^ c, 7);
+
+template <int RND>
+__device__ __forceinline__ void quad_round(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d, const uint32_t (&m)[16],
+                                           bool q_odd, bool q_hi) {
+    const uint32_t m0 = quad_msg<RND, 0>(m, q_odd, q_hi), m1 = quad_msg<RND, 1>(m, q_odd, q_hi);
+    const uint32_t m2 = quad_msg<RND, 2>(m, q_odd, q_hi), m3 = quad_msg<RND, 3>(m, q_odd, q_hi);
+    Z0P_Q4_G(m0, m1);
+    b = quad_dpp<QP_NEXT1>(b);
+    c = quad_dpp<QP_NEXT2>(c);
+    d = quad_dpp<QP_NEXT3>(d);
+    Z0P_Q4_G(m2, m3);
+    b = quad_dpp<QP_NEXT3>(b);
+    c = quad_dpp<QP_NEXT2>(c);
+    d = quad_dpp<QP_NEXT1>(d);
+}
+
+// One compression; (ha, hb) = chaining-value words (q, 4 + q) in and out.
+__device__ __forceinline__ void quad_compress(uint32_t& ha, uint32_t& hb, const uint32_t (&m)[16], uint32_t q,
+                                              uint32_t block_len, uint32_t flags) {
+    const bool q_odd = (q & 1) != 0, q_hi = (q & 2) != 0;
+    uint32_t a = ha, b = hb;
+    uint32_t c = sel4(q_odd, q_hi, b3::IV0, b3::IV1, b3::IV2, b3::IV3);
+    uint32_t d = sel4(q_odd, q_hi, 0u, 0u, block_len, flags);  // counter 0
+    quad_round<0>(a, b, c, d, m, q_odd, q_hi);
+    quad_round<1>(a, b, c, d, m, q_odd, q_hi);
+    quad_round<2>(a, b, c, d, m, q_odd, q_hi);
+    quad_round<3>(a, b, c, d, m, q_odd, q_hi);
+    quad_round<4>(a, b, c, d, m, q_odd, q_hi);
+    quad_round<5>(a, b, c, d, m, q_odd, q_hi);
+    quad_round<6>(a, b, c, d, m, q_odd, q_hi);
+    ha = a ^ c;
+    hb = b ^ d;
+}
+
+// Node digest BLAKE3(0x01 || l || r) by a quad (node_words, four lanes):
+// returns words (q, 4 + q) of the digest in (oa, ob).
+__device__ __forceinline__ void quad_node(const uint32_t (&l)[8], const uint32_t (&r)[8], uint32_t q, uint32_t& oa,
+                                          uint32_t& ob) {
+    uint32_t m[16];
+    m[0] = 0x01u | (l[0] << 8);
+#pragma unroll
+    for (int k = 1; k < 8; ++k) m[k] = (l[k - 1] >> 24) | (l[k] << 8);
+    m[8] = (l[7] >> 24) | (r[0] << 8);
+#pragma unroll
+    for (int k = 9; k < 16; ++k) m[k] = (r[k - 9] >> 24) | (r[k - 8] << 8);
+    const bool q_odd = (q & 1) != 0, q_hi = (q & 2) != 0;
+    uint32_t ha = sel4(q_odd, q_hi, b3::IV0, b3::IV1, b3::IV2, b3::IV3);
+    uint32_t hb = sel4(q_odd, q_hi, b3::IV4, b3::IV5, b3::IV6, b3::IV7);
+    quad_compress(ha, hb, m, q, 64, b3::CHUNK_START);
+    uint32_t m2[16];
+    m2[0] = r[7] >> 24;
+#pragma unroll
+    for (int k = 1; k < 16; ++k) m2[k] = 0;
+    quad_compress(ha, hb, m2, q, 1, b3::CHUNK_END | b3::ROOT);
+    oa = ha;
+    ob = hb;
+}
+
+// Narrow levels: up to 9 levels per 512-digest group in LDS, one quad per
+// parent (1024 threads: the 256 parents of the first level at once).
+constexpr int TAIL_WG = 1024;
+
+__global__ __launch_bounds__(TAIL_WG) void k_tail(TailJob inl, const TailJob* tab, uint32_t njobs) {
     __shared__ uint32_t bufA[8][SUB_IN];
     __shared__ uint32_t bufB[8][SUB_IN];
     uint32_t local;
     const TailJob J = job_of(inl, tab, njobs, local);
     const uint64_t first = uint64_t(local) * SUB_IN;
     const int here = int(J.count - first < uint64_t(SUB_IN) ? J.count - first : uint64_t(SUB_IN));
-    for (int t = threadIdx.x; t < here; t += blockDim.x) {
-        uint32_t d[8];
-        load_digest(J.level + (first + t) * 8, d);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) bufA[k][t] = d[k];
-    }
+    const int tid = threadIdx.x;
+    const gcptr level_in = gc(J.level);
+    for (int w = tid; w < here * 8; w += TAIL_WG) bufA[w & 7][w >> 3] = level_in[(first * 8) + w];
     __syncthreads();
-    subtree_levels(bufA, bufB, here, J.nlevels, J.level, J.count, first);
+    uint32_t(*cur)[SUB_IN] = bufA;
+    uint32_t(*nxt)[SUB_IN] = bufB;
+    const uint32_t q = uint32_t(tid & 3);
+    const int g = tid >> 2;
+    uint32_t* lvl = J.level;
+    uint64_t lc = J.count, f = first;
+    int count = here;
+    for (int l = 1; l <= J.nlevels && count > 1; ++l) {
+        uint32_t* next_lvl = lvl + lc * 8;
+        const int pc = count >> 1;
+        if (g < pc) {  // quad-uniform
+            uint32_t L[8], R[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                L[k] = cur[k][2 * g];
+                R[k] = cur[k][2 * g + 1];
+            }
+            uint32_t oa, ob;
+            quad_node(L, R, q, oa, ob);
+            nxt[q][g] = oa;
+            nxt[4 + q][g] = ob;
+            const gptr dst = gm(next_lvl) + (f / 2 + uint64_t(g)) * 8;
+            dst[q] = oa;
+            dst[4 + q] = ob;
+        }
+        __syncthreads();
+        auto tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+        count = pc;
+        lvl = next_lvl;
+        lc >>= 1;
+        f >>= 1;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -760,13 +914,12 @@ __global__ __launch_bounds__(WG) void k_fold(FoldJob inl, const FoldJob* tab, ui
     uint32_t inv_x = mont_mul(pow_lookup(J.winv, e << (2 * J.c.depth)), J.c.ginv_m);
     E4 q[4];
     if (J.base_field) {
-        uint4 v = reinterpret_cast<const uint4*>(J.layer)[i];
+        const u32x4 v = ld4(J.layer, i);
         q[0] = e4_base(v.x); q[1] = e4_base(v.y); q[2] = e4_base(v.z); q[3] = e4_base(v.w);
     } else {
-        const uint4* src = reinterpret_cast<const uint4*>(J.layer) + i * 4;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            uint4 v = src[k];
+            const u32x4 v = ld4(J.layer, i * 4 + k);
             q[k] = E4{{v.x, v.y, v.z, v.w}};
         }
     }
@@ -784,7 +937,7 @@ __global__ __launch_bounds__(WG) void k_fold(FoldJob inl, const FoldJob* tab, ui
     acc = e4_scale_m(e4_add(acc, e4_mul_m(even, ch1)), inv_x);
     acc = e4_add(acc, e4_mul_m(odd, ch0));
     E4 out = e4_add(mean, e4_scale_m(acc, inv_x));
-    reinterpret_cast<uint4*>(J.next)[i] = make_uint4(out.v[0], out.v[1], out.v[2], out.v[3]);
+    st4(J.next, i, u32x4{out.v[0], out.v[1], out.v[2], out.v[3]});
 }
 
 // ---------------------------------------------------------------------------
@@ -1049,7 +1202,12 @@ void launch_levels(const Launch& L, std::vector<LevelsJob> jobs) {
 
 void launch_tail(const Launch& L, std::vector<TailJob> jobs) {
     unsigned g = assign_blocks(jobs, [](const TailJob& j) { return grid_for(j.count, SUB_IN); });
-    run(L, "tree_tail", k_tail, jobs, g, 0);
+    if (jobs.empty() || g == 0) return;
+    const TailJob* table = table_of(L, jobs);
+    if (L.dry) return;
+    const uint32_t n = jobs.size() <= 1 ? 0u : uint32_t(jobs.size());
+    ProfScope ps(L.prof, "tree_tail", L.stream);
+    hipLaunchKernelGGL(k_tail, dim3(g), dim3(TAIL_WG), 0, L.stream, jobs[0], table, n);
 }
 
 void launch_fold(const Launch& L, std::vector<FoldJob> jobs) {

@@ -135,7 +135,8 @@ struct LevelsJob {
     uint32_t block_begin;
 };
 
-// Up to 9 levels in LDS per 512-digest group: the narrow tail.
+// Up to 9 levels in LDS per 512-digest group: the narrow tail (one
+// four-lane quad per parent node).
 struct TailJob {
     uint32_t* level;
     uint64_t count;
@@ -211,9 +212,10 @@ NttPlan plan_intt(const uint32_t* evals, uint32_t* coeffs_br, uint32_t* scratch,
 NttPlan plan_lde(const uint32_t* coeffs_br, uint32_t* lde_br, uint32_t* scratch, int logn, int logb,
                  const uint32_t* tw4096, PowTable w_m, CosetTables ct);
 
-// A level of at most this many digests is reduced by the LDS tail kernel;
-// wider levels use the one-node-per-thread kernel.
-constexpr uint64_t TAIL_MAX_COUNT = uint64_t(1) << 13;
+// A level of at most this many digests is reduced by the LDS tail kernel
+// (latency-bound: a quad of lanes per node); wider levels use the
+// one-node-per-thread kernel (throughput-bound).
+constexpr uint64_t TAIL_MAX_COUNT = uint64_t(1) << 16;
 
 }  // namespace z0p
 
