@@ -31,6 +31,8 @@ struct GeometryTables {
     CosetTables coset{};
     const uint32_t* zinv = nullptr;  // [B] 1/(s_j^n - 1), Montgomery
     const uint32_t* zvan = nullptr;  // [B] s_j^n - 1, Montgomery
+    const uint32_t* xn = nullptr;    // [n] omega_n^brev(q), Montgomery
+    const uint32_t* finv = nullptr;  // [m/4] Omega_m^-brev(i), Montgomery
     ~GeometryTables() {
         for (void* p : owned) (void)hipFree(p);
     }

@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "blake3.hpp"
 #include "field.hpp"
 #include "kernels.hpp"
@@ -12,6 +14,17 @@ namespace z0p {
 namespace dev {
 
 constexpr int WG = 256;
+
+// Compile-time loop: f(integral_constant<int, I>) for I in [B, E). Register
+// arrays indexed by the constant stay in registers even where the compiler
+// would not fully unroll an ordinary loop with a large body.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
 
 // Descriptor pointers are generic; these casts state that they address
 // global memory, so the compiler emits global (not flat) accesses and does

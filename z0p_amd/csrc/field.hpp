@@ -30,25 +30,37 @@ constexpr uint32_t GENERATOR = 31;
 constexpr uint32_t TWO_ADICITY = 27;
 constexpr uint32_t NONRESIDUE = 11;
 
+// a, b < p. Both select by an unsigned minimum: the wrong candidate has
+// wrapped past 2^32 - p and is the larger one (2p < 2^32).
 Z0P_HD uint32_t add(uint32_t a, uint32_t b) {
-    uint32_t s = a + b;
-    return s >= P ? s - P : s;
+    const uint32_t s = a + b, t = s - P;
+    return s < t ? s : t;
 }
 Z0P_HD uint32_t sub(uint32_t a, uint32_t b) {
-    uint32_t d = a - b;
-    return a >= b ? d : d + P;
+    const uint32_t d = a - b, e = d + P;
+    return d < e ? d : e;
 }
 Z0P_HD uint32_t neg(uint32_t a) { return a == 0 ? 0 : P - a; }
 Z0P_HD uint32_t halve(uint32_t a) { return (a + ((a & 1u) ? P : 0u)) >> 1; }
 Z0P_HD uint32_t dbl(uint32_t a) { return add(a, a); }
 
-// Montgomery multiply: a*b*2^{-32} mod p for a, b < p; output < p.
-Z0P_HD uint32_t mont_mul(uint32_t a, uint32_t b) {
-    uint64_t t = uint64_t(a) * b;
+// Montgomery reduction T * 2^{-32} mod p for T < p * 2^32; output < p.
+// m*p + T < 2 p 2^32 < 2^64, and the high word is below 2p.
+// The final subtraction is an unsigned minimum (r < 2p < 2^32), which keeps
+// the result in vector registers with no compare-mask hazard.
+Z0P_HD uint32_t redc(uint64_t t) {
     uint32_t m = uint32_t(t) * P_NEG_INV;
-    uint64_t u = uint64_t(m) * P + t;  // < 2^63.2, no overflow
-    uint32_t r = uint32_t(u >> 32);
-    return r >= P ? r - P : r;
+    uint64_t u = uint64_t(m) * P + t;
+    uint32_t r = uint32_t(u >> 32), s = r - P;
+    return r < s ? r : s;
+}
+
+// Montgomery multiply: a*b*2^{-32} mod p for a, b < p; output < p.
+Z0P_HD uint32_t mont_mul(uint32_t a, uint32_t b) { return redc(uint64_t(a) * b); }
+
+// a*b + c*d reduced once (all < p: the sum is below 2p^2 < p 2^32).
+Z0P_HD uint32_t mont_mul2(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    return redc(uint64_t(a) * b + uint64_t(c) * d);
 }
 
 Z0P_HD uint32_t to_mont(uint32_t a) { return mont_mul(a, R2); }
@@ -122,6 +134,29 @@ Z0P_HD E4 e4_mul_m(const E4& x, const E4& ym) {
     uint32_t cr0 = sub(sub(x0, ac0), bd0), cr1 = sub(sub(x1, ac1), bd1);
     // bd * u = (11 bd1) + bd0 u
     return E4{{add(ac0, mul11(bd1)), add(ac1, bd0), cr0, cr1}};
+}
+
+// Multiplication by a fixed y as the 4x4 matrix it induces on the basis
+// (1, u, v, uv) (u^2 = 11, v^2 = u):
+//   out0 = x0 y0 + x1 11y1 + x2 11y3 + x3 11y2
+//   out1 = x0 y1 + x1 y0   + x2 y2   + x3 11y3
+//   out2 = x0 y2 + x1 11y3 + x2 y0   + x3 11y1
+//   out3 = x0 y3 + x1 y2   + x2 y1   + x3 y0
+// The seven distinct entries (y0, y1, y2, y3, 11y1, 11y2, 11y3) are given in
+// Montgomery form; each output is two pair sums reduced once each.
+struct E4Mat {
+    uint32_t y0, y1, y2, y3, y1e, y2e, y3e, pad;
+};
+Z0P_HD E4Mat e4_mat_m(const E4& y) {
+    return E4Mat{to_mont(y.v[0]), to_mont(y.v[1]), to_mont(y.v[2]), to_mont(y.v[3]),
+                 to_mont(mul11(y.v[1])), to_mont(mul11(y.v[2])), to_mont(mul11(y.v[3])), 0};
+}
+Z0P_HD E4 e4_mul_mat(const E4& x, const E4Mat& M) {
+    const uint32_t x0 = x.v[0], x1 = x.v[1], x2 = x.v[2], x3 = x.v[3];
+    return E4{{add(mont_mul2(x0, M.y0, x1, M.y1e), mont_mul2(x2, M.y3e, x3, M.y2e)),
+               add(mont_mul2(x0, M.y1, x1, M.y0), mont_mul2(x2, M.y2, x3, M.y3e)),
+               add(mont_mul2(x0, M.y2, x1, M.y3e), mont_mul2(x2, M.y0, x3, M.y1e)),
+               add(mont_mul2(x0, M.y3, x1, M.y2), mont_mul2(x2, M.y1, x3, M.y0))}};
 }
 
 Z0P_HD E4 e4_to_mont(const E4& a) {

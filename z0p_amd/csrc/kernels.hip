@@ -554,23 +554,31 @@ __global__ __launch_bounds__(WG) void k_linked(LinkedJob inl, const LinkedJob* t
     const LinkedJob J = job_of(inl, tab, njobs, local);
     const uint32_t m = 1u << (J.logn + J.logb);
     const uint32_t base = local * (WG * ELT) + threadIdx.x;
-    uint32_t den[ELT];
-    PosInfo pi[ELT];
+    const gcptr A = gc(J.a);
+    uint32_t den[ELT], an[ELT];
+    // The next-row loads are scattered inside the coset block: issue them
+    // first so their latency overlaps the batched inversion.
 #pragma unroll
     for (int e = 0; e < ELT; ++e) {
-        uint32_t p = base + e * WG;
+        const uint32_t p = base + e * WG;
         if (p < m) {
-            pi[e] = pos_info(p, J.logn, J.logb);
-            den[e] = sub(point_m(J.w, J.shift_m, pi[e].j, pi[e].t, J.logb), J.w_last_m);
+            const PosInfo pi = pos_info(p, J.logn, J.logb);
+            an[e] = A[pi.pnext];
+            const uint32_t x = mont_mul(gc(J.xn)[p & ((1u << J.logn) - 1)], gc(J.shift_m)[pi.j]);
+            den[e] = sub(x, J.w_last_m);
         } else {
+            an[e] = 0;
             den[e] = R1;
         }
     }
     batch_inv_m<ELT>(den);
 #pragma unroll
     for (int e = 0; e < ELT; ++e) {
-        uint32_t p = base + e * WG;
-        if (p < m) gm(J.b)[p] = add(gc(J.a)[pi[e].pnext], mont_mul(den[e], mont_mul(J.coeff, gc(J.zvan_m)[pi[e].j])));
+        const uint32_t p = base + e * WG;
+        if (p < m) {
+            const uint32_t j = brev_bits(p >> J.logn, J.logb);
+            gm(J.b)[p] = add(an[e], mont_mul(den[e], mont_mul(J.coeff, gc(J.zvan_m)[j])));
+        }
     }
 }
 
@@ -582,48 +590,51 @@ __global__ __launch_bounds__(WG) void k_compose(ComposeJob inl, const ComposeJob
     const ComposeJob J = job_of(inl, tab, njobs, local);
     const uint32_t m = 1u << (J.logn + J.logb);
     const uint32_t base = local * (WG * ELT) + threadIdx.x;
-    uint32_t d1[ELT], dw[ELT], pr[ELT];
+    const gcptr A = gc(J.a), Bv = gc(J.b);
+    // x - w_last = (x - 1) - (w_last - 1): only x - 1 is kept per element.
+    const uint32_t wl1 = sub(J.c.w_last_m, R1);
+    uint32_t d1[ELT], pr[ELT], na[ELT], nb[ELT];
+    // Scattered next-row loads first, overlapping the batched inversion.
 #pragma unroll
     for (int e = 0; e < ELT; ++e) {
-        uint32_t p = base + e * WG;
+        const uint32_t p = base + e * WG;
         if (p < m) {
-            PosInfo pi = pos_info(p, J.logn, J.logb);
-            uint32_t x = point_m(J.w, J.shift_m, pi.j, pi.t, J.logb);
+            const PosInfo pi = pos_info(p, J.logn, J.logb);
+            na[e] = A[pi.pnext];
+            nb[e] = Bv[pi.pnext];
+            const uint32_t x = mont_mul(gc(J.xn)[p & ((1u << J.logn) - 1)], gc(J.shift_m)[pi.j]);
             d1[e] = sub(x, R1);
-            dw[e] = sub(x, J.c.w_last_m);
-            pr[e] = mont_mul(d1[e], dw[e]);
+            pr[e] = mont_mul(d1[e], sub(d1[e], wl1));
         } else {
-            d1[e] = dw[e] = pr[e] = R1;
+            na[e] = nb[e] = 0;
+            d1[e] = pr[e] = R1;
         }
     }
     batch_inv_m<ELT>(pr);
-    const gcptr A = gc(J.a), Bv = gc(J.b);
-#pragma unroll
-    for (int e = 0; e < ELT; ++e) {
-        uint32_t p = base + e * WG;
-        if (p >= m) continue;
-        PosInfo pi = pos_info(p, J.logn, J.logb);
-        uint32_t i1 = mont_mul(dw[e], pr[e]);
-        uint32_t iw = mont_mul(d1[e], pr[e]);
-        uint32_t iz = mont_mul(dw[e], gc(J.zinv_m)[pi.j]);
-        uint32_t va = A[p], vb = Bv[p], na = A[pi.pnext], nb = Bv[pi.pnext];
-        uint32_t s0 = mont_mul(sub(na, vb), iz);
-        uint32_t s1 = mont_mul(sub(nb, add(va, vb)), iz);
-        uint32_t s2 = mont_mul(sub(va, J.c.a0), i1);
-        uint32_t s3 = mont_mul(sub(vb, J.c.b0), i1);
-        uint32_t s4 = mont_mul(sub(vb, J.c.b_last), iw);
+    static_for<0, ELT>([&](auto ec) {
+        constexpr int e = decltype(ec)::value;
+        const uint32_t p = base + e * WG;
+        if (p >= m) return;
+        const uint32_t j = brev_bits(p >> J.logn, J.logb);
+        const uint32_t dw = sub(d1[e], wl1);
+        const uint32_t i1 = mont_mul(dw, pr[e]);
+        const uint32_t iw = mont_mul(d1[e], pr[e]);
+        const uint32_t iz = mont_mul(dw, gc(J.zinv_m)[j]);
+        const uint32_t va = A[p], vb = Bv[p];
+        const uint32_t s0 = mont_mul(sub(na[e], vb), iz);
+        const uint32_t s1 = mont_mul(sub(nb[e], add(va, vb)), iz);
+        const uint32_t s2 = mont_mul(sub(va, J.c.a0), i1);
+        const uint32_t s3 = mont_mul(sub(vb, J.c.b0), i1);
+        const uint32_t s4 = mont_mul(sub(vb, J.c.b_last), iw);
         uint32_t o[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            uint32_t acc = mont_mul(s0, J.c.alpha_m[0][k]);
-            acc = add(acc, mont_mul(s1, J.c.alpha_m[1][k]));
-            acc = add(acc, mont_mul(s2, J.c.alpha_m[2][k]));
-            acc = add(acc, mont_mul(s3, J.c.alpha_m[3][k]));
-            acc = add(acc, mont_mul(s4, J.c.alpha_m[4][k]));
-            o[k] = acc;
+            const uint32_t p01 = mont_mul2(s0, J.c.alpha_m[0][k], s1, J.c.alpha_m[1][k]);
+            const uint32_t p23 = mont_mul2(s2, J.c.alpha_m[2][k], s3, J.c.alpha_m[3][k]);
+            o[k] = add(add(p01, p23), mont_mul(s4, J.c.alpha_m[4][k]));
         }
         st4(J.layer0, p, u32x4{o[0], o[1], o[2], o[3]});
-    }
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -910,8 +921,7 @@ __global__ __launch_bounds__(WG) void k_fold(FoldJob inl, const FoldJob* tab, ui
     const uint64_t quads = uint64_t(1) << (J.c.log_len - 2);
     const uint64_t i = uint64_t(local) * WG + threadIdx.x;
     if (i >= quads) return;
-    uint32_t e = brev_bits(uint32_t(i), J.c.log_len - 2);
-    uint32_t inv_x = mont_mul(pow_lookup(J.winv, e << (2 * J.c.depth)), J.c.ginv_m);
+    const uint32_t inv_x = mont_mul(gc(J.finv)[i], J.c.ginv_m);
     E4 q[4];
     if (J.base_field) {
         const u32x4 v = ld4(J.layer, i);
@@ -923,20 +933,14 @@ __global__ __launch_bounds__(WG) void k_fold(FoldJob inl, const FoldJob* tab, ui
             q[k] = E4{{v.x, v.y, v.z, v.w}};
         }
     }
-    E4 s01 = e4_add(q[0], q[1]), s23 = e4_add(q[2], q[3]);
-    E4 odd_diff = e4_sub(q[0], q[1]);
-    E4 skew_diff = e4_scale_m(e4_sub(q[2], q[3]), J.c.unit_m);
-    E4 mean = e4_halve(e4_halve(e4_add(s01, s23)));
-    E4 odd = e4_halve(e4_halve(e4_sub(odd_diff, skew_diff)));
-    E4 even = e4_halve(e4_halve(e4_sub(s01, s23)));
-    E4 skew = e4_halve(e4_halve(e4_add(odd_diff, skew_diff)));
-    E4 ch0{{J.c.ch_m[0][0], J.c.ch_m[0][1], J.c.ch_m[0][2], J.c.ch_m[0][3]}};
-    E4 ch1{{J.c.ch_m[1][0], J.c.ch_m[1][1], J.c.ch_m[1][2], J.c.ch_m[1][3]}};
-    E4 ch2{{J.c.ch_m[2][0], J.c.ch_m[2][1], J.c.ch_m[2][2], J.c.ch_m[2][3]}};
-    E4 acc = e4_scale_m(e4_mul_m(skew, ch2), inv_x);
-    acc = e4_scale_m(e4_add(acc, e4_mul_m(even, ch1)), inv_x);
-    acc = e4_add(acc, e4_mul_m(odd, ch0));
-    E4 out = e4_add(mean, e4_scale_m(acc, inv_x));
+    const E4 s01 = e4_add(q[0], q[1]), s23 = e4_add(q[2], q[3]);
+    const E4 dodd = e4_sub(q[0], q[1]);
+    const E4 dskew = e4_scale_m(e4_sub(q[2], q[3]), J.c.unit_m);
+    const E4 M = e4_add(s01, s23), O = e4_sub(dodd, dskew), Ev = e4_sub(s01, s23), S = e4_add(dodd, dskew);
+    E4 acc = e4_scale_m(e4_mul_mat(S, J.c.cm[2]), inv_x);
+    acc = e4_scale_m(e4_add(acc, e4_mul_mat(Ev, J.c.cm[1])), inv_x);
+    acc = e4_add(acc, e4_mul_mat(O, J.c.cm[0]));
+    const E4 out = e4_add(e4_scale_m(M, J.c.inv4_m), e4_scale_m(acc, inv_x));
     st4(J.next, i, u32x4{out.v[0], out.v[1], out.v[2], out.v[3]});
 }
 

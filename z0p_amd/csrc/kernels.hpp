@@ -12,6 +12,8 @@
 #include <cstdint>
 #include <vector>
 
+#include "field.hpp"
+
 namespace z0p {
 
 struct Profiler;
@@ -85,7 +87,7 @@ struct NttJob {
 struct LinkedJob {
     const uint32_t* a;
     uint32_t* b;
-    PowTable w;
+    const uint32_t* xn;  // [n] omega_n^brev(q), Montgomery (point = shift_j * xn[q])
     const uint32_t* shift_m;
     const uint32_t* zvan_m;  // [B] s_j^n - 1, Montgomery
     uint32_t coeff;          // (b_last - a0) w_last / n, canonical
@@ -103,7 +105,7 @@ struct ComposeJob {
     const uint32_t* a;
     const uint32_t* b;
     uint32_t* layer0;
-    PowTable w;
+    const uint32_t* xn;  // [n] omega_n^brev(q), Montgomery
     const uint32_t* shift_m;
     const uint32_t* zinv_m;  // [B] 1/(s_j^n - 1)
     ComposeConsts c;
@@ -144,17 +146,34 @@ struct TailJob {
     uint32_t block_begin;
 };
 
+// Fold of layer d with challenges r0, r1, r2: out = M/4 + x^-1 (O r0/4 +
+// x^-1 (E r1/4 + x^-1 S r2/4)) with (M, O, E, S) the 4-point inverse
+// transform of the quad without its 1/4 (fri.rs:1273-1331).
 struct FoldConsts {
-    uint32_t ch_m[3][4];
-    uint32_t unit_m;
+    E4Mat cm[3];      // r_k / 4 as multiplication matrices (Montgomery)
+    uint32_t inv4_m;  // 1/4, Montgomery
+    uint32_t unit_m;  // two_adic_generator(2), Montgomery
     uint32_t ginv_m;  // 31^{-4^d}, Montgomery
     int log_len;      // log2 of this layer's length
     int depth;        // fold depth d
 };
+inline FoldConsts make_fold_consts(const E4 (&r)[3], int log_len, int depth) {
+    FoldConsts fc{};
+    const uint32_t inv4 = inv_c(4);
+    for (int k = 0; k < 3; ++k) fc.cm[k] = e4_mat_m(e4_mul_base(r[k], inv4));
+    fc.inv4_m = to_mont(inv4);
+    fc.unit_m = to_mont(root_of_unity(2));
+    fc.ginv_m = to_mont(pow_c(inv_c(GENERATOR), uint64_t(1) << (2 * depth)));
+    fc.log_len = log_len;
+    fc.depth = depth;
+    return fc;
+}
 struct FoldJob {
     const uint32_t* layer;
     uint32_t* next;
-    PowTable winv;
+    // [m/4] Omega_m^-brev(i) over the layer-0 quads (Montgomery); layer d
+    // uses its first m/4^(d+1) entries: brev_k(i) 4^d = brev_{k+2d}(i).
+    const uint32_t* finv;
     FoldConsts c;
     int base_field;
     uint32_t block_begin;

@@ -119,6 +119,20 @@ const GeometryTables& Ctx::geometry(int logn, int logb) {
     t->coset.shift = upload(t->owned, shift);
     t->zinv = upload(t->owned, zinv);
     t->zvan = upload(t->owned, zvan);
+    {
+        // Point tables in bit-reversed order, so consecutive positions load
+        // consecutive words.
+        auto wn = geometric_m(root_of_unity(uint32_t(logn)), n);
+        std::vector<uint32_t> xn(n);
+        for (uint64_t q = 0; q < n; ++q) xn[q] = wn[brev_host(q, logn)];
+        t->xn = upload(t->owned, xn);
+        const uint64_t quads = std::max<uint64_t>(m / 4, 1);
+        const int lq = int(ilog2(quads));
+        auto wi = geometric_m(wm_inv, quads);
+        std::vector<uint32_t> finv(quads);
+        for (uint64_t i = 0; i < quads; ++i) finv[i] = wi[brev_host(i, lq)];
+        t->finv = upload(t->owned, finv);
+    }
     NttShape sh = ntt_shape(logn);
     if (sh.log1 == 0) {
         std::vector<uint32_t> small;
@@ -562,7 +576,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 LinkedJob lj{};
                 lj.a = P.A;
                 lj.b = P.Bc;
-                lj.w = gt.w;
+                lj.xn = gt.xn;
                 lj.shift_m = gt.coset.shift;
                 lj.zvan_m = gt.zvan;
                 lj.coeff = P.coeff_c;
@@ -591,7 +605,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
             cj.a = P.A;
             cj.b = P.Bc;
             cj.layer0 = P.vals[0];
-            cj.w = gt.w;
+            cj.xn = gt.xn;
             cj.shift_m = gt.coset.shift;
             cj.zinv_m = gt.zinv;
             cj.c = P.cc;
@@ -607,7 +621,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
             FoldJob fj{};
             fj.layer = P.vals[d];
             fj.next = P.vals[d + 1];
-            fj.winv = gt.winv;
+            fj.finv = gt.finv;
             fj.c = P.fold;
             fj.base_field = d == 0 && !P.air ? 1 : 0;
             ch.push_back(fj);
@@ -671,15 +685,9 @@ void commit_layer(Proof& P, const uint32_t* w, size_t d) {
     auto cap = read_digests(w, uint64_t(1) << g.drop);
     for (auto& dg : cap) P.tr->absorb("fri-layer-root", dg.data(), 32);
     P.caps.push_back(std::move(cap));
-    FoldConsts fc{};
-    for (int k = 0; k < 3; ++k) {
-        E4 ch = P.tr->challenge_ext("fri-fold");
-        for (int q = 0; q < 4; ++q) fc.ch_m[k][q] = to_mont(ch.v[q]);
-    }
-    fc.unit_m = to_mont(root_of_unity(2));
-    fc.ginv_m = to_mont(pow_c(inv_c(GENERATOR), uint64_t(1) << (2 * d)));
-    fc.log_len = int(ilog2(P.lens[d]));
-    fc.depth = int(d);
+    E4 r[3];
+    for (int k = 0; k < 3; ++k) r[k] = P.tr->challenge_ext("fri-fold");
+    const FoldConsts fc = make_fold_consts(r, int(ilog2(P.lens[d])), int(d));
     P.fold = fc;
     P.depth = d;
     P.phase = PH_FOLD;

@@ -172,7 +172,7 @@ int z0p_test_linked(z0p_ctx* ctx, const uint32_t* a, size_t n, size_t blowup, ui
         LinkedJob lj{};
         lj.a = da;
         lj.b = db;
-        lj.w = gt.w;
+        lj.xn = gt.xn;
         lj.shift_m = gt.coset.shift;
         lj.zvan_m = gt.zvan;
         lj.coeff = coeff;
@@ -215,7 +215,7 @@ int z0p_test_compose(z0p_ctx* ctx, const uint32_t* a, const uint32_t* b, size_t 
         cj.a = da;
         cj.b = db;
         cj.layer0 = dl;
-        cj.w = gt.w;
+        cj.xn = gt.xn;
         cj.shift_m = gt.coset.shift;
         cj.zinv_m = gt.zinv;
         cj.c = cc;
@@ -237,20 +237,17 @@ int z0p_test_fold(z0p_ctx* ctx, const uint32_t* layer, size_t len, size_t lde_le
         Ctx& c = *ctx->c;
         int logm = int(ilog2(lde_len));
         const GeometryTables& gt = c.geometry(logm - 4, 4);
-        FoldConsts fc{};
+        E4 r[3];
         for (int k = 0; k < 3; ++k)
-            for (int q = 0; q < 4; ++q) fc.ch_m[k][q] = to_mont(challenges[4 * k + q] % P);
-        fc.unit_m = to_mont(root_of_unity(2));
-        fc.ginv_m = to_mont(pow_c(inv_c(GENERATOR), uint64_t(1) << (2 * depth)));
-        fc.log_len = int(ilog2(len));
-        fc.depth = int(depth);
+            for (int q = 0; q < 4; ++q) r[k].v[q] = challenges[4 * k + q] % P;
+        const FoldConsts fc = make_fold_consts(r, int(ilog2(len)), int(depth));
         c.reserve_scratch(len * 20 + (1 << 20));
         uint32_t* dl = to_dev(c, layer, 4 * len);
         uint32_t* dn = c.scratch_alloc<uint32_t>(len);
         FoldJob fj{};
         fj.layer = dl;
         fj.next = dn;
-        fj.winv = gt.winv;
+        fj.finv = gt.finv;
         fj.c = fc;
         fj.base_field = 0;
         run_chain(c, {fj});
