@@ -168,44 +168,6 @@ __device__ __forceinline__ void store_digest(uint32_t* dst, const uint32_t d[8])
     st4(dst, 1, u32x4{d[4], d[5], d[6], d[7]});
 }
 
-// Merkle subtree reduction in LDS. Digests word-major (SoA) so a thread
-// reads both children of word k with one 8-byte access.
-constexpr int SUB_IN = 2 * WG;
-
-// cur holds `count` digests of the level at level_ptr (count_total digests,
-// this group starting at index first); writes up to nlevels parent levels,
-// which follow each other contiguously in global memory.
-__device__ __forceinline__ void subtree_levels(uint32_t (*cur)[SUB_IN], uint32_t (*nxt)[SUB_IN], int count,
-                                               int nlevels, uint32_t* level_ptr, uint64_t count_total,
-                                               uint64_t first) {
-    uint32_t* lvl = level_ptr;
-    uint64_t lc = count_total;
-    for (int l = 1; l <= nlevels && count > 1; ++l) {
-        uint32_t* next_lvl = lvl + lc * 8;
-        int pc = count >> 1;
-        for (int t = threadIdx.x; t < pc; t += blockDim.x) {
-            uint32_t L[8], R[8], o[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                L[k] = cur[k][2 * t];
-                R[k] = cur[k][2 * t + 1];
-            }
-            node_words(L, R, o);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) nxt[k][t] = o[k];
-            store_digest(next_lvl + (first / 2 + t) * 8, o);
-        }
-        __syncthreads();
-        auto tmp = cur;
-        cur = nxt;
-        nxt = tmp;
-        count = pc;
-        lvl = next_lvl;
-        lc >>= 1;
-        first >>= 1;
-    }
-}
-
 // This workgroup's job: table sorted by block_begin (njobs > 0), or the
 // inline descriptor when njobs == 0. `local` is the block index inside the job.
 template <typename D>

@@ -546,6 +546,9 @@ __global__ __launch_bounds__(WG) void k_bitrev(BitrevJob inl, const BitrevJob* t
 // Pointwise kernels over the bit-reversed LDE: ELT elements per thread,
 // strided by the workgroup size so every access is coalesced.
 constexpr int ELT = 16;
+// Composition keeps four values per element live across its inversion; a
+// smaller batch keeps its register use (and occupancy) in check.
+constexpr int CELT = 8;
 
 // B[i] = A[i + B] + coeff*(x^n - 1)/(x - w_last) (air.rs:806-844);
 // x^n - 1 = s_j^n - 1 is constant per coset (table zvan_m).
@@ -589,14 +592,14 @@ __global__ __launch_bounds__(WG) void k_compose(ComposeJob inl, const ComposeJob
     uint32_t local;
     const ComposeJob J = job_of(inl, tab, njobs, local);
     const uint32_t m = 1u << (J.logn + J.logb);
-    const uint32_t base = local * (WG * ELT) + threadIdx.x;
+    const uint32_t base = local * (WG * CELT) + threadIdx.x;
     const gcptr A = gc(J.a), Bv = gc(J.b);
     // x - w_last = (x - 1) - (w_last - 1): only x - 1 is kept per element.
     const uint32_t wl1 = sub(J.c.w_last_m, R1);
-    uint32_t d1[ELT], pr[ELT], na[ELT], nb[ELT];
+    uint32_t d1[CELT], pr[CELT], na[CELT], nb[CELT];
     // Scattered next-row loads first, overlapping the batched inversion.
 #pragma unroll
-    for (int e = 0; e < ELT; ++e) {
+    for (int e = 0; e < CELT; ++e) {
         const uint32_t p = base + e * WG;
         if (p < m) {
             const PosInfo pi = pos_info(p, J.logn, J.logb);
@@ -610,8 +613,8 @@ __global__ __launch_bounds__(WG) void k_compose(ComposeJob inl, const ComposeJob
             d1[e] = pr[e] = R1;
         }
     }
-    batch_inv_m<ELT>(pr);
-    static_for<0, ELT>([&](auto ec) {
+    batch_inv_m<CELT>(pr);
+    static_for<0, CELT>([&](auto ec) {
         constexpr int e = decltype(ec)::value;
         const uint32_t p = base + e * WG;
         if (p >= m) return;
@@ -860,23 +863,25 @@ __device__ __forceinline__ void quad_node(const uint32_t (&l)[8], const uint32_t
     ob = hb;
 }
 
-// Narrow levels: up to 9 levels per 512-digest group in LDS, one quad per
-// parent (1024 threads: the 256 parents of the first level at once).
-constexpr int TAIL_WG = 1024;
+// Narrow levels: up to TAIL_LEVELS levels per 128-digest group in LDS, one
+// quad per parent (256 threads: the 64 parents of the first level at once).
+// Small groups spread a level over many compute units.
+constexpr int TAIL_SUB = 1 << TAIL_LEVELS;
+constexpr int TAIL_WG = 2 * TAIL_SUB;
 
 __global__ __launch_bounds__(TAIL_WG) void k_tail(TailJob inl, const TailJob* tab, uint32_t njobs) {
-    __shared__ uint32_t bufA[8][SUB_IN];
-    __shared__ uint32_t bufB[8][SUB_IN];
+    __shared__ uint32_t bufA[8][TAIL_SUB];
+    __shared__ uint32_t bufB[8][TAIL_SUB];
     uint32_t local;
     const TailJob J = job_of(inl, tab, njobs, local);
-    const uint64_t first = uint64_t(local) * SUB_IN;
-    const int here = int(J.count - first < uint64_t(SUB_IN) ? J.count - first : uint64_t(SUB_IN));
+    const uint64_t first = uint64_t(local) * TAIL_SUB;
+    const int here = int(J.count - first < uint64_t(TAIL_SUB) ? J.count - first : uint64_t(TAIL_SUB));
     const int tid = threadIdx.x;
     const gcptr level_in = gc(J.level);
     for (int w = tid; w < here * 8; w += TAIL_WG) bufA[w & 7][w >> 3] = level_in[(first * 8) + w];
     __syncthreads();
-    uint32_t(*cur)[SUB_IN] = bufA;
-    uint32_t(*nxt)[SUB_IN] = bufB;
+    uint32_t(*cur)[TAIL_SUB] = bufA;
+    uint32_t(*nxt)[TAIL_SUB] = bufB;
     const uint32_t q = uint32_t(tid & 3);
     const int g = tid >> 2;
     uint32_t* lvl = J.level;
@@ -991,7 +996,7 @@ __global__ __launch_bounds__(WG) void k_grind(GrindJob inl, const GrindJob* tab,
 }
 
 // ---------------------------------------------------------------------------
-constexpr int COLLECT_WG = 1024;
+constexpr int COLLECT_WG = 64;
 
 __global__ __launch_bounds__(COLLECT_WG) void k_collect(CollectItem inl, const CollectItem* tab, uint32_t nitems,
                                                         uint64_t* flag, uint64_t seq) {
@@ -1185,7 +1190,7 @@ void launch_linked(const Launch& L, std::vector<LinkedJob> jobs) {
 
 void launch_compose(const Launch& L, std::vector<ComposeJob> jobs) {
     unsigned g = assign_blocks(jobs, [](const ComposeJob& j) {
-        return grid_for(uint64_t(1) << (j.logn + j.logb), WG * ELT);
+        return grid_for(uint64_t(1) << (j.logn + j.logb), WG * CELT);
     });
     run(L, "compose", k_compose, jobs, g, 0);
 }
@@ -1205,7 +1210,7 @@ void launch_levels(const Launch& L, std::vector<LevelsJob> jobs) {
 }
 
 void launch_tail(const Launch& L, std::vector<TailJob> jobs) {
-    unsigned g = assign_blocks(jobs, [](const TailJob& j) { return grid_for(j.count, SUB_IN); });
+    unsigned g = assign_blocks(jobs, [](const TailJob& j) { return grid_for(j.count, TAIL_SUB); });
     if (jobs.empty() || g == 0) return;
     const TailJob* table = table_of(L, jobs);
     if (L.dry) return;

@@ -137,8 +137,8 @@ struct LevelsJob {
     uint32_t block_begin;
 };
 
-// Up to 9 levels in LDS per 512-digest group: the narrow tail (one
-// four-lane quad per parent node).
+// Up to TAIL_LEVELS levels in LDS per 2^TAIL_LEVELS-digest group: the
+// narrow tail (one four-lane quad per parent node).
 struct TailJob {
     uint32_t* level;
     uint64_t count;
@@ -234,7 +234,8 @@ NttPlan plan_lde(const uint32_t* coeffs_br, uint32_t* lde_br, uint32_t* scratch,
 // A level of at most this many digests is reduced by the LDS tail kernel
 // (latency-bound: a quad of lanes per node); wider levels use the
 // one-node-per-thread kernel (throughput-bound).
-constexpr uint64_t TAIL_MAX_COUNT = uint64_t(1) << 16;
+constexpr uint64_t TAIL_MAX_COUNT = uint64_t(1) << 17;
+constexpr int TAIL_LEVELS = 7;
 
 }  // namespace z0p
 

@@ -218,7 +218,7 @@ void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, Lea
             chain.push_back(LevelsJob{lvl, count, fuse, 0});
             have += fuse;
         } else {
-            const int k = std::min(9, remaining);
+            const int k = std::min(TAIL_LEVELS, remaining);
             chain.push_back(TailJob{lvl, count, k, 0});
             have += k;
         }
