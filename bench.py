@@ -274,14 +274,34 @@ def main():
     if "merkle" in families:
         families["merkle"]["compressions"] = comp
         families["merkle"]["gcompress_per_s"] = round(comp / (fam_time["merkle"] * 1e-3) / 1e9, 2)
+    for f, v in families.items():
+        v["hbm_frac"] = round(v["achieved_gbs"] / HBM_PEAK_GBS, 4)
+    # Measured HBM traffic per proof (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    # calibrated on a plain copy kernel), committed with the profile.
+    traffic = {}
+    tpath = os.path.join(ROOT, "profiles", "air_2p20_traffic.json")
+    if n == N_STEPS and os.path.exists(tpath):
+        with open(tpath) as fh:
+            for f, v in json.load(fh)["families"].items():
+                traffic[f] = v["fetch_bytes_per_proof"] + v["write_bytes_per_proof"]
     dom = max(fam_time, key=fam_time.get) if fam_time else None
     roofline = None
     if dom:
         t_ms = fam_time[dom]
         achieved = alg[dom] / (t_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic.get(dom),
+                    "traffic_unit": "HBM bytes per proof (profiles/air_2p20_traffic.json)" if dom in traffic else None,
                     "ms_per_proof": round(t_ms, 4), "alg_bytes": alg[dom]}
+        if dom == "merkle":
+            # BLAKE3 hashing is integer-VALU work: 12 VALU ops per G function,
+            # 56 per compression, 672 ops; MI355X issues 256 CUs x 4 SIMDs x
+            # 16 lanes per clock at 2.4 GHz.
+            peak_gc = 256 * 4 * 16 * 2.4e9 / 672 / 1e9
+            got = families["merkle"]["gcompress_per_s"]
+            roofline["valu"] = {"achieved": got, "peak": round(peak_gc, 1), "unit": "Gcompress/s",
+                                "frac": round(got / peak_gc, 4)}
+
     batch = batched(z, ctx, args.batch) if args.batch else None
     serve = served(z, args.serve_budget_gib, args.serve_s) if args.serve_s > 0 else None
 
