@@ -1016,6 +1016,12 @@ __global__ void k_copy(const uint4* __restrict__ in, uint4* __restrict__ out, ui
     for (; i < n16; i += stride) out[i] = in[i];
 }
 
+__global__ __launch_bounds__(WG) void k_pow_brev(PowTable pt, uint32_t* out, int bits, int shift) {
+    const uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x;
+    if (i >> bits) return;
+    gm(out)[i] = pow_lookup(pt, brev_bits(uint32_t(i), bits) << shift);
+}
+
 inline unsigned grid_for(uint64_t work, uint64_t per_block) { return unsigned((work + per_block - 1) / per_block); }
 
 // Assign block_begin, return the total grid.
@@ -1251,6 +1257,12 @@ void launch_collect(const Launch& L, std::vector<CollectItem> items, uint64_t* f
     ProfScope ps(L.prof, "collect", L.stream);
     hipLaunchKernelGGL(k_collect, dim3(1), dim3(COLLECT_WG), 0, L.stream, inl, table, uint32_t(items.size()), flag,
                        seq);
+}
+
+void launch_pow_brev(const Launch& L, PowTable pt, uint32_t* out, int bits, int shift) {
+    if (L.dry) return;
+    hipLaunchKernelGGL(k_pow_brev, dim3(grid_for(uint64_t(1) << bits, WG)), dim3(WG), 0, L.stream, pt, out, bits,
+                       shift);
 }
 
 void launch_copy(const Launch& L, const uint32_t* in, uint32_t* out, uint64_t words) {

@@ -220,6 +220,9 @@ struct CollectItem {
 };
 void launch_collect(const Launch& L, std::vector<CollectItem> items, uint64_t* flag, uint64_t seq);
 void launch_copy(const Launch& L, const uint32_t* in, uint32_t* out, uint64_t words);
+// out[i] = w^(brev_bits(i) << shift) for i < 2^bits, w's powers from `pt`
+// (Montgomery form): the bit-reversed point tables of a geometry.
+void launch_pow_brev(const Launch& L, PowTable pt, uint32_t* out, int bits, int shift);
 
 // NTT job builders: the passes of one inverse / coset LDE. pass2 is empty
 // for single-pass sizes; otherwise pass1 must complete before pass2.

@@ -31,12 +31,7 @@ namespace z0p {
 constexpr uint32_t P_MOD = P;
 
 uint32_t brev_host(uint64_t x, int bits) {
-    uint64_t r = 0;
-    for (int i = 0; i < bits; ++i) {
-        r = (r << 1) | (x & 1);
-        x >>= 1;
-    }
-    return uint32_t(r);
+    return bits == 0 ? 0u : uint32_t(__builtin_bitreverse64(x) >> (64 - bits));
 }
 
 namespace {
@@ -121,17 +116,20 @@ const GeometryTables& Ctx::geometry(int logn, int logb) {
     t->zvan = upload(t->owned, zvan);
     {
         // Point tables in bit-reversed order, so consecutive positions load
-        // consecutive words.
-        auto wn = geometric_m(root_of_unity(uint32_t(logn)), n);
-        std::vector<uint32_t> xn(n);
-        for (uint64_t q = 0; q < n; ++q) xn[q] = wn[brev_host(q, logn)];
-        t->xn = upload(t->owned, xn);
+        // consecutive words; generated on the device from the power tables.
+        void* xn = nullptr;
+        void* finv = nullptr;
         const uint64_t quads = std::max<uint64_t>(m / 4, 1);
-        const int lq = int(ilog2(quads));
-        auto wi = geometric_m(wm_inv, quads);
-        std::vector<uint32_t> finv(quads);
-        for (uint64_t i = 0; i < quads; ++i) finv[i] = wi[brev_host(i, lq)];
-        t->finv = upload(t->owned, finv);
+        hip_check(hipMalloc(&xn, n * 4));
+        t->owned.push_back(xn);
+        hip_check(hipMalloc(&finv, quads * 4));
+        t->owned.push_back(finv);
+        Launch L = launch();
+        launch_pow_brev(L, t->w, static_cast<uint32_t*>(xn), logn, logb);
+        launch_pow_brev(L, t->winv, static_cast<uint32_t*>(finv), int(ilog2(quads)), 0);
+        hip_check(hipStreamSynchronize(stream));
+        t->xn = static_cast<uint32_t*>(xn);
+        t->finv = static_cast<uint32_t*>(finv);
     }
     NttShape sh = ntt_shape(logn);
     if (sh.log1 == 0) {
