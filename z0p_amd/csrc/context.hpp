@@ -4,10 +4,15 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <condition_variable>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "kernels.hpp"
@@ -149,6 +154,67 @@ class DescRing final : public TableSink {
     bool replay_ = false;
 };
 
+// Persistent host worker threads for the per-proof transcript steps of a
+// scheduler iteration (each proof owns its transcript, so they are
+// independent). run() also works on the calling thread.
+class HostPool {
+   public:
+    explicit HostPool(unsigned workers) {
+        for (unsigned i = 0; i < workers; ++i) threads_.emplace_back([this] { loop(); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : threads_) t.join();
+    }
+    size_t workers() const { return threads_.size(); }
+    void run(size_t count, const std::function<void(size_t)>& f) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &f;
+            count_ = count;
+            next_.store(0);
+            busy_ = threads_.size();
+            ++gen_;
+        }
+        cv_.notify_all();
+        drain();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [this] { return busy_ == 0; });
+        job_ = nullptr;
+    }
+
+   private:
+    void drain() {
+        for (size_t i = next_.fetch_add(1); i < count_; i = next_.fetch_add(1)) (*job_)(i);
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            drain();
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--busy_ == 0) done_.notify_all();
+        }
+    }
+    std::vector<std::thread> threads_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(size_t)>* job_ = nullptr;
+    size_t count_ = 0, busy_ = 0;
+    std::atomic<size_t> next_{0};
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -175,6 +241,7 @@ struct Ctx {
     uint32_t* tw4096 = nullptr;      // [2048] omega_4096^e, Montgomery
     uint32_t* tw4096_inv = nullptr;  // [2048] omega_4096^{-e}, Montgomery
     std::map<std::pair<int, int>, std::unique_ptr<GeometryTables>> tables;
+    std::unique_ptr<HostPool> host_pool;  // created on first use by a batch
 
     Launch launch() { return Launch{stream, &prof, &ring, false}; }
 

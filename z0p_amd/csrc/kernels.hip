@@ -996,13 +996,13 @@ __global__ __launch_bounds__(WG) void k_grind(GrindJob inl, const GrindJob* tab,
 }
 
 // ---------------------------------------------------------------------------
-constexpr int COLLECT_WG = 64;
+constexpr int COLLECT_WG_MAX = 1024;
 
-__global__ __launch_bounds__(COLLECT_WG) void k_collect(CollectItem inl, const CollectItem* tab, uint32_t nitems,
-                                                        uint64_t* flag, uint64_t seq) {
+__global__ __launch_bounds__(COLLECT_WG_MAX) void k_collect(CollectItem inl, const CollectItem* tab, uint32_t nitems,
+                                                            uint64_t* flag, uint64_t seq) {
     for (uint32_t i = 0; i < nitems; ++i) {
         const CollectItem J = nitems == 1 ? inl : tab[i];
-        for (uint32_t k = threadIdx.x; k < J.words; k += COLLECT_WG) J.dst[k] = J.src[k];
+        for (uint32_t k = threadIdx.x; k < J.words; k += blockDim.x) J.dst[k] = gc(J.src)[k];
     }
     __threadfence_system();
     __syncthreads();
@@ -1253,10 +1253,13 @@ void launch_bitrev(const Launch& L, std::vector<BitrevJob> jobs) {
 void launch_collect(const Launch& L, std::vector<CollectItem> items, uint64_t* flag, uint64_t seq) {
     const CollectItem* table = table_of(L, items);
     if (L.dry) return;
+    uint64_t words = 0;
+    for (const CollectItem& it : items) words += it.words;
+    // One wave for the few words of one proof; a full workgroup for a batch.
+    const unsigned threads = words <= 4096 ? 64u : unsigned(COLLECT_WG_MAX);
     const CollectItem inl = items.empty() ? CollectItem{nullptr, nullptr, 0, 0} : items[0];
     ProfScope ps(L.prof, "collect", L.stream);
-    hipLaunchKernelGGL(k_collect, dim3(1), dim3(COLLECT_WG), 0, L.stream, inl, table, uint32_t(items.size()), flag,
-                       seq);
+    hipLaunchKernelGGL(k_collect, dim3(1), dim3(threads), 0, L.stream, inl, table, uint32_t(items.size()), flag, seq);
 }
 
 void launch_pow_brev(const Launch& L, PowTable pt, uint32_t* out, int bits, int shift) {

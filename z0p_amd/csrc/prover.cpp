@@ -931,19 +931,51 @@ bool Engine::iterate() {
     std::vector<CollectItem> items;
     size_t used = 0;
     for (size_t i = 0; i < active_.size(); ++i) build(c_, *active_[i], chains[i], items, used);
+    // Grind sweeps: a job's workgroups scan its nonces in order, so a
+    // smaller sweep per job wastes fewer hashes above the smallest passing
+    // nonce; spread about 4096 workgroups over the jobs grinding now.
+    size_t grinding = 0;
+    for (auto& ch : chains)
+        for (auto& st : ch) grinding += std::holds_alternative<GrindJob>(st) ? 1 : 0;
+    if (grinding) {
+        const uint32_t per = uint32_t(std::min<size_t>(2048, std::max<size_t>(16, 4096 / grinding)));
+        for (auto& ch : chains)
+            for (auto& st : ch)
+                if (auto* g = std::get_if<GrindJob>(&st)) g->nblocks = per;
+    }
     std::vector<const std::vector<Step>*> ptrs;
     for (auto& ch : chains) ptrs.push_back(&ch);
     const uint64_t seq = ++c_.seq;
     c_.enqueue([&](const Launch& L) { launch_chains(L, ptrs, items, c_.flag_dev, seq); });
     c_.wait_flag(seq);
     ++iterations_;
-    std::vector<Proof*> still;
-    for (Proof* p : active_) {
+    // Host transcript steps: independent per proof, so a batch spreads them
+    // over the host pool.
+    std::vector<int> err(active_.size(), Z0P_OK);
+    auto step = [&](size_t i) {
         try {
-            host_step(c_, *p);
+            host_step(c_, *active_[i]);
         } catch (const Status& s) {
+            err[i] = s.code;
+        } catch (...) {
+            err[i] = Z0P_ERR_DEVICE;
+        }
+    };
+    if (active_.size() >= 4) {
+        if (!c_.host_pool) {
+            const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+            c_.host_pool.reset(new HostPool(std::min(hw, 16u) - 1));
+        }
+        c_.host_pool->run(active_.size(), step);
+    } else {
+        for (size_t i = 0; i < active_.size(); ++i) step(i);
+    }
+    std::vector<Proof*> still;
+    for (size_t i = 0; i < active_.size(); ++i) {
+        Proof* p = active_[i];
+        if (err[i] != Z0P_OK) {
             stage_need_ -= p->stage_need;
-            finish(p, s.code);
+            finish(p, err[i]);
             continue;
         }
         if (p->phase == PH_DONE) {
