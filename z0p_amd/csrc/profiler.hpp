@@ -35,6 +35,15 @@ struct Profiler {
         return e;
     }
 
+    // Host-side intervals of the scheduler (build + launch, device wait,
+    // transcript steps), recorded under the given name whether or not
+    // kernel events are enabled (two clock reads per iteration).
+    void host(const char* name, double ms) {
+        Stat& st = stats[name];
+        st.launches += 1;
+        st.total_ms += ms;
+    }
+
     // Call after the stream has been synchronised.
     void resolve() {
         for (auto& p : pending) {

@@ -927,6 +927,7 @@ void Engine::admit() {
 bool Engine::iterate() {
     admit();
     if (active_.empty()) return !queue_.empty();
+    const double t_build = now_ms();
     std::vector<std::vector<Step>> chains(active_.size());
     std::vector<CollectItem> items;
     size_t used = 0;
@@ -947,7 +948,9 @@ bool Engine::iterate() {
     for (auto& ch : chains) ptrs.push_back(&ch);
     const uint64_t seq = ++c_.seq;
     c_.enqueue([&](const Launch& L) { launch_chains(L, ptrs, items, c_.flag_dev, seq); });
+    const double t_wait = now_ms();
     c_.wait_flag(seq);
+    const double t_host = now_ms();
     ++iterations_;
     // Host transcript steps: independent per proof, so a batch spreads them
     // over the host pool.
@@ -970,6 +973,9 @@ bool Engine::iterate() {
     } else {
         for (size_t i = 0; i < active_.size(); ++i) step(i);
     }
+    c_.prof.host("~host_build_launch", t_wait - t_build);
+    c_.prof.host("~host_wait_device", t_host - t_wait);
+    c_.prof.host("~host_transcript", now_ms() - t_host);
     std::vector<Proof*> still;
     for (size_t i = 0; i < active_.size(); ++i) {
         Proof* p = active_[i];
