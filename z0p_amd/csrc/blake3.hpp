@@ -54,9 +54,7 @@ Z0P_HD void compress8(const uint32_t cv[8], const uint32_t m[16], uint32_t count
     uint32_t s0 = cv[0], s1 = cv[1], s2 = cv[2], s3 = cv[3], s4 = cv[4], s5 = cv[5], s6 = cv[6], s7 = cv[7];
     uint32_t s8 = b3::IV0, s9 = b3::IV1, s10 = b3::IV2, s11 = b3::IV3;
     uint32_t s12 = counter_lo, s13 = counter_hi, s14 = block_len, s15 = flags;
-#if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
-#endif
     for (int r = 0; r < 7; ++r) {
         Z0P_B3_G(s0, s4, s8, s12, m[S.idx[r][0]], m[S.idx[r][1]]);
         Z0P_B3_G(s1, s5, s9, s13, m[S.idx[r][2]], m[S.idx[r][3]]);

@@ -756,8 +756,12 @@ void write_fri_proof_body(Writer& w, const Proof& P, const uint32_t* words) {
 
 // Proof::to_bytes (air.rs:262-281) / the FRI commitment and proof.
 void serialize(Proof& P, const uint32_t* words) {
+    // Opened words dominate the size: 4 bytes each plus a direction byte
+    // per path digest; caps and headers add a few kilobytes.
+    const size_t estimate = size_t(P.wpq) * P.queries.size() * 5 + 65536;
     if (P.air) {
         Writer w;
+        w.out.reserve(estimate);
         w.byte(WIRE_VERSION);
         write_cap(w, P.column_cap);
         write_fri_commitment_body(w, P);
@@ -788,6 +792,7 @@ void serialize(Proof& P, const uint32_t* words) {
         write_fri_commitment_body(wc, P);
         P.commitment = std::move(wc.out);
         Writer wp;
+        wp.out.reserve(estimate);
         wp.byte(WIRE_VERSION);
         write_fri_proof_body(wp, P, words);
         P.proof = std::move(wp.out);
