@@ -233,6 +233,14 @@ struct Ctx {
     uint64_t* flag_host = nullptr;
     uint64_t* flag_dev = nullptr;
     uint64_t seq = 0;
+    // Mailbox for pre-enqueued phases (pinned host memory, device-mapped):
+    // word 0 = gate counter the device polls, word 1 = gate timeout flag,
+    // then MAILBOX_SLOTS constant slots of MAILBOX_SLOT bytes.
+    static constexpr size_t MAILBOX_SLOT = 256, MAILBOX_SLOTS = 32, MAILBOX_HEAD = 256;
+    uint8_t* mailbox_host = nullptr;
+    uint8_t* mailbox_dev = nullptr;
+    uint64_t gate = 0;
+    bool pipeline_single = true;  // pre-enqueue the phases of a lone proof
 
     // Scratch arena for the kernel-level diagnostic hooks.
     uint8_t* scratch = nullptr;
@@ -274,7 +282,9 @@ struct Ctx {
     // Waits until the flag reaches `s` (spinning: the kernels are short and
     // the host transcript step is on the critical path). The stream is
     // queried now and then so a device failure surfaces as an error.
-    void wait_flag(uint64_t s) {
+    // sync_events: when profiling, synchronise to resolve kernel events
+    // (not while pre-enqueued phases still wait on host gates).
+    void wait_flag(uint64_t s, bool sync_events = true) {
         const volatile uint64_t* f = flag_host;
         for (uint64_t spins = 1;; ++spins) {
             if (__atomic_load_n(const_cast<const uint64_t*>(f), __ATOMIC_ACQUIRE) >= s) break;
@@ -287,7 +297,7 @@ struct Ctx {
                 if (e != hipErrorNotReady) hip_check(e);
             }
         }
-        if (prof.enabled) {
+        if (prof.enabled && sync_events) {
             hip_check(hipStreamSynchronize(stream));
             prof.resolve();
         }

@@ -595,7 +595,8 @@ __global__ __launch_bounds__(WG) void k_compose(ComposeJob inl, const ComposeJob
     const uint32_t base = local * (WG * CELT) + threadIdx.x;
     const gcptr A = gc(J.a), Bv = gc(J.b);
     // x - w_last = (x - 1) - (w_last - 1): only x - 1 is kept per element.
-    const uint32_t wl1 = sub(J.c.w_last_m, R1);
+    const ComposeConsts C = *J.c;
+    const uint32_t wl1 = sub(C.w_last_m, R1);
     uint32_t d1[CELT], pr[CELT], na[CELT], nb[CELT];
     // Scattered next-row loads first, overlapping the batched inversion.
 #pragma unroll
@@ -626,15 +627,15 @@ __global__ __launch_bounds__(WG) void k_compose(ComposeJob inl, const ComposeJob
         const uint32_t va = A[p], vb = Bv[p];
         const uint32_t s0 = mont_mul(sub(na[e], vb), iz);
         const uint32_t s1 = mont_mul(sub(nb[e], add(va, vb)), iz);
-        const uint32_t s2 = mont_mul(sub(va, J.c.a0), i1);
-        const uint32_t s3 = mont_mul(sub(vb, J.c.b0), i1);
-        const uint32_t s4 = mont_mul(sub(vb, J.c.b_last), iw);
+        const uint32_t s2 = mont_mul(sub(va, C.a0), i1);
+        const uint32_t s3 = mont_mul(sub(vb, C.b0), i1);
+        const uint32_t s4 = mont_mul(sub(vb, C.b_last), iw);
         uint32_t o[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const uint32_t p01 = mont_mul2(s0, J.c.alpha_m[0][k], s1, J.c.alpha_m[1][k]);
-            const uint32_t p23 = mont_mul2(s2, J.c.alpha_m[2][k], s3, J.c.alpha_m[3][k]);
-            o[k] = add(add(p01, p23), mont_mul(s4, J.c.alpha_m[4][k]));
+            const uint32_t p01 = mont_mul2(s0, C.alpha_m[0][k], s1, C.alpha_m[1][k]);
+            const uint32_t p23 = mont_mul2(s2, C.alpha_m[2][k], s3, C.alpha_m[3][k]);
+            o[k] = add(add(p01, p23), mont_mul(s4, C.alpha_m[4][k]));
         }
         st4(J.layer0, p, u32x4{o[0], o[1], o[2], o[3]});
     });
@@ -923,10 +924,11 @@ __global__ __launch_bounds__(TAIL_WG) void k_tail(TailJob inl, const TailJob* ta
 __global__ __launch_bounds__(WG) void k_fold(FoldJob inl, const FoldJob* tab, uint32_t njobs) {
     uint32_t local;
     const FoldJob J = job_of(inl, tab, njobs, local);
-    const uint64_t quads = uint64_t(1) << (J.c.log_len - 2);
+    const uint64_t quads = uint64_t(1) << (J.log_len - 2);
     const uint64_t i = uint64_t(local) * WG + threadIdx.x;
     if (i >= quads) return;
-    const uint32_t inv_x = mont_mul(gc(J.finv)[i], J.c.ginv_m);
+    const FoldConsts& C = *J.c;
+    const uint32_t inv_x = mont_mul(gc(J.finv)[i], C.ginv_m);
     E4 q[4];
     if (J.base_field) {
         const u32x4 v = ld4(J.layer, i);
@@ -940,12 +942,12 @@ __global__ __launch_bounds__(WG) void k_fold(FoldJob inl, const FoldJob* tab, ui
     }
     const E4 s01 = e4_add(q[0], q[1]), s23 = e4_add(q[2], q[3]);
     const E4 dodd = e4_sub(q[0], q[1]);
-    const E4 dskew = e4_scale_m(e4_sub(q[2], q[3]), J.c.unit_m);
+    const E4 dskew = e4_scale_m(e4_sub(q[2], q[3]), C.unit_m);
     const E4 M = e4_add(s01, s23), O = e4_sub(dodd, dskew), Ev = e4_sub(s01, s23), S = e4_add(dodd, dskew);
-    E4 acc = e4_scale_m(e4_mul_mat(S, J.c.cm[2]), inv_x);
-    acc = e4_scale_m(e4_add(acc, e4_mul_mat(Ev, J.c.cm[1])), inv_x);
-    acc = e4_add(acc, e4_mul_mat(O, J.c.cm[0]));
-    const E4 out = e4_add(e4_scale_m(M, J.c.inv4_m), e4_scale_m(acc, inv_x));
+    E4 acc = e4_scale_m(e4_mul_mat(S, C.cm[2]), inv_x);
+    acc = e4_scale_m(e4_add(acc, e4_mul_mat(Ev, C.cm[1])), inv_x);
+    acc = e4_add(acc, e4_mul_mat(O, C.cm[0]));
+    const E4 out = e4_add(e4_scale_m(M, C.inv4_m), e4_scale_m(acc, inv_x));
     st4(J.next, i, u32x4{out.v[0], out.v[1], out.v[2], out.v[3]});
 }
 
@@ -970,6 +972,9 @@ __global__ void k_grind_init(GrindJob inl, const GrindJob* tab, uint32_t njobs) 
 __global__ __launch_bounds__(WG) void k_grind(GrindJob inl, const GrindJob* tab, uint32_t njobs) {
     uint32_t local;
     const GrindJob J = job_of(inl, tab, njobs, local);
+    uint32_t hw[16];
+#pragma unroll
+    for (int w = 0; w < 16; ++w) hw[w] = gc(J.hw)[w];
     const uint64_t chunk = uint64_t(WG) * GRIND_PER_THREAD;
     const uint64_t stride = uint64_t(J.nblocks) * chunk;
     for (uint64_t start = uint64_t(local) * chunk; start < J.limit; start += stride) {
@@ -979,7 +984,7 @@ __global__ __launch_bounds__(WG) void k_grind(GrindJob inl, const GrindJob* tab,
             if (nonce >= J.limit) break;
             uint32_t m[16];
 #pragma unroll
-            for (int w = 0; w < 16; ++w) m[w] = J.hw[w];
+            for (int w = 0; w < 16; ++w) m[w] = hw[w];
             // nonce bytes occupy preimage bytes 47..54: word 11 byte 3, words 12-13
             uint32_t lo = uint32_t(nonce), hi = uint32_t(nonce >> 32);
             m[11] |= lo << 24;
@@ -1007,6 +1012,29 @@ __global__ __launch_bounds__(COLLECT_WG_MAX) void k_collect(CollectItem inl, con
     __threadfence_system();
     __syncthreads();
     if (threadIdx.x == 0 && flag) __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Device-side gate of a pre-enqueued phase: lane 0 polls a host-written
+// flag (system scope) until it reaches `want`, then the wave copies the
+// phase's constants from host memory to device memory. The poll is bounded:
+// after `spin_limit` polls the kernel records a timeout and exits, so a
+// host that never answers cannot hang the device.
+__global__ __launch_bounds__(64) void k_wait_host(const uint64_t* go, uint64_t want, const uint32_t* src,
+                                                  uint32_t* dst, uint32_t words, uint32_t spin_limit,
+                                                  uint32_t* timeout) {
+    __shared__ uint32_t ok;
+    if (threadIdx.x == 0) {
+        uint32_t i = 0;
+        for (; i < spin_limit; ++i) {
+            if (__hip_atomic_load(go, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) >= want) break;
+            __builtin_amdgcn_s_sleep(2);
+        }
+        ok = i < spin_limit ? 1u : 0u;
+        if (!ok) __hip_atomic_store(timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < words; k += 64)
+        dst[k] = __hip_atomic_load(src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Plain streaming copy: the calibration reference for FETCH/WRITE_SIZE.
@@ -1226,7 +1254,7 @@ void launch_tail(const Launch& L, std::vector<TailJob> jobs) {
 }
 
 void launch_fold(const Launch& L, std::vector<FoldJob> jobs) {
-    unsigned g = assign_blocks(jobs, [](const FoldJob& j) { return grid_for(uint64_t(1) << (j.c.log_len - 2), WG); });
+    unsigned g = assign_blocks(jobs, [](const FoldJob& j) { return grid_for(uint64_t(1) << (j.log_len - 2), WG); });
     run(L, "fold", k_fold, jobs, g, 0);
 }
 
@@ -1266,6 +1294,13 @@ void launch_pow_brev(const Launch& L, PowTable pt, uint32_t* out, int bits, int 
     if (L.dry) return;
     hipLaunchKernelGGL(k_pow_brev, dim3(grid_for(uint64_t(1) << bits, WG)), dim3(WG), 0, L.stream, pt, out, bits,
                        shift);
+}
+
+void launch_wait_host(const Launch& L, const uint64_t* go, uint64_t want, const uint32_t* src, uint32_t* dst,
+                      uint32_t words, uint32_t spin_limit, uint32_t* timeout) {
+    if (L.dry) return;
+    ProfScope ps(L.prof, "wait_host", L.stream);
+    hipLaunchKernelGGL(k_wait_host, dim3(1), dim3(64), 0, L.stream, go, want, src, dst, words, spin_limit, timeout);
 }
 
 void launch_copy(const Launch& L, const uint32_t* in, uint32_t* out, uint64_t words) {

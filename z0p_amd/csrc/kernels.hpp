@@ -108,7 +108,7 @@ struct ComposeJob {
     const uint32_t* xn;  // [n] omega_n^brev(q), Montgomery
     const uint32_t* shift_m;
     const uint32_t* zinv_m;  // [B] 1/(s_j^n - 1)
-    ComposeConsts c;
+    const ComposeConsts* c;  // device memory: the alphas come from the transcript
     int logn, logb;
     uint32_t block_begin;
 };
@@ -174,13 +174,14 @@ struct FoldJob {
     // [m/4] Omega_m^-brev(i) over the layer-0 quads (Montgomery); layer d
     // uses its first m/4^(d+1) entries: brev_k(i) 4^d = brev_{k+2d}(i).
     const uint32_t* finv;
-    FoldConsts c;
+    const FoldConsts* c;  // device memory: the challenges come from the transcript
+    int log_len;          // = c->log_len, for the grid
     int base_field;
     uint32_t block_begin;
 };
 
 struct GrindJob {
-    uint32_t hw[16];  // 47-byte head as LE words, zero-padded
+    const uint32_t* hw;  // device memory: 47-byte head as 16 LE words, zero-padded
     uint32_t bits;
     uint32_t nblocks;
     uint64_t limit;
@@ -220,6 +221,11 @@ struct CollectItem {
 };
 void launch_collect(const Launch& L, std::vector<CollectItem> items, uint64_t* flag, uint64_t seq);
 void launch_copy(const Launch& L, const uint32_t* in, uint32_t* out, uint64_t words);
+// Gate for work enqueued ahead of its inputs: waits (bounded by spin_limit
+// polls) until the host-mapped *go reaches `want`, then copies `words`
+// constants from host-mapped src to device dst; sets *timeout on expiry.
+void launch_wait_host(const Launch& L, const uint64_t* go, uint64_t want, const uint32_t* src, uint32_t* dst,
+                      uint32_t words, uint32_t spin_limit, uint32_t* timeout);
 // out[i] = w^(brev_bits(i) << shift) for i < 2^bits, w's powers from `pt`
 // (Montgomery form): the bit-reversed point tables of a geometry.
 void launch_pow_brev(const Launch& L, PowTable pt, uint32_t* out, int bits, int shift);

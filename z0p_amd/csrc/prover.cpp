@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <variant>
@@ -164,6 +165,7 @@ Ctx::~Ctx() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (stage_host) (void)hipHostFree(stage_host);
     if (flag_host) (void)hipHostFree(flag_host);
+    if (mailbox_host) (void)hipHostFree(mailbox_host);
     if (scratch) (void)hipFree(scratch);
     if (tw4096) (void)hipFree(tw4096);
     if (tw4096_inv) (void)hipFree(tw4096_inv);
@@ -185,10 +187,15 @@ Ctx* ctx_create(int device) {
     hip_check(hipMemcpy(c->tw4096, fwd.data(), 2048 * 4, hipMemcpyHostToDevice));
     hip_check(hipMemcpy(c->tw4096_inv, inv.data(), 2048 * 4, hipMemcpyHostToDevice));
     c->ring.init(size_t(32) << 20);
+    if (const char* e = std::getenv("Z0P_PIPELINE")) c->pipeline_single = std::strcmp(e, "0") != 0;
     c->reserve_stage(size_t(1) << 16);
     hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->flag_host), 256, hipHostMallocMapped | hipHostMallocCoherent));
     hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->flag_dev), c->flag_host, 0));
     *c->flag_host = 0;
+    const size_t mb = Ctx::MAILBOX_HEAD + Ctx::MAILBOX_SLOTS * Ctx::MAILBOX_SLOT;
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->mailbox_host), mb, hipHostMallocMapped | hipHostMallocCoherent));
+    hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->mailbox_dev), c->mailbox_host, 0));
+    std::memset(c->mailbox_host, 0, mb);
     return c.release();
 }
 
@@ -392,6 +399,7 @@ struct Proof {
     uint32_t *col = nullptr, *coeff = nullptr, *scratch = nullptr, *A = nullptr, *Bc = nullptr, *trace_tree = nullptr;
     std::vector<uint32_t*> vals, trees;
     unsigned long long* grind_res = nullptr;
+    uint8_t* consts = nullptr;  // per-phase transcript constants (pre-enqueued phases)
     uint32_t wpq = 0, fri_wpq = 0;  // opening words per query (all / FRI part)
     // AIR constants
     uint32_t a0 = 0, b0 = 0, b_last = 0, coeff_c = 0, w_last = 0;
@@ -439,6 +447,7 @@ size_t layout(Proof& P, uint8_t* base) {
         if (d < P.folds) P.trees[d] = take(P.geoms[d].words());
     }
     P.grind_res = reinterpret_cast<unsigned long long*>(take(2));
+    P.consts = reinterpret_cast<uint8_t*>(take(Ctx::MAILBOX_SLOTS * Ctx::MAILBOX_SLOT / 4));
     return off;
 }
 
@@ -557,8 +566,11 @@ LeavesJob leaves_of(int kind, const uint32_t* src) {
     return j;
 }
 
-// The device work of this proof's current phase.
-void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& items, size_t& used) {
+// The device work of this proof's current phase. `cst` is the device
+// address of this phase's transcript constants when they are supplied
+// later (pre-enqueued phases); otherwise they are uploaded now.
+void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& items, size_t& used,
+           const void* cst = nullptr) {
     const GeometryTables& gt = *P.gt;
     switch (P.phase) {
         case PH_FRONT: {
@@ -606,7 +618,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
             cj.xn = gt.xn;
             cj.shift_m = gt.coset.shift;
             cj.zinv_m = gt.zinv;
-            cj.c = P.cc;
+            cj.c = static_cast<const ComposeConsts*>(cst ? cst : c.ring.push(&P.cc, sizeof(P.cc)));
             cj.logn = P.logn;
             cj.logb = P.logb;
             ch.push_back(cj);
@@ -620,7 +632,8 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
             fj.layer = P.vals[d];
             fj.next = P.vals[d + 1];
             fj.finv = gt.finv;
-            fj.c = P.fold;
+            fj.c = static_cast<const FoldConsts*>(cst ? cst : c.ring.push(&P.fold, sizeof(P.fold)));
+            fj.log_len = int(ilog2(P.lens[d]));
             fj.base_field = d == 0 && !P.air ? 1 : 0;
             ch.push_back(fj);
             if (d + 1 == P.folds) {
@@ -635,7 +648,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
         }
         case PH_GRIND: {
             GrindJob g{};
-            std::memcpy(g.hw, P.pow_hw, sizeof(g.hw));
+            g.hw = static_cast<const uint32_t*>(cst ? cst : c.ring.push(P.pow_hw, sizeof(P.pow_hw)));
             g.bits = P.fp.grinding_bits;
             g.nblocks = 2048;
             g.limit = uint64_t(1) << std::min<uint64_t>(63, 24 + 4 * uint64_t(P.window));
@@ -932,6 +945,17 @@ void Engine::admit() {
 bool Engine::iterate() {
     admit();
     if (active_.empty()) return !queue_.empty();
+    if (pipelined_ && c_.pipeline_single && active_.size() == 1 && queue_.empty() && active_[0]->phase == PH_FRONT) {
+        Proof* p = active_[0];
+        try {
+            pipeline(*p);
+        } catch (const Status& s) {
+            stage_need_ -= p->stage_need;
+            finish(p, s.code);
+            active_.clear();
+        }
+        return !active_.empty() || !queue_.empty();
+    }
     const double t_build = now_ms();
     std::vector<std::vector<Step>> chains(active_.size());
     std::vector<CollectItem> items;
@@ -1000,6 +1024,87 @@ bool Engine::iterate() {
     return !active_.empty() || !queue_.empty();
 }
 
+// All phases from FRONT through the first proof-of-work window enqueued at
+// once. Phase k > 0 starts behind a device gate that opens when the host
+// has run phase k-1's transcript step and written phase k's constants
+// (composition alphas, fold challenges, PoW head) to the mailbox.
+void Engine::pipeline(Proof& P) {
+    struct Ph {
+        Phase ph;
+        size_t depth;
+        size_t stage_off;
+    };
+    std::vector<Ph> phs{{PH_FRONT, 0, 0}};
+    if (P.air) phs.push_back({PH_COMPOSE, 0, 0});
+    for (size_t d = 0; d < P.folds; ++d) phs.push_back({PH_FOLD, d, 0});
+    if (P.fp.grinding_bits > 0) phs.push_back({PH_GRIND, 0, 0});
+    require(phs.size() <= Ctx::MAILBOX_SLOTS, Z0P_ERR_DEVICE);
+    constexpr size_t SLOT = Ctx::MAILBOX_SLOT;
+    static_assert(sizeof(ComposeConsts) <= SLOT && sizeof(FoldConsts) <= SLOT && 64 <= SLOT, "mailbox slot");
+    c_.reserve_stage(P.stage_need * phs.size());
+    std::vector<std::vector<Step>> chains(phs.size());
+    std::vector<std::vector<CollectItem>> items(phs.size());
+    size_t used = 0;
+    const double t_build = now_ms();
+    for (size_t k = 0; k < phs.size(); ++k) {
+        P.phase = phs[k].ph;
+        P.depth = phs[k].depth;
+        P.window = 0;
+        build(c_, P, chains[k], items[k], used, k == 0 ? nullptr : P.consts + k * SLOT);
+        phs[k].stage_off = P.stage_off;
+    }
+    P.phase = PH_FRONT;
+    P.depth = 0;
+    uint64_t* gate = reinterpret_cast<uint64_t*>(c_.mailbox_host);
+    uint32_t* timeout = reinterpret_cast<uint32_t*>(c_.mailbox_host + 8);
+    __atomic_store_n(timeout, 0u, __ATOMIC_RELEASE);
+    const uint64_t g0 = c_.gate, g_end = g0 + phs.size(), seq0 = c_.seq;
+    c_.enqueue([&](const Launch& L) {
+        for (size_t k = 0; k < phs.size(); ++k) {
+            if (k > 0)
+                launch_wait_host(L, reinterpret_cast<const uint64_t*>(c_.mailbox_dev), g0 + k,
+                                 reinterpret_cast<const uint32_t*>(c_.mailbox_dev + Ctx::MAILBOX_HEAD + k * SLOT),
+                                 reinterpret_cast<uint32_t*>(P.consts + k * SLOT), uint32_t(SLOT / 4), 1u << 22,
+                                 reinterpret_cast<uint32_t*>(c_.mailbox_dev + 8));
+            const std::vector<const std::vector<Step>*> one{&chains[k]};
+            launch_chains(L, one, items[k], c_.flag_dev, seq0 + 1 + k);
+        }
+    });
+    c_.seq = seq0 + phs.size();
+    c_.prof.host("~host_build_launch", now_ms() - t_build);
+    try {
+        for (size_t k = 0; k < phs.size(); ++k) {
+            const double t_wait = now_ms();
+            c_.wait_flag(seq0 + 1 + k, false);
+            if (__atomic_load_n(timeout, __ATOMIC_ACQUIRE)) throw Status(Z0P_ERR_DEVICE);
+            const double t_host = now_ms();
+            P.phase = phs[k].ph;
+            P.depth = phs[k].depth;
+            P.stage_off = phs[k].stage_off;
+            host_step(c_, P);
+            if (k + 1 < phs.size()) {
+                const Ph& nx = phs[k + 1];
+                require(P.phase == nx.ph && (nx.ph != PH_FOLD || P.depth == nx.depth), Z0P_ERR_DEVICE);
+                uint8_t* slot = c_.mailbox_host + Ctx::MAILBOX_HEAD + (k + 1) * SLOT;
+                if (nx.ph == PH_COMPOSE) std::memcpy(slot, &P.cc, sizeof(P.cc));
+                else if (nx.ph == PH_FOLD) std::memcpy(slot, &P.fold, sizeof(P.fold));
+                else std::memcpy(slot, P.pow_hw, sizeof(P.pow_hw));
+                __atomic_store_n(gate, g0 + k + 1, __ATOMIC_RELEASE);
+            }
+            c_.prof.host("~host_wait_device", t_host - t_wait);
+            c_.prof.host("~host_transcript", now_ms() - t_host);
+        }
+    } catch (...) {
+        // Open every remaining gate so the enqueued work drains, then fail.
+        __atomic_store_n(gate, g_end, __ATOMIC_RELEASE);
+        (void)hipStreamSynchronize(c_.stream);
+        c_.gate = g_end;
+        c_.ring.reset();
+        throw;
+    }
+    c_.gate = g_end;
+}
+
 void Engine::fail_all(int status) {
     for (Proof* p : active_) finish(p, status);
     active_.clear();
@@ -1052,6 +1157,7 @@ uint64_t proof_tag(const Proof* p) { return p->tag; }
 std::vector<uint8_t> air_prove(Ctx& ctx, const z0p_air_params& p, Transcript& tr) {
     std::unique_ptr<Proof> P(proof_new_air(ctx, p, &tr));
     Engine e(ctx, 0, 1);
+    e.set_pipelined(true);
     e.submit(P.get());
     e.run();
     if (P->status != Z0P_OK) throw Status(P->status);
@@ -1062,6 +1168,7 @@ void fri_commit_open(Ctx& ctx, const uint32_t* coeffs, uint64_t n, const z0p_fri
                      std::vector<uint8_t>& commitment, std::vector<uint8_t>& proof) {
     std::unique_ptr<Proof> P(proof_new_fri(ctx, coeffs, n, fp, &tr));
     Engine e(ctx, 0, 1);
+    e.set_pipelined(true);
     e.submit(P.get());
     e.run();
     if (P->status != Z0P_OK) throw Status(P->status);

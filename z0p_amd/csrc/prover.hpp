@@ -78,6 +78,10 @@ class Engine {
     // Ends every queued and active proof with `status`.
     void fail_all(int status);
     void on_done(std::function<void(Proof*)> f) { on_done_ = std::move(f); }
+    // Latency mode: a proof alone in the scheduler has all its phases up to
+    // the proof-of-work enqueued at once, each gated on the device by the
+    // host's transcript step (no launch latency between phases).
+    void set_pipelined(bool on) { pipelined_ = on; }
     size_t active() const { return active_.size(); }
     size_t queued() const { return queue_.size(); }
     uint64_t iterations() const { return iterations_; }
@@ -85,6 +89,7 @@ class Engine {
    private:
     void admit();
     void finish(Proof* p, int status);
+    void pipeline(Proof& p);
     Ctx& c_;
     size_t budget_, max_active_;
     std::deque<Proof*> queue_;
@@ -92,6 +97,7 @@ class Engine {
     size_t stage_need_ = 0;
     uint64_t iterations_ = 0;
     std::function<void(Proof*)> on_done_;
+    bool pipelined_ = false;
 };
 
 // Budget for proofs needing `want` bytes in total: at most 80% of the device

@@ -218,7 +218,7 @@ int z0p_test_compose(z0p_ctx* ctx, const uint32_t* a, const uint32_t* b, size_t 
         cj.xn = gt.xn;
         cj.shift_m = gt.coset.shift;
         cj.zinv_m = gt.zinv;
-        cj.c = cc;
+        cj.c = to_dev(c, &cc, 1);
         cj.logn = logn;
         cj.logb = logb;
         run_chain(c, {cj});
@@ -248,7 +248,8 @@ int z0p_test_fold(z0p_ctx* ctx, const uint32_t* layer, size_t len, size_t lde_le
         fj.layer = dl;
         fj.next = dn;
         fj.finv = gt.finv;
-        fj.c = fc;
+        fj.c = to_dev(c, &fc, 1);
+        fj.log_len = fc.log_len;
         fj.base_field = 0;
         run_chain(c, {fj});
         to_host(c, out, dn, len);
@@ -284,7 +285,9 @@ int z0p_test_grind(z0p_ctx* ctx, const uint8_t* head, size_t head_len, uint32_t 
         Ctx& c = *ctx->c;
         c.reserve_scratch(1 << 20);
         GrindJob g{};
-        for (size_t i = 0; i < head_len; ++i) g.hw[i >> 2] |= uint32_t(head[i]) << (8 * (i & 3));
+        uint32_t hw[16] = {0};
+        for (size_t i = 0; i < head_len; ++i) hw[i >> 2] |= uint32_t(head[i]) << (8 * (i & 3));
+        g.hw = to_dev(c, hw, 16);
         g.bits = bits;
         g.nblocks = 2048;
         g.result = c.scratch_alloc<unsigned long long>(1);
