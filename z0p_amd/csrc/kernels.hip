@@ -1004,7 +1004,7 @@ __global__ __launch_bounds__(WG) void k_grind(GrindJob inl, const GrindJob* tab,
 constexpr int COLLECT_WG_MAX = 1024;
 
 __global__ __launch_bounds__(COLLECT_WG_MAX) void k_collect(CollectItem inl, const CollectItem* tab, uint32_t nitems,
-                                                            uint64_t* flag, uint64_t seq) {
+                                                            uint64_t* flag, uint64_t seq, HostGate gate) {
     for (uint32_t i = 0; i < nitems; ++i) {
         const CollectItem J = nitems == 1 ? inl : tab[i];
         for (uint32_t k = threadIdx.x; k < J.words; k += blockDim.x) J.dst[k] = gc(J.src)[k];
@@ -1012,29 +1012,22 @@ __global__ __launch_bounds__(COLLECT_WG_MAX) void k_collect(CollectItem inl, con
     __threadfence_system();
     __syncthreads();
     if (threadIdx.x == 0 && flag) __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-
-// Device-side gate of a pre-enqueued phase: lane 0 polls a host-written
-// flag (system scope) until it reaches `want`, then the wave copies the
-// phase's constants from host memory to device memory. The poll is bounded:
-// after `spin_limit` polls the kernel records a timeout and exits, so a
-// host that never answers cannot hang the device.
-__global__ __launch_bounds__(64) void k_wait_host(const uint64_t* go, uint64_t want, const uint32_t* src,
-                                                  uint32_t* dst, uint32_t words, uint32_t spin_limit,
-                                                  uint32_t* timeout) {
-    __shared__ uint32_t ok;
+    if (gate.go == nullptr) return;
+    // Gate: lane 0 polls the host counter (system scope, bounded), then the
+    // workgroup copies the next phase's constants from host memory.
+    __shared__ uint32_t open;
     if (threadIdx.x == 0) {
         uint32_t i = 0;
-        for (; i < spin_limit; ++i) {
-            if (__hip_atomic_load(go, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) >= want) break;
+        for (; i < gate.spin_limit; ++i) {
+            if (__hip_atomic_load(gate.go, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) >= gate.want) break;
             __builtin_amdgcn_s_sleep(2);
         }
-        ok = i < spin_limit ? 1u : 0u;
-        if (!ok) __hip_atomic_store(timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        open = i < gate.spin_limit ? 1u : 0u;
+        if (!open) __hip_atomic_store(gate.timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < words; k += 64)
-        dst[k] = __hip_atomic_load(src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    for (uint32_t k = threadIdx.x; k < gate.words; k += blockDim.x)
+        gate.dst[k] = __hip_atomic_load(gate.src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Plain streaming copy: the calibration reference for FETCH/WRITE_SIZE.
@@ -1278,7 +1271,8 @@ void launch_bitrev(const Launch& L, std::vector<BitrevJob> jobs) {
     run(L, "bitrev", k_bitrev, jobs, g, 0);
 }
 
-void launch_collect(const Launch& L, std::vector<CollectItem> items, uint64_t* flag, uint64_t seq) {
+void launch_collect(const Launch& L, std::vector<CollectItem> items, uint64_t* flag, uint64_t seq,
+                    const HostGate& gate) {
     const CollectItem* table = table_of(L, items);
     if (L.dry) return;
     uint64_t words = 0;
@@ -1287,20 +1281,14 @@ void launch_collect(const Launch& L, std::vector<CollectItem> items, uint64_t* f
     const unsigned threads = words <= 4096 ? 64u : unsigned(COLLECT_WG_MAX);
     const CollectItem inl = items.empty() ? CollectItem{nullptr, nullptr, 0, 0} : items[0];
     ProfScope ps(L.prof, "collect", L.stream);
-    hipLaunchKernelGGL(k_collect, dim3(1), dim3(threads), 0, L.stream, inl, table, uint32_t(items.size()), flag, seq);
+    hipLaunchKernelGGL(k_collect, dim3(1), dim3(threads), 0, L.stream, inl, table, uint32_t(items.size()), flag, seq,
+                       gate);
 }
 
 void launch_pow_brev(const Launch& L, PowTable pt, uint32_t* out, int bits, int shift) {
     if (L.dry) return;
     hipLaunchKernelGGL(k_pow_brev, dim3(grid_for(uint64_t(1) << bits, WG)), dim3(WG), 0, L.stream, pt, out, bits,
                        shift);
-}
-
-void launch_wait_host(const Launch& L, const uint64_t* go, uint64_t want, const uint32_t* src, uint32_t* dst,
-                      uint32_t words, uint32_t spin_limit, uint32_t* timeout) {
-    if (L.dry) return;
-    ProfScope ps(L.prof, "wait_host", L.stream);
-    hipLaunchKernelGGL(k_wait_host, dim3(1), dim3(64), 0, L.stream, go, want, src, dst, words, spin_limit, timeout);
 }
 
 void launch_copy(const Launch& L, const uint32_t* in, uint32_t* out, uint64_t words) {

@@ -219,13 +219,24 @@ struct CollectItem {
     uint32_t words;
     uint32_t block_begin;
 };
-void launch_collect(const Launch& L, std::vector<CollectItem> items, uint64_t* flag, uint64_t seq);
+// Optional gate run by the same kernel after the flag: wait (bounded by
+// spin_limit polls) until the host-mapped *go reaches `want`, then copy
+// `words` constants from host-mapped src to device dst; a wait that
+// expires sets *timeout. This is how work enqueued ahead of its transcript
+// inputs is released (go == nullptr: no gate).
+struct HostGate {
+    const uint64_t* go;
+    uint64_t want;
+    const uint32_t* src;
+    uint32_t* dst;
+    uint32_t words;
+    uint32_t spin_limit;
+    uint32_t* timeout;
+};
+void launch_collect(const Launch& L, std::vector<CollectItem> items, uint64_t* flag, uint64_t seq,
+                    const HostGate& gate);
 void launch_copy(const Launch& L, const uint32_t* in, uint32_t* out, uint64_t words);
-// Gate for work enqueued ahead of its inputs: waits (bounded by spin_limit
-// polls) until the host-mapped *go reaches `want`, then copies `words`
-// constants from host-mapped src to device dst; sets *timeout on expiry.
-void launch_wait_host(const Launch& L, const uint64_t* go, uint64_t want, const uint32_t* src, uint32_t* dst,
-                      uint32_t words, uint32_t spin_limit, uint32_t* timeout);
+
 // out[i] = w^(brev_bits(i) << shift) for i < 2^bits, w's powers from `pt`
 // (Montgomery form): the bit-reversed point tables of a geometry.
 void launch_pow_brev(const Launch& L, PowTable pt, uint32_t* out, int bits, int shift);

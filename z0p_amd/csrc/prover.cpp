@@ -279,7 +279,7 @@ struct Buckets {
 }  // namespace
 
 void launch_chains(const Launch& L, const std::vector<const std::vector<Step>*>& chains,
-                   const std::vector<CollectItem>& collect, uint64_t* flag, uint64_t seq) {
+                   const std::vector<CollectItem>& collect, uint64_t* flag, uint64_t seq, const HostGate& gate) {
     size_t rounds = 0;
     for (auto* c : chains) rounds = std::max(rounds, c->size());
     for (size_t r = 0; r < rounds; ++r) {
@@ -288,7 +288,7 @@ void launch_chains(const Launch& L, const std::vector<const std::vector<Step>*>&
             if (r < c->size()) b.add((*c)[r]);
         b.launch(L);
     }
-    if (flag || !collect.empty()) launch_collect(L, collect, flag, seq);
+    if (flag || !collect.empty() || gate.go) launch_collect(L, collect, flag, seq, gate);
 }
 
 // ---------------------------------------------------------------------------
@@ -1061,13 +1061,19 @@ void Engine::pipeline(Proof& P) {
     const uint64_t g0 = c_.gate, g_end = g0 + phs.size(), seq0 = c_.seq;
     c_.enqueue([&](const Launch& L) {
         for (size_t k = 0; k < phs.size(); ++k) {
-            if (k > 0)
-                launch_wait_host(L, reinterpret_cast<const uint64_t*>(c_.mailbox_dev), g0 + k,
-                                 reinterpret_cast<const uint32_t*>(c_.mailbox_dev + Ctx::MAILBOX_HEAD + k * SLOT),
-                                 reinterpret_cast<uint32_t*>(P.consts + k * SLOT), uint32_t(SLOT / 4), 1u << 22,
-                                 reinterpret_cast<uint32_t*>(c_.mailbox_dev + 8));
+            // Phase k's collect also gates phase k + 1 (its constants).
+            HostGate gate{};
+            if (k + 1 < phs.size()) {
+                gate.go = reinterpret_cast<const uint64_t*>(c_.mailbox_dev);
+                gate.want = g0 + k + 1;
+                gate.src = reinterpret_cast<const uint32_t*>(c_.mailbox_dev + Ctx::MAILBOX_HEAD + (k + 1) * SLOT);
+                gate.dst = reinterpret_cast<uint32_t*>(P.consts + (k + 1) * SLOT);
+                gate.words = uint32_t(SLOT / 4);
+                gate.spin_limit = 1u << 22;
+                gate.timeout = reinterpret_cast<uint32_t*>(c_.mailbox_dev + 8);
+            }
             const std::vector<const std::vector<Step>*> one{&chains[k]};
-            launch_chains(L, one, items[k], c_.flag_dev, seq0 + 1 + k);
+            launch_chains(L, one, items[k], c_.flag_dev, seq0 + 1 + k, gate);
         }
     });
     c_.seq = seq0 + phs.size();

@@ -45,7 +45,8 @@ void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, Lea
 // type per r), then the collect items, which end by storing `seq` to
 // `flag` (when not null) once every earlier write is visible to the host.
 void launch_chains(const Launch& L, const std::vector<const std::vector<Step>*>& chains,
-                   const std::vector<CollectItem>& collect, uint64_t* flag = nullptr, uint64_t seq = 0);
+                   const std::vector<CollectItem>& collect, uint64_t* flag = nullptr, uint64_t seq = 0,
+                   const HostGate& gate = HostGate{});
 
 // air::build_trace(a0, b0, n).1[n-1] by fast doubling (air.rs:341-354).
 uint32_t fib_b_last(uint32_t a0, uint32_t b0, uint64_t n);
