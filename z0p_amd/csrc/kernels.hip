@@ -921,22 +921,18 @@ __global__ __launch_bounds__(TAIL_WG) void k_tail(TailJob inl, const TailJob* ta
 // Arity-4 fold of a bit-reversed layer (fri.rs:1273-1331). Quad i sits at
 // (x, -x, j0 x, -j0 x), x the point of position 4i;
 // x^{-1} = 31^{-4^d} * Omega_m^{-(brev(i) << 2d)}.
-__global__ __launch_bounds__(WG) void k_fold(FoldJob inl, const FoldJob* tab, uint32_t njobs) {
-    uint32_t local;
-    const FoldJob J = job_of(inl, tab, njobs, local);
-    const uint64_t quads = uint64_t(1) << (J.log_len - 2);
-    const uint64_t i = uint64_t(local) * WG + threadIdx.x;
-    if (i >= quads) return;
-    const FoldConsts& C = *J.c;
-    const uint32_t inv_x = mont_mul(gc(J.finv)[i], C.ginv_m);
+// Output i of the fold of `layer` (length 2^log_len).
+__device__ __forceinline__ E4 fold_one(const uint32_t* layer, int base_field, const uint32_t* finv,
+                                        const FoldConsts& C, uint64_t i) {
+    const uint32_t inv_x = mont_mul(gc(finv)[i], C.ginv_m);
     E4 q[4];
-    if (J.base_field) {
-        const u32x4 v = ld4(J.layer, i);
+    if (base_field) {
+        const u32x4 v = ld4(layer, i);
         q[0] = e4_base(v.x); q[1] = e4_base(v.y); q[2] = e4_base(v.z); q[3] = e4_base(v.w);
     } else {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const u32x4 v = ld4(J.layer, i * 4 + k);
+            const u32x4 v = ld4(layer, i * 4 + k);
             q[k] = E4{{v.x, v.y, v.z, v.w}};
         }
     }
@@ -947,8 +943,98 @@ __global__ __launch_bounds__(WG) void k_fold(FoldJob inl, const FoldJob* tab, ui
     E4 acc = e4_scale_m(e4_mul_mat(S, C.cm[2]), inv_x);
     acc = e4_scale_m(e4_add(acc, e4_mul_mat(Ev, C.cm[1])), inv_x);
     acc = e4_add(acc, e4_mul_mat(O, C.cm[0]));
-    const E4 out = e4_add(e4_scale_m(M, C.inv4_m), e4_scale_m(acc, inv_x));
+    return e4_add(e4_scale_m(M, C.inv4_m), e4_scale_m(acc, inv_x));
+}
+
+__global__ __launch_bounds__(WG) void k_fold(FoldJob inl, const FoldJob* tab, uint32_t njobs) {
+    uint32_t local;
+    const FoldJob J = job_of(inl, tab, njobs, local);
+    const uint64_t quads = uint64_t(1) << (J.log_len - 2);
+    const uint64_t i = uint64_t(local) * WG + threadIdx.x;
+    if (i >= quads) return;
+    const E4 out = fold_one(J.layer, J.base_field, J.finv, *J.c, i);
     st4(J.next, i, u32x4{out.v[0], out.v[1], out.v[2], out.v[3]});
+}
+
+// ---------------------------------------------------------------------------
+// A small FRI layer in one workgroup: fold layer d into layer d+1 (at most
+// SMALL_MAX_VALUES values), hash its quad leaves, and build its tree up to
+// the cap level in LDS. Thread t folds outputs 4t..4t+3, which are exactly
+// leaf t, so the leaf message never leaves registers. Levels above the
+// leaves use a quad of lanes per node. One dispatch replaces fold, leaves
+// and tree kernels whose fixed costs dominate at these sizes.
+constexpr int SMALL_WG = 1024;
+
+__global__ __launch_bounds__(SMALL_WG) void k_small_commit(SmallCommitJob inl, const SmallCommitJob* tab,
+                                                           uint32_t njobs) {
+    __shared__ uint32_t bufA[8][SMALL_MAX_VALUES / 4];
+    __shared__ uint32_t bufB[8][SMALL_MAX_VALUES / 8];
+    uint32_t local;
+    const SmallCommitJob J = job_of(inl, tab, njobs, local);
+    const FoldConsts C = *J.c;
+    const int leaves = int(J.leaves);
+    const int t = threadIdx.x;
+    if (t < leaves) {
+        uint32_t m[16];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint64_t i = uint64_t(4 * t + k);
+            const E4 out = fold_one(J.layer, J.base_field, J.finv, C, i);
+            st4(J.next, i, u32x4{out.v[0], out.v[1], out.v[2], out.v[3]});
+            m[4 * k] = out.v[0];
+            m[4 * k + 1] = out.v[1];
+            m[4 * k + 2] = out.v[2];
+            m[4 * k + 3] = out.v[3];
+        }
+        uint32_t d[8];
+        leaf64_words(m, d);
+        store_digest(J.tree + uint64_t(t) * 8, d);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) bufA[k][t] = d[k];
+    }
+    __syncthreads();
+    // Parent levels: quads of lanes, SMALL_WG / 4 nodes per round.
+    uint32_t* lvl = J.tree;
+    int count = leaves;
+    const uint32_t q = uint32_t(t & 3);
+    const int g0 = t >> 2;
+    for (int l = 1; l <= J.target; ++l) {
+        uint32_t* next_lvl = lvl + uint64_t(count) * 8;
+        const int pc = count >> 1;
+        // Even levels read bufA and write bufB, odd levels the reverse.
+        uint32_t(*cur)[SMALL_MAX_VALUES / 4] = bufA;
+        for (int g = g0; g < pc; g += SMALL_WG / 4) {  // quad-uniform
+            uint32_t Lw[8], Rw[8];
+            if (l & 1) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    Lw[k] = cur[k][2 * g];
+                    Rw[k] = cur[k][2 * g + 1];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    Lw[k] = bufB[k][2 * g];
+                    Rw[k] = bufB[k][2 * g + 1];
+                }
+            }
+            uint32_t oa, ob;
+            quad_node(Lw, Rw, q, oa, ob);
+            if (l & 1) {
+                bufB[q][g] = oa;
+                bufB[4 + q][g] = ob;
+            } else {
+                bufA[q][g] = oa;
+                bufA[4 + q][g] = ob;
+            }
+            const gptr dst = gm(next_lvl) + uint64_t(g) * 8;
+            dst[q] = oa;
+            dst[4 + q] = ob;
+        }
+        __syncthreads();
+        count = pc;
+        lvl = next_lvl;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1244,6 +1330,16 @@ void launch_tail(const Launch& L, std::vector<TailJob> jobs) {
     const uint32_t n = jobs.size() <= 1 ? 0u : uint32_t(jobs.size());
     ProfScope ps(L.prof, "tree_tail", L.stream);
     hipLaunchKernelGGL(k_tail, dim3(g), dim3(TAIL_WG), 0, L.stream, jobs[0], table, n);
+}
+
+void launch_small_commit(const Launch& L, std::vector<SmallCommitJob> jobs) {
+    unsigned g = assign_blocks(jobs, [](const SmallCommitJob&) { return uint64_t(1); });
+    if (jobs.empty()) return;
+    const SmallCommitJob* table = table_of(L, jobs);
+    if (L.dry) return;
+    const uint32_t n = jobs.size() <= 1 ? 0u : uint32_t(jobs.size());
+    ProfScope ps(L.prof, "small_commit", L.stream);
+    hipLaunchKernelGGL(k_small_commit, dim3(g), dim3(SMALL_WG), 0, L.stream, jobs[0], table, n);
 }
 
 void launch_fold(const Launch& L, std::vector<FoldJob> jobs) {

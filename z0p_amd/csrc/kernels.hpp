@@ -180,6 +180,22 @@ struct FoldJob {
     uint32_t block_begin;
 };
 
+// Fold of layer d into a small layer d+1 plus its whole Merkle commitment
+// (quad leaves and parent levels up to the cap) in one workgroup.
+constexpr uint32_t SMALL_MAX_VALUES = 1024;  // values of layer d+1 (one compute unit)
+struct SmallCommitJob {
+    const uint32_t* layer;  // layer d
+    uint32_t* next;         // layer d+1 (Ext4)
+    const uint32_t* finv;
+    const FoldConsts* c;
+    uint32_t* tree;         // layer d+1's tree, level-contiguous
+    uint32_t leaves;        // = values / 4 (<= SMALL_MAX_VALUES / 4)
+    int target;             // levels above the leaves to build
+    int base_field;
+    uint32_t block_begin;
+};
+void launch_small_commit(const Launch& L, std::vector<SmallCommitJob> jobs);
+
 struct GrindJob {
     const uint32_t* hw;  // device memory: 47-byte head as 16 LE words, zero-padded
     uint32_t bits;

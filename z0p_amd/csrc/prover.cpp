@@ -243,6 +243,7 @@ struct Buckets {
     std::vector<LevelsJob> levels;
     std::vector<TailJob> tail;
     std::vector<FoldJob> fold;
+    std::vector<SmallCommitJob> small;
     std::vector<GrindJob> grind;
     std::vector<OpenJob> open;
 
@@ -255,6 +256,7 @@ struct Buckets {
     std::vector<LevelsJob>& of(const LevelsJob&) { return levels; }
     std::vector<TailJob>& of(const TailJob&) { return tail; }
     std::vector<FoldJob>& of(const FoldJob&) { return fold; }
+    std::vector<SmallCommitJob>& of(const SmallCommitJob&) { return small; }
     std::vector<GrindJob>& of(const GrindJob&) { return grind; }
     std::vector<OpenJob>& of(const OpenJob&) { return open; }
 
@@ -271,6 +273,7 @@ struct Buckets {
         if (!levels.empty()) launch_levels(L, levels);
         if (!tail.empty()) launch_tail(L, tail);
         if (!fold.empty()) launch_fold(L, fold);
+        if (!small.empty()) launch_small_commit(L, small);
         if (!grind.empty()) launch_grind(L, grind);
         if (!open.empty()) launch_open(L, open);
     }
@@ -628,11 +631,27 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
         }
         case PH_FOLD: {
             const size_t d = P.depth;
+            const FoldConsts* fc = static_cast<const FoldConsts*>(cst ? cst : c.ring.push(&P.fold, sizeof(P.fold)));
+            if (d + 1 < P.folds && P.lens[d + 1] <= SMALL_MAX_VALUES) {
+                // Small next layer: fold, leaves and tree in one workgroup.
+                SmallCommitJob sj{};
+                sj.layer = P.vals[d];
+                sj.next = P.vals[d + 1];
+                sj.finv = gt.finv;
+                sj.c = fc;
+                sj.tree = P.trees[d + 1];
+                sj.leaves = uint32_t(P.geoms[d + 1].leaves);
+                sj.target = P.geoms[d + 1].target;
+                sj.base_field = d == 0 && !P.air ? 1 : 0;
+                ch.push_back(sj);
+                collect_cap(c, P, P.trees[d + 1], P.geoms[d + 1], items, used);
+                break;
+            }
             FoldJob fj{};
             fj.layer = P.vals[d];
             fj.next = P.vals[d + 1];
             fj.finv = gt.finv;
-            fj.c = static_cast<const FoldConsts*>(cst ? cst : c.ring.push(&P.fold, sizeof(P.fold)));
+            fj.c = fc;
             fj.log_len = int(ilog2(P.lens[d]));
             fj.base_field = d == 0 && !P.air ? 1 : 0;
             ch.push_back(fj);

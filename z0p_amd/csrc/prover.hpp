@@ -36,7 +36,7 @@ TreeGeom tree_geom(uint64_t leaves, uint64_t cap_height);
 // One kernel step of one proof; a proof's steps of one iteration form a
 // chain that must run in order.
 using Step = std::variant<FibJob, BitrevJob, NttJob, LinkedJob, ComposeJob, LeavesJob, LevelsJob, TailJob, FoldJob,
-                          GrindJob, OpenJob>;
+                          SmallCommitJob, GrindJob, OpenJob>;
 
 // Leaves, then wide levels, then LDS tail launches up to the cap level.
 void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves);
