@@ -128,6 +128,36 @@ int z0p_air_verify(const z0p_air_params* params, z0p_transcript* transcript, con
 int z0p_fri_commit_open(z0p_ctx* ctx, const uint32_t* coeffs, size_t n, const z0p_fri_params* params,
                         z0p_transcript* transcript, uint8_t** commitment, size_t* commitment_len, uint8_t** proof,
                         size_t* proof_len);
+/* air::prove_with_trace (air.rs:385-406): explicit columns of len_a = len_b =
+ * n_steps canonical values (else Z0P_ERR_MALFORMED_INPUT), both interpolated
+ * and extended on the device; then Proof::to_bytes. */
+int z0p_air_prove_with_trace(z0p_ctx* ctx, const z0p_air_params* params, const uint32_t* col_a, size_t len_a,
+                             const uint32_t* col_b, size_t len_b, z0p_transcript* transcript, uint8_t** proof,
+                             size_t* proof_len);
+
+/* ---- hiding: replaces rng::ChaCha20 (src/rng.rs) and air::prove_zk ---- */
+/* Opaque ChaCha20 stream (rng.rs:82-88). Not clonable, as in the reference. */
+typedef struct z0p_chacha20 z0p_chacha20;
+/* ChaCha20::new(key, nonce) (rng.rs:107-116); NULL on a null argument. */
+z0p_chacha20* z0p_chacha20_new(const uint8_t key[32], const uint8_t nonce[12]);
+/* rng::fresh_stream (rng.rs:185-209): key from /dev/urandom, zero nonce;
+ * Z0P_ERR_RNG_UNAVAILABLE when it cannot be read. */
+int z0p_chacha20_from_os(z0p_chacha20** out);
+void z0p_chacha20_free(z0p_chacha20* rng);
+/* ChaCha20::fill_bytes / next_u32 / next_u64 / next_field (rng.rs:118-168). */
+void z0p_chacha20_fill_bytes(z0p_chacha20* rng, uint8_t* out, size_t len);
+uint32_t z0p_chacha20_next_u32(z0p_chacha20* rng);
+uint64_t z0p_chacha20_next_u64(z0p_chacha20* rng);
+uint32_t z0p_chacha20_next_field(z0p_chacha20* rng);
+/* air::prove_zk (air.rs:439-464): both columns masked by
+ * blind::mask_coeffs (blind.rs:38-79) with n_rand >= 16 * num_queries
+ * maskers each, drawn from rng (advanced as the reference advances it);
+ * the masked LDE has next_pow2(n_steps + n_rand) * blowup points, so
+ * final_size must leave a power-of-four fold ratio over it. Verified by the
+ * same z0p_air_verify. */
+int z0p_air_prove_zk(z0p_ctx* ctx, const z0p_air_params* params, uint64_t n_rand, z0p_chacha20* rng,
+                     z0p_transcript* transcript, uint8_t** proof, size_t* proof_len);
+
 /* Commitment/Proof::from_bytes + fri::verify (fri.rs:686-859). Host only. */
 int z0p_fri_verify(const uint8_t* commitment, size_t commitment_len, const uint8_t* proof, size_t proof_len,
                    const z0p_fri_params* params, z0p_transcript* transcript);

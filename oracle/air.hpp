@@ -1,9 +1,10 @@
 // ORACLE — test infrastructure only (see bb.hpp).
 //
-// Fibonacci AIR prover/verifier (non-ZK), restated from src/air.rs.
+// Fibonacci AIR prover/verifier (plain and zero-knowledge), restated from src/air.rs.
 #pragma once
 #include <vector>
 
+#include "blind.hpp"
 #include "fri.hpp"
 
 namespace oracle {
@@ -387,6 +388,20 @@ inline Proof prove_with_trace(const Params& params, const std::vector<Field>& co
     if (col_a.size() != n || col_b.size() != n) throw Z0pError(ERR_MALFORMED_INPUT, "trace columns must match n_steps");
     std::vector<Field> ca = interpolate(col_a), cb = interpolate(col_b);
     return prove_inner(params, ca, &cb, n - 1, nullptr, transcript);
+}
+
+// air.rs:439-464 — both columns masked with n_rand maskers each (A first).
+inline Proof prove_zk(const Params& params, size_t n_rand, ChaCha20& rng, Transcript& transcript) {
+    params.validate();
+    if (params.num_queries > SIZE_MAX / 16) throw Z0pError(ERR_MALFORMED_INPUT, "query budget overflows the hiding requirement");
+    if (n_rand < params.num_queries * 16)
+        throw Z0pError(ERR_MALFORMED_INPUT, "n_rand must cover every revealed leaf row for hiding");
+    size_t n = params.n_steps;
+    auto cols = build_trace(params.a0, params.b0, n);
+    std::vector<Field> ca = interpolate(cols.first), cb = interpolate(cols.second);
+    std::vector<Field> ma = mask_coeffs(ca, n, n_rand, rng);
+    std::vector<Field> mb = mask_coeffs(cb, n, n_rand, rng);
+    return prove_inner(params, ma, &mb, n + n_rand - 1, nullptr, transcript);
 }
 
 // air.rs:607-752

@@ -333,6 +333,34 @@ int oracle_air_prove_with_trace(const oracle_air_params* p, const uint32_t* col_
         *out = dup_bytes(pr.to_bytes(), len);
     })
 }
+int oracle_air_prove_zk(const oracle_air_params* p, size_t n_rand, void* rng, void* t, uint8_t** out, size_t* len) {
+    GUARD({
+        air::Proof pr = air::prove_zk(to_air(p), n_rand, *static_cast<ChaCha20*>(rng), *static_cast<Transcript*>(t));
+        *out = dup_bytes(pr.to_bytes(), len);
+    })
+}
+
+// rng.rs — stream handle for the tests.
+void* oracle_chacha20_new(const uint8_t* key, const uint8_t* nonce) { return new ChaCha20(key, nonce); }
+void oracle_chacha20_free(void* r) { delete static_cast<ChaCha20*>(r); }
+void oracle_chacha20_fill(void* r, uint8_t* out, size_t n) { static_cast<ChaCha20*>(r)->fill_bytes(out, n); }
+uint32_t oracle_chacha20_next_u32(void* r) { return static_cast<ChaCha20*>(r)->next_u32(); }
+uint64_t oracle_chacha20_next_u64(void* r) { return static_cast<ChaCha20*>(r)->next_u64(); }
+uint32_t oracle_chacha20_next_field(void* r) { return static_cast<ChaCha20*>(r)->next_field().v; }
+void oracle_chacha20_block(const uint8_t* key, uint32_t counter, const uint8_t* nonce, uint8_t* out) {
+    chacha_block(key, counter, nonce, out);
+}
+void oracle_chacha20_quarter(uint32_t* state16, int a, int b, int c, int d) { chacha_quarter(state16, a, b, c, d); }
+// blind.rs:38-79; out holds the padded length (returned in *out_len).
+int oracle_mask_coeffs(const uint32_t* coeffs, size_t len, size_t n, size_t n_rand, void* rng, uint32_t* out,
+                       size_t cap, size_t* out_len) {
+    GUARD({
+        std::vector<Field> m = mask_coeffs(to_fields(coeffs, len), n, n_rand, *static_cast<ChaCha20*>(rng));
+        *out_len = m.size();
+        if (m.size() > cap) throw Z0pError(ERR_INDEX_OUT_OF_BOUNDS);
+        for (size_t i = 0; i < m.size(); ++i) out[i] = m[i].v;
+    })
+}
 int oracle_air_verify(const oracle_air_params* p, void* t, const uint8_t* proof, size_t len) {
     GUARD({
         air::Proof pr = air::Proof::from_bytes(proof, len);

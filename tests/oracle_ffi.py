@@ -114,6 +114,17 @@ def lib():
         "oracle_air_prove": (ctypes.c_int, [ctypes.POINTER(AirParams), vp, ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
         "oracle_air_prove_with_trace": (ctypes.c_int, [ctypes.POINTER(AirParams), u32p, u32p, vp,
                                                        ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+        "oracle_air_prove_zk": (ctypes.c_int, [ctypes.POINTER(AirParams), sz, vp, vp, ctypes.POINTER(u8p),
+                                               ctypes.POINTER(sz)]),
+        "oracle_chacha20_new": (vp, [u8p, u8p]),
+        "oracle_chacha20_free": (None, [vp]),
+        "oracle_chacha20_fill": (None, [vp, u8p, sz]),
+        "oracle_chacha20_next_u32": (ctypes.c_uint32, [vp]),
+        "oracle_chacha20_next_u64": (ctypes.c_uint64, [vp]),
+        "oracle_chacha20_next_field": (ctypes.c_uint32, [vp]),
+        "oracle_chacha20_block": (None, [u8p, ctypes.c_uint32, u8p, u8p]),
+        "oracle_chacha20_quarter": (None, [u32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "oracle_mask_coeffs": (ctypes.c_int, [u32p, sz, sz, sz, vp, u32p, sz, ctypes.POINTER(sz)]),
         "oracle_air_verify": (ctypes.c_int, [ctypes.POINTER(AirParams), vp, u8p, sz]),
         "oracle_fri_commit_open": (ctypes.c_int, [u32p, sz, ctypes.POINTER(FriParams), vp, ctypes.POINTER(u8p),
                                                   ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
@@ -270,6 +281,73 @@ def air_prove_with_trace(params: AirParams, col_a, col_b, transcript: Transcript
                                            ctypes.byref(out), ctypes.byref(n))
     if rc:
         raise ValueError("oracle prove error %d" % rc)
+    return _take(out, n.value)
+
+
+class OracleError(ValueError):
+    """An oracle entry point returned a nonzero status (src/error.rs order)."""
+
+    def __init__(self, code):
+        super().__init__("oracle error %d" % code)
+        self.code = code
+
+
+class ChaCha20:
+    """Oracle ChaCha20 stream (rng.rs:81-177)."""
+
+    def __init__(self, key: bytes, nonce: bytes):
+        assert len(key) == 32 and len(nonce) == 12
+        self.h = lib().oracle_chacha20_new(_buf(key), _buf(nonce))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_chacha20_free(self.h)
+            self.h = None
+
+    def fill_bytes(self, n: int) -> bytes:
+        out = (ctypes.c_uint8 * max(n, 1))()
+        lib().oracle_chacha20_fill(self.h, out, n)
+        return bytes(out)[:n]
+
+    def next_u32(self) -> int:
+        return lib().oracle_chacha20_next_u32(self.h)
+
+    def next_u64(self) -> int:
+        return lib().oracle_chacha20_next_u64(self.h)
+
+    def next_field(self) -> int:
+        return lib().oracle_chacha20_next_field(self.h)
+
+
+def chacha20_block(key: bytes, counter: int, nonce: bytes) -> bytes:
+    out = (ctypes.c_uint8 * 64)()
+    lib().oracle_chacha20_block(_buf(key), counter, _buf(nonce), out)
+    return bytes(out)
+
+
+def chacha20_quarter(state, a, b, c, d):
+    arr = u32_array(state)
+    lib().oracle_chacha20_quarter(arr, a, b, c, d)
+    return list(arr)
+
+
+def mask_coeffs(coeffs, n, n_rand, rng: ChaCha20):
+    cap = max(1, 2 * (n + n_rand))
+    out = (ctypes.c_uint32 * cap)()
+    used = ctypes.c_size_t()
+    rc = lib().oracle_mask_coeffs(u32_array(coeffs), len(coeffs), n, n_rand, rng.h, out, cap, ctypes.byref(used))
+    if rc:
+        raise OracleError(rc)
+    return list(out)[:used.value]
+
+
+def air_prove_zk(params: AirParams, n_rand: int, rng: ChaCha20, transcript: Transcript) -> bytes:
+    out = u8p()
+    n = ctypes.c_size_t()
+    rc = lib().oracle_air_prove_zk(ctypes.byref(params), n_rand, rng.h, transcript.h, ctypes.byref(out),
+                                   ctypes.byref(n))
+    if rc:
+        raise OracleError(rc)
     return _take(out, n.value)
 
 

@@ -165,3 +165,34 @@ def test_no_gpu_context_fails_loudly():
     with pytest.raises(z.Z0pError) as e:
         z.Context(0)
     assert e.value.name == "NoDevice"
+
+
+def test_chacha20_stream_matches_oracle():
+    # The product's host ChaCha20 (rng.rs:82-168) against the pinned oracle:
+    # RFC 8439 section 2.3.2 key/nonce, unaligned reads across blocks, and
+    # the field sampler's rejection over a long run of draws.
+    key, nonce = bytes(range(32)), bytes([0, 0, 0, 9, 0, 0, 0, 0x4A, 0, 0, 0, 0])
+    a, b = z.ChaCha20(key, nonce), o.ChaCha20(key, nonce)
+    assert a.fill_bytes(64) == o.chacha20_block(key, 0, nonce)
+    assert a.fill_bytes(3) == b.fill_bytes(67)[64:]
+    assert a.next_u32() == b.next_u32() and a.next_u64() == b.next_u64()
+    assert [a.next_field() for _ in range(20000)] == [b.next_field() for _ in range(20000)]
+    assert a.fill_bytes(200) == b.fill_bytes(200)
+    with pytest.raises(ValueError):
+        z.ChaCha20(bytes(31), bytes(12))
+    r = z.ChaCha20.from_os()
+    assert 0 <= r.next_field() < P
+
+
+def test_verifier_accepts_oracle_zk_proofs():
+    # air.rs:607-632: masked proofs carry a larger LDE in the commitment.
+    n = 256
+    bl = o.fib_b_last(3, 4, n)
+    for n_rand, final in [(128, 16), (640, 32)]:
+        op = o.air_params_new(n, 8, 8, final, 3, 4, bl)
+        pr = o.air_prove_zk(op, n_rand, o.ChaCha20(bytes([1] * 32), bytes([2] * 12)), o.Transcript(b"zk"))
+        zp = z.air.params_new(n, 8, 8, final, 3, 4, bl)
+        z.air.verify(zp, pr, z.Transcript(b"zk"))
+        bad = bytearray(pr)
+        bad[len(bad) // 2] ^= 4
+        _expect_reject(zp, bytes(bad), b"zk", op)

@@ -11,7 +11,7 @@ import ctypes
 
 from ._ffi import Z0pError, buf, check, lib, take
 
-__all__ = ["Z0pError", "Transcript", "Context", "Server", "air", "fri", "default_context"]
+__all__ = ["Z0pError", "Transcript", "ChaCha20", "Context", "Server", "air", "fri", "default_context"]
 
 
 class Transcript:
@@ -43,6 +43,50 @@ class Transcript:
         out = (ctypes.c_uint8 * 32)()
         lib().z0p_transcript_challenge_bytes(self._h, label, out)
         return bytes(out)
+
+
+class ChaCha20:
+    """ChaCha20 blinding stream (src/rng.rs:82-168): ``ChaCha20(key, nonce)``
+    or ``ChaCha20.from_os()`` (rng::fresh_stream)."""
+
+    def __init__(self, key: bytes = None, nonce: bytes = None, _handle=None):
+        if _handle is None:
+            if key is None or nonce is None or len(key) != 32 or len(nonce) != 12:
+                raise ValueError("ChaCha20 needs a 32-byte key and a 12-byte nonce")
+            _handle = lib().z0p_chacha20_new(buf(key), buf(nonce))
+            if not _handle:
+                raise MemoryError("rng allocation failed")
+        self._h = _handle
+
+    @classmethod
+    def from_os(cls):
+        h = ctypes.c_void_p()
+        check(lib().z0p_chacha20_from_os(ctypes.byref(h)))
+        return cls(_handle=h.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().z0p_chacha20_free(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def fill_bytes(self, n: int) -> bytes:
+        out = (ctypes.c_uint8 * max(n, 1))()
+        lib().z0p_chacha20_fill_bytes(self._h, out, n)
+        return bytes(out)[:n]
+
+    def next_u32(self) -> int:
+        return lib().z0p_chacha20_next_u32(self._h)
+
+    def next_u64(self) -> int:
+        return lib().z0p_chacha20_next_u64(self._h)
+
+    def next_field(self) -> int:
+        return lib().z0p_chacha20_next_field(self._h)
 
 
 class Context:

@@ -79,6 +79,17 @@ SIGNATURES = {
     "z0p_air_b_last": (u32, [u32, u32, u64]),
     "z0p_air_prove": (ctypes.c_int, [vp, ctypes.POINTER(AirParamsC), vp, ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
     "z0p_air_verify": (ctypes.c_int, [ctypes.POINTER(AirParamsC), vp, u8p, sz]),
+    "z0p_air_prove_with_trace": (ctypes.c_int, [vp, ctypes.POINTER(AirParamsC), u32p, sz, u32p, sz, vp,
+                                                ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+    "z0p_chacha20_new": (vp, [u8p, u8p]),
+    "z0p_chacha20_from_os": (ctypes.c_int, [ctypes.POINTER(vp)]),
+    "z0p_chacha20_free": (None, [vp]),
+    "z0p_chacha20_fill_bytes": (None, [vp, u8p, sz]),
+    "z0p_chacha20_next_u32": (u32, [vp]),
+    "z0p_chacha20_next_u64": (u64, [vp]),
+    "z0p_chacha20_next_field": (u32, [vp]),
+    "z0p_air_prove_zk": (ctypes.c_int, [vp, ctypes.POINTER(AirParamsC), u64, vp, vp, ctypes.POINTER(u8p),
+                                        ctypes.POINTER(sz)]),
     "z0p_fri_commit_open": (ctypes.c_int, [vp, u32p, sz, ctypes.POINTER(FriParamsC), vp, ctypes.POINTER(u8p),
                                            ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
     "z0p_fri_verify": (ctypes.c_int, [u8p, sz, u8p, sz, ctypes.POINTER(FriParamsC), vp]),

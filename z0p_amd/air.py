@@ -54,6 +54,38 @@ def prove(params: Params, transcript, ctx=None) -> bytes:
     return take(out, n.value)
 
 
+def _u32_column(values):
+    import numpy as np
+    arr = np.ascontiguousarray(np.asarray(values, dtype=np.uint32))
+    return arr, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+def prove_with_trace(params: Params, col_a, col_b, transcript, ctx=None) -> bytes:
+    """air::prove_with_trace (air.rs:385-406) on the GPU: explicit columns of
+    n_steps canonical field values each, both interpolated and extended."""
+    from . import default_context
+    ctx = ctx or default_context()
+    a, pa = _u32_column(col_a)
+    b, pb = _u32_column(col_b)
+    out = u8p()
+    n = ctypes.c_size_t()
+    check(lib().z0p_air_prove_with_trace(ctx.handle, ctypes.byref(params), pa, a.size, pb, b.size,
+                                         transcript.handle, ctypes.byref(out), ctypes.byref(n)))
+    return take(out, n.value)
+
+
+def prove_zk(params: Params, n_rand: int, rng, transcript, ctx=None) -> bytes:
+    """air::prove_zk (air.rs:439-464) on the GPU: both columns masked with
+    ``n_rand`` maskers drawn from ``rng`` (a ``z0p_amd.ChaCha20``, advanced)."""
+    from . import default_context
+    ctx = ctx or default_context()
+    out = u8p()
+    n = ctypes.c_size_t()
+    check(lib().z0p_air_prove_zk(ctx.handle, ctypes.byref(params), n_rand, rng.handle, transcript.handle,
+                                 ctypes.byref(out), ctypes.byref(n)))
+    return take(out, n.value)
+
+
 def prove_batch(params_list, transcripts, ctx=None):
     """``air.prove`` for every (params, transcript) pair in one batch: the
     instances share every kernel launch and are admitted as device memory
@@ -82,4 +114,5 @@ def verify(params: Params, proof: bytes, transcript) -> None:
     check(lib().z0p_air_verify(ctypes.byref(params), transcript.handle, buf(proof), len(proof)))
 
 
-__all__ = ["Params", "params_new", "for_security", "b_last", "prove", "prove_batch", "verify", "Z0pError"]
+__all__ = ["Params", "params_new", "for_security", "b_last", "prove", "prove_with_trace", "prove_zk", "prove_batch",
+           "verify", "Z0pError"]

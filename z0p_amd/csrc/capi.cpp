@@ -1,6 +1,7 @@
 // extern "C" boundary (include/z0p.h): translate statuses, own outputs.
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 
@@ -207,6 +208,66 @@ int z0p_air_prove(z0p_ctx* ctx, const z0p_air_params* params, z0p_transcript* tr
     });
     return g != Z0P_OK ? g : rc;
 }
+
+int z0p_air_prove_with_trace(z0p_ctx* ctx, const z0p_air_params* params, const uint32_t* col_a, size_t len_a,
+                             const uint32_t* col_b, size_t len_b, z0p_transcript* transcript, uint8_t** proof,
+                             size_t* proof_len) {
+    if (!ctx || !params || !transcript || !proof || !proof_len || (!col_a && len_a) || (!col_b && len_b))
+        return Z0P_ERR_INVALID_ARGUMENT;
+    int rc = Z0P_OK;
+    int g = guarded([&] {
+        std::vector<uint8_t> bytes =
+            z0p::air_prove_with_trace(*ctx->c, *params, col_a, len_a, col_b, len_b, transcript->t);
+        rc = emit(bytes, proof, proof_len);
+    });
+    return g != Z0P_OK ? g : rc;
+}
+
+int z0p_air_prove_zk(z0p_ctx* ctx, const z0p_air_params* params, uint64_t n_rand, z0p_chacha20* rng,
+                     z0p_transcript* transcript, uint8_t** proof, size_t* proof_len) {
+    if (!ctx || !params || !rng || !transcript || !proof || !proof_len) return Z0P_ERR_INVALID_ARGUMENT;
+    int rc = Z0P_OK;
+    int g = guarded([&] {
+        std::vector<uint8_t> bytes = z0p::air_prove_zk(*ctx->c, *params, n_rand, *rng->r, transcript->t);
+        rc = emit(bytes, proof, proof_len);
+    });
+    return g != Z0P_OK ? g : rc;
+}
+
+z0p_chacha20* z0p_chacha20_new(const uint8_t key[32], const uint8_t nonce[12]) {
+    if (!key || !nonce) return nullptr;
+    auto* h = new (std::nothrow) z0p_chacha20;
+    if (!h) return nullptr;
+    h->r = new (std::nothrow) z0p::ChaCha20(key, nonce);
+    if (!h->r) {
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+int z0p_chacha20_from_os(z0p_chacha20** out) {
+    if (!out) return Z0P_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    return guarded([&] {
+        auto h = std::make_unique<z0p_chacha20>();
+        h->r = z0p::ChaCha20::from_os();
+        *out = h.release();
+    });
+}
+
+void z0p_chacha20_free(z0p_chacha20* rng) {
+    if (!rng) return;
+    delete rng->r;
+    delete rng;
+}
+
+void z0p_chacha20_fill_bytes(z0p_chacha20* rng, uint8_t* out, size_t len) {
+    if (rng && (out || !len)) rng->r->fill(out, len);
+}
+uint32_t z0p_chacha20_next_u32(z0p_chacha20* rng) { return rng ? rng->r->next_u32() : 0; }
+uint64_t z0p_chacha20_next_u64(z0p_chacha20* rng) { return rng ? rng->r->next_u64() : 0; }
+uint32_t z0p_chacha20_next_field(z0p_chacha20* rng) { return rng ? rng->r->next_field() : 0; }
 
 int z0p_air_verify(const z0p_air_params* params, z0p_transcript* transcript, const uint8_t* proof,
                    size_t proof_len) {

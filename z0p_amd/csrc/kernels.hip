@@ -293,6 +293,22 @@ __device__ void ntt_intt_small(const NttJob& J, uint32_t* lds) {
     for (int q = threadIdx.x; q < n; q += blockDim.x) J.out[q] = mont_mul(s[q], J.ninv_m);
 }
 
+// Coefficient k of the masked polynomial on coset j: the mask term
+// (x^n - 1) R(x) is (z_j - 1) sum_b z_j^b R_b(x) there, R_b being the b-th
+// n-coefficient block of R.
+template <bool MASK>
+__device__ __forceinline__ uint32_t lde_coeff(const NttJob& J, uint32_t c, uint32_t k, uint32_t j) {
+    if (!MASK || k >= J.mask_len) return c;
+    uint32_t f = gc(J.zvan)[j];
+    const uint32_t z = add(f, R1);
+    for (uint32_t i = k; i < J.mask_len; i += 1u << J.logn) {
+        c = add(c, mont_mul(gc(J.mask)[i], f));
+        f = mont_mul(f, z);
+    }
+    return c;
+}
+
+template <bool MASK>
 __device__ void ntt_lde_small(const NttJob& J, uint32_t* lds, uint32_t j) {
     const int n = 1 << J.logn;
     if (J.logn == 12) {
@@ -304,7 +320,7 @@ __device__ void ntt_lde_small(const NttJob& J, uint32_t* lds, uint32_t j) {
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
             const uint32_t k = q + 256u * t;
-            x[t] = mont_mul(in[brev_bits(k, 12)], sc[k]);
+            x[t] = mont_mul(lde_coeff<MASK>(J, in[brev_bits(k, 12)], k, j), sc[k]);
         }
         dif4096(x, lds, gc(J.tw4096));
         __syncthreads();
@@ -322,7 +338,7 @@ __device__ void ntt_lde_small(const NttJob& J, uint32_t* lds, uint32_t j) {
     const uint32_t* sc = J.tab_a + size_t(j) * n;
     for (int P = threadIdx.x; P < n; P += blockDim.x) {
         uint32_t k = brev_bits(P, J.logn);
-        s[k] = mont_mul(J.in[P], sc[k]);
+        s[k] = mont_mul(lde_coeff<MASK>(J, J.in[P], k, j), sc[k]);
     }
     __syncthreads();
     lds_dif(s, J.logn, 1, n, tw);
@@ -403,6 +419,7 @@ __device__ void ntt_intt_rows(const NttJob& J, uint32_t* lds, uint32_t u) {
 
 // The coefficient row is already bit-reversed, so a DIT transform consumes
 // it in place; the row table is stored bit-reversed to match.
+template <bool MASK>
 __device__ void ntt_lde_rows(const NttJob& J, uint32_t* lds, uint32_t R, uint32_t j) {
     const int n1 = 1 << J.log1, n2 = 1 << J.log2;
     const int logn = J.log1 + J.log2;
@@ -417,7 +434,7 @@ __device__ void ntt_lde_rows(const NttJob& J, uint32_t* lds, uint32_t R, uint32_
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
             const int C = int(q) + 256 * t;
-            x[t] = mont_mul(in[C], rt[C]);
+            x[t] = mont_mul(lde_coeff<MASK>(J, in[C], brev_bits(R * uint32_t(n2) + uint32_t(C), logn), j), rt[C]);
         }
 #pragma unroll
         for (int t = 0; t < 16; ++t) lds[pad16(int(q) + 256 * t)] = x[t];
@@ -442,7 +459,8 @@ __device__ void ntt_lde_rows(const NttJob& J, uint32_t* lds, uint32_t R, uint32_
     const uint32_t c = brev_bits(R, J.log1);
     const uint32_t* rt = J.tab_a + size_t(j) * n2;
     const uint32_t* in = J.in + size_t(R) * n2;
-    for (int Cc = threadIdx.x; Cc < n2; Cc += blockDim.x) s[Cc] = mont_mul(in[Cc], rt[Cc]);
+    for (int Cc = threadIdx.x; Cc < n2; Cc += blockDim.x)
+        s[Cc] = mont_mul(lde_coeff<MASK>(J, in[Cc], brev_bits(R * uint32_t(n2) + uint32_t(Cc), logn), j), rt[Cc]);
     __syncthreads();
     lds_dit(s, J.log2, 1, n2, tw);
     const uint32_t colscale = J.tab_b[size_t(j) * n1 + c];
@@ -517,11 +535,17 @@ __global__ __launch_bounds__(WG) void k_ntt(NttJob inl, const NttJob* tab, uint3
             ntt_intt_rows(J, lds, local);
             break;
         case NTT_LDE_SMALL:
-            ntt_lde_small(J, lds, local);
+            if (J.mask)
+                ntt_lde_small<true>(J, lds, local);
+            else
+                ntt_lde_small<false>(J, lds, local);
             break;
         case NTT_LDE_ROWS: {
             const uint32_t n1 = 1u << J.log1;
-            ntt_lde_rows(J, lds, local % n1, local / n1);
+            if (J.mask)
+                ntt_lde_rows<true>(J, lds, local % n1, local / n1);
+            else
+                ntt_lde_rows<false>(J, lds, local % n1, local / n1);
             break;
         }
         case NTT_LDE_COLS: {

@@ -81,6 +81,12 @@ struct NttJob {
     const uint32_t* tab_b;   // LDE_ROWS: coltab
     uint32_t ninv_m;
     int kind, logn, log1, log2, logb, cols;
+    // LDE kinds, hiding mask (blind.rs:38-79): coefficient k of coset j
+    // gains sum_b (z_j - 1) z_j^b mask[b n + k] for b n + k < mask_len, with
+    // z_j = s_j^n and zvan[j] = z_j - 1 (Montgomery). mask is canonical.
+    const uint32_t* mask;
+    const uint32_t* zvan;
+    uint32_t mask_len;
     uint32_t block_begin;
 };
 

@@ -386,6 +386,12 @@ struct Proof {
     z0p_fri_params fp{};
     Transcript* tr = nullptr;
     const uint32_t* coeff_src = nullptr;  // FRI: host coefficients
+    // AIR with both columns interpolated and extended (air.rs:385-406,
+    // 439-464) instead of the linked second column.
+    bool two_col = false;
+    const uint32_t *host_a = nullptr, *host_b = nullptr;  // explicit trace columns
+    std::vector<uint32_t> mask;                           // hiding: A's n_rand maskers, then B's
+    uint64_t n_rand = 0;
     uint64_t tag = 0;
     // geometry
     uint64_t n = 0, B = 0, m = 0, poly_len = 0, degree_bound = 0;
@@ -400,6 +406,7 @@ struct Proof {
     size_t mem_bytes = 0;
     size_t stage_need = 0;  // words of staging any one iteration may use
     uint32_t *col = nullptr, *coeff = nullptr, *scratch = nullptr, *A = nullptr, *Bc = nullptr, *trace_tree = nullptr;
+    uint32_t *colB = nullptr, *coeffB = nullptr, *mask_dev = nullptr;
     std::vector<uint32_t*> vals, trees;
     unsigned long long* grind_res = nullptr;
     uint8_t* consts = nullptr;  // per-phase transcript constants (pre-enqueued phases)
@@ -437,6 +444,11 @@ size_t layout(Proof& P, uint8_t* base) {
     };
     P.col = take(P.n);
     P.coeff = take(P.n);
+    if (P.two_col) {
+        P.colB = take(P.n);
+        P.coeffB = take(P.n);
+        if (P.n_rand) P.mask_dev = take(2 * P.n_rand);
+    }
     P.scratch = take(P.m);
     if (P.air) {
         P.A = take(P.m);
@@ -472,20 +484,21 @@ void init_fri_shape(Proof& P, const z0p_fri_params& fp) {
     P.stage_need = size_t(fp.num_queries) * P.wpq + 256 * 8 + P.lens[P.folds] * 4 + 256;
 }
 
-void init_air(Ctx& c, Proof& P, const z0p_air_params& p, Transcript* tr) {
-    validate_air(p);
+// AIR geometry over an m-point LDE: m / n cosets of n points (m = n * blowup,
+// or larger for a masked trace), composition degree below degree_bound + 1.
+void init_air_shape(Ctx& c, Proof& P, const z0p_air_params& p, Transcript* tr, uint64_t m, uint64_t degree_bound) {
     const z0p_fri_params fp = air_fri_params(p);
     validate_fri(fp);
     P.air = true;
     P.ap = p;
     P.tr = tr;
     P.n = p.n_steps;
-    P.B = p.blowup;
-    P.m = P.n * P.B;
+    P.m = m;
+    P.B = m / P.n;
     P.logn = int(ilog2(P.n));
     P.logb = int(ilog2(P.B));
     P.poly_len = P.m;
-    P.degree_bound = P.n - 1;
+    P.degree_bound = degree_bound;
     P.tg = tree_geom(P.m / 8, p.cap_height);
     init_fri_shape(P, fp);
     P.a0 = p.a0 % P_MOD;
@@ -497,6 +510,44 @@ void init_air(Ctx& c, Proof& P, const z0p_air_params& p, Transcript* tr) {
     P.coeff_c = mul(mul(sub(actual_b_last, P.a0), P.w_last), inv_c(uint32_t(P.n % P_MOD)));
     P.gt = &c.geometry(P.logn, P.logb);
     P.mem_bytes = layout(P, nullptr);
+}
+
+// air::prove (air.rs:365-372): the linked second column.
+void init_air(Ctx& c, Proof& P, const z0p_air_params& p, Transcript* tr) {
+    validate_air(p);
+    init_air_shape(c, P, p, tr, p.n_steps * p.blowup, p.n_steps - 1);
+}
+
+// air::prove_with_trace (air.rs:385-406): caller columns of n canonical values.
+void init_air_trace(Ctx& c, Proof& P, const z0p_air_params& p, const uint32_t* col_a, uint64_t len_a,
+                    const uint32_t* col_b, uint64_t len_b, Transcript* tr) {
+    validate_air(p);
+    require(len_a == p.n_steps && len_b == p.n_steps, Z0P_ERR_MALFORMED_INPUT);
+    for (uint64_t i = 0; i < len_a; ++i) require(col_a[i] < P_MOD && col_b[i] < P_MOD, Z0P_ERR_MALFORMED_INPUT);
+    P.two_col = true;
+    P.host_a = col_a;
+    P.host_b = col_b;
+    init_air_shape(c, P, p, tr, p.n_steps * p.blowup, p.n_steps - 1);
+}
+
+// air::prove_zk (air.rs:439-464) with blind::mask_coeffs (blind.rs:38-79):
+// each column becomes T + (x^n - 1) R over next_pow2(n + n_rand)
+// coefficients. The checks and the rng draws come in the reference's order.
+void init_air_zk(Ctx& c, Proof& P, const z0p_air_params& p, uint64_t n_rand, ChaCha20& rng, Transcript* tr) {
+    validate_air(p);
+    require(p.num_queries <= UINT64_MAX / 16, Z0P_ERR_MALFORMED_INPUT);
+    require(n_rand >= 16 * p.num_queries, Z0P_ERR_MALFORMED_INPUT);
+    const uint64_t n = p.n_steps;
+    require(n_rand <= MAX_DOMAIN, Z0P_ERR_INVALID_DOMAIN_SIZE);
+    uint64_t padded = 1;
+    while (padded < n + n_rand) padded <<= 1;
+    require(padded <= MAX_DOMAIN, Z0P_ERR_INVALID_DOMAIN_SIZE);
+    P.mask.resize(2 * n_rand);
+    for (auto& v : P.mask) v = rng.next_field();
+    require(padded <= MAX_DOMAIN / p.blowup, Z0P_ERR_INVALID_DOMAIN_SIZE);  // evaluate_on_coset (ntt.rs:341-352)
+    P.two_col = true;
+    P.n_rand = n_rand;
+    init_air_shape(c, P, p, tr, padded * p.blowup, n + n_rand - 1);
 }
 
 void init_fri(Ctx& c, Proof& P, const uint32_t* coeffs, uint64_t n, const z0p_fri_params& fp, Transcript* tr) {
@@ -537,6 +588,12 @@ void start(Ctx& c, Proof& P) {
         store_le32(pb + 48, P.b0);
         store_le32(pb + 52, P.b_last);
         tr.absorb("fib/params", pb, 56);
+        if (P.host_a) {
+            hip_check(hipMemcpyAsync(P.col, P.host_a, P.n * 4, hipMemcpyHostToDevice, c.stream));
+            hip_check(hipMemcpyAsync(P.colB, P.host_b, P.n * 4, hipMemcpyHostToDevice, c.stream));
+        }
+        if (P.n_rand)
+            hip_check(hipMemcpyAsync(P.mask_dev, P.mask.data(), P.mask.size() * 4, hipMemcpyHostToDevice, c.stream));
     } else {
         uint8_t dims[24];
         store_le64(dims, P.m);
@@ -577,7 +634,39 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
     const GeometryTables& gt = *P.gt;
     switch (P.phase) {
         case PH_FRONT: {
-            if (P.air) {
+            if (P.air && P.two_col) {
+                if (!P.host_a) {
+                    // B[i] = A[i+1]: the same recurrence started at (b0, a0 + b0).
+                    ch.push_back(FibJob{P.col, P.n, P.a0, P.b0, 0});
+                    ch.push_back(FibJob{P.colB, P.n, P.b0, add(P.a0, P.b0), 0});
+                }
+                const uint32_t ninv = to_mont(inv_c(uint32_t(P.n % P_MOD)));
+                for (int k = 0; k < 2; ++k) {
+                    NttPlan ip = plan_intt(k ? P.colB : P.col, k ? P.coeffB : P.coeff, P.scratch, P.logn,
+                                           c.tw4096_inv, gt.winv, P.logb, ninv);
+                    for (auto& j : ip.pass1) ch.push_back(j);
+                    for (auto& j : ip.pass2) ch.push_back(j);
+                }
+                for (int k = 0; k < 2; ++k) {
+                    NttPlan lp = plan_lde(k ? P.coeffB : P.coeff, k ? P.Bc : P.A, P.scratch, P.logn, P.logb,
+                                          c.tw4096, gt.w, gt.coset);
+                    for (auto& j : lp.pass1) {
+                        if (P.n_rand) {
+                            j.mask = P.mask_dev + k * P.n_rand;
+                            j.zvan = gt.zvan;
+                            j.mask_len = uint32_t(P.n_rand);
+                        }
+                        ch.push_back(j);
+                    }
+                    for (auto& j : lp.pass2) ch.push_back(j);
+                }
+                LeavesJob t = leaves_of(LEAF_TRACE, P.A);
+                t.src_b = P.Bc;
+                t.logn = P.logn;
+                t.logb = P.logb;
+                tree_steps(ch, P.trace_tree, P.tg, t);
+                collect_cap(c, P, P.trace_tree, P.tg, items, used);
+            } else if (P.air) {
                 ch.push_back(FibJob{P.col, P.n, P.a0, P.b0, 0});
                 NttPlan ip = plan_intt(P.col, P.coeff, P.scratch, P.logn, c.tw4096_inv, gt.winv, P.logb,
                                        to_mont(inv_c(uint32_t(P.n % P_MOD))));
@@ -1160,6 +1249,19 @@ Proof* proof_new_air(Ctx& c, const z0p_air_params& p, Transcript* tr) {
     return P.release();
 }
 
+Proof* proof_new_air_trace(Ctx& c, const z0p_air_params& p, const uint32_t* col_a, uint64_t len_a,
+                          const uint32_t* col_b, uint64_t len_b, Transcript* tr) {
+    auto P = std::make_unique<Proof>();
+    init_air_trace(c, *P, p, col_a, len_a, col_b, len_b, tr);
+    return P.release();
+}
+
+Proof* proof_new_air_zk(Ctx& c, const z0p_air_params& p, uint64_t n_rand, ChaCha20& rng, Transcript* tr) {
+    auto P = std::make_unique<Proof>();
+    init_air_zk(c, *P, p, n_rand, rng, tr);
+    return P.release();
+}
+
 Proof* proof_new_fri(Ctx& c, const uint32_t* coeffs, uint64_t n, const z0p_fri_params& fp, Transcript* tr) {
     auto P = std::make_unique<Proof>();
     init_fri(c, *P, coeffs, n, fp, tr);
@@ -1179,14 +1281,29 @@ void proof_times(const Proof* p, double& submit, double& admit, double& done) {
 void proof_set_tag(Proof* p, uint64_t tag) { p->tag = tag; }
 uint64_t proof_tag(const Proof* p) { return p->tag; }
 
-std::vector<uint8_t> air_prove(Ctx& ctx, const z0p_air_params& p, Transcript& tr) {
-    std::unique_ptr<Proof> P(proof_new_air(ctx, p, &tr));
+namespace {
+std::vector<uint8_t> run_one(Ctx& ctx, Proof* raw) {
+    std::unique_ptr<Proof> P(raw);
     Engine e(ctx, 0, 1);
     e.set_pipelined(true);
     e.submit(P.get());
     e.run();
     if (P->status != Z0P_OK) throw Status(P->status);
     return std::move(P->proof);
+}
+}  // namespace
+
+std::vector<uint8_t> air_prove(Ctx& ctx, const z0p_air_params& p, Transcript& tr) {
+    return run_one(ctx, proof_new_air(ctx, p, &tr));
+}
+
+std::vector<uint8_t> air_prove_with_trace(Ctx& ctx, const z0p_air_params& p, const uint32_t* col_a, uint64_t len_a,
+                                          const uint32_t* col_b, uint64_t len_b, Transcript& tr) {
+    return run_one(ctx, proof_new_air_trace(ctx, p, col_a, len_a, col_b, len_b, &tr));
+}
+
+std::vector<uint8_t> air_prove_zk(Ctx& ctx, const z0p_air_params& p, uint64_t n_rand, ChaCha20& rng, Transcript& tr) {
+    return run_one(ctx, proof_new_air_zk(ctx, p, n_rand, rng, &tr));
 }
 
 void fri_commit_open(Ctx& ctx, const uint32_t* coeffs, uint64_t n, const z0p_fri_params& fp, Transcript& tr,

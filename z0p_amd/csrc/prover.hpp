@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "context.hpp"
+#include "rng.hpp"
 #include "transcript.hpp"
 
 namespace z0p {
@@ -54,6 +55,9 @@ uint32_t fib_b_last(uint32_t a0, uint32_t b0, uint64_t n);
 // One proof request and its state (defined in prover.cpp).
 struct Proof;
 Proof* proof_new_air(Ctx& c, const z0p_air_params& p, Transcript* tr);
+Proof* proof_new_air_trace(Ctx& c, const z0p_air_params& p, const uint32_t* col_a, uint64_t len_a,
+                          const uint32_t* col_b, uint64_t len_b, Transcript* tr);
+Proof* proof_new_air_zk(Ctx& c, const z0p_air_params& p, uint64_t n_rand, ChaCha20& rng, Transcript* tr);
 Proof* proof_new_fri(Ctx& c, const uint32_t* coeffs, uint64_t n, const z0p_fri_params& fp, Transcript* tr);
 void proof_free(Proof* p);
 int proof_status(const Proof* p);
@@ -106,6 +110,9 @@ class Engine {
 size_t auto_budget(Ctx& c, size_t want);
 
 std::vector<uint8_t> air_prove(Ctx& ctx, const z0p_air_params& p, Transcript& tr);
+std::vector<uint8_t> air_prove_with_trace(Ctx& ctx, const z0p_air_params& p, const uint32_t* col_a, uint64_t len_a,
+                                          const uint32_t* col_b, uint64_t len_b, Transcript& tr);
+std::vector<uint8_t> air_prove_zk(Ctx& ctx, const z0p_air_params& p, uint64_t n_rand, ChaCha20& rng, Transcript& tr);
 void fri_commit_open(Ctx& ctx, const uint32_t* coeffs, uint64_t n, const z0p_fri_params& fp, Transcript& tr,
                      std::vector<uint8_t>& commitment, std::vector<uint8_t>& proof);
 
@@ -122,4 +129,7 @@ struct z0p_transcript {
 };
 struct z0p_ctx {
     z0p::Ctx* c;
+};
+struct z0p_chacha20 {
+    z0p::ChaCha20* r = nullptr;
 };
