@@ -176,6 +176,30 @@ int z0p_ctx_set_hbm_budget(z0p_ctx* ctx, uint64_t bytes);
 int z0p_air_prove_batch(z0p_ctx* ctx, const z0p_air_params* params, z0p_transcript* const* transcripts,
                         size_t count, uint8_t** proofs, size_t* proof_lens, int* statuses);
 
+/* ---- kernel schedule and autotuner (no reference counterpart: launch
+ * shapes of this library; proof bytes never depend on them) ---- */
+typedef struct z0p_schedule {
+    int32_t tail_max_log;       /* tree levels of at most 2^k digests use the quad-lane tail (2..24) */
+    int32_t tail_group_max_log; /* tail workgroup: at most 2^k digests (2..9) */
+    int32_t tail_spread_log;    /* smaller tail groups so a level spans ~2^k workgroups (0..16) */
+    int32_t levels_fuse;        /* wide tree levels per launch (1 or 2) */
+    int32_t small_max_log;      /* fused fold + commit for FRI layers of at most 2^k values (0 = off, 4..10) */
+    int32_t grind_blocks;       /* proof-of-work workgroups per launch (16..2^20) */
+} z0p_schedule;
+int z0p_ctx_get_schedule(z0p_ctx* ctx, z0p_schedule* out);
+/* Z0P_ERR_INVALID_ARGUMENT for a value outside the ranges above. */
+int z0p_ctx_set_schedule(z0p_ctx* ctx, const z0p_schedule* s);
+/* Proves `work` (one pipelined proof when count is 1, else one batch) under
+ * candidate schedules, coordinate descent over the knobs, `reps` timed runs
+ * per candidate (median), and leaves the fastest set on the ctx. Outputs
+ * (each may be NULL): the schedule, its and the starting schedule's
+ * milliseconds per run of the workload. */
+int z0p_ctx_autotune(z0p_ctx* ctx, const z0p_air_params* work, size_t count, uint32_t reps, z0p_schedule* out,
+                     double* best_ms, double* start_ms);
+/* "knob=value:ms;..." for every trial of the last autotune on this ctx;
+ * returns the length needed including the terminating NUL. */
+size_t z0p_ctx_autotune_log(z0p_ctx* ctx, char* buf, size_t cap);
+
 /* ---- serving (no reference counterpart) ----
  * An asynchronous request loop on its own host thread, which takes over the
  * ctx until z0p_server_destroy. Requests arriving while others run are

@@ -129,6 +129,33 @@ class Context:
     def reset_stats(self):
         lib().z0p_ctx_reset_stats(self._h)
 
+    def get_schedule(self) -> dict:
+        """The kernel launch schedule (z0p_schedule) as a dict."""
+        from ._ffi import ScheduleC
+        s = ScheduleC()
+        check(lib().z0p_ctx_get_schedule(self.handle, ctypes.byref(s)))
+        return s.to_dict()
+
+    def set_schedule(self, schedule: dict):
+        from ._ffi import ScheduleC
+        s = ScheduleC.from_dict(schedule)
+        check(lib().z0p_ctx_set_schedule(self.handle, ctypes.byref(s)))
+
+    def autotune(self, params_list, reps: int = 3):
+        """Times the workload (one proof, or a batch) under candidate
+        schedules and keeps the fastest on this context. Returns
+        (schedule dict, best ms, starting ms, trial log)."""
+        from ._ffi import AirParamsC, ScheduleC
+        ps = (AirParamsC * len(params_list))(*params_list)
+        out = ScheduleC()
+        best, start = ctypes.c_double(), ctypes.c_double()
+        check(lib().z0p_ctx_autotune(self.handle, ps, len(params_list), reps, ctypes.byref(out), ctypes.byref(best),
+                                     ctypes.byref(start)))
+        n = lib().z0p_ctx_autotune_log(self.handle, None, 0)
+        buf = ctypes.create_string_buffer(n)
+        lib().z0p_ctx_autotune_log(self.handle, buf, n)
+        return out.to_dict(), best.value, start.value, buf.value.decode()
+
     def set_hbm_budget(self, nbytes: int):
         """Device-memory budget of batched proving on this context (0: auto)."""
         check(lib().z0p_ctx_set_hbm_budget(self._h, nbytes))

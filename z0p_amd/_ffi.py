@@ -57,6 +57,21 @@ class FriParamsC(ctypes.Structure):
     ]
 
 
+class ScheduleC(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int32) for name in (
+        "tail_max_log", "tail_group_max_log", "tail_spread_log", "levels_fuse", "small_max_log", "grind_blocks")]
+
+    def to_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+    @classmethod
+    def from_dict(cls, d):
+        s = cls()
+        for name, _ in cls._fields_:
+            setattr(s, name, int(d[name]))
+        return s
+
+
 # name -> (restype, argtypes); every symbol declared in include/z0p.h and
 # include/z0p_testing.h.
 SIGNATURES = {
@@ -94,6 +109,11 @@ SIGNATURES = {
                                            ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
     "z0p_fri_verify": (ctypes.c_int, [u8p, sz, u8p, sz, ctypes.POINTER(FriParamsC), vp]),
     "z0p_ctx_set_hbm_budget": (ctypes.c_int, [vp, u64]),
+    "z0p_ctx_get_schedule": (ctypes.c_int, [vp, ctypes.POINTER(ScheduleC)]),
+    "z0p_ctx_set_schedule": (ctypes.c_int, [vp, ctypes.POINTER(ScheduleC)]),
+    "z0p_ctx_autotune": (ctypes.c_int, [vp, ctypes.POINTER(AirParamsC), sz, u32, ctypes.POINTER(ScheduleC),
+                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "z0p_ctx_autotune_log": (sz, [vp, ctypes.c_char_p, sz]),
     "z0p_air_prove_batch": (ctypes.c_int, [vp, ctypes.POINTER(AirParamsC), ctypes.POINTER(vp), sz,
                                            ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_int)]),
     "z0p_server_create": (ctypes.c_int, [vp, u64, u32, ctypes.POINTER(vp)]),

@@ -97,6 +97,56 @@ const char* z0p_status_string(int s) {
 
 void z0p_free(void* p) { std::free(p); }
 
+namespace {
+z0p_schedule to_c(const z0p::Schedule& s) {
+    return z0p_schedule{s.tail_max_log, s.tail_group_max_log, s.tail_spread_log, s.levels_fuse, s.small_max_log,
+                        s.grind_blocks};
+}
+}  // namespace
+
+int z0p_ctx_get_schedule(z0p_ctx* ctx, z0p_schedule* out) {
+    if (!ctx || !out) return Z0P_ERR_INVALID_ARGUMENT;
+    *out = to_c(ctx->c->sched);
+    return Z0P_OK;
+}
+
+int z0p_ctx_set_schedule(z0p_ctx* ctx, const z0p_schedule* in) {
+    if (!ctx || !in) return Z0P_ERR_INVALID_ARGUMENT;
+    z0p::Schedule s;
+    s.tail_max_log = in->tail_max_log;
+    s.tail_group_max_log = in->tail_group_max_log;
+    s.tail_spread_log = in->tail_spread_log;
+    s.levels_fuse = in->levels_fuse;
+    s.small_max_log = in->small_max_log;
+    s.grind_blocks = in->grind_blocks;
+    if (!s.valid()) return Z0P_ERR_INVALID_ARGUMENT;
+    ctx->c->sched = s;
+    return Z0P_OK;
+}
+
+int z0p_ctx_autotune(z0p_ctx* ctx, const z0p_air_params* work, size_t count, uint32_t reps, z0p_schedule* out,
+                     double* best_ms, double* start_ms) {
+    if (!ctx || !work || count == 0 || reps == 0) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] {
+        z0p::TuneResult r = z0p::autotune(*ctx->c, std::vector<z0p_air_params>(work, work + count), reps);
+        ctx->tune_log = r.log;
+        if (out) *out = to_c(r.best);
+        if (best_ms) *best_ms = r.best_ms;
+        if (start_ms) *start_ms = r.start_ms;
+    });
+}
+
+size_t z0p_ctx_autotune_log(z0p_ctx* ctx, char* buf, size_t cap) {
+    if (!ctx) return 0;
+    const std::string& s = ctx->tune_log;
+    if (buf && cap) {
+        size_t n = std::min(cap - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = 0;
+    }
+    return s.size() + 1;
+}
+
 z0p_transcript* z0p_transcript_new(const uint8_t* label, size_t label_len) {
     auto* t = new (std::nothrow) z0p_transcript;
     if (t) t->t = z0p::Transcript(label, label_len);

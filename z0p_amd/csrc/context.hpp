@@ -215,6 +215,33 @@ class HostPool {
     bool stop_ = false;
 };
 
+// Launch shapes that change speed and never results; set by hand
+// (z0p_ctx_set_schedule) or measured by the autotuner (tuner.cpp).
+struct Schedule {
+    int tail_max_log = 17;       // levels of at most 2^k digests go to the quad-lane tail
+    int tail_group_max_log = 7;  // tail workgroup: at most 2^k digests (2^(k+1) threads)
+    int tail_spread_log = 0;     // smaller groups so a level spans about 2^k workgroups
+    int levels_fuse = 2;         // wide levels per launch (1 or 2)
+    int small_max_log = 10;      // fold + commit in one workgroup for layers of at most 2^k values (0: off)
+    int grind_blocks = 4096;     // grind workgroups per launch, shared by the proofs grinding
+
+    // Digests per tail workgroup (log2) for a level of `count` digests.
+    int tail_group_log(uint64_t count) const {
+        int lc = 0;
+        while ((uint64_t(1) << lc) < count) ++lc;
+        int g = lc - tail_spread_log;
+        if (g > tail_group_max_log) g = tail_group_max_log;
+        if (g < TAIL_GROUP_MIN_LOG) g = TAIL_GROUP_MIN_LOG;
+        return g;
+    }
+    bool valid() const {
+        return tail_max_log >= TAIL_GROUP_MIN_LOG && tail_max_log <= 24 && tail_group_max_log >= TAIL_GROUP_MIN_LOG &&
+               tail_group_max_log <= TAIL_GROUP_MAX_LOG && tail_spread_log >= 0 && tail_spread_log <= 16 &&
+               (levels_fuse == 1 || levels_fuse == 2) && small_max_log >= 0 && small_max_log <= 10 &&
+               (small_max_log == 0 || small_max_log >= 4) && grind_blocks >= 16 && grind_blocks <= (1 << 20);
+    }
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -222,6 +249,7 @@ struct Ctx {
     DevicePool pool;
     DescRing ring;
     size_t hbm_budget = 0;  // 0: the pool grows to fit (between batches)
+    Schedule sched;
 
     // Staging for the words the host transcript needs: pinned host memory
     // the kernels write directly (stage_dev is its device address).

@@ -888,25 +888,27 @@ __device__ __forceinline__ void quad_node(const uint32_t (&l)[8], const uint32_t
     ob = hb;
 }
 
-// Narrow levels: up to TAIL_LEVELS levels per 128-digest group in LDS, one
-// quad per parent (256 threads: the 64 parents of the first level at once).
-// Small groups spread a level over many compute units.
-constexpr int TAIL_SUB = 1 << TAIL_LEVELS;
-constexpr int TAIL_WG = 2 * TAIL_SUB;
+// Narrow levels: up to group_log levels per 2^group_log-digest group in LDS,
+// one quad per parent (2^(group_log+1) threads: every parent of the first
+// level at once). Active quads stay packed in the lowest waves, so a narrow
+// level occupies as few waves as it can; small groups spread a level over
+// many compute units, large ones need fewer launches.
+constexpr int TAIL_MAX_SUB = 1 << TAIL_GROUP_MAX_LOG;
 
-__global__ __launch_bounds__(TAIL_WG) void k_tail(TailJob inl, const TailJob* tab, uint32_t njobs) {
-    __shared__ uint32_t bufA[8][TAIL_SUB];
-    __shared__ uint32_t bufB[8][TAIL_SUB];
+__global__ __launch_bounds__(2 * TAIL_MAX_SUB) void k_tail(TailJob inl, const TailJob* tab, uint32_t njobs) {
+    __shared__ uint32_t bufA[8][TAIL_MAX_SUB];
+    __shared__ uint32_t bufB[8][TAIL_MAX_SUB];
     uint32_t local;
     const TailJob J = job_of(inl, tab, njobs, local);
-    const uint64_t first = uint64_t(local) * TAIL_SUB;
-    const int here = int(J.count - first < uint64_t(TAIL_SUB) ? J.count - first : uint64_t(TAIL_SUB));
+    const uint64_t sub = uint64_t(1) << J.group_log;
+    const uint64_t first = uint64_t(local) * sub;
+    const int here = int(J.count - first < sub ? J.count - first : sub);
     const int tid = threadIdx.x;
     const gcptr level_in = gc(J.level);
-    for (int w = tid; w < here * 8; w += TAIL_WG) bufA[w & 7][w >> 3] = level_in[(first * 8) + w];
+    for (int w = tid; w < here * 8; w += int(blockDim.x)) bufA[w & 7][w >> 3] = level_in[(first * 8) + w];
     __syncthreads();
-    uint32_t(*cur)[TAIL_SUB] = bufA;
-    uint32_t(*nxt)[TAIL_SUB] = bufB;
+    uint32_t(*cur)[TAIL_MAX_SUB] = bufA;
+    uint32_t(*nxt)[TAIL_MAX_SUB] = bufB;
     const uint32_t q = uint32_t(tid & 3);
     const int g = tid >> 2;
     uint32_t* lvl = J.level;
@@ -1346,14 +1348,20 @@ void launch_levels(const Launch& L, std::vector<LevelsJob> jobs) {
     run(L, "tree_levels", k_levels, jobs, g, 0);
 }
 
-void launch_tail(const Launch& L, std::vector<TailJob> jobs) {
-    unsigned g = assign_blocks(jobs, [](const TailJob& j) { return grid_for(j.count, TAIL_SUB); });
-    if (jobs.empty() || g == 0) return;
-    const TailJob* table = table_of(L, jobs);
-    if (L.dry) return;
-    const uint32_t n = jobs.size() <= 1 ? 0u : uint32_t(jobs.size());
-    ProfScope ps(L.prof, "tree_tail", L.stream);
-    hipLaunchKernelGGL(k_tail, dim3(g), dim3(TAIL_WG), 0, L.stream, jobs[0], table, n);
+// One launch per group size: the workgroup is two threads per digest.
+void launch_tail(const Launch& L, std::vector<TailJob> all) {
+    for (int gl = TAIL_GROUP_MIN_LOG; gl <= TAIL_GROUP_MAX_LOG; ++gl) {
+        std::vector<TailJob> jobs;
+        for (const TailJob& j : all)
+            if (j.group_log == gl) jobs.push_back(j);
+        unsigned g = assign_blocks(jobs, [gl](const TailJob& j) { return grid_for(j.count, uint64_t(1) << gl); });
+        if (jobs.empty() || g == 0) continue;
+        const TailJob* table = table_of(L, jobs);
+        if (L.dry) continue;
+        const uint32_t n = jobs.size() <= 1 ? 0u : uint32_t(jobs.size());
+        ProfScope ps(L.prof, "tree_tail", L.stream);
+        hipLaunchKernelGGL(k_tail, dim3(g), dim3(2u << gl), 0, L.stream, jobs[0], table, n);
+    }
 }
 
 void launch_small_commit(const Launch& L, std::vector<SmallCommitJob> jobs) {

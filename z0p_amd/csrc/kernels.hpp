@@ -143,12 +143,14 @@ struct LevelsJob {
     uint32_t block_begin;
 };
 
-// Up to TAIL_LEVELS levels in LDS per 2^TAIL_LEVELS-digest group: the
-// narrow tail (one four-lane quad per parent node).
+// Up to group_log levels in LDS per 2^group_log-digest group (one
+// workgroup of 2^(group_log+1) threads): the narrow tail, one four-lane quad
+// per parent node.
 struct TailJob {
     uint32_t* level;
     uint64_t count;
     int nlevels;
+    int group_log;  // TAIL_GROUP_MIN_LOG..TAIL_GROUP_MAX_LOG
     uint32_t block_begin;
 };
 
@@ -276,8 +278,8 @@ NttPlan plan_lde(const uint32_t* coeffs_br, uint32_t* lde_br, uint32_t* scratch,
 // A level of at most this many digests is reduced by the LDS tail kernel
 // (latency-bound: a quad of lanes per node); wider levels use the
 // one-node-per-thread kernel (throughput-bound).
-constexpr uint64_t TAIL_MAX_COUNT = uint64_t(1) << 17;
-constexpr int TAIL_LEVELS = 7;
+constexpr int TAIL_GROUP_MIN_LOG = 2;
+constexpr int TAIL_GROUP_MAX_LOG = 9;  // 512 digests, 1024 threads
 
 }  // namespace z0p
 

@@ -208,7 +208,7 @@ TreeGeom tree_geom(uint64_t leaves, uint64_t cap_height) {
     return g;
 }
 
-void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves) {
+void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves, const Schedule& s) {
     leaves.levels = tree;
     leaves.leaves = g.leaves;
     leaves.with_parent = g.target >= 1 ? 1 : 0;
@@ -218,13 +218,14 @@ void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, Lea
         const uint64_t count = g.leaves >> have;
         const int remaining = g.target - have;
         uint32_t* lvl = tree + g.level_offset_words(have);
-        if (count > TAIL_MAX_COUNT) {
-            const int fuse = remaining >= 2 ? 2 : 1;
+        if (count > (uint64_t(1) << s.tail_max_log)) {
+            const int fuse = std::min(s.levels_fuse, remaining);
             chain.push_back(LevelsJob{lvl, count, fuse, 0});
             have += fuse;
         } else {
-            const int k = std::min(TAIL_LEVELS, remaining);
-            chain.push_back(TailJob{lvl, count, k, 0});
+            const int gl = s.tail_group_log(count);
+            const int k = std::min(gl, remaining);
+            chain.push_back(TailJob{lvl, count, k, gl, 0});
             have += k;
         }
     }
@@ -664,7 +665,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 t.src_b = P.Bc;
                 t.logn = P.logn;
                 t.logb = P.logb;
-                tree_steps(ch, P.trace_tree, P.tg, t);
+                tree_steps(ch, P.trace_tree, P.tg, t, c.sched);
                 collect_cap(c, P, P.trace_tree, P.tg, items, used);
             } else if (P.air) {
                 ch.push_back(FibJob{P.col, P.n, P.a0, P.b0, 0});
@@ -690,14 +691,14 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 t.src_b = P.Bc;
                 t.logn = P.logn;
                 t.logb = P.logb;
-                tree_steps(ch, P.trace_tree, P.tg, t);
+                tree_steps(ch, P.trace_tree, P.tg, t, c.sched);
                 collect_cap(c, P, P.trace_tree, P.tg, items, used);
             } else {
                 ch.push_back(BitrevJob{P.col, P.coeff, P.logn, 0});
                 NttPlan lp = plan_lde(P.coeff, P.vals[0], P.scratch, P.logn, P.logb, c.tw4096, gt.w, gt.coset);
                 for (auto& j : lp.pass1) ch.push_back(j);
                 for (auto& j : lp.pass2) ch.push_back(j);
-                tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_BASE, P.vals[0]));
+                tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_BASE, P.vals[0]), c.sched);
                 collect_cap(c, P, P.trees[0], P.geoms[0], items, used);
             }
             break;
@@ -714,14 +715,14 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
             cj.logn = P.logn;
             cj.logb = P.logb;
             ch.push_back(cj);
-            tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_EXT, P.vals[0]));
+            tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_EXT, P.vals[0]), c.sched);
             collect_cap(c, P, P.trees[0], P.geoms[0], items, used);
             break;
         }
         case PH_FOLD: {
             const size_t d = P.depth;
             const FoldConsts* fc = static_cast<const FoldConsts*>(cst ? cst : c.ring.push(&P.fold, sizeof(P.fold)));
-            if (d + 1 < P.folds && P.lens[d + 1] <= SMALL_MAX_VALUES) {
+            if (d + 1 < P.folds && c.sched.small_max_log > 0 && P.lens[d + 1] <= (uint64_t(1) << c.sched.small_max_log)) {
                 // Small next layer: fold, leaves and tree in one workgroup.
                 SmallCommitJob sj{};
                 sj.layer = P.vals[d];
@@ -749,7 +750,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 uint32_t* dst = stage_take(c, used, words, P.stage_off);
                 items.push_back(CollectItem{P.vals[d + 1], dst, words, 0});
             } else {
-                tree_steps(ch, P.trees[d + 1], P.geoms[d + 1], leaves_of(LEAF_EXT, P.vals[d + 1]));
+                tree_steps(ch, P.trees[d + 1], P.geoms[d + 1], leaves_of(LEAF_EXT, P.vals[d + 1]), c.sched);
                 collect_cap(c, P, P.trees[d + 1], P.geoms[d + 1], items, used);
             }
             break;
@@ -758,7 +759,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
             GrindJob g{};
             g.hw = static_cast<const uint32_t*>(cst ? cst : c.ring.push(P.pow_hw, sizeof(P.pow_hw)));
             g.bits = P.fp.grinding_bits;
-            g.nblocks = 2048;
+            g.nblocks = uint32_t(std::min(2048, std::max(16, c.sched.grind_blocks)));  // one proof; batches re-spread
             g.limit = uint64_t(1) << std::min<uint64_t>(63, 24 + 4 * uint64_t(P.window));
             g.result = P.grind_res;
             ch.push_back(g);
@@ -1071,12 +1072,12 @@ bool Engine::iterate() {
     for (size_t i = 0; i < active_.size(); ++i) build(c_, *active_[i], chains[i], items, used);
     // Grind sweeps: a job's workgroups scan its nonces in order, so a
     // smaller sweep per job wastes fewer hashes above the smallest passing
-    // nonce; spread about 4096 workgroups over the jobs grinding now.
+    // nonce; spread the schedule's workgroups over the jobs grinding now.
     size_t grinding = 0;
     for (auto& ch : chains)
         for (auto& st : ch) grinding += std::holds_alternative<GrindJob>(st) ? 1 : 0;
     if (grinding) {
-        const uint32_t per = uint32_t(std::min<size_t>(2048, std::max<size_t>(16, 4096 / grinding)));
+        const uint32_t per = uint32_t(std::min<size_t>(2048, std::max<size_t>(16, size_t(c_.sched.grind_blocks) / grinding)));
         for (auto& ch : chains)
             for (auto& st : ch)
                 if (auto* g = std::get_if<GrindJob>(&st)) g->nblocks = per;

@@ -3,6 +3,7 @@
 #include <cstdint>
 #include <deque>
 #include <functional>
+#include <string>
 #include <variant>
 #include <vector>
 
@@ -40,7 +41,7 @@ using Step = std::variant<FibJob, BitrevJob, NttJob, LinkedJob, ComposeJob, Leav
                           SmallCommitJob, GrindJob, OpenJob>;
 
 // Leaves, then wide levels, then LDS tail launches up to the cap level.
-void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves);
+void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves, const Schedule& s);
 
 // Launches step r of every chain for r = 0, 1, ... (one launch per kernel
 // type per r), then the collect items, which end by storing `seq` to
@@ -109,6 +110,17 @@ class Engine {
 // memory that is free (counting an idle pool as free).
 size_t auto_budget(Ctx& c, size_t want);
 
+// Autotuner result (tuner.cpp): the chosen schedule (also left set on the
+// context), its time and the starting schedule's, and every trial as
+// "knob=value:ms;".
+struct TuneResult {
+    Schedule best;
+    double start_ms = 0, best_ms = 0;
+    unsigned trials = 0;
+    std::string log;
+};
+TuneResult autotune(Ctx& c, const std::vector<z0p_air_params>& work, unsigned reps);
+
 std::vector<uint8_t> air_prove(Ctx& ctx, const z0p_air_params& p, Transcript& tr);
 std::vector<uint8_t> air_prove_with_trace(Ctx& ctx, const z0p_air_params& p, const uint32_t* col_a, uint64_t len_a,
                                           const uint32_t* col_b, uint64_t len_b, Transcript& tr);
@@ -129,6 +141,7 @@ struct z0p_transcript {
 };
 struct z0p_ctx {
     z0p::Ctx* c;
+    std::string tune_log;  // trials of the last z0p_ctx_autotune
 };
 struct z0p_chacha20 {
     z0p::ChaCha20* r = nullptr;

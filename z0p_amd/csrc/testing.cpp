@@ -114,7 +114,7 @@ int z0p_test_merkle64(z0p_ctx* ctx, const uint8_t* flat, size_t leaves, uint8_t*
         lj.kind = LEAF_RAW;
         lj.src = din;
         std::vector<Step> chain;
-        tree_steps(chain, tree, g, lj);
+        tree_steps(chain, tree, g, lj, c.sched);
         run_chain(c, chain);
         to_host(c, reinterpret_cast<uint32_t*>(levels), tree, g.words());
     });
@@ -147,7 +147,7 @@ int z0p_test_trace_tree(z0p_ctx* ctx, const uint32_t* a, const uint32_t* b, size
         lj.logn = logn;
         lj.logb = logb;
         std::vector<Step> chain;
-        tree_steps(chain, tree, g, lj);
+        tree_steps(chain, tree, g, lj, c.sched);
         run_chain(c, chain);
         to_host(c, reinterpret_cast<uint32_t*>(levels), tree, g.words());
     });
