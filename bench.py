@@ -72,15 +72,28 @@ def seeded_instance(z, k, seed, label):
     return z.air.for_security(n, a0, b0, z.air.b_last(a0, b0, n)), label
 
 
-def batched(z, ctx, count, seed=2024):
+def tune(ctx, params_list, reps):
+    """Autotune the context's kernel schedule on this workload (untimed)."""
+    if reps <= 0:
+        return None
+    t0 = time.perf_counter()
+    sched, best, start, log = ctx.autotune(params_list, reps)
+    return {"schedule": sched, "ms_tuned": round(best, 4), "ms_default": round(start, 4),
+            "trials": log.count(";"), "tune_s": round(time.perf_counter() - t0, 2)}
+
+
+def batched(z, ctx, count, tune_reps, seed=2024):
     """256 independent proofs, log2 steps drawn uniformly from 10..18, proved
-    as one batch; beside it the same instances one at a time."""
+    as one batch; beside it the same instances one at a time (under the
+    context's single-proof schedule)."""
     st = seed
     inst = []
     for i in range(count):
         st, r = splitmix64(st)
         inst.append(seeded_instance(z, 10 + r % 9, seed * 7919 + i, b"batch/%d" % i))
     steps = sum(p.n_steps for p, _ in inst)
+    single_sched = ctx.get_schedule()
+    tuned = tune(ctx, [p for p, _ in inst], tune_reps)
     # Warm-up: the whole batch once (device tables of every geometry, the
     # memory pool at its batch size), then the timed batch.
     z.air.prove_batch([p for p, _ in inst], [z.Transcript(l) for _, l in inst], ctx)
@@ -88,10 +101,13 @@ def batched(z, ctx, count, seed=2024):
     got = z.air.prove_batch([p for p, _ in inst], [z.Transcript(l) for _, l in inst], ctx)
     t_batch = time.perf_counter() - t0
     assert all(isinstance(g, bytes) for g in got)
+    batch_sched = ctx.get_schedule()
+    ctx.set_schedule(single_sched)
     t0 = time.perf_counter()
     for p, l in inst:
         z.air.prove(p, z.Transcript(l), ctx)
     t_single = time.perf_counter() - t0
+    ctx.set_schedule(batch_sched)
     return {
         "workload": "%d AIR proofs, steps 2^10..2^18 log-uniform (seed %d), production schedule" % (count, seed),
         "total_steps": steps,
@@ -101,10 +117,12 @@ def batched(z, ctx, count, seed=2024):
         "one_at_a_time_s": round(t_single, 4),
         "one_at_a_time_proofs_per_s": round(count / t_single, 1),
         "speedup_vs_one_at_a_time": round(t_single / t_batch, 2),
+        "autotune": tuned,
+        "schedule": batch_sched,
     }
 
 
-def served(z, budget_gib, duration_s, seed=77):
+def served(z, budget_gib, duration_s, schedule, seed=77):
     """Open-loop Poisson arrivals of AIR proofs with log2 steps uniform in
     10..20 against a fixed HBM budget: saturated throughput first (all
     requests at once), then three offered loads below it."""
@@ -149,6 +167,8 @@ def served(z, budget_gib, duration_s, seed=77):
     # One long-lived server, as in deployment; one request of every size
     # first brings each geometry's device tables in.
     sctx = z.Context(0)
+    if schedule:
+        sctx.set_schedule(schedule)
     levels = []
     with z.Server(sctx, hbm_budget=budget_gib << 30) as srv:
         run(srv, [seeded_instance(z, k, k, b"serve/warm/%d" % k) for k in range(10, 21)], [0.0] * 11)
@@ -178,6 +198,7 @@ def served(z, budget_gib, duration_s, seed=77):
         "saturated_proofs_per_s": round(sat, 1),
         "one_at_a_time_proofs_per_s": round(single, 1),
         "saturated_vs_one_at_a_time": round(sat / single, 2),
+        "schedule": schedule,
         "levels": levels,
     }
 
@@ -218,6 +239,11 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="instances in the batched workload; 0 skips it")
     ap.add_argument("--serve-s", type=float, default=2.0, help="seconds of arrivals per served load; 0 skips it")
     ap.add_argument("--serve-budget-gib", type=int, default=32)
+    ap.add_argument("--tune-reps", type=int, default=5,
+                    help="autotune the kernel schedule on each workload first (timed runs per candidate); "
+                         "0 keeps the default schedule")
+    ap.add_argument("--schedule", default=None,
+                    help="JSON kernel schedule to use instead of tuning (z0p_schedule fields)")
     ap.add_argument("--copy-mib", type=int, default=256,
                     help="size of the plain copy kernel run after the proofs (counter calibration and "
                          "achievable-bandwidth reference); 0 disables")
@@ -234,6 +260,11 @@ def main():
     params = z.air.for_security(n, 1, 1, bl)
     ctx = z.Context(0)
     proof = None
+    single_tune = None
+    if args.schedule:
+        ctx.set_schedule(json.loads(args.schedule))
+    else:
+        single_tune = tune(ctx, [params], args.tune_reps)
     for _ in range(args.warmup):
         proof = z.air.prove(params, z.Transcript(LABEL), ctx)
     # Timed region: plain proofs, no per-kernel events (each call returns
@@ -318,8 +349,10 @@ def main():
             roofline["valu"] = {"achieved": got, "peak": round(peak_gc, 1), "unit": "Gcompress/s",
                                 "frac": round(got / peak_gc, 4)}
 
-    batch = batched(z, ctx, args.batch) if args.batch else None
-    serve = served(z, args.serve_budget_gib, args.serve_s) if args.serve_s > 0 else None
+    single_sched = ctx.get_schedule()
+    batch = batched(z, ctx, args.batch, 0 if args.schedule else max(1, args.tune_reps // 2)) if args.batch else None
+    serve_sched = batch["schedule"] if batch else single_sched
+    serve = served(z, args.serve_budget_gib, args.serve_s, serve_sched) if args.serve_s > 0 else None
 
     out = {
         "metric": METRIC,
@@ -338,6 +371,8 @@ def main():
                    "proof_bytes": len(proof), "proofs_per_s": round(1000.0 / ms, 2)},
         "kernels_ms_per_proof": {k: round(v[1] / per, 4) for k, v in sorted(stats.items()) if not k.startswith("~")},
         "host_ms_per_proof": host,
+        "schedule": single_sched,
+        "autotune": single_tune,
         "families": families,
         "roofline": roofline,
         "copy_reference": copy,

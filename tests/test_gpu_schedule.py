@@ -13,7 +13,7 @@ from z0p_amd import _ffi
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = dict(tail_max_log=17, tail_group_max_log=7, tail_spread_log=0, levels_fuse=2, small_max_log=10,
+DEFAULT = dict(tail_max_log=17, tail_group_max_log=8, tail_spread_log=0, levels_fuse=2, small_max_log=10,
                grind_blocks=4096)
 
 SCHEDULES = [

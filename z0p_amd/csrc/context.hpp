@@ -219,7 +219,7 @@ class HostPool {
 // (z0p_ctx_set_schedule) or measured by the autotuner (tuner.cpp).
 struct Schedule {
     int tail_max_log = 17;       // levels of at most 2^k digests go to the quad-lane tail
-    int tail_group_max_log = 7;  // tail workgroup: at most 2^k digests (2^(k+1) threads)
+    int tail_group_max_log = 8;  // tail workgroup: at most 2^k digests (2^(k+1) threads)
     int tail_spread_log = 0;     // smaller groups so a level spans about 2^k workgroups
     int levels_fuse = 2;         // wide levels per launch (1 or 2)
     int small_max_log = 10;      // fold + commit in one workgroup for layers of at most 2^k values (0: off)
