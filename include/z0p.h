@@ -158,6 +158,23 @@ uint32_t z0p_chacha20_next_field(z0p_chacha20* rng);
 int z0p_air_prove_zk(z0p_ctx* ctx, const z0p_air_params* params, uint64_t n_rand, z0p_chacha20* rng,
                      z0p_transcript* transcript, uint8_t** proof, size_t* proof_len);
 
+/* fri::commit_ext (fri.rs:444-461) with ntt::evaluate_on_coset_ext
+ * (ntt.rs:394-420) + public_commitment + open: n Ext4 coefficients, 4 words
+ * each (coordinate 0 first). Outputs as z0p_fri_commit_open. */
+int z0p_fri_commit_open_ext(z0p_ctx* ctx, const uint32_t* coeffs4, size_t n, const z0p_fri_params* params,
+                            z0p_transcript* transcript, uint8_t** commitment, size_t* commitment_len, uint8_t** proof,
+                            size_t* proof_len);
+/* fri::commit_evals (fri.rs:473-500) + public_commitment + open: m base-field
+ * LDE values in natural order on the 31-coset, declared degree_bound < m. */
+int z0p_fri_commit_open_evals(z0p_ctx* ctx, const uint32_t* evals, size_t m, uint64_t degree_bound,
+                              const z0p_fri_params* params, z0p_transcript* transcript, uint8_t** commitment,
+                              size_t* commitment_len, uint8_t** proof, size_t* proof_len);
+/* fri::commit_ext_evals (fri.rs:510-549) + public_commitment + open: m Ext4
+ * LDE values (4 words each) in natural order, declared degree_bound < m. */
+int z0p_fri_commit_open_ext_evals(z0p_ctx* ctx, const uint32_t* evals4, size_t m, uint64_t degree_bound,
+                                  const z0p_fri_params* params, z0p_transcript* transcript, uint8_t** commitment,
+                                  size_t* commitment_len, uint8_t** proof, size_t* proof_len);
+
 /* Commitment/Proof::from_bytes + fri::verify (fri.rs:686-859). Host only. */
 int z0p_fri_verify(const uint8_t* commitment, size_t commitment_len, const uint8_t* proof, size_t proof_len,
                    const z0p_fri_params* params, z0p_transcript* transcript);

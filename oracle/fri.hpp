@@ -441,6 +441,27 @@ inline ProverState commit(const std::vector<Field>& coeffs, const Params& params
     return commit_inner(std::move(ext), coeffs.size() - 1, coeffs.size(), params, transcript);
 }
 
+// fri.rs:444-461
+inline ProverState commit_ext(const std::vector<Ext4>& coeffs, const Params& params, Transcript& transcript) {
+    params.validate();
+    if (coeffs.empty()) throw Z0pError(ERR_EMPTY_INPUT);
+    if (!is_pow2(coeffs.size())) throw Z0pError(ERR_INVALID_DOMAIN_SIZE);
+    std::vector<Ext4> lde = evaluate_on_coset_ext(coeffs, params.blowup, lde_shift());
+    return commit_inner(std::move(lde), coeffs.size() - 1, coeffs.size(), params, transcript);
+}
+
+// fri.rs:473-500
+inline ProverState commit_evals(const std::vector<Field>& evals, size_t degree_bound, const Params& params,
+                                Transcript& transcript) {
+    params.validate();
+    if (evals.empty()) throw Z0pError(ERR_EMPTY_INPUT);
+    if (!is_pow2(evals.size()) || evals.size() > (size_t(1) << Field::TWO_ADICITY)) throw Z0pError(ERR_INVALID_DOMAIN_SIZE);
+    if (degree_bound >= evals.size()) throw Z0pError(ERR_MALFORMED_INPUT, "degree bound must be below LDE size (rate < 1)");
+    std::vector<Ext4> ext(evals.size());
+    for (size_t i = 0; i < evals.size(); ++i) ext[i] = Ext4::from_base(evals[i]);
+    return commit_inner(std::move(ext), degree_bound, evals.size(), params, transcript);
+}
+
 // fri.rs:527-549
 inline ProverState commit_ext_evals_owned(std::vector<Ext4> evals, size_t degree_bound, const Params& params,
                                           Transcript& transcript) {

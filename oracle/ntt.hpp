@@ -185,4 +185,18 @@ inline std::vector<Field> evaluate_on_coset(const std::vector<Field>& coeffs, si
     return ext;
 }
 
+// ntt.rs:394-420 — each coordinate extended independently over the base field.
+inline std::vector<Ext4> evaluate_on_coset_ext(const std::vector<Ext4>& coeffs, size_t blowup, Field shift) {
+    if (coeffs.empty()) throw Z0pError(ERR_EMPTY_INPUT);
+    std::vector<Field> coord[4];
+    for (int c = 0; c < 4; ++c) {
+        std::vector<Field> v(coeffs.size());
+        for (size_t i = 0; i < coeffs.size(); ++i) v[i] = coeffs[i].c[c];
+        coord[c] = evaluate_on_coset(v, blowup, shift);
+    }
+    std::vector<Ext4> out(coord[0].size());
+    for (size_t i = 0; i < out.size(); ++i) out[i] = Ext4::make(coord[0][i], coord[1][i], coord[2][i], coord[3][i]);
+    return out;
+}
+
 }  // namespace oracle

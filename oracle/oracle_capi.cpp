@@ -378,6 +378,48 @@ int oracle_fri_commit_open(const uint32_t* coeffs, size_t n, const oracle_fri_pa
         *pf = dup_bytes(opened.first.to_bytes(), pf_len);
     })
 }
+namespace {
+std::vector<Ext4> to_ext(const uint32_t* v, size_t n) {
+    std::vector<Ext4> out(n);
+    for (size_t i = 0; i < n; ++i)
+        out[i] = Ext4::make(Field::make(v[4 * i]), Field::make(v[4 * i + 1]), Field::make(v[4 * i + 2]),
+                            Field::make(v[4 * i + 3]));
+    return out;
+}
+int fri_finish(fri::ProverState st, Transcript& tr, uint8_t** cm, size_t* cm_len, uint8_t** pf, size_t* pf_len) {
+    fri::Commitment c = fri::public_commitment(st);
+    auto opened = fri::open(st, tr);
+    *cm = dup_bytes(c.to_bytes(), cm_len);
+    *pf = dup_bytes(opened.first.to_bytes(), pf_len);
+    return 0;
+}
+}  // namespace
+
+// kind 0: commit_ext (Ext4 coefficients), 1: commit_evals (base LDE values),
+// 2: commit_ext_evals (Ext4 LDE values); values are 4 words per Ext4.
+int oracle_fri_commit_open_kind(int kind, const uint32_t* vals, size_t n, size_t degree_bound,
+                                const oracle_fri_params* p, void* t, uint8_t** cm, size_t* cm_len, uint8_t** pf,
+                                size_t* pf_len) {
+    GUARD({
+        Transcript& tr = *static_cast<Transcript*>(t);
+        if (kind == 0)
+            fri_finish(fri::commit_ext(to_ext(vals, n), to_fri(p), tr), tr, cm, cm_len, pf, pf_len);
+        else if (kind == 1)
+            fri_finish(fri::commit_evals(to_fields(vals, n), degree_bound, to_fri(p), tr), tr, cm, cm_len, pf, pf_len);
+        else
+            fri_finish(fri::commit_ext_evals_owned(to_ext(vals, n), degree_bound, to_fri(p), tr), tr, cm, cm_len, pf,
+                       pf_len);
+    })
+}
+// ntt.rs:394-420, coordinates interleaved as 4 words per value.
+int oracle_evaluate_on_coset_ext(const uint32_t* coeffs4, size_t n, size_t blowup, uint32_t* out4) {
+    GUARD({
+        std::vector<Ext4> v = evaluate_on_coset_ext(to_ext(coeffs4, n), blowup, fri::lde_shift());
+        for (size_t i = 0; i < v.size(); ++i)
+            for (int c = 0; c < 4; ++c) out4[4 * i + c] = v[i].c[c].v;
+    })
+}
+
 int oracle_fri_verify(const uint8_t* cm, size_t cm_len, const uint8_t* pf, size_t pf_len, const oracle_fri_params* p,
                       void* t) {
     GUARD({

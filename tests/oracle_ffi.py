@@ -129,6 +129,10 @@ def lib():
         "oracle_fri_commit_open": (ctypes.c_int, [u32p, sz, ctypes.POINTER(FriParams), vp, ctypes.POINTER(u8p),
                                                   ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
         "oracle_fri_verify": (ctypes.c_int, [u8p, sz, u8p, sz, ctypes.POINTER(FriParams), vp]),
+        "oracle_fri_commit_open_kind": (ctypes.c_int, [ctypes.c_int, u32p, sz, sz, ctypes.POINTER(FriParams), vp,
+                                                       ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.POINTER(u8p),
+                                                       ctypes.POINTER(sz)]),
+        "oracle_evaluate_on_coset_ext": (ctypes.c_int, [u32p, sz, sz, u32p]),
         "oracle_stage_timers_clear": (None, []),
         "oracle_stage_timers": (sz, [ctypes.c_char_p, sz]),
     }
@@ -363,6 +367,43 @@ def fri_commit_open(coeffs, params: FriParams, transcript: Transcript):
     if rc:
         raise ValueError("oracle fri error %d" % rc)
     return _take(cm, cn.value), _take(pf, pn.value)
+
+
+def fri_params_new(blowup, num_queries, final_size, grinding_bits=0, cap_height=0) -> FriParams:
+    """fri::Params::new (fri.rs:141-158) plus its two builders; validated
+    when used, as the reference validates at every entry point."""
+    p = FriParams()
+    p.blowup, p.num_queries, p.final_size = blowup, num_queries, final_size
+    p.grinding_bits, p.cap_height = grinding_bits, cap_height
+    return p
+
+
+FRI_KIND_EXT, FRI_KIND_EVALS, FRI_KIND_EXT_EVALS = 0, 1, 2
+
+
+def fri_commit_open_kind(kind, values, params: FriParams, transcript: Transcript, degree_bound=0):
+    """fri::commit_ext (kind 0, Ext4 coefficients as 4 words each),
+    commit_evals (1, base LDE values) or commit_ext_evals (2, Ext4 LDE
+    values), then public_commitment + open; returns (commitment, proof)."""
+    words = [int(v) for v in values]
+    count = len(words) if kind == FRI_KIND_EVALS else len(words) // 4
+    cm, pf = u8p(), u8p()
+    cn, pn = ctypes.c_size_t(), ctypes.c_size_t()
+    rc = lib().oracle_fri_commit_open_kind(kind, u32_array(words), count, degree_bound, ctypes.byref(params),
+                                           transcript.h, ctypes.byref(cm), ctypes.byref(cn), ctypes.byref(pf),
+                                           ctypes.byref(pn))
+    if rc:
+        raise OracleError(rc)
+    return _take(cm, cn.value), _take(pf, pn.value)
+
+
+def evaluate_on_coset_ext(coeffs4, blowup):
+    n = len(coeffs4) // 4
+    out = (ctypes.c_uint32 * (4 * n * blowup))()
+    rc = lib().oracle_evaluate_on_coset_ext(u32_array([int(v) for v in coeffs4]), n, blowup, out)
+    if rc:
+        raise OracleError(rc)
+    return list(out)
 
 
 def fri_verify(commitment: bytes, proof: bytes, params: FriParams, transcript: Transcript) -> int:

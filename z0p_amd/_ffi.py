@@ -108,6 +108,14 @@ SIGNATURES = {
     "z0p_fri_commit_open": (ctypes.c_int, [vp, u32p, sz, ctypes.POINTER(FriParamsC), vp, ctypes.POINTER(u8p),
                                            ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
     "z0p_fri_verify": (ctypes.c_int, [u8p, sz, u8p, sz, ctypes.POINTER(FriParamsC), vp]),
+    "z0p_fri_commit_open_ext": (ctypes.c_int, [vp, u32p, sz, ctypes.POINTER(FriParamsC), vp, ctypes.POINTER(u8p),
+                                               ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+    "z0p_fri_commit_open_evals": (ctypes.c_int, [vp, u32p, sz, u64, ctypes.POINTER(FriParamsC), vp,
+                                                 ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.POINTER(u8p),
+                                                 ctypes.POINTER(sz)]),
+    "z0p_fri_commit_open_ext_evals": (ctypes.c_int, [vp, u32p, sz, u64, ctypes.POINTER(FriParamsC), vp,
+                                                     ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.POINTER(u8p),
+                                                     ctypes.POINTER(sz)]),
     "z0p_ctx_set_hbm_budget": (ctypes.c_int, [vp, u64]),
     "z0p_ctx_get_schedule": (ctypes.c_int, [vp, ctypes.POINTER(ScheduleC)]),
     "z0p_ctx_set_schedule": (ctypes.c_int, [vp, ctypes.POINTER(ScheduleC)]),

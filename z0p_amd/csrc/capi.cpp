@@ -325,16 +325,17 @@ int z0p_air_verify(const z0p_air_params* params, z0p_transcript* transcript, con
     return guarded([&] { z0p::air_verify(*params, transcript->t, proof, proof_len); });
 }
 
-int z0p_fri_commit_open(z0p_ctx* ctx, const uint32_t* coeffs, size_t n, const z0p_fri_params* params,
-                        z0p_transcript* transcript, uint8_t** commitment, size_t* commitment_len, uint8_t** proof,
-                        size_t* proof_len) {
+namespace {
+int fri_entry(z0p_ctx* ctx, const uint32_t* src, size_t count, int kind, uint64_t degree_bound,
+              const z0p_fri_params* params, z0p_transcript* transcript, uint8_t** commitment, size_t* commitment_len,
+              uint8_t** proof, size_t* proof_len) {
     if (!ctx || !params || !transcript || !commitment || !commitment_len || !proof || !proof_len ||
-        (!coeffs && n))
+        (!src && count))
         return Z0P_ERR_INVALID_ARGUMENT;
     int rc = Z0P_OK;
     int g = guarded([&] {
         std::vector<uint8_t> c, p;
-        z0p::fri_commit_open(*ctx->c, coeffs, n, *params, transcript->t, c, p);
+        z0p::fri_commit_open(*ctx->c, src, count, kind, degree_bound, *params, transcript->t, c, p);
         rc = emit(c, commitment, commitment_len);
         if (rc == Z0P_OK) {
             rc = emit(p, proof, proof_len);
@@ -342,6 +343,35 @@ int z0p_fri_commit_open(z0p_ctx* ctx, const uint32_t* coeffs, size_t n, const z0
         }
     });
     return g != Z0P_OK ? g : rc;
+}
+}  // namespace
+
+int z0p_fri_commit_open(z0p_ctx* ctx, const uint32_t* coeffs, size_t n, const z0p_fri_params* params,
+                        z0p_transcript* transcript, uint8_t** commitment, size_t* commitment_len, uint8_t** proof,
+                        size_t* proof_len) {
+    return fri_entry(ctx, coeffs, n, z0p::FRI_COEFFS, 0, params, transcript, commitment, commitment_len, proof,
+                     proof_len);
+}
+
+int z0p_fri_commit_open_ext(z0p_ctx* ctx, const uint32_t* coeffs4, size_t n, const z0p_fri_params* params,
+                            z0p_transcript* transcript, uint8_t** commitment, size_t* commitment_len, uint8_t** proof,
+                            size_t* proof_len) {
+    return fri_entry(ctx, coeffs4, n, z0p::FRI_COEFFS_EXT, 0, params, transcript, commitment, commitment_len, proof,
+                     proof_len);
+}
+
+int z0p_fri_commit_open_evals(z0p_ctx* ctx, const uint32_t* evals, size_t m, uint64_t degree_bound,
+                              const z0p_fri_params* params, z0p_transcript* transcript, uint8_t** commitment,
+                              size_t* commitment_len, uint8_t** proof, size_t* proof_len) {
+    return fri_entry(ctx, evals, m, z0p::FRI_EVALS, degree_bound, params, transcript, commitment, commitment_len,
+                     proof, proof_len);
+}
+
+int z0p_fri_commit_open_ext_evals(z0p_ctx* ctx, const uint32_t* evals4, size_t m, uint64_t degree_bound,
+                                  const z0p_fri_params* params, z0p_transcript* transcript, uint8_t** commitment,
+                                  size_t* commitment_len, uint8_t** proof, size_t* proof_len) {
+    return fri_entry(ctx, evals4, m, z0p::FRI_EVALS_EXT, degree_bound, params, transcript, commitment, commitment_len,
+                     proof, proof_len);
 }
 
 int z0p_fri_verify(const uint8_t* commitment, size_t commitment_len, const uint8_t* proof, size_t proof_len,

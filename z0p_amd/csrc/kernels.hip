@@ -563,7 +563,17 @@ __global__ __launch_bounds__(WG) void k_bitrev(BitrevJob inl, const BitrevJob* t
     const BitrevJob J = job_of(inl, tab, njobs, local);
     uint64_t i = uint64_t(local) * WG + threadIdx.x;
     if (i >> J.logn) return;
-    J.out[i] = J.in[brev_bits(uint32_t(i), J.logn)] % P;
+    const uint64_t src = uint64_t(brev_bits(uint32_t(i), J.logn)) * uint64_t(J.in_stride);
+    for (int k = 0; k < J.width; ++k) J.out[i * uint64_t(J.width) + k] = gc(J.in)[src + k] % P;
+}
+
+__global__ __launch_bounds__(WG) void k_interleave(InterleaveJob inl, const InterleaveJob* tab, uint32_t njobs) {
+    uint32_t local;
+    const InterleaveJob J = job_of(inl, tab, njobs, local);
+    const uint64_t i = uint64_t(local) * WG + threadIdx.x;
+    if (i >= J.count) return;
+    const gcptr pl = gc(J.planes);
+    st4(J.out, i, u32x4{pl[i], pl[J.count + i], pl[2 * J.count + i], pl[3 * J.count + i]});
 }
 
 // ---------------------------------------------------------------------------
@@ -1397,6 +1407,11 @@ void launch_grind(const Launch& L, std::vector<GrindJob> jobs) {
 void launch_bitrev(const Launch& L, std::vector<BitrevJob> jobs) {
     unsigned g = assign_blocks(jobs, [](const BitrevJob& j) { return grid_for(uint64_t(1) << j.logn, WG); });
     run(L, "bitrev", k_bitrev, jobs, g, 0);
+}
+
+void launch_interleave(const Launch& L, std::vector<InterleaveJob> jobs) {
+    unsigned g = assign_blocks(jobs, [](const InterleaveJob& j) { return grid_for(j.count, WG); });
+    run(L, "interleave", k_interleave, jobs, g, 0);
 }
 
 void launch_collect(const Launch& L, std::vector<CollectItem> items, uint64_t* flag, uint64_t seq,

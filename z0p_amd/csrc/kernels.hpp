@@ -226,13 +226,26 @@ void launch_fold(const Launch& L, std::vector<FoldJob> jobs);
 void launch_grind(const Launch& L, std::vector<GrindJob> jobs);
 // out[i] = in[brev(i)] mod p: raw coefficients to the bit-reversed,
 // canonical order the coset LDE consumes.
+// out[i * width + k] = in[brev(i) * in_stride + k] mod p, i < 2^logn, k < width:
+// base values (1, 1), one coordinate of Ext4 values (4, 1), Ext4 values (4, 4).
 struct BitrevJob {
     const uint32_t* in;
     uint32_t* out;
     int logn;
     uint32_t block_begin;
+    int in_stride = 1;
+    int width = 1;
 };
 void launch_bitrev(const Launch& L, std::vector<BitrevJob> jobs);
+
+// Ext4 values from four coordinate planes: out[4i + c] = planes[c * count + i].
+struct InterleaveJob {
+    const uint32_t* planes;
+    uint32_t* out;
+    uint64_t count;
+    uint32_t block_begin;
+};
+void launch_interleave(const Launch& L, std::vector<InterleaveJob> jobs);
 // Copies of small regions (caps, final layers, nonces of many proofs) into
 // the host-visible staging area, by one workgroup, which then stores `seq`
 // to `flag` (if not null) after a system-scope fence: the host waits on the

@@ -237,6 +237,7 @@ namespace {
 struct Buckets {
     std::vector<FibJob> fib;
     std::vector<BitrevJob> bitrev;
+    std::vector<InterleaveJob> interleave;
     std::vector<NttJob> ntt;
     std::vector<LinkedJob> linked;
     std::vector<ComposeJob> compose;
@@ -250,6 +251,7 @@ struct Buckets {
 
     std::vector<FibJob>& of(const FibJob&) { return fib; }
     std::vector<BitrevJob>& of(const BitrevJob&) { return bitrev; }
+    std::vector<InterleaveJob>& of(const InterleaveJob&) { return interleave; }
     std::vector<NttJob>& of(const NttJob&) { return ntt; }
     std::vector<LinkedJob>& of(const LinkedJob&) { return linked; }
     std::vector<ComposeJob>& of(const ComposeJob&) { return compose; }
@@ -268,6 +270,7 @@ struct Buckets {
         if (!fib.empty()) launch_fib(L, fib);
         if (!bitrev.empty()) launch_bitrev(L, bitrev);
         if (!ntt.empty()) launch_ntt(L, ntt);
+        if (!interleave.empty()) launch_interleave(L, interleave);
         if (!linked.empty()) launch_linked(L, linked);
         if (!compose.empty()) launch_compose(L, compose);
         if (!leaves.empty()) launch_leaves(L, leaves);
@@ -386,7 +389,15 @@ struct Proof {
     z0p_air_params ap{};
     z0p_fri_params fp{};
     Transcript* tr = nullptr;
-    const uint32_t* coeff_src = nullptr;  // FRI: host coefficients
+    const uint32_t* coeff_src = nullptr;  // FRI: host coefficients or LDE values
+    int fri_in = FRI_COEFFS;
+    // Layer 0 holds base-field values (4 bytes each): the FRI commitments
+    // of base coefficients or base LDE values.
+    bool base0() const { return !air && (fri_in == FRI_COEFFS || fri_in == FRI_EVALS); }
+    uint64_t src_words() const {
+        if (air) return n;
+        return (fri_in == FRI_COEFFS_EXT || fri_in == FRI_EVALS_EXT ? 4 : 1) * n;
+    }
     // AIR with both columns interpolated and extended (air.rs:385-406,
     // 439-464) instead of the linked second column.
     bool two_col = false;
@@ -408,6 +419,7 @@ struct Proof {
     size_t stage_need = 0;  // words of staging any one iteration may use
     uint32_t *col = nullptr, *coeff = nullptr, *scratch = nullptr, *A = nullptr, *Bc = nullptr, *trace_tree = nullptr;
     uint32_t *colB = nullptr, *coeffB = nullptr, *mask_dev = nullptr;
+    uint32_t* planar = nullptr;  // FRI_COEFFS_EXT: the four coordinate LDEs
     std::vector<uint32_t*> vals, trees;
     unsigned long long* grind_res = nullptr;
     uint8_t* consts = nullptr;  // per-phase transcript constants (pre-enqueued phases)
@@ -443,8 +455,9 @@ size_t layout(Proof& P, uint8_t* base) {
         off += (size_t(words) * 4 + 255) & ~size_t(255);
         return p;
     };
-    P.col = take(P.n);
-    P.coeff = take(P.n);
+    P.col = take(P.src_words());
+    P.coeff = take(!P.air && P.fri_in == FRI_COEFFS_EXT ? 4 * P.n : P.n);
+    if (!P.air && P.fri_in == FRI_COEFFS_EXT) P.planar = take(4 * P.m);
     if (P.two_col) {
         P.colB = take(P.n);
         P.coeffB = take(P.n);
@@ -459,7 +472,7 @@ size_t layout(Proof& P, uint8_t* base) {
     P.vals.assign(P.folds + 1, nullptr);
     P.trees.assign(P.folds, nullptr);
     for (size_t d = 0; d <= P.folds; ++d) {
-        P.vals[d] = take(d == 0 && !P.air ? P.m : P.lens[d] * 4);
+        P.vals[d] = take(d == 0 && P.base0() ? P.m : P.lens[d] * 4);
         if (d < P.folds) P.trees[d] = take(P.geoms[d].words());
     }
     P.grind_res = reinterpret_cast<unsigned long long*>(take(2));
@@ -478,7 +491,7 @@ void init_fri_shape(Proof& P, const z0p_fri_params& fp) {
         if (d < P.folds) P.geoms.push_back(tree_geom(len / 4, fp.cap_height));
     }
     uint32_t w = 0;
-    for (size_t d = 0; d < P.folds; ++d) w += (d == 0 && !P.air ? 4 : 16) + 8 * uint32_t(P.geoms[d].target);
+    for (size_t d = 0; d < P.folds; ++d) w += (d == 0 && P.base0() ? 4 : 16) + 8 * uint32_t(P.geoms[d].target);
     P.fri_wpq = w;
     if (P.air) w += 32 + 16 * uint32_t(P.tg.target);
     P.wpq = w;
@@ -551,21 +564,34 @@ void init_air_zk(Ctx& c, Proof& P, const z0p_air_params& p, uint64_t n_rand, Cha
     init_air_shape(c, P, p, tr, padded * p.blowup, n + n_rand - 1);
 }
 
-void init_fri(Ctx& c, Proof& P, const uint32_t* coeffs, uint64_t n, const z0p_fri_params& fp, Transcript* tr) {
+// fri::commit / commit_ext (fri.rs:408-461): count coefficients, extended
+// by the blowup. fri::commit_evals / commit_ext_evals (fri.rs:473-549): count
+// LDE values with a declared degree bound. Checks in the reference's order.
+void init_fri(Ctx& c, Proof& P, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,
+              const z0p_fri_params& fp, Transcript* tr) {
     validate_fri(fp);
-    require(n != 0, Z0P_ERR_EMPTY_INPUT);
-    require(pow2(n) && n <= MAX_DOMAIN, Z0P_ERR_INVALID_DOMAIN_SIZE);
-    require(n <= MAX_DOMAIN / fp.blowup, Z0P_ERR_INVALID_DOMAIN_SIZE);
+    require(count != 0, Z0P_ERR_EMPTY_INPUT);
+    require(pow2(count) && count <= MAX_DOMAIN, Z0P_ERR_INVALID_DOMAIN_SIZE);
     P.air = false;
     P.tr = tr;
-    P.coeff_src = coeffs;
-    P.n = n;
-    P.B = fp.blowup;
-    P.m = n * fp.blowup;
-    P.logn = int(ilog2(n));
+    P.coeff_src = src;
+    P.fri_in = kind;
+    P.n = count;
+    if (kind == FRI_COEFFS || kind == FRI_COEFFS_EXT) {
+        require(count <= MAX_DOMAIN / fp.blowup, Z0P_ERR_INVALID_DOMAIN_SIZE);
+        P.B = fp.blowup;
+        P.m = count * fp.blowup;
+        P.poly_len = count;
+        P.degree_bound = count - 1;
+    } else {
+        require(degree_bound < count, Z0P_ERR_MALFORMED_INPUT);
+        P.B = 1;  // the values are layer 0 itself; one "coset" of m points
+        P.m = count;
+        P.poly_len = count;
+        P.degree_bound = degree_bound;
+    }
+    P.logn = int(ilog2(P.n));
     P.logb = int(ilog2(P.B));
-    P.poly_len = n;
-    P.degree_bound = n - 1;
     init_fri_shape(P, fp);
     P.gt = &c.geometry(P.logn, P.logb);
     P.mem_bytes = layout(P, nullptr);
@@ -601,7 +627,7 @@ void start(Ctx& c, Proof& P) {
         store_le64(dims + 8, P.poly_len);
         store_le64(dims + 16, P.degree_bound);
         tr.absorb("fri-dims", dims, 24);
-        hip_check(hipMemcpyAsync(P.col, P.coeff_src, P.n * 4, hipMemcpyHostToDevice, c.stream));
+        hip_check(hipMemcpyAsync(P.col, P.coeff_src, P.src_words() * 4, hipMemcpyHostToDevice, c.stream));
     }
     P.phase = PH_FRONT;
 }
@@ -694,11 +720,44 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 tree_steps(ch, P.trace_tree, P.tg, t, c.sched);
                 collect_cap(c, P, P.trace_tree, P.tg, items, used);
             } else {
-                ch.push_back(BitrevJob{P.col, P.coeff, P.logn, 0});
-                NttPlan lp = plan_lde(P.coeff, P.vals[0], P.scratch, P.logn, P.logb, c.tw4096, gt.w, gt.coset);
-                for (auto& j : lp.pass1) ch.push_back(j);
-                for (auto& j : lp.pass2) ch.push_back(j);
-                tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_BASE, P.vals[0]), c.sched);
+                switch (P.fri_in) {
+                    case FRI_COEFFS: {
+                        ch.push_back(BitrevJob{P.col, P.coeff, P.logn, 0});
+                        NttPlan lp = plan_lde(P.coeff, P.vals[0], P.scratch, P.logn, P.logb, c.tw4096, gt.w, gt.coset);
+                        for (auto& j : lp.pass1) ch.push_back(j);
+                        for (auto& j : lp.pass2) ch.push_back(j);
+                        break;
+                    }
+                    case FRI_COEFFS_EXT: {
+                        // Each coordinate extended over the base field (ntt.rs:394-420),
+                        // then interleaved into Ext4 layer 0.
+                        for (int k = 0; k < 4; ++k) {
+                            BitrevJob b{P.col + k, P.coeff + k * P.n, P.logn, 0};
+                            b.in_stride = 4;
+                            ch.push_back(b);
+                        }
+                        for (int k = 0; k < 4; ++k) {
+                            NttPlan lp = plan_lde(P.coeff + k * P.n, P.planar + k * P.m, P.scratch, P.logn, P.logb,
+                                                  c.tw4096, gt.w, gt.coset);
+                            for (auto& j : lp.pass1) ch.push_back(j);
+                            for (auto& j : lp.pass2) ch.push_back(j);
+                        }
+                        ch.push_back(InterleaveJob{P.planar, P.vals[0], P.m, 0});
+                        break;
+                    }
+                    case FRI_EVALS:
+                        ch.push_back(BitrevJob{P.col, P.vals[0], P.logn, 0});
+                        break;
+                    default: {  // FRI_EVALS_EXT
+                        BitrevJob b{P.col, P.vals[0], P.logn, 0};
+                        b.in_stride = 4;
+                        b.width = 4;
+                        ch.push_back(b);
+                        break;
+                    }
+                }
+                tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(P.base0() ? LEAF_BASE : LEAF_EXT, P.vals[0]),
+                           c.sched);
                 collect_cap(c, P, P.trees[0], P.geoms[0], items, used);
             }
             break;
@@ -732,7 +791,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 sj.tree = P.trees[d + 1];
                 sj.leaves = uint32_t(P.geoms[d + 1].leaves);
                 sj.target = P.geoms[d + 1].target;
-                sj.base_field = d == 0 && !P.air ? 1 : 0;
+                sj.base_field = d == 0 && P.base0() ? 1 : 0;
                 ch.push_back(sj);
                 collect_cap(c, P, P.trees[d + 1], P.geoms[d + 1], items, used);
                 break;
@@ -743,7 +802,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
             fj.finv = gt.finv;
             fj.c = fc;
             fj.log_len = int(ilog2(P.lens[d]));
-            fj.base_field = d == 0 && !P.air ? 1 : 0;
+            fj.base_field = d == 0 && P.base0() ? 1 : 0;
             ch.push_back(fj);
             if (d + 1 == P.folds) {
                 const uint32_t words = uint32_t(P.lens[d + 1] * 4);
@@ -774,7 +833,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 o.tree[d] = P.trees[d];
                 o.tree_leaves[d] = P.geoms[d].leaves;
                 o.target[d] = P.geoms[d].target;
-                o.base[d] = d == 0 && !P.air ? 1 : 0;
+                o.base[d] = d == 0 && P.base0() ? 1 : 0;
             }
             o.nlayers = int(P.folds);
             if (P.air) {
@@ -863,7 +922,7 @@ void write_fri_proof_body(Writer& w, const Proof& P, const uint32_t* words) {
         w.u32(uint32_t(P.folds));
         uint64_t index = P.queries[q];
         for (size_t d = 0; d < P.folds; ++d) {
-            if (d == 0 && !P.air) {
+            if (d == 0 && P.base0()) {
                 for (int k = 0; k < 4; ++k) w.e4(e4_base(p[k]));
                 p += 4;
             } else {
@@ -1263,9 +1322,10 @@ Proof* proof_new_air_zk(Ctx& c, const z0p_air_params& p, uint64_t n_rand, ChaCha
     return P.release();
 }
 
-Proof* proof_new_fri(Ctx& c, const uint32_t* coeffs, uint64_t n, const z0p_fri_params& fp, Transcript* tr) {
+Proof* proof_new_fri(Ctx& c, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,
+                     const z0p_fri_params& fp, Transcript* tr) {
     auto P = std::make_unique<Proof>();
-    init_fri(c, *P, coeffs, n, fp, tr);
+    init_fri(c, *P, src, count, kind, degree_bound, fp, tr);
     return P.release();
 }
 
@@ -1307,9 +1367,10 @@ std::vector<uint8_t> air_prove_zk(Ctx& ctx, const z0p_air_params& p, uint64_t n_
     return run_one(ctx, proof_new_air_zk(ctx, p, n_rand, rng, &tr));
 }
 
-void fri_commit_open(Ctx& ctx, const uint32_t* coeffs, uint64_t n, const z0p_fri_params& fp, Transcript& tr,
-                     std::vector<uint8_t>& commitment, std::vector<uint8_t>& proof) {
-    std::unique_ptr<Proof> P(proof_new_fri(ctx, coeffs, n, fp, &tr));
+void fri_commit_open(Ctx& ctx, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,
+                     const z0p_fri_params& fp, Transcript& tr, std::vector<uint8_t>& commitment,
+                     std::vector<uint8_t>& proof) {
+    std::unique_ptr<Proof> P(proof_new_fri(ctx, src, count, kind, degree_bound, fp, &tr));
     Engine e(ctx, 0, 1);
     e.set_pipelined(true);
     e.submit(P.get());

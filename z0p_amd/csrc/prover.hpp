@@ -37,8 +37,8 @@ TreeGeom tree_geom(uint64_t leaves, uint64_t cap_height);
 
 // One kernel step of one proof; a proof's steps of one iteration form a
 // chain that must run in order.
-using Step = std::variant<FibJob, BitrevJob, NttJob, LinkedJob, ComposeJob, LeavesJob, LevelsJob, TailJob, FoldJob,
-                          SmallCommitJob, GrindJob, OpenJob>;
+using Step = std::variant<FibJob, BitrevJob, InterleaveJob, NttJob, LinkedJob, ComposeJob, LeavesJob, LevelsJob,
+                          TailJob, FoldJob, SmallCommitJob, GrindJob, OpenJob>;
 
 // Leaves, then wide levels, then LDS tail launches up to the cap level.
 void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves, const Schedule& s);
@@ -59,7 +59,15 @@ Proof* proof_new_air(Ctx& c, const z0p_air_params& p, Transcript* tr);
 Proof* proof_new_air_trace(Ctx& c, const z0p_air_params& p, const uint32_t* col_a, uint64_t len_a,
                           const uint32_t* col_b, uint64_t len_b, Transcript* tr);
 Proof* proof_new_air_zk(Ctx& c, const z0p_air_params& p, uint64_t n_rand, ChaCha20& rng, Transcript* tr);
-Proof* proof_new_fri(Ctx& c, const uint32_t* coeffs, uint64_t n, const z0p_fri_params& fp, Transcript* tr);
+// What a FRI commitment starts from (fri.rs:408-549).
+enum FriInput : int {
+    FRI_COEFFS = 0,     // commit: base-field coefficients
+    FRI_COEFFS_EXT,     // commit_ext: Ext4 coefficients (4 words each)
+    FRI_EVALS,          // commit_evals: base-field LDE values, natural order
+    FRI_EVALS_EXT,      // commit_ext_evals: Ext4 LDE values, natural order
+};
+Proof* proof_new_fri(Ctx& c, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,
+                     const z0p_fri_params& fp, Transcript* tr);
 void proof_free(Proof* p);
 int proof_status(const Proof* p);
 std::vector<uint8_t>& proof_bytes(Proof* p);
@@ -125,7 +133,8 @@ std::vector<uint8_t> air_prove(Ctx& ctx, const z0p_air_params& p, Transcript& tr
 std::vector<uint8_t> air_prove_with_trace(Ctx& ctx, const z0p_air_params& p, const uint32_t* col_a, uint64_t len_a,
                                           const uint32_t* col_b, uint64_t len_b, Transcript& tr);
 std::vector<uint8_t> air_prove_zk(Ctx& ctx, const z0p_air_params& p, uint64_t n_rand, ChaCha20& rng, Transcript& tr);
-void fri_commit_open(Ctx& ctx, const uint32_t* coeffs, uint64_t n, const z0p_fri_params& fp, Transcript& tr,
+void fri_commit_open(Ctx& ctx, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,
+                     const z0p_fri_params& fp, Transcript& tr,
                      std::vector<uint8_t>& commitment, std::vector<uint8_t>& proof);
 
 // Host verifier (air.rs:607-752, fri.rs:686-859); throws Status on rejection.

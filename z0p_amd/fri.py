@@ -46,10 +46,43 @@ def commit_open(coeffs, params: Params, transcript, ctx=None):
     return take(cm, cn.value), take(pf, pn.value)
 
 
+def _commit_values(fn, values, params, transcript, ctx, per, degree_bound=None):
+    from . import default_context
+    ctx = ctx or default_context()
+    arr, ptr = u32_buf(values)
+    if arr.size % per:
+        raise ValueError("Ext4 values take 4 words each")
+    cm, pf = u8p(), u8p()
+    cn, pn = ctypes.c_size_t(), ctypes.c_size_t()
+    args = [ctx.handle, ptr, arr.size // per] + ([] if degree_bound is None else [degree_bound])
+    check(fn(*args, ctypes.byref(params), transcript.handle, ctypes.byref(cm), ctypes.byref(cn), ctypes.byref(pf),
+             ctypes.byref(pn)))
+    return take(cm, cn.value), take(pf, pn.value)
+
+
+def commit_open_ext(coeffs4, params: Params, transcript, ctx=None):
+    """fri::commit_ext (fri.rs:444-461) + public_commitment + open on the GPU;
+    Ext4 coefficients as 4 words each, coordinate 0 first."""
+    return _commit_values(lib().z0p_fri_commit_open_ext, coeffs4, params, transcript, ctx, 4)
+
+
+def commit_open_evals(evals, degree_bound, params: Params, transcript, ctx=None):
+    """fri::commit_evals (fri.rs:473-500) + public_commitment + open on the
+    GPU: base-field LDE values in natural order."""
+    return _commit_values(lib().z0p_fri_commit_open_evals, evals, params, transcript, ctx, 1, degree_bound)
+
+
+def commit_open_ext_evals(evals4, degree_bound, params: Params, transcript, ctx=None):
+    """fri::commit_ext_evals (fri.rs:510-549) + public_commitment + open on the
+    GPU: Ext4 LDE values (4 words each) in natural order."""
+    return _commit_values(lib().z0p_fri_commit_open_ext_evals, evals4, params, transcript, ctx, 4, degree_bound)
+
+
 def verify(commitment: bytes, proof: bytes, params: Params, transcript) -> None:
     """Commitment/Proof::from_bytes + fri::verify (fri.rs:686-859); raises Z0pError."""
     check(lib().z0p_fri_verify(buf(commitment), len(commitment), buf(proof), len(proof), ctypes.byref(params),
                                transcript.handle))
 
 
-__all__ = ["Params", "params_new", "for_security", "commit_open", "verify", "Z0pError"]
+__all__ = ["Params", "params_new", "for_security", "commit_open", "commit_open_ext", "commit_open_evals",
+           "commit_open_ext_evals", "verify", "Z0pError"]
