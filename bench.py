@@ -101,6 +101,11 @@ def batched(z, ctx, count, tune_reps, seed=2024):
     got = z.air.prove_batch([p for p, _ in inst], [z.Transcript(l) for _, l in inst], ctx)
     t_batch = time.perf_counter() - t0
     assert all(isinstance(g, bytes) for g in got)
+    # Batched verification of the same proofs on host threads.
+    t0 = time.perf_counter()
+    ok = z.air.verify_batch([p for p, _ in inst], got, [z.Transcript(l) for _, l in inst])
+    t_verify = time.perf_counter() - t0
+    assert all(v is None for v in ok)
     batch_sched = ctx.get_schedule()
     ctx.set_schedule(single_sched)
     t0 = time.perf_counter()
@@ -117,6 +122,8 @@ def batched(z, ctx, count, tune_reps, seed=2024):
         "one_at_a_time_s": round(t_single, 4),
         "one_at_a_time_proofs_per_s": round(count / t_single, 1),
         "speedup_vs_one_at_a_time": round(t_single / t_batch, 2),
+        "verify_proofs_per_s": round(count / t_verify, 1),
+        "verify_threads": os.cpu_count(),
         "autotune": tuned,
         "schedule": batch_sched,
     }

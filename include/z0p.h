@@ -193,6 +193,16 @@ int z0p_ctx_set_hbm_budget(z0p_ctx* ctx, uint64_t bytes);
 int z0p_air_prove_batch(z0p_ctx* ctx, const z0p_air_params* params, z0p_transcript* const* transcripts,
                         size_t count, uint8_t** proofs, size_t* proof_lens, int* statuses);
 
+/* ---- batched verification (no reference counterpart: `count` calls of
+ * air::verify, air.rs:607-752, on host threads) ---- */
+/* Verifies proofs[i] under params[i] and transcripts[i] (each advanced as its
+ * own z0p_air_verify would; null or shared transcripts are rejected with
+ * Z0P_ERR_INVALID_ARGUMENT). statuses[i] receives instance i's status.
+ * threads 0: one per hardware thread. */
+int z0p_air_verify_batch(const z0p_air_params* params, z0p_transcript* const* transcripts,
+                         const uint8_t* const* proofs, const size_t* proof_lens, size_t count, uint32_t threads,
+                         int* statuses);
+
 /* ---- kernel schedule and autotuner (no reference counterpart: launch
  * shapes of this library; proof bytes never depend on them) ---- */
 typedef struct z0p_schedule {

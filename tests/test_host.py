@@ -196,3 +196,38 @@ def test_verifier_accepts_oracle_zk_proofs():
         bad = bytearray(pr)
         bad[len(bad) // 2] ^= 4
         _expect_reject(zp, bytes(bad), b"zk", op)
+
+
+def test_verify_batch_matches_single_verifies():
+    # Batched verification on host threads: each status equals that of its
+    # own verify, and each transcript ends where its own call leaves it.
+    items = []
+    for i, n in enumerate([64, 256, 1024, 256]):
+        a0, b0 = seeded_claims(40 + i)
+        bl = o.fib_b_last(a0, b0, n)
+        proof = o.air_prove(o.air_params_for_security(n, a0, b0, bl), o.Transcript(b"vb%d" % i))
+        items.append((z.air.for_security(n, a0, b0, bl), proof, b"vb%d" % i))
+    zk_p = o.air_params_new(256, 8, 8, 16, 3, 4, o.fib_b_last(3, 4, 256))
+    zk = o.air_prove_zk(zk_p, 128, o.ChaCha20(bytes(32), bytes(12)), o.Transcript(b"vbzk"))
+    items.append((z.air.params_new(256, 8, 8, 16, 3, 4, o.fib_b_last(3, 4, 256)), zk, b"vbzk"))
+    bad = bytearray(items[2][1])
+    bad[len(bad) // 3] ^= 1
+    items.append((items[2][0], bytes(bad), items[2][2]))
+    items.append((items[0][0], items[0][1][:-5], items[0][2]))  # truncated
+    ts = [z.Transcript(lab) for _, _, lab in items]
+    got = z.air.verify_batch([p for p, _, _ in items], [b for _, b, _ in items], ts, threads=3)
+    for (p, b, lab), g, t in zip(items, got, ts):
+        single = z.Transcript(lab)
+        try:
+            z.air.verify(p, b, single)
+            want = None
+        except z.Z0pError as e:
+            want = e.code
+        assert (g.code if g else None) == want
+        assert t.challenge_bytes(b"after") == single.challenge_bytes(b"after")
+    assert got[:5] == [None] * 5 and got[5] is not None and got[6] is not None
+    shared = z.Transcript(b"s")
+    with pytest.raises(z.Z0pError) as e:
+        z.air.verify_batch([items[0][0]] * 2, [items[0][1]] * 2, [shared, shared])
+    assert e.value.name == "InvalidArgument"
+    assert z.air.verify_batch([], [], []) == []

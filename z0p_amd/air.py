@@ -114,5 +114,21 @@ def verify(params: Params, proof: bytes, transcript) -> None:
     check(lib().z0p_air_verify(ctypes.byref(params), transcript.handle, buf(proof), len(proof)))
 
 
+def verify_batch(params_list, proofs, transcripts, threads=0):
+    """air::verify for every (params, proof, transcript) on host threads;
+    returns a list of None (accepted) or Z0pError per instance."""
+    count = len(params_list)
+    if len(proofs) != count or len(transcripts) != count:
+        raise ValueError("one proof and one transcript per instance")
+    ps = (Params * max(count, 1))(*params_list)
+    ts = (ctypes.c_void_p * max(count, 1))(*[t.handle for t in transcripts])
+    keep = [(ctypes.c_uint8 * max(len(b), 1)).from_buffer_copy(b or b"\0") for b in proofs]
+    pp = (u8p * max(count, 1))(*[ctypes.cast(k, u8p) for k in keep])
+    lens = (ctypes.c_size_t * max(count, 1))(*[len(b) for b in proofs])
+    sts = (ctypes.c_int * max(count, 1))()
+    check(lib().z0p_air_verify_batch(ps, ts, pp, lens, count, threads, sts))
+    return [None if sts[i] == 0 else Z0pError(sts[i]) for i in range(count)]
+
+
 __all__ = ["Params", "params_new", "for_security", "b_last", "prove", "prove_with_trace", "prove_zk", "prove_batch",
-           "verify", "Z0pError"]
+           "verify", "verify_batch", "Z0pError"]

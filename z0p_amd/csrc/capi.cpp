@@ -1,9 +1,13 @@
 // extern "C" boundary (include/z0p.h): translate statuses, own outputs.
+#include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
+#include <set>
 #include <string>
+#include <thread>
 
 #include "params.hpp"
 #include "prover.hpp"
@@ -323,6 +327,28 @@ int z0p_air_verify(const z0p_air_params* params, z0p_transcript* transcript, con
                    size_t proof_len) {
     if (!params || !transcript || (!proof && proof_len)) return Z0P_ERR_INVALID_ARGUMENT;
     return guarded([&] { z0p::air_verify(*params, transcript->t, proof, proof_len); });
+}
+
+int z0p_air_verify_batch(const z0p_air_params* params, z0p_transcript* const* transcripts,
+                         const uint8_t* const* proofs, const size_t* proof_lens, size_t count, uint32_t threads,
+                         int* statuses) {
+    if (count && (!params || !transcripts || !proofs || !proof_lens || !statuses)) return Z0P_ERR_INVALID_ARGUMENT;
+    std::set<const z0p_transcript*> seen;
+    for (size_t i = 0; i < count; ++i)
+        if (!transcripts[i] || !seen.insert(transcripts[i]).second || (!proofs[i] && proof_lens[i]))
+            return Z0P_ERR_INVALID_ARGUMENT;
+    size_t workers = threads ? threads : std::max(1u, std::thread::hardware_concurrency());
+    workers = std::min(workers, count);
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+        for (size_t i = next.fetch_add(1); i < count; i = next.fetch_add(1))
+            statuses[i] = guarded([&] { z0p::air_verify(params[i], transcripts[i]->t, proofs[i], proof_lens[i]); });
+    };
+    std::vector<std::thread> pool;
+    for (size_t w = 1; w < workers; ++w) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    return Z0P_OK;
 }
 
 namespace {
