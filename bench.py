@@ -48,7 +48,8 @@ def algorithmic_bytes(n, m, layers):
 # Kernel launch names -> families. Tree level kernels are shared by the
 # trace and FRI trees and are reported as one "merkle" family beside them.
 FAMILY = {
-    "intt": "intt", "lde": "lde", "linked_b": "linked_b", "compose": "compose",
+    "intt": "intt", "lde": "lde", "linked_b": "linked_b", "compose": "compose", "compose_leaves": "compose",
+    "fold_leaves": "fold",
     "trace_leaves": "merkle", "fri_leaves": "merkle", "tree_levels": "merkle", "tree_tail": "merkle",
     "fold": "fold",
 }
@@ -79,7 +80,7 @@ def tune(ctx, params_list, reps):
     t0 = time.perf_counter()
     sched, best, start, log = ctx.autotune(params_list, reps)
     return {"schedule": sched, "ms_tuned": round(best, 4), "ms_default": round(start, 4),
-            "trials": log.count(";"), "tune_s": round(time.perf_counter() - t0, 2)}
+            "trials": log.count(";"), "tune_s": round(time.perf_counter() - t0, 2), "log": log}
 
 
 def batched(z, ctx, count, tune_reps, seed=2024):

@@ -212,6 +212,8 @@ typedef struct z0p_schedule {
     int32_t levels_fuse;        /* wide tree levels per launch (1 or 2) */
     int32_t small_max_log;      /* fused fold + commit for FRI layers of at most 2^k values (0 = off, 4..10) */
     int32_t grind_blocks;       /* proof-of-work workgroups per launch (16..2^20) */
+    int32_t fold_fuse_max_log;  /* fold + next layer's leaves in one kernel for layers of at most 2^k values (0 = off, 3..27) */
+    int32_t compose_fuse;       /* composition + FRI layer 0's leaves in one kernel (0 or 1) */
 } z0p_schedule;
 int z0p_ctx_get_schedule(z0p_ctx* ctx, z0p_schedule* out);
 /* Z0P_ERR_INVALID_ARGUMENT for a value outside the ranges above. */

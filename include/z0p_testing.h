@@ -36,14 +36,20 @@ int z0p_test_linked(z0p_ctx* ctx, const uint32_t* a, size_t n, size_t blowup, ui
                     uint32_t* b);
 
 /* Composition over natural-order columns; out is m Ext4 (4 words each) in
- * natural order (air.rs:847-929). alphas: 5 Ext4 = 20 words. */
+ * natural order (air.rs:847-929). alphas: 5 Ext4 = 20 words. With levels01
+ * non-null, the kernel that also hashes FRI layer 0 runs instead and
+ * levels01 receives that layer's quad-leaf digests (m/4) then their parents
+ * (m/8), 32 bytes each. */
 int z0p_test_compose(z0p_ctx* ctx, const uint32_t* a, const uint32_t* b, size_t n, size_t blowup, uint32_t a0,
-                     uint32_t b0, uint32_t b_last, const uint32_t* alphas, uint32_t* out);
+                     uint32_t b0, uint32_t b_last, const uint32_t* alphas, uint32_t* out, uint8_t* levels01);
 
 /* One arity-4 fold of a bit-reversed Ext4 layer of length len at fold depth
- * `depth` of an LDE of size lde_len; challenges: 3 Ext4 = 12 words. */
+ * `depth` of an LDE of size lde_len; challenges: 3 Ext4 = 12 words; out: the
+ * len/4 folded values. With levels01 non-null (len >= 32), the kernel that
+ * also hashes the folded layer runs instead and levels01 receives its quad-
+ * leaf digests (len/16) then their parents (len/32), 32 bytes each. */
 int z0p_test_fold(z0p_ctx* ctx, const uint32_t* layer, size_t len, size_t lde_len, size_t depth,
-                  const uint32_t* challenges, uint32_t* out);
+                  const uint32_t* challenges, uint32_t* out, uint8_t* levels01);
 
 /* Proof-of-work: smallest nonce whose BLAKE3(head || u64 nonce) has at least
  * `bits` leading zero bits; head must be 47 bytes. */

@@ -99,8 +99,33 @@ def test_composition(ctx, logn, logb):
     alphas = rand_field(rng, 20)
     out = np.zeros(4 * m, dtype=np.uint32)
     print("compose logn", logn, flush=True)
-    _check(_ffi.lib().z0p_test_compose(ctx.handle, _ptr(a), _ptr(b), n, B, a0, b0, bl, _ptr(alphas), _ptr(out)))
-    np.testing.assert_array_equal(out.reshape(-1, 4), o.compose(a, b, n, a0, b0, bl, alphas))
+    _check(_ffi.lib().z0p_test_compose(ctx.handle, _ptr(a), _ptr(b), n, B, a0, b0, bl, _ptr(alphas), _ptr(out),
+                                       None))
+    want = o.compose(a, b, n, a0, b0, bl, alphas)
+    np.testing.assert_array_equal(out.reshape(-1, 4), want)
+
+
+@pytest.mark.parametrize("logn,logb", [(1, 2), (2, 3), (6, 4), (12, 4), (13, 3), (16, 4)])
+def test_composition_with_layer0_leaves(ctx, logn, logb):
+    # The fused kernel: values as above, plus FRI layer 0's quad leaves and
+    # their parents, against the oracle tree over the bit-reversed layer.
+    rng = np.random.default_rng(650 + logn)
+    n, B = 1 << logn, 1 << logb
+    m = n * B
+    a = rand_field(rng, m)
+    b = rand_field(rng, m)
+    a0, b0, bl = (int(x) for x in rng.integers(0, P, size=3))
+    alphas = rand_field(rng, 20)
+    out = np.zeros(4 * m, dtype=np.uint32)
+    lv = np.zeros(32 * (m // 4 + m // 8), dtype=np.uint8)
+    print("compose+leaves logn", logn, "logb", logb, flush=True)
+    _check(_ffi.lib().z0p_test_compose(ctx.handle, _ptr(a), _ptr(b), n, B, a0, b0, bl, _ptr(alphas), _ptr(out),
+                                       lv.ctypes.data_as(_ffi.u8p)))
+    want = np.asarray(o.compose(a, b, n, a0, b0, bl, alphas), dtype=np.uint32)
+    np.testing.assert_array_equal(out.reshape(-1, 4), want)
+    layer = want[brev_perm(logn + logb)]  # FRI layer 0: bit-reversed order
+    tree = o.merkle_flat64(layer.astype("<u4").tobytes())
+    assert lv.tobytes() == tree[:32 * (m // 4 + m // 8)]
 
 
 @pytest.mark.parametrize("loglde,depth", [(4, 0), (8, 1), (14, 0), (14, 3), (18, 2)])
@@ -112,7 +137,7 @@ def test_fold(ctx, loglde, depth):
     ch = rand_field(rng, 12)
     out = np.zeros(length, dtype=np.uint32)
     print("fold lde", lde, "depth", depth, flush=True)
-    _check(_ffi.lib().z0p_test_fold(ctx.handle, _ptr(layer), length, lde, depth, _ptr(ch), _ptr(out)))
+    _check(_ffi.lib().z0p_test_fold(ctx.handle, _ptr(layer), length, lde, depth, _ptr(ch), _ptr(out), None))
     unit = o.two_adic_generator(2)
     quads = layer.reshape(-1, 16)
     got = out.reshape(-1, 4)
@@ -120,6 +145,35 @@ def test_fold(ctx, loglde, depth):
     for i in idx:
         inv = o.field_inv(o.point_at(lde, depth, 4 * int(i)))
         np.testing.assert_array_equal(got[i], o.fold_values(quads[i], inv, ch, unit), err_msg="quad %d" % i)
+
+
+@pytest.mark.parametrize("loglde,depth", [(5, 0), (8, 1), (14, 0), (16, 2), (20, 0)])
+def test_fold_with_next_leaves(ctx, loglde, depth):
+    # The fused kernel: every folded value against the oracle fold, and the
+    # folded layer's quad leaves and parents against the oracle tree.
+    rng = np.random.default_rng(750 + loglde + depth)
+    lde = 1 << loglde
+    length = lde >> (2 * depth)
+    layer = rand_field(rng, 4 * length)
+    ch = rand_field(rng, 12)
+    out = np.zeros(length, dtype=np.uint32)
+    leaves = length // 16
+    lv = np.zeros(32 * (leaves + leaves // 2), dtype=np.uint8)
+    print("fold+leaves lde", lde, "depth", depth, flush=True)
+    _check(_ffi.lib().z0p_test_fold(ctx.handle, _ptr(layer), length, lde, depth, _ptr(ch), _ptr(out),
+                                    lv.ctypes.data_as(_ffi.u8p)))
+    _check(_ffi.lib().z0p_test_fold(ctx.handle, _ptr(layer), length, lde, depth, _ptr(ch), _ptr(plain := np.zeros(
+        length, dtype=np.uint32)), None))
+    np.testing.assert_array_equal(out, plain)
+    unit = o.two_adic_generator(2)
+    quads = layer.reshape(-1, 16)
+    got = out.reshape(-1, 4)
+    idx = list(range(min(32, quads.shape[0]))) + list(rng.integers(0, quads.shape[0], size=32))
+    for i in idx:
+        inv = o.field_inv(o.point_at(lde, depth, 4 * int(i)))
+        np.testing.assert_array_equal(got[i], o.fold_values(quads[i], inv, ch, unit), err_msg="quad %d" % i)
+    tree = o.merkle_flat64(out.astype("<u4").tobytes())
+    assert lv.tobytes() == tree[:len(lv)]
 
 
 @pytest.mark.parametrize("bits", [0, 1, 8, 14, 18])

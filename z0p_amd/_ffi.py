@@ -59,7 +59,8 @@ class FriParamsC(ctypes.Structure):
 
 class ScheduleC(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int32) for name in (
-        "tail_max_log", "tail_group_max_log", "tail_spread_log", "levels_fuse", "small_max_log", "grind_blocks")]
+        "tail_max_log", "tail_group_max_log", "tail_spread_log", "levels_fuse", "small_max_log", "grind_blocks",
+        "fold_fuse_max_log", "compose_fuse")]
 
     def to_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -139,8 +140,8 @@ SIGNATURES = {
     "z0p_test_merkle64": (ctypes.c_int, [vp, u8p, sz, u8p]),
     "z0p_test_trace_tree": (ctypes.c_int, [vp, u32p, u32p, sz, sz, u8p]),
     "z0p_test_linked": (ctypes.c_int, [vp, u32p, sz, sz, u32, u32, u32p]),
-    "z0p_test_compose": (ctypes.c_int, [vp, u32p, u32p, sz, sz, u32, u32, u32, u32p, u32p]),
-    "z0p_test_fold": (ctypes.c_int, [vp, u32p, sz, sz, sz, u32p, u32p]),
+    "z0p_test_compose": (ctypes.c_int, [vp, u32p, u32p, sz, sz, u32, u32, u32, u32p, u32p, u8p]),
+    "z0p_test_fold": (ctypes.c_int, [vp, u32p, sz, sz, sz, u32p, u32p, u8p]),
     "z0p_test_grind": (ctypes.c_int, [vp, u8p, sz, u32, ctypes.POINTER(u64)]),
     "z0p_test_copy": (ctypes.c_int, [vp, sz, ctypes.POINTER(ctypes.c_float)]),
 }

@@ -224,6 +224,8 @@ struct Schedule {
     int levels_fuse = 2;         // wide levels per launch (1 or 2)
     int small_max_log = 10;      // fold + commit in one workgroup for layers of at most 2^k values (0: off)
     int grind_blocks = 4096;     // grind workgroups per launch, shared by the proofs grinding
+    int fold_fuse_max_log = 0;   // fold + next layer's leaves in one kernel for layers of at most 2^k values (0: off)
+    int compose_fuse = 1;        // composition + FRI layer 0's leaves in one kernel (0: separate kernels)
 
     // Digests per tail workgroup (log2) for a level of `count` digests.
     int tail_group_log(uint64_t count) const {
@@ -238,7 +240,9 @@ struct Schedule {
         return tail_max_log >= TAIL_GROUP_MIN_LOG && tail_max_log <= 24 && tail_group_max_log >= TAIL_GROUP_MIN_LOG &&
                tail_group_max_log <= TAIL_GROUP_MAX_LOG && tail_spread_log >= 0 && tail_spread_log <= 16 &&
                (levels_fuse == 1 || levels_fuse == 2) && small_max_log >= 0 && small_max_log <= 10 &&
-               (small_max_log == 0 || small_max_log >= 4) && grind_blocks >= 16 && grind_blocks <= (1 << 20);
+               (small_max_log == 0 || small_max_log >= 4) && grind_blocks >= 16 && grind_blocks <= (1 << 20) &&
+               (fold_fuse_max_log == 0 || (fold_fuse_max_log >= 3 && fold_fuse_max_log <= 27)) &&
+               (compose_fuse == 0 || compose_fuse == 1);
     }
 };
 

@@ -801,6 +801,99 @@ __global__ __launch_bounds__(WG) void k_levels(LevelsJob inl, const LevelsJob* t
     store_digest(next + g * 8, p);
 }
 
+// Layer values staged in LDS for hashing: 2048 Ext4 values per workgroup,
+// one padding word per 8 values so that a thread's 8 consecutive values (two
+// quad leaves, 32 words) start 33 words after its neighbour's: conflict-free
+// across lanes.
+constexpr int FUSE_VALUES = 8 * WG;
+constexpr int FUSE_LDS = FUSE_VALUES * 4 + FUSE_VALUES / 8;
+__device__ __forceinline__ int fuse_at(int q, int k) { return q * 4 + k + (q >> 3); }
+
+// Thread t hashes staged values 8t..8t+7 (leaves 2t, 2t+1 of the workgroup's
+// first leaf `leaf0`) and, with_parent, their parent into the next level.
+__device__ __forceinline__ void hash_staged(const uint32_t* lds, uint32_t* tree, uint64_t leaves, uint64_t leaf0,
+                                            int with_parent, int nvals) {
+    const int t = threadIdx.x;
+    if (8 * t >= nvals) return;
+    uint32_t m0[16], m1[16];
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+        m0[w] = lds[fuse_at(8 * t + (w >> 2), w & 3)];
+        m1[w] = lds[fuse_at(8 * t + 4 + (w >> 2), w & 3)];
+    }
+    uint32_t dg0[8], dg1[8];
+    leaf64_words(m0, dg0);
+    leaf64_words(m1, dg1);
+    const uint64_t leaf = leaf0 + 2 * uint64_t(t);
+    store_digest(tree + leaf * 8, dg0);
+    store_digest(tree + (leaf + 1) * 8, dg1);
+    if (with_parent) {
+        uint32_t par[8];
+        node_words(dg0, dg1, par);
+        store_digest(tree + (leaves + leaf / 2) * 8, par);
+    }
+}
+
+// Composition fused with layer 0's Merkle leaves: the workgroup computes
+// 2048 consecutive layer positions as k_compose does (coalesced), stages
+// them in LDS, then hashes them as 512 quad leaves (and their parents).
+__global__ __launch_bounds__(WG) void k_compose_leaves(ComposeJob inl, const ComposeJob* tab, uint32_t njobs) {
+    __shared__ uint32_t lds[FUSE_LDS];
+    uint32_t local;
+    const ComposeJob J = job_of(inl, tab, njobs, local);
+    const uint32_t m = 1u << (J.logn + J.logb);
+    const uint32_t first = local * FUSE_VALUES;
+    const uint32_t base = first + threadIdx.x;
+    const gcptr A = gc(J.a), Bv = gc(J.b);
+    const ComposeConsts C = *J.c;
+    const uint32_t wl1 = sub(C.w_last_m, R1);
+    uint32_t d1[CELT], pr[CELT], na[CELT], nb[CELT];
+#pragma unroll
+    for (int e = 0; e < CELT; ++e) {
+        const uint32_t p = base + e * WG;
+        if (p < m) {
+            const PosInfo pi = pos_info(p, J.logn, J.logb);
+            na[e] = A[pi.pnext];
+            nb[e] = Bv[pi.pnext];
+            const uint32_t x = mont_mul(gc(J.xn)[p & ((1u << J.logn) - 1)], gc(J.shift_m)[pi.j]);
+            d1[e] = sub(x, R1);
+            pr[e] = mont_mul(d1[e], sub(d1[e], wl1));
+        } else {
+            na[e] = nb[e] = 0;
+            d1[e] = pr[e] = R1;
+        }
+    }
+    batch_inv_m<CELT>(pr);
+    static_for<0, CELT>([&](auto ec) {
+        constexpr int e = decltype(ec)::value;
+        const uint32_t p = base + e * WG;
+        if (p >= m) return;
+        const uint32_t j = brev_bits(p >> J.logn, J.logb);
+        const uint32_t dw = sub(d1[e], wl1);
+        const uint32_t i1 = mont_mul(dw, pr[e]);
+        const uint32_t iw = mont_mul(d1[e], pr[e]);
+        const uint32_t iz = mont_mul(dw, gc(J.zinv_m)[j]);
+        const uint32_t va = A[p], vb = Bv[p];
+        const uint32_t s0 = mont_mul(sub(na[e], vb), iz);
+        const uint32_t s1 = mont_mul(sub(nb[e], add(va, vb)), iz);
+        const uint32_t s2 = mont_mul(sub(va, C.a0), i1);
+        const uint32_t s3 = mont_mul(sub(vb, C.b0), i1);
+        const uint32_t s4 = mont_mul(sub(vb, C.b_last), iw);
+        uint32_t o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t p01 = mont_mul2(s0, C.alpha_m[0][k], s1, C.alpha_m[1][k]);
+            const uint32_t p23 = mont_mul2(s2, C.alpha_m[2][k], s3, C.alpha_m[3][k]);
+            o[k] = add(add(p01, p23), mont_mul(s4, C.alpha_m[4][k]));
+            lds[fuse_at(int(threadIdx.x) + e * WG, k)] = o[k];
+        }
+        st4(J.layer0, p, u32x4{o[0], o[1], o[2], o[3]});
+    });
+    __syncthreads();
+    const int nvals = int(m - first < uint32_t(FUSE_VALUES) ? m - first : uint32_t(FUSE_VALUES));
+    hash_staged(lds, J.tree, m >> 2, first >> 2, J.with_parent, nvals);
+}
+
 // ---- quad-lane BLAKE3 for the narrow tree levels ------------------------
 // Four lanes of a quad compute one compression: lane q holds state column q
 // (v[q], v[4+q], v[8+q], v[12+q]) and runs the column G on it; for the
@@ -990,6 +1083,31 @@ __global__ __launch_bounds__(WG) void k_fold(FoldJob inl, const FoldJob* tab, ui
     if (i >= quads) return;
     const E4 out = fold_one(J.layer, J.base_field, J.finv, *J.c, i);
     st4(J.next, i, u32x4{out.v[0], out.v[1], out.v[2], out.v[3]});
+}
+
+// Fold fused with the next layer's Merkle leaves: the workgroup folds 2048
+// consecutive outputs as k_fold does (coalesced quads), stages them in LDS,
+// then hashes them as 512 quad leaves of layer d+1 (and their parents).
+__global__ __launch_bounds__(WG) void k_fold_leaves(FoldJob inl, const FoldJob* tab, uint32_t njobs) {
+    __shared__ uint32_t lds[FUSE_LDS];
+    uint32_t local;
+    const FoldJob J = job_of(inl, tab, njobs, local);
+    const uint64_t outs = uint64_t(1) << (J.log_len - 2);
+    const uint64_t first = uint64_t(local) * FUSE_VALUES;
+    const FoldConsts C = *J.c;
+#pragma unroll 2
+    for (int e = 0; e < 8; ++e) {
+        const uint64_t i = first + threadIdx.x + uint64_t(e) * WG;
+        if (i < outs) {
+            const E4 v = fold_one(J.layer, J.base_field, J.finv, C, i);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) lds[fuse_at(int(threadIdx.x) + e * WG, k)] = v.v[k];
+            st4(J.next, i, u32x4{v.v[0], v.v[1], v.v[2], v.v[3]});
+        }
+    }
+    __syncthreads();
+    const int nvals = int(outs - first < uint64_t(FUSE_VALUES) ? outs - first : uint64_t(FUSE_VALUES));
+    hash_staged(lds, J.tree, outs >> 2, first >> 2, J.with_parent, nvals);
 }
 
 // ---------------------------------------------------------------------------
@@ -1337,11 +1455,21 @@ void launch_linked(const Launch& L, std::vector<LinkedJob> jobs) {
     run(L, "linked_b", k_linked, jobs, g, 0);
 }
 
-void launch_compose(const Launch& L, std::vector<ComposeJob> jobs) {
-    unsigned g = assign_blocks(jobs, [](const ComposeJob& j) {
-        return grid_for(uint64_t(1) << (j.logn + j.logb), WG * CELT);
-    });
-    run(L, "compose", k_compose, jobs, g, 0);
+void launch_compose(const Launch& L, std::vector<ComposeJob> all) {
+    std::vector<ComposeJob> plain, fused;
+    for (const ComposeJob& j : all) (j.tree ? fused : plain).push_back(j);
+    if (!plain.empty()) {
+        unsigned g = assign_blocks(plain, [](const ComposeJob& j) {
+            return grid_for(uint64_t(1) << (j.logn + j.logb), WG * CELT);
+        });
+        run(L, "compose", k_compose, plain, g, 0);
+    }
+    if (!fused.empty()) {
+        unsigned g = assign_blocks(fused, [](const ComposeJob& j) {
+            return grid_for(uint64_t(1) << (j.logn + j.logb), FUSE_VALUES);
+        });
+        run(L, "compose_leaves", k_compose_leaves, fused, g, 0);
+    }
 }
 
 void launch_leaves(const Launch& L, std::vector<LeavesJob> jobs) {
@@ -1384,9 +1512,19 @@ void launch_small_commit(const Launch& L, std::vector<SmallCommitJob> jobs) {
     hipLaunchKernelGGL(k_small_commit, dim3(g), dim3(SMALL_WG), 0, L.stream, jobs[0], table, n);
 }
 
-void launch_fold(const Launch& L, std::vector<FoldJob> jobs) {
-    unsigned g = assign_blocks(jobs, [](const FoldJob& j) { return grid_for(uint64_t(1) << (j.log_len - 2), WG); });
-    run(L, "fold", k_fold, jobs, g, 0);
+void launch_fold(const Launch& L, std::vector<FoldJob> all) {
+    std::vector<FoldJob> plain, fused;
+    for (const FoldJob& j : all) (j.tree ? fused : plain).push_back(j);
+    if (!plain.empty()) {
+        unsigned g = assign_blocks(plain, [](const FoldJob& j) { return grid_for(uint64_t(1) << (j.log_len - 2), WG); });
+        run(L, "fold", k_fold, plain, g, 0);
+    }
+    if (!fused.empty()) {
+        unsigned g = assign_blocks(fused, [](const FoldJob& j) {
+            return grid_for(uint64_t(1) << (j.log_len - 2), FUSE_VALUES);
+        });
+        run(L, "fold_leaves", k_fold_leaves, fused, g, 0);
+    }
 }
 
 void launch_grind(const Launch& L, std::vector<GrindJob> jobs) {

@@ -117,6 +117,11 @@ struct ComposeJob {
     const ComposeConsts* c;  // device memory: the alphas come from the transcript
     int logn, logb;
     uint32_t block_begin;
+    // Non-null: also hash layer 0's quad leaves into level 0 of this tree
+    // (and their parents into level 1 when with_parent), one leaf pair per
+    // thread, so the values are hashed while still in registers.
+    uint32_t* tree = nullptr;
+    int with_parent = 0;
 };
 
 // Merkle leaves. Kinds: trace rows (8 joint A||B rows per leaf from the
@@ -186,6 +191,10 @@ struct FoldJob {
     int log_len;          // = c->log_len, for the grid
     int base_field;
     uint32_t block_begin;
+    // Non-null: also hash layer d+1's quad leaves into level 0 of this tree
+    // (parents into level 1 when with_parent), eight outputs per thread.
+    uint32_t* tree = nullptr;
+    int with_parent = 0;
 };
 
 // Fold of layer d into a small layer d+1 plus its whole Merkle commitment

@@ -208,12 +208,7 @@ TreeGeom tree_geom(uint64_t leaves, uint64_t cap_height) {
     return g;
 }
 
-void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves, const Schedule& s) {
-    leaves.levels = tree;
-    leaves.leaves = g.leaves;
-    leaves.with_parent = g.target >= 1 ? 1 : 0;
-    chain.push_back(leaves);
-    int have = leaves.with_parent;
+void tree_levels(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, int have, const Schedule& s) {
     while (have < g.target) {
         const uint64_t count = g.leaves >> have;
         const int remaining = g.target - have;
@@ -229,6 +224,14 @@ void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, Lea
             have += k;
         }
     }
+}
+
+void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves, const Schedule& s) {
+    leaves.levels = tree;
+    leaves.leaves = g.leaves;
+    leaves.with_parent = g.target >= 1 ? 1 : 0;
+    chain.push_back(leaves);
+    tree_levels(chain, tree, g, leaves.with_parent, s);
 }
 
 namespace {
@@ -773,8 +776,16 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
             cj.c = static_cast<const ComposeConsts*>(cst ? cst : c.ring.push(&P.cc, sizeof(P.cc)));
             cj.logn = P.logn;
             cj.logb = P.logb;
-            ch.push_back(cj);
-            tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_EXT, P.vals[0]), c.sched);
+            if (c.sched.compose_fuse) {
+                // Layer 0's leaves (and their parents) hashed by the composition kernel.
+                cj.tree = P.trees[0];
+                cj.with_parent = P.geoms[0].target >= 1 ? 1 : 0;
+                ch.push_back(cj);
+                tree_levels(ch, P.trees[0], P.geoms[0], cj.with_parent, c.sched);
+            } else {
+                ch.push_back(cj);
+                tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_EXT, P.vals[0]), c.sched);
+            }
             collect_cap(c, P, P.trees[0], P.geoms[0], items, used);
             break;
         }
@@ -803,12 +814,20 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
             fj.c = fc;
             fj.log_len = int(ilog2(P.lens[d]));
             fj.base_field = d == 0 && P.base0() ? 1 : 0;
-            ch.push_back(fj);
             if (d + 1 == P.folds) {
+                ch.push_back(fj);
                 const uint32_t words = uint32_t(P.lens[d + 1] * 4);
                 uint32_t* dst = stage_take(c, used, words, P.stage_off);
                 items.push_back(CollectItem{P.vals[d + 1], dst, words, 0});
+            } else if (c.sched.fold_fuse_max_log > 0 && P.lens[d + 1] <= (uint64_t(1) << c.sched.fold_fuse_max_log)) {
+                // Layer d+1's leaves (and parents) hashed by the fold kernel.
+                fj.tree = P.trees[d + 1];
+                fj.with_parent = P.geoms[d + 1].target >= 1 ? 1 : 0;
+                ch.push_back(fj);
+                tree_levels(ch, P.trees[d + 1], P.geoms[d + 1], fj.with_parent, c.sched);
+                collect_cap(c, P, P.trees[d + 1], P.geoms[d + 1], items, used);
             } else {
+                ch.push_back(fj);
                 tree_steps(ch, P.trees[d + 1], P.geoms[d + 1], leaves_of(LEAF_EXT, P.vals[d + 1]), c.sched);
                 collect_cap(c, P, P.trees[d + 1], P.geoms[d + 1], items, used);
             }

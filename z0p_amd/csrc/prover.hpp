@@ -42,6 +42,8 @@ using Step = std::variant<FibJob, BitrevJob, InterleaveJob, NttJob, LinkedJob, C
 
 // Leaves, then wide levels, then LDS tail launches up to the cap level.
 void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves, const Schedule& s);
+// The tree levels above `have` (levels 0..have already written) up to the cap.
+void tree_levels(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, int have, const Schedule& s);
 
 // Launches step r of every chain for r = 0, 1, ... (one launch per kernel
 // type per r), then the collect items, which end by storing `seq` to

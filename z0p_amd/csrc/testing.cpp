@@ -186,7 +186,7 @@ int z0p_test_linked(z0p_ctx* ctx, const uint32_t* a, size_t n, size_t blowup, ui
 }
 
 int z0p_test_compose(z0p_ctx* ctx, const uint32_t* a, const uint32_t* b, size_t n, size_t blowup, uint32_t a0,
-                     uint32_t b0, uint32_t b_last, const uint32_t* alphas, uint32_t* out) {
+                     uint32_t b0, uint32_t b_last, const uint32_t* alphas, uint32_t* out, uint8_t* levels01) {
     if (!ctx || !a || !b || !alphas || !out) return Z0P_ERR_INVALID_ARGUMENT;
     return guarded([&] {
         check_geometry(n, blowup);
@@ -207,7 +207,7 @@ int z0p_test_compose(z0p_ctx* ctx, const uint32_t* a, const uint32_t* b, size_t 
         cc.a0 = a0 % P;
         cc.b0 = b0 % P;
         cc.b_last = b_last % P;
-        c.reserve_scratch(m * 24 + (1 << 20));
+        c.reserve_scratch(m * 36 + (1 << 20));
         uint32_t* da = to_dev(c, abr.data(), m);
         uint32_t* db = to_dev(c, bbr.data(), m);
         uint32_t* dl = c.scratch_alloc<uint32_t>(4 * m);
@@ -221,14 +221,21 @@ int z0p_test_compose(z0p_ctx* ctx, const uint32_t* a, const uint32_t* b, size_t 
         cj.c = to_dev(c, &cc, 1);
         cj.logn = logn;
         cj.logb = logb;
+        uint32_t* tree = nullptr;
+        if (levels01) {
+            tree = c.scratch_alloc<uint32_t>(8 * (m / 4 + m / 8));
+            cj.tree = tree;
+            cj.with_parent = 1;
+        }
         run_chain(c, {cj});
         to_host(c, o.data(), dl, 4 * m);
         for (size_t i = 0; i < m; ++i) std::memcpy(out + 4 * i, &o[4 * br_pos(i, logn, logb)], 16);
+        if (levels01) to_host(c, reinterpret_cast<uint32_t*>(levels01), tree, 8 * (m / 4 + m / 8));
     });
 }
 
 int z0p_test_fold(z0p_ctx* ctx, const uint32_t* layer, size_t len, size_t lde_len, size_t depth,
-                  const uint32_t* challenges, uint32_t* out) {
+                  const uint32_t* challenges, uint32_t* out, uint8_t* levels01) {
     if (!ctx || !layer || !challenges || !out) return Z0P_ERR_INVALID_ARGUMENT;
     return guarded([&] {
         require(pow2(len) && len >= 4 && pow2(lde_len) && lde_len <= MAX_DOMAIN && (len << (2 * depth)) == lde_len,
@@ -241,7 +248,7 @@ int z0p_test_fold(z0p_ctx* ctx, const uint32_t* layer, size_t len, size_t lde_le
         for (int k = 0; k < 3; ++k)
             for (int q = 0; q < 4; ++q) r[k].v[q] = challenges[4 * k + q] % P;
         const FoldConsts fc = make_fold_consts(r, int(ilog2(len)), int(depth));
-        c.reserve_scratch(len * 20 + (1 << 20));
+        c.reserve_scratch(len * 24 + (1 << 20));
         uint32_t* dl = to_dev(c, layer, 4 * len);
         uint32_t* dn = c.scratch_alloc<uint32_t>(len);
         FoldJob fj{};
@@ -251,8 +258,17 @@ int z0p_test_fold(z0p_ctx* ctx, const uint32_t* layer, size_t len, size_t lde_le
         fj.c = to_dev(c, &fc, 1);
         fj.log_len = fc.log_len;
         fj.base_field = 0;
+        uint32_t* tree = nullptr;
+        const size_t leaves = len / 16;  // quads of the len/4 outputs
+        if (levels01) {
+            require(len >= 32, Z0P_ERR_INVALID_DOMAIN_SIZE);
+            tree = c.scratch_alloc<uint32_t>(8 * (leaves + leaves / 2));
+            fj.tree = tree;
+            fj.with_parent = 1;
+        }
         run_chain(c, {fj});
         to_host(c, out, dn, len);
+        if (levels01) to_host(c, reinterpret_cast<uint32_t*>(levels01), tree, 8 * (leaves + leaves / 2));
     });
 }
 

@@ -70,6 +70,8 @@ TuneResult autotune(Ctx& c, const std::vector<z0p_air_params>& work, unsigned re
         {"levels_fuse", &Schedule::levels_fuse, {1, 2}},
         {"small_max_log", &Schedule::small_max_log, {0, 8, 10}},
         {"grind_blocks", &Schedule::grind_blocks, {1024, 2048, 4096, 8192}},
+        {"fold_fuse_max_log", &Schedule::fold_fuse_max_log, {0, 12, 14, 16, 18, 22}},
+        {"compose_fuse", &Schedule::compose_fuse, {0, 1}},
     };
     // A candidate must beat the incumbent by this fraction to replace it, so
     // run-to-run noise does not move the schedule.
