@@ -231,3 +231,41 @@ def test_verify_batch_matches_single_verifies():
         z.air.verify_batch([items[0][0]] * 2, [items[0][1]] * 2, [shared, shared])
     assert e.value.name == "InvalidArgument"
     assert z.air.verify_batch([], [], []) == []
+
+
+def test_fibonacci_interpolant_closed_form():
+    # The device writes the Fibonacci columns' coefficients in closed form
+    # (Binet: A[i] = c1 phi^i + c2 psi^i, phi, psi the roots of x^2 = x + 1,
+    # which lie in BabyBear since p = 1 mod 5; each coefficient a geometric
+    # sum). The identity must equal the oracle's INTT of build_trace
+    # (air.rs:341-372) coefficient for coefficient, for A and for B.
+    # sqrt(5) by Tonelli-Shanks (p - 1 = 2^27 * 15)
+    q, s = P - 1, 0
+    while q % 2 == 0:
+        q, s = q // 2, s + 1
+    z = 2
+    while pow(z, (P - 1) // 2, P) != P - 1:
+        z += 1
+    m, c, t, r5 = s, pow(z, q, P), pow(5, q, P), pow(5, (q + 1) // 2, P)
+    while t != 1:
+        i, t2 = 0, t
+        while t2 != 1:
+            t2, i = t2 * t2 % P, i + 1
+        b = pow(c, 1 << (m - i - 1), P)
+        m, c, t, r5 = i, b * b % P, t * b * b % P, r5 * b % P
+    assert r5 * r5 % P == 5
+    inv = lambda x: pow(x, P - 2, P)
+    phi, psi = (1 + r5) * inv(2) % P, (1 - r5) * inv(2) % P
+    assert pow(phi, 1 << 27, P) != 1 and pow(psi, 1 << 27, P) != 1  # no vanishing denominator
+    for n, (a0, b0) in [(2, (1, 1)), (8, seeded_claims(1)), (1024, seeded_claims(2))]:
+        cols = o.build_trace(a0, b0, n)
+        c1 = (a0 * inv(phi) + b0) * inv(r5) % P
+        c2 = -(a0 * inv(psi) + b0) * inv(r5) % P
+        winv = inv(o.two_adic_generator(n.bit_length() - 1))
+        for col, (d1, d2) in zip(cols, [(c1, c2), (c1 * phi % P, c2 * psi % P)]):
+            want = o.ntt_inverse([int(v) for v in col])
+            k1 = d1 * (1 - pow(phi, n, P)) * inv(n) % P
+            k2 = d2 * (1 - pow(psi, n, P)) * inv(n) % P
+            got = [(k1 * inv((1 - phi * pow(winv, k, P)) % P) + k2 * inv((1 - psi * pow(winv, k, P)) % P)) % P
+                   for k in range(n)]
+            assert got == [int(v) for v in want]

@@ -25,36 +25,32 @@ using namespace dev;
 namespace {
 
 // ---------------------------------------------------------------------------
-// Fibonacci trace column A: A[i] = F(i-1) a0 + F(i) b0 (F(-1) = 1, F(0) = 0).
-// Each thread jumps to its chunk start with fast doubling, then iterates.
-constexpr int FIB_CHUNK = 64;
+// Fibonacci column coefficients in closed form (see FibCoeffJob): eight
+// positions per thread, strided by the workgroup (coalesced stores), with one
+// batched inversion of their 16 denominators.
+constexpr int FIB_ELT = 8;
 
-__global__ __launch_bounds__(WG) void k_fib(FibJob inl, const FibJob* tab, uint32_t njobs) {
+__global__ __launch_bounds__(WG) void k_fib_coeffs(FibCoeffJob inl, const FibCoeffJob* tab, uint32_t njobs) {
     uint32_t local;
-    const FibJob J = job_of(inl, tab, njobs, local);
-    uint64_t start = (uint64_t(local) * WG + threadIdx.x) * FIB_CHUNK;
-    if (start >= J.n) return;
-    uint32_t f = 0, g = R1;  // (F(k), F(k+1)) in Montgomery form
-    for (int b = 63; b >= 0; --b) {
-        uint32_t f2 = mont_mul(f, sub(dbl(g), f));
-        uint32_t g2 = add(mont_mul(f, f), mont_mul(g, g));
-        if ((start >> b) & 1) {
-            f = g2;
-            g = add(f2, g2);
-        } else {
-            f = f2;
-            g = g2;
-        }
+    const FibCoeffJob J = job_of(inl, tab, njobs, local);
+    const uint32_t n = 1u << J.logn;
+    const uint32_t base = local * (WG * FIB_ELT) + threadIdx.x;
+    uint32_t den[2 * FIB_ELT];
+#pragma unroll
+    for (int e = 0; e < FIB_ELT; ++e) {
+        const uint32_t P = base + e * WG;
+        const uint32_t k = P < n ? brev_bits(P, J.logn) : 0u;
+        const uint32_t w = pow_lookup(J.winv, k << J.logb);
+        den[2 * e] = sub(R1, mont_mul(J.phi_m, w));
+        den[2 * e + 1] = sub(R1, mont_mul(J.psi_m, w));
     }
-    uint32_t fm1 = sub(g, f);
-    uint32_t a = add(mont_mul(J.a0, fm1), mont_mul(J.b0, f));
-    uint32_t bb = add(mont_mul(J.a0, f), mont_mul(J.b0, g));
-    uint64_t end = start + FIB_CHUNK < J.n ? start + FIB_CHUNK : J.n;
-    for (uint64_t i = start; i < end; ++i) {
-        J.col[i] = a;
-        uint32_t nb = add(a, bb);
-        a = bb;
-        bb = nb;
+    batch_inv_m<2 * FIB_ELT>(den);
+#pragma unroll
+    for (int e = 0; e < FIB_ELT; ++e) {
+        const uint32_t P = base + e * WG;
+        if (P >= n) break;
+        const uint32_t c = mont_mul2(J.k1_m, den[2 * e], J.k2_m, den[2 * e + 1]);
+        gm(J.out)[P] = mont_mul(c, 1u);  // out of Montgomery form
     }
 }
 
@@ -1432,9 +1428,9 @@ NttPlan plan_lde(const uint32_t* coeffs_br, uint32_t* lde_br, uint32_t* scratch,
     return plan;
 }
 
-void launch_fib(const Launch& L, std::vector<FibJob> jobs) {
-    unsigned g = assign_blocks(jobs, [](const FibJob& j) { return grid_for((j.n + FIB_CHUNK - 1) / FIB_CHUNK, WG); });
-    run(L, "fib_trace", k_fib, jobs, g, 0);
+void launch_fib_coeffs(const Launch& L, std::vector<FibCoeffJob> jobs) {
+    unsigned g = assign_blocks(jobs, [](const FibCoeffJob& j) { return grid_for(uint64_t(1) << j.logn, WG * FIB_ELT); });
+    run(L, "fib_coeffs", k_fib_coeffs, jobs, g, 0);
 }
 
 void launch_ntt(const Launch& L, std::vector<NttJob> jobs) {

@@ -57,10 +57,17 @@ struct NttShape {
 NttShape ntt_shape(int logn);
 
 // ---- descriptors ----------------------------------------------------------
-struct FibJob {
-    uint32_t* col;
-    uint64_t n;
-    uint32_t a0, b0;
+// Interpolant coefficients of a column c1 phi^i + c2 psi^i (i < n), phi and
+// psi the roots of x^2 = x + 1: coefficient k is
+// k1 / (1 - phi w^-k) + k2 / (1 - psi w^-k), with k1 = c1 (1 - phi^n) / n,
+// k2 = c2 (1 - psi^n) / n and w = omega_n (a geometric sum per root). Written
+// bit-reversed, as the inverse NTT writes them. Requires phi^n != 1 and
+// psi^n != 1 (no denominator vanishes).
+struct FibCoeffJob {
+    uint32_t* out;
+    PowTable winv;  // omega_m^-e; omega_n^-k = omega_m^-(k << logb)
+    uint32_t phi_m, psi_m, k1_m, k2_m;
+    int logn, logb;
     uint32_t block_begin;
 };
 
@@ -223,7 +230,7 @@ struct GrindJob {
 };
 
 // ---- launchers ---------------------------------------------------------------
-void launch_fib(const Launch& L, std::vector<FibJob> jobs);
+void launch_fib_coeffs(const Launch& L, std::vector<FibCoeffJob> jobs);
 void launch_ntt(const Launch& L, std::vector<NttJob> jobs);
 void launch_linked(const Launch& L, std::vector<LinkedJob> jobs);
 void launch_compose(const Launch& L, std::vector<ComposeJob> jobs);

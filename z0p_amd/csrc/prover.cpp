@@ -199,6 +199,66 @@ Ctx* ctx_create(int device) {
     return c.release();
 }
 
+// Binet form of the Fibonacci AIR columns. p = 1 mod 5, so 5 is a square and
+// x^2 = x + 1 has the roots phi, psi = (1 +- sqrt 5)/2 in the field; with
+// F(k) = (phi^k - psi^k)/sqrt 5 (F(-1) = 1, as phi psi = -1),
+// A[i] = F(i-1) a0 + F(i) b0 = c1 phi^i + c2 psi^i for
+// c1 = (a0/phi + b0)/sqrt 5 and c2 = -(a0/psi + b0)/sqrt 5.
+struct FibRoots {
+    uint32_t phi = 0, psi = 0, inv_s5 = 0;
+    FibRoots() {
+        // Tonelli-Shanks square root of 5: p - 1 = 2^27 * 15.
+        const uint32_t q = (P - 1) >> TWO_ADICITY;
+        uint32_t z = 2;
+        while (pow_c(z, (P - 1) / 2) != P - 1) ++z;
+        uint32_t m = TWO_ADICITY, c = pow_c(z, q), t = pow_c(5, q), r = pow_c(5, (q + 1) / 2);
+        while (t != 1) {
+            uint32_t i = 0, t2 = t;
+            while (t2 != 1) t2 = mul(t2, t2), ++i;
+            const uint32_t b = pow_c(c, uint64_t(1) << (m - i - 1));
+            m = i, c = mul(b, b), t = mul(t, c), r = mul(r, b);
+        }
+        const uint32_t half = inv_c(2);
+        phi = mul(add(1, r), half);
+        psi = mul(sub(1, r), half);
+        inv_s5 = inv_c(r);
+    }
+};
+
+// The closed-form coefficient job for the column c1 phi^i + c2 psi^i, i < n.
+FibCoeffJob fib_coeff_job(uint32_t* out, const GeometryTables& gt, int logn, int logb, uint32_t c1, uint32_t c2) {
+    static const FibRoots F;
+    const uint64_t n = uint64_t(1) << logn;
+    const uint32_t phin = pow_c(F.phi, n), psin = pow_c(F.psi, n);
+    // phi and psi are no 2^k-th roots of unity (phi^(2^27) != 1), so no
+    // denominator 1 - phi omega^-k vanishes for n <= 2^27.
+    require(phin != 1 && psin != 1, Z0P_ERR_DEVICE);
+    const uint32_t ninv = inv_c(uint32_t(n % P));
+    FibCoeffJob j{};
+    j.out = out;
+    j.winv = gt.winv;
+    j.phi_m = to_mont(F.phi);
+    j.psi_m = to_mont(F.psi);
+    j.k1_m = to_mont(mul(mul(c1, sub(1, phin)), ninv));
+    j.k2_m = to_mont(mul(mul(c2, sub(1, psin)), ninv));
+    j.logn = logn;
+    j.logb = logb;
+    return j;
+}
+
+// (c1, c2) of column A for claims (a0, b0); column B[i] = A[i+1] is
+// (c1 phi, c2 psi).
+std::pair<uint32_t, uint32_t> fib_binet(uint32_t a0, uint32_t b0) {
+    static const FibRoots F;
+    const uint32_t c1 = mul(add(mul(a0, inv_c(F.phi)), b0), F.inv_s5);
+    const uint32_t c2 = sub(0, mul(add(mul(a0, inv_c(F.psi)), b0), F.inv_s5));
+    return {c1, c2};
+}
+std::pair<uint32_t, uint32_t> fib_binet_next(std::pair<uint32_t, uint32_t> c) {
+    static const FibRoots F;
+    return {mul(c.first, F.phi), mul(c.second, F.psi)};
+}
+
 TreeGeom tree_geom(uint64_t leaves, uint64_t cap_height) {
     TreeGeom g;
     g.leaves = leaves;
@@ -238,7 +298,7 @@ namespace {
 
 // Per-round buckets: step r of every chain, grouped by kernel type.
 struct Buckets {
-    std::vector<FibJob> fib;
+    std::vector<FibCoeffJob> fib;
     std::vector<BitrevJob> bitrev;
     std::vector<InterleaveJob> interleave;
     std::vector<NttJob> ntt;
@@ -252,7 +312,7 @@ struct Buckets {
     std::vector<GrindJob> grind;
     std::vector<OpenJob> open;
 
-    std::vector<FibJob>& of(const FibJob&) { return fib; }
+    std::vector<FibCoeffJob>& of(const FibCoeffJob&) { return fib; }
     std::vector<BitrevJob>& of(const BitrevJob&) { return bitrev; }
     std::vector<InterleaveJob>& of(const InterleaveJob&) { return interleave; }
     std::vector<NttJob>& of(const NttJob&) { return ntt; }
@@ -270,7 +330,7 @@ struct Buckets {
         std::visit([this](const auto& j) { of(j).push_back(j); }, s);
     }
     void launch(const Launch& L) const {
-        if (!fib.empty()) launch_fib(L, fib);
+        if (!fib.empty()) launch_fib_coeffs(L, fib);
         if (!bitrev.empty()) launch_bitrev(L, bitrev);
         if (!ntt.empty()) launch_ntt(L, ntt);
         if (!interleave.empty()) launch_interleave(L, interleave);
@@ -666,16 +726,19 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
         case PH_FRONT: {
             if (P.air && P.two_col) {
                 if (!P.host_a) {
-                    // B[i] = A[i+1]: the same recurrence started at (b0, a0 + b0).
-                    ch.push_back(FibJob{P.col, P.n, P.a0, P.b0, 0});
-                    ch.push_back(FibJob{P.colB, P.n, P.b0, add(P.a0, P.b0), 0});
-                }
-                const uint32_t ninv = to_mont(inv_c(uint32_t(P.n % P_MOD)));
-                for (int k = 0; k < 2; ++k) {
-                    NttPlan ip = plan_intt(k ? P.colB : P.col, k ? P.coeffB : P.coeff, P.scratch, P.logn,
-                                           c.tw4096_inv, gt.winv, P.logb, ninv);
-                    for (auto& j : ip.pass1) ch.push_back(j);
-                    for (auto& j : ip.pass2) ch.push_back(j);
+                    // Both Fibonacci columns' coefficients in closed form.
+                    const auto ca = fib_binet(P.a0, P.b0);
+                    const auto cb = fib_binet_next(ca);
+                    ch.push_back(fib_coeff_job(P.coeff, gt, P.logn, P.logb, ca.first, ca.second));
+                    ch.push_back(fib_coeff_job(P.coeffB, gt, P.logn, P.logb, cb.first, cb.second));
+                } else {
+                    const uint32_t ninv = to_mont(inv_c(uint32_t(P.n % P_MOD)));
+                    for (int k = 0; k < 2; ++k) {
+                        NttPlan ip = plan_intt(k ? P.colB : P.col, k ? P.coeffB : P.coeff, P.scratch, P.logn,
+                                               c.tw4096_inv, gt.winv, P.logb, ninv);
+                        for (auto& j : ip.pass1) ch.push_back(j);
+                        for (auto& j : ip.pass2) ch.push_back(j);
+                    }
                 }
                 for (int k = 0; k < 2; ++k) {
                     NttPlan lp = plan_lde(k ? P.coeffB : P.coeff, k ? P.Bc : P.A, P.scratch, P.logn, P.logb,
@@ -697,11 +760,9 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 tree_steps(ch, P.trace_tree, P.tg, t, c.sched);
                 collect_cap(c, P, P.trace_tree, P.tg, items, used);
             } else if (P.air) {
-                ch.push_back(FibJob{P.col, P.n, P.a0, P.b0, 0});
-                NttPlan ip = plan_intt(P.col, P.coeff, P.scratch, P.logn, c.tw4096_inv, gt.winv, P.logb,
-                                       to_mont(inv_c(uint32_t(P.n % P_MOD))));
-                for (auto& j : ip.pass1) ch.push_back(j);
-                for (auto& j : ip.pass2) ch.push_back(j);
+                // interpolate(build_trace(a0, b0, n).0) (air.rs:368-369) in closed form.
+                const auto ca = fib_binet(P.a0, P.b0);
+                ch.push_back(fib_coeff_job(P.coeff, gt, P.logn, P.logb, ca.first, ca.second));
                 NttPlan lp = plan_lde(P.coeff, P.A, P.scratch, P.logn, P.logb, c.tw4096, gt.w, gt.coset);
                 for (auto& j : lp.pass1) ch.push_back(j);
                 for (auto& j : lp.pass2) ch.push_back(j);

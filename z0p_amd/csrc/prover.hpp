@@ -37,7 +37,7 @@ TreeGeom tree_geom(uint64_t leaves, uint64_t cap_height);
 
 // One kernel step of one proof; a proof's steps of one iteration form a
 // chain that must run in order.
-using Step = std::variant<FibJob, BitrevJob, InterleaveJob, NttJob, LinkedJob, ComposeJob, LeavesJob, LevelsJob,
+using Step = std::variant<FibCoeffJob, BitrevJob, InterleaveJob, NttJob, LinkedJob, ComposeJob, LeavesJob, LevelsJob,
                           TailJob, FoldJob, SmallCommitJob, GrindJob, OpenJob>;
 
 // Leaves, then wide levels, then LDS tail launches up to the cap level.
