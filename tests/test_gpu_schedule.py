@@ -14,7 +14,7 @@ from z0p_amd import _ffi
 pytestmark = pytest.mark.gpu
 
 DEFAULT = dict(tail_max_log=17, tail_group_max_log=8, tail_spread_log=0, levels_fuse=2, small_max_log=10,
-               grind_blocks=4096, fold_fuse_max_log=0, compose_fuse=1)
+               grind_blocks=4096, fold_fuse_max_log=0, compose_fuse=1, trace_fuse=1)
 
 SCHEDULES = [
     dict(DEFAULT, tail_group_max_log=9, tail_spread_log=8),
@@ -23,7 +23,7 @@ SCHEDULES = [
     dict(DEFAULT, tail_max_log=4, levels_fuse=2, small_max_log=0),
     dict(DEFAULT, tail_max_log=20, tail_group_max_log=8, small_max_log=4, grind_blocks=16),
     dict(DEFAULT, small_max_log=8, grind_blocks=1 << 20),
-    dict(DEFAULT, fold_fuse_max_log=16, compose_fuse=0),
+    dict(DEFAULT, fold_fuse_max_log=16, compose_fuse=0, trace_fuse=0),
     dict(DEFAULT, fold_fuse_max_log=27, small_max_log=0),
 ]
 
@@ -85,7 +85,8 @@ def test_merkle_levels_under_tail_shapes(sched_ctx, leaves, tail):
 def test_set_schedule_rejects_out_of_range(sched_ctx):
     for bad in [dict(DEFAULT, tail_group_max_log=10), dict(DEFAULT, levels_fuse=3), dict(DEFAULT, small_max_log=11),
                 dict(DEFAULT, small_max_log=2), dict(DEFAULT, grind_blocks=8), dict(DEFAULT, tail_max_log=1),
-                dict(DEFAULT, fold_fuse_max_log=2), dict(DEFAULT, fold_fuse_max_log=28), dict(DEFAULT, compose_fuse=2)]:
+                dict(DEFAULT, fold_fuse_max_log=2), dict(DEFAULT, fold_fuse_max_log=28), dict(DEFAULT, compose_fuse=2),
+                dict(DEFAULT, trace_fuse=-1)]:
         with pytest.raises(z.Z0pError) as e:
             sched_ctx.set_schedule(bad)
         assert e.value.name == "InvalidArgument"

@@ -226,6 +226,7 @@ struct Schedule {
     int grind_blocks = 4096;     // grind workgroups per launch, shared by the proofs grinding
     int fold_fuse_max_log = 0;   // fold + next layer's leaves in one kernel for layers of at most 2^k values (0: off)
     int compose_fuse = 1;        // composition + FRI layer 0's leaves in one kernel (0: separate kernels)
+    int trace_fuse = 1;          // linked column + trace leaves in one kernel (0: separate kernels)
 
     // Digests per tail workgroup (log2) for a level of `count` digests.
     int tail_group_log(uint64_t count) const {
@@ -242,7 +243,7 @@ struct Schedule {
                (levels_fuse == 1 || levels_fuse == 2) && small_max_log >= 0 && small_max_log <= 10 &&
                (small_max_log == 0 || small_max_log >= 4) && grind_blocks >= 16 && grind_blocks <= (1 << 20) &&
                (fold_fuse_max_log == 0 || (fold_fuse_max_log >= 3 && fold_fuse_max_log <= 27)) &&
-               (compose_fuse == 0 || compose_fuse == 1);
+               (compose_fuse == 0 || compose_fuse == 1) && (trace_fuse == 0 || trace_fuse == 1);
     }
 };
 

@@ -750,11 +750,64 @@ __device__ __forceinline__ void leaves_body(const LeavesJob& J, uint64_t tau) {
     }
 }
 
+// Trace leaves with the linked column computed in the same pass: a thread's
+// two leaves are 16 natural rows i = j + B t (scattered over the coset
+// blocks); for each it loads A at the row and at the next row (same coset,
+// t + 1), forms B with one 16-way batched inversion of x - w_last, stores B
+// and hashes the rows as A||B words.
+__device__ __forceinline__ void leaves_linked(const LeavesJob& J, uint64_t tau) {
+    const uint64_t pairs = J.leaves >> 1;
+    if (tau >= pairs) return;
+    const uint64_t g = trace_pair_of(J, tau);
+    const int logn = J.logn, logb = J.logb;
+    const uint32_t bmask = (1u << logb) - 1, nmask = (1u << logn) - 1;
+    const gcptr A = gc(J.src);
+    uint32_t a[16], an[16], den[16], pos[16], jj[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const uint64_t i = g * 16 + r;
+        const uint32_t j = uint32_t(i) & bmask;
+        const uint32_t t = uint32_t(i >> logb);
+        const uint32_t blk = brev_bits(j, logb) << logn;
+        pos[r] = blk | brev_bits(t, logn);
+        jj[r] = j;
+        a[r] = A[pos[r]];
+        an[r] = A[blk | brev_bits((t + 1) & nmask, logn)];
+        const uint32_t x = mont_mul(gc(J.xn)[pos[r] & nmask], gc(J.shift_m)[j]);
+        den[r] = sub(x, J.w_last_m);
+    }
+    batch_inv_m<16>(den);
+    uint32_t m0[16], m1[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const uint32_t b = add(an[r], mont_mul(den[r], mont_mul(J.coeff, gc(J.zvan_m)[jj[r]])));
+        gm(J.b_out)[pos[r]] = b;
+        if (r < 8) {
+            m0[2 * r] = a[r];
+            m0[2 * r + 1] = b;
+        } else {
+            m1[2 * (r - 8)] = a[r];
+            m1[2 * (r - 8) + 1] = b;
+        }
+    }
+    uint32_t d0[8], d1[8];
+    leaf64_words(m0, d0);
+    leaf64_words(m1, d1);
+    store_digest(J.levels + (2 * g) * 8, d0);
+    store_digest(J.levels + (2 * g + 1) * 8, d1);
+    if (J.with_parent) {
+        uint32_t p[8];
+        node_words(d0, d1, p);
+        store_digest(J.levels + (J.leaves + g) * 8, p);
+    }
+}
+
 __global__ __launch_bounds__(WG) void k_leaves(LeavesJob inl, const LeavesJob* tab, uint32_t njobs) {
     uint32_t local;
     const LeavesJob J = job_of(inl, tab, njobs, local);
     const uint64_t tau = uint64_t(local) * WG + threadIdx.x;
-    if (J.kind == LEAF_TRACE) leaves_body<LEAF_TRACE>(J, tau);
+    if (J.kind == LEAF_TRACE_LINKED) leaves_linked(J, tau);
+    else if (J.kind == LEAF_TRACE) leaves_body<LEAF_TRACE>(J, tau);
     else if (J.kind == LEAF_BASE) leaves_body<LEAF_BASE>(J, tau);
     else leaves_body<LEAF_EXT>(J, tau);
 }
@@ -1470,7 +1523,7 @@ void launch_compose(const Launch& L, std::vector<ComposeJob> all) {
 
 void launch_leaves(const Launch& L, std::vector<LeavesJob> jobs) {
     bool trace = false;
-    for (const LeavesJob& j : jobs) trace = trace || j.kind == LEAF_TRACE;
+    for (const LeavesJob& j : jobs) trace = trace || j.kind == LEAF_TRACE || j.kind == LEAF_TRACE_LINKED;
     unsigned g = assign_blocks(jobs, [](const LeavesJob& j) {
         return j.leaves == 1 ? 1u : grid_for(j.leaves / 2, WG);
     });

@@ -134,7 +134,9 @@ struct ComposeJob {
 // Merkle leaves. Kinds: trace rows (8 joint A||B rows per leaf from the
 // bit-reversed columns), FRI Ext4 quads, FRI base-field quads (upper
 // coordinates zero), raw 64-byte leaves.
-enum LeafKind : int { LEAF_TRACE = 0, LEAF_EXT = 1, LEAF_BASE = 2, LEAF_RAW = 3 };
+// LEAF_TRACE_LINKED: as LEAF_TRACE, but column B is computed from A in the
+// same pass (the linked column, air.rs:806-844) and written to b_out.
+enum LeafKind : int { LEAF_TRACE = 0, LEAF_EXT = 1, LEAF_BASE = 2, LEAF_RAW = 3, LEAF_TRACE_LINKED = 4 };
 struct LeavesJob {
     const uint32_t* src;
     const uint32_t* src_b;  // trace column B
@@ -144,6 +146,13 @@ struct LeavesJob {
     int with_parent;        // also write level 1 (needs leaves >= 2)
     int logn, logb;         // trace geometry
     uint32_t block_begin;
+    // LEAF_TRACE_LINKED (leaves >= 2): B = A[next row] + coeff (x^n - 1)/(x - w_last).
+    uint32_t* b_out = nullptr;
+    const uint32_t* xn = nullptr;       // [n] omega_n^brev(q), Montgomery
+    const uint32_t* shift_m = nullptr;  // [B] s_j, Montgomery
+    const uint32_t* zvan_m = nullptr;   // [B] s_j^n - 1, Montgomery
+    uint32_t coeff = 0;                 // (b_last - a0) w_last / n, canonical
+    uint32_t w_last_m = 0;
 };
 
 // One or two parent levels from a level of `count` digests, one node (or

@@ -766,21 +766,32 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 NttPlan lp = plan_lde(P.coeff, P.A, P.scratch, P.logn, P.logb, c.tw4096, gt.w, gt.coset);
                 for (auto& j : lp.pass1) ch.push_back(j);
                 for (auto& j : lp.pass2) ch.push_back(j);
-                LinkedJob lj{};
-                lj.a = P.A;
-                lj.b = P.Bc;
-                lj.xn = gt.xn;
-                lj.shift_m = gt.coset.shift;
-                lj.zvan_m = gt.zvan;
-                lj.coeff = P.coeff_c;
-                lj.w_last_m = to_mont(P.w_last);
-                lj.logn = P.logn;
-                lj.logb = P.logb;
-                ch.push_back(lj);
                 LeavesJob t = leaves_of(LEAF_TRACE, P.A);
                 t.src_b = P.Bc;
                 t.logn = P.logn;
                 t.logb = P.logb;
+                if (c.sched.trace_fuse && P.tg.leaves >= 2) {
+                    // The linked column computed by the trace-leaves kernel.
+                    t.kind = LEAF_TRACE_LINKED;
+                    t.b_out = P.Bc;
+                    t.xn = gt.xn;
+                    t.shift_m = gt.coset.shift;
+                    t.zvan_m = gt.zvan;
+                    t.coeff = P.coeff_c;
+                    t.w_last_m = to_mont(P.w_last);
+                } else {
+                    LinkedJob lj{};
+                    lj.a = P.A;
+                    lj.b = P.Bc;
+                    lj.xn = gt.xn;
+                    lj.shift_m = gt.coset.shift;
+                    lj.zvan_m = gt.zvan;
+                    lj.coeff = P.coeff_c;
+                    lj.w_last_m = to_mont(P.w_last);
+                    lj.logn = P.logn;
+                    lj.logb = P.logb;
+                    ch.push_back(lj);
+                }
                 tree_steps(ch, P.trace_tree, P.tg, t, c.sched);
                 collect_cap(c, P, P.trace_tree, P.tg, items, used);
             } else {

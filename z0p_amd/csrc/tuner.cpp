@@ -72,6 +72,7 @@ TuneResult autotune(Ctx& c, const std::vector<z0p_air_params>& work, unsigned re
         {"grind_blocks", &Schedule::grind_blocks, {1024, 2048, 4096, 8192}},
         {"fold_fuse_max_log", &Schedule::fold_fuse_max_log, {0, 12, 14, 16, 18, 22}},
         {"compose_fuse", &Schedule::compose_fuse, {0, 1}},
+        {"trace_fuse", &Schedule::trace_fuse, {0, 1}},
     };
     // A candidate must beat the incumbent by this fraction to replace it, so
     // run-to-run noise does not move the schedule.
