@@ -432,12 +432,12 @@ void write_cap(Writer& w, const std::vector<Digest>& cap) {
 // Authentication path from `count` digests of 8 words; one direction byte
 // per step, 1 when the opened node is the left child (merkle.rs path order).
 const uint32_t* write_path(Writer& w, const uint32_t* p, int count, uint64_t index) {
+    static_assert(__BYTE_ORDER__ == __ORDER_LITTLE_ENDIAN__, "digest words are stored little-endian");
     w.u32(uint32_t(count));
-    for (int l = 0; l < count; ++l) {
-        uint8_t d[32];
-        digest_words_to_bytes(p + 8 * l, d);
-        w.raw(d, 32);
-        w.byte(((index >> l) & 1) == 0 ? 1 : 0);
+    uint8_t* o = w.room(33 * size_t(count));
+    for (int l = 0; l < count; ++l, o += 33) {
+        std::memcpy(o, p + 8 * l, 32);
+        o[32] = ((index >> l) & 1) == 0 ? 1 : 0;
     }
     return p + 8 * count;
 }
@@ -1017,7 +1017,7 @@ void write_fri_proof_body(Writer& w, const Proof& P, const uint32_t* words) {
                 for (int k = 0; k < 4; ++k) w.e4(e4_base(p[k]));
                 p += 4;
             } else {
-                for (int k = 0; k < 16; ++k) w.u32(p[k]);
+                w.words(p, 16);
                 p += 16;
             }
             p = write_path(w, p, P.geoms[d].target, index / 4);
@@ -1029,11 +1029,11 @@ void write_fri_proof_body(Writer& w, const Proof& P, const uint32_t* words) {
 // Proof::to_bytes (air.rs:262-281) / the FRI commitment and proof.
 void serialize(Proof& P, const uint32_t* words) {
     // Opened words dominate the size: 4 bytes each plus a direction byte
-    // per path digest; caps and headers add a few kilobytes.
-    const size_t estimate = size_t(P.wpq) * P.queries.size() * 5 + 65536;
+    // per path digest (one per 8 words); caps and headers add a few kilobytes.
+    const size_t estimate = size_t(P.wpq) * P.queries.size() * 33 / 8 + 32768;
     if (P.air) {
         Writer w;
-        w.out.reserve(estimate);
+        w.reserve(estimate);
         w.byte(WIRE_VERSION);
         write_cap(w, P.column_cap);
         write_fri_commitment_body(w, P);
@@ -1054,20 +1054,20 @@ void serialize(Proof& P, const uint32_t* words) {
             w.u32(sw[2 * (shifted & 7) + 1]);
             paths = write_path(w, paths, P.tg.target, index / 8);
             write_path(w, paths, P.tg.target, shifted / 8);
-            for (int k = 0; k < 16; ++k) w.u32(lw[k]);
-            for (int k = 0; k < 16; ++k) w.u32(sw[k]);
+            w.words(lw, 16);
+            w.words(sw, 16);
         }
-        P.proof = std::move(w.out);
+        P.proof = std::move(w.finish());
     } else {
         Writer wc;
         wc.byte(WIRE_VERSION);
         write_fri_commitment_body(wc, P);
-        P.commitment = std::move(wc.out);
+        P.commitment = std::move(wc.finish());
         Writer wp;
-        wp.out.reserve(estimate);
+        wp.reserve(estimate);
         wp.byte(WIRE_VERSION);
         write_fri_proof_body(wp, P, words);
-        P.proof = std::move(wp.out);
+        P.proof = std::move(wp.finish());
     }
 }
 

@@ -120,25 +120,36 @@ constexpr uint32_t MAX_PATH_STEPS = 128;
 constexpr uint32_t MAX_CAP_DIGESTS = 256;
 constexpr uint32_t MAX_ELEMS = 1u << 27;
 
+// Appends through a cursor into a buffer sized up front (reserve() with the
+// expected total); finish() trims it to the bytes written.
 struct Writer {
     std::vector<uint8_t> out;
-    void byte(uint8_t b) { out.push_back(b); }
-    void u32(uint32_t x) {
-        uint8_t b[4];
-        store_le32(b, x);
-        out.insert(out.end(), b, b + 4);
+    size_t pos = 0;
+    void reserve(size_t bytes) {
+        if (out.size() < bytes) out.resize(bytes);
     }
-    void u64(uint64_t x) {
-        uint8_t b[8];
-        store_le64(b, x);
-        out.insert(out.end(), b, b + 8);
+    uint8_t* room(size_t k) {
+        if (pos + k > out.size()) out.resize(std::max(2 * out.size(), pos + k));
+        uint8_t* p = out.data() + pos;
+        pos += k;
+        return p;
     }
-    void raw(const void* p, size_t n) {
-        const uint8_t* q = static_cast<const uint8_t*>(p);
-        out.insert(out.end(), q, q + n);
-    }
+    void byte(uint8_t b) { *room(1) = b; }
+    void u32(uint32_t x) { store_le32(room(4), x); }
+    void u64(uint64_t x) { store_le64(room(8), x); }
+    void raw(const void* p, size_t n) { std::memcpy(room(n), p, n); }
     void e4(const E4& e) {
-        for (int i = 0; i < 4; ++i) u32(e.v[i]);
+        uint8_t* p = room(16);
+        for (int i = 0; i < 4; ++i) store_le32(p + 4 * i, e.v[i]);
+    }
+    // `count` u32 words as little-endian bytes (the host is little-endian).
+    void words(const uint32_t* w, size_t count) {
+        static_assert(__BYTE_ORDER__ == __ORDER_LITTLE_ENDIAN__, "little-endian host");
+        std::memcpy(room(4 * count), w, 4 * count);
+    }
+    std::vector<uint8_t>& finish() {
+        out.resize(pos);
+        return out;
     }
 };
 
