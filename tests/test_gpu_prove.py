@@ -133,3 +133,26 @@ def test_air_2_20_byte_identical(ctx):
     """The headline instance (a0 = b0 = 1, N = 2^20): 203,553 bytes, identical to the oracle."""
     proof = _air_case(1 << 20, 1, 1, b"bench/air-scale", ctx)
     assert len(proof) == 203553
+
+
+@pytest.mark.parametrize("kind", ["air", "fri"])
+def test_grind_beyond_first_window(ctx, kind):
+    # 26-bit proof-of-work: the smallest nonce usually lies beyond the first
+    # window of 2^24 nonces, so the openings enqueued ahead are cancelled and
+    # the search continues window by window; bytes must not change.
+    if kind == "air":
+        n = 64
+        bl = o.fib_b_last(2, 3, n)
+        zp = z.air.params_new(n, 8, 6, 8, 2, 3, bl)
+        op = o.air_params_new(n, 8, 6, 8, 2, 3, bl)
+        zp.grinding_bits = op.grinding_bits = 26
+        print("air grind 26", flush=True)
+        got = z.air.prove(zp, z.Transcript(b"g26a"), ctx)  # smallest nonce 103,769,303 > 2^24
+        assert got == o.air_prove(op, o.Transcript(b"g26a"))
+    else:
+        coeffs = fri_bench_coeffs(256)
+        zp = z.fri.params_new(4, 6, 4, 26, 1)
+        op = o.fri_params_new(4, 6, 4, 26, 1)
+        print("fri grind 26", flush=True)
+        got = z.fri.commit_open(coeffs, zp, z.Transcript(b"g26"), ctx)  # smallest nonce 167,873,451
+        assert got == o.fri_commit_open(coeffs, op, o.Transcript(b"g26"))

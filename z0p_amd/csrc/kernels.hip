@@ -1687,7 +1687,7 @@ __global__ __launch_bounds__(WG) void k_open(const OpenJob* tab, uint32_t njobs)
     }
     const OpenJob& J = tab[lo];
     const uint32_t q = blockIdx.x - J.block_begin;
-    if (q < J.nq) {
+    if (q < J.nq && (J.live == nullptr || *J.live != 0)) {
         const uint64_t leaf = J.queries[q];
         uint32_t* o = J.out + uint64_t(q) * J.words_per_query;
         for (uint32_t w = threadIdx.x; w < J.words_per_query; w += WG) o[w] = *open_src(J, leaf, w);

@@ -349,6 +349,9 @@ struct OpenJob {
     uint32_t words_per_query;
     uint32_t* out;
     uint32_t block_begin;
+    // Non-null: the job was enqueued before its queries were known; it runs
+    // only if *live (written with the queries) is non-zero.
+    const uint64_t* live = nullptr;
 };
 void launch_open(const Launch& L, std::vector<OpenJob> jobs);
 

@@ -936,10 +936,16 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
             }
             o.logn = P.logn;
             o.logb = P.logb;
-            o.queries = static_cast<const uint64_t*>(c.ring.push(P.queries.data(), P.queries.size() * 8));
-            o.nq = uint32_t(P.queries.size());
+            if (cst) {
+                // Enqueued ahead: [u64 live][u64 queries...] arrive in the phase's constants.
+                o.live = static_cast<const uint64_t*>(cst);
+                o.queries = o.live + 1;
+            } else {
+                o.queries = static_cast<const uint64_t*>(c.ring.push(P.queries.data(), P.queries.size() * 8));
+            }
+            o.nq = uint32_t(P.fp.num_queries);
             o.words_per_query = P.wpq;
-            o.out = stage_take(c, used, size_t(P.wpq) * P.queries.size(), P.stage_off);
+            o.out = stage_take(c, used, size_t(P.wpq) * P.fp.num_queries, P.stage_off);
             ch.push_back(o);
             break;
         }
@@ -1208,6 +1214,11 @@ bool Engine::iterate() {
         Proof* p = active_[0];
         try {
             pipeline(*p);
+            if (p->phase == PH_DONE) {
+                stage_need_ -= p->stage_need;
+                finish(p, Z0P_OK);
+                active_.clear();
+            }
         } catch (const Status& s) {
             stage_need_ -= p->stage_need;
             finish(p, s.code);
@@ -1297,8 +1308,13 @@ void Engine::pipeline(Proof& P) {
     if (P.air) phs.push_back({PH_COMPOSE, 0, 0});
     for (size_t d = 0; d < P.folds; ++d) phs.push_back({PH_FOLD, d, 0});
     if (P.fp.grinding_bits > 0) phs.push_back({PH_GRIND, 0, 0});
-    require(phs.size() <= Ctx::MAILBOX_SLOTS, Z0P_ERR_DEVICE);
     constexpr size_t SLOT = Ctx::MAILBOX_SLOT;
+    // The openings too, when their query list fits the remaining slots; they
+    // run only if the first proof-of-work window finds the nonce.
+    const size_t open_slots = (8 + 8 * size_t(P.fp.num_queries) + SLOT - 1) / SLOT;
+    const bool with_open = phs.size() + open_slots <= Ctx::MAILBOX_SLOTS;
+    if (with_open) phs.push_back({PH_OPEN, 0, 0});
+    require(phs.size() <= Ctx::MAILBOX_SLOTS, Z0P_ERR_DEVICE);
     static_assert(sizeof(ComposeConsts) <= SLOT && sizeof(FoldConsts) <= SLOT && 64 <= SLOT, "mailbox slot");
     c_.reserve_stage(P.stage_need * phs.size());
     std::vector<std::vector<Step>> chains(phs.size());
@@ -1327,7 +1343,7 @@ void Engine::pipeline(Proof& P) {
                 gate.want = g0 + k + 1;
                 gate.src = reinterpret_cast<const uint32_t*>(c_.mailbox_dev + Ctx::MAILBOX_HEAD + (k + 1) * SLOT);
                 gate.dst = reinterpret_cast<uint32_t*>(P.consts + (k + 1) * SLOT);
-                gate.words = uint32_t(SLOT / 4);
+                gate.words = uint32_t((phs[k + 1].ph == PH_OPEN ? open_slots : 1) * SLOT / 4);
                 gate.spin_limit = 1u << 22;
                 gate.timeout = reinterpret_cast<uint32_t*>(c_.mailbox_dev + 8);
             }
@@ -1338,26 +1354,47 @@ void Engine::pipeline(Proof& P) {
     c_.seq = seq0 + phs.size();
     c_.prof.host("~host_build_launch", now_ms() - t_build);
     try {
+        bool cancelled = false;
         for (size_t k = 0; k < phs.size(); ++k) {
             const double t_wait = now_ms();
             c_.wait_flag(seq0 + 1 + k, false);
             if (__atomic_load_n(timeout, __ATOMIC_ACQUIRE)) throw Status(Z0P_ERR_DEVICE);
             const double t_host = now_ms();
+            if (cancelled) break;  // the openings drained without running
+            const Phase resume = P.phase;
             P.phase = phs[k].ph;
             P.depth = phs[k].depth;
             P.stage_off = phs[k].stage_off;
-            host_step(c_, P);
+            if (phs[k].ph == PH_OPEN && resume != PH_OPEN) P.phase = resume;
+            else host_step(c_, P);
             if (k + 1 < phs.size()) {
                 const Ph& nx = phs[k + 1];
-                require(P.phase == nx.ph && (nx.ph != PH_FOLD || P.depth == nx.depth), Z0P_ERR_DEVICE);
                 uint8_t* slot = c_.mailbox_host + Ctx::MAILBOX_HEAD + (k + 1) * SLOT;
-                if (nx.ph == PH_COMPOSE) std::memcpy(slot, &P.cc, sizeof(P.cc));
-                else if (nx.ph == PH_FOLD) std::memcpy(slot, &P.fold, sizeof(P.fold));
-                else std::memcpy(slot, P.pow_hw, sizeof(P.pow_hw));
+                if (nx.ph == PH_OPEN) {
+                    // Nonce found: hand over the queries; otherwise cancel the
+                    // openings and continue the search stepwise.
+                    const uint64_t live = P.phase == PH_OPEN ? 1 : 0;
+                    std::memcpy(slot, &live, 8);
+                    if (live) {
+                        require(P.queries.size() == P.fp.num_queries, Z0P_ERR_DEVICE);
+                        std::memcpy(slot + 8, P.queries.data(), 8 * P.queries.size());
+                    }
+                    cancelled = !live;
+                } else {
+                    require(P.phase == nx.ph && (nx.ph != PH_FOLD || P.depth == nx.depth), Z0P_ERR_DEVICE);
+                    if (nx.ph == PH_COMPOSE) std::memcpy(slot, &P.cc, sizeof(P.cc));
+                    else if (nx.ph == PH_FOLD) std::memcpy(slot, &P.fold, sizeof(P.fold));
+                    else std::memcpy(slot, P.pow_hw, sizeof(P.pow_hw));
+                }
                 __atomic_store_n(gate, g0 + k + 1, __ATOMIC_RELEASE);
             }
             c_.prof.host("~host_wait_device", t_host - t_wait);
             c_.prof.host("~host_transcript", now_ms() - t_host);
+        }
+        if (c_.prof.enabled) {
+            // Every gate is open by now: resolve the per-launch events.
+            hip_check(hipStreamSynchronize(c_.stream));
+            c_.prof.resolve();
         }
     } catch (...) {
         // Open every remaining gate so the enqueued work drains, then fail.
