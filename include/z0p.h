@@ -231,13 +231,18 @@ int z0p_ctx_autotune(z0p_ctx* ctx, const z0p_air_params* work, size_t count, uin
 size_t z0p_ctx_autotune_log(z0p_ctx* ctx, char* buf, size_t cap);
 
 /* ---- serving (no reference counterpart) ----
- * An asynchronous request loop on its own host thread, which takes over the
- * ctx until z0p_server_destroy. Requests arriving while others run are
- * admitted into the running batch as device memory and slots free up, so
- * their device work overlaps; every result is byte-identical to the
- * proof z0p_air_prove returns for the same request. */
+ * An asynchronous request loop that takes over the ctx until
+ * z0p_server_destroy. Requests below 2^16 steps and the larger ones run in two
+ * lanes, each a scheduler on its own host thread and HIP stream (the large
+ * lane on a second context the server creates on the same device with the
+ * ctx's kernel schedule), so short requests are not held behind long ones.
+ * Requests arriving while others run are admitted into their lane's running
+ * batch as device memory and slots free up, so their device work overlaps;
+ * a request alone in its lane runs all its phases back to back. Every result
+ * is byte-identical to the proof z0p_air_prove returns for the same request. */
 typedef struct z0p_server z0p_server;
-/* hbm_budget 0: half of the free device memory; max_active 0: 256. */
+/* hbm_budget 0: half of the free device memory; the small lane gets half of
+ * the budget, at most 1 GiB. max_active 0: 256 per lane. */
 int z0p_server_create(z0p_ctx* ctx, uint64_t hbm_budget, uint32_t max_active, z0p_server** out);
 /* Queues one AIR proof. The transcript is copied: the request proves over
  * its own copy and the caller's transcript is left unchanged. */

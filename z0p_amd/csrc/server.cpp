@@ -1,9 +1,11 @@
 // Batched and served proving on one device (include/z0p.h: z0p_air_prove_batch,
 // z0p_server_*). Both feed the proof scheduler (prover.cpp: Engine): proofs
 // share every kernel launch and are admitted as pool memory frees up. The
-// server runs the scheduler on its own host thread and admits requests that
-// arrive while others are on the device into the running batch (continuous
-// batching); a request's proof bytes do not depend on what it ran beside.
+// server runs two schedulers (small and large requests), each on its own host
+// thread and stream, and admits requests that arrive while others are on the
+// device into the running batch of their lane (continuous batching); a
+// request's proof bytes do not depend on what it ran beside.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstdlib>
@@ -63,18 +65,37 @@ struct ServerRequest {
     }
 };
 
-struct z0p_server {
+// Requests of at least 2^LARGE_LOG steps go to the large-proof lane, the rest
+// to the small-proof lane. Each lane is one scheduler on its own stream (and
+// its own context), run by its own host thread: short proofs keep cycling
+// through their phases while long ones occupy the other lane, and the two
+// streams' kernels share the device.
+constexpr uint64_t LARGE_LOG = 16;
+constexpr int LANES = 2;
+
+struct z0p_server;
+
+struct Lane {
+    z0p_server* s = nullptr;
     z0p::Ctx* c = nullptr;
-    size_t budget = 0, max_active = DEFAULT_MAX_ACTIVE;
+    bool owned = false;  // created by the server (the large lane)
+    size_t budget = 0;
     std::unique_ptr<z0p::Engine> engine;
     std::thread worker;
+    std::condition_variable cv_work;
+    std::deque<ServerRequest*> incoming;
+    std::map<uint64_t, ServerRequest*> running;
+    void loop();
+};
+
+struct z0p_server {
+    size_t max_active = DEFAULT_MAX_ACTIVE;
+    Lane lanes[LANES];
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_done;
     bool stop = false;
     uint64_t next_id = 1;
     size_t pending = 0;  // submitted and not yet polled
-    std::deque<ServerRequest*> incoming;
-    std::map<uint64_t, ServerRequest*> running;
     std::deque<ServerRequest*> finished;
 
     void complete(ServerRequest* r) {
@@ -83,65 +104,71 @@ struct z0p_server {
         finished.push_back(r);
         cv_done.notify_all();
     }
-
-    void loop() {
-        (void)hipSetDevice(c->device);
-        z0p::Engine* e = engine.get();
-        auto on_done = [this](Proof* p) {
-            ServerRequest* r = nullptr;
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                auto it = running.find(z0p::proof_tag(p));
-                if (it == running.end()) return;
-                r = it->second;
-                running.erase(it);
-            }
-            r->status = z0p::proof_status(p);
-            double engine_submit = 0;
-            z0p::proof_times(p, engine_submit, r->admit_ms, r->done_ms);
-            if (r->admit_ms == 0) r->admit_ms = r->done_ms;  // never admitted (over budget)
-            complete(r);
-        };
-        e->on_done(on_done);
-        while (true) {
-            std::deque<ServerRequest*> arrived;
-            {
-                std::unique_lock<std::mutex> lk(mu);
-                cv_work.wait(lk, [&] { return stop || !incoming.empty() || e->active() + e->queued() > 0; });
-                if (stop && incoming.empty() && e->active() + e->queued() == 0) break;
-                arrived.swap(incoming);
-            }
-            for (ServerRequest* r : arrived) {
-                try {
-                    r->proof = z0p::proof_new_air(*c, r->params, &r->tr);
-                } catch (const z0p::Status& s) {
-                    r->status = s.code;
-                    complete(r);
-                    continue;
-                } catch (...) {
-                    r->status = Z0P_ERR_DEVICE;
-                    complete(r);
-                    continue;
-                }
-                z0p::proof_set_tag(r->proof, r->id);
-                {
-                    std::lock_guard<std::mutex> lk(mu);
-                    running[r->id] = r;
-                }
-                e->submit(r->proof);
-            }
-            // A device failure ends the proofs on the device with its status;
-            // the scheduler is left empty and later requests run as usual.
-            try {
-                e->iterate();
-            } catch (const z0p::Status& s) {
-                e->fail_all(s.code);
-            } catch (...) {
-                e->fail_all(Z0P_ERR_DEVICE);
-            }
+    ~z0p_server() {
+        for (Lane& l : lanes) {
+            l.engine.reset();
+            if (l.owned) delete l.c;
         }
     }
 };
+
+void Lane::loop() {
+    (void)hipSetDevice(c->device);
+    z0p::Engine* e = engine.get();
+    auto on_done = [this](Proof* p) {
+        ServerRequest* r = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            auto it = running.find(z0p::proof_tag(p));
+            if (it == running.end()) return;
+            r = it->second;
+            running.erase(it);
+        }
+        r->status = z0p::proof_status(p);
+        double engine_submit = 0;
+        z0p::proof_times(p, engine_submit, r->admit_ms, r->done_ms);
+        if (r->admit_ms == 0) r->admit_ms = r->done_ms;  // never admitted (over budget)
+        s->complete(r);
+    };
+    e->on_done(on_done);
+    while (true) {
+        std::deque<ServerRequest*> arrived;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            cv_work.wait(lk, [&] { return s->stop || !incoming.empty() || e->active() + e->queued() > 0; });
+            if (s->stop && incoming.empty() && e->active() + e->queued() == 0) break;
+            arrived.swap(incoming);
+        }
+        for (ServerRequest* r : arrived) {
+            try {
+                r->proof = z0p::proof_new_air(*c, r->params, &r->tr);
+            } catch (const z0p::Status& st) {
+                r->status = st.code;
+                s->complete(r);
+                continue;
+            } catch (...) {
+                r->status = Z0P_ERR_DEVICE;
+                s->complete(r);
+                continue;
+            }
+            z0p::proof_set_tag(r->proof, r->id);
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                running[r->id] = r;
+            }
+            e->submit(r->proof);
+        }
+        // A device failure ends the proofs on the device with its status;
+        // the scheduler is left empty and later requests run as usual.
+        try {
+            e->iterate();
+        } catch (const z0p::Status& st) {
+            e->fail_all(st.code);
+        } catch (...) {
+            e->fail_all(Z0P_ERR_DEVICE);
+        }
+    }
+}
 
 extern "C" {
 
@@ -202,18 +229,36 @@ int z0p_server_create(z0p_ctx* ctx, uint64_t hbm_budget, uint32_t max_active, z0
     if (!ctx || !out) return Z0P_ERR_INVALID_ARGUMENT;
     return guarded([&] {
         auto s = std::make_unique<z0p_server>();
-        s->c = ctx->c;
+        z0p::Ctx* small_ctx = ctx->c;
         if (hbm_budget == 0) {
             size_t free_b = 0, total_b = 0;
             z0p::hip_check(hipMemGetInfo(&free_b, &total_b));
-            if (s->c->pool.in_use() == 0) free_b += s->c->pool.capacity();
+            if (small_ctx->pool.in_use() == 0) free_b += small_ctx->pool.capacity();
             hbm_budget = free_b / 2;
         }
-        s->budget = size_t(hbm_budget);
         s->max_active = max_active ? max_active : DEFAULT_MAX_ACTIVE;
-        s->engine.reset(new z0p::Engine(*s->c, s->budget, s->max_active));
+        // The large lane gets its own context (stream, pool, tables) with the
+        // caller's kernel schedule. The small lane's share of the budget is
+        // half of it, at most 1 GiB (a 2^15-step proof needs ~32 MiB).
+        Lane& small = s->lanes[0];
+        Lane& large = s->lanes[1];
+        small.c = small_ctx;
+        large.c = z0p::ctx_create(small_ctx->device);
+        large.owned = true;
+        large.c->sched = small_ctx->sched;
+        small.budget = std::min<size_t>(size_t(hbm_budget) / 2, size_t(1) << 30);
+        large.budget = size_t(hbm_budget) - small.budget;
+        for (Lane& l : s->lanes) {
+            l.s = s.get();
+            l.engine.reset(new z0p::Engine(*l.c, l.budget, s->max_active));
+            // A request alone in its lane runs all its phases back to back.
+            l.engine->set_pipelined(true);
+        }
         z0p_server* raw = s.get();
-        s->worker = std::thread([raw] { raw->loop(); });
+        for (Lane& l : raw->lanes) {
+            Lane* lp = &l;
+            l.worker = std::thread([lp] { lp->loop(); });
+        }
         *out = s.release();
     });
 }
@@ -229,12 +274,13 @@ int z0p_server_submit_air(z0p_server* s, const z0p_air_params* params, const z0p
         r->tr = transcript->t;
         r->params = *params;
         r->submit_ms = now_ms();
+        Lane& lane = s->lanes[params->n_steps >= (uint64_t(1) << LARGE_LOG) ? 1 : 0];
         std::lock_guard<std::mutex> lk(s->mu);
         r->id = s->next_id++;
         *request_id = r->id;
-        s->incoming.push_back(r.release());
+        lane.incoming.push_back(r.release());
         ++s->pending;
-        s->cv_work.notify_one();
+        lane.cv_work.notify_one();
     });
 }
 
@@ -281,11 +327,13 @@ void z0p_server_destroy(z0p_server* s) {
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
-        s->cv_work.notify_all();
+        for (Lane& l : s->lanes) l.cv_work.notify_all();
     }
-    if (s->worker.joinable()) s->worker.join();
+    for (Lane& l : s->lanes)
+        if (l.worker.joinable()) l.worker.join();
     for (ServerRequest* r : s->finished) delete r;
-    for (ServerRequest* r : s->incoming) delete r;
+    for (Lane& l : s->lanes)
+        for (ServerRequest* r : l.incoming) delete r;
     delete s;
 }
 
