@@ -29,12 +29,13 @@ int guarded(F&& f) {
     }
 }
 
-int emit(const std::vector<uint8_t>& v, uint8_t** out, size_t* len) {
-    uint8_t* p = static_cast<uint8_t*>(std::malloc(v.empty() ? 1 : v.size()));
+// Hands the library's buffer to the caller (freed with z0p_free).
+int emit(z0p::Bytes& v, uint8_t** out, size_t* len) {
+    const size_t n = v.size();
+    uint8_t* p = v.release();
     if (!p) return Z0P_ERR_OUT_OF_MEMORY;
-    if (!v.empty()) std::memcpy(p, v.data(), v.size());
     *out = p;
-    *len = v.size();
+    *len = n;
     return Z0P_OK;
 }
 
@@ -260,7 +261,7 @@ int z0p_air_prove(z0p_ctx* ctx, const z0p_air_params* params, z0p_transcript* tr
     if (!ctx || !params || !transcript || !proof || !proof_len) return Z0P_ERR_INVALID_ARGUMENT;
     int rc = Z0P_OK;
     int g = guarded([&] {
-        std::vector<uint8_t> bytes = z0p::air_prove(*ctx->c, *params, transcript->t);
+        z0p::Bytes bytes = z0p::air_prove(*ctx->c, *params, transcript->t);
         rc = emit(bytes, proof, proof_len);
     });
     return g != Z0P_OK ? g : rc;
@@ -273,7 +274,7 @@ int z0p_air_prove_with_trace(z0p_ctx* ctx, const z0p_air_params* params, const u
         return Z0P_ERR_INVALID_ARGUMENT;
     int rc = Z0P_OK;
     int g = guarded([&] {
-        std::vector<uint8_t> bytes =
+        z0p::Bytes bytes =
             z0p::air_prove_with_trace(*ctx->c, *params, col_a, len_a, col_b, len_b, transcript->t);
         rc = emit(bytes, proof, proof_len);
     });
@@ -285,7 +286,7 @@ int z0p_air_prove_zk(z0p_ctx* ctx, const z0p_air_params* params, uint64_t n_rand
     if (!ctx || !params || !rng || !transcript || !proof || !proof_len) return Z0P_ERR_INVALID_ARGUMENT;
     int rc = Z0P_OK;
     int g = guarded([&] {
-        std::vector<uint8_t> bytes = z0p::air_prove_zk(*ctx->c, *params, n_rand, *rng->r, transcript->t);
+        z0p::Bytes bytes = z0p::air_prove_zk(*ctx->c, *params, n_rand, *rng->r, transcript->t);
         rc = emit(bytes, proof, proof_len);
     });
     return g != Z0P_OK ? g : rc;
@@ -363,7 +364,7 @@ int fri_entry(z0p_ctx* ctx, const uint32_t* src, size_t count, int kind, uint64_
         return Z0P_ERR_INVALID_ARGUMENT;
     int rc = Z0P_OK;
     int g = guarded([&] {
-        std::vector<uint8_t> c, p;
+        z0p::Bytes c, p;
         z0p::fri_commit_open(*ctx->c, src, count, kind, degree_bound, *params, transcript->t, c, p);
         rc = emit(c, commitment, commitment_len);
         if (rc == Z0P_OK) {

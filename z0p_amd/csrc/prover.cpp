@@ -505,7 +505,7 @@ struct Proof {
     size_t stage_off = 0;  // words, this iteration
     // result
     int status = Z0P_OK;
-    std::vector<uint8_t> proof, commitment;
+    Bytes proof, commitment;
     double t_submit = 0, t_admit = 0, t_done = 0;
 };
 
@@ -1063,17 +1063,17 @@ void serialize(Proof& P, const uint32_t* words) {
             w.words(lw, 16);
             w.words(sw, 16);
         }
-        P.proof = std::move(w.finish());
+        P.proof = w.finish();
     } else {
         Writer wc;
         wc.byte(WIRE_VERSION);
         write_fri_commitment_body(wc, P);
-        P.commitment = std::move(wc.finish());
+        P.commitment = wc.finish();
         Writer wp;
         wp.reserve(estimate);
         wp.byte(WIRE_VERSION);
         write_fri_proof_body(wp, P, words);
-        P.proof = std::move(wp.finish());
+        P.proof = wp.finish();
     }
 }
 
@@ -1459,8 +1459,8 @@ Proof* proof_new_fri(Ctx& c, const uint32_t* src, uint64_t count, int kind, uint
 
 void proof_free(Proof* p) { delete p; }
 int proof_status(const Proof* p) { return p->status; }
-std::vector<uint8_t>& proof_bytes(Proof* p) { return p->proof; }
-std::vector<uint8_t>& proof_commitment(Proof* p) { return p->commitment; }
+Bytes& proof_bytes(Proof* p) { return p->proof; }
+Bytes& proof_commitment(Proof* p) { return p->commitment; }
 size_t proof_mem_bytes(const Proof* p) { return p->mem_bytes; }
 void proof_times(const Proof* p, double& submit, double& admit, double& done) {
     submit = p->t_submit;
@@ -1471,7 +1471,7 @@ void proof_set_tag(Proof* p, uint64_t tag) { p->tag = tag; }
 uint64_t proof_tag(const Proof* p) { return p->tag; }
 
 namespace {
-std::vector<uint8_t> run_one(Ctx& ctx, Proof* raw) {
+Bytes run_one(Ctx& ctx, Proof* raw) {
     std::unique_ptr<Proof> P(raw);
     Engine e(ctx, 0, 1);
     e.set_pipelined(true);
@@ -1482,22 +1482,21 @@ std::vector<uint8_t> run_one(Ctx& ctx, Proof* raw) {
 }
 }  // namespace
 
-std::vector<uint8_t> air_prove(Ctx& ctx, const z0p_air_params& p, Transcript& tr) {
+Bytes air_prove(Ctx& ctx, const z0p_air_params& p, Transcript& tr) {
     return run_one(ctx, proof_new_air(ctx, p, &tr));
 }
 
-std::vector<uint8_t> air_prove_with_trace(Ctx& ctx, const z0p_air_params& p, const uint32_t* col_a, uint64_t len_a,
+Bytes air_prove_with_trace(Ctx& ctx, const z0p_air_params& p, const uint32_t* col_a, uint64_t len_a,
                                           const uint32_t* col_b, uint64_t len_b, Transcript& tr) {
     return run_one(ctx, proof_new_air_trace(ctx, p, col_a, len_a, col_b, len_b, &tr));
 }
 
-std::vector<uint8_t> air_prove_zk(Ctx& ctx, const z0p_air_params& p, uint64_t n_rand, ChaCha20& rng, Transcript& tr) {
+Bytes air_prove_zk(Ctx& ctx, const z0p_air_params& p, uint64_t n_rand, ChaCha20& rng, Transcript& tr) {
     return run_one(ctx, proof_new_air_zk(ctx, p, n_rand, rng, &tr));
 }
 
 void fri_commit_open(Ctx& ctx, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,
-                     const z0p_fri_params& fp, Transcript& tr, std::vector<uint8_t>& commitment,
-                     std::vector<uint8_t>& proof) {
+                     const z0p_fri_params& fp, Transcript& tr, Bytes& commitment, Bytes& proof) {
     std::unique_ptr<Proof> P(proof_new_fri(ctx, src, count, kind, degree_bound, fp, &tr));
     Engine e(ctx, 0, 1);
     e.set_pipelined(true);

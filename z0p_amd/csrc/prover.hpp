@@ -72,8 +72,8 @@ Proof* proof_new_fri(Ctx& c, const uint32_t* src, uint64_t count, int kind, uint
                      const z0p_fri_params& fp, Transcript* tr);
 void proof_free(Proof* p);
 int proof_status(const Proof* p);
-std::vector<uint8_t>& proof_bytes(Proof* p);
-std::vector<uint8_t>& proof_commitment(Proof* p);
+Bytes& proof_bytes(Proof* p);
+Bytes& proof_commitment(Proof* p);
 size_t proof_mem_bytes(const Proof* p);
 void proof_times(const Proof* p, double& submit_ms, double& admit_ms, double& done_ms);
 void proof_set_tag(Proof* p, uint64_t tag);
@@ -131,13 +131,13 @@ struct TuneResult {
 };
 TuneResult autotune(Ctx& c, const std::vector<z0p_air_params>& work, unsigned reps);
 
-std::vector<uint8_t> air_prove(Ctx& ctx, const z0p_air_params& p, Transcript& tr);
-std::vector<uint8_t> air_prove_with_trace(Ctx& ctx, const z0p_air_params& p, const uint32_t* col_a, uint64_t len_a,
+Bytes air_prove(Ctx& ctx, const z0p_air_params& p, Transcript& tr);
+Bytes air_prove_with_trace(Ctx& ctx, const z0p_air_params& p, const uint32_t* col_a, uint64_t len_a,
                                           const uint32_t* col_b, uint64_t len_b, Transcript& tr);
-std::vector<uint8_t> air_prove_zk(Ctx& ctx, const z0p_air_params& p, uint64_t n_rand, ChaCha20& rng, Transcript& tr);
+Bytes air_prove_zk(Ctx& ctx, const z0p_air_params& p, uint64_t n_rand, ChaCha20& rng, Transcript& tr);
 void fri_commit_open(Ctx& ctx, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,
                      const z0p_fri_params& fp, Transcript& tr,
-                     std::vector<uint8_t>& commitment, std::vector<uint8_t>& proof);
+                     Bytes& commitment, Bytes& proof);
 
 // Host verifier (air.rs:607-752, fri.rs:686-859); throws Status on rejection.
 void air_verify(const z0p_air_params& p, Transcript& tr, const uint8_t* proof, size_t len);

@@ -35,10 +35,10 @@ int guarded(F&& f) {
     }
 }
 
-uint8_t* copy_out(const std::vector<uint8_t>& v) {
-    uint8_t* p = static_cast<uint8_t*>(std::malloc(v.empty() ? 1 : v.size()));
-    if (p && !v.empty()) std::memcpy(p, v.data(), v.size());
-    return p;
+// The proof's buffer, handed to the caller (freed with z0p_free).
+uint8_t* hand_over(z0p::Bytes& v, size_t* len) {
+    *len = v.size();
+    return v.release();
 }
 
 constexpr size_t DEFAULT_MAX_ACTIVE = 256;
@@ -214,13 +214,11 @@ int z0p_air_prove_batch(z0p_ctx* ctx, const z0p_air_params* params, z0p_transcri
             if (!ps[i]) continue;
             statuses[i] = z0p::proof_status(ps[i].get());
             if (statuses[i] != Z0P_OK) continue;
-            const std::vector<uint8_t>& b = z0p::proof_bytes(ps[i].get());
-            proofs[i] = copy_out(b);
+            proofs[i] = hand_over(z0p::proof_bytes(ps[i].get()), &proof_lens[i]);
             if (!proofs[i]) {
                 statuses[i] = Z0P_ERR_OUT_OF_MEMORY;
-                continue;
+                proof_lens[i] = 0;
             }
-            proof_lens[i] = b.size();
         }
     });
 }
@@ -305,13 +303,12 @@ int z0p_server_poll(z0p_server* s, int timeout_ms, uint64_t* request_id, int* st
     if (latency_ms) *latency_ms = r->done_ms - r->submit_ms;
     if (queue_ms) *queue_ms = r->admit_ms - r->submit_ms;
     if (r->status == Z0P_OK && r->proof) {
-        const std::vector<uint8_t>& b = z0p::proof_bytes(r->proof);
-        *proof = copy_out(b);
+        *proof = hand_over(z0p::proof_bytes(r->proof), proof_len);
         if (!*proof) {
             *status = Z0P_ERR_OUT_OF_MEMORY;
+            *proof_len = 0;
             return Z0P_OK;
         }
-        *proof_len = b.size();
     }
     return Z0P_OK;
 }

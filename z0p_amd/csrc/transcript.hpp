@@ -9,6 +9,8 @@
 // Codec: the VERSION = 6 wire format of src/codec.rs (u32 counts, 4-byte
 // canonical fields, 32-byte digests, one direction byte per path step).
 #pragma once
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -120,17 +122,55 @@ constexpr uint32_t MAX_PATH_STEPS = 128;
 constexpr uint32_t MAX_CAP_DIGESTS = 256;
 constexpr uint32_t MAX_ELEMS = 1u << 27;
 
-// Appends through a cursor into a buffer sized up front (reserve() with the
-// expected total); finish() trims it to the bytes written.
+// Serialized bytes in a malloc'd buffer: the C boundary hands it to the
+// caller (freed with z0p_free) without copying.
+class Bytes {
+   public:
+    Bytes() = default;
+    Bytes(const Bytes&) = delete;
+    Bytes& operator=(const Bytes&) = delete;
+    Bytes(Bytes&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr, o.n_ = 0; }
+    Bytes& operator=(Bytes&& o) noexcept {
+        if (this != &o) {
+            std::free(p_);
+            p_ = o.p_, n_ = o.n_;
+            o.p_ = nullptr, o.n_ = 0;
+        }
+        return *this;
+    }
+    ~Bytes() { std::free(p_); }
+    Bytes(uint8_t* p, size_t n) : p_(p), n_(n) {}
+    const uint8_t* data() const { return p_; }
+    size_t size() const { return n_; }
+    bool empty() const { return n_ == 0; }
+    bool operator==(const Bytes& o) const { return n_ == o.n_ && (n_ == 0 || std::memcmp(p_, o.p_, n_) == 0); }
+    // Ownership to the caller (a 1-byte allocation for an empty buffer).
+    uint8_t* release() {
+        uint8_t* r = p_ ? p_ : static_cast<uint8_t*>(std::malloc(1));
+        p_ = nullptr, n_ = 0;
+        return r;
+    }
+
+   private:
+    uint8_t* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// Appends through a cursor into a malloc'd buffer sized up front (reserve()
+// with the expected total); finish() trims it to the bytes written.
 struct Writer {
-    std::vector<uint8_t> out;
-    size_t pos = 0;
+    uint8_t* buf = nullptr;
+    size_t cap = 0, pos = 0;
+    Writer() = default;
+    Writer(const Writer&) = delete;
+    Writer& operator=(const Writer&) = delete;
+    ~Writer() { std::free(buf); }
     void reserve(size_t bytes) {
-        if (out.size() < bytes) out.resize(bytes);
+        if (cap < bytes) grow(bytes);
     }
     uint8_t* room(size_t k) {
-        if (pos + k > out.size()) out.resize(std::max(2 * out.size(), pos + k));
-        uint8_t* p = out.data() + pos;
+        if (pos + k > cap) grow(std::max(2 * cap, pos + k));
+        uint8_t* p = buf + pos;
         pos += k;
         return p;
     }
@@ -147,9 +187,19 @@ struct Writer {
         static_assert(__BYTE_ORDER__ == __ORDER_LITTLE_ENDIAN__, "little-endian host");
         std::memcpy(room(4 * count), w, 4 * count);
     }
-    std::vector<uint8_t>& finish() {
-        out.resize(pos);
+    Bytes finish() {
+        uint8_t* p = pos ? static_cast<uint8_t*>(std::realloc(buf, pos)) : buf;
+        if (!p) p = buf;
+        Bytes out(p, pos);
+        buf = nullptr, cap = pos = 0;
         return out;
+    }
+
+   private:
+    void grow(size_t bytes) {
+        uint8_t* p = static_cast<uint8_t*>(std::realloc(buf, bytes));
+        if (!p) throw Status(Z0P_ERR_OUT_OF_MEMORY);
+        buf = p, cap = bytes;
     }
 };
 
