@@ -1291,12 +1291,35 @@ __global__ __launch_bounds__(WG) void k_grind(GrindJob inl, const GrindJob* tab,
 
 // ---------------------------------------------------------------------------
 constexpr int COLLECT_WG_MAX = 1024;
+constexpr uint32_t COLLECT_LDS_ITEMS = 8192;
 
 __global__ __launch_bounds__(COLLECT_WG_MAX) void k_collect(CollectItem inl, const CollectItem* tab, uint32_t nitems,
-                                                            uint64_t* flag, uint64_t seq, HostGate gate) {
-    for (uint32_t i = 0; i < nitems; ++i) {
-        const CollectItem J = nitems == 1 ? inl : tab[i];
-        for (uint32_t k = threadIdx.x; k < J.words; k += blockDim.x) J.dst[k] = gc(J.src)[k];
+                                                            uint32_t total, uint64_t* flag, uint64_t seq,
+                                                            HostGate gate) {
+    if (nitems == 1) {
+        for (uint32_t k = threadIdx.x; k < inl.words; k += blockDim.x) inl.dst[k] = gc(inl.src)[k];
+    } else if (nitems <= COLLECT_LDS_ITEMS) {
+        // All items' words as one range: every thread has independent copies
+        // in flight, whatever the items' sizes (a batch has hundreds). The
+        // items' first words sit in LDS for the search.
+        __shared__ uint32_t first[COLLECT_LDS_ITEMS];
+        for (uint32_t i = threadIdx.x; i < nitems; i += blockDim.x) first[i] = tab[i].first;
+        __syncthreads();
+        for (uint32_t g = threadIdx.x; g < total; g += blockDim.x) {
+            uint32_t lo = 0, hi = nitems;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (first[mid] <= g) lo = mid;
+                else hi = mid;
+            }
+            const CollectItem& J = tab[lo];
+            J.dst[g - first[lo]] = gc(J.src)[g - first[lo]];
+        }
+    } else {
+        for (uint32_t i = 0; i < nitems; ++i) {
+            const CollectItem& J = tab[i];
+            for (uint32_t k = threadIdx.x; k < J.words; k += blockDim.x) J.dst[k] = gc(J.src)[k];
+        }
     }
     __threadfence_system();
     __syncthreads();
@@ -1603,16 +1626,19 @@ void launch_interleave(const Launch& L, std::vector<InterleaveJob> jobs) {
 
 void launch_collect(const Launch& L, std::vector<CollectItem> items, uint64_t* flag, uint64_t seq,
                     const HostGate& gate) {
+    uint64_t words = 0;
+    for (CollectItem& it : items) {
+        it.first = uint32_t(words);
+        words += it.words;
+    }
     const CollectItem* table = table_of(L, items);
     if (L.dry) return;
-    uint64_t words = 0;
-    for (const CollectItem& it : items) words += it.words;
     // One wave for the few words of one proof; a full workgroup for a batch.
     const unsigned threads = words <= 4096 ? 64u : unsigned(COLLECT_WG_MAX);
     const CollectItem inl = items.empty() ? CollectItem{nullptr, nullptr, 0, 0} : items[0];
     ProfScope ps(L.prof, "collect", L.stream);
-    hipLaunchKernelGGL(k_collect, dim3(1), dim3(threads), 0, L.stream, inl, table, uint32_t(items.size()), flag, seq,
-                       gate);
+    hipLaunchKernelGGL(k_collect, dim3(1), dim3(threads), 0, L.stream, inl, table, uint32_t(items.size()),
+                       uint32_t(words), flag, seq, gate);
 }
 
 void launch_pow_brev(const Launch& L, PowTable pt, uint32_t* out, int bits, int shift) {

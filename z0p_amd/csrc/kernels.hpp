@@ -279,7 +279,7 @@ struct CollectItem {
     const uint32_t* src;
     uint32_t* dst;
     uint32_t words;
-    uint32_t block_begin;
+    uint32_t first;  // words of the items before this one (set by the launcher)
 };
 // Optional gate run by the same kernel after the flag: wait (bounded by
 // spin_limit polls) until the host-mapped *go reaches `want`, then copy
