@@ -26,23 +26,65 @@ LABEL = b"bench/air-scale"
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(n, m, layers):
+def tree_work(m, layers, cap_height, sched):
+    """BLAKE3 compressions and algorithmic bytes of the proof's Merkle trees,
+    per kernel family under the schedule's fusions. The composition kernel
+    hashes layer 0's leaves and their parents (compose_fuse); a fold kernel
+    hashes the next layer's leaves and parents for layers of at most
+    2^fold_fuse_max_log values; a layer of at most 2^small_max_log values is
+    folded and its whole tree built by one kernel (both "fold"). The rest is
+    the "merkle" family. Bytes count the leaf input once and every digest
+    written once; a leaf is one compression, a node two."""
+    cap = 1 << cap_height
+    work = {}
+
+    def add(fam, comp, nbytes):
+        w = work.setdefault(fam, [0, 0])
+        w[0] += comp
+        w[1] += nbytes
+
+    def tree(leaves, fam, parents_too=False, whole=False):
+        # `fam` hashes the leaves (and their parents, or the whole tree);
+        # the levels left over run in the "merkle" family.
+        top = min(cap, leaves)  # digests at the cap level
+        add(fam, leaves, (64 + 32) * leaves)  # 64-byte leaves
+        nodes = leaves - top
+        if parents_too and leaves > top:
+            add(fam, leaves, 32 * (leaves // 2))
+            nodes -= leaves // 2
+        add(fam if whole else "merkle", 2 * nodes, 32 * nodes)
+
+    tree(m // 8, "merkle")  # trace tree: 8 rows of A || B per leaf
+    for d, L in enumerate(layers):
+        if d == 0:
+            fused = bool(sched["compose_fuse"])
+            tree(L // 4, "compose" if fused else "merkle", parents_too=fused)
+        elif sched["small_max_log"] > 0 and L <= 1 << sched["small_max_log"]:
+            tree(L // 4, "fold", whole=True)
+        elif sched["fold_fuse_max_log"] > 0 and L <= 1 << sched["fold_fuse_max_log"]:
+            tree(L // 4, "fold", parents_too=True)
+        else:
+            tree(L // 4, "merkle")
+    return work
+
+
+def algorithmic_bytes(n, m, layers, cap_height, sched):
     """Algorithmic HBM bytes per proof for each kernel family: every array a
     family must read or produce, counted once (u32 words, Ext4 = 16 bytes,
-    digests = 32 bytes). Tree families count the leaf input once and every
-    digest written once."""
-    def tree_digests(leaves):
-        return 2 * leaves - 1
-    fri_tree = sum(16 * L + 32 * tree_digests(L // 4) for L in layers)
-    return {
+    digests = 32 bytes), with the tree hashing of each family from
+    tree_work. Returns (bytes by family, compressions by family)."""
+    work = tree_work(m, layers, cap_height, sched)
+    alg = {
         "intt": 4 * n + 4 * n,                       # evaluations in, coefficients out
         "lde": 4 * n + 4 * m,                        # coefficients in, m evaluations out
         "linked_b": 4 * m + 4 * m,                   # A in, B out
         "compose": 8 * m + 16 * m,                   # A, B in, Ext4 layer 0 out
-        "trace_tree": 8 * m + 32 * tree_digests(m // 8),
-        "fri_tree": fri_tree,
         "fold": sum(16 * L + 4 * L for L in layers),  # layer in, layer/4 Ext4 out
+        "merkle": 0,
     }
+    for fam, (_, nbytes) in work.items():
+        alg[fam] += nbytes
+    return alg, {fam: comp for fam, (comp, _) in work.items()}
 
 
 # Kernel launch names -> families. Tree level kernels are shared by the
@@ -51,7 +93,7 @@ FAMILY = {
     "intt": "intt", "lde": "lde", "linked_b": "linked_b", "compose": "compose", "compose_leaves": "compose",
     "fold_leaves": "fold",
     "trace_leaves": "merkle", "fri_leaves": "merkle", "tree_levels": "merkle", "tree_tail": "merkle",
-    "fold": "fold",
+    "fold": "fold", "small_commit": "fold",
 }
 
 
@@ -311,8 +353,7 @@ def main():
     while L > params.final_size:
         layers.append(L)
         L //= 4
-    alg = algorithmic_bytes(n, m, layers)
-    alg["merkle"] = alg.pop("trace_tree") + alg.pop("fri_tree")
+    alg, comps = algorithmic_bytes(n, m, layers, params.cap_height, ctx.get_schedule())
     per = max(args.prof_steps, 1)
     fam_time, fam_launches = {}, {}
     for k, (cnt, tot) in stats.items():
@@ -324,11 +365,10 @@ def main():
                     "alg_bytes": alg[f], "achieved_gbs": round(alg[f] / (t * 1e-3) / 1e9, 1)}
                 for f, t in sorted(fam_time.items())}
     # Merkle hashing is integer-VALU work: report BLAKE3 compressions/s too.
-    cap = 1 << params.cap_height
-    comp = (m // 8) + 2 * (m // 8 - cap) + sum(L // 4 + 2 * (L // 4 - cap) for L in layers)
-    if "merkle" in families:
-        families["merkle"]["compressions"] = comp
-        families["merkle"]["gcompress_per_s"] = round(comp / (fam_time["merkle"] * 1e-3) / 1e9, 2)
+    for f, comp in comps.items():
+        if f in families:
+            families[f]["compressions"] = comp
+            families[f]["gcompress_per_s"] = round(comp / (fam_time[f] * 1e-3) / 1e9, 2)
     for f, v in families.items():
         v["hbm_frac"] = round(v["achieved_gbs"] / HBM_PEAK_GBS, 4)
     # Measured HBM traffic per proof (rocprofv3 FETCH_SIZE + WRITE_SIZE,

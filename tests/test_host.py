@@ -269,3 +269,28 @@ def test_fibonacci_interpolant_closed_form():
             got = [(k1 * inv((1 - phi * pow(winv, k, P)) % P) + k2 * inv((1 - psi * pow(winv, k, P)) % P)) % P
                    for k in range(n)]
             assert got == [int(v) for v in want]
+
+
+@pytest.mark.parametrize("logn,cap_height", [(20, 4), (10, 4), (14, 0)])
+def test_bench_tree_work_is_schedule_invariant(logn, cap_height):
+    # The bench attributes each tree's hashing to the kernel family that runs
+    # it under the schedule's fusions; the proof's totals must not move.
+    import bench
+    n, m = 1 << logn, 1 << (logn + 4)
+    layers, L = [], m
+    while L > 16:
+        layers.append(L)
+        L //= 4
+    base = dict(tail_max_log=17, tail_group_max_log=8, tail_spread_log=0, levels_fuse=2, small_max_log=0,
+                grind_blocks=4096, fold_fuse_max_log=0, compose_fuse=0, trace_fuse=1)
+    # unfused: every tree in the merkle family, leaf + 2 per node up to the cap
+    cap = 1 << cap_height
+    want = sum(k + 2 * (k - min(cap, k)) for k in [m // 8] + [x // 4 for x in layers])
+    alg0, comp0 = bench.algorithmic_bytes(n, m, layers, cap_height, base)
+    assert comp0 == {"merkle": want}
+    for sched in [dict(base, compose_fuse=1), dict(base, small_max_log=10), dict(base, fold_fuse_max_log=18),
+                  dict(base, compose_fuse=1, small_max_log=8, fold_fuse_max_log=22)]:
+        alg, comp = bench.algorithmic_bytes(n, m, layers, cap_height, sched)
+        assert sum(comp.values()) == want
+        tree_bytes = lambda a: sum(a.values()) - sum(alg0.values())
+        assert tree_bytes(alg) == 0, sched
