@@ -111,6 +111,17 @@ inline bool verify_capped64(const Digest& cap_entry, const uint8_t* leaf, size_t
     return cur == cap_entry;
 }
 
+// merkle.rs:286-309 — a variable-length leaf (hashed as BLAKE3(0x00 || data)).
+inline bool verify_capped(const Digest& cap_entry, const uint8_t* leaf, size_t leaf_len, size_t index,
+                          size_t total_depth, const Path& path) {
+    if (total_depth >= 64 || path.size() > total_depth || (index >> total_depth) != 0 ||
+        !directions_match_index(index, path))
+        return false;
+    Digest cur = hash_leaf(leaf, leaf_len);
+    for (auto& step : path) cur = step.second ? hash_node(cur, step.first) : hash_node(step.first, cur);
+    return cur == cap_entry;
+}
+
 // merkle.rs:333-337 — (drop, depth).
 inline std::pair<size_t, size_t> capped_shape(size_t num_leaves, size_t cap_height) {
     size_t depth = size_t(__builtin_ctzll(num_leaves));

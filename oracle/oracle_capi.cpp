@@ -8,6 +8,7 @@
 #include <string>
 
 #include "air.hpp"
+#include "range.hpp"
 
 using namespace oracle;
 
@@ -25,6 +26,13 @@ typedef struct {
     uint32_t grinding_bits, reserved0;
     uint64_t cap_height;
 } oracle_fri_params;
+
+typedef struct {
+    uint64_t n_rows, n_bits, blowup, num_queries, final_size;
+    uint32_t grinding_bits, reserved0;
+    uint64_t cap_height;
+    uint32_t value, reserved1;
+} oracle_range_params;
 
 }  // extern "C"
 
@@ -55,6 +63,31 @@ void from_air(const air::Params& q, oracle_air_params* p) {
     p->a0 = q.a0.v;
     p->b0 = q.b0.v;
     p->b_last = q.b_last.v;
+}
+
+range::Params to_range(const oracle_range_params* p) {
+    range::Params q;
+    q.n_rows = p->n_rows;
+    q.n_bits = p->n_bits;
+    q.blowup = p->blowup;
+    q.num_queries = p->num_queries;
+    q.final_size = p->final_size;
+    q.grinding_bits = p->grinding_bits;
+    q.cap_height = p->cap_height;
+    q.value = Field::make(p->value);
+    return q;
+}
+
+void from_range(const range::Params& q, oracle_range_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->n_rows = q.n_rows;
+    p->n_bits = q.n_bits;
+    p->blowup = q.blowup;
+    p->num_queries = q.num_queries;
+    p->final_size = q.final_size;
+    p->grinding_bits = q.grinding_bits;
+    p->cap_height = q.cap_height;
+    p->value = q.value.v;
 }
 
 fri::Params to_fri(const oracle_fri_params* p) {
@@ -426,6 +459,70 @@ int oracle_fri_verify(const uint8_t* cm, size_t cm_len, const uint8_t* pf, size_
         fri::Commitment c = fri::Commitment::from_bytes(cm, cm_len);
         fri::Proof pr = fri::Proof::from_bytes(pf, pf_len);
         fri::verify(c, pr, to_fri(p), *static_cast<Transcript*>(t));
+    })
+}
+
+// range.rs: parameters, the three provers and the verifier. Columns arrive
+// as n_columns blocks of `len` words, V first.
+int oracle_range_params_new(uint64_t n_rows, uint64_t n_bits, uint64_t blowup, uint64_t nq, uint64_t fs,
+                            uint32_t value, oracle_range_params* out) {
+    GUARD(from_range(range::Params::make(n_rows, n_bits, blowup, nq, fs, Field::make(value)), out))
+}
+int oracle_range_params_for_security(uint64_t n_rows, uint64_t n_bits, uint32_t value, uint32_t bits,
+                                     oracle_range_params* out) {
+    GUARD(from_range(range::Params::for_security(n_rows, n_bits, Field::make(value), bits), out))
+}
+int oracle_range_prove(const oracle_range_params* p, void* t, uint8_t** out, size_t* len) {
+    GUARD({
+        range::Proof pr = range::prove(to_range(p), *static_cast<Transcript*>(t));
+        *out = dup_bytes(pr.to_bytes(), len);
+    })
+}
+int oracle_range_prove_with_columns(const oracle_range_params* p, const uint32_t* cols, size_t ncols, size_t len,
+                                    void* t, uint8_t** out, size_t* out_len) {
+    GUARD({
+        std::vector<Field> v = ncols ? to_fields(cols, len) : std::vector<Field>();
+        std::vector<std::vector<Field>> bits;
+        for (size_t c = 1; c < ncols; ++c) bits.push_back(to_fields(cols + c * len, len));
+        range::Proof pr = range::prove_with_columns(to_range(p), v, bits, *static_cast<Transcript*>(t));
+        *out = dup_bytes(pr.to_bytes(), out_len);
+    })
+}
+// range.rs:436 directly: coefficient columns with a declared degree bound.
+int oracle_range_prove_coeffs(const oracle_range_params* p, const uint32_t* cols, size_t ncols, size_t len,
+                              size_t degree_bound, void* t, uint8_t** out, size_t* out_len) {
+    GUARD({
+        range::Params rp = to_range(p);
+        rp.validate();
+        std::vector<std::vector<Field>> c;
+        for (size_t k = 0; k < ncols; ++k) c.push_back(to_fields(cols + k * len, len));
+        range::Proof pr = range::prove_from_columns(rp, c, degree_bound, *static_cast<Transcript*>(t));
+        *out = dup_bytes(pr.to_bytes(), out_len);
+    })
+}
+int oracle_range_prove_zk(const oracle_range_params* p, size_t n_rand, void* rng, void* t, uint8_t** out,
+                          size_t* len) {
+    GUARD({
+        range::Proof pr =
+            range::prove_zk(to_range(p), n_rand, *static_cast<ChaCha20*>(rng), *static_cast<Transcript*>(t));
+        *out = dup_bytes(pr.to_bytes(), len);
+    })
+}
+int oracle_range_verify(const oracle_range_params* p, void* t, const uint8_t* proof, size_t len) {
+    GUARD({
+        range::Proof pr = range::Proof::from_bytes(proof, len);
+        range::verify(to_range(p), pr, *static_cast<Transcript*>(t));
+    })
+}
+// range.rs:752 at one point, for the composition pins.
+int oracle_range_composition_at(uint32_t v, const uint32_t* bits, size_t n_bits, uint32_t point, uint32_t value,
+                                uint64_t n, const uint32_t* alphas4, uint32_t* out4) {
+    GUARD({
+        std::vector<Ext4> al = to_ext(alphas4, n_bits + 2);
+        std::vector<Ext4> ab(al.begin() + 1, al.begin() + 1 + n_bits);
+        Ext4 r = range::composition_at(Field::make(v), to_fields(bits, n_bits), Field::make(point), Field::make(value),
+                                       n, al[0], ab, al[n_bits + 1]);
+        for (int c = 0; c < 4; ++c) out4[c] = r.c[c].v;
     })
 }
 

@@ -51,6 +51,34 @@ class FriParams(ctypes.Structure):
                                      self.final_size)
 
 
+class RangeParams(ctypes.Structure):
+    """Layout shared with ``z0p_range_params`` (include/z0p.h)."""
+
+    _fields_ = [
+        ("n_rows", ctypes.c_uint64),
+        ("n_bits", ctypes.c_uint64),
+        ("blowup", ctypes.c_uint64),
+        ("num_queries", ctypes.c_uint64),
+        ("final_size", ctypes.c_uint64),
+        ("grinding_bits", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
+        ("cap_height", ctypes.c_uint64),
+        ("value", ctypes.c_uint32),
+        ("reserved1", ctypes.c_uint32),
+    ]
+
+    def shape(self):
+        return "B%dQ%dG%dC%dF%d" % (self.blowup, self.num_queries, self.grinding_bits, self.cap_height,
+                                     self.final_size)
+
+    def replace(self, **kw):
+        q = RangeParams()
+        ctypes.pointer(q)[0] = self
+        for k, v in kw.items():
+            setattr(q, k, v)
+        return q
+
+
 _lib = None
 
 
@@ -133,6 +161,21 @@ def lib():
                                                        ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.POINTER(u8p),
                                                        ctypes.POINTER(sz)]),
         "oracle_evaluate_on_coset_ext": (ctypes.c_int, [u32p, sz, sz, u32p]),
+        "oracle_range_params_new": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                                   ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(RangeParams)]),
+        "oracle_range_params_for_security": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
+                                                            ctypes.c_uint32, ctypes.POINTER(RangeParams)]),
+        "oracle_range_prove": (ctypes.c_int, [ctypes.POINTER(RangeParams), vp, ctypes.POINTER(u8p),
+                                              ctypes.POINTER(sz)]),
+        "oracle_range_prove_with_columns": (ctypes.c_int, [ctypes.POINTER(RangeParams), u32p, sz, sz, vp,
+                                                           ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+        "oracle_range_prove_coeffs": (ctypes.c_int, [ctypes.POINTER(RangeParams), u32p, sz, sz, sz, vp,
+                                                     ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+        "oracle_range_prove_zk": (ctypes.c_int, [ctypes.POINTER(RangeParams), sz, vp, vp, ctypes.POINTER(u8p),
+                                                 ctypes.POINTER(sz)]),
+        "oracle_range_verify": (ctypes.c_int, [ctypes.POINTER(RangeParams), vp, u8p, sz]),
+        "oracle_range_composition_at": (ctypes.c_int, [ctypes.c_uint32, u32p, sz, ctypes.c_uint32, ctypes.c_uint32,
+                                                       ctypes.c_uint64, u32p, u32p]),
         "oracle_stage_timers_clear": (None, []),
         "oracle_stage_timers": (sz, [ctypes.c_char_p, sz]),
     }
@@ -558,3 +601,68 @@ def build_trace(a0, b0, n):
     b = np.zeros(n, dtype=np.uint32)
     lib().oracle_build_trace(a0, b0, n, _u32ptr(a), _u32ptr(b))
     return a, b
+
+
+# range.rs: the range-check AIR.
+def range_params_new(n_rows, n_bits, blowup, nq, final_size, value, grinding_bits=0, cap_height=0) -> RangeParams:
+    p = RangeParams()
+    rc = lib().oracle_range_params_new(n_rows, n_bits, blowup, nq, final_size, value, ctypes.byref(p))
+    if rc:
+        raise OracleError(rc)
+    p.grinding_bits, p.cap_height = grinding_bits, cap_height
+    return p
+
+
+def range_params_for_security(n_rows, n_bits, value, bits=100) -> RangeParams:
+    p = RangeParams()
+    rc = lib().oracle_range_params_for_security(n_rows, n_bits, value, bits, ctypes.byref(p))
+    if rc:
+        raise OracleError(rc)
+    return p
+
+
+def _range_out(call):
+    out = u8p()
+    n = ctypes.c_size_t()
+    rc = call(ctypes.byref(out), ctypes.byref(n))
+    if rc:
+        raise OracleError(rc)
+    return _take(out, n.value)
+
+
+def range_prove(params: RangeParams, transcript: Transcript) -> bytes:
+    return _range_out(lambda o, n: lib().oracle_range_prove(ctypes.byref(params), transcript.h, o, n))
+
+
+def range_prove_with_columns(params: RangeParams, columns, transcript: Transcript) -> bytes:
+    """columns: V then the bit columns, each a list of n_rows values."""
+    length = len(columns[0]) if columns else 0
+    flat = [v for c in columns for v in c]
+    return _range_out(lambda o, n: lib().oracle_range_prove_with_columns(
+        ctypes.byref(params), u32_array(flat), len(columns), length, transcript.h, o, n))
+
+
+def range_prove_coeffs(params: RangeParams, columns, degree_bound, transcript: Transcript) -> bytes:
+    length = len(columns[0]) if columns else 0
+    flat = [v for c in columns for v in c]
+    return _range_out(lambda o, n: lib().oracle_range_prove_coeffs(
+        ctypes.byref(params), u32_array(flat), len(columns), length, degree_bound, transcript.h, o, n))
+
+
+def range_prove_zk(params: RangeParams, n_rand: int, rng: ChaCha20, transcript: Transcript) -> bytes:
+    return _range_out(lambda o, n: lib().oracle_range_prove_zk(ctypes.byref(params), n_rand, rng.h, transcript.h,
+                                                               o, n))
+
+
+def range_verify(params: RangeParams, transcript: Transcript, proof: bytes) -> int:
+    return lib().oracle_range_verify(ctypes.byref(params), transcript.h, _buf(proof), len(proof))
+
+
+def range_composition_at(v, bits, point, value, n, alphas):
+    """alphas: n_bits + 2 Ext4 values as 4-tuples (decomposition, bits, boundary)."""
+    out = (ctypes.c_uint32 * 4)()
+    flat = [c for a in alphas for c in a]
+    rc = lib().oracle_range_composition_at(v, u32_array(bits), len(bits), point, value, n, u32_array(flat), out)
+    if rc:
+        raise OracleError(rc)
+    return tuple(out)
