@@ -179,6 +179,44 @@ int z0p_fri_commit_open_ext_evals(z0p_ctx* ctx, const uint32_t* evals4, size_t m
 int z0p_fri_verify(const uint8_t* commitment, size_t commitment_len, const uint8_t* proof, size_t proof_len,
                    const z0p_fri_params* params, z0p_transcript* transcript);
 
+/* ---- range-check AIR (src/range.rs) on the same device path: the value
+ * column V and n_bits bit columns extended on the 31-coset, one joint row
+ * tree (leaf = BLAKE3(0x00 || V || C_0 || ...) per LDE row), the composition
+ * of range.rs:767-823 committed by fri::commit_ext_evals ---- */
+/* range::Params (range.rs:29-52); value is reduced mod p. */
+typedef struct z0p_range_params {
+    uint64_t n_rows;
+    uint64_t n_bits;
+    uint64_t blowup;
+    uint64_t num_queries;
+    uint64_t final_size;
+    uint32_t grinding_bits;
+    uint32_t reserved0;
+    uint64_t cap_height;
+    uint32_t value;
+    uint32_t reserved1;
+} z0p_range_params;
+/* range::Params::new (range.rs:59-98): grinding 0, cap height 0. */
+int z0p_range_params_new(uint64_t n_rows, uint64_t n_bits, uint64_t blowup, uint64_t num_queries,
+                         uint64_t final_size, uint32_t value, z0p_range_params* out);
+/* range::Params::for_security (range.rs:124-158). */
+int z0p_range_params_for_security(uint64_t n_rows, uint64_t n_bits, uint32_t value, uint32_t target_bits,
+                                  z0p_range_params* out);
+/* range::prove (range.rs:334-342): constant columns of value and its bits. */
+int z0p_range_prove(z0p_ctx* ctx, const z0p_range_params* params, z0p_transcript* transcript, uint8_t** proof,
+                    size_t* proof_len);
+/* range::prove_with_columns (range.rs:357-382): ncols = n_bits + 1 columns of
+ * len = n_rows canonical values each, V first, contiguous. */
+int z0p_range_prove_with_columns(z0p_ctx* ctx, const z0p_range_params* params, const uint32_t* columns, size_t ncols,
+                                 size_t len, z0p_transcript* transcript, uint8_t** proof, size_t* proof_len);
+/* range::prove_zk (range.rs:401-428): every column masked with n_rand >=
+ * num_queries maskers from rng (V's first); degree bound n_rows + 2 n_rand - 2. */
+int z0p_range_prove_zk(z0p_ctx* ctx, const z0p_range_params* params, uint64_t n_rand, z0p_chacha20* rng,
+                       z0p_transcript* transcript, uint8_t** proof, size_t* proof_len);
+/* Proof::from_bytes + range::verify (range.rs:280-306, 521-721). Host only. */
+int z0p_range_verify(const z0p_range_params* params, z0p_transcript* transcript, const uint8_t* proof,
+                     size_t proof_len);
+
 /* ---- batched proving (no reference counterpart: the reference proves one
  * instance per air::prove call; this is `count` such calls at once) ---- */
 /* Device-memory budget of the batch and server schedulers on this ctx, in

@@ -294,3 +294,73 @@ def test_bench_tree_work_is_schedule_invariant(logn, cap_height):
         assert sum(comp.values()) == want
         tree_bytes = lambda a: sum(a.values()) - sum(alg0.values())
         assert tree_bytes(alg) == 0, sched
+
+
+@pytest.mark.parametrize("args", [(8, 16, 12_345, 12), (1 << 10, 8, 200, 100), (1 << 14, 24, 0xABCDEF, 100),
+                                  (2, 1, 1, 100), (1 << 20, 16, 7, 128), (3, 16, 1, 12), (8, 0, 1, 12),
+                                  (8, 25, 1, 12), (1 << 24, 8, 1, 100)])
+def test_range_params_match_oracle(args):
+    try:
+        want = o.range_params_for_security(*args)
+    except o.OracleError as e:
+        with pytest.raises(z.Z0pError) as ze:
+            z.range.for_security(*args)
+        assert ze.value.code == e.code
+        return
+    got = z.range.for_security(*args)
+    assert bytes(got) == bytes(want)
+
+
+def test_range_params_new_codes_match_oracle():
+    for args in [(8, 16, 8, 8, 4, 5), (3, 16, 8, 8, 4, 5), (8, 0, 8, 8, 4, 5), (8, 25, 8, 8, 4, 5),
+                 (8, 16, 1, 8, 4, 5), (8, 16, 8, 0, 4, 5), (8, 16, 8, 8, 1, 5), (1 << 40, 16, 4, 8, 4, 5),
+                 (8, 16, 2, 8, 4, P + 3)]:
+        try:
+            want = o.range_params_new(*args)
+        except o.OracleError as e:
+            with pytest.raises(z.Z0pError) as ze:
+                z.range.params_new(*args)
+            assert ze.value.code == e.code, args
+            continue
+        assert bytes(z.range.params_new(*args)) == bytes(want), args
+
+
+def test_range_verifier_matches_oracle():
+    # The host verifier accepts and rejects exactly the oracle's proofs of
+    # the reference's range tests (range.rs:785-1022).
+    def pair(n_rows, n_bits, blowup, nq, fs, value, **kw):
+        op = o.range_params_new(n_rows, n_bits, blowup, nq, fs, value).replace(**kw)
+        return op, z.range.params_new(n_rows, n_bits, blowup, nq, fs, value).replace(**kw)
+
+    def agree(op, zp, label, proof):
+        ok_o = o.range_verify(op, o.Transcript(label), proof)
+        try:
+            z.range.verify(zp, proof, z.Transcript(label))
+            ok_z = 0
+        except z.Z0pError as e:
+            ok_z = e.code
+        assert ok_z == ok_o, (label, ok_o, ok_z)
+        return ok_o
+
+    op, zp = pair(8, 16, 8, 8, 4, 12_345)
+    pr = o.range_prove(op, o.Transcript(b"bench/range"))
+    assert agree(op, zp, b"bench/range", pr) == 0
+    assert agree(op, zp, b"bench/range", pr[:-1]) == 8
+    assert agree(op, zp, b"other", pr) != 0
+    for off in range(5, len(pr), 97):
+        bad = bytearray(pr)
+        bad[off] ^= 0x10
+        agree(op, zp, b"bench/range", bytes(bad))
+    cp, zc = pair(8, 16, 8, 8, 4, 12_345, cap_height=2, grinding_bits=5)
+    prc = o.range_prove(cp, o.Transcript(b"cap"))
+    assert agree(cp, zc, b"cap", prc) == 0
+    assert agree(op, zp, b"cap", prc) != 0
+    oop, ozp = pair(8, 16, 8, 8, 4, 70_000)
+    assert agree(oop, ozp, b"oor", o.range_prove(oop, o.Transcript(b"oor"))) != 0
+    assert agree(op.replace(value=1), zp.replace(value=1), b"bench/range", pr) != 0
+    kp, kz = pair(8, 16, 8, 8, 8, 999)
+    zk = o.range_prove_zk(kp, 8, o.ChaCha20(bytes([33] * 32), bytes([44] * 12)), o.Transcript(b"zk"))
+    assert agree(kp, kz, b"zk", zk) == 0
+    down = o.range_prove_coeffs(op, [[12_345] + [0] * 7] + [[(12_345 >> j) & 1] + [0] * 7 for j in range(16)], 8,
+                                o.Transcript(b"down"))
+    assert agree(op, zp, b"down", down) != 0

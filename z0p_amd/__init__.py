@@ -1,9 +1,10 @@
-"""z0p-mi355x: MI355X (gfx950) drop-in for the z0p Fibonacci AIR prover and
-FRI polynomial commitment.
+"""z0p-mi355x: MI355X (gfx950) drop-in for the z0p Fibonacci AIR prover, the
+range-check AIR and the FRI polynomial commitment.
 
-Mirrors the reference crate's public API (src/air.rs, src/fri.rs,
-src/transcript.rs): ``Transcript``, ``air.Params``/``air.prove``/
-``air.verify`` and ``fri.Params``/``fri.commit_open``/``fri.verify``. Proofs
+Mirrors the reference crate's public API (src/air.rs, src/range.rs,
+src/fri.rs, src/transcript.rs): ``Transcript``, ``air.Params``/``air.prove``/
+``air.verify``, ``range.Params``/``range.prove``/``range.verify`` and
+``fri.Params``/``fri.commit_open``/``fri.verify``. Proofs
 are the reference's canonical VERSION = 6 bytes. Proving runs on the GPU
 through libz0p_amd.so; verification is host code, as in the reference.
 """
@@ -11,7 +12,7 @@ import ctypes
 
 from ._ffi import Z0pError, buf, check, lib, take
 
-__all__ = ["Z0pError", "Transcript", "ChaCha20", "Context", "Server", "air", "fri", "default_context"]
+__all__ = ["Z0pError", "Transcript", "ChaCha20", "Context", "Server", "air", "fri", "range", "default_context"]
 
 
 class Transcript:
@@ -223,4 +224,4 @@ def default_context() -> Context:
     return _default
 
 
-from . import air, fri  # noqa: E402
+from . import air, fri, range  # noqa: E402

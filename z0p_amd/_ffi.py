@@ -50,6 +50,13 @@ class AirParamsC(ctypes.Structure):
     ]
 
 
+class RangeParamsC(ctypes.Structure):
+    _fields_ = [
+        ("n_rows", u64), ("n_bits", u64), ("blowup", u64), ("num_queries", u64), ("final_size", u64),
+        ("grinding_bits", u32), ("reserved0", u32), ("cap_height", u64), ("value", u32), ("reserved1", u32),
+    ]
+
+
 class FriParamsC(ctypes.Structure):
     _fields_ = [
         ("blowup", u64), ("num_queries", u64), ("final_size", u64),
@@ -106,6 +113,15 @@ SIGNATURES = {
     "z0p_chacha20_next_field": (u32, [vp]),
     "z0p_air_prove_zk": (ctypes.c_int, [vp, ctypes.POINTER(AirParamsC), u64, vp, vp, ctypes.POINTER(u8p),
                                         ctypes.POINTER(sz)]),
+    "z0p_range_params_new": (ctypes.c_int, [u64, u64, u64, u64, u64, u32, ctypes.POINTER(RangeParamsC)]),
+    "z0p_range_params_for_security": (ctypes.c_int, [u64, u64, u32, u32, ctypes.POINTER(RangeParamsC)]),
+    "z0p_range_prove": (ctypes.c_int, [vp, ctypes.POINTER(RangeParamsC), vp, ctypes.POINTER(u8p),
+                                       ctypes.POINTER(sz)]),
+    "z0p_range_prove_with_columns": (ctypes.c_int, [vp, ctypes.POINTER(RangeParamsC), u32p, sz, sz, vp,
+                                                    ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+    "z0p_range_prove_zk": (ctypes.c_int, [vp, ctypes.POINTER(RangeParamsC), u64, vp, vp, ctypes.POINTER(u8p),
+                                          ctypes.POINTER(sz)]),
+    "z0p_range_verify": (ctypes.c_int, [ctypes.POINTER(RangeParamsC), vp, u8p, sz]),
     "z0p_fri_commit_open": (ctypes.c_int, [vp, u32p, sz, ctypes.POINTER(FriParamsC), vp, ctypes.POINTER(u8p),
                                            ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
     "z0p_fri_verify": (ctypes.c_int, [u8p, sz, u8p, sz, ctypes.POINTER(FriParamsC), vp]),

@@ -292,6 +292,90 @@ int z0p_air_prove_zk(z0p_ctx* ctx, const z0p_air_params* params, uint64_t n_rand
     return g != Z0P_OK ? g : rc;
 }
 
+int z0p_range_params_new(uint64_t n_rows, uint64_t n_bits, uint64_t blowup, uint64_t num_queries,
+                         uint64_t final_size, uint32_t value, z0p_range_params* out) {
+    if (!out) return Z0P_ERR_INVALID_ARGUMENT;
+    using namespace z0p;
+    if (n_rows < 2 || !pow2(n_rows)) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    if (n_bits == 0 || n_bits > 24) return Z0P_ERR_MALFORMED_INPUT;
+    if (blowup < 2 || !pow2(blowup)) return Z0P_ERR_INVALID_BLOWUP;
+    if (num_queries == 0) return Z0P_ERR_NO_QUERIES;
+    if (final_size < 2 || !pow2(final_size)) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    if (n_rows > MAX_DOMAIN || blowup > MAX_DOMAIN || n_rows * blowup > MAX_DOMAIN) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    std::memset(out, 0, sizeof(*out));
+    out->n_rows = n_rows;
+    out->n_bits = n_bits;
+    out->blowup = blowup;
+    out->num_queries = num_queries;
+    out->final_size = final_size;
+    out->value = value % P;
+    return Z0P_OK;
+}
+
+int z0p_range_params_for_security(uint64_t n_rows, uint64_t n_bits, uint32_t value, uint32_t target_bits,
+                                  z0p_range_params* out) {
+    if (!out) return Z0P_ERR_INVALID_ARGUMENT;
+    using namespace z0p;
+    if (n_rows < 2 || !pow2(n_rows)) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    if (n_bits == 0 || n_bits > 24) return Z0P_ERR_MALFORMED_INPUT;
+    const uint64_t blowup = 16;
+    if (n_rows > MAX_DOMAIN / blowup) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    const uint64_t lde = n_rows * blowup;
+    auto strength = tune_strength(target_bits, composition_millibits(lde, n_rows - 1));
+    // one column opening: 8 index bytes plus 4 per column
+    auto shape = tune_shape(lde, strength.first, true, lde, 8 + 4 * (n_bits + 1));
+    std::memset(out, 0, sizeof(*out));
+    out->n_rows = n_rows;
+    out->n_bits = n_bits;
+    out->blowup = blowup;
+    out->num_queries = strength.first;
+    out->final_size = shape.first;
+    out->grinding_bits = strength.second;
+    out->cap_height = shape.second;
+    out->value = value % P;
+    return Z0P_OK;
+}
+
+int z0p_range_prove(z0p_ctx* ctx, const z0p_range_params* params, z0p_transcript* transcript, uint8_t** proof,
+                    size_t* proof_len) {
+    if (!ctx || !params || !transcript || !proof || !proof_len) return Z0P_ERR_INVALID_ARGUMENT;
+    int rc = Z0P_OK;
+    int g = guarded([&] {
+        z0p::Bytes bytes = z0p::range_prove(*ctx->c, *params, transcript->t);
+        rc = emit(bytes, proof, proof_len);
+    });
+    return g != Z0P_OK ? g : rc;
+}
+
+int z0p_range_prove_with_columns(z0p_ctx* ctx, const z0p_range_params* params, const uint32_t* columns, size_t ncols,
+                                 size_t len, z0p_transcript* transcript, uint8_t** proof, size_t* proof_len) {
+    if (!ctx || !params || !transcript || !proof || !proof_len || (!columns && ncols && len))
+        return Z0P_ERR_INVALID_ARGUMENT;
+    int rc = Z0P_OK;
+    int g = guarded([&] {
+        z0p::Bytes bytes = z0p::range_prove_with_columns(*ctx->c, *params, columns, ncols, len, transcript->t);
+        rc = emit(bytes, proof, proof_len);
+    });
+    return g != Z0P_OK ? g : rc;
+}
+
+int z0p_range_prove_zk(z0p_ctx* ctx, const z0p_range_params* params, uint64_t n_rand, z0p_chacha20* rng,
+                       z0p_transcript* transcript, uint8_t** proof, size_t* proof_len) {
+    if (!ctx || !params || !rng || !transcript || !proof || !proof_len) return Z0P_ERR_INVALID_ARGUMENT;
+    int rc = Z0P_OK;
+    int g = guarded([&] {
+        z0p::Bytes bytes = z0p::range_prove_zk(*ctx->c, *params, n_rand, *rng->r, transcript->t);
+        rc = emit(bytes, proof, proof_len);
+    });
+    return g != Z0P_OK ? g : rc;
+}
+
+int z0p_range_verify(const z0p_range_params* params, z0p_transcript* transcript, const uint8_t* proof,
+                     size_t proof_len) {
+    if (!params || !transcript || (!proof && proof_len)) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] { z0p::range_verify(*params, transcript->t, proof, proof_len); });
+}
+
 z0p_chacha20* z0p_chacha20_new(const uint8_t key[32], const uint8_t nonce[12]) {
     if (!key || !nonce) return nullptr;
     auto* h = new (std::nothrow) z0p_chacha20;

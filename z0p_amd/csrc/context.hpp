@@ -269,7 +269,7 @@ struct Ctx {
     // Mailbox for pre-enqueued phases (pinned host memory, device-mapped):
     // word 0 = gate counter the device polls, word 1 = gate timeout flag,
     // then MAILBOX_SLOTS constant slots of MAILBOX_SLOT bytes.
-    static constexpr size_t MAILBOX_SLOT = 256, MAILBOX_SLOTS = 32, MAILBOX_HEAD = 256;
+    static constexpr size_t MAILBOX_SLOT = 512, MAILBOX_SLOTS = 32, MAILBOX_HEAD = 256;
     uint8_t* mailbox_host = nullptr;
     uint8_t* mailbox_dev = nullptr;
     uint64_t gate = 0;

@@ -671,6 +671,58 @@ __global__ __launch_bounds__(WG) void k_compose(ComposeJob inl, const ComposeJob
     });
 }
 
+// Range-check composition per point (range.rs:767-823), as FRI layer 0.
+// 1/Z = 1/(s_j^n - 1) is constant per coset; 1/(x - 1) is batch-inverted.
+// The booleanity terms share 1/Z: sum_c a_c [C_c (C_c - 1) / R] is summed in
+// pairs, then multiplied once by 1/Z in the R^2 domain.
+__global__ __launch_bounds__(WG) void k_range_compose(RangeComposeJob inl, const RangeComposeJob* tab,
+                                                      uint32_t njobs) {
+    uint32_t local;
+    const RangeComposeJob J = job_of(inl, tab, njobs, local);
+    const uint32_t m = 1u << (J.logn + J.logb);
+    const uint32_t base = local * (WG * CELT) + threadIdx.x;
+    const gcptr cols = gc(J.cols);
+    const RangeConsts* C = J.c;
+    const uint32_t nb = C->nbits, value = C->value;
+    uint32_t d1[CELT];
+#pragma unroll
+    for (int e = 0; e < CELT; ++e) {
+        const uint32_t p = base + e * WG;
+        if (p < m) {
+            const uint32_t j = brev_bits(p >> J.logn, J.logb);
+            const uint32_t x = mont_mul(gc(J.xn)[p & ((1u << J.logn) - 1)], gc(J.shift_m)[j]);
+            d1[e] = sub(x, R1);
+        } else {
+            d1[e] = R1;
+        }
+    }
+    batch_inv_m<CELT>(d1);
+#pragma unroll 1
+    for (int e = 0; e < CELT; ++e) {
+        const uint32_t p = base + e * WG;
+        if (p >= m) break;
+        const uint32_t j = brev_bits(p >> J.logn, J.logb);
+        const uint32_t iz = gc(J.zinv_m)[j];
+        const uint32_t v = cols[p];
+        uint32_t rec = 0, acc[4] = {0, 0, 0, 0};
+        for (uint32_t c = nb; c-- > 0;) {
+            const uint32_t b = cols[uint64_t(c + 1) * J.col_stride + p];
+            rec = add(dbl(rec), b);
+            const uint32_t q = mont_mul(b, sub(b, 1u));  // C (C - 1) / R
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[k] = add(acc[k], mont_mul(q, C->alpha_m[1 + c][k]));
+        }
+        const uint32_t izz = to_mont(iz);  // 1/Z in the R^2 domain
+        const uint32_t sd = mont_mul(sub(v, rec), iz);
+        const uint32_t sb = mont_mul(sub(v, value), d1[e]);
+        uint32_t o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            o[k] = add(mont_mul2(sd, C->alpha_m[0][k], sb, C->alpha_m[nb + 1][k]), mont_mul(acc[k], izz));
+        st4(J.layer0, p, u32x4{o[0], o[1], o[2], o[3]});
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Merkle leaves: thread g hashes leaves 2g and 2g+1 and, when asked, their
 // parent (one node of level 1). No LDS: wide levels run at full occupancy.
@@ -802,11 +854,65 @@ __device__ __forceinline__ void leaves_linked(const LeavesJob& J, uint64_t tau) 
     }
 }
 
+// Range AIR row leaf: BLAKE3(0x00 || w_0 || .. || w_{C-1}) over the C column
+// words at LDE position p, 4C + 1 bytes (one block for C <= 15, two above).
+// The 0x00 prefix shifts every word by one byte: message word k is
+// (w_{k-1} >> 24) | (w_k << 8).
+constexpr int ROW_MAX_COLS = RANGE_MAX_BITS + 1;
+__device__ __forceinline__ void row_leaf(const LeavesJob& J, uint64_t p, uint32_t (&d)[8]) {
+    const int C = J.ncols;
+    uint32_t w[ROW_MAX_COLS];
+#pragma unroll
+    for (int c = 0; c < ROW_MAX_COLS; ++c) w[c] = c < C ? gc(J.src)[uint64_t(c) * J.col_stride + p] : 0u;
+    uint32_t m[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+        const uint32_t lo = k >= 1 && k - 1 < ROW_MAX_COLS ? w[k - 1] >> 24 : 0u;
+        const uint32_t hi = k < ROW_MAX_COLS ? w[k] << 8 : 0u;
+        m[k] = lo | hi;
+    }
+    const uint32_t len = 4u * uint32_t(C) + 1u;
+    uint32_t iv[8];
+    b3_iv(iv);
+    if (len <= 64) {
+        compress8(iv, m, 0, 0, len, b3::CHUNK_START | b3::CHUNK_END | b3::ROOT, d);
+    } else {
+        uint32_t cv[8];
+        compress8(iv, m, 0, 0, 64, b3::CHUNK_START, cv);
+        compress8(cv, m + 16, 0, 0, len - 64, b3::CHUNK_END | b3::ROOT, d);
+    }
+}
+
+// Thread tau < m/2 hashes the rows at positions tau and tau + m/2: with
+// p = brev(j) n + brev(t) and B >= 2 cosets, they are natural rows
+// i = j + B t (j even) and i + 1, a leaf pair, so the thread also forms
+// their parent. Column loads are coalesced; digests land at natural indices.
+__device__ __forceinline__ void leaves_rows(const LeavesJob& J, uint64_t tau) {
+    const uint64_t half = J.leaves >> 1;
+    if (tau >= half) return;
+    const int logn = J.logn, logb = J.logb;
+    const uint32_t nmask = (1u << logn) - 1;
+    const uint32_t j = brev_bits(uint32_t(tau >> logn), logb);
+    const uint32_t t = brev_bits(uint32_t(tau) & nmask, logn);
+    const uint64_t i = uint64_t(j) + (uint64_t(t) << logb);
+    uint32_t d0[8], d1[8];
+    row_leaf(J, tau, d0);
+    row_leaf(J, tau + half, d1);
+    store_digest(J.levels + i * 8, d0);
+    store_digest(J.levels + (i + 1) * 8, d1);
+    if (J.with_parent) {
+        uint32_t p[8];
+        node_words(d0, d1, p);
+        store_digest(J.levels + (J.leaves + i / 2) * 8, p);
+    }
+}
+
 __global__ __launch_bounds__(WG) void k_leaves(LeavesJob inl, const LeavesJob* tab, uint32_t njobs) {
     uint32_t local;
     const LeavesJob J = job_of(inl, tab, njobs, local);
     const uint64_t tau = uint64_t(local) * WG + threadIdx.x;
     if (J.kind == LEAF_TRACE_LINKED) leaves_linked(J, tau);
+    else if (J.kind == LEAF_ROWS) leaves_rows(J, tau);
     else if (J.kind == LEAF_TRACE) leaves_body<LEAF_TRACE>(J, tau);
     else if (J.kind == LEAF_BASE) leaves_body<LEAF_BASE>(J, tau);
     else leaves_body<LEAF_EXT>(J, tau);
@@ -1544,13 +1650,23 @@ void launch_compose(const Launch& L, std::vector<ComposeJob> all) {
     }
 }
 
+void launch_range_compose(const Launch& L, std::vector<RangeComposeJob> jobs) {
+    unsigned g = assign_blocks(jobs, [](const RangeComposeJob& j) {
+        return grid_for(uint64_t(1) << (j.logn + j.logb), WG * CELT);
+    });
+    run(L, "range_compose", k_range_compose, jobs, g, 0);
+}
+
 void launch_leaves(const Launch& L, std::vector<LeavesJob> jobs) {
-    bool trace = false;
-    for (const LeavesJob& j : jobs) trace = trace || j.kind == LEAF_TRACE || j.kind == LEAF_TRACE_LINKED;
+    bool trace = false, rows = false;
+    for (const LeavesJob& j : jobs) {
+        trace = trace || j.kind == LEAF_TRACE || j.kind == LEAF_TRACE_LINKED;
+        rows = rows || j.kind == LEAF_ROWS;
+    }
     unsigned g = assign_blocks(jobs, [](const LeavesJob& j) {
         return j.leaves == 1 ? 1u : grid_for(j.leaves / 2, WG);
     });
-    run(L, trace ? "trace_leaves" : "fri_leaves", k_leaves, jobs, g, 0);
+    run(L, rows ? "row_leaves" : trace ? "trace_leaves" : "fri_leaves", k_leaves, jobs, g, 0);
 }
 
 void launch_levels(const Launch& L, std::vector<LevelsJob> jobs) {
@@ -1680,8 +1796,17 @@ __device__ const uint32_t* open_src(const OpenJob& J, uint64_t leaf, uint32_t w)
         w -= pw;
         index /= 4;
     }
-    // AIR trace rows: leaves of natural rows idx and idx + B, then their paths.
     const int logm = J.logn + J.logb;
+    if (J.cols) {
+        // Range rows: the columns at the query's position (natural row
+        // brev(leaf)), then the row's path in the one-leaf-per-row tree.
+        if (w < uint32_t(J.ncols)) return J.cols + uint64_t(w) * J.col_stride + leaf;
+        w -= uint32_t(J.ncols);
+        const uint64_t row = brev_bits(uint32_t(leaf), logm);
+        const int l = int(w >> 3);
+        return J.trace_tree + level_offset_words(J.trace_leaves, l) + (((row >> l) ^ 1) * 8) + (w & 7);
+    }
+    // AIR trace rows: leaves of natural rows idx and idx + B, then their paths.
     const uint64_t m = uint64_t(1) << logm, B = uint64_t(1) << J.logb;
     const uint64_t idx = brev_bits(uint32_t(leaf), logm);
     const uint64_t shifted = (idx + B) & (m - 1);

@@ -131,12 +131,36 @@ struct ComposeJob {
     int with_parent = 0;
 };
 
+// Range-check composition (range.rs:767-823), written as FRI layer 0:
+//   a_d (V - sum_c 2^c C_c)/Z + sum_c a_c C_c (C_c - 1)/Z + a_b (V - value)/(x - 1)
+// over the columns V, C_0..C_{nbits-1} (bit-reversed LDEs, col_stride apart).
+constexpr int RANGE_MAX_BITS = 24;
+struct RangeConsts {
+    uint32_t value, nbits;
+    uint32_t alpha_m[RANGE_MAX_BITS + 2][4];  // decomposition, bits, boundary (Montgomery)
+};
+struct RangeComposeJob {
+    const uint32_t* cols;
+    uint64_t col_stride;
+    uint32_t* layer0;
+    const uint32_t* xn;      // [n] omega_n^brev(q), Montgomery
+    const uint32_t* shift_m;
+    const uint32_t* zinv_m;  // [B] 1/(s_j^n - 1)
+    const RangeConsts* c;    // device memory: the alphas come from the transcript
+    int logn, logb;
+    uint32_t block_begin;
+};
+void launch_range_compose(const Launch& L, std::vector<RangeComposeJob> jobs);
+
 // Merkle leaves. Kinds: trace rows (8 joint A||B rows per leaf from the
 // bit-reversed columns), FRI Ext4 quads, FRI base-field quads (upper
 // coordinates zero), raw 64-byte leaves.
 // LEAF_TRACE_LINKED: as LEAF_TRACE, but column B is computed from A in the
 // same pass (the linked column, air.rs:806-844) and written to b_out.
-enum LeafKind : int { LEAF_TRACE = 0, LEAF_EXT = 1, LEAF_BASE = 2, LEAF_RAW = 3, LEAF_TRACE_LINKED = 4 };
+// LEAF_ROWS: one leaf per LDE row of ncols bit-reversed columns (col_stride
+// words apart), BLAKE3(0x00 || row words LE) at the row's natural index
+// (merkle.rs:62-79 with hash.rs:352-382): the range AIR's joint row tree.
+enum LeafKind : int { LEAF_TRACE = 0, LEAF_EXT = 1, LEAF_BASE = 2, LEAF_RAW = 3, LEAF_TRACE_LINKED = 4, LEAF_ROWS = 5 };
 struct LeavesJob {
     const uint32_t* src;
     const uint32_t* src_b;  // trace column B
@@ -153,6 +177,9 @@ struct LeavesJob {
     const uint32_t* zvan_m = nullptr;   // [B] s_j^n - 1, Montgomery
     uint32_t coeff = 0;                 // (b_last - a0) w_last / n, canonical
     uint32_t w_last_m = 0;
+    // LEAF_ROWS: src holds ncols columns of col_stride words each.
+    int ncols = 0;
+    uint64_t col_stride = 0;
 };
 
 // One or two parent levels from a level of `count` digests, one node (or
@@ -344,6 +371,11 @@ struct OpenJob {
     uint64_t trace_leaves;
     int trace_target;
     int logn, logb;
+    // Range AIR rows (cols != nullptr, instead of a/b): ncols words at the
+    // query's position, then the row tree path (trace_tree, one leaf per row).
+    const uint32_t* cols = nullptr;
+    int ncols = 0;
+    uint64_t col_stride = 0;
     const uint64_t* queries;
     uint32_t nq;
     uint32_t words_per_query;

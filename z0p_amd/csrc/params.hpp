@@ -124,6 +124,32 @@ inline z0p_fri_params air_fri_params(const z0p_air_params& a) {
     return f;
 }
 
+// range::Params::validate (range.rs:182-215).
+inline void validate_range(const z0p_range_params& p) {
+    require(p.n_rows >= 2 && pow2(p.n_rows), Z0P_ERR_INVALID_DOMAIN_SIZE);
+    require(p.n_bits >= 1 && p.n_bits <= 24, Z0P_ERR_MALFORMED_INPUT);
+    require(p.blowup >= 2 && pow2(p.blowup), Z0P_ERR_INVALID_BLOWUP);
+    require(p.num_queries != 0, Z0P_ERR_NO_QUERIES);
+    require(p.final_size >= 2 && pow2(p.final_size), Z0P_ERR_INVALID_DOMAIN_SIZE);
+    require(p.cap_height <= 8, Z0P_ERR_MALFORMED_INPUT);
+    require(p.n_rows <= MAX_DOMAIN && p.blowup <= MAX_DOMAIN && p.n_rows * p.blowup <= MAX_DOMAIN,
+            Z0P_ERR_INVALID_DOMAIN_SIZE);
+}
+
+// range::Params::fri_params (range.rs:172-179).
+inline z0p_fri_params range_fri_params(const z0p_range_params& a) {
+    require(a.blowup >= 2 && pow2(a.blowup), Z0P_ERR_INVALID_BLOWUP);
+    require(a.num_queries != 0, Z0P_ERR_NO_QUERIES);
+    require(a.final_size >= 2 && pow2(a.final_size) && a.final_size <= MAX_FINAL_SIZE, Z0P_ERR_INVALID_DOMAIN_SIZE);
+    z0p_fri_params f{};
+    f.blowup = a.blowup;
+    f.num_queries = a.num_queries;
+    f.final_size = a.final_size;
+    f.grinding_bits = a.grinding_bits;
+    f.cap_height = a.cap_height;
+    return f;
+}
+
 // FRI layer-shape checks shared by commit (fri.rs:556-569, 1125-1140).
 inline size_t fri_num_folds(uint64_t lde_len, uint64_t final_size, uint64_t degree_bound) {
     require(final_size < lde_len && lde_len % final_size == 0, Z0P_ERR_INVALID_DOMAIN_SIZE);

@@ -1,5 +1,5 @@
-// Host orchestration of the GPU prover: Fibonacci AIR (air.rs:365-599) and
-// the FRI commitment (fri.rs:408-666).
+// Host orchestration of the GPU prover: Fibonacci AIR (air.rs:365-599), the
+// range-check AIR (range.rs:334-508) and the FRI commitment (fri.rs:408-666).
 //
 // Every proof is a small state machine whose phases are cut by the
 // Fiat-Shamir transcript: the trace cap gates the composition challenges
@@ -304,6 +304,7 @@ struct Buckets {
     std::vector<NttJob> ntt;
     std::vector<LinkedJob> linked;
     std::vector<ComposeJob> compose;
+    std::vector<RangeComposeJob> rcompose;
     std::vector<LeavesJob> leaves;
     std::vector<LevelsJob> levels;
     std::vector<TailJob> tail;
@@ -318,6 +319,7 @@ struct Buckets {
     std::vector<NttJob>& of(const NttJob&) { return ntt; }
     std::vector<LinkedJob>& of(const LinkedJob&) { return linked; }
     std::vector<ComposeJob>& of(const ComposeJob&) { return compose; }
+    std::vector<RangeComposeJob>& of(const RangeComposeJob&) { return rcompose; }
     std::vector<LeavesJob>& of(const LeavesJob&) { return leaves; }
     std::vector<LevelsJob>& of(const LevelsJob&) { return levels; }
     std::vector<TailJob>& of(const TailJob&) { return tail; }
@@ -336,6 +338,7 @@ struct Buckets {
         if (!interleave.empty()) launch_interleave(L, interleave);
         if (!linked.empty()) launch_linked(L, linked);
         if (!compose.empty()) launch_compose(L, compose);
+        if (!rcompose.empty()) launch_range_compose(L, rcompose);
         if (!leaves.empty()) launch_leaves(L, leaves);
         if (!levels.empty()) launch_levels(L, levels);
         if (!tail.empty()) launch_tail(L, tail);
@@ -449,16 +452,19 @@ enum Phase : int { PH_FRONT = 0, PH_COMPOSE, PH_FOLD, PH_GRIND, PH_OPEN, PH_DONE
 struct Proof {
     // request
     bool air = true;
+    bool range = false;  // the range-check AIR (air == false)
     z0p_air_params ap{};
+    z0p_range_params rp{};
     z0p_fri_params fp{};
     Transcript* tr = nullptr;
     const uint32_t* coeff_src = nullptr;  // FRI: host coefficients or LDE values
     int fri_in = FRI_COEFFS;
     // Layer 0 holds base-field values (4 bytes each): the FRI commitments
     // of base coefficients or base LDE values.
-    bool base0() const { return !air && (fri_in == FRI_COEFFS || fri_in == FRI_EVALS); }
+    bool base0() const { return !air && !range && (fri_in == FRI_COEFFS || fri_in == FRI_EVALS); }
     uint64_t src_words() const {
         if (air) return n;
+        if (range) return host_a ? ncols * n : 0;
         return (fri_in == FRI_COEFFS_EXT || fri_in == FRI_EVALS_EXT ? 4 : 1) * n;
     }
     // AIR with both columns interpolated and extended (air.rs:385-406,
@@ -467,6 +473,13 @@ struct Proof {
     const uint32_t *host_a = nullptr, *host_b = nullptr;  // explicit trace columns
     std::vector<uint32_t> mask;                           // hiding: A's n_rand maskers, then B's
     uint64_t n_rand = 0;
+    // range AIR: n_bits + 1 columns, V first; the constant columns of
+    // range::prove as their constant coefficients (host_a == nullptr), or
+    // the caller's evaluation columns in host_a.
+    uint64_t ncols = 0;
+    std::vector<uint32_t> const_cols;
+    uint32_t* rows = nullptr;  // ncols bit-reversed LDE columns, m words apart
+    RangeConsts rc{};
     uint64_t tag = 0;
     // geometry
     uint64_t n = 0, B = 0, m = 0, poly_len = 0, degree_bound = 0;
@@ -519,8 +532,15 @@ size_t layout(Proof& P, uint8_t* base) {
         return p;
     };
     P.col = take(P.src_words());
-    P.coeff = take(!P.air && P.fri_in == FRI_COEFFS_EXT ? 4 * P.n : P.n);
-    if (!P.air && P.fri_in == FRI_COEFFS_EXT) P.planar = take(4 * P.m);
+    if (P.range) {
+        P.coeff = take(P.ncols * P.n);
+        if (P.n_rand) P.mask_dev = take(P.ncols * P.n_rand);
+        P.rows = take(P.ncols * P.m);
+        P.trace_tree = take(P.tg.words());
+    } else {
+        P.coeff = take(!P.air && P.fri_in == FRI_COEFFS_EXT ? 4 * P.n : P.n);
+    }
+    if (!P.air && !P.range && P.fri_in == FRI_COEFFS_EXT) P.planar = take(4 * P.m);
     if (P.two_col) {
         P.colB = take(P.n);
         P.coeffB = take(P.n);
@@ -557,6 +577,7 @@ void init_fri_shape(Proof& P, const z0p_fri_params& fp) {
     for (size_t d = 0; d < P.folds; ++d) w += (d == 0 && P.base0() ? 4 : 16) + 8 * uint32_t(P.geoms[d].target);
     P.fri_wpq = w;
     if (P.air) w += 32 + 16 * uint32_t(P.tg.target);
+    if (P.range) w += uint32_t(P.ncols) + 8 * uint32_t(P.tg.target);
     P.wpq = w;
     P.stage_need = size_t(fp.num_queries) * P.wpq + 256 * 8 + P.lens[P.folds] * 4 + 256;
 }
@@ -627,6 +648,77 @@ void init_air_zk(Ctx& c, Proof& P, const z0p_air_params& p, uint64_t n_rand, Cha
     init_air_shape(c, P, p, tr, padded * p.blowup, n + n_rand - 1);
 }
 
+// Range AIR geometry over an m-point LDE (m / n cosets of n points), one
+// tree leaf per LDE row, composition degree below degree_bound + 1.
+void init_range_shape(Ctx& c, Proof& P, const z0p_range_params& p, Transcript* tr, uint64_t m, uint64_t degree_bound) {
+    const z0p_fri_params fp = range_fri_params(p);
+    validate_fri(fp);
+    P.air = false;
+    P.range = true;
+    P.rp = p;
+    P.rp.value = p.value % P_MOD;
+    P.tr = tr;
+    P.n = p.n_rows;
+    P.ncols = p.n_bits + 1;
+    P.m = m;
+    P.B = m / P.n;
+    P.logn = int(ilog2(P.n));
+    P.logb = int(ilog2(P.B));
+    P.poly_len = P.m;
+    P.degree_bound = degree_bound;
+    P.tg = tree_geom(P.m, p.cap_height);
+    init_fri_shape(P, fp);
+    P.rc.value = P.rp.value;
+    P.rc.nbits = uint32_t(p.n_bits);
+    P.gt = &c.geometry(P.logn, P.logb);
+    P.mem_bytes = layout(P, nullptr);
+}
+
+// range::build_columns (range.rs:312-327) as coefficients: V = value and
+// C_j = bit j of value interpolate to constants.
+void range_const_cols(Proof& P, uint32_t value, uint64_t n_bits) {
+    P.const_cols.assign(n_bits + 1, 0);
+    P.const_cols[0] = value;
+    for (uint64_t j = 0; j < n_bits; ++j) P.const_cols[j + 1] = (value >> j) & 1u;
+}
+
+// range::prove (range.rs:334-342).
+void init_range(Ctx& c, Proof& P, const z0p_range_params& p, Transcript* tr) {
+    validate_range(p);
+    range_const_cols(P, p.value % P_MOD, p.n_bits);
+    init_range_shape(c, P, p, tr, p.n_rows * p.blowup, p.n_rows - 1);
+}
+
+// range::prove_with_columns (range.rs:357-382): ncols columns of len
+// canonical values, V first.
+void init_range_cols(Ctx& c, Proof& P, const z0p_range_params& p, const uint32_t* cols, uint64_t ncols, uint64_t len,
+                     Transcript* tr) {
+    validate_range(p);
+    require(ncols == p.n_bits + 1 && len == p.n_rows, Z0P_ERR_MALFORMED_INPUT);
+    for (uint64_t i = 0; i < ncols * len; ++i) require(cols[i] < P_MOD, Z0P_ERR_MALFORMED_INPUT);
+    P.host_a = cols;
+    init_range_shape(c, P, p, tr, p.n_rows * p.blowup, p.n_rows - 1);
+}
+
+// range::prove_zk (range.rs:401-428): each column masked by
+// blind::mask_coeffs (blind.rs:38-79), V's maskers first. The checks and the
+// rng draws come in the reference's order.
+void init_range_zk(Ctx& c, Proof& P, const z0p_range_params& p, uint64_t n_rand, ChaCha20& rng, Transcript* tr) {
+    validate_range(p);
+    require(n_rand >= p.num_queries, Z0P_ERR_MALFORMED_INPUT);
+    const uint64_t n = p.n_rows;
+    require(n_rand <= MAX_DOMAIN, Z0P_ERR_INVALID_DOMAIN_SIZE);
+    uint64_t padded = 1;
+    while (padded < n + n_rand) padded <<= 1;
+    require(padded <= MAX_DOMAIN, Z0P_ERR_INVALID_DOMAIN_SIZE);
+    P.mask.resize((p.n_bits + 1) * n_rand);
+    for (auto& v : P.mask) v = rng.next_field();
+    require(padded <= MAX_DOMAIN / p.blowup, Z0P_ERR_INVALID_DOMAIN_SIZE);  // evaluate_on_coset (ntt.rs:341-352)
+    range_const_cols(P, p.value % P_MOD, p.n_bits);
+    P.n_rand = n_rand;
+    init_range_shape(c, P, p, tr, padded * p.blowup, n + 2 * n_rand - 2);
+}
+
 // fri::commit / commit_ext (fri.rs:408-461): count coefficients, extended
 // by the blowup. fri::commit_evals / commit_ext_evals (fri.rs:473-549): count
 // LDE values with a declared degree bound. Checks in the reference's order.
@@ -684,6 +776,29 @@ void start(Ctx& c, Proof& P) {
         }
         if (P.n_rand)
             hip_check(hipMemcpyAsync(P.mask_dev, P.mask.data(), P.mask.size() * 4, hipMemcpyHostToDevice, c.stream));
+    } else if (P.range) {
+        // absorb_params (range.rs:723-735): 56 bytes, grinding_bits as u32.
+        const z0p_range_params& p = P.rp;
+        uint8_t pb[56];
+        store_le64(pb, p.n_rows);
+        store_le64(pb + 8, p.n_bits);
+        store_le64(pb + 16, p.blowup);
+        store_le64(pb + 24, p.num_queries);
+        store_le64(pb + 32, p.final_size);
+        store_le32(pb + 40, p.grinding_bits);
+        store_le64(pb + 44, p.cap_height);
+        store_le32(pb + 52, p.value);
+        tr.absorb("range/params", pb, 56);
+        if (P.host_a) {
+            hip_check(hipMemcpyAsync(P.col, P.host_a, P.ncols * P.n * 4, hipMemcpyHostToDevice, c.stream));
+        } else {
+            // Constant columns: coefficient 0 of each column, zeros above.
+            hip_check(hipMemsetAsync(P.coeff, 0, P.ncols * P.n * 4, c.stream));
+            hip_check(hipMemcpy2DAsync(P.coeff, P.n * 4, P.const_cols.data(), 4, 4, P.ncols, hipMemcpyHostToDevice,
+                                       c.stream));
+        }
+        if (P.n_rand)
+            hip_check(hipMemcpyAsync(P.mask_dev, P.mask.data(), P.mask.size() * 4, hipMemcpyHostToDevice, c.stream));
     } else {
         uint8_t dims[24];
         store_le64(dims, P.m);
@@ -724,7 +839,35 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
     const GeometryTables& gt = *P.gt;
     switch (P.phase) {
         case PH_FRONT: {
-            if (P.air && P.two_col) {
+            if (P.range) {
+                const uint32_t ninv = to_mont(inv_c(uint32_t(P.n % P_MOD)));
+                for (uint64_t k = 0; k < P.ncols; ++k) {
+                    uint32_t* coeff = P.coeff + k * P.n;
+                    if (P.host_a) {
+                        NttPlan ip = plan_intt(P.col + k * P.n, coeff, P.scratch, P.logn, c.tw4096_inv, gt.winv,
+                                               P.logb, ninv);
+                        for (auto& j : ip.pass1) ch.push_back(j);
+                        for (auto& j : ip.pass2) ch.push_back(j);
+                    }
+                    NttPlan lp = plan_lde(coeff, P.rows + k * P.m, P.scratch, P.logn, P.logb, c.tw4096, gt.w, gt.coset);
+                    for (auto& j : lp.pass1) {
+                        if (P.n_rand) {
+                            j.mask = P.mask_dev + k * P.n_rand;
+                            j.zvan = gt.zvan;
+                            j.mask_len = uint32_t(P.n_rand);
+                        }
+                        ch.push_back(j);
+                    }
+                    for (auto& j : lp.pass2) ch.push_back(j);
+                }
+                LeavesJob t = leaves_of(LEAF_ROWS, P.rows);
+                t.ncols = int(P.ncols);
+                t.col_stride = P.m;
+                t.logn = P.logn;
+                t.logb = P.logb;
+                tree_steps(ch, P.trace_tree, P.tg, t, c.sched);
+                collect_cap(c, P, P.trace_tree, P.tg, items, used);
+            } else if (P.air && P.two_col) {
                 if (!P.host_a) {
                     // Both Fibonacci columns' coefficients in closed form.
                     const auto ca = fib_binet(P.a0, P.b0);
@@ -838,6 +981,22 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
             break;
         }
         case PH_COMPOSE: {
+            if (P.range) {
+                RangeComposeJob rj{};
+                rj.cols = P.rows;
+                rj.col_stride = P.m;
+                rj.layer0 = P.vals[0];
+                rj.xn = gt.xn;
+                rj.shift_m = gt.coset.shift;
+                rj.zinv_m = gt.zinv;
+                rj.c = static_cast<const RangeConsts*>(cst ? cst : c.ring.push(&P.rc, sizeof(P.rc)));
+                rj.logn = P.logn;
+                rj.logb = P.logb;
+                ch.push_back(rj);
+                tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_EXT, P.vals[0]), c.sched);
+                collect_cap(c, P, P.trees[0], P.geoms[0], items, used);
+                break;
+            }
             ComposeJob cj{};
             cj.a = P.A;
             cj.b = P.Bc;
@@ -930,6 +1089,13 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
             if (P.air) {
                 o.a = P.A;
                 o.b = P.Bc;
+                o.trace_tree = P.trace_tree;
+                o.trace_leaves = P.tg.leaves;
+                o.trace_target = P.tg.target;
+            } else if (P.range) {
+                o.cols = P.rows;
+                o.ncols = int(P.ncols);
+                o.col_stride = P.m;
                 o.trace_tree = P.trace_tree;
                 o.trace_leaves = P.tg.leaves;
                 o.trace_target = P.tg.target;
@@ -1064,6 +1230,26 @@ void serialize(Proof& P, const uint32_t* words) {
             w.words(sw, 16);
         }
         P.proof = w.finish();
+    } else if (P.range) {
+        // range::Proof::to_bytes (range.rs:240-260).
+        Writer w;
+        w.reserve(estimate);
+        w.byte(WIRE_VERSION);
+        write_cap(w, P.column_cap);
+        write_fri_commitment_body(w, P);
+        write_fri_proof_body(w, P, words);
+        w.u32(uint32_t(P.queries.size()));
+        const int logm = P.logn + P.logb;
+        for (size_t q = 0; q < P.queries.size(); ++q) {
+            const uint32_t* rw = words + q * P.wpq + P.fri_wpq;
+            const uint64_t index = brev_host(P.queries[q], logm);
+            w.u64(index);
+            w.u32(rw[0]);
+            w.u32(uint32_t(P.ncols - 1));
+            w.words(rw + 1, P.ncols - 1);
+            write_path(w, rw + P.ncols, P.tg.target, index);
+        }
+        P.proof = w.finish();
     } else {
         Writer wc;
         wc.byte(WIRE_VERSION);
@@ -1095,6 +1281,22 @@ void host_step(Ctx& c, Proof& P) {
                 P.cc.a0 = P.a0;
                 P.cc.b0 = P.b0;
                 P.cc.b_last = P.b_last;
+                uint8_t dims[24];
+                store_le64(dims, P.m);
+                store_le64(dims + 8, P.poly_len);
+                store_le64(dims + 16, P.degree_bound);
+                P.tr->absorb("fri-dims", dims, 24);
+                P.phase = PH_COMPOSE;
+            } else if (P.range) {
+                // range.rs:464-475: the row cap, then 1 + n_bits + 1 alphas.
+                P.column_cap = read_digests(w, uint64_t(1) << P.tg.drop);
+                std::vector<uint8_t> cb(P.column_cap.size() * 32);
+                for (size_t i = 0; i < P.column_cap.size(); ++i) std::memcpy(cb.data() + 32 * i, P.column_cap[i].data(), 32);
+                P.tr->absorb("range/column", cb);
+                for (uint64_t k = 0; k < P.ncols + 1; ++k) {
+                    E4 al = P.tr->challenge_ext("range/alpha");
+                    for (int q = 0; q < 4; ++q) P.rc.alpha_m[k][q] = to_mont(al.v[q]);
+                }
                 uint8_t dims[24];
                 store_le64(dims, P.m);
                 store_le64(dims + 8, P.poly_len);
@@ -1305,7 +1507,7 @@ void Engine::pipeline(Proof& P) {
         size_t stage_off;
     };
     std::vector<Ph> phs{{PH_FRONT, 0, 0}};
-    if (P.air) phs.push_back({PH_COMPOSE, 0, 0});
+    if (P.air || P.range) phs.push_back({PH_COMPOSE, 0, 0});
     for (size_t d = 0; d < P.folds; ++d) phs.push_back({PH_FOLD, d, 0});
     if (P.fp.grinding_bits > 0) phs.push_back({PH_GRIND, 0, 0});
     constexpr size_t SLOT = Ctx::MAILBOX_SLOT;
@@ -1315,7 +1517,9 @@ void Engine::pipeline(Proof& P) {
     const bool with_open = phs.size() + open_slots <= Ctx::MAILBOX_SLOTS;
     if (with_open) phs.push_back({PH_OPEN, 0, 0});
     require(phs.size() <= Ctx::MAILBOX_SLOTS, Z0P_ERR_DEVICE);
-    static_assert(sizeof(ComposeConsts) <= SLOT && sizeof(FoldConsts) <= SLOT && 64 <= SLOT, "mailbox slot");
+    static_assert(sizeof(ComposeConsts) <= SLOT && sizeof(RangeConsts) <= SLOT && sizeof(FoldConsts) <= SLOT &&
+                      64 <= SLOT,
+                  "mailbox slot");
     c_.reserve_stage(P.stage_need * phs.size());
     std::vector<std::vector<Step>> chains(phs.size());
     std::vector<std::vector<CollectItem>> items(phs.size());
@@ -1382,7 +1586,8 @@ void Engine::pipeline(Proof& P) {
                     cancelled = !live;
                 } else {
                     require(P.phase == nx.ph && (nx.ph != PH_FOLD || P.depth == nx.depth), Z0P_ERR_DEVICE);
-                    if (nx.ph == PH_COMPOSE) std::memcpy(slot, &P.cc, sizeof(P.cc));
+                    if (nx.ph == PH_COMPOSE && P.range) std::memcpy(slot, &P.rc, sizeof(P.rc));
+                    else if (nx.ph == PH_COMPOSE) std::memcpy(slot, &P.cc, sizeof(P.cc));
                     else if (nx.ph == PH_FOLD) std::memcpy(slot, &P.fold, sizeof(P.fold));
                     else std::memcpy(slot, P.pow_hw, sizeof(P.pow_hw));
                 }
@@ -1450,6 +1655,25 @@ Proof* proof_new_air_zk(Ctx& c, const z0p_air_params& p, uint64_t n_rand, ChaCha
     return P.release();
 }
 
+Proof* proof_new_range(Ctx& c, const z0p_range_params& p, Transcript* tr) {
+    auto P = std::make_unique<Proof>();
+    init_range(c, *P, p, tr);
+    return P.release();
+}
+
+Proof* proof_new_range_cols(Ctx& c, const z0p_range_params& p, const uint32_t* cols, uint64_t ncols, uint64_t len,
+                            Transcript* tr) {
+    auto P = std::make_unique<Proof>();
+    init_range_cols(c, *P, p, cols, ncols, len, tr);
+    return P.release();
+}
+
+Proof* proof_new_range_zk(Ctx& c, const z0p_range_params& p, uint64_t n_rand, ChaCha20& rng, Transcript* tr) {
+    auto P = std::make_unique<Proof>();
+    init_range_zk(c, *P, p, n_rand, rng, tr);
+    return P.release();
+}
+
 Proof* proof_new_fri(Ctx& c, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,
                      const z0p_fri_params& fp, Transcript* tr) {
     auto P = std::make_unique<Proof>();
@@ -1493,6 +1717,19 @@ Bytes air_prove_with_trace(Ctx& ctx, const z0p_air_params& p, const uint32_t* co
 
 Bytes air_prove_zk(Ctx& ctx, const z0p_air_params& p, uint64_t n_rand, ChaCha20& rng, Transcript& tr) {
     return run_one(ctx, proof_new_air_zk(ctx, p, n_rand, rng, &tr));
+}
+
+Bytes range_prove(Ctx& ctx, const z0p_range_params& p, Transcript& tr) {
+    return run_one(ctx, proof_new_range(ctx, p, &tr));
+}
+
+Bytes range_prove_with_columns(Ctx& ctx, const z0p_range_params& p, const uint32_t* cols, uint64_t ncols,
+                               uint64_t len, Transcript& tr) {
+    return run_one(ctx, proof_new_range_cols(ctx, p, cols, ncols, len, &tr));
+}
+
+Bytes range_prove_zk(Ctx& ctx, const z0p_range_params& p, uint64_t n_rand, ChaCha20& rng, Transcript& tr) {
+    return run_one(ctx, proof_new_range_zk(ctx, p, n_rand, rng, &tr));
 }
 
 void fri_commit_open(Ctx& ctx, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,

@@ -37,8 +37,8 @@ TreeGeom tree_geom(uint64_t leaves, uint64_t cap_height);
 
 // One kernel step of one proof; a proof's steps of one iteration form a
 // chain that must run in order.
-using Step = std::variant<FibCoeffJob, BitrevJob, InterleaveJob, NttJob, LinkedJob, ComposeJob, LeavesJob, LevelsJob,
-                          TailJob, FoldJob, SmallCommitJob, GrindJob, OpenJob>;
+using Step = std::variant<FibCoeffJob, BitrevJob, InterleaveJob, NttJob, LinkedJob, ComposeJob, RangeComposeJob,
+                          LeavesJob, LevelsJob, TailJob, FoldJob, SmallCommitJob, GrindJob, OpenJob>;
 
 // Leaves, then wide levels, then LDS tail launches up to the cap level.
 void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves, const Schedule& s);
@@ -61,6 +61,11 @@ Proof* proof_new_air(Ctx& c, const z0p_air_params& p, Transcript* tr);
 Proof* proof_new_air_trace(Ctx& c, const z0p_air_params& p, const uint32_t* col_a, uint64_t len_a,
                           const uint32_t* col_b, uint64_t len_b, Transcript* tr);
 Proof* proof_new_air_zk(Ctx& c, const z0p_air_params& p, uint64_t n_rand, ChaCha20& rng, Transcript* tr);
+// range::prove / prove_with_columns / prove_zk (range.rs:334-428).
+Proof* proof_new_range(Ctx& c, const z0p_range_params& p, Transcript* tr);
+Proof* proof_new_range_cols(Ctx& c, const z0p_range_params& p, const uint32_t* cols, uint64_t ncols, uint64_t len,
+                            Transcript* tr);
+Proof* proof_new_range_zk(Ctx& c, const z0p_range_params& p, uint64_t n_rand, ChaCha20& rng, Transcript* tr);
 // What a FRI commitment starts from (fri.rs:408-549).
 enum FriInput : int {
     FRI_COEFFS = 0,     // commit: base-field coefficients
@@ -138,11 +143,16 @@ Bytes air_prove_zk(Ctx& ctx, const z0p_air_params& p, uint64_t n_rand, ChaCha20&
 void fri_commit_open(Ctx& ctx, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,
                      const z0p_fri_params& fp, Transcript& tr,
                      Bytes& commitment, Bytes& proof);
+Bytes range_prove(Ctx& ctx, const z0p_range_params& p, Transcript& tr);
+Bytes range_prove_with_columns(Ctx& ctx, const z0p_range_params& p, const uint32_t* cols, uint64_t ncols,
+                               uint64_t len, Transcript& tr);
+Bytes range_prove_zk(Ctx& ctx, const z0p_range_params& p, uint64_t n_rand, ChaCha20& rng, Transcript& tr);
 
 // Host verifier (air.rs:607-752, fri.rs:686-859); throws Status on rejection.
 void air_verify(const z0p_air_params& p, Transcript& tr, const uint8_t* proof, size_t len);
 void fri_verify(const uint8_t* cm, size_t cm_len, const uint8_t* pf, size_t pf_len, const z0p_fri_params& fp,
                 Transcript& tr);
+void range_verify(const z0p_range_params& p, Transcript& tr, const uint8_t* proof, size_t len);
 
 }  // namespace z0p
 

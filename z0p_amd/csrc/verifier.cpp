@@ -1,5 +1,6 @@
-// Host verifier for the AIR and FRI proofs (the reference verifies on the
-// CPU: air.rs:607-752, fri.rs:686-859, 1153-1227, 1339-1364). It parses the
+// Host verifier for the AIR, range-check and FRI proofs (the reference
+// verifies on the CPU: air.rs:607-752, range.rs:521-721, fri.rs:686-859,
+// 1153-1227, 1339-1364). It parses the
 // VERSION = 6 bytes fail-closed (codec.rs:118-274) and replays the
 // transcript exactly as the prover advanced it.
 #include <algorithm>
@@ -338,6 +339,35 @@ E4 composition_at(uint32_t a, uint32_t b, uint32_t an, uint32_t bn, uint32_t x, 
     return r;
 }
 
+// range.rs:752-774
+E4 range_composition_at(uint32_t v, const std::vector<uint32_t>& bits, uint32_t x, uint32_t value, uint64_t n,
+                        const std::vector<E4>& al) {
+    const uint32_t zi = inv_c(sub(pow_c(x, n), 1));
+    uint32_t rec = 0, power = 1;
+    for (uint32_t b : bits) {
+        rec = add(rec, mul(power, b));
+        power = add(power, power);
+    }
+    E4 r = e4_mul_base(al[0], mul(sub(v, rec), zi));
+    for (size_t j = 0; j < bits.size(); ++j)
+        r = e4_add(r, e4_mul_base(al[1 + j], mul(mul(bits[j], sub(bits[j], 1)), zi)));
+    return e4_add(r, e4_mul_base(al[bits.size() + 1], mul(sub(v, value), inv_c(sub(x, 1)))));
+}
+
+// merkle.rs:286-309: a variable-length leaf, BLAKE3(0x00 || data).
+bool verify_capped_row(const Digest& entry, const std::vector<uint8_t>& row, uint64_t index, uint64_t total_depth,
+                       const Path& path) {
+    if (total_depth >= 64 || path.size() > total_depth || (index >> total_depth) != 0 ||
+        !directions_match(index, path))
+        return false;
+    std::vector<uint8_t> buf(row.size() + 1, 0);
+    std::memcpy(buf.data() + 1, row.data(), row.size());
+    Digest cur;
+    blake3_hash(buf.data(), buf.size(), cur.data());
+    for (auto& st : path) cur = st.second ? hash_node(cur, st.first) : hash_node(st.first, cur);
+    return cur == entry;
+}
+
 }  // namespace
 
 void fri_verify(const uint8_t* cmb, size_t cm_len, const uint8_t* pfb, size_t pf_len, const z0p_fri_params& fp,
@@ -443,6 +473,92 @@ void air_verify(const z0p_air_params& p, Transcript& tr, const uint8_t* bytes, s
                 Z0P_ERR_BAD_MERKLE_PATH);
         uint32_t x = mul(GENERATOR, pow_c(omega_m, index));
         E4 expected = composition_at(o.a, o.b, o.an, o.bn, x, w_last, a0, b0, b_last, n, al);
+        require(!pf.queries[q].empty(), Z0P_ERR_FRI_VERIFICATION_FAILED);
+        require(e4_eq(pf.queries[q][0].values[leaf % 4], expected), Z0P_ERR_FRI_VERIFICATION_FAILED);
+    }
+}
+
+void range_verify(const z0p_range_params& p, Transcript& tr, const uint8_t* bytes, size_t len) {
+    // Proof::from_bytes (range.rs:280-306)
+    struct RowOpening {
+        uint64_t index = 0;
+        uint32_t v = 0;
+        std::vector<uint32_t> bits;
+        Path path;
+    };
+    Reader r(bytes, len);
+    r.version();
+    std::vector<Digest> column_cap = read_cap(r);
+    CommitmentV cm = read_commitment_body(r);
+    FriProofV pf = read_fri_proof_body(r);
+    std::vector<RowOpening> ops;
+    uint32_t no = r.count(MAX_QUERIES);
+    for (uint32_t i = 0; i < no; ++i) {
+        RowOpening o;
+        o.index = r.u64();
+        o.v = r.field();
+        uint32_t nb = r.count(MAX_ELEMS);
+        for (uint32_t k = 0; k < nb; ++k) o.bits.push_back(r.field());
+        o.path = read_path(r);
+        ops.push_back(std::move(o));
+    }
+    r.end();
+
+    // range::verify (range.rs:521-721)
+    validate_range(p);
+    const uint64_t n = p.n_rows;
+    const uint64_t m = cm.lde_len;
+    const uint64_t expected_m = n * p.blowup;  // validated: <= 2^27
+    require(m >= expected_m && m % expected_m == 0, Z0P_ERR_FRI_VERIFICATION_FAILED);
+    if (m == expected_m) {
+        require(cm.degree_bound == n - 1, Z0P_ERR_FRI_VERIFICATION_FAILED);
+    } else {
+        // masked columns: at most the largest honest masked degree
+        const uint64_t padded = m / p.blowup;
+        require(2 * padded >= n + 2 && cm.degree_bound <= 2 * padded - (n + 2), Z0P_ERR_FRI_VERIFICATION_FAILED);
+    }
+    require(cm.poly_len == m, Z0P_ERR_FRI_VERIFICATION_FAILED);
+    const uint32_t value = p.value % P;
+    uint8_t pb[56];
+    store_le64(pb, p.n_rows);
+    store_le64(pb + 8, p.n_bits);
+    store_le64(pb + 16, p.blowup);
+    store_le64(pb + 24, p.num_queries);
+    store_le64(pb + 32, p.final_size);
+    store_le32(pb + 40, p.grinding_bits);
+    store_le64(pb + 44, p.cap_height);
+    store_le32(pb + 52, value);
+    tr.absorb("range/params", pb, 56);
+    require(ops.size() == p.num_queries, Z0P_ERR_FRI_VERIFICATION_FAILED);
+    {
+        std::vector<uint8_t> cb;
+        for (auto& d : column_cap) cb.insert(cb.end(), d.begin(), d.end());
+        tr.absorb("range/column", cb);
+    }
+    std::vector<E4> al(p.n_bits + 2);
+    for (auto& a : al) a = tr.challenge_ext("range/alpha");
+    const z0p_fri_params fp = range_fri_params(p);
+    std::vector<uint64_t> idx = fri_verify_impl(cm, pf, fp, tr);
+    require(pow2(m), Z0P_ERR_FRI_VERIFICATION_FAILED);
+    const uint64_t log_m = ilog2(m);
+    const uint32_t omega_m = root_of_unity(uint32_t(log_m));
+    const size_t count = std::min(std::min(pf.queries.size(), ops.size()), idx.size());
+    for (size_t q = 0; q < count; ++q) {
+        const RowOpening& o = ops[q];
+        const uint64_t leaf = idx[q];
+        const uint64_t index = brev64(leaf, log_m);
+        require(o.index == index && o.bits.size() == p.n_bits, Z0P_ERR_FRI_VERIFICATION_FAILED);
+        std::vector<uint8_t> row(4 * (p.n_bits + 1));
+        store_le32(row.data(), o.v);
+        for (size_t j = 0; j < o.bits.size(); ++j) store_le32(row.data() + 4 * (j + 1), o.bits[j]);
+        const uint64_t depth = log_m;
+        const uint64_t drop = std::min<uint64_t>(p.cap_height, depth);
+        require(column_cap.size() == (uint64_t(1) << drop) && o.path.size() == depth - drop, Z0P_ERR_BAD_MERKLE_PATH);
+        const uint64_t e = index >> (depth - drop);
+        require(e < column_cap.size(), Z0P_ERR_BAD_MERKLE_PATH);
+        require(verify_capped_row(column_cap[e], row, index, depth, o.path), Z0P_ERR_BAD_MERKLE_PATH);
+        const uint32_t x = mul(GENERATOR, pow_c(omega_m, index));
+        const E4 expected = range_composition_at(o.v, o.bits, x, value, n, al);
         require(!pf.queries[q].empty(), Z0P_ERR_FRI_VERIFICATION_FAILED);
         require(e4_eq(pf.queries[q][0].values[leaf % 4], expected), Z0P_ERR_FRI_VERIFICATION_FAILED);
     }
