@@ -9,6 +9,7 @@
 
 #include "air.hpp"
 #include "range.hpp"
+#include "table.hpp"
 
 using namespace oracle;
 
@@ -26,6 +27,13 @@ typedef struct {
     uint32_t grinding_bits, reserved0;
     uint64_t cap_height;
 } oracle_fri_params;
+
+typedef struct {
+    uint64_t n_rows, blowup, num_queries, final_size;
+    uint32_t grinding_bits, reserved0;
+    uint64_t cap_height;
+    uint32_t value, reserved1;
+} oracle_table_params;
 
 typedef struct {
     uint64_t n_rows, n_bits, blowup, num_queries, final_size;
@@ -82,6 +90,29 @@ void from_range(const range::Params& q, oracle_range_params* p) {
     std::memset(p, 0, sizeof(*p));
     p->n_rows = q.n_rows;
     p->n_bits = q.n_bits;
+    p->blowup = q.blowup;
+    p->num_queries = q.num_queries;
+    p->final_size = q.final_size;
+    p->grinding_bits = q.grinding_bits;
+    p->cap_height = q.cap_height;
+    p->value = q.value.v;
+}
+
+table::Params to_table(const oracle_table_params* p) {
+    table::Params q;
+    q.n_rows = p->n_rows;
+    q.blowup = p->blowup;
+    q.num_queries = p->num_queries;
+    q.final_size = p->final_size;
+    q.grinding_bits = p->grinding_bits;
+    q.cap_height = p->cap_height;
+    q.value = Field::make(p->value);
+    return q;
+}
+
+void from_table(const table::Params& q, oracle_table_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->n_rows = q.n_rows;
     p->blowup = q.blowup;
     p->num_queries = q.num_queries;
     p->final_size = q.final_size;
@@ -523,6 +554,51 @@ int oracle_range_composition_at(uint32_t v, const uint32_t* bits, size_t n_bits,
         Ext4 r = range::composition_at(Field::make(v), to_fields(bits, n_bits), Field::make(point), Field::make(value),
                                        n, al[0], ab, al[n_bits + 1]);
         for (int c = 0; c < 4; ++c) out4[c] = r.c[c].v;
+    })
+}
+
+// table.rs: parameters, the provers and the verifier; the table is `tlen`
+// words (reduced mod p).
+int oracle_table_params_new(uint64_t n_rows, uint64_t blowup, uint64_t nq, uint64_t fs, uint32_t value,
+                            oracle_table_params* out) {
+    GUARD(from_table(table::Params::make(n_rows, blowup, nq, fs, Field::make(value)), out))
+}
+int oracle_table_params_for_security(uint64_t n_rows, uint64_t table_len, uint32_t value, uint32_t bits,
+                                     oracle_table_params* out) {
+    GUARD(from_table(table::Params::for_security(n_rows, table_len, Field::make(value), bits), out))
+}
+uint64_t oracle_table_membership_bound(uint64_t table_len, uint64_t n, uint64_t n_rand) {
+    return table::membership_bound(table_len, n, n_rand);
+}
+int oracle_table_prove(const oracle_table_params* p, const uint32_t* tab, size_t tlen, void* t, uint8_t** out,
+                       size_t* len) {
+    GUARD({
+        table::Proof pr = table::prove(to_table(p), to_fields(tab, tlen), *static_cast<Transcript*>(t));
+        *out = dup_bytes(pr.to_bytes(), len);
+    })
+}
+int oracle_table_prove_coeffs(const oracle_table_params* p, const uint32_t* tab, size_t tlen, const uint32_t* coeffs,
+                              size_t clen, size_t degree_bound, void* t, uint8_t** out, size_t* len) {
+    GUARD({
+        table::Params tp = to_table(p);
+        table::Proof pr = table::prove_from_coeffs(tp, to_fields(tab, tlen), to_fields(coeffs, clen), degree_bound,
+                                                   *static_cast<Transcript*>(t));
+        *out = dup_bytes(pr.to_bytes(), len);
+    })
+}
+int oracle_table_prove_zk(const oracle_table_params* p, const uint32_t* tab, size_t tlen, size_t n_rand, void* rng,
+                          void* t, uint8_t** out, size_t* len) {
+    GUARD({
+        table::Proof pr = table::prove_zk(to_table(p), to_fields(tab, tlen), n_rand, *static_cast<ChaCha20*>(rng),
+                                          *static_cast<Transcript*>(t));
+        *out = dup_bytes(pr.to_bytes(), len);
+    })
+}
+int oracle_table_verify(const oracle_table_params* p, const uint32_t* tab, size_t tlen, void* t,
+                        const uint8_t* proof, size_t len) {
+    GUARD({
+        table::Proof pr = table::Proof::from_bytes(proof, len);
+        table::verify(to_table(p), to_fields(tab, tlen), pr, *static_cast<Transcript*>(t));
     })
 }
 

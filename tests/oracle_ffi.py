@@ -51,6 +51,29 @@ class FriParams(ctypes.Structure):
                                      self.final_size)
 
 
+class TableParams(ctypes.Structure):
+    """Layout shared with ``z0p_table_params`` (include/z0p.h)."""
+
+    _fields_ = [
+        ("n_rows", ctypes.c_uint64),
+        ("blowup", ctypes.c_uint64),
+        ("num_queries", ctypes.c_uint64),
+        ("final_size", ctypes.c_uint64),
+        ("grinding_bits", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
+        ("cap_height", ctypes.c_uint64),
+        ("value", ctypes.c_uint32),
+        ("reserved1", ctypes.c_uint32),
+    ]
+
+    def replace(self, **kw):
+        q = TableParams()
+        ctypes.pointer(q)[0] = self
+        for k, v in kw.items():
+            setattr(q, k, v)
+        return q
+
+
 class RangeParams(ctypes.Structure):
     """Layout shared with ``z0p_range_params`` (include/z0p.h)."""
 
@@ -176,6 +199,18 @@ def lib():
         "oracle_range_verify": (ctypes.c_int, [ctypes.POINTER(RangeParams), vp, u8p, sz]),
         "oracle_range_composition_at": (ctypes.c_int, [ctypes.c_uint32, u32p, sz, ctypes.c_uint32, ctypes.c_uint32,
                                                        ctypes.c_uint64, u32p, u32p]),
+        "oracle_table_params_new": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                                   ctypes.c_uint32, ctypes.POINTER(TableParams)]),
+        "oracle_table_params_for_security": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
+                                                            ctypes.c_uint32, ctypes.POINTER(TableParams)]),
+        "oracle_table_membership_bound": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]),
+        "oracle_table_prove": (ctypes.c_int, [ctypes.POINTER(TableParams), u32p, sz, vp, ctypes.POINTER(u8p),
+                                              ctypes.POINTER(sz)]),
+        "oracle_table_prove_coeffs": (ctypes.c_int, [ctypes.POINTER(TableParams), u32p, sz, u32p, sz, sz, vp,
+                                                     ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+        "oracle_table_prove_zk": (ctypes.c_int, [ctypes.POINTER(TableParams), u32p, sz, sz, vp, vp,
+                                                 ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+        "oracle_table_verify": (ctypes.c_int, [ctypes.POINTER(TableParams), u32p, sz, vp, u8p, sz]),
         "oracle_stage_timers_clear": (None, []),
         "oracle_stage_timers": (sz, [ctypes.c_char_p, sz]),
     }
@@ -666,3 +701,46 @@ def range_composition_at(v, bits, point, value, n, alphas):
     if rc:
         raise OracleError(rc)
     return tuple(out)
+
+
+# table.rs: the table-membership AIR.
+def table_params_new(n_rows, blowup, nq, final_size, value, grinding_bits=0, cap_height=0) -> TableParams:
+    p = TableParams()
+    rc = lib().oracle_table_params_new(n_rows, blowup, nq, final_size, value, ctypes.byref(p))
+    if rc:
+        raise OracleError(rc)
+    p.grinding_bits, p.cap_height = grinding_bits, cap_height
+    return p
+
+
+def table_params_for_security(n_rows, table_len, value, bits=100) -> TableParams:
+    p = TableParams()
+    rc = lib().oracle_table_params_for_security(n_rows, table_len, value, bits, ctypes.byref(p))
+    if rc:
+        raise OracleError(rc)
+    return p
+
+
+def table_membership_bound(table_len, n, n_rand):
+    return lib().oracle_table_membership_bound(table_len, n, n_rand)
+
+
+def table_prove(params: TableParams, table, transcript: Transcript) -> bytes:
+    return _range_out(lambda o, n: lib().oracle_table_prove(ctypes.byref(params), u32_array(table), len(table),
+                                                            transcript.h, o, n))
+
+
+def table_prove_coeffs(params: TableParams, table, coeffs, degree_bound, transcript: Transcript) -> bytes:
+    return _range_out(lambda o, n: lib().oracle_table_prove_coeffs(
+        ctypes.byref(params), u32_array(table), len(table), u32_array(coeffs), len(coeffs), degree_bound,
+        transcript.h, o, n))
+
+
+def table_prove_zk(params: TableParams, table, n_rand: int, rng: ChaCha20, transcript: Transcript) -> bytes:
+    return _range_out(lambda o, n: lib().oracle_table_prove_zk(ctypes.byref(params), u32_array(table), len(table),
+                                                               n_rand, rng.h, transcript.h, o, n))
+
+
+def table_verify(params: TableParams, table, transcript: Transcript, proof: bytes) -> int:
+    return lib().oracle_table_verify(ctypes.byref(params), u32_array(table), len(table), transcript.h, _buf(proof),
+                                     len(proof))
