@@ -217,6 +217,44 @@ int z0p_range_prove_zk(z0p_ctx* ctx, const z0p_range_params* params, uint64_t n_
 int z0p_range_verify(const z0p_range_params* params, z0p_transcript* transcript, const uint8_t* proof,
                      size_t proof_len);
 
+/* ---- table-membership AIR (src/table.rs) on the same device path: the
+ * value column extended on the 31-coset, one tree leaf per LDE value
+ * (BLAKE3(0x00 || V)), the composition of table.rs:597-644 committed by
+ * fri::commit_ext_evals; the public table is table_len >= 2 entries, each
+ * reduced mod p ---- */
+/* table::Params (table.rs:27-46); value is reduced mod p. */
+typedef struct z0p_table_params {
+    uint64_t n_rows;
+    uint64_t blowup;
+    uint64_t num_queries;
+    uint64_t final_size;
+    uint32_t grinding_bits;
+    uint32_t reserved0;
+    uint64_t cap_height;
+    uint32_t value;
+    uint32_t reserved1;
+} z0p_table_params;
+/* table::Params::new (table.rs:53-87): grinding 0, cap height 0. */
+int z0p_table_params_new(uint64_t n_rows, uint64_t blowup, uint64_t num_queries, uint64_t final_size, uint32_t value,
+                         z0p_table_params* out);
+/* table::Params::for_security (table.rs:114-168): the smallest blowup >= 4
+ * with rate <= 1/4 at this table length. */
+int z0p_table_params_for_security(uint64_t n_rows, uint64_t table_len, uint32_t value, uint32_t target_bits,
+                                  z0p_table_params* out);
+/* table::membership_bound (table.rs:357-363), saturating. */
+uint64_t z0p_table_membership_bound(uint64_t table_len, uint64_t n_rows, uint64_t n_rand);
+/* table::prove (table.rs:305-318). */
+int z0p_table_prove(z0p_ctx* ctx, const z0p_table_params* params, const uint32_t* table, size_t table_len,
+                    z0p_transcript* transcript, uint8_t** proof, size_t* proof_len);
+/* table::prove_zk (table.rs:329-355): the column masked with n_rand >=
+ * num_queries maskers from rng. */
+int z0p_table_prove_zk(z0p_ctx* ctx, const z0p_table_params* params, const uint32_t* table, size_t table_len,
+                       uint64_t n_rand, z0p_chacha20* rng, z0p_transcript* transcript, uint8_t** proof,
+                       size_t* proof_len);
+/* Proof::from_bytes + table::verify (table.rs:265-288, 430-538). Host only. */
+int z0p_table_verify(const z0p_table_params* params, const uint32_t* table, size_t table_len,
+                     z0p_transcript* transcript, const uint8_t* proof, size_t proof_len);
+
 /* ---- batched proving (no reference counterpart: the reference proves one
  * instance per air::prove call; this is `count` such calls at once) ---- */
 /* Device-memory budget of the batch and server schedulers on this ctx, in

@@ -364,3 +364,65 @@ def test_range_verifier_matches_oracle():
     down = o.range_prove_coeffs(op, [[12_345] + [0] * 7] + [[(12_345 >> j) & 1] + [0] * 7 for j in range(16)], 8,
                                 o.Transcript(b"down"))
     assert agree(op, zp, b"down", down) != 0
+
+
+@pytest.mark.parametrize("args", [(8, 4, 30, 12), (1 << 10, 16, 5, 100), (1 << 14, 3, 7, 100), (2, 2, 1, 100),
+                                  (3, 4, 1, 12), (8, 1, 1, 12), (8, (1 << 64) - 1, 1, 12), (1 << 20, 1 << 10, 9, 100)])
+def test_table_params_match_oracle(args):
+    try:
+        want = o.table_params_for_security(*args)
+    except o.OracleError as e:
+        with pytest.raises(z.Z0pError) as ze:
+            z.table.for_security(*args)
+        assert ze.value.code == e.code
+        return
+    assert bytes(z.table.for_security(*args)) == bytes(want)
+
+
+def test_table_bound_and_params_new_match_oracle():
+    for t, n, r in [(4, 8, 0), (4, 8, 8), (1 << 63, 1 << 62, 0), (2, 2, 0), (17, 1 << 20, 300)]:
+        assert z.table.membership_bound(t, n, r) == o.table_membership_bound(t, n, r)
+    for args in [(8, 8, 8, 4, 30), (3, 8, 8, 4, 1), (8, 1, 8, 4, 1), (8, 8, 0, 4, 1), (8, 8, 8, 1, 1),
+                 (1 << 40, 4, 8, 4, 1), (8, 2, 8, 4, P + 5)]:
+        try:
+            want = o.table_params_new(*args)
+        except o.OracleError as e:
+            with pytest.raises(z.Z0pError) as ze:
+                z.table.params_new(*args)
+            assert ze.value.code == e.code, args
+            continue
+        assert bytes(z.table.params_new(*args)) == bytes(want), args
+
+
+def test_table_verifier_matches_oracle():
+    T = [10, 20, 30, 40]
+
+    def agree(op, zp, label, proof, table=T):
+        ok_o = o.table_verify(op, table, o.Transcript(label), proof)
+        try:
+            z.table.verify(zp, table, proof, z.Transcript(label))
+            ok_z = 0
+        except z.Z0pError as e:
+            ok_z = e.code
+        assert ok_z == ok_o, (label, ok_o, ok_z)
+        return ok_o
+
+    op, zp = o.table_params_new(8, 8, 8, 4, 30), z.table.params_new(8, 8, 8, 4, 30)
+    pr = o.table_prove(op, T, o.Transcript(b"bench/table"))
+    assert agree(op, zp, b"bench/table", pr) == 0
+    assert agree(op, zp, b"bench/table", pr[:-1]) == 8
+    assert agree(op, zp, b"bench/table", pr, [10, 21, 30, 40]) != 0
+    assert agree(op, zp, b"bench/table", pr, [10]) == 8
+    for off in range(5, len(pr), 89):
+        bad = bytearray(pr)
+        bad[off] ^= 0x08
+        agree(op, zp, b"bench/table", bytes(bad))
+    cp, cz = op.replace(cap_height=2, grinding_bits=4), zp.replace(cap_height=2, grinding_bits=4)
+    assert agree(cp, cz, b"cap", o.table_prove(cp, T, o.Transcript(b"cap"))) == 0
+    np_, nz = o.table_params_new(8, 8, 8, 4, 99), z.table.params_new(8, 8, 8, 4, 99)
+    assert agree(np_, nz, b"non", o.table_prove(np_, T, o.Transcript(b"non"))) != 0
+    kp, kz = o.table_params_new(8, 8, 8, 8, 40), z.table.params_new(8, 8, 8, 8, 40)
+    zk = o.table_prove_zk(kp, T, 8, o.ChaCha20(bytes([55] * 32), bytes([66] * 12)), o.Transcript(b"zk"))
+    assert agree(kp, kz, b"zk", zk) == 0
+    down = o.table_prove_coeffs(op, T, [30] + [0] * 7, 21, o.Transcript(b"down"))
+    assert agree(op, zp, b"down", down) != 0

@@ -1,9 +1,10 @@
 """z0p-mi355x: MI355X (gfx950) drop-in for the z0p Fibonacci AIR prover, the
-range-check AIR and the FRI polynomial commitment.
+range-check and table-membership AIRs and the FRI polynomial commitment.
 
 Mirrors the reference crate's public API (src/air.rs, src/range.rs,
-src/fri.rs, src/transcript.rs): ``Transcript``, ``air.Params``/``air.prove``/
-``air.verify``, ``range.Params``/``range.prove``/``range.verify`` and
+src/table.rs, src/fri.rs, src/transcript.rs): ``Transcript``, ``air.Params``/``air.prove``/
+``air.verify``, ``range.Params``/``range.prove``/``range.verify``,
+``table.Params``/``table.prove``/``table.verify`` and
 ``fri.Params``/``fri.commit_open``/``fri.verify``. Proofs
 are the reference's canonical VERSION = 6 bytes. Proving runs on the GPU
 through libz0p_amd.so; verification is host code, as in the reference.
@@ -12,7 +13,7 @@ import ctypes
 
 from ._ffi import Z0pError, buf, check, lib, take
 
-__all__ = ["Z0pError", "Transcript", "ChaCha20", "Context", "Server", "air", "fri", "range", "default_context"]
+__all__ = ["Z0pError", "Transcript", "ChaCha20", "Context", "Server", "air", "fri", "range", "table", "default_context"]
 
 
 class Transcript:
@@ -224,4 +225,4 @@ def default_context() -> Context:
     return _default
 
 
-from . import air, fri, range  # noqa: E402
+from . import air, fri, range, table  # noqa: E402

@@ -57,6 +57,13 @@ class RangeParamsC(ctypes.Structure):
     ]
 
 
+class TableParamsC(ctypes.Structure):
+    _fields_ = [
+        ("n_rows", u64), ("blowup", u64), ("num_queries", u64), ("final_size", u64),
+        ("grinding_bits", u32), ("reserved0", u32), ("cap_height", u64), ("value", u32), ("reserved1", u32),
+    ]
+
+
 class FriParamsC(ctypes.Structure):
     _fields_ = [
         ("blowup", u64), ("num_queries", u64), ("final_size", u64),
@@ -122,6 +129,14 @@ SIGNATURES = {
     "z0p_range_prove_zk": (ctypes.c_int, [vp, ctypes.POINTER(RangeParamsC), u64, vp, vp, ctypes.POINTER(u8p),
                                           ctypes.POINTER(sz)]),
     "z0p_range_verify": (ctypes.c_int, [ctypes.POINTER(RangeParamsC), vp, u8p, sz]),
+    "z0p_table_params_new": (ctypes.c_int, [u64, u64, u64, u64, u32, ctypes.POINTER(TableParamsC)]),
+    "z0p_table_params_for_security": (ctypes.c_int, [u64, u64, u32, u32, ctypes.POINTER(TableParamsC)]),
+    "z0p_table_membership_bound": (u64, [u64, u64, u64]),
+    "z0p_table_prove": (ctypes.c_int, [vp, ctypes.POINTER(TableParamsC), u32p, sz, vp, ctypes.POINTER(u8p),
+                                       ctypes.POINTER(sz)]),
+    "z0p_table_prove_zk": (ctypes.c_int, [vp, ctypes.POINTER(TableParamsC), u32p, sz, u64, vp, vp,
+                                          ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+    "z0p_table_verify": (ctypes.c_int, [ctypes.POINTER(TableParamsC), u32p, sz, vp, u8p, sz]),
     "z0p_fri_commit_open": (ctypes.c_int, [vp, u32p, sz, ctypes.POINTER(FriParamsC), vp, ctypes.POINTER(u8p),
                                            ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
     "z0p_fri_verify": (ctypes.c_int, [u8p, sz, u8p, sz, ctypes.POINTER(FriParamsC), vp]),

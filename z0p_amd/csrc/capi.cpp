@@ -376,6 +376,91 @@ int z0p_range_verify(const z0p_range_params* params, z0p_transcript* transcript,
     return guarded([&] { z0p::range_verify(*params, transcript->t, proof, proof_len); });
 }
 
+int z0p_table_params_new(uint64_t n_rows, uint64_t blowup, uint64_t num_queries, uint64_t final_size, uint32_t value,
+                         z0p_table_params* out) {
+    if (!out) return Z0P_ERR_INVALID_ARGUMENT;
+    using namespace z0p;
+    if (n_rows < 2 || !pow2(n_rows)) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    if (blowup < 2 || !pow2(blowup)) return Z0P_ERR_INVALID_BLOWUP;
+    if (num_queries == 0) return Z0P_ERR_NO_QUERIES;
+    if (final_size < 2 || !pow2(final_size)) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    if (n_rows > MAX_DOMAIN || blowup > MAX_DOMAIN || n_rows * blowup > MAX_DOMAIN) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    std::memset(out, 0, sizeof(*out));
+    out->n_rows = n_rows;
+    out->blowup = blowup;
+    out->num_queries = num_queries;
+    out->final_size = final_size;
+    out->value = value % P;
+    return Z0P_OK;
+}
+
+int z0p_table_params_for_security(uint64_t n_rows, uint64_t table_len, uint32_t value, uint32_t target_bits,
+                                  z0p_table_params* out) {
+    if (!out) return Z0P_ERR_INVALID_ARGUMENT;
+    using namespace z0p;
+    if (n_rows < 2 || !pow2(n_rows)) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    if (table_len < 2) return Z0P_ERR_MALFORMED_INPUT;
+    const unsigned __int128 prod = (unsigned __int128)table_len * (n_rows - 1);
+    if (prod > UINT64_MAX || prod < n_rows) return Z0P_ERR_MALFORMED_INPUT;
+    const uint64_t bound = uint64_t(prod) - n_rows;
+    const uint64_t need = bound == UINT64_MAX ? bound : bound + 1;
+    // the smallest blowup from 4 with need <= lde / 4
+    uint64_t blowup = 4, lde = 0;
+    while (true) {
+        if (n_rows > UINT64_MAX / blowup) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+        lde = n_rows * blowup;
+        if (need <= lde / 4 || blowup >= (uint64_t(1) << 27)) break;
+        blowup *= 2;
+    }
+    if (lde > MAX_DOMAIN || need > lde / 4) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    auto strength = tune_strength(target_bits, composition_millibits(lde, bound));
+    auto shape = tune_shape(lde, strength.first, true, lde, 12);  // 8 index bytes + one field
+    std::memset(out, 0, sizeof(*out));
+    out->n_rows = n_rows;
+    out->blowup = blowup;
+    out->num_queries = strength.first;
+    out->final_size = shape.first;
+    out->grinding_bits = strength.second;
+    out->cap_height = shape.second;
+    out->value = value % P;
+    return Z0P_OK;
+}
+
+uint64_t z0p_table_membership_bound(uint64_t table_len, uint64_t n_rows, uint64_t n_rand) {
+    return z0p::membership_bound(table_len, n_rows, n_rand);
+}
+
+int z0p_table_prove(z0p_ctx* ctx, const z0p_table_params* params, const uint32_t* table, size_t table_len,
+                    z0p_transcript* transcript, uint8_t** proof, size_t* proof_len) {
+    if (!ctx || !params || !transcript || !proof || !proof_len || (!table && table_len))
+        return Z0P_ERR_INVALID_ARGUMENT;
+    int rc = Z0P_OK;
+    int g = guarded([&] {
+        z0p::Bytes bytes = z0p::table_prove(*ctx->c, *params, table, table_len, transcript->t);
+        rc = emit(bytes, proof, proof_len);
+    });
+    return g != Z0P_OK ? g : rc;
+}
+
+int z0p_table_prove_zk(z0p_ctx* ctx, const z0p_table_params* params, const uint32_t* table, size_t table_len,
+                       uint64_t n_rand, z0p_chacha20* rng, z0p_transcript* transcript, uint8_t** proof,
+                       size_t* proof_len) {
+    if (!ctx || !params || !rng || !transcript || !proof || !proof_len || (!table && table_len))
+        return Z0P_ERR_INVALID_ARGUMENT;
+    int rc = Z0P_OK;
+    int g = guarded([&] {
+        z0p::Bytes bytes = z0p::table_prove_zk(*ctx->c, *params, table, table_len, n_rand, *rng->r, transcript->t);
+        rc = emit(bytes, proof, proof_len);
+    });
+    return g != Z0P_OK ? g : rc;
+}
+
+int z0p_table_verify(const z0p_table_params* params, const uint32_t* table, size_t table_len,
+                     z0p_transcript* transcript, const uint8_t* proof, size_t proof_len) {
+    if (!params || !transcript || (!proof && proof_len) || (!table && table_len)) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] { z0p::table_verify(*params, table, table_len, transcript->t, proof, proof_len); });
+}
+
 z0p_chacha20* z0p_chacha20_new(const uint8_t key[32], const uint8_t nonce[12]) {
     if (!key || !nonce) return nullptr;
     auto* h = new (std::nothrow) z0p_chacha20;

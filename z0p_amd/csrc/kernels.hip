@@ -723,6 +723,47 @@ __global__ __launch_bounds__(WG) void k_range_compose(RangeComposeJob inl, const
     }
 }
 
+// Table-membership composition per point (table.rs:597-644), as FRI layer 0.
+// The vanishing product runs in the Montgomery domain over the table.
+__global__ __launch_bounds__(WG) void k_table_compose(TableComposeJob inl, const TableComposeJob* tab,
+                                                      uint32_t njobs) {
+    uint32_t local;
+    const TableComposeJob J = job_of(inl, tab, njobs, local);
+    const uint32_t m = 1u << (J.logn + J.logb);
+    const uint32_t base = local * (WG * CELT) + threadIdx.x;
+    const TableConsts C = *J.c;
+    const gcptr T = gc(J.table_m);
+    uint32_t d1[CELT];
+#pragma unroll
+    for (int e = 0; e < CELT; ++e) {
+        const uint32_t p = base + e * WG;
+        if (p < m) {
+            const uint32_t j = brev_bits(p >> J.logn, J.logb);
+            const uint32_t x = mont_mul(gc(J.xn)[p & ((1u << J.logn) - 1)], gc(J.shift_m)[j]);
+            d1[e] = sub(x, R1);
+        } else {
+            d1[e] = R1;
+        }
+    }
+    batch_inv_m<CELT>(d1);
+#pragma unroll 1
+    for (int e = 0; e < CELT; ++e) {
+        const uint32_t p = base + e * WG;
+        if (p >= m) break;
+        const uint32_t j = brev_bits(p >> J.logn, J.logb);
+        const uint32_t v = gc(J.v)[p];
+        const uint32_t vm = to_mont(v);
+        uint32_t pm = R1;
+        for (uint32_t k = 0; k < J.table_len; ++k) pm = mont_mul(pm, sub(vm, T[k]));
+        const uint32_t sm = mont_mul(from_mont(pm), gc(J.zinv_m)[j]);  // prod / Z
+        const uint32_t sb = mont_mul(sub(v, C.value), d1[e]);            // (V - value) / (x - 1)
+        uint32_t o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) o[k] = mont_mul2(sm, C.alpha_m[0][k], sb, C.alpha_m[1][k]);
+        st4(J.layer0, p, u32x4{o[0], o[1], o[2], o[3]});
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Merkle leaves: thread g hashes leaves 2g and 2g+1 and, when asked, their
 // parent (one node of level 1). No LDS: wide levels run at full occupancy.
@@ -1655,6 +1696,13 @@ void launch_range_compose(const Launch& L, std::vector<RangeComposeJob> jobs) {
         return grid_for(uint64_t(1) << (j.logn + j.logb), WG * CELT);
     });
     run(L, "range_compose", k_range_compose, jobs, g, 0);
+}
+
+void launch_table_compose(const Launch& L, std::vector<TableComposeJob> jobs) {
+    unsigned g = assign_blocks(jobs, [](const TableComposeJob& j) {
+        return grid_for(uint64_t(1) << (j.logn + j.logb), WG * CELT);
+    });
+    run(L, "table_compose", k_table_compose, jobs, g, 0);
 }
 
 void launch_leaves(const Launch& L, std::vector<LeavesJob> jobs) {

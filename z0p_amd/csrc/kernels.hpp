@@ -152,6 +152,26 @@ struct RangeComposeJob {
 };
 void launch_range_compose(const Launch& L, std::vector<RangeComposeJob> jobs);
 
+// Table-membership composition (table.rs:597-644), written as FRI layer 0:
+//   a_m prod_k (V - t_k) / Z + a_b (V - value) / (x - 1).
+struct TableConsts {
+    uint32_t value, reserved;
+    uint32_t alpha_m[2][4];  // membership, boundary (Montgomery)
+};
+struct TableComposeJob {
+    const uint32_t* v;        // bit-reversed LDE of the value column
+    uint32_t* layer0;
+    const uint32_t* xn;       // [n] omega_n^brev(q), Montgomery
+    const uint32_t* shift_m;
+    const uint32_t* zinv_m;   // [B] 1/(s_j^n - 1)
+    const TableConsts* c;     // device memory: the alphas come from the transcript
+    const uint32_t* table_m;  // the public table, Montgomery
+    uint32_t table_len;
+    int logn, logb;
+    uint32_t block_begin;
+};
+void launch_table_compose(const Launch& L, std::vector<TableComposeJob> jobs);
+
 // Merkle leaves. Kinds: trace rows (8 joint A||B rows per leaf from the
 // bit-reversed columns), FRI Ext4 quads, FRI base-field quads (upper
 // coordinates zero), raw 64-byte leaves.

@@ -305,6 +305,7 @@ struct Buckets {
     std::vector<LinkedJob> linked;
     std::vector<ComposeJob> compose;
     std::vector<RangeComposeJob> rcompose;
+    std::vector<TableComposeJob> tcompose;
     std::vector<LeavesJob> leaves;
     std::vector<LevelsJob> levels;
     std::vector<TailJob> tail;
@@ -320,6 +321,7 @@ struct Buckets {
     std::vector<LinkedJob>& of(const LinkedJob&) { return linked; }
     std::vector<ComposeJob>& of(const ComposeJob&) { return compose; }
     std::vector<RangeComposeJob>& of(const RangeComposeJob&) { return rcompose; }
+    std::vector<TableComposeJob>& of(const TableComposeJob&) { return tcompose; }
     std::vector<LeavesJob>& of(const LeavesJob&) { return leaves; }
     std::vector<LevelsJob>& of(const LevelsJob&) { return levels; }
     std::vector<TailJob>& of(const TailJob&) { return tail; }
@@ -339,6 +341,7 @@ struct Buckets {
         if (!linked.empty()) launch_linked(L, linked);
         if (!compose.empty()) launch_compose(L, compose);
         if (!rcompose.empty()) launch_range_compose(L, rcompose);
+        if (!tcompose.empty()) launch_table_compose(L, tcompose);
         if (!leaves.empty()) launch_leaves(L, leaves);
         if (!levels.empty()) launch_levels(L, levels);
         if (!tail.empty()) launch_tail(L, tail);
@@ -448,23 +451,27 @@ const uint32_t* write_path(Writer& w, const uint32_t* p, int count, uint64_t ind
 }  // namespace
 
 enum Phase : int { PH_FRONT = 0, PH_COMPOSE, PH_FOLD, PH_GRIND, PH_OPEN, PH_DONE };
+// The one-leaf-per-row AIRs: value (and bit) columns on the LDE, a joint row
+// tree, an AIR-specific composition, then the shared FRI path.
+enum RowAir : int { ROW_NONE = 0, ROW_RANGE, ROW_TABLE };
 
 struct Proof {
     // request
     bool air = true;
-    bool range = false;  // the range-check AIR (air == false)
+    int row = ROW_NONE;  // a row AIR (air == false)
     z0p_air_params ap{};
     z0p_range_params rp{};
+    z0p_table_params tp{};
     z0p_fri_params fp{};
     Transcript* tr = nullptr;
     const uint32_t* coeff_src = nullptr;  // FRI: host coefficients or LDE values
     int fri_in = FRI_COEFFS;
     // Layer 0 holds base-field values (4 bytes each): the FRI commitments
     // of base coefficients or base LDE values.
-    bool base0() const { return !air && !range && (fri_in == FRI_COEFFS || fri_in == FRI_EVALS); }
+    bool base0() const { return !air && !row && (fri_in == FRI_COEFFS || fri_in == FRI_EVALS); }
     uint64_t src_words() const {
         if (air) return n;
-        if (range) return host_a ? ncols * n : 0;
+        if (row) return host_a ? ncols * n : 0;
         return (fri_in == FRI_COEFFS_EXT || fri_in == FRI_EVALS_EXT ? 4 : 1) * n;
     }
     // AIR with both columns interpolated and extended (air.rs:385-406,
@@ -473,13 +480,16 @@ struct Proof {
     const uint32_t *host_a = nullptr, *host_b = nullptr;  // explicit trace columns
     std::vector<uint32_t> mask;                           // hiding: A's n_rand maskers, then B's
     uint64_t n_rand = 0;
-    // range AIR: n_bits + 1 columns, V first; the constant columns of
-    // range::prove as their constant coefficients (host_a == nullptr), or
-    // the caller's evaluation columns in host_a.
+    // Row AIRs: ncols columns, V first (range: n_bits + 1; table: 1); the
+    // constant columns of the provers as their constant coefficients
+    // (host_a == nullptr), or the caller's evaluation columns in host_a.
     uint64_t ncols = 0;
     std::vector<uint32_t> const_cols;
     uint32_t* rows = nullptr;  // ncols bit-reversed LDE columns, m words apart
     RangeConsts rc{};
+    TableConsts tc{};
+    std::vector<uint32_t> table;  // the public table, canonical
+    uint32_t* table_dev = nullptr;  // the table in Montgomery form
     uint64_t tag = 0;
     // geometry
     uint64_t n = 0, B = 0, m = 0, poly_len = 0, degree_bound = 0;
@@ -532,15 +542,16 @@ size_t layout(Proof& P, uint8_t* base) {
         return p;
     };
     P.col = take(P.src_words());
-    if (P.range) {
+    if (P.row) {
         P.coeff = take(P.ncols * P.n);
+        if (P.row == ROW_TABLE) P.table_dev = take(P.table.size());
         if (P.n_rand) P.mask_dev = take(P.ncols * P.n_rand);
         P.rows = take(P.ncols * P.m);
         P.trace_tree = take(P.tg.words());
     } else {
         P.coeff = take(!P.air && P.fri_in == FRI_COEFFS_EXT ? 4 * P.n : P.n);
     }
-    if (!P.air && !P.range && P.fri_in == FRI_COEFFS_EXT) P.planar = take(4 * P.m);
+    if (!P.air && !P.row && P.fri_in == FRI_COEFFS_EXT) P.planar = take(4 * P.m);
     if (P.two_col) {
         P.colB = take(P.n);
         P.coeffB = take(P.n);
@@ -577,7 +588,7 @@ void init_fri_shape(Proof& P, const z0p_fri_params& fp) {
     for (size_t d = 0; d < P.folds; ++d) w += (d == 0 && P.base0() ? 4 : 16) + 8 * uint32_t(P.geoms[d].target);
     P.fri_wpq = w;
     if (P.air) w += 32 + 16 * uint32_t(P.tg.target);
-    if (P.range) w += uint32_t(P.ncols) + 8 * uint32_t(P.tg.target);
+    if (P.row) w += uint32_t(P.ncols) + 8 * uint32_t(P.tg.target);
     P.wpq = w;
     P.stage_need = size_t(fp.num_queries) * P.wpq + 256 * 8 + P.lens[P.folds] * 4 + 256;
 }
@@ -648,30 +659,36 @@ void init_air_zk(Ctx& c, Proof& P, const z0p_air_params& p, uint64_t n_rand, Cha
     init_air_shape(c, P, p, tr, padded * p.blowup, n + n_rand - 1);
 }
 
-// Range AIR geometry over an m-point LDE (m / n cosets of n points), one
-// tree leaf per LDE row, composition degree below degree_bound + 1.
-void init_range_shape(Ctx& c, Proof& P, const z0p_range_params& p, Transcript* tr, uint64_t m, uint64_t degree_bound) {
-    const z0p_fri_params fp = range_fri_params(p);
+// Row AIR geometry over an m-point LDE (m / n cosets of n points) of ncols
+// columns, one tree leaf per LDE row, composition degree below
+// degree_bound + 1, committed by fri::commit_ext_evals (fri.rs:527-549).
+void init_row_shape(Ctx& c, Proof& P, int kind, const z0p_fri_params& fp, uint64_t n, uint64_t ncols,
+                    Transcript* tr, uint64_t m, uint64_t degree_bound) {
     validate_fri(fp);
+    require(degree_bound < m, Z0P_ERR_MALFORMED_INPUT);  // rate < 1 (fri.rs:535-540)
     P.air = false;
-    P.range = true;
-    P.rp = p;
-    P.rp.value = p.value % P_MOD;
+    P.row = kind;
     P.tr = tr;
-    P.n = p.n_rows;
-    P.ncols = p.n_bits + 1;
+    P.n = n;
+    P.ncols = ncols;
     P.m = m;
     P.B = m / P.n;
     P.logn = int(ilog2(P.n));
     P.logb = int(ilog2(P.B));
     P.poly_len = P.m;
     P.degree_bound = degree_bound;
-    P.tg = tree_geom(P.m, p.cap_height);
+    P.tg = tree_geom(P.m, fp.cap_height);
     init_fri_shape(P, fp);
-    P.rc.value = P.rp.value;
-    P.rc.nbits = uint32_t(p.n_bits);
     P.gt = &c.geometry(P.logn, P.logb);
     P.mem_bytes = layout(P, nullptr);
+}
+
+void init_range_shape(Ctx& c, Proof& P, const z0p_range_params& p, Transcript* tr, uint64_t m, uint64_t degree_bound) {
+    P.rp = p;
+    P.rp.value = p.value % P_MOD;
+    P.rc.value = P.rp.value;
+    P.rc.nbits = uint32_t(p.n_bits);
+    init_row_shape(c, P, ROW_RANGE, range_fri_params(p), p.n_rows, p.n_bits + 1, tr, m, degree_bound);
 }
 
 // range::build_columns (range.rs:312-327) as coefficients: V = value and
@@ -717,6 +734,45 @@ void init_range_zk(Ctx& c, Proof& P, const z0p_range_params& p, uint64_t n_rand,
     range_const_cols(P, p.value % P_MOD, p.n_bits);
     P.n_rand = n_rand;
     init_range_shape(c, P, p, tr, padded * p.blowup, n + 2 * n_rand - 2);
+}
+
+// table.rs:291-298: at least two entries.
+void init_table_common(Proof& P, const z0p_table_params& p, const uint32_t* table, uint64_t table_len) {
+    validate_table(p);
+    require(table_len >= 2 && table, Z0P_ERR_MALFORMED_INPUT);
+    P.tp = p;
+    P.tp.value = p.value % P_MOD;
+    P.tc.value = P.tp.value;
+    P.table.resize(table_len);
+    for (uint64_t k = 0; k < table_len; ++k) P.table[k] = table[k] % P_MOD;
+    P.const_cols.assign(1, P.tp.value);
+}
+
+// table::prove (table.rs:305-318): the constant value column.
+void init_table(Ctx& c, Proof& P, const z0p_table_params& p, const uint32_t* table, uint64_t table_len,
+                Transcript* tr) {
+    init_table_common(P, p, table, table_len);
+    init_row_shape(c, P, ROW_TABLE, table_fri_params(p), p.n_rows, 1, tr, p.n_rows * p.blowup,
+                   membership_bound(table_len, p.n_rows, 0));
+}
+
+// table::prove_zk (table.rs:329-355): the column masked by blind::mask_coeffs
+// (blind.rs:38-79); checks and rng draws in the reference's order.
+void init_table_zk(Ctx& c, Proof& P, const z0p_table_params& p, const uint32_t* table, uint64_t table_len,
+                   uint64_t n_rand, ChaCha20& rng, Transcript* tr) {
+    init_table_common(P, p, table, table_len);
+    require(n_rand >= p.num_queries, Z0P_ERR_MALFORMED_INPUT);
+    const uint64_t n = p.n_rows;
+    require(n_rand <= MAX_DOMAIN, Z0P_ERR_INVALID_DOMAIN_SIZE);
+    uint64_t padded = 1;
+    while (padded < n + n_rand) padded <<= 1;
+    require(padded <= MAX_DOMAIN, Z0P_ERR_INVALID_DOMAIN_SIZE);
+    P.mask.resize(n_rand);
+    for (auto& v : P.mask) v = rng.next_field();
+    require(padded <= MAX_DOMAIN / p.blowup, Z0P_ERR_INVALID_DOMAIN_SIZE);  // evaluate_on_coset (ntt.rs:341-352)
+    P.n_rand = n_rand;
+    init_row_shape(c, P, ROW_TABLE, table_fri_params(p), n, 1, tr, padded * p.blowup,
+                   membership_bound(table_len, n, n_rand));
 }
 
 // fri::commit / commit_ext (fri.rs:408-461): count coefficients, extended
@@ -776,19 +832,41 @@ void start(Ctx& c, Proof& P) {
         }
         if (P.n_rand)
             hip_check(hipMemcpyAsync(P.mask_dev, P.mask.data(), P.mask.size() * 4, hipMemcpyHostToDevice, c.stream));
-    } else if (P.range) {
-        // absorb_params (range.rs:723-735): 56 bytes, grinding_bits as u32.
-        const z0p_range_params& p = P.rp;
-        uint8_t pb[56];
-        store_le64(pb, p.n_rows);
-        store_le64(pb + 8, p.n_bits);
-        store_le64(pb + 16, p.blowup);
-        store_le64(pb + 24, p.num_queries);
-        store_le64(pb + 32, p.final_size);
-        store_le32(pb + 40, p.grinding_bits);
-        store_le64(pb + 44, p.cap_height);
-        store_le32(pb + 52, p.value);
-        tr.absorb("range/params", pb, 56);
+    } else if (P.row) {
+        if (P.row == ROW_RANGE) {
+            // absorb_params (range.rs:723-735): 56 bytes, grinding_bits as u32.
+            const z0p_range_params& p = P.rp;
+            uint8_t pb[56];
+            store_le64(pb, p.n_rows);
+            store_le64(pb + 8, p.n_bits);
+            store_le64(pb + 16, p.blowup);
+            store_le64(pb + 24, p.num_queries);
+            store_le64(pb + 32, p.final_size);
+            store_le32(pb + 40, p.grinding_bits);
+            store_le64(pb + 44, p.cap_height);
+            store_le32(pb + 52, p.value);
+            tr.absorb("range/params", pb, 56);
+        } else {
+            // absorb_params (table.rs:540-555): 48 bytes, |T| as u64, the entries.
+            const z0p_table_params& p = P.tp;
+            std::vector<uint8_t> pb(56 + 4 * P.table.size());
+            store_le64(pb.data(), p.n_rows);
+            store_le64(pb.data() + 8, p.blowup);
+            store_le64(pb.data() + 16, p.num_queries);
+            store_le64(pb.data() + 24, p.final_size);
+            store_le32(pb.data() + 32, p.grinding_bits);
+            store_le64(pb.data() + 36, p.cap_height);
+            store_le32(pb.data() + 44, p.value);
+            store_le64(pb.data() + 48, P.table.size());
+            std::vector<uint32_t> tm(P.table.size());
+            for (size_t k = 0; k < P.table.size(); ++k) {
+                store_le32(pb.data() + 56 + 4 * k, P.table[k]);
+                tm[k] = to_mont(P.table[k]);
+            }
+            tr.absorb("table/params", pb);
+            hip_check(hipMemcpyAsync(P.table_dev, tm.data(), tm.size() * 4, hipMemcpyHostToDevice, c.stream));
+            hip_check(hipStreamSynchronize(c.stream));  // tm is a local
+        }
         if (P.host_a) {
             hip_check(hipMemcpyAsync(P.col, P.host_a, P.ncols * P.n * 4, hipMemcpyHostToDevice, c.stream));
         } else {
@@ -839,7 +917,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
     const GeometryTables& gt = *P.gt;
     switch (P.phase) {
         case PH_FRONT: {
-            if (P.range) {
+            if (P.row) {
                 const uint32_t ninv = to_mont(inv_c(uint32_t(P.n % P_MOD)));
                 for (uint64_t k = 0; k < P.ncols; ++k) {
                     uint32_t* coeff = P.coeff + k * P.n;
@@ -981,7 +1059,24 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
             break;
         }
         case PH_COMPOSE: {
-            if (P.range) {
+            if (P.row == ROW_TABLE) {
+                TableComposeJob tj{};
+                tj.v = P.rows;
+                tj.layer0 = P.vals[0];
+                tj.xn = gt.xn;
+                tj.shift_m = gt.coset.shift;
+                tj.zinv_m = gt.zinv;
+                tj.c = static_cast<const TableConsts*>(cst ? cst : c.ring.push(&P.tc, sizeof(P.tc)));
+                tj.table_m = P.table_dev;
+                tj.table_len = uint32_t(P.table.size());
+                tj.logn = P.logn;
+                tj.logb = P.logb;
+                ch.push_back(tj);
+                tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_EXT, P.vals[0]), c.sched);
+                collect_cap(c, P, P.trees[0], P.geoms[0], items, used);
+                break;
+            }
+            if (P.row == ROW_RANGE) {
                 RangeComposeJob rj{};
                 rj.cols = P.rows;
                 rj.col_stride = P.m;
@@ -1092,7 +1187,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 o.trace_tree = P.trace_tree;
                 o.trace_leaves = P.tg.leaves;
                 o.trace_target = P.tg.target;
-            } else if (P.range) {
+            } else if (P.row) {
                 o.cols = P.rows;
                 o.ncols = int(P.ncols);
                 o.col_stride = P.m;
@@ -1230,8 +1325,9 @@ void serialize(Proof& P, const uint32_t* words) {
             w.words(sw, 16);
         }
         P.proof = w.finish();
-    } else if (P.range) {
-        // range::Proof::to_bytes (range.rs:240-260).
+    } else if (P.row) {
+        // range::Proof::to_bytes (range.rs:240-260), table::Proof::to_bytes
+        // (table.rs:244-262: no bit columns).
         Writer w;
         w.reserve(estimate);
         w.byte(WIRE_VERSION);
@@ -1245,8 +1341,10 @@ void serialize(Proof& P, const uint32_t* words) {
             const uint64_t index = brev_host(P.queries[q], logm);
             w.u64(index);
             w.u32(rw[0]);
-            w.u32(uint32_t(P.ncols - 1));
-            w.words(rw + 1, P.ncols - 1);
+            if (P.row == ROW_RANGE) {
+                w.u32(uint32_t(P.ncols - 1));
+                w.words(rw + 1, P.ncols - 1);
+            }
             write_path(w, rw + P.ncols, P.tg.target, index);
         }
         P.proof = w.finish();
@@ -1287,15 +1385,21 @@ void host_step(Ctx& c, Proof& P) {
                 store_le64(dims + 16, P.degree_bound);
                 P.tr->absorb("fri-dims", dims, 24);
                 P.phase = PH_COMPOSE;
-            } else if (P.range) {
-                // range.rs:464-475: the row cap, then 1 + n_bits + 1 alphas.
+            } else if (P.row) {
+                // range.rs:464-475: the row cap, then 1 + n_bits + 1 alphas;
+                // table.rs:391-397: the column cap, then 2 alphas.
                 P.column_cap = read_digests(w, uint64_t(1) << P.tg.drop);
                 std::vector<uint8_t> cb(P.column_cap.size() * 32);
                 for (size_t i = 0; i < P.column_cap.size(); ++i) std::memcpy(cb.data() + 32 * i, P.column_cap[i].data(), 32);
-                P.tr->absorb("range/column", cb);
-                for (uint64_t k = 0; k < P.ncols + 1; ++k) {
-                    E4 al = P.tr->challenge_ext("range/alpha");
-                    for (int q = 0; q < 4; ++q) P.rc.alpha_m[k][q] = to_mont(al.v[q]);
+                const bool rng_air = P.row == ROW_RANGE;
+                P.tr->absorb(rng_air ? "range/column" : "table/column", cb);
+                const uint64_t nalpha = rng_air ? P.ncols + 1 : 2;
+                for (uint64_t k = 0; k < nalpha; ++k) {
+                    E4 al = P.tr->challenge_ext(rng_air ? "range/alpha" : "table/alpha");
+                    for (int q = 0; q < 4; ++q) {
+                        if (rng_air) P.rc.alpha_m[k][q] = to_mont(al.v[q]);
+                        else P.tc.alpha_m[k][q] = to_mont(al.v[q]);
+                    }
                 }
                 uint8_t dims[24];
                 store_le64(dims, P.m);
@@ -1507,7 +1611,7 @@ void Engine::pipeline(Proof& P) {
         size_t stage_off;
     };
     std::vector<Ph> phs{{PH_FRONT, 0, 0}};
-    if (P.air || P.range) phs.push_back({PH_COMPOSE, 0, 0});
+    if (P.air || P.row) phs.push_back({PH_COMPOSE, 0, 0});
     for (size_t d = 0; d < P.folds; ++d) phs.push_back({PH_FOLD, d, 0});
     if (P.fp.grinding_bits > 0) phs.push_back({PH_GRIND, 0, 0});
     constexpr size_t SLOT = Ctx::MAILBOX_SLOT;
@@ -1517,7 +1621,8 @@ void Engine::pipeline(Proof& P) {
     const bool with_open = phs.size() + open_slots <= Ctx::MAILBOX_SLOTS;
     if (with_open) phs.push_back({PH_OPEN, 0, 0});
     require(phs.size() <= Ctx::MAILBOX_SLOTS, Z0P_ERR_DEVICE);
-    static_assert(sizeof(ComposeConsts) <= SLOT && sizeof(RangeConsts) <= SLOT && sizeof(FoldConsts) <= SLOT &&
+    static_assert(sizeof(ComposeConsts) <= SLOT && sizeof(RangeConsts) <= SLOT && sizeof(TableConsts) <= SLOT &&
+                      sizeof(FoldConsts) <= SLOT &&
                       64 <= SLOT,
                   "mailbox slot");
     c_.reserve_stage(P.stage_need * phs.size());
@@ -1586,7 +1691,8 @@ void Engine::pipeline(Proof& P) {
                     cancelled = !live;
                 } else {
                     require(P.phase == nx.ph && (nx.ph != PH_FOLD || P.depth == nx.depth), Z0P_ERR_DEVICE);
-                    if (nx.ph == PH_COMPOSE && P.range) std::memcpy(slot, &P.rc, sizeof(P.rc));
+                    if (nx.ph == PH_COMPOSE && P.row == ROW_RANGE) std::memcpy(slot, &P.rc, sizeof(P.rc));
+                    else if (nx.ph == PH_COMPOSE && P.row == ROW_TABLE) std::memcpy(slot, &P.tc, sizeof(P.tc));
                     else if (nx.ph == PH_COMPOSE) std::memcpy(slot, &P.cc, sizeof(P.cc));
                     else if (nx.ph == PH_FOLD) std::memcpy(slot, &P.fold, sizeof(P.fold));
                     else std::memcpy(slot, P.pow_hw, sizeof(P.pow_hw));
@@ -1674,6 +1780,20 @@ Proof* proof_new_range_zk(Ctx& c, const z0p_range_params& p, uint64_t n_rand, Ch
     return P.release();
 }
 
+Proof* proof_new_table(Ctx& c, const z0p_table_params& p, const uint32_t* table, uint64_t table_len,
+                       Transcript* tr) {
+    auto P = std::make_unique<Proof>();
+    init_table(c, *P, p, table, table_len, tr);
+    return P.release();
+}
+
+Proof* proof_new_table_zk(Ctx& c, const z0p_table_params& p, const uint32_t* table, uint64_t table_len,
+                          uint64_t n_rand, ChaCha20& rng, Transcript* tr) {
+    auto P = std::make_unique<Proof>();
+    init_table_zk(c, *P, p, table, table_len, n_rand, rng, tr);
+    return P.release();
+}
+
 Proof* proof_new_fri(Ctx& c, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,
                      const z0p_fri_params& fp, Transcript* tr) {
     auto P = std::make_unique<Proof>();
@@ -1730,6 +1850,15 @@ Bytes range_prove_with_columns(Ctx& ctx, const z0p_range_params& p, const uint32
 
 Bytes range_prove_zk(Ctx& ctx, const z0p_range_params& p, uint64_t n_rand, ChaCha20& rng, Transcript& tr) {
     return run_one(ctx, proof_new_range_zk(ctx, p, n_rand, rng, &tr));
+}
+
+Bytes table_prove(Ctx& ctx, const z0p_table_params& p, const uint32_t* table, uint64_t table_len, Transcript& tr) {
+    return run_one(ctx, proof_new_table(ctx, p, table, table_len, &tr));
+}
+
+Bytes table_prove_zk(Ctx& ctx, const z0p_table_params& p, const uint32_t* table, uint64_t table_len, uint64_t n_rand,
+                     ChaCha20& rng, Transcript& tr) {
+    return run_one(ctx, proof_new_table_zk(ctx, p, table, table_len, n_rand, rng, &tr));
 }
 
 void fri_commit_open(Ctx& ctx, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,

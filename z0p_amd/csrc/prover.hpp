@@ -38,7 +38,7 @@ TreeGeom tree_geom(uint64_t leaves, uint64_t cap_height);
 // One kernel step of one proof; a proof's steps of one iteration form a
 // chain that must run in order.
 using Step = std::variant<FibCoeffJob, BitrevJob, InterleaveJob, NttJob, LinkedJob, ComposeJob, RangeComposeJob,
-                          LeavesJob, LevelsJob, TailJob, FoldJob, SmallCommitJob, GrindJob, OpenJob>;
+                          TableComposeJob, LeavesJob, LevelsJob, TailJob, FoldJob, SmallCommitJob, GrindJob, OpenJob>;
 
 // Leaves, then wide levels, then LDS tail launches up to the cap level.
 void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves, const Schedule& s);
@@ -66,6 +66,11 @@ Proof* proof_new_range(Ctx& c, const z0p_range_params& p, Transcript* tr);
 Proof* proof_new_range_cols(Ctx& c, const z0p_range_params& p, const uint32_t* cols, uint64_t ncols, uint64_t len,
                             Transcript* tr);
 Proof* proof_new_range_zk(Ctx& c, const z0p_range_params& p, uint64_t n_rand, ChaCha20& rng, Transcript* tr);
+// table::prove / prove_zk (table.rs:305-355): table_len >= 2 entries.
+Proof* proof_new_table(Ctx& c, const z0p_table_params& p, const uint32_t* table, uint64_t table_len,
+                       Transcript* tr);
+Proof* proof_new_table_zk(Ctx& c, const z0p_table_params& p, const uint32_t* table, uint64_t table_len,
+                          uint64_t n_rand, ChaCha20& rng, Transcript* tr);
 // What a FRI commitment starts from (fri.rs:408-549).
 enum FriInput : int {
     FRI_COEFFS = 0,     // commit: base-field coefficients
@@ -147,12 +152,24 @@ Bytes range_prove(Ctx& ctx, const z0p_range_params& p, Transcript& tr);
 Bytes range_prove_with_columns(Ctx& ctx, const z0p_range_params& p, const uint32_t* cols, uint64_t ncols,
                                uint64_t len, Transcript& tr);
 Bytes range_prove_zk(Ctx& ctx, const z0p_range_params& p, uint64_t n_rand, ChaCha20& rng, Transcript& tr);
+Bytes table_prove(Ctx& ctx, const z0p_table_params& p, const uint32_t* table, uint64_t table_len, Transcript& tr);
+Bytes table_prove_zk(Ctx& ctx, const z0p_table_params& p, const uint32_t* table, uint64_t table_len, uint64_t n_rand,
+                     ChaCha20& rng, Transcript& tr);
 
 // Host verifier (air.rs:607-752, fri.rs:686-859); throws Status on rejection.
 void air_verify(const z0p_air_params& p, Transcript& tr, const uint8_t* proof, size_t len);
 void fri_verify(const uint8_t* cm, size_t cm_len, const uint8_t* pf, size_t pf_len, const z0p_fri_params& fp,
                 Transcript& tr);
 void range_verify(const z0p_range_params& p, Transcript& tr, const uint8_t* proof, size_t len);
+void table_verify(const z0p_table_params& p, const uint32_t* table, uint64_t table_len, Transcript& tr,
+                  const uint8_t* proof, size_t len);
+// table.rs:357-363: the saturating bound |T| (n + n_rand - 1) - n.
+inline uint64_t membership_bound(uint64_t table_len, uint64_t n, uint64_t n_rand) {
+    const uint64_t rows = n_rand > UINT64_MAX - n ? UINT64_MAX : n + n_rand;
+    const unsigned __int128 prod = (unsigned __int128)table_len * (rows == 0 ? 0 : rows - 1);
+    const uint64_t sat = prod > UINT64_MAX ? UINT64_MAX : uint64_t(prod);
+    return sat < n ? 0 : sat - n;
+}
 
 }  // namespace z0p
 

@@ -1,6 +1,6 @@
-// Host verifier for the AIR, range-check and FRI proofs (the reference
-// verifies on the CPU: air.rs:607-752, range.rs:521-721, fri.rs:686-859,
-// 1153-1227, 1339-1364). It parses the
+// Host verifier for the AIR, range-check, table-membership and FRI proofs
+// (the reference verifies on the CPU: air.rs:607-752, range.rs:521-721,
+// table.rs:430-538, fri.rs:686-859, 1153-1227, 1339-1364). It parses the
 // VERSION = 6 bytes fail-closed (codec.rs:118-274) and replays the
 // transcript exactly as the prover advanced it.
 #include <algorithm>
@@ -559,6 +559,100 @@ void range_verify(const z0p_range_params& p, Transcript& tr, const uint8_t* byte
         require(verify_capped_row(column_cap[e], row, index, depth, o.path), Z0P_ERR_BAD_MERKLE_PATH);
         const uint32_t x = mul(GENERATOR, pow_c(omega_m, index));
         const E4 expected = range_composition_at(o.v, o.bits, x, value, n, al);
+        require(!pf.queries[q].empty(), Z0P_ERR_FRI_VERIFICATION_FAILED);
+        require(e4_eq(pf.queries[q][0].values[leaf % 4], expected), Z0P_ERR_FRI_VERIFICATION_FAILED);
+    }
+}
+
+void table_verify(const z0p_table_params& p, const uint32_t* table_in, uint64_t table_len, Transcript& tr,
+                  const uint8_t* bytes, size_t len) {
+    // Proof::from_bytes (table.rs:265-288)
+    struct ValueOpening {
+        uint64_t index = 0;
+        uint32_t v = 0;
+        Path path;
+    };
+    Reader r(bytes, len);
+    r.version();
+    std::vector<Digest> column_cap = read_cap(r);
+    CommitmentV cm = read_commitment_body(r);
+    FriProofV pf = read_fri_proof_body(r);
+    std::vector<ValueOpening> ops;
+    uint32_t no = r.count(MAX_QUERIES);
+    for (uint32_t i = 0; i < no; ++i) {
+        ValueOpening o;
+        o.index = r.u64();
+        o.v = r.field();
+        o.path = read_path(r);
+        ops.push_back(std::move(o));
+    }
+    r.end();
+
+    // table::verify (table.rs:430-538)
+    validate_table(p);
+    require(table_len >= 2 && table_in, Z0P_ERR_MALFORMED_INPUT);
+    std::vector<uint32_t> table(table_len);
+    for (uint64_t k = 0; k < table_len; ++k) table[k] = table_in[k] % P;
+    const uint64_t n = p.n_rows;
+    const uint64_t m = cm.lde_len;
+    const uint64_t expected_m = n * p.blowup;  // validated: <= 2^27
+    require(m >= expected_m && m % expected_m == 0, Z0P_ERR_FRI_VERIFICATION_FAILED);
+    if (m == expected_m) {
+        require(cm.degree_bound == membership_bound(table_len, n, 0), Z0P_ERR_FRI_VERIFICATION_FAILED);
+    } else {
+        // masked column: p = m / blowup coefficients bound the honest degree
+        const uint64_t padded = m / p.blowup;
+        const unsigned __int128 prod = (unsigned __int128)table_len * (padded == 0 ? 0 : padded - 1);
+        require(prod <= UINT64_MAX && prod >= n && cm.degree_bound <= uint64_t(prod) - n,
+                Z0P_ERR_FRI_VERIFICATION_FAILED);
+    }
+    require(cm.poly_len == m, Z0P_ERR_FRI_VERIFICATION_FAILED);
+    const uint32_t value = p.value % P;
+    std::vector<uint8_t> pb(56 + 4 * table_len);
+    store_le64(pb.data(), p.n_rows);
+    store_le64(pb.data() + 8, p.blowup);
+    store_le64(pb.data() + 16, p.num_queries);
+    store_le64(pb.data() + 24, p.final_size);
+    store_le32(pb.data() + 32, p.grinding_bits);
+    store_le64(pb.data() + 36, p.cap_height);
+    store_le32(pb.data() + 44, value);
+    store_le64(pb.data() + 48, table_len);
+    for (uint64_t k = 0; k < table_len; ++k) store_le32(pb.data() + 56 + 4 * k, table[k]);
+    tr.absorb("table/params", pb);
+    require(ops.size() == p.num_queries, Z0P_ERR_FRI_VERIFICATION_FAILED);
+    {
+        std::vector<uint8_t> cb;
+        for (auto& d : column_cap) cb.insert(cb.end(), d.begin(), d.end());
+        tr.absorb("table/column", cb);
+    }
+    const E4 a_member = tr.challenge_ext("table/alpha");
+    const E4 a_bound = tr.challenge_ext("table/alpha");
+    const z0p_fri_params fp = table_fri_params(p);
+    std::vector<uint64_t> idx = fri_verify_impl(cm, pf, fp, tr);
+    require(pow2(m), Z0P_ERR_FRI_VERIFICATION_FAILED);
+    const uint64_t log_m = ilog2(m);
+    const uint32_t omega_m = root_of_unity(uint32_t(log_m));
+    const size_t count = std::min(std::min(pf.queries.size(), ops.size()), idx.size());
+    for (size_t q = 0; q < count; ++q) {
+        const ValueOpening& o = ops[q];
+        const uint64_t leaf = idx[q];
+        const uint64_t index = brev64(leaf, log_m);
+        require(o.index == index, Z0P_ERR_FRI_VERIFICATION_FAILED);
+        const uint64_t depth = log_m;
+        const uint64_t drop = std::min<uint64_t>(p.cap_height, depth);
+        require(column_cap.size() == (uint64_t(1) << drop) && o.path.size() == depth - drop, Z0P_ERR_BAD_MERKLE_PATH);
+        const uint64_t e = index >> (depth - drop);
+        require(e < column_cap.size(), Z0P_ERR_BAD_MERKLE_PATH);
+        std::vector<uint8_t> row(4);
+        store_le32(row.data(), o.v);
+        require(verify_capped_row(column_cap[e], row, index, depth, o.path), Z0P_ERR_BAD_MERKLE_PATH);
+        // table.rs:646-663
+        const uint32_t x = mul(GENERATOR, pow_c(omega_m, index));
+        uint32_t prod = 1;
+        for (uint32_t t : table) prod = mul(prod, sub(o.v, t));
+        const uint32_t s_m = mul(prod, inv_c(sub(pow_c(x, n), 1)));
+        const uint32_t s_b = mul(sub(o.v, value), inv_c(sub(x, 1)));
+        const E4 expected = e4_add(e4_mul_base(a_member, s_m), e4_mul_base(a_bound, s_b));
         require(!pf.queries[q].empty(), Z0P_ERR_FRI_VERIFICATION_FAILED);
         require(e4_eq(pf.queries[q][0].values[leaf % 4], expected), Z0P_ERR_FRI_VERIFICATION_FAILED);
     }
