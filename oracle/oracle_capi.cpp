@@ -10,6 +10,7 @@
 #include "air.hpp"
 #include "range.hpp"
 #include "table.hpp"
+#include "fold.hpp"
 
 using namespace oracle;
 
@@ -34,6 +35,13 @@ typedef struct {
     uint64_t cap_height;
     uint32_t value, reserved1;
 } oracle_table_params;
+
+typedef struct {
+    uint64_t seg_steps, blowup, num_queries, final_size;
+    uint32_t grinding_bits, reserved0;
+    uint64_t cap_height;
+    uint32_t in_a, in_b, out_b, reserved1;
+} oracle_fold_params;
 
 typedef struct {
     uint64_t n_rows, n_bits, blowup, num_queries, final_size;
@@ -119,6 +127,33 @@ void from_table(const table::Params& q, oracle_table_params* p) {
     p->grinding_bits = q.grinding_bits;
     p->cap_height = q.cap_height;
     p->value = q.value.v;
+}
+
+fold::Params to_fold(const oracle_fold_params* p) {
+    fold::Params q;
+    q.seg_steps = p->seg_steps;
+    q.blowup = p->blowup;
+    q.num_queries = p->num_queries;
+    q.final_size = p->final_size;
+    q.grinding_bits = p->grinding_bits;
+    q.cap_height = p->cap_height;
+    q.in_a = Field::make(p->in_a);
+    q.in_b = Field::make(p->in_b);
+    q.out_b = Field::make(p->out_b);
+    return q;
+}
+
+void from_fold(const fold::Params& q, oracle_fold_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->seg_steps = q.seg_steps;
+    p->blowup = q.blowup;
+    p->num_queries = q.num_queries;
+    p->final_size = q.final_size;
+    p->grinding_bits = q.grinding_bits;
+    p->cap_height = q.cap_height;
+    p->in_a = q.in_a.v;
+    p->in_b = q.in_b.v;
+    p->out_b = q.out_b.v;
 }
 
 fri::Params to_fri(const oracle_fri_params* p) {
@@ -599,6 +634,67 @@ int oracle_table_verify(const oracle_table_params* p, const uint32_t* tab, size_
     GUARD({
         table::Proof pr = table::Proof::from_bytes(proof, len);
         table::verify(to_table(p), to_fields(tab, tlen), pr, *static_cast<Transcript*>(t));
+    })
+}
+
+// fold.rs: parameters, the accumulator and the verifier.
+int oracle_fold_params_new(uint64_t seg, uint64_t blowup, uint64_t nq, uint64_t fs, uint32_t in_a, uint32_t in_b,
+                           uint32_t out_b, oracle_fold_params* out) {
+    GUARD(from_fold(fold::Params::make(seg, blowup, nq, fs, Field::make(in_a), Field::make(in_b), Field::make(out_b)),
+                    out))
+}
+int oracle_fold_params_for_security(uint64_t seg, uint32_t in_a, uint32_t in_b, uint32_t out_b, uint32_t bits,
+                                    oracle_fold_params* out) {
+    GUARD(from_fold(fold::Params::for_security(seg, Field::make(in_a), Field::make(in_b), Field::make(out_b), bits),
+                    out))
+}
+void* oracle_fold_acc_new(const oracle_fold_params* p) { return new fold::Accumulator(to_fold(p)); }
+void oracle_fold_acc_free(void* a) { delete static_cast<fold::Accumulator*>(a); }
+size_t oracle_fold_acc_len(void* a) { return static_cast<fold::Accumulator*>(a)->len(); }
+// Accumulator::params is fixed at creation; tests change it through this
+// call to drive the reference's error paths.
+void oracle_fold_acc_set_params(void* a, const oracle_fold_params* p) {
+    static_cast<fold::Accumulator*>(a)->params = to_fold(p);
+}
+int oracle_fold_acc_add_segment(void* a, const uint32_t* col_a, size_t len_a, const uint32_t* col_b, size_t len_b,
+                                void* t) {
+    GUARD(static_cast<fold::Accumulator*>(a)->add_segment(to_fields(col_a, len_a), to_fields(col_b, len_b),
+                                                          *static_cast<Transcript*>(t)))
+}
+int oracle_fold_acc_finalize(void* a, void* t, uint8_t** out, size_t* len) {
+    GUARD({
+        fold::IvcProof pr = static_cast<fold::Accumulator*>(a)->finalize(*static_cast<Transcript*>(t));
+        *out = dup_bytes(pr.to_bytes(), len);
+    })
+}
+// fold.rs:927-992 (forged_segment_a_jump_rejected): step `k`'s out_a claim
+// replaced, the transcript replayed from `label` and every composition
+// recomputed, so the proof is internally consistent apart from the claim.
+int oracle_fold_acc_finalize_forged_out_a(void* a, size_t k, uint32_t out_a, const uint8_t* label, size_t label_len,
+                                          uint8_t** out, size_t* len) {
+    GUARD({
+        fold::Accumulator forged = *static_cast<fold::Accumulator*>(a);
+        if (k >= forged.steps.size()) throw Z0pError(ERR_INDEX_OUT_OF_BOUNDS);
+        forged.steps[k].claims.out_a = Field::make(out_a);
+        Transcript tr(label, label_len);
+        fold::absorb_params(forged.params, tr);
+        size_t s = forged.params.seg_steps, m = forged.params.lde_len();
+        Field w_last = primitive_root(s).pow(s - 1);
+        size_t drop = capped_shape(m, forged.params.cap_height).first;
+        for (auto& st : forged.steps) {
+            fold::absorb_step(st.claims, tr);
+            tr.absorb("fold/trace", cap_bytes(st.tree.cap(drop)));
+            std::array<Ext4, fold::ALPHAS> al;
+            for (auto& x : al) x = tr.challenge_ext("fold/alpha");
+            st.composition = fold::compose_step(st.ldes, w_last, st.claims, s, m / s, al);
+        }
+        *out = dup_bytes(forged.finalize(tr).to_bytes(), len);
+    })
+}
+int oracle_fold_verify(const oracle_fold_params* p, void* t, const uint8_t* proof, size_t len) {
+    GUARD({
+        fold::IvcProof pr = fold::IvcProof::from_bytes(proof, len);
+        fold::verify(to_fold(p), pr, *static_cast<Transcript*>(t));
     })
 }
 

@@ -74,6 +74,31 @@ class TableParams(ctypes.Structure):
         return q
 
 
+class FoldParams(ctypes.Structure):
+    """Layout shared with ``z0p_fold_params`` (include/z0p.h)."""
+
+    _fields_ = [
+        ("seg_steps", ctypes.c_uint64),
+        ("blowup", ctypes.c_uint64),
+        ("num_queries", ctypes.c_uint64),
+        ("final_size", ctypes.c_uint64),
+        ("grinding_bits", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
+        ("cap_height", ctypes.c_uint64),
+        ("in_a", ctypes.c_uint32),
+        ("in_b", ctypes.c_uint32),
+        ("out_b", ctypes.c_uint32),
+        ("reserved1", ctypes.c_uint32),
+    ]
+
+    def replace(self, **kw):
+        q = FoldParams()
+        ctypes.pointer(q)[0] = self
+        for k, v in kw.items():
+            setattr(q, k, v)
+        return q
+
+
 class RangeParams(ctypes.Structure):
     """Layout shared with ``z0p_range_params`` (include/z0p.h)."""
 
@@ -211,6 +236,21 @@ def lib():
         "oracle_table_prove_zk": (ctypes.c_int, [ctypes.POINTER(TableParams), u32p, sz, sz, vp, vp,
                                                  ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
         "oracle_table_verify": (ctypes.c_int, [ctypes.POINTER(TableParams), u32p, sz, vp, u8p, sz]),
+        "oracle_fold_params_new": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                  ctypes.POINTER(FoldParams)]),
+        "oracle_fold_params_for_security": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                                           ctypes.c_uint32, ctypes.c_uint32,
+                                                           ctypes.POINTER(FoldParams)]),
+        "oracle_fold_acc_new": (vp, [ctypes.POINTER(FoldParams)]),
+        "oracle_fold_acc_free": (None, [vp]),
+        "oracle_fold_acc_len": (sz, [vp]),
+        "oracle_fold_acc_set_params": (None, [vp, ctypes.POINTER(FoldParams)]),
+        "oracle_fold_acc_add_segment": (ctypes.c_int, [vp, u32p, sz, u32p, sz, vp]),
+        "oracle_fold_acc_finalize": (ctypes.c_int, [vp, vp, ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+        "oracle_fold_acc_finalize_forged_out_a": (ctypes.c_int, [vp, sz, ctypes.c_uint32, u8p, sz,
+                                                                 ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+        "oracle_fold_verify": (ctypes.c_int, [ctypes.POINTER(FoldParams), vp, u8p, sz]),
         "oracle_stage_timers_clear": (None, []),
         "oracle_stage_timers": (sz, [ctypes.c_char_p, sz]),
     }
@@ -744,3 +784,70 @@ def table_prove_zk(params: TableParams, table, n_rand: int, rng: ChaCha20, trans
 def table_verify(params: TableParams, table, transcript: Transcript, proof: bytes) -> int:
     return lib().oracle_table_verify(ctypes.byref(params), u32_array(table), len(table), transcript.h, _buf(proof),
                                      len(proof))
+
+
+# fold.rs: the IVC accumulator.
+def fold_params_new(seg, blowup, nq, final_size, in_a, in_b, out_b, grinding_bits=0, cap_height=0) -> FoldParams:
+    p = FoldParams()
+    rc = lib().oracle_fold_params_new(seg, blowup, nq, final_size, in_a, in_b, out_b, ctypes.byref(p))
+    if rc:
+        raise OracleError(rc)
+    p.grinding_bits, p.cap_height = grinding_bits, cap_height
+    return p
+
+
+def fold_params_for_security(seg, in_a, in_b, out_b, bits=100) -> FoldParams:
+    p = FoldParams()
+    rc = lib().oracle_fold_params_for_security(seg, in_a, in_b, out_b, bits, ctypes.byref(p))
+    if rc:
+        raise OracleError(rc)
+    return p
+
+
+class FoldAccumulator:
+    """fold::Accumulator (fold.rs:361-570)."""
+
+    def __init__(self, params: FoldParams):
+        self.h = lib().oracle_fold_acc_new(ctypes.byref(params))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_fold_acc_free(self.h)
+            self.h = None
+
+    def __len__(self):
+        return lib().oracle_fold_acc_len(self.h)
+
+    def set_params(self, params: FoldParams):
+        lib().oracle_fold_acc_set_params(self.h, ctypes.byref(params))
+
+    def add_segment(self, col_a, col_b, transcript: Transcript):
+        rc = lib().oracle_fold_acc_add_segment(self.h, u32_array(col_a), len(col_a), u32_array(col_b), len(col_b),
+                                               transcript.h)
+        if rc:
+            raise OracleError(rc)
+
+    def finalize(self, transcript: Transcript) -> bytes:
+        return _range_out(lambda o, n: lib().oracle_fold_acc_finalize(self.h, transcript.h, o, n))
+
+    def finalize_forged_out_a(self, step, out_a, label: bytes) -> bytes:
+        return _range_out(lambda o, n: lib().oracle_fold_acc_finalize_forged_out_a(self.h, step, out_a, _buf(label),
+                                                                                  len(label), o, n))
+
+
+def fold_verify(params: FoldParams, transcript: Transcript, proof: bytes) -> int:
+    return lib().oracle_fold_verify(ctypes.byref(params), transcript.h, _buf(proof), len(proof))
+
+
+def chained_traces(seg, count, a=1, b=1):
+    """fold.rs:895-905: `count` segments of `seg` rows, each starting at the
+    previous segment's last row."""
+    out = []
+    for _ in range(count):
+        ca = (ctypes.c_uint32 * seg)()
+        cb = (ctypes.c_uint32 * seg)()
+        lib().oracle_build_trace(a, b, seg, ca, cb)
+        ca, cb = list(ca), list(cb)
+        a, b = ca[-1], cb[-1]
+        out.append((ca, cb))
+    return out
