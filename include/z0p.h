@@ -255,6 +255,48 @@ int z0p_table_prove_zk(z0p_ctx* ctx, const z0p_table_params* params, const uint3
 int z0p_table_verify(const z0p_table_params* params, const uint32_t* table, size_t table_len,
                      z0p_transcript* transcript, const uint8_t* proof, size_t proof_len);
 
+/* ---- IVC accumulation (src/fold.rs): hash-chained Fibonacci segments, each
+ * committed with its own row tree and six alphas, and one batched FRI over
+ * the gamma-combination of their compositions ---- */
+/* fold::Params (fold.rs:41-65); in_a, in_b, out_b are reduced mod p. */
+typedef struct z0p_fold_params {
+    uint64_t seg_steps;
+    uint64_t blowup;
+    uint64_t num_queries;
+    uint64_t final_size;
+    uint32_t grinding_bits;
+    uint32_t reserved0;
+    uint64_t cap_height;
+    uint32_t in_a;
+    uint32_t in_b;
+    uint32_t out_b;
+    uint32_t reserved1;
+} z0p_fold_params;
+/* fold::Params::new (fold.rs:72-111): grinding 0, cap height 0. */
+int z0p_fold_params_new(uint64_t seg_steps, uint64_t blowup, uint64_t num_queries, uint64_t final_size,
+                        uint32_t in_a, uint32_t in_b, uint32_t out_b, z0p_fold_params* out);
+/* fold::Params::for_security (fold.rs:136-170). */
+int z0p_fold_params_for_security(uint64_t seg_steps, uint32_t in_a, uint32_t in_b, uint32_t out_b,
+                                 uint32_t target_bits, z0p_fold_params* out);
+/* fold::Accumulator (fold.rs:361-400): segments stay in device memory on
+ * ctx until finalize. The ctx must not run other work between the calls
+ * that use one accumulator and its transcript. */
+typedef struct z0p_fold_acc z0p_fold_acc;
+int z0p_fold_acc_new(z0p_ctx* ctx, const z0p_fold_params* params, z0p_fold_acc** out);
+void z0p_fold_acc_free(z0p_fold_acc* acc);
+/* Accumulator::len (fold.rs:388-391). */
+size_t z0p_fold_acc_len(const z0p_fold_acc* acc);
+/* Accumulator::add_segment (fold.rs:407-492): columns of seg_steps canonical
+ * values each (else Z0P_ERR_MALFORMED_INPUT). */
+int z0p_fold_acc_add_segment(z0p_fold_acc* acc, const uint32_t* col_a, size_t len_a, const uint32_t* col_b,
+                             size_t len_b, z0p_transcript* transcript);
+/* Accumulator::finalize (fold.rs:494-570) + IvcProof::to_bytes (fold.rs:281-310).
+ * Consumes the segments: the accumulator is empty afterwards. */
+int z0p_fold_acc_finalize(z0p_fold_acc* acc, z0p_transcript* transcript, uint8_t** proof, size_t* proof_len);
+/* IvcProof::from_bytes + fold::verify (fold.rs:312-359, 572-667). Host only. */
+int z0p_fold_verify(const z0p_fold_params* params, z0p_transcript* transcript, const uint8_t* proof,
+                    size_t proof_len);
+
 /* ---- batched proving (no reference counterpart: the reference proves one
  * instance per air::prove call; this is `count` such calls at once) ---- */
 /* Device-memory budget of the batch and server schedulers on this ctx, in

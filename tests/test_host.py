@@ -426,3 +426,72 @@ def test_table_verifier_matches_oracle():
     assert agree(kp, kz, b"zk", zk) == 0
     down = o.table_prove_coeffs(op, T, [30] + [0] * 7, 21, o.Transcript(b"down"))
     assert agree(op, zp, b"down", down) != 0
+
+
+@pytest.mark.parametrize("args", [(8, 1, 1, 5, 12), (1 << 10, 1, 1, 7, 100), (1 << 16, 3, 4, 9, 100), (2, 1, 1, 1, 100),
+                                  (3, 1, 1, 1, 12), (1 << 24, 1, 1, 1, 100)])
+def test_fold_params_match_oracle(args):
+    try:
+        want = o.fold_params_for_security(*args)
+    except o.OracleError as e:
+        with pytest.raises(z.Z0pError) as ze:
+            z.fold.for_security(*args)
+        assert ze.value.code == e.code
+        return
+    assert bytes(z.fold.for_security(*args)) == bytes(want)
+    for a2 in [(8, 8, 8, 4, 1, 1, 5), (3, 8, 8, 4, 1, 1, 1), (8, 2, 8, 4, 1, 1, 1), (8, 8, 0, 4, 1, 1, 1),
+               (8, 8, 8, 1, 1, 1, 1), (1 << 40, 4, 8, 4, 1, 1, 1), (8, 4, 8, 4, P + 1, 2, P)]:
+        try:
+            w2 = o.fold_params_new(*a2)
+        except o.OracleError as e:
+            with pytest.raises(z.Z0pError) as ze:
+                z.fold.params_new(*a2)
+            assert ze.value.code == e.code, a2
+            continue
+        assert bytes(z.fold.params_new(*a2)) == bytes(w2), a2
+
+
+def test_fold_verifier_matches_oracle():
+    def prove(params, traces, label):
+        t = o.Transcript(label)
+        acc = o.FoldAccumulator(params)
+        for ca, cb in traces:
+            acc.add_segment(ca, cb, t)
+        return acc, acc.finalize(t)
+
+    def agree(op, zp, label, proof):
+        ok_o = o.fold_verify(op, o.Transcript(label), proof)
+        try:
+            z.fold.verify(zp, proof, z.Transcript(label))
+            ok_z = 0
+        except z.Z0pError as e:
+            ok_z = e.code
+        assert ok_z == ok_o, (label, ok_o, ok_z)
+        return ok_o
+
+    traces = o.chained_traces(8, 4)
+    out_b = traces[-1][1][-1]
+    op, zp = o.fold_params_new(8, 8, 8, 4, 1, 1, out_b), z.fold.params_new(8, 8, 8, 4, 1, 1, out_b)
+    acc, pr = prove(op, traces, b"bench/fold")
+    assert len(pr) == 16845 and agree(op, zp, b"bench/fold", pr) == 0
+    assert agree(op, zp, b"bench/fold", pr[:-1]) == 8
+    assert agree(op, zp, b"fold-B", pr) != 0
+    assert agree(op.replace(out_b=out_b + 1), zp.replace(out_b=out_b + 1), b"bench/fold", pr) != 0
+    for off in range(5, len(pr), 173):
+        bad = bytearray(pr)
+        bad[off] ^= 0x20
+        agree(op, zp, b"bench/fold", bytes(bad))
+    first = o.chained_traces(8, 1)[0]
+    second = o.chained_traces(8, 1, 99, first[1][-1])[0]
+    fp, fz = o.fold_params_new(8, 8, 8, 4, 1, 1, second[1][-1]), z.fold.params_new(8, 8, 8, 4, 1, 1, second[1][-1])
+    facc = o.FoldAccumulator(fp)
+    t = o.Transcript(b"forge")
+    facc.add_segment(*first, t)
+    facc.add_segment(*second, t)
+    assert agree(fp, fz, b"forge", facc.finalize_forged_out_a(0, 99, b"forge")) != 0
+    cp, cz = op.replace(cap_height=2, grinding_bits=3), zp.replace(cap_height=2, grinding_bits=3)
+    _, cpr = prove(cp, traces[:3], b"cap")
+    cp3, cz3 = cp.replace(out_b=traces[2][1][-1]), cz.replace(out_b=traces[2][1][-1])
+    agree(cp3, cz3, b"cap", cpr)
+    _, cpr = prove(cp3, traces[:3], b"cap")
+    assert agree(cp3, cz3, b"cap", cpr) == 0

@@ -1,10 +1,12 @@
 """z0p-mi355x: MI355X (gfx950) drop-in for the z0p Fibonacci AIR prover, the
-range-check and table-membership AIRs and the FRI polynomial commitment.
+range-check and table-membership AIRs, the IVC accumulator and the FRI
+polynomial commitment.
 
 Mirrors the reference crate's public API (src/air.rs, src/range.rs,
-src/table.rs, src/fri.rs, src/transcript.rs): ``Transcript``, ``air.Params``/``air.prove``/
+src/table.rs, src/fold.rs, src/fri.rs, src/transcript.rs): ``Transcript``, ``air.Params``/``air.prove``/
 ``air.verify``, ``range.Params``/``range.prove``/``range.verify``,
-``table.Params``/``table.prove``/``table.verify`` and
+``table.Params``/``table.prove``/``table.verify``,
+``fold.Accumulator``/``fold.verify`` and
 ``fri.Params``/``fri.commit_open``/``fri.verify``. Proofs
 are the reference's canonical VERSION = 6 bytes. Proving runs on the GPU
 through libz0p_amd.so; verification is host code, as in the reference.
@@ -13,7 +15,7 @@ import ctypes
 
 from ._ffi import Z0pError, buf, check, lib, take
 
-__all__ = ["Z0pError", "Transcript", "ChaCha20", "Context", "Server", "air", "fri", "range", "table", "default_context"]
+__all__ = ["Z0pError", "Transcript", "ChaCha20", "Context", "Server", "air", "fri", "range", "table", "fold", "default_context"]
 
 
 class Transcript:
@@ -225,4 +227,4 @@ def default_context() -> Context:
     return _default
 
 
-from . import air, fri, range, table  # noqa: E402
+from . import air, fold, fri, range, table  # noqa: E402

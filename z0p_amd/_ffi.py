@@ -64,6 +64,14 @@ class TableParamsC(ctypes.Structure):
     ]
 
 
+class FoldParamsC(ctypes.Structure):
+    _fields_ = [
+        ("seg_steps", u64), ("blowup", u64), ("num_queries", u64), ("final_size", u64),
+        ("grinding_bits", u32), ("reserved0", u32), ("cap_height", u64),
+        ("in_a", u32), ("in_b", u32), ("out_b", u32), ("reserved1", u32),
+    ]
+
+
 class FriParamsC(ctypes.Structure):
     _fields_ = [
         ("blowup", u64), ("num_queries", u64), ("final_size", u64),
@@ -137,6 +145,14 @@ SIGNATURES = {
     "z0p_table_prove_zk": (ctypes.c_int, [vp, ctypes.POINTER(TableParamsC), u32p, sz, u64, vp, vp,
                                           ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
     "z0p_table_verify": (ctypes.c_int, [ctypes.POINTER(TableParamsC), u32p, sz, vp, u8p, sz]),
+    "z0p_fold_params_new": (ctypes.c_int, [u64, u64, u64, u64, u32, u32, u32, ctypes.POINTER(FoldParamsC)]),
+    "z0p_fold_params_for_security": (ctypes.c_int, [u64, u32, u32, u32, u32, ctypes.POINTER(FoldParamsC)]),
+    "z0p_fold_acc_new": (ctypes.c_int, [vp, ctypes.POINTER(FoldParamsC), ctypes.POINTER(vp)]),
+    "z0p_fold_acc_free": (None, [vp]),
+    "z0p_fold_acc_len": (sz, [vp]),
+    "z0p_fold_acc_add_segment": (ctypes.c_int, [vp, u32p, sz, u32p, sz, vp]),
+    "z0p_fold_acc_finalize": (ctypes.c_int, [vp, vp, ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+    "z0p_fold_verify": (ctypes.c_int, [ctypes.POINTER(FoldParamsC), vp, u8p, sz]),
     "z0p_fri_commit_open": (ctypes.c_int, [vp, u32p, sz, ctypes.POINTER(FriParamsC), vp, ctypes.POINTER(u8p),
                                            ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
     "z0p_fri_verify": (ctypes.c_int, [u8p, sz, u8p, sz, ctypes.POINTER(FriParamsC), vp]),

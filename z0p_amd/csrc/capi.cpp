@@ -461,6 +461,94 @@ int z0p_table_verify(const z0p_table_params* params, const uint32_t* table, size
     return guarded([&] { z0p::table_verify(*params, table, table_len, transcript->t, proof, proof_len); });
 }
 
+int z0p_fold_params_new(uint64_t seg_steps, uint64_t blowup, uint64_t num_queries, uint64_t final_size,
+                        uint32_t in_a, uint32_t in_b, uint32_t out_b, z0p_fold_params* out) {
+    if (!out) return Z0P_ERR_INVALID_ARGUMENT;
+    using namespace z0p;
+    if (seg_steps < 2 || !pow2(seg_steps)) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    if (blowup < 4 || !pow2(blowup)) return Z0P_ERR_INVALID_BLOWUP;
+    if (num_queries == 0) return Z0P_ERR_NO_QUERIES;
+    if (final_size < 2 || !pow2(final_size)) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    if (seg_steps > MAX_DOMAIN || blowup > MAX_DOMAIN || seg_steps * blowup > MAX_DOMAIN)
+        return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    std::memset(out, 0, sizeof(*out));
+    out->seg_steps = seg_steps;
+    out->blowup = blowup;
+    out->num_queries = num_queries;
+    out->final_size = final_size;
+    out->in_a = in_a % P;
+    out->in_b = in_b % P;
+    out->out_b = out_b % P;
+    return Z0P_OK;
+}
+
+int z0p_fold_params_for_security(uint64_t seg_steps, uint32_t in_a, uint32_t in_b, uint32_t out_b,
+                                 uint32_t target_bits, z0p_fold_params* out) {
+    if (!out) return Z0P_ERR_INVALID_ARGUMENT;
+    using namespace z0p;
+    if (seg_steps < 2 || !pow2(seg_steps)) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    const uint64_t blowup = 16;
+    if (seg_steps > MAX_DOMAIN / blowup) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    const uint64_t lde = seg_steps * blowup;
+    auto strength = tune_strength(target_bits, composition_millibits(lde, seg_steps - 1));
+    auto shape = tune_shape(lde, strength.first, true, lde, 24);  // 8 index bytes + 4 fields
+    std::memset(out, 0, sizeof(*out));
+    out->seg_steps = seg_steps;
+    out->blowup = blowup;
+    out->num_queries = strength.first;
+    out->final_size = shape.first;
+    out->grinding_bits = strength.second;
+    out->cap_height = shape.second;
+    out->in_a = in_a % P;
+    out->in_b = in_b % P;
+    out->out_b = out_b % P;
+    return Z0P_OK;
+}
+
+int z0p_fold_acc_new(z0p_ctx* ctx, const z0p_fold_params* params, z0p_fold_acc** out) {
+    if (!ctx || !params || !out) return Z0P_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto* h = new (std::nothrow) z0p_fold_acc;
+    if (!h) return Z0P_ERR_OUT_OF_MEMORY;
+    int g = guarded([&] { h->a = z0p::fold_acc_new(*ctx->c, *params); });
+    if (g != Z0P_OK) {
+        delete h;
+        return g;
+    }
+    *out = h;
+    return Z0P_OK;
+}
+
+void z0p_fold_acc_free(z0p_fold_acc* acc) {
+    if (!acc) return;
+    z0p::fold_acc_free(acc->a);
+    delete acc;
+}
+
+size_t z0p_fold_acc_len(const z0p_fold_acc* acc) { return acc ? z0p::fold_acc_len(acc->a) : 0; }
+
+int z0p_fold_acc_add_segment(z0p_fold_acc* acc, const uint32_t* col_a, size_t len_a, const uint32_t* col_b,
+                             size_t len_b, z0p_transcript* transcript) {
+    if (!acc || !transcript || (!col_a && len_a) || (!col_b && len_b)) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] { z0p::fold_acc_add_segment(acc->a, col_a, len_a, col_b, len_b, transcript->t); });
+}
+
+int z0p_fold_acc_finalize(z0p_fold_acc* acc, z0p_transcript* transcript, uint8_t** proof, size_t* proof_len) {
+    if (!acc || !transcript || !proof || !proof_len) return Z0P_ERR_INVALID_ARGUMENT;
+    int rc = Z0P_OK;
+    int g = guarded([&] {
+        z0p::Bytes bytes = z0p::fold_acc_finalize(acc->a, transcript->t);
+        rc = emit(bytes, proof, proof_len);
+    });
+    return g != Z0P_OK ? g : rc;
+}
+
+int z0p_fold_verify(const z0p_fold_params* params, z0p_transcript* transcript, const uint8_t* proof,
+                    size_t proof_len) {
+    if (!params || !transcript || (!proof && proof_len)) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] { z0p::fold_verify(*params, transcript->t, proof, proof_len); });
+}
+
 z0p_chacha20* z0p_chacha20_new(const uint8_t key[32], const uint8_t nonce[12]) {
     if (!key || !nonce) return nullptr;
     auto* h = new (std::nothrow) z0p_chacha20;

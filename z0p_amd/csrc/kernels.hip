@@ -764,6 +764,64 @@ __global__ __launch_bounds__(WG) void k_table_compose(TableComposeJob inl, const
     }
 }
 
+// IVC batched composition (fold.rs:785-854 per step, 494-521 combined):
+// the points' inverses are shared by every step; each step adds its six
+// terms, scaled by beta = gamma * alpha, to the Ext4 accumulator.
+//   t1 = (A' - B)/Z, t2 = (B' - A - B)/Z with 1/Z = (x - w)/(x^n - 1),
+//   (A - in_a)/(x - 1), (B - in_b)/(x - 1), (A - out_a)/(x - w), (B - out_b)/(x - w).
+__global__ __launch_bounds__(WG) void k_fold_combine(FoldCombineJob inl, const FoldCombineJob* tab,
+                                                     uint32_t njobs) {
+    uint32_t local;
+    const FoldCombineJob J = job_of(inl, tab, njobs, local);
+    const uint32_t m = 1u << (J.logn + J.logb);
+    const uint32_t base = local * (WG * CELT) + threadIdx.x;
+    const uint32_t wl1 = sub(J.w_last_m, R1);
+    uint32_t d1[CELT], pr[CELT];
+#pragma unroll
+    for (int e = 0; e < CELT; ++e) {
+        const uint32_t p = base + e * WG;
+        if (p < m) {
+            const uint32_t j = brev_bits(p >> J.logn, J.logb);
+            const uint32_t x = mont_mul(gc(J.xn)[p & ((1u << J.logn) - 1)], gc(J.shift_m)[j]);
+            d1[e] = sub(x, R1);
+            pr[e] = mont_mul(d1[e], sub(d1[e], wl1));
+        } else {
+            d1[e] = pr[e] = R1;
+        }
+    }
+    batch_inv_m<CELT>(pr);
+#pragma unroll 1
+    for (int e = 0; e < CELT; ++e) {
+        const uint32_t p = base + e * WG;
+        if (p >= m) break;
+        const PosInfo pi = pos_info(p, J.logn, J.logb);
+        const uint32_t dw = sub(d1[e], wl1);            // x - w_last
+        const uint32_t i1 = mont_mul(dw, pr[e]);        // 1/(x - 1)
+        const uint32_t iw = mont_mul(d1[e], pr[e]);     // 1/(x - w_last)
+        const uint32_t iz = mont_mul(dw, gc(J.zinv_m)[pi.j]);  // 1/Z
+        uint32_t acc[4] = {0, 0, 0, 0};
+        for (uint32_t k = 0; k < J.nsteps; ++k) {
+            const FoldStepDesc& S = J.steps[k];
+            const uint32_t a = gc(S.a)[p], b = gc(S.b)[p];
+            const uint32_t an = gc(S.a)[pi.pnext], bn = gc(S.b)[pi.pnext];
+            const uint32_t t1 = mont_mul(sub(an, b), iz);
+            const uint32_t t2 = mont_mul(sub(bn, add(a, b)), iz);
+            const uint32_t ia = mont_mul(sub(a, S.in_a), i1);
+            const uint32_t ib = mont_mul(sub(b, S.in_b), i1);
+            const uint32_t oa = mont_mul(sub(a, S.out_a), iw);
+            const uint32_t ob = mont_mul(sub(b, S.out_b), iw);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t s01 = mont_mul2(t1, S.beta_m[0][q], t2, S.beta_m[1][q]);
+                const uint32_t s23 = mont_mul2(ia, S.beta_m[2][q], ib, S.beta_m[3][q]);
+                const uint32_t s45 = mont_mul2(oa, S.beta_m[4][q], ob, S.beta_m[5][q]);
+                acc[q] = add(acc[q], add(add(s01, s23), s45));
+            }
+        }
+        st4(J.layer0, p, u32x4{acc[0], acc[1], acc[2], acc[3]});
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Merkle leaves: thread g hashes leaves 2g and 2g+1 and, when asked, their
 // parent (one node of level 1). No LDS: wide levels run at full occupancy.
@@ -1705,6 +1763,13 @@ void launch_table_compose(const Launch& L, std::vector<TableComposeJob> jobs) {
     run(L, "table_compose", k_table_compose, jobs, g, 0);
 }
 
+void launch_fold_combine(const Launch& L, std::vector<FoldCombineJob> jobs) {
+    unsigned g = assign_blocks(jobs, [](const FoldCombineJob& j) {
+        return grid_for(uint64_t(1) << (j.logn + j.logb), WG * CELT);
+    });
+    run(L, "fold_combine", k_fold_combine, jobs, g, 0);
+}
+
 void launch_leaves(const Launch& L, std::vector<LeavesJob> jobs) {
     bool trace = false, rows = false;
     for (const LeavesJob& j : jobs) {
@@ -1904,6 +1969,51 @@ void launch_open(const Launch& L, std::vector<OpenJob> jobs) {
     if (L.dry) return;
     ProfScope ps(L.prof, "open", L.stream);
     hipLaunchKernelGGL(k_open, dim3(g), dim3(WG), 0, L.stream, table, uint32_t(jobs.size()));
+}
+
+}  // namespace z0p
+
+namespace z0p {
+namespace {
+
+// One workgroup per (step, query) pair, grid-strided; one thread per word.
+__global__ __launch_bounds__(WG) void k_fold_open(FoldOpenJob J) {
+    const int logm = J.logn + J.logb;
+    const uint64_t m = uint64_t(1) << logm, B = uint64_t(1) << J.logb;
+    const uint32_t pw = uint32_t(J.target) * 8u;
+    const uint64_t pairs = uint64_t(J.nsteps) * J.nq;
+    for (uint64_t g = blockIdx.x; g < pairs; g += gridDim.x) {
+        const uint32_t k = uint32_t(g / J.nq), q = uint32_t(g % J.nq);
+        const FoldStepDesc& S = J.steps[k];
+        const uint64_t leaf = J.queries[q];
+        const uint64_t idx = brev_bits(uint32_t(leaf), logm);
+        const uint64_t sh = (idx + B) & (m - 1);
+        uint32_t* o = J.out + g * J.words_per_query;
+        for (uint32_t w = threadIdx.x; w < J.words_per_query; w += WG) {
+            uint32_t v;
+            if (w < 4) {
+                const uint64_t pos = w < 2 ? leaf : brev_bits(uint32_t(sh), logm);
+                v = gc((w & 1) ? S.b : S.a)[pos];
+            } else {
+                uint32_t r = w - 4;
+                const uint64_t row = r >= pw ? sh : idx;
+                if (r >= pw) r -= pw;
+                const int l = int(r >> 3);
+                v = gc(S.tree)[level_offset_words(m, l) + ((row >> l) ^ 1) * 8 + (r & 7)];
+            }
+            o[w] = v;
+        }
+    }
+}
+
+}  // namespace
+
+void launch_fold_open(const Launch& L, const FoldOpenJob& job) {
+    if (L.dry) return;
+    const uint64_t pairs = uint64_t(job.nsteps) * job.nq;
+    if (pairs == 0) return;
+    ProfScope ps(L.prof, "fold_open", L.stream);
+    hipLaunchKernelGGL(k_fold_open, dim3(unsigned(std::min<uint64_t>(pairs, 65536))), dim3(WG), 0, L.stream, job);
 }
 
 }  // namespace z0p

@@ -172,6 +172,45 @@ struct TableComposeJob {
 };
 void launch_table_compose(const Launch& L, std::vector<TableComposeJob> jobs);
 
+// IVC accumulation (fold.rs:494-570): one committed segment of the
+// Fibonacci relation. a, b are its bit-reversed LDE columns (m words each),
+// tree its row tree (one leaf per LDE row); beta_m[t] = gamma * alpha_t
+// (Montgomery), so sum_k gamma_k C_k = sum_k sum_t beta_{k,t} s_{k,t}.
+struct FoldStepDesc {
+    const uint32_t* a;
+    const uint32_t* b;
+    const uint32_t* tree;
+    uint32_t in_a, in_b, out_a, out_b;
+    uint32_t beta_m[6][4];
+};
+// Layer 0 of the batched FRI: the gamma-combination of every step's
+// composition (fold.rs:785-854), written in bit-reversed order.
+struct FoldCombineJob {
+    const FoldStepDesc* steps;
+    uint32_t nsteps;
+    uint32_t* layer0;
+    const uint32_t* xn;      // [n] omega_n^brev(q), Montgomery
+    const uint32_t* shift_m;
+    const uint32_t* zinv_m;  // [B] 1/(s_j^n - 1)
+    uint32_t w_last_m;
+    int logn, logb;
+    uint32_t block_begin;
+};
+void launch_fold_combine(const Launch& L, std::vector<FoldCombineJob> jobs);
+// Step openings (fold.rs:540-566): for step k and query q, written to
+// out + (k nq + q) words_per_query: A, B at the natural index brev(query)
+// and at the next row (index + m/n), then both row paths (target digests
+// each, leaf level first).
+struct FoldOpenJob {
+    const FoldStepDesc* steps;
+    uint32_t nsteps, nq;
+    const uint64_t* queries;  // device memory
+    int logn, logb, target;
+    uint32_t words_per_query;
+    uint32_t* out;
+};
+void launch_fold_open(const Launch& L, const FoldOpenJob& job);
+
 // Merkle leaves. Kinds: trace rows (8 joint A||B rows per leaf from the
 // bit-reversed columns), FRI Ext4 quads, FRI base-field quads (upper
 // coordinates zero), raw 64-byte leaves.

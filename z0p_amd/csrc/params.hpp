@@ -175,6 +175,31 @@ inline z0p_fri_params table_fri_params(const z0p_table_params& a) {
     return f;
 }
 
+// fold::Params::validate (fold.rs:186-218).
+inline void validate_fold(const z0p_fold_params& p) {
+    require(p.seg_steps >= 2 && pow2(p.seg_steps), Z0P_ERR_INVALID_DOMAIN_SIZE);
+    require(p.blowup >= 4 && pow2(p.blowup), Z0P_ERR_INVALID_BLOWUP);
+    require(p.num_queries != 0, Z0P_ERR_NO_QUERIES);
+    require(p.final_size >= 2 && pow2(p.final_size), Z0P_ERR_INVALID_DOMAIN_SIZE);
+    require(p.cap_height <= 8, Z0P_ERR_MALFORMED_INPUT);
+    require(p.seg_steps <= MAX_DOMAIN && p.blowup <= MAX_DOMAIN && p.seg_steps * p.blowup <= MAX_DOMAIN,
+            Z0P_ERR_INVALID_DOMAIN_SIZE);
+}
+
+// fold::Params::fri_params (fold.rs:177-183).
+inline z0p_fri_params fold_fri_params(const z0p_fold_params& a) {
+    require(a.blowup >= 2 && pow2(a.blowup), Z0P_ERR_INVALID_BLOWUP);
+    require(a.num_queries != 0, Z0P_ERR_NO_QUERIES);
+    require(a.final_size >= 2 && pow2(a.final_size) && a.final_size <= MAX_FINAL_SIZE, Z0P_ERR_INVALID_DOMAIN_SIZE);
+    z0p_fri_params f{};
+    f.blowup = a.blowup;
+    f.num_queries = a.num_queries;
+    f.final_size = a.final_size;
+    f.grinding_bits = a.grinding_bits;
+    f.cap_height = a.cap_height;
+    return f;
+}
+
 // FRI layer-shape checks shared by commit (fri.rs:556-569, 1125-1140).
 inline size_t fri_num_folds(uint64_t lde_len, uint64_t final_size, uint64_t degree_bound) {
     require(final_size < lde_len && lde_len % final_size == 0, Z0P_ERR_INVALID_DOMAIN_SIZE);

@@ -306,6 +306,7 @@ struct Buckets {
     std::vector<ComposeJob> compose;
     std::vector<RangeComposeJob> rcompose;
     std::vector<TableComposeJob> tcompose;
+    std::vector<FoldCombineJob> fcombine;
     std::vector<LeavesJob> leaves;
     std::vector<LevelsJob> levels;
     std::vector<TailJob> tail;
@@ -322,6 +323,7 @@ struct Buckets {
     std::vector<ComposeJob>& of(const ComposeJob&) { return compose; }
     std::vector<RangeComposeJob>& of(const RangeComposeJob&) { return rcompose; }
     std::vector<TableComposeJob>& of(const TableComposeJob&) { return tcompose; }
+    std::vector<FoldCombineJob>& of(const FoldCombineJob&) { return fcombine; }
     std::vector<LeavesJob>& of(const LeavesJob&) { return leaves; }
     std::vector<LevelsJob>& of(const LevelsJob&) { return levels; }
     std::vector<TailJob>& of(const TailJob&) { return tail; }
@@ -342,6 +344,7 @@ struct Buckets {
         if (!compose.empty()) launch_compose(L, compose);
         if (!rcompose.empty()) launch_range_compose(L, rcompose);
         if (!tcompose.empty()) launch_table_compose(L, tcompose);
+        if (!fcombine.empty()) launch_fold_combine(L, fcombine);
         if (!leaves.empty()) launch_leaves(L, leaves);
         if (!levels.empty()) launch_levels(L, levels);
         if (!tail.empty()) launch_tail(L, tail);
@@ -472,6 +475,7 @@ struct Proof {
     uint64_t src_words() const {
         if (air) return n;
         if (row) return host_a ? ncols * n : 0;
+        if (fri_in == FRI_FOLD_COMBINE) return 0;
         return (fri_in == FRI_COEFFS_EXT || fri_in == FRI_EVALS_EXT ? 4 : 1) * n;
     }
     // AIR with both columns interpolated and extended (air.rs:385-406,
@@ -490,6 +494,9 @@ struct Proof {
     TableConsts tc{};
     std::vector<uint32_t> table;  // the public table, canonical
     uint32_t* table_dev = nullptr;  // the table in Montgomery form
+    // FRI_FOLD_COMBINE: the committed IVC steps (device table)
+    const FoldStepDesc* fold_steps = nullptr;
+    uint32_t fold_nsteps = 0;
     uint64_t tag = 0;
     // geometry
     uint64_t n = 0, B = 0, m = 0, poly_len = 0, degree_bound = 0;
@@ -808,6 +815,33 @@ void init_fri(Ctx& c, Proof& P, const uint32_t* src, uint64_t count, int kind, u
     P.mem_bytes = layout(P, nullptr);
 }
 
+// fold::Accumulator::finalize (fold.rs:510-524): fri::commit_ext_evals of the
+// combined compositions over the steps' shared m-point domain (m / n cosets
+// of n = seg_steps points), declared degree seg_steps - 1.
+void init_fold_fri(Ctx& c, Proof& P, const FoldStepDesc* steps, uint32_t nsteps, uint64_t seg_steps, uint64_t blowup,
+                   const z0p_fri_params& fp, Transcript* tr) {
+    validate_fri(fp);
+    require(nsteps > 0 && steps, Z0P_ERR_EMPTY_INPUT);
+    require(pow2(seg_steps) && seg_steps >= 2 && pow2(blowup) && seg_steps * blowup <= MAX_DOMAIN,
+            Z0P_ERR_INVALID_DOMAIN_SIZE);
+    P.air = false;
+    P.tr = tr;
+    P.fri_in = FRI_FOLD_COMBINE;
+    P.fold_steps = steps;
+    P.fold_nsteps = nsteps;
+    P.n = seg_steps;
+    P.B = blowup;
+    P.m = seg_steps * blowup;
+    P.poly_len = P.m;
+    P.degree_bound = seg_steps - 1;
+    require(P.degree_bound < P.m, Z0P_ERR_MALFORMED_INPUT);
+    P.logn = int(ilog2(P.n));
+    P.logb = int(ilog2(P.B));
+    init_fri_shape(P, fp);
+    P.gt = &c.geometry(P.logn, P.logb);
+    P.mem_bytes = layout(P, nullptr);
+}
+
 // Host work when a proof is admitted (its memory assigned).
 void start(Ctx& c, Proof& P) {
     layout(P, P.mem);
@@ -883,7 +917,8 @@ void start(Ctx& c, Proof& P) {
         store_le64(dims + 8, P.poly_len);
         store_le64(dims + 16, P.degree_bound);
         tr.absorb("fri-dims", dims, 24);
-        hip_check(hipMemcpyAsync(P.col, P.coeff_src, P.src_words() * 4, hipMemcpyHostToDevice, c.stream));
+        if (P.src_words())
+            hip_check(hipMemcpyAsync(P.col, P.coeff_src, P.src_words() * 4, hipMemcpyHostToDevice, c.stream));
     }
     P.phase = PH_FRONT;
 }
@@ -1044,6 +1079,20 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                     case FRI_EVALS:
                         ch.push_back(BitrevJob{P.col, P.vals[0], P.logn, 0});
                         break;
+                    case FRI_FOLD_COMBINE: {
+                        FoldCombineJob fj{};
+                        fj.steps = P.fold_steps;
+                        fj.nsteps = P.fold_nsteps;
+                        fj.layer0 = P.vals[0];
+                        fj.xn = gt.xn;
+                        fj.shift_m = gt.coset.shift;
+                        fj.zinv_m = gt.zinv;
+                        fj.w_last_m = to_mont(pow_c(root_of_unity(uint32_t(P.logn)), P.n - 1));
+                        fj.logn = P.logn;
+                        fj.logb = P.logb;
+                        ch.push_back(fj);
+                        break;
+                    }
                     default: {  // FRI_EVALS_EXT
                         BitrevJob b{P.col, P.vals[0], P.logn, 0};
                         b.in_stride = 4;
@@ -1801,6 +1850,14 @@ Proof* proof_new_fri(Ctx& c, const uint32_t* src, uint64_t count, int kind, uint
     return P.release();
 }
 
+Proof* proof_new_fold_fri(Ctx& c, const FoldStepDesc* steps, uint32_t nsteps, uint64_t seg_steps, uint64_t blowup,
+                          const z0p_fri_params& fp, Transcript* tr) {
+    auto P = std::make_unique<Proof>();
+    init_fold_fri(c, *P, steps, nsteps, seg_steps, blowup, fp, tr);
+    return P.release();
+}
+
+const std::vector<uint64_t>& proof_queries(const Proof* p) { return p->queries; }
 void proof_free(Proof* p) { delete p; }
 int proof_status(const Proof* p) { return p->status; }
 Bytes& proof_bytes(Proof* p) { return p->proof; }
@@ -1814,14 +1871,18 @@ void proof_times(const Proof* p, double& submit, double& admit, double& done) {
 void proof_set_tag(Proof* p, uint64_t tag) { p->tag = tag; }
 uint64_t proof_tag(const Proof* p) { return p->tag; }
 
+void run_proof(Ctx& ctx, Proof* p) {
+    Engine e(ctx, 0, 1);
+    e.set_pipelined(true);
+    e.submit(p);
+    e.run();
+    if (p->status != Z0P_OK) throw Status(p->status);
+}
+
 namespace {
 Bytes run_one(Ctx& ctx, Proof* raw) {
     std::unique_ptr<Proof> P(raw);
-    Engine e(ctx, 0, 1);
-    e.set_pipelined(true);
-    e.submit(P.get());
-    e.run();
-    if (P->status != Z0P_OK) throw Status(P->status);
+    run_proof(ctx, P.get());
     return std::move(P->proof);
 }
 }  // namespace

@@ -38,7 +38,8 @@ TreeGeom tree_geom(uint64_t leaves, uint64_t cap_height);
 // One kernel step of one proof; a proof's steps of one iteration form a
 // chain that must run in order.
 using Step = std::variant<FibCoeffJob, BitrevJob, InterleaveJob, NttJob, LinkedJob, ComposeJob, RangeComposeJob,
-                          TableComposeJob, LeavesJob, LevelsJob, TailJob, FoldJob, SmallCommitJob, GrindJob, OpenJob>;
+                          TableComposeJob, FoldCombineJob, LeavesJob, LevelsJob, TailJob, FoldJob, SmallCommitJob,
+                          GrindJob, OpenJob>;
 
 // Leaves, then wide levels, then LDS tail launches up to the cap level.
 void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves, const Schedule& s);
@@ -77,9 +78,17 @@ enum FriInput : int {
     FRI_COEFFS_EXT,     // commit_ext: Ext4 coefficients (4 words each)
     FRI_EVALS,          // commit_evals: base-field LDE values, natural order
     FRI_EVALS_EXT,      // commit_ext_evals: Ext4 LDE values, natural order
+    FRI_FOLD_COMBINE,   // commit_ext_evals of the IVC steps' gamma-combination (fold.rs:494-524)
 };
 Proof* proof_new_fri(Ctx& c, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,
                      const z0p_fri_params& fp, Transcript* tr);
+// The batched FRI of fold::Accumulator::finalize: layer 0 is combined on the
+// device from `nsteps` committed segments (device table), seg_steps rows
+// each, extended by `blowup`; degree bound seg_steps - 1.
+Proof* proof_new_fold_fri(Ctx& c, const FoldStepDesc* steps, uint32_t nsteps, uint64_t seg_steps, uint64_t blowup,
+                          const z0p_fri_params& fp, Transcript* tr);
+// The FRI query indices of a finished proof (layer-0 positions).
+const std::vector<uint64_t>& proof_queries(const Proof* p);
 void proof_free(Proof* p);
 int proof_status(const Proof* p);
 Bytes& proof_bytes(Proof* p);
@@ -148,6 +157,8 @@ Bytes air_prove_zk(Ctx& ctx, const z0p_air_params& p, uint64_t n_rand, ChaCha20&
 void fri_commit_open(Ctx& ctx, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,
                      const z0p_fri_params& fp, Transcript& tr,
                      Bytes& commitment, Bytes& proof);
+// One proof alone through the scheduler (latency mode); throws its status.
+void run_proof(Ctx& ctx, Proof* p);
 Bytes range_prove(Ctx& ctx, const z0p_range_params& p, Transcript& tr);
 Bytes range_prove_with_columns(Ctx& ctx, const z0p_range_params& p, const uint32_t* cols, uint64_t ncols,
                                uint64_t len, Transcript& tr);
@@ -156,11 +167,21 @@ Bytes table_prove(Ctx& ctx, const z0p_table_params& p, const uint32_t* table, ui
 Bytes table_prove_zk(Ctx& ctx, const z0p_table_params& p, const uint32_t* table, uint64_t table_len, uint64_t n_rand,
                      ChaCha20& rng, Transcript& tr);
 
+// IVC accumulator (fold.rs:361-570; fold.cpp).
+struct FoldAcc;
+FoldAcc* fold_acc_new(Ctx& c, const z0p_fold_params& p);
+void fold_acc_free(FoldAcc* a);
+size_t fold_acc_len(const FoldAcc* a);
+void fold_acc_add_segment(FoldAcc* a, const uint32_t* col_a, uint64_t len_a, const uint32_t* col_b, uint64_t len_b,
+                          Transcript& tr);
+Bytes fold_acc_finalize(FoldAcc* a, Transcript& tr);
+
 // Host verifier (air.rs:607-752, fri.rs:686-859); throws Status on rejection.
 void air_verify(const z0p_air_params& p, Transcript& tr, const uint8_t* proof, size_t len);
 void fri_verify(const uint8_t* cm, size_t cm_len, const uint8_t* pf, size_t pf_len, const z0p_fri_params& fp,
                 Transcript& tr);
 void range_verify(const z0p_range_params& p, Transcript& tr, const uint8_t* proof, size_t len);
+void fold_verify(const z0p_fold_params& p, Transcript& tr, const uint8_t* proof, size_t len);
 void table_verify(const z0p_table_params& p, const uint32_t* table, uint64_t table_len, Transcript& tr,
                   const uint8_t* proof, size_t len);
 // table.rs:357-363: the saturating bound |T| (n + n_rand - 1) - n.
@@ -183,4 +204,7 @@ struct z0p_ctx {
 };
 struct z0p_chacha20 {
     z0p::ChaCha20* r = nullptr;
+};
+struct z0p_fold_acc {
+    z0p::FoldAcc* a = nullptr;
 };

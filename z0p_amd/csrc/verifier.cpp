@@ -1,6 +1,7 @@
-// Host verifier for the AIR, range-check, table-membership and FRI proofs
-// (the reference verifies on the CPU: air.rs:607-752, range.rs:521-721,
-// table.rs:430-538, fri.rs:686-859, 1153-1227, 1339-1364). It parses the
+// Host verifier for the AIR, range-check, table-membership, IVC and FRI
+// proofs (the reference verifies on the CPU: air.rs:607-752,
+// range.rs:521-721, table.rs:430-538, fold.rs:572-727, fri.rs:686-859,
+// 1153-1227, 1339-1364). It parses the
 // VERSION = 6 bytes fail-closed (codec.rs:118-274) and replays the
 // transcript exactly as the prover advanced it.
 #include <algorithm>
@@ -655,6 +656,125 @@ void table_verify(const z0p_table_params& p, const uint32_t* table_in, uint64_t 
         const E4 expected = e4_add(e4_mul_base(a_member, s_m), e4_mul_base(a_bound, s_b));
         require(!pf.queries[q].empty(), Z0P_ERR_FRI_VERIFICATION_FAILED);
         require(e4_eq(pf.queries[q][0].values[leaf % 4], expected), Z0P_ERR_FRI_VERIFICATION_FAILED);
+    }
+}
+
+void fold_verify(const z0p_fold_params& p, Transcript& tr, const uint8_t* bytes, size_t len) {
+    // IvcProof::from_bytes (fold.rs:312-359)
+    struct StepOpeningV {
+        uint64_t index = 0;
+        uint32_t cols[4] = {0, 0, 0, 0};
+        Path path, shift_path;
+    };
+    struct StepV {
+        uint32_t claims[4];  // in_a, in_b, out_a, out_b
+        std::vector<Digest> cap;
+        std::vector<StepOpeningV> openings;
+    };
+    constexpr uint32_t MAX_STEPS = 1u << 20;  // codec.rs:34
+    Reader r(bytes, len);
+    r.version();
+    std::vector<StepV> steps(r.count(MAX_STEPS));
+    for (auto& st : steps) {
+        for (auto& c : st.claims) c = r.field();
+        st.cap = read_cap(r);
+        st.openings.resize(r.count(MAX_QUERIES));
+        for (auto& o : st.openings) {
+            o.index = r.u64();
+            for (auto& v : o.cols) v = r.field();
+            o.path = read_path(r);
+            o.shift_path = read_path(r);
+        }
+    }
+    CommitmentV cm = read_commitment_body(r);
+    FriProofV pf = read_fri_proof_body(r);
+    r.end();
+
+    // fold::verify (fold.rs:572-667)
+    validate_fold(p);
+    require(!steps.empty(), Z0P_ERR_EMPTY_INPUT);
+    const uint64_t s = p.seg_steps, m = s * p.blowup;
+    require(cm.lde_len == m && cm.poly_len == m && cm.degree_bound == s - 1, Z0P_ERR_FRI_VERIFICATION_FAILED);
+    const uint64_t offset = m / s;
+    uint8_t pb[56];
+    store_le64(pb, p.seg_steps);
+    store_le64(pb + 8, p.blowup);
+    store_le64(pb + 16, p.num_queries);
+    store_le64(pb + 24, p.final_size);
+    store_le32(pb + 32, p.grinding_bits);
+    store_le64(pb + 36, p.cap_height);
+    const uint32_t in_a = p.in_a % P, in_b = p.in_b % P, out_b = p.out_b % P;
+    store_le32(pb + 44, in_a);
+    store_le32(pb + 48, in_b);
+    store_le32(pb + 52, out_b);
+    tr.absorb("fold/params", pb, 56);
+    // check_chain (fold.rs:710-727)
+    require(steps[0].claims[0] == in_a && steps[0].claims[1] == in_b, Z0P_ERR_FRI_VERIFICATION_FAILED);
+    for (size_t k = 0; k + 1 < steps.size(); ++k)
+        require(steps[k].claims[2] == steps[k + 1].claims[0] && steps[k].claims[3] == steps[k + 1].claims[1],
+                Z0P_ERR_FRI_VERIFICATION_FAILED);
+    require(steps.back().claims[3] == out_b, Z0P_ERR_FRI_VERIFICATION_FAILED);
+    std::vector<std::array<E4, 6>> alphas(steps.size());
+    for (size_t k = 0; k < steps.size(); ++k) {
+        uint8_t cb[16];
+        for (int i = 0; i < 4; ++i) store_le32(cb + 4 * i, steps[k].claims[i]);
+        tr.absorb("fold/step", cb, 16);
+        std::vector<uint8_t> capb;
+        for (auto& d : steps[k].cap) capb.insert(capb.end(), d.begin(), d.end());
+        tr.absorb("fold/trace", capb);
+        for (auto& a : alphas[k]) a = tr.challenge_ext("fold/alpha");
+        require(steps[k].openings.size() == p.num_queries, Z0P_ERR_FRI_VERIFICATION_FAILED);
+    }
+    std::vector<E4> gammas(steps.size());
+    for (auto& g : gammas) g = tr.challenge_ext("fold/gamma");
+    const z0p_fri_params fp = fold_fri_params(p);
+    std::vector<uint64_t> idx = fri_verify_impl(cm, pf, fp, tr);
+    const uint64_t log_m = ilog2(m);
+    const uint32_t omega_m = root_of_unity(uint32_t(log_m));
+    const uint32_t w_last = pow_c(root_of_unity(ilog2(s)), s - 1);
+    const uint64_t depth = log_m, drop = std::min<uint64_t>(p.cap_height, depth);
+    const size_t count = std::min(pf.queries.size(), idx.size());
+    for (size_t q = 0; q < count; ++q) {
+        const uint64_t leaf = idx[q];
+        const uint64_t index = brev64(leaf, log_m);
+        const uint32_t x = mul(GENERATOR, pow_c(omega_m, index));
+        E4 combined{{0, 0, 0, 0}};
+        for (size_t k = 0; k < steps.size(); ++k) {
+            const StepV& st = steps[k];
+            const StepOpeningV& o = st.openings[q];
+            require(o.index == index, Z0P_ERR_FRI_VERIFICATION_FAILED);
+            // check_step_openings (fold.rs:669-708)
+            require(st.cap.size() == (uint64_t(1) << drop) && o.path.size() == depth - drop &&
+                        o.shift_path.size() == depth - drop,
+                    Z0P_ERR_BAD_MERKLE_PATH);
+            std::vector<uint8_t> row(8);
+            store_le32(row.data(), o.cols[0]);
+            store_le32(row.data() + 4, o.cols[1]);
+            const uint64_t e = index >> (depth - drop);
+            require(e < st.cap.size() && verify_capped_row(st.cap[e], row, index, depth, o.path),
+                    Z0P_ERR_BAD_MERKLE_PATH);
+            const uint64_t shifted = (index + offset) % m;
+            store_le32(row.data(), o.cols[2]);
+            store_le32(row.data() + 4, o.cols[3]);
+            const uint64_t se = shifted >> (depth - drop);
+            require(se < st.cap.size() && verify_capped_row(st.cap[se], row, shifted, depth, o.shift_path),
+                    Z0P_ERR_BAD_MERKLE_PATH);
+            // composition_at (fold.rs:856-879)
+            const uint32_t a = o.cols[0], b = o.cols[1], an = o.cols[2], bn = o.cols[3];
+            const uint32_t zi = inv_c(mul(sub(pow_c(x, s), 1), inv_c(sub(x, w_last))));
+            const uint32_t i1 = inv_c(sub(x, 1)), iw = inv_c(sub(x, w_last));
+            const uint32_t terms[6] = {mul(sub(an, b), zi),
+                                       mul(sub(bn, add(a, b)), zi),
+                                       mul(sub(a, st.claims[0]), i1),
+                                       mul(sub(b, st.claims[1]), i1),
+                                       mul(sub(a, st.claims[2]), iw),
+                                       mul(sub(b, st.claims[3]), iw)};
+            E4 c{{0, 0, 0, 0}};
+            for (int t = 0; t < 6; ++t) c = e4_add(c, e4_mul_base(alphas[k][t], terms[t]));
+            combined = e4_add(combined, e4_mul(gammas[k], c));
+        }
+        require(!pf.queries[q].empty(), Z0P_ERR_FRI_VERIFICATION_FAILED);
+        require(e4_eq(pf.queries[q][0].values[leaf % 4], combined), Z0P_ERR_FRI_VERIFICATION_FAILED);
     }
 }
 
