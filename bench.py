@@ -277,6 +277,103 @@ def cpu_baseline(n, repeats):
     }
 
 
+def widened(z, ctx, reps, with_cpu):
+    """The other provers on the same device path, each at a production-size
+    workload: GPU time per proof (median of `reps` after one warm-up) and,
+    unless skipped, the oracle's time on the host for the same proof (one
+    run, all host threads) with a byte comparison of the two proofs."""
+    import numpy as np
+    o = None
+    if with_cpu:
+        import oracle_ffi as o
+        o.set_threads(os.cpu_count() or 1)
+    key, nonce = bytes(range(32)), bytes(range(12))
+    rng = np.random.default_rng(5)
+    cases = []
+
+    # ZK AIR: 2^20 steps, 640 maskers per column (16 x 40 queries), final 32.
+    n = 1 << 20
+    zp = z.air.for_security(n, 1, 1, z.air.b_last(1, 1, n))
+    zp.final_size = 32
+
+    def zk_cpu():
+        op = o.air_params_for_security(n, 1, 1, o.fib_b_last(1, 1, n))
+        op.final_size = 32
+        return o.air_prove_zk(op, 640, o.ChaCha20(key, nonce), o.Transcript(b"w/zk"))
+
+    cases.append(("air_zk", "AIR Fibonacci N=2^20 zero-knowledge, n_rand 640 (masked LDE 2^25)",
+                  lambda: z.air.prove_zk(zp, 640, z.ChaCha20(key, nonce), z.Transcript(b"w/zk"), ctx), zk_cpu))
+    # FRI over 2^20 Ext4 coefficients (2^24-point LDE).
+    ce = rng.integers(0, P, 4 << 20, dtype=np.uint32)
+    fe = z.fri.for_security(1 << 20)
+    cases.append(("fri_commit_ext", "fri::commit_ext + open, 2^20 Ext4 coefficients, B16",
+                  lambda: z.fri.commit_open_ext(ce, fe, z.Transcript(b"w/ext"), ctx),
+                  lambda: o.fri_commit_open_kind(o.FRI_KIND_EXT, ce, o.fri_params_for_security(1 << 20),
+                                                 o.Transcript(b"w/ext"))))
+    # Range check: 2^16 rows, 16 bits (17 columns, 2^20-point LDE).
+    rz = z.range.for_security(1 << 16, 16, 0xBEEF)
+    cases.append(("range", "range::prove, 2^16 rows x 16 bits, production shape",
+                  lambda: z.range.prove(rz, z.Transcript(b"w/range"), ctx),
+                  lambda: o.range_prove(o.range_params_for_security(1 << 16, 16, 0xBEEF), o.Transcript(b"w/range"))))
+    # Table membership: 2^16 rows, 16-entry table.
+    table = [int(v) for v in rng.integers(0, P, 16)]
+    tz = z.table.for_security(1 << 16, 16, table[3])
+    cases.append(("table", "table::prove, 2^16 rows, 16-entry table, production shape",
+                  lambda: z.table.prove(tz, table, z.Transcript(b"w/table"), ctx),
+                  lambda: o.table_prove(o.table_params_for_security(1 << 16, 16, table[3]), table,
+                                        o.Transcript(b"w/table"))))
+    # IVC: 4 chained segments of 2^16 rows.
+    seg, count = 1 << 16, 4
+    traces = []
+    a = b = 1
+    for _ in range(count):
+        ca = np.zeros(seg, dtype=np.uint32)
+        cb = np.zeros(seg, dtype=np.uint32)
+        ca[0], cb[0] = a, b
+        for i in range(seg - 1):
+            ca[i + 1] = cb[i]
+            cb[i + 1] = (int(ca[i]) + int(cb[i])) % P
+        a, b = int(ca[-1]), int(cb[-1])
+        traces.append((ca, cb))
+    fz = z.fold.for_security(seg, 1, 1, b)
+
+    def fold_gpu():
+        acc = z.fold.Accumulator(fz, ctx)
+        t = z.Transcript(b"w/fold")
+        for ca, cb in traces:
+            acc.add_segment(ca, cb, t)
+        return acc.finalize(t)
+
+    def fold_cpu():
+        acc = o.FoldAccumulator(o.fold_params_for_security(seg, 1, 1, b))
+        t = o.Transcript(b"w/fold")
+        for ca, cb in traces:
+            acc.add_segment(list(map(int, ca)), list(map(int, cb)), t)
+        return acc.finalize(t)
+
+    cases.append(("fold", "fold::Accumulator, 4 segments of 2^16 rows, production shape", fold_gpu, fold_cpu))
+
+    out = {}
+    for name, workload, gpu, cpu in cases:
+        got = gpu()  # warm-up
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            got = gpu()
+            times.append((time.perf_counter() - t0) * 1e3)
+        times.sort()
+        row = {"workload": workload, "gpu_ms": round(times[len(times) // 2], 3),
+               "bytes": len(got[1]) if isinstance(got, tuple) else len(got)}
+        if with_cpu:
+            t0 = time.perf_counter()
+            want = cpu()
+            cpu_ms = (time.perf_counter() - t0) * 1e3
+            row.update({"cpu_ms": round(cpu_ms, 1), "cpu_threads": os.cpu_count() or 1,
+                        "speedup": round(cpu_ms / row["gpu_ms"], 1), "identical": got == want})
+        out[name] = row
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -294,6 +391,8 @@ def main():
                          "0 keeps the default schedule")
     ap.add_argument("--schedule", default=None,
                     help="JSON kernel schedule to use instead of tuning (z0p_schedule fields)")
+    ap.add_argument("--widened", type=int, default=3,
+                    help="timed proofs per widened workload (ZK AIR, FRI ext, range, table, fold); 0 skips them")
     ap.add_argument("--copy-mib", type=int, default=256,
                     help="size of the plain copy kernel run after the proofs (counter calibration and "
                          "achievable-bandwidth reference); 0 disables")
@@ -401,6 +500,8 @@ def main():
     batch = batched(z, ctx, args.batch, 0 if args.schedule else max(1, args.tune_reps // 2)) if args.batch else None
     serve_sched = batch["schedule"] if batch else single_sched
     serve = served(z, args.serve_budget_gib, args.serve_s, serve_sched) if args.serve_s > 0 else None
+    ctx.set_schedule(single_sched)
+    wide = widened(z, ctx, args.widened, not args.no_cpu) if args.widened > 0 else None
 
     out = {
         "metric": METRIC,
@@ -426,6 +527,7 @@ def main():
         "copy_reference": copy,
         "batched": batch,
         "served": serve,
+        "widened": wide,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(n, args.cpu_repeats),
     }
     print(json.dumps(out))
