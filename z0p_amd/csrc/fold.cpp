@@ -70,6 +70,9 @@ struct FoldAcc {
     Ctx& c;
     z0p_fold_params p;
     std::vector<Segment> segs;
+    // The segments' evaluations (2s), coefficients (2s) and NTT scratch (m),
+    // kept across add_segment calls.
+    std::unique_ptr<DevBuf> work;
     FoldAcc(Ctx& ctx, const z0p_fold_params& params) : c(ctx), p(params) {}
 
     uint64_t m() const { return p.seg_steps * p.blowup; }
@@ -92,9 +95,8 @@ struct FoldAcc {
         g.mem = std::make_unique<DevBuf>((2 * mm + tg.words()) * 4);
         uint32_t* A = g.mem->as<uint32_t>();
         uint32_t* tree = A + 2 * mm;
-        // work: the evaluations (2s), their coefficients (2s), NTT scratch (m)
-        DevBuf work((4 * s + mm) * 4);
-        uint32_t *evals = work.as<uint32_t>(), *coeff = evals + 2 * s, *scratch = coeff + 2 * s;
+        if (!work) work = std::make_unique<DevBuf>((4 * s + mm) * 4);
+        uint32_t *evals = work->as<uint32_t>(), *coeff = evals + 2 * s, *scratch = coeff + 2 * s;
         hip_check(hipMemcpyAsync(evals, col_a, s * 4, hipMemcpyHostToDevice, c.stream));
         hip_check(hipMemcpyAsync(evals + s, col_b, s * 4, hipMemcpyHostToDevice, c.stream));
         std::vector<Step> chain;
@@ -122,7 +124,6 @@ struct FoldAcc {
         const uint64_t seq = ++c.seq;
         c.enqueue([&](const Launch& L) { launch_chains(L, chains, items, c.flag_dev, seq); });
         c.wait_flag(seq);
-        hip_check(hipStreamSynchronize(c.stream));  // `work` is released below
         g.cap.resize(size_t(1) << tg.drop);
         for (size_t i = 0; i < g.cap.size(); ++i) std::memcpy(g.cap[i].data(), c.stage_host + 8 * i, 32);
         // transcript (fold.rs:418-478)
