@@ -279,8 +279,9 @@ int z0p_fold_params_new(uint64_t seg_steps, uint64_t blowup, uint64_t num_querie
 int z0p_fold_params_for_security(uint64_t seg_steps, uint32_t in_a, uint32_t in_b, uint32_t out_b,
                                  uint32_t target_bits, z0p_fold_params* out);
 /* fold::Accumulator (fold.rs:361-400): segments stay in device memory on
- * ctx until finalize. The ctx must not run other work between the calls
- * that use one accumulator and its transcript. */
+ * ctx until finalize; their buffers then return to the ctx for the next
+ * accumulator and are freed with the ctx. Free every accumulator before its
+ * ctx. Calls on one accumulator and other calls on its ctx must not overlap. */
 typedef struct z0p_fold_acc z0p_fold_acc;
 int z0p_fold_acc_new(z0p_ctx* ctx, const z0p_fold_params* params, z0p_fold_acc** out);
 void z0p_fold_acc_free(z0p_fold_acc* acc);

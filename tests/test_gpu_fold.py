@@ -116,3 +116,29 @@ def test_errors_match_oracle(ctx):
     with pytest.raises(z.Z0pError) as ze:
         zacc.finalize(zt)
     assert ze.value.code == oe.value.code
+
+
+def test_smallest_segments(ctx):
+    # Segments of two rows (the smallest), a 16-point LDE per segment.
+    traces = o.chained_traces(2, 3)
+    out_b = traces[-1][1][-1]
+    op, zp = o.fold_params_new(2, 8, 4, 4, 1, 1, out_b), z.fold.params_new(2, 8, 4, 4, 1, 1, out_b)
+    got, want = _both(op, zp, traces, b"tiny", ctx)
+    assert got == want
+    assert _verdict(op, zp, b"tiny", got)
+
+
+def test_accumulator_reused_after_finalize(ctx):
+    # finalize consumes the segments; the same accumulator then proves anew.
+    traces = o.chained_traces(16, 2)
+    out_b = traces[-1][1][-1]
+    op, zp = o.fold_params_new(16, 8, 6, 8, 1, 1, out_b), z.fold.params_new(16, 8, 6, 8, 1, 1, out_b)
+    zacc = z.fold.Accumulator(zp, ctx)
+    for label in [b"r1", b"r2"]:
+        zt, ot = z.Transcript(label), o.Transcript(label)
+        oacc = o.FoldAccumulator(op)
+        for ca, cb in traces:
+            zacc.add_segment(ca, cb, zt)
+            oacc.add_segment(ca, cb, ot)
+        assert zacc.finalize(zt) == oacc.finalize(ot)
+        assert len(zacc) == 0

@@ -153,3 +153,13 @@ def test_error_codes_match_oracle(ctx):
     with pytest.raises(z.Z0pError) as ze:
         z.range.prove_with_columns(zp, [P] * 8, [[0] * 8] * 16, z.Transcript(b"e"), ctx)
     assert ze.value.name == "MalformedInput"
+
+
+@pytest.mark.parametrize("n_rows,n_bits,blowup,fs", [(2, 3, 8, 4), (4, 24, 2, 2)])
+def test_smallest_domains(ctx, n_rows, n_bits, blowup, fs):
+    # Two rows (the smallest trace) and a 2-blowup LDE of 8 points.
+    op, zp = _pair(n_rows, n_bits, blowup, 4, fs, 5)
+    print("range tiny", n_rows, n_bits, blowup, flush=True)
+    got = z.range.prove(zp, z.Transcript(b"tiny"), ctx)
+    assert got == o.range_prove(op, o.Transcript(b"tiny"))
+    assert _verdict(op, zp, b"tiny", got)

@@ -119,3 +119,11 @@ def test_error_codes_match_oracle(ctx):
         with pytest.raises(z.Z0pError) as ze:
             got()
         assert ze.value.code == oe.value.code, k
+
+
+def test_smallest_domain(ctx):
+    # Two rows, a two-entry table, a 16-point LDE.
+    op, zp = _pair(2, 8, 4, 4, 7)
+    got = z.table.prove(zp, [7, 9], z.Transcript(b"tiny"), ctx)
+    assert got == o.table_prove(op, [7, 9], o.Transcript(b"tiny"))
+    assert _verdict(op, zp, [7, 9], b"tiny", got)

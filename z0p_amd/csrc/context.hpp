@@ -363,6 +363,25 @@ struct Ctx {
         ring.reset();
     }
 
+    // Device buffers released by IVC accumulators, kept by size for the
+    // next accumulator on this context (freed with the context).
+    std::multimap<size_t, void*> spare;
+    void* take_buffer(size_t bytes) {
+        auto it = spare.find(bytes);
+        if (it != spare.end()) {
+            void* p = it->second;
+            spare.erase(it);
+            return p;
+        }
+        void* p = nullptr;
+        hip_check(hipMalloc(&p, bytes));
+        return p;
+    }
+    // The caller has synchronised every use of `p` on this context's stream.
+    void give_buffer(size_t bytes, void* p) {
+        if (p) spare.emplace(bytes, p);
+    }
+
     const GeometryTables& geometry(int logn, int logb);
 
     ~Ctx();

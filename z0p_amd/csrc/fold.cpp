@@ -45,14 +45,18 @@ void absorb_fold_step(const uint32_t c[4], Transcript& tr) {
     tr.absorb("fold/step", b, 16);
 }
 
+// A device buffer from the context's spare list, returned there on release
+// once the context's stream has drained (an error path may leave work queued).
 struct DevBuf {
+    Ctx& c;
+    size_t bytes;
     void* p = nullptr;
-    DevBuf() = default;
-    explicit DevBuf(size_t bytes) { hip_check(hipMalloc(&p, bytes ? bytes : 4)); }
+    DevBuf(Ctx& ctx, size_t n) : c(ctx), bytes(n ? n : 4) { p = c.take_buffer(bytes); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() {
-        if (p) (void)hipFree(p);
+        (void)hipStreamSynchronize(c.stream);
+        c.give_buffer(bytes, p);
     }
     template <typename T>
     T* as() const { return static_cast<T*>(p); }
@@ -92,10 +96,10 @@ struct FoldAcc {
         const uint64_t mm = m();
         const TreeGeom tg = tree_geom(mm, p.cap_height);
         const GeometryTables& gt = c.geometry(logn, logb);
-        g.mem = std::make_unique<DevBuf>((2 * mm + tg.words()) * 4);
+        g.mem = std::make_unique<DevBuf>(c, (2 * mm + tg.words()) * 4);
         uint32_t* A = g.mem->as<uint32_t>();
         uint32_t* tree = A + 2 * mm;
-        if (!work) work = std::make_unique<DevBuf>((4 * s + mm) * 4);
+        if (!work) work = std::make_unique<DevBuf>(c, (4 * s + mm) * 4);
         uint32_t *evals = work->as<uint32_t>(), *coeff = evals + 2 * s, *scratch = coeff + 2 * s;
         hip_check(hipMemcpyAsync(evals, col_a, s * 4, hipMemcpyHostToDevice, c.stream));
         hip_check(hipMemcpyAsync(evals + s, col_b, s * 4, hipMemcpyHostToDevice, c.stream));
@@ -160,7 +164,7 @@ struct FoldAcc {
                 for (int q = 0; q < 4; ++q) d.beta_m[t][q] = to_mont(beta.v[q]);
             }
         }
-        DevBuf desc_dev(desc.size() * sizeof(FoldStepDesc));
+        DevBuf desc_dev(c, desc.size() * sizeof(FoldStepDesc));
         hip_check(hipMemcpyAsync(desc_dev.p, desc.data(), desc.size() * sizeof(FoldStepDesc), hipMemcpyHostToDevice,
                                  c.stream));
         z0p_fri_params fp = fold_fri_params(p);
@@ -172,7 +176,7 @@ struct FoldAcc {
         // Step openings: A, B at the index and the next row, two paths.
         const uint32_t wpq = 4 + 16 * uint32_t(tg.target);
         const size_t nq = queries.size(), words = segs.size() * nq * size_t(wpq);
-        DevBuf qdev(nq * 8), odev(words * 4);
+        DevBuf qdev(c, nq * 8), odev(c, words * 4);
         hip_check(hipMemcpyAsync(qdev.p, queries.data(), nq * 8, hipMemcpyHostToDevice, c.stream));
         FoldOpenJob oj{};
         oj.steps = desc_dev.as<FoldStepDesc>();

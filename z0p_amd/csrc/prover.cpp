@@ -167,6 +167,7 @@ Ctx::~Ctx() {
     if (flag_host) (void)hipHostFree(flag_host);
     if (mailbox_host) (void)hipHostFree(mailbox_host);
     if (scratch) (void)hipFree(scratch);
+    for (auto& kv : spare) (void)hipFree(kv.second);
     if (tw4096) (void)hipFree(tw4096);
     if (tw4096_inv) (void)hipFree(tw4096_inv);
     if (stream) (void)hipStreamDestroy(stream);
