@@ -142,3 +142,13 @@ def test_accumulator_reused_after_finalize(ctx):
             oacc.add_segment(ca, cb, ot)
         assert zacc.finalize(zt) == oacc.finalize(ot)
         assert len(zacc) == 0
+
+
+def test_cap_covers_every_row(ctx):
+    traces = o.chained_traces(8, 3)
+    out_b = traces[-1][1][-1]
+    op = o.fold_params_new(8, 8, 8, 4, 1, 1, out_b).replace(cap_height=8)
+    zp = z.fold.params_new(8, 8, 8, 4, 1, 1, out_b).replace(cap_height=8)
+    got, want = _both(op, zp, traces, b"c8", ctx)
+    assert got == want
+    assert _verdict(op, zp, b"c8", got)

@@ -163,3 +163,11 @@ def test_smallest_domains(ctx, n_rows, n_bits, blowup, fs):
     got = z.range.prove(zp, z.Transcript(b"tiny"), ctx)
     assert got == o.range_prove(op, o.Transcript(b"tiny"))
     assert _verdict(op, zp, b"tiny", got)
+
+
+def test_cap_covers_every_row(ctx):
+    # cap height 8 over a 64-row LDE: the cap is the leaf level, paths are empty.
+    op, zp = _pair(8, 16, 8, 8, 4, 12_345, cap_height=8)
+    got = z.range.prove(zp, z.Transcript(b"c8"), ctx)
+    assert got == o.range_prove(op, o.Transcript(b"c8"))
+    assert _verdict(op, zp, b"c8", got)

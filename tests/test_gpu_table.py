@@ -127,3 +127,10 @@ def test_smallest_domain(ctx):
     got = z.table.prove(zp, [7, 9], z.Transcript(b"tiny"), ctx)
     assert got == o.table_prove(op, [7, 9], o.Transcript(b"tiny"))
     assert _verdict(op, zp, [7, 9], b"tiny", got)
+
+
+def test_cap_covers_every_row(ctx):
+    op, zp = _pair(8, 8, 8, 4, 30, cap_height=8)
+    got = z.table.prove(zp, TABLE, z.Transcript(b"c8"), ctx)
+    assert got == o.table_prove(op, TABLE, o.Transcript(b"c8"))
+    assert _verdict(op, zp, TABLE, b"c8", got)
