@@ -704,9 +704,16 @@ __global__ __launch_bounds__(WG) void k_range_compose(RangeComposeJob inl, const
         const uint32_t j = brev_bits(p >> J.logn, J.logb);
         const uint32_t iz = gc(J.zinv_m)[j];
         const uint32_t v = cols[p];
+        // every bit column's load in flight before the arithmetic
+        uint32_t bits[RANGE_MAX_BITS];
+#pragma unroll
+        for (int c = 0; c < RANGE_MAX_BITS; ++c)
+            bits[c] = uint32_t(c) < nb ? cols[uint64_t(c + 1) * J.col_stride + p] : 0u;
         uint32_t rec = 0, acc[4] = {0, 0, 0, 0};
-        for (uint32_t c = nb; c-- > 0;) {
-            const uint32_t b = cols[uint64_t(c + 1) * J.col_stride + p];
+#pragma unroll
+        for (int c = RANGE_MAX_BITS - 1; c >= 0; --c) {
+            if (uint32_t(c) >= nb) continue;
+            const uint32_t b = bits[c];
             rec = add(dbl(rec), b);
             const uint32_t q = mont_mul(b, sub(b, 1u));  // C (C - 1) / R
 #pragma unroll
@@ -746,16 +753,27 @@ __global__ __launch_bounds__(WG) void k_table_compose(TableComposeJob inl, const
         }
     }
     batch_inv_m<CELT>(d1);
-#pragma unroll 1
+    // The CELT vanishing products advance together: independent chains.
+    uint32_t vv[CELT], vm[CELT], pm[CELT];
+#pragma unroll
     for (int e = 0; e < CELT; ++e) {
         const uint32_t p = base + e * WG;
-        if (p >= m) break;
+        vv[e] = p < m ? gc(J.v)[p] : 0u;
+        vm[e] = to_mont(vv[e]);
+        pm[e] = R1;
+    }
+    for (uint32_t k = 0; k < J.table_len; ++k) {
+        const uint32_t t = T[k];
+#pragma unroll
+        for (int e = 0; e < CELT; ++e) pm[e] = mont_mul(pm[e], sub(vm[e], t));
+    }
+#pragma unroll
+    for (int e = 0; e < CELT; ++e) {
+        const uint32_t p = base + e * WG;
+        if (p >= m) continue;
         const uint32_t j = brev_bits(p >> J.logn, J.logb);
-        const uint32_t v = gc(J.v)[p];
-        const uint32_t vm = to_mont(v);
-        uint32_t pm = R1;
-        for (uint32_t k = 0; k < J.table_len; ++k) pm = mont_mul(pm, sub(vm, T[k]));
-        const uint32_t sm = mont_mul(from_mont(pm), gc(J.zinv_m)[j]);  // prod / Z
+        const uint32_t v = vv[e];
+        const uint32_t sm = mont_mul(from_mont(pm[e]), gc(J.zinv_m)[j]);  // prod / Z
         const uint32_t sb = mont_mul(sub(v, C.value), d1[e]);            // (V - value) / (x - 1)
         uint32_t o[4];
 #pragma unroll
