@@ -1702,7 +1702,19 @@ void Engine::pipeline(Proof& P) {
                 gate.want = g0 + k + 1;
                 gate.src = reinterpret_cast<const uint32_t*>(c_.mailbox_dev + Ctx::MAILBOX_HEAD + (k + 1) * SLOT);
                 gate.dst = reinterpret_cast<uint32_t*>(P.consts + (k + 1) * SLOT);
-                gate.words = uint32_t((phs[k + 1].ph == PH_OPEN ? open_slots : 1) * SLOT / 4);
+                // only the next phase's constants cross PCIe
+                size_t bytes = 64;  // the proof-of-work head
+                switch (phs[k + 1].ph) {
+                    case PH_COMPOSE:
+                        bytes = P.row == ROW_RANGE ? sizeof(RangeConsts)
+                                : P.row == ROW_TABLE ? sizeof(TableConsts)
+                                                     : sizeof(ComposeConsts);
+                        break;
+                    case PH_FOLD: bytes = sizeof(FoldConsts); break;
+                    case PH_OPEN: bytes = 8 + 8 * size_t(P.fp.num_queries); break;
+                    default: break;
+                }
+                gate.words = uint32_t((bytes + 3) / 4);
                 gate.spin_limit = 1u << 22;
                 gate.timeout = reinterpret_cast<uint32_t*>(c_.mailbox_dev + 8);
             }
