@@ -303,6 +303,11 @@ int z0p_fold_verify(const z0p_fold_params* params, z0p_transcript* transcript, c
 /* Device-memory budget of the batch and server schedulers on this ctx, in
  * bytes; 0 (the default) lets the library choose from free device memory. */
 int z0p_ctx_set_hbm_budget(z0p_ctx* ctx, uint64_t bytes);
+/* Batch and server schedulers on this ctx split the active proofs into two
+ * alternating waves once at least `min_active` are active (0: never), so the
+ * host's transcript steps for one wave overlap the other wave's kernels. The
+ * proofs are the same either way. Default 8. */
+int z0p_ctx_set_waves(z0p_ctx* ctx, uint32_t min_active);
 /* Runs air::prove for every params[i] with transcripts[i] (each advanced as
  * that single call would advance it). The instances share every kernel
  * launch and are admitted as device memory allows. statuses[i] receives

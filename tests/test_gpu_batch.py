@@ -158,3 +158,27 @@ def test_server_rejects_invalid_request(ctx):
         assert e.value.name == "InvalidBlowup"
         assert srv.pending() == 0
     sctx.close()
+
+
+@pytest.mark.parametrize("waves", [0, 2])
+def test_batch_waves_on_and_off(ctx, waves):
+    """One launch for all proofs, or two alternating waves (from 2 active
+    proofs, also under a budget that holds only a few): the same bytes and
+    the same transcript states."""
+    inst = _instances(10, 6, 13, 17)
+    params = [z.air.for_security(n, a0, b0, bl) for n, a0, b0, bl, _ in inst]
+    ots = [o.Transcript(lab) for *_, lab in inst]
+    want = [o.air_prove(o.air_params_for_security(n, a0, b0, bl), t) for (n, a0, b0, bl, _), t in zip(inst, ots)]
+    ctx.set_waves(waves)
+    try:
+        trs = [z.Transcript(lab) for *_, lab in inst]
+        got = z.air.prove_batch(params, trs, ctx)
+        ctx.set_hbm_budget(16 << 20)
+        got_tight = z.air.prove_batch(params, [z.Transcript(lab) for *_, lab in inst], ctx)
+    finally:
+        ctx.set_hbm_budget(0)
+        ctx.set_waves(8)
+    assert got == want
+    assert got_tight == want
+    for t, ot in zip(trs, ots):
+        assert t.challenge_bytes(b"after") == ot.challenge_bytes(b"after")

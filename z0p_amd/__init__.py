@@ -164,6 +164,12 @@ class Context:
         """Device-memory budget of batched proving on this context (0: auto)."""
         check(lib().z0p_ctx_set_hbm_budget(self._h, nbytes))
 
+    def set_waves(self, min_active: int):
+        """Batched and served proving alternate two waves of proofs (host
+        transcript steps of one overlapping the other's kernels) once at
+        least ``min_active`` are active; 0 turns waves off."""
+        check(lib().z0p_ctx_set_waves(self._h, min_active))
+
 
 class Server:
     """Asynchronous proving loop (z0p_server) that takes over a Context:

@@ -165,6 +165,7 @@ SIGNATURES = {
                                                      ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.POINTER(u8p),
                                                      ctypes.POINTER(sz)]),
     "z0p_ctx_set_hbm_budget": (ctypes.c_int, [vp, u64]),
+    "z0p_ctx_set_waves": (ctypes.c_int, [vp, u32]),
     "z0p_air_verify_batch": (ctypes.c_int, [ctypes.POINTER(AirParamsC), ctypes.POINTER(vp), ctypes.POINTER(u8p),
                                             ctypes.POINTER(sz), sz, u32, ctypes.POINTER(ctypes.c_int)]),
     "z0p_ctx_get_schedule": (ctypes.c_int, [vp, ctypes.POINTER(ScheduleC)]),

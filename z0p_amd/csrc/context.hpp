@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <condition_variable>
 #include <cstdint>
@@ -13,6 +14,7 @@
 #include <memory>
 #include <mutex>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "kernels.hpp"
@@ -108,7 +110,8 @@ class DevicePool {
 // return the matching device address; flush() uploads everything pushed so
 // far with one copy. In replay mode pushes only recompute addresses, which
 // match the recorded ones when the same pushes repeat in the same order.
-// Reset after the stream has been synchronised.
+// Reset after the stream has been synchronised, or, while other work is in
+// flight, restart in a region that work does not use (reset_to).
 class DescRing final : public TableSink {
    public:
     ~DescRing() override {
@@ -118,14 +121,15 @@ class DescRing final : public TableSink {
     void init(size_t bytes) {
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_), bytes, hipHostMallocDefault));
         hip_check(hipMalloc(reinterpret_cast<void**>(&dev_), bytes));
-        cap_ = bytes;
+        cap_ = limit_ = bytes;
     }
+    size_t capacity() const { return cap_; }
     const void* push(const void* p, size_t bytes) override {
         size_t at = (used_ + 255) & ~size_t(255);
         if (replay_) {
             require(at + bytes <= flushed_, Z0P_ERR_DEVICE);
         } else {
-            require(at + bytes <= cap_, Z0P_ERR_OUT_OF_MEMORY);
+            require(at + bytes <= limit_, Z0P_ERR_OUT_OF_MEMORY);
             std::memcpy(host_ + at, p, bytes);
         }
         used_ = at + bytes;
@@ -142,15 +146,18 @@ class DescRing final : public TableSink {
         replay_ = true;
     }
     void end_replay() { replay_ = false; }
-    void reset() {
-        used_ = flushed_ = 0;
+    void reset() { reset_to(0, cap_); }
+    // Pushes fill [base, limit) from now on.
+    void reset_to(size_t base, size_t limit) {
+        used_ = flushed_ = base;
+        limit_ = std::min(limit, cap_);
         replay_ = false;
     }
 
    private:
     uint8_t* host_ = nullptr;
     uint8_t* dev_ = nullptr;
-    size_t cap_ = 0, used_ = 0, flushed_ = 0;
+    size_t cap_ = 0, limit_ = 0, used_ = 0, flushed_ = 0;
     bool replay_ = false;
 };
 
@@ -261,6 +268,14 @@ struct Ctx {
     uint32_t* stage_dev = nullptr;
     uint32_t* stage_host = nullptr;
     size_t stage_words = 0;
+    // A second staging buffer: the scheduler alternates two waves of proofs,
+    // and the wave on the device writes one buffer while the host reads the
+    // other wave's words from the other (swap_stage exchanges them).
+    uint32_t* alt_stage_dev = nullptr;
+    uint32_t* alt_stage_host = nullptr;
+    size_t alt_stage_words = 0;
+    // Two waves once this many proofs are active (0: never).
+    size_t waves_min = 8;
     // Completion flag in pinned host memory: the iteration's last kernel
     // stores the iteration number there after a system-scope fence.
     uint64_t* flag_host = nullptr;
@@ -300,6 +315,12 @@ struct Ctx {
         ring.end_replay();
     }
 
+    void swap_stage() {
+        std::swap(stage_dev, alt_stage_dev);
+        std::swap(stage_host, alt_stage_host);
+        std::swap(stage_words, alt_stage_words);
+    }
+
     void reserve_stage(size_t words) {
         if (words <= stage_words) return;
         hip_check(hipStreamSynchronize(stream));
@@ -317,7 +338,9 @@ struct Ctx {
     // queried now and then so a device failure surfaces as an error.
     // sync_events: when profiling, synchronise to resolve kernel events
     // (not while pre-enqueued phases still wait on host gates).
-    void wait_flag(uint64_t s, bool sync_events = true) {
+    // reset_ring: also restart the descriptor ring (not while another
+    // launch that reads it is still queued).
+    void wait_flag(uint64_t s, bool sync_events = true, bool reset_ring = true) {
         const volatile uint64_t* f = flag_host;
         for (uint64_t spins = 1;; ++spins) {
             if (__atomic_load_n(const_cast<const uint64_t*>(f), __ATOMIC_ACQUIRE) >= s) break;
@@ -334,7 +357,7 @@ struct Ctx {
             hip_check(hipStreamSynchronize(stream));
             prof.resolve();
         }
-        ring.reset();
+        if (reset_ring) ring.reset();
     }
 
     void reserve_scratch(size_t bytes) {

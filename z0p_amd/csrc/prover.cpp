@@ -164,6 +164,7 @@ Ctx::~Ctx() {
     tables.clear();
     if (stream) (void)hipStreamSynchronize(stream);
     if (stage_host) (void)hipHostFree(stage_host);
+    if (alt_stage_host) (void)hipHostFree(alt_stage_host);
     if (flag_host) (void)hipHostFree(flag_host);
     if (mailbox_host) (void)hipHostFree(mailbox_host);
     if (scratch) (void)hipFree(scratch);
@@ -189,6 +190,7 @@ Ctx* ctx_create(int device) {
     hip_check(hipMemcpy(c->tw4096_inv, inv.data(), 2048 * 4, hipMemcpyHostToDevice));
     c->ring.init(size_t(32) << 20);
     if (const char* e = std::getenv("Z0P_PIPELINE")) c->pipeline_single = std::strcmp(e, "0") != 0;
+    if (const char* e = std::getenv("Z0P_WAVES")) c->waves_min = size_t(std::strtoull(e, nullptr, 10));
     c->reserve_stage(size_t(1) << 16);
     hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->flag_host), 256, hipHostMallocMapped | hipHostMallocCoherent));
     hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->flag_dev), c->flag_host, 0));
@@ -534,6 +536,7 @@ struct Proof {
     Phase phase = PH_FRONT;
     size_t depth = 0;      // layer folded in the current PH_FOLD iteration
     size_t stage_off = 0;  // words, this iteration
+    bool in_flight = false;  // launched; its host step has not run yet
     // result
     int status = Z0P_OK;
     Bytes proof, commitment;
@@ -1513,6 +1516,8 @@ size_t auto_budget(Ctx& c, size_t want) {
 }
 
 Engine::~Engine() {
+    // Proof memory goes back to the pool once no launch can still use it.
+    if (!flight_.empty()) (void)hipStreamSynchronize(c_.stream);
     for (Proof* p : active_)
         if (p->mem) c_.pool.release(p->mem, p->mem_bytes);
 }
@@ -1556,37 +1561,21 @@ void Engine::admit() {
         queue_.pop_front();
         p->mem = mem;
         p->t_admit = now_ms();
-        stage_need_ += p->stage_need;
-        c_.reserve_stage(stage_need_);
         start(c_, *p);
         active_.push_back(p);
     }
 }
 
-bool Engine::iterate() {
-    admit();
-    if (active_.empty()) return !queue_.empty();
-    if (pipelined_ && c_.pipeline_single && active_.size() == 1 && queue_.empty() && active_[0]->phase == PH_FRONT) {
-        Proof* p = active_[0];
-        try {
-            pipeline(*p);
-            if (p->phase == PH_DONE) {
-                stage_need_ -= p->stage_need;
-                finish(p, Z0P_OK);
-                active_.clear();
-            }
-        } catch (const Status& s) {
-            stage_need_ -= p->stage_need;
-            finish(p, s.code);
-            active_.clear();
-        }
-        return !active_.empty() || !queue_.empty();
-    }
-    const double t_build = now_ms();
-    std::vector<std::vector<Step>> chains(active_.size());
+// Builds and enqueues the current phase of every proof in `ws` as one launch
+// sequence; returns the sequence number its completion flag will carry.
+uint64_t Engine::launch(const std::vector<Proof*>& ws) {
+    size_t need = 0;
+    for (Proof* p : ws) need += p->stage_need;
+    c_.reserve_stage(need);
+    std::vector<std::vector<Step>> chains(ws.size());
     std::vector<CollectItem> items;
     size_t used = 0;
-    for (size_t i = 0; i < active_.size(); ++i) build(c_, *active_[i], chains[i], items, used);
+    for (size_t i = 0; i < ws.size(); ++i) build(c_, *ws[i], chains[i], items, used);
     // Grind sweeps: a job's workgroups scan its nonces in order, so a
     // smaller sweep per job wastes fewer hashes above the smallest passing
     // nonce; spread the schedule's workgroups over the jobs grinding now.
@@ -1603,50 +1592,117 @@ bool Engine::iterate() {
     for (auto& ch : chains) ptrs.push_back(&ch);
     const uint64_t seq = ++c_.seq;
     c_.enqueue([&](const Launch& L) { launch_chains(L, ptrs, items, c_.flag_dev, seq); });
+    for (Proof* p : ws) p->in_flight = true;
+    return seq;
+}
+
+// Waits for launch `seq` of the proofs in `ws`, runs their host transcript
+// steps and ends those that are done (or failed).
+void Engine::complete(const std::vector<Proof*>& ws, uint64_t seq, bool reset_ring) {
     const double t_wait = now_ms();
-    c_.wait_flag(seq);
+    c_.wait_flag(seq, true, reset_ring);
     const double t_host = now_ms();
     ++iterations_;
     // Host transcript steps: independent per proof, so a batch spreads them
     // over the host pool.
-    std::vector<int> err(active_.size(), Z0P_OK);
+    std::vector<int> err(ws.size(), Z0P_OK);
     auto step = [&](size_t i) {
         try {
-            host_step(c_, *active_[i]);
+            host_step(c_, *ws[i]);
         } catch (const Status& s) {
             err[i] = s.code;
         } catch (...) {
             err[i] = Z0P_ERR_DEVICE;
         }
     };
-    if (active_.size() >= 4) {
+    if (ws.size() >= 4) {
         if (!c_.host_pool) {
             const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
             c_.host_pool.reset(new HostPool(std::min(hw, 16u) - 1));
         }
-        c_.host_pool->run(active_.size(), step);
+        c_.host_pool->run(ws.size(), step);
     } else {
-        for (size_t i = 0; i < active_.size(); ++i) step(i);
+        for (size_t i = 0; i < ws.size(); ++i) step(i);
     }
-    c_.prof.host("~host_build_launch", t_wait - t_build);
     c_.prof.host("~host_wait_device", t_host - t_wait);
     c_.prof.host("~host_transcript", now_ms() - t_host);
-    std::vector<Proof*> still;
-    for (size_t i = 0; i < active_.size(); ++i) {
-        Proof* p = active_[i];
-        if (err[i] != Z0P_OK) {
-            stage_need_ -= p->stage_need;
+    bool ended = false;
+    for (size_t i = 0; i < ws.size(); ++i) {
+        Proof* p = ws[i];
+        p->in_flight = false;
+        if (err[i] != Z0P_OK || p->phase == PH_DONE) {
             finish(p, err[i]);
-            continue;
-        }
-        if (p->phase == PH_DONE) {
-            stage_need_ -= p->stage_need;
-            finish(p, Z0P_OK);
-        } else {
-            still.push_back(p);
+            ended = true;
         }
     }
-    active_.swap(still);
+    if (ended)
+        active_.erase(std::remove_if(active_.begin(), active_.end(), [](Proof* p) { return p->phase == PH_DONE; }),
+                      active_.end());
+}
+
+bool Engine::iterate() {
+    admit();
+    if (flight_.empty()) {
+        if (active_.empty()) return !queue_.empty();
+        c_.ring.reset();  // every launch has completed
+        if (pipelined_ && c_.pipeline_single && active_.size() == 1 && queue_.empty() &&
+            active_[0]->phase == PH_FRONT) {
+            Proof* p = active_[0];
+            try {
+                pipeline(*p);
+                if (p->phase == PH_DONE) {
+                    finish(p, Z0P_OK);
+                    active_.clear();
+                }
+            } catch (const Status& s) {
+                finish(p, s.code);
+                active_.clear();
+            }
+            return !active_.empty() || !queue_.empty();
+        }
+        if (c_.waves_min == 0 || active_.size() < c_.waves_min) {
+            // One wave: every active proof's phase in one launch.
+            const double t_build = now_ms();
+            const std::vector<Proof*> ws = active_;
+            const uint64_t seq = launch(ws);
+            c_.prof.host("~host_build_launch", now_ms() - t_build);
+            complete(ws, seq, true);
+            return !active_.empty() || !queue_.empty();
+        }
+    }
+    // Two waves: the host runs one wave's transcript steps while the device
+    // runs the other wave's phase. Launch the proofs that are not in flight
+    // (oldest first), stopping near half the active weight so the waves stay
+    // even, then complete the wave launched before. Each wave has its own
+    // staging buffer and half of the descriptor ring.
+    const double t_build = now_ms();
+    size_t total = 0;
+    for (Proof* p : active_) total += p->mem_bytes;
+    std::vector<Proof*> ws;
+    size_t took = 0;
+    for (Proof* p : active_) {
+        if (p->in_flight) continue;
+        if (!ws.empty() && 2 * took >= total) break;
+        ws.push_back(p);
+        took += p->mem_bytes;
+    }
+    uint64_t seq = 0;
+    if (!ws.empty()) {
+        c_.swap_stage();
+        const size_t half = c_.ring.capacity() / 2;
+        ring_half_ ^= 1;
+        c_.ring.reset_to(ring_half_ * half, (ring_half_ + 1) * half);
+        seq = launch(ws);
+    }
+    c_.prof.host("~host_build_launch", now_ms() - t_build);
+    if (!flight_.empty()) {
+        // The earlier wave staged its words in the other buffer.
+        if (!ws.empty()) c_.swap_stage();
+        complete(flight_, flight_seq_, false);
+        if (!ws.empty()) c_.swap_stage();
+    }
+    flight_ = std::move(ws);
+    flight_seq_ = seq;
     return !active_.empty() || !queue_.empty();
 }
 
@@ -1781,9 +1837,13 @@ void Engine::pipeline(Proof& P) {
 }
 
 void Engine::fail_all(int status) {
-    for (Proof* p : active_) finish(p, status);
+    if (!flight_.empty()) (void)hipStreamSynchronize(c_.stream);
+    flight_.clear();
+    for (Proof* p : active_) {
+        p->in_flight = false;
+        finish(p, status);
+    }
     active_.clear();
-    stage_need_ = 0;
     while (!queue_.empty()) {
         finish(queue_.front(), status);
         queue_.pop_front();

@@ -125,11 +125,15 @@ class Engine {
     void admit();
     void finish(Proof* p, int status);
     void pipeline(Proof& p);
+    uint64_t launch(const std::vector<Proof*>& ws);
+    void complete(const std::vector<Proof*>& ws, uint64_t seq, bool reset_ring);
     Ctx& c_;
     size_t budget_, max_active_;
     std::deque<Proof*> queue_;
     std::vector<Proof*> active_;
-    size_t stage_need_ = 0;
+    std::vector<Proof*> flight_;  // launched wave awaiting its host steps
+    uint64_t flight_seq_ = 0;
+    int ring_half_ = 0;
     uint64_t iterations_ = 0;
     std::function<void(Proof*)> on_done_;
     bool pipelined_ = false;

@@ -178,6 +178,12 @@ int z0p_ctx_set_hbm_budget(z0p_ctx* ctx, uint64_t bytes) {
     return Z0P_OK;
 }
 
+int z0p_ctx_set_waves(z0p_ctx* ctx, uint32_t min_active) {
+    if (!ctx) return Z0P_ERR_INVALID_ARGUMENT;
+    ctx->c->waves_min = min_active;
+    return Z0P_OK;
+}
+
 int z0p_air_prove_batch(z0p_ctx* ctx, const z0p_air_params* params, z0p_transcript* const* transcripts,
                         size_t count, uint8_t** proofs, size_t* proof_lens, int* statuses) {
     if (!ctx || (count && (!params || !transcripts || !proofs || !proof_lens || !statuses)))
