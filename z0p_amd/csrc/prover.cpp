@@ -1837,7 +1837,9 @@ void Engine::pipeline(Proof& P) {
 }
 
 void Engine::fail_all(int status) {
-    if (!flight_.empty()) (void)hipStreamSynchronize(c_.stream);
+    // Nothing enqueued (a wave in flight, a launch cut short) may still use
+    // the memory the proofs hand back.
+    (void)hipStreamSynchronize(c_.stream);
     flight_.clear();
     for (Proof* p : active_) {
         p->in_flight = false;
