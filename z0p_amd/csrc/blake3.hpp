@@ -14,10 +14,6 @@
 
 #include "field.hpp"
 
-#if !defined(__HIP_DEVICE_COMPILE__)
-#include <immintrin.h>
-#endif
-
 namespace z0p {
 
 namespace b3 {
@@ -51,67 +47,9 @@ struct B3Schedule {
     c = c + d;                       \
     b = rotr(b ^ c, 7);
 
-#if !defined(__HIP_DEVICE_COMPILE__)
-// Host compression with the state's four rows in SSE registers: the column
-// step is one G per lane, and the diagonal step runs after rotating rows 1-3
-// by 1, 2 and 3 lanes. The 16- and 8-bit rotations are SSSE3 byte shuffles.
-// Same output as the scalar form; used when the CPU has SSSE3.
-__attribute__((target("ssse3"))) inline void b3_g4(__m128i& a, __m128i& b, __m128i& c, __m128i& d, __m128i x,
-                                                     __m128i y) {
-    const __m128i r16 = _mm_setr_epi8(2, 3, 0, 1, 6, 7, 4, 5, 10, 11, 8, 9, 14, 15, 12, 13);
-    const __m128i r8 = _mm_setr_epi8(1, 2, 3, 0, 5, 6, 7, 4, 9, 10, 11, 8, 13, 14, 15, 12);
-    a = _mm_add_epi32(_mm_add_epi32(a, b), x);
-    d = _mm_shuffle_epi8(_mm_xor_si128(d, a), r16);
-    c = _mm_add_epi32(c, d);
-    b = _mm_xor_si128(b, c);
-    b = _mm_or_si128(_mm_srli_epi32(b, 12), _mm_slli_epi32(b, 20));
-    a = _mm_add_epi32(_mm_add_epi32(a, b), y);
-    d = _mm_shuffle_epi8(_mm_xor_si128(d, a), r8);
-    c = _mm_add_epi32(c, d);
-    b = _mm_xor_si128(b, c);
-    b = _mm_or_si128(_mm_srli_epi32(b, 7), _mm_slli_epi32(b, 25));
-}
-
-__attribute__((target("ssse3"))) inline void compress8_ssse3(const uint32_t cv[8], const uint32_t m[16],
-                                                              uint32_t counter_lo, uint32_t counter_hi,
-                                                              uint32_t block_len, uint32_t flags, uint32_t out[8]) {
-    constexpr B3Schedule S;
-    __m128i a = _mm_loadu_si128(reinterpret_cast<const __m128i*>(cv));
-    __m128i b = _mm_loadu_si128(reinterpret_cast<const __m128i*>(cv + 4));
-    __m128i c = _mm_setr_epi32(int(b3::IV0), int(b3::IV1), int(b3::IV2), int(b3::IV3));
-    __m128i d = _mm_setr_epi32(int(counter_lo), int(counter_hi), int(block_len), int(flags));
-    for (int r = 0; r < 7; ++r) {
-        const uint8_t* s = S.idx[r];
-        b3_g4(a, b, c, d, _mm_setr_epi32(int(m[s[0]]), int(m[s[2]]), int(m[s[4]]), int(m[s[6]])),
-              _mm_setr_epi32(int(m[s[1]]), int(m[s[3]]), int(m[s[5]]), int(m[s[7]])));
-        b = _mm_shuffle_epi32(b, _MM_SHUFFLE(0, 3, 2, 1));
-        c = _mm_shuffle_epi32(c, _MM_SHUFFLE(1, 0, 3, 2));
-        d = _mm_shuffle_epi32(d, _MM_SHUFFLE(2, 1, 0, 3));
-        b3_g4(a, b, c, d, _mm_setr_epi32(int(m[s[8]]), int(m[s[10]]), int(m[s[12]]), int(m[s[14]])),
-              _mm_setr_epi32(int(m[s[9]]), int(m[s[11]]), int(m[s[13]]), int(m[s[15]])));
-        b = _mm_shuffle_epi32(b, _MM_SHUFFLE(2, 1, 0, 3));
-        c = _mm_shuffle_epi32(c, _MM_SHUFFLE(1, 0, 3, 2));
-        d = _mm_shuffle_epi32(d, _MM_SHUFFLE(0, 3, 2, 1));
-    }
-    _mm_storeu_si128(reinterpret_cast<__m128i*>(out), _mm_xor_si128(a, c));
-    _mm_storeu_si128(reinterpret_cast<__m128i*>(out + 4), _mm_xor_si128(b, d));
-}
-
-inline bool host_has_ssse3() {
-    static const bool has = __builtin_cpu_supports("ssse3");
-    return has;
-}
-#endif
-
 // Compress one block; writes the 8-word chaining value to out (may alias cv).
 Z0P_HD void compress8(const uint32_t cv[8], const uint32_t m[16], uint32_t counter_lo, uint32_t counter_hi,
                       uint32_t block_len, uint32_t flags, uint32_t out[8]) {
-#if !defined(__HIP_DEVICE_COMPILE__)
-    if (host_has_ssse3()) {
-        compress8_ssse3(cv, m, counter_lo, counter_hi, block_len, flags, out);
-        return;
-    }
-#endif
     constexpr B3Schedule S;
     uint32_t s0 = cv[0], s1 = cv[1], s2 = cv[2], s3 = cv[3], s4 = cv[4], s5 = cv[5], s6 = cv[6], s7 = cv[7];
     uint32_t s8 = b3::IV0, s9 = b3::IV1, s10 = b3::IV2, s11 = b3::IV3;
