@@ -66,7 +66,7 @@ TuneResult autotune(Ctx& c, const std::vector<z0p_air_params>& work, unsigned re
     const std::vector<Knob> knobs = {
         {"tail_group_max_log", &Schedule::tail_group_max_log, {5, 6, 7, 8, 9}},
         {"tail_spread_log", &Schedule::tail_spread_log, {0, 6, 7, 8}},
-        {"tail_max_log", &Schedule::tail_max_log, {15, 16, 17, 18}},
+        {"tail_max_log", &Schedule::tail_max_log, {10, 12, 14, 15, 16, 17, 18}},
         {"levels_fuse", &Schedule::levels_fuse, {1, 2}},
         {"small_max_log", &Schedule::small_max_log, {0, 8, 10}},
         {"grind_blocks", &Schedule::grind_blocks, {1024, 2048, 4096, 8192}},
