@@ -11,6 +11,7 @@ data is produced and kept in HBM, so no host buffers enter the timed region.
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -175,10 +176,12 @@ def batched(z, ctx, count, tune_reps, seed=2024):
     }
 
 
-def served(z, budget_gib, duration_s, schedule, seed=77):
+def served(z, budget_gib, duration_s, schedule, single_schedule=None, seed=77):
     """Open-loop Poisson arrivals of AIR proofs with log2 steps uniform in
     10..20 against a fixed HBM budget: saturated throughput first (all
-    requests at once), then three offered loads below it."""
+    requests at once, median of three rounds), then three offered loads
+    below it. The one-at-a-time comparison runs under the single-proof
+    schedule, as the batched comparison does."""
     import random
     import threading
     rng = random.Random(seed)
@@ -226,8 +229,7 @@ def served(z, budget_gib, duration_s, schedule, seed=77):
     with z.Server(sctx, hbm_budget=budget_gib << 30) as srv:
         run(srv, [seeded_instance(z, k, k, b"serve/warm/%d" % k) for k in range(10, 21)], [0.0] * 11)
         sat_n = 400
-        wall, lat = run(srv, make(sat_n, "sat"), [0.0] * sat_n)
-        sat = sat_n / wall
+        sat = statistics.median([sat_n / run(srv, make(sat_n, "sat%d" % k), [0.0] * sat_n)[0] for k in range(3)])
         for frac in (0.5, 0.8, 0.95):
             qps = frac * sat
             count = max(50, min(4000, int(qps * duration_s)))
@@ -238,12 +240,17 @@ def served(z, budget_gib, duration_s, schedule, seed=77):
             wall, lat = run(srv, make(count, "q%d" % int(frac * 100)), arr)
             levels.append({"offered_qps": round(qps, 1), "requests": count, "proofs_per_s": round(count / wall, 1),
                            "p50_ms": pct(lat, 0.50), "p99_ms": pct(lat, 0.99), "max_ms": round(lat[-1], 3)})
-    # The one-at-a-time GPU path on a sample of the same request mix.
-    sample = make(200, "single")
-    t0 = time.perf_counter()
-    for p, l in sample:
-        z.air.prove(p, z.Transcript(l), sctx)
-    single = len(sample) / (time.perf_counter() - t0)
+    # The one-at-a-time GPU path on samples of the same request mix.
+    if single_schedule:
+        sctx.set_schedule(single_schedule)
+    rates = []
+    for k in range(3):
+        sample = make(200, "single%d" % k)
+        t0 = time.perf_counter()
+        for p, l in sample:
+            z.air.prove(p, z.Transcript(l), sctx)
+        rates.append(len(sample) / (time.perf_counter() - t0))
+    single = statistics.median(rates)
     sctx.close()
     return {
         "workload": "Poisson AIR requests, steps 2^10..2^20 log-uniform, production schedule, HBM budget %d GiB"
@@ -502,7 +509,7 @@ def main():
     single_sched = ctx.get_schedule()
     batch = batched(z, ctx, args.batch, 0 if args.schedule else max(1, args.tune_reps // 2)) if args.batch else None
     serve_sched = batch["schedule"] if batch else single_sched
-    serve = served(z, args.serve_budget_gib, args.serve_s, serve_sched) if args.serve_s > 0 else None
+    serve = served(z, args.serve_budget_gib, args.serve_s, serve_sched, single_sched) if args.serve_s > 0 else None
     ctx.set_schedule(single_sched)
     wide = widened(z, ctx, args.widened, not args.no_cpu) if args.widened > 0 else None
 
