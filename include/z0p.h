@@ -306,7 +306,8 @@ int z0p_ctx_set_hbm_budget(z0p_ctx* ctx, uint64_t bytes);
 /* Batch and server schedulers on this ctx split the active proofs into two
  * alternating waves once at least `min_active` are active (0: never), so the
  * host's transcript steps for one wave overlap the other wave's kernels. The
- * proofs are the same either way. Default 8. */
+ * proofs are the same either way. Default 8. A server started on the ctx
+ * uses the setting in both of its lanes. */
 int z0p_ctx_set_waves(z0p_ctx* ctx, uint32_t min_active);
 /* Runs air::prove for every params[i] with transcripts[i] (each advanced as
  * that single call would advance it). The instances share every kernel

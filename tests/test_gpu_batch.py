@@ -94,10 +94,12 @@ def test_batch_rejects_shared_transcript(ctx):
     assert e.value.name == "InvalidArgument"
 
 
-def _serve(requests, budget, gaps_s):
+def _serve(requests, budget, gaps_s, waves=None):
     """Submit from a producer thread with the given inter-arrival gaps, poll
     from this thread; returns {request index: (proof, latency, queue)}."""
     sctx = z.Context(0)
+    if waves is not None:
+        sctx.set_waves(waves)
     out = {}
     with z.Server(sctx, hbm_budget=budget) as srv:
         ids = {}
@@ -182,3 +184,13 @@ def test_batch_waves_on_and_off(ctx, waves):
     assert got_tight == want
     for t, ot in zip(trs, ots):
         assert t.challenge_bytes(b"after") == ot.challenge_bytes(b"after")
+
+
+def test_server_two_waves_match_alone(ctx):
+    """Both lanes alternating two waves from 2 active proofs: every result is
+    the proof the request gets alone."""
+    inst = _instances(24, 8, 17, 41)
+    params = [z.air.for_security(n, a0, b0, bl) for n, a0, b0, bl, _ in inst]
+    alone = [z.air.prove(p, z.Transcript(lab), ctx) for p, (*_, lab) in zip(params, inst)]
+    out = _serve([(p, lab) for p, (*_, lab) in zip(params, inst)], 0, [0.0] * len(inst), waves=2)
+    assert [out[i][0] for i in range(len(inst))] == alone

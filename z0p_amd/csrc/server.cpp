@@ -242,7 +242,7 @@ int z0p_server_create(z0p_ctx* ctx, uint64_t hbm_budget, uint32_t max_active, z0
         }
         s->max_active = max_active ? max_active : DEFAULT_MAX_ACTIVE;
         // The large lane gets its own context (stream, pool, tables) with the
-        // caller's kernel schedule. The small lane's share of the budget is
+        // caller's kernel schedule and wave setting. The small lane's share of the budget is
         // half of it, at most 1 GiB (a 2^15-step proof needs ~32 MiB).
         Lane& small = s->lanes[0];
         Lane& large = s->lanes[1];
@@ -250,6 +250,7 @@ int z0p_server_create(z0p_ctx* ctx, uint64_t hbm_budget, uint32_t max_active, z0
         large.c = z0p::ctx_create(small_ctx->device);
         large.owned = true;
         large.c->sched = small_ctx->sched;
+        large.c->waves_min = small_ctx->waves_min;
         small.budget = std::min<size_t>(size_t(hbm_budget) / 2, size_t(1) << 30);
         large.budget = size_t(hbm_budget) - small.budget;
         for (Lane& l : s->lanes) {
