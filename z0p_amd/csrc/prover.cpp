@@ -1748,6 +1748,10 @@ void Engine::pipeline(Proof& P) {
     uint64_t* gate = reinterpret_cast<uint64_t*>(c_.mailbox_host);
     uint32_t* timeout = reinterpret_cast<uint32_t*>(c_.mailbox_host + 8);
     __atomic_store_n(timeout, 0u, __ATOMIC_RELEASE);
+    // The openings' slot starts cancelled (live = 0): if a failure opens
+    // every gate to drain the enqueued work, k_open then skips instead of
+    // reading an earlier proof's queries.
+    if (with_open) std::memset(c_.mailbox_host + Ctx::MAILBOX_HEAD + (phs.size() - 1) * SLOT, 0, 8);
     const uint64_t g0 = c_.gate, g_end = g0 + phs.size(), seq0 = c_.seq;
     c_.enqueue([&](const Launch& L) {
         for (size_t k = 0; k < phs.size(); ++k) {
