@@ -172,6 +172,18 @@ class Blake3 {
     }
     void update(const std::vector<uint8_t>& v) { update(v.data(), v.size()); }
 
+    // Copies this state into `o` for a side computation, moving only the
+    // live entries of the subtree stack.
+    void fork_into(Blake3& o) const {
+        std::memcpy(o.chunk_cv_, chunk_cv_, sizeof chunk_cv_);
+        o.chunk_counter_ = chunk_counter_;
+        o.blocks_done_ = blocks_done_;
+        std::memcpy(o.buf_, buf_, buf_len_);
+        o.buf_len_ = buf_len_;
+        std::memcpy(o.stack_, stack_, 32 * stack_len_);
+        o.stack_len_ = stack_len_;
+    }
+
     // Digest of everything absorbed so far; the state stays usable.
     void digest(uint8_t out[32]) const {
         uint32_t m[16];

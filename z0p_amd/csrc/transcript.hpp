@@ -39,7 +39,8 @@ class Transcript {
     void absorb(const char* label, const std::vector<uint8_t>& v) { absorb(label, v.data(), v.size()); }
 
     void squeeze(const char* label, uint8_t out[32]) {
-        Blake3 h = h_;
+        Blake3 h;
+        h_.fork_into(h);
         size_t ll = std::strlen(label);
         uint8_t len8[8];
         store_le64(len8, ll);
