@@ -99,8 +99,31 @@ z0p_transcript* z0p_transcript_clone(const z0p_transcript* t);
 void z0p_transcript_free(z0p_transcript* t);
 /* Transcript::absorb (transcript.rs:72-77). */
 void z0p_transcript_absorb(z0p_transcript* t, const char* label, const uint8_t* data, size_t len);
-/* Transcript::challenge_bytes (transcript.rs:107-109). */
+/* Transcript::challenge_bytes (transcript.rs:100-102). */
 void z0p_transcript_challenge_bytes(z0p_transcript* t, const char* label, uint8_t out[32]);
+/* Transcript::challenge_bb (transcript.rs:124-129): u64 LE of the squeeze
+ * mod p, canonical. */
+uint32_t z0p_transcript_challenge_bb(z0p_transcript* t, const char* label);
+/* Transcript::challenge_ext (transcript.rs:136-141): the four LE words of
+ * the squeeze, each reduced mod p (coordinate 0 first). */
+void z0p_transcript_challenge_ext(z0p_transcript* t, const char* label, uint32_t out[4]);
+/* Transcript::challenge_index (transcript.rs:148-160): *out = low u64 of the
+ * squeeze masked below domain_size. Z0P_ERR_INVALID_DOMAIN_SIZE (the
+ * reference's None) unless domain_size is a nonzero power of two; the
+ * transcript is then left unchanged. */
+int z0p_transcript_challenge_index(z0p_transcript* t, const char* label, uint64_t domain_size, uint64_t* out);
+/* Transcript::grind (transcript.rs:206-228): the smallest u64 nonce whose
+ * BLAKE3(BLAKE3(history) || u64 len || label || u64 nonce) has at least
+ * difficulty_bits leading zero bits, searched on ctx's device, then absorbed
+ * under label. difficulty_bits > 32 absorbs and returns 0, as the reference
+ * does. Labels up to 976 bytes (head and nonce in one BLAKE3 chunk), else
+ * Z0P_ERR_MALFORMED_INPUT. */
+int z0p_transcript_grind(z0p_ctx* ctx, z0p_transcript* t, const char* label, uint32_t difficulty_bits,
+                         uint64_t* nonce);
+/* Transcript::verify_grind (transcript.rs:235-246): 1 when the nonce meets the
+ * difficulty (absorbed as grind absorbs it), 0 otherwise (nothing absorbed;
+ * also for difficulty_bits > 256 or a null argument). Host only. */
+int z0p_transcript_verify_grind(z0p_transcript* t, const char* label, uint64_t nonce, uint32_t difficulty_bits);
 
 /* ---- parameters ---- */
 /* air::Params::new (air.rs:63-101) with grinding_bits = cap_height = 0. */
@@ -124,10 +147,13 @@ int z0p_air_prove(z0p_ctx* ctx, const z0p_air_params* params, z0p_transcript* tr
 int z0p_air_verify(const z0p_air_params* params, z0p_transcript* transcript, const uint8_t* proof, size_t proof_len);
 /* fri::commit (fri.rs:408-433) + fri::public_commitment (fri.rs:863-870) +
  * fri::open (fri.rs:645-666); outputs Commitment::to_bytes and
- * Proof::to_bytes (fri.rs:875-880, 331-336). */
+ * Proof::to_bytes (fri.rs:875-880, 331-336). indices (may be NULL) receives
+ * the num_queries sampled leaf indices that fri::open returns beside the
+ * proof (bit-reversed layer-0 order); indices_cap must be at least
+ * num_queries when it is not NULL (else Z0P_ERR_INVALID_ARGUMENT). */
 int z0p_fri_commit_open(z0p_ctx* ctx, const uint32_t* coeffs, size_t n, const z0p_fri_params* params,
                         z0p_transcript* transcript, uint8_t** commitment, size_t* commitment_len, uint8_t** proof,
-                        size_t* proof_len);
+                        size_t* proof_len, uint64_t* indices, size_t indices_cap);
 /* air::prove_with_trace (air.rs:385-406): explicit columns of len_a = len_b =
  * n_steps canonical values (else Z0P_ERR_MALFORMED_INPUT), both interpolated
  * and extended on the device; then Proof::to_bytes. */
@@ -163,21 +189,52 @@ int z0p_air_prove_zk(z0p_ctx* ctx, const z0p_air_params* params, uint64_t n_rand
  * each (coordinate 0 first). Outputs as z0p_fri_commit_open. */
 int z0p_fri_commit_open_ext(z0p_ctx* ctx, const uint32_t* coeffs4, size_t n, const z0p_fri_params* params,
                             z0p_transcript* transcript, uint8_t** commitment, size_t* commitment_len, uint8_t** proof,
-                            size_t* proof_len);
+                            size_t* proof_len, uint64_t* indices, size_t indices_cap);
 /* fri::commit_evals (fri.rs:473-500) + public_commitment + open: m base-field
  * LDE values in natural order on the 31-coset, declared degree_bound < m. */
 int z0p_fri_commit_open_evals(z0p_ctx* ctx, const uint32_t* evals, size_t m, uint64_t degree_bound,
                               const z0p_fri_params* params, z0p_transcript* transcript, uint8_t** commitment,
-                              size_t* commitment_len, uint8_t** proof, size_t* proof_len);
+                              size_t* commitment_len, uint8_t** proof, size_t* proof_len, uint64_t* indices,
+                              size_t indices_cap);
 /* fri::commit_ext_evals (fri.rs:510-549) + public_commitment + open: m Ext4
  * LDE values (4 words each) in natural order, declared degree_bound < m. */
 int z0p_fri_commit_open_ext_evals(z0p_ctx* ctx, const uint32_t* evals4, size_t m, uint64_t degree_bound,
                                   const z0p_fri_params* params, z0p_transcript* transcript, uint8_t** commitment,
-                                  size_t* commitment_len, uint8_t** proof, size_t* proof_len);
+                                  size_t* commitment_len, uint8_t** proof, size_t* proof_len, uint64_t* indices,
+                                  size_t indices_cap);
 
-/* Commitment/Proof::from_bytes + fri::verify (fri.rs:686-859). Host only. */
+/* Commitment/Proof::from_bytes + fri::verify (fri.rs:686-859). Host only.
+ * indices (may be NULL; indices_cap >= num_queries) receives the sampled
+ * query indices fri::verify returns (fri.rs:686), filled only on Z0P_OK. */
 int z0p_fri_verify(const uint8_t* commitment, size_t commitment_len, const uint8_t* proof, size_t proof_len,
-                   const z0p_fri_params* params, z0p_transcript* transcript);
+                   const z0p_fri_params* params, z0p_transcript* transcript, uint64_t* indices, size_t indices_cap);
+
+/* fri::ProverState (fri.rs:276-294): the committed layers and their trees,
+ * kept in device memory that the state owns, so fri::open can run later
+ * (and more than once) over the same commitment. A state uses its ctx: calls
+ * on it must not overlap other calls on that ctx, and it must be freed
+ * before the ctx is destroyed. */
+typedef struct z0p_fri_state z0p_fri_state;
+/* fri::commit (fri.rs:408-433): coefficients as z0p_fri_commit_open. */
+int z0p_fri_commit(z0p_ctx* ctx, const uint32_t* coeffs, size_t n, const z0p_fri_params* params,
+                   z0p_transcript* transcript, z0p_fri_state** out);
+/* fri::commit_ext (fri.rs:444-461). */
+int z0p_fri_commit_ext(z0p_ctx* ctx, const uint32_t* coeffs4, size_t n, const z0p_fri_params* params,
+                       z0p_transcript* transcript, z0p_fri_state** out);
+/* fri::commit_evals (fri.rs:473-500). */
+int z0p_fri_commit_evals(z0p_ctx* ctx, const uint32_t* evals, size_t m, uint64_t degree_bound,
+                         const z0p_fri_params* params, z0p_transcript* transcript, z0p_fri_state** out);
+/* fri::commit_ext_evals (fri.rs:510-549). */
+int z0p_fri_commit_ext_evals(z0p_ctx* ctx, const uint32_t* evals4, size_t m, uint64_t degree_bound,
+                             const z0p_fri_params* params, z0p_transcript* transcript, z0p_fri_state** out);
+/* fri::public_commitment (fri.rs:863-870) + Commitment::to_bytes. */
+int z0p_fri_state_commitment(const z0p_fri_state* state, uint8_t** commitment, size_t* commitment_len);
+/* fri::open (fri.rs:645-666) + Proof::to_bytes: the proof-of-work, the
+ * queries and their openings, advancing transcript; indices as in
+ * z0p_fri_commit_open. */
+int z0p_fri_open(z0p_fri_state* state, z0p_transcript* transcript, uint8_t** proof, size_t* proof_len,
+                 uint64_t* indices, size_t indices_cap);
+void z0p_fri_state_free(z0p_fri_state* state);
 
 /* ---- range-check AIR (src/range.rs) on the same device path: the value
  * column V and n_bits bit columns extended on the 31-coset, one joint row

@@ -52,13 +52,19 @@ int z0p_test_fold(z0p_ctx* ctx, const uint32_t* layer, size_t len, size_t lde_le
                   const uint32_t* challenges, uint32_t* out, uint8_t* levels01);
 
 /* Proof-of-work: smallest nonce whose BLAKE3(head || u64 nonce) has at least
- * `bits` leading zero bits; head must be 47 bytes. */
+ * `bits` (<= 32) leading zero bits; head_len + 8 <= 1024 (one chunk). */
 int z0p_test_grind(z0p_ctx* ctx, const uint8_t* head, size_t head_len, uint32_t bits, uint64_t* nonce);
 
 /* Plain device copy of `bytes` (multiple of 16) between two arena buffers;
  * *ms receives its event-timed duration. The calibration kernel for the
  * FETCH_SIZE / WRITE_SIZE counters. */
 int z0p_test_copy(z0p_ctx* ctx, size_t bytes, float* ms);
+
+/* Polls of host memory that a pre-enqueued phase's gate makes before giving
+ * up (default 2^22). 0 makes every gate give up at once: a lone proof then
+ * fails with Z0P_ERR_DEVICE while every kernel still runs to completion, the
+ * way a host that never answers would end it. */
+int z0p_test_set_gate_spin(z0p_ctx* ctx, uint32_t polls);
 
 #ifdef __cplusplus
 }

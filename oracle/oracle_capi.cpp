@@ -466,17 +466,6 @@ int oracle_air_verify(const oracle_air_params* p, void* t, const uint8_t* proof,
         air::verify(to_air(p), pr, *static_cast<Transcript*>(t));
     })
 }
-int oracle_fri_commit_open(const uint32_t* coeffs, size_t n, const oracle_fri_params* p, void* t, uint8_t** cm,
-                           size_t* cm_len, uint8_t** pf, size_t* pf_len) {
-    GUARD({
-        Transcript& tr = *static_cast<Transcript*>(t);
-        fri::ProverState st = fri::commit(to_fields(coeffs, n), to_fri(p), tr);
-        fri::Commitment c = fri::public_commitment(st);
-        auto opened = fri::open(st, tr);
-        *cm = dup_bytes(c.to_bytes(), cm_len);
-        *pf = dup_bytes(opened.first.to_bytes(), pf_len);
-    })
-}
 namespace {
 std::vector<Ext4> to_ext(const uint32_t* v, size_t n) {
     std::vector<Ext4> out(n);
@@ -485,31 +474,68 @@ std::vector<Ext4> to_ext(const uint32_t* v, size_t n) {
                             Field::make(v[4 * i + 3]));
     return out;
 }
-int fri_finish(fri::ProverState st, Transcript& tr, uint8_t** cm, size_t* cm_len, uint8_t** pf, size_t* pf_len) {
+// fri::open's indices (fri.rs:645-666) as u64 when idx is not null.
+void put_idx(const std::vector<size_t>& q, uint64_t* idx) {
+    if (idx)
+        for (size_t i = 0; i < q.size(); ++i) idx[i] = q[i];
+}
+int fri_finish(fri::ProverState st, Transcript& tr, uint8_t** cm, size_t* cm_len, uint8_t** pf, size_t* pf_len,
+               uint64_t* idx) {
     fri::Commitment c = fri::public_commitment(st);
     auto opened = fri::open(st, tr);
     *cm = dup_bytes(c.to_bytes(), cm_len);
     *pf = dup_bytes(opened.first.to_bytes(), pf_len);
+    put_idx(opened.second, idx);
     return 0;
+}
+// kind -1: commit (base coefficients), 0: commit_ext (Ext4 coefficients),
+// 1: commit_evals (base LDE values), 2: commit_ext_evals (Ext4 LDE values);
+// Ext4 values are 4 words each.
+fri::ProverState commit_kind(int kind, const uint32_t* vals, size_t n, size_t degree_bound, const oracle_fri_params* p,
+                             Transcript& tr) {
+    if (kind < 0) return fri::commit(to_fields(vals, n), to_fri(p), tr);
+    if (kind == 0) return fri::commit_ext(to_ext(vals, n), to_fri(p), tr);
+    if (kind == 1) return fri::commit_evals(to_fields(vals, n), degree_bound, to_fri(p), tr);
+    return fri::commit_ext_evals_owned(to_ext(vals, n), degree_bound, to_fri(p), tr);
 }
 }  // namespace
 
-// kind 0: commit_ext (Ext4 coefficients), 1: commit_evals (base LDE values),
-// 2: commit_ext_evals (Ext4 LDE values); values are 4 words per Ext4.
-int oracle_fri_commit_open_kind(int kind, const uint32_t* vals, size_t n, size_t degree_bound,
-                                const oracle_fri_params* p, void* t, uint8_t** cm, size_t* cm_len, uint8_t** pf,
-                                size_t* pf_len) {
+int oracle_fri_commit_open(const uint32_t* coeffs, size_t n, const oracle_fri_params* p, void* t, uint8_t** cm,
+                           size_t* cm_len, uint8_t** pf, size_t* pf_len, uint64_t* idx) {
     GUARD({
         Transcript& tr = *static_cast<Transcript*>(t);
-        if (kind == 0)
-            fri_finish(fri::commit_ext(to_ext(vals, n), to_fri(p), tr), tr, cm, cm_len, pf, pf_len);
-        else if (kind == 1)
-            fri_finish(fri::commit_evals(to_fields(vals, n), degree_bound, to_fri(p), tr), tr, cm, cm_len, pf, pf_len);
-        else
-            fri_finish(fri::commit_ext_evals_owned(to_ext(vals, n), degree_bound, to_fri(p), tr), tr, cm, cm_len, pf,
-                       pf_len);
+        fri_finish(fri::commit(to_fields(coeffs, n), to_fri(p), tr), tr, cm, cm_len, pf, pf_len, idx);
     })
 }
+
+int oracle_fri_commit_open_kind(int kind, const uint32_t* vals, size_t n, size_t degree_bound,
+                                const oracle_fri_params* p, void* t, uint8_t** cm, size_t* cm_len, uint8_t** pf,
+                                size_t* pf_len, uint64_t* idx) {
+    GUARD({
+        Transcript& tr = *static_cast<Transcript*>(t);
+        fri_finish(commit_kind(kind, vals, n, degree_bound, p, tr), tr, cm, cm_len, pf, pf_len, idx);
+    })
+}
+
+// fri::commit* alone (fri.rs:408-549): an owned ProverState for a later
+// oracle_fri_state_open; kinds as commit_kind.
+int oracle_fri_commit_state(int kind, const uint32_t* vals, size_t n, size_t degree_bound, const oracle_fri_params* p,
+                            void* t, void** out) {
+    GUARD({
+        *out = new fri::ProverState(commit_kind(kind, vals, n, degree_bound, p, *static_cast<Transcript*>(t)));
+    })
+}
+int oracle_fri_state_commitment(void* st, uint8_t** cm, size_t* cm_len) {
+    GUARD({ *cm = dup_bytes(fri::public_commitment(*static_cast<fri::ProverState*>(st)).to_bytes(), cm_len); })
+}
+int oracle_fri_state_open(void* st, void* t, uint8_t** pf, size_t* pf_len, uint64_t* idx) {
+    GUARD({
+        auto opened = fri::open(*static_cast<fri::ProverState*>(st), *static_cast<Transcript*>(t));
+        *pf = dup_bytes(opened.first.to_bytes(), pf_len);
+        put_idx(opened.second, idx);
+    })
+}
+void oracle_fri_state_free(void* st) { delete static_cast<fri::ProverState*>(st); }
 // ntt.rs:394-420, coordinates interleaved as 4 words per value.
 int oracle_evaluate_on_coset_ext(const uint32_t* coeffs4, size_t n, size_t blowup, uint32_t* out4) {
     GUARD({
@@ -520,11 +546,11 @@ int oracle_evaluate_on_coset_ext(const uint32_t* coeffs4, size_t n, size_t blowu
 }
 
 int oracle_fri_verify(const uint8_t* cm, size_t cm_len, const uint8_t* pf, size_t pf_len, const oracle_fri_params* p,
-                      void* t) {
+                      void* t, uint64_t* idx) {
     GUARD({
         fri::Commitment c = fri::Commitment::from_bytes(cm, cm_len);
         fri::Proof pr = fri::Proof::from_bytes(pf, pf_len);
-        fri::verify(c, pr, to_fri(p), *static_cast<Transcript*>(t));
+        put_idx(fri::verify(c, pr, to_fri(p), *static_cast<Transcript*>(t)), idx);
     })
 }
 

@@ -14,6 +14,19 @@ def splitmix64(state):
     return state, z ^ (z >> 31)
 
 
+def splitmix_fields(n, seed):
+    """c_i = Field::from_u64(splitmix64(&mut st)) for i < n as a uint32 numpy
+    array (the same stream as n calls of splitmix64, vectorised: the state
+    after i + 1 calls is seed + (i + 1) * golden)."""
+    i = np.arange(1, n + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + i * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z % np.uint64(P)).astype(np.uint32)
+
+
 def seeded_claims(seed):
     """(a0, b0) for a synthetic Fibonacci instance: the reference's test idiom
     Field::from_u64(splitmix64(&mut st)) twice (ntt.rs:453-455)."""

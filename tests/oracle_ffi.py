@@ -10,6 +10,7 @@ LIB_PATH = ROOT / "oracle" / "build" / "liboracle.so"
 
 u8p = ctypes.POINTER(ctypes.c_uint8)
 u32p = ctypes.POINTER(ctypes.c_uint32)
+u64p = ctypes.POINTER(ctypes.c_uint64)
 
 
 class AirParams(ctypes.Structure):
@@ -203,11 +204,16 @@ def lib():
         "oracle_mask_coeffs": (ctypes.c_int, [u32p, sz, sz, sz, vp, u32p, sz, ctypes.POINTER(sz)]),
         "oracle_air_verify": (ctypes.c_int, [ctypes.POINTER(AirParams), vp, u8p, sz]),
         "oracle_fri_commit_open": (ctypes.c_int, [u32p, sz, ctypes.POINTER(FriParams), vp, ctypes.POINTER(u8p),
-                                                  ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
-        "oracle_fri_verify": (ctypes.c_int, [u8p, sz, u8p, sz, ctypes.POINTER(FriParams), vp]),
+                                                  ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz), u64p]),
+        "oracle_fri_verify": (ctypes.c_int, [u8p, sz, u8p, sz, ctypes.POINTER(FriParams), vp, u64p]),
         "oracle_fri_commit_open_kind": (ctypes.c_int, [ctypes.c_int, u32p, sz, sz, ctypes.POINTER(FriParams), vp,
                                                        ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.POINTER(u8p),
-                                                       ctypes.POINTER(sz)]),
+                                                       ctypes.POINTER(sz), u64p]),
+        "oracle_fri_commit_state": (ctypes.c_int, [ctypes.c_int, u32p, sz, sz, ctypes.POINTER(FriParams), vp,
+                                                   ctypes.POINTER(vp)]),
+        "oracle_fri_state_commitment": (ctypes.c_int, [vp, ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+        "oracle_fri_state_open": (ctypes.c_int, [vp, vp, ctypes.POINTER(u8p), ctypes.POINTER(sz), u64p]),
+        "oracle_fri_state_free": (None, [vp]),
         "oracle_evaluate_on_coset_ext": (ctypes.c_int, [u32p, sz, sz, u32p]),
         "oracle_range_params_new": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                                    ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(RangeParams)]),
@@ -477,14 +483,61 @@ def air_verify(params: AirParams, transcript: Transcript, proof: bytes) -> int:
     return lib().oracle_air_verify(ctypes.byref(params), transcript.h, _buf(proof), len(proof))
 
 
-def fri_commit_open(coeffs, params: FriParams, transcript: Transcript):
+def _idx(params):
+    n = int(params.num_queries)
+    return (ctypes.c_uint64 * max(n, 1))(), n
+
+
+def fri_commit_open(coeffs, params: FriParams, transcript: Transcript, return_indices=False):
+    """fri::commit + public_commitment + open: (commitment, proof) and, with
+    return_indices, the indices fri::open returns (fri.rs:645-666)."""
     cm, pf = u8p(), u8p()
     cn, pn = ctypes.c_size_t(), ctypes.c_size_t()
+    idx, nq = _idx(params)
     rc = lib().oracle_fri_commit_open(u32_array(coeffs), len(coeffs), ctypes.byref(params), transcript.h,
-                                      ctypes.byref(cm), ctypes.byref(cn), ctypes.byref(pf), ctypes.byref(pn))
+                                      ctypes.byref(cm), ctypes.byref(cn), ctypes.byref(pf), ctypes.byref(pn), idx)
     if rc:
         raise ValueError("oracle fri error %d" % rc)
-    return _take(cm, cn.value), _take(pf, pn.value)
+    out = (_take(cm, cn.value), _take(pf, pn.value))
+    return out + (list(idx[:nq]),) if return_indices else out
+
+
+FRI_KIND_BASE = -1
+
+
+class FriState:
+    """fri::ProverState from fri::commit* (kinds: FRI_KIND_BASE coefficients,
+    FRI_KIND_EXT, FRI_KIND_EVALS, FRI_KIND_EXT_EVALS)."""
+
+    def __init__(self, kind, values, params: FriParams, transcript: Transcript, degree_bound=0):
+        words = [int(v) for v in values]
+        count = len(words) if kind in (FRI_KIND_BASE, FRI_KIND_EVALS) else len(words) // 4
+        h = ctypes.c_void_p()
+        rc = lib().oracle_fri_commit_state(kind, u32_array(words), count, degree_bound, ctypes.byref(params),
+                                           transcript.h, ctypes.byref(h))
+        if rc:
+            raise OracleError(rc)
+        self.h, self.params = h.value, params
+
+    def commitment(self) -> bytes:
+        cm, cn = u8p(), ctypes.c_size_t()
+        rc = lib().oracle_fri_state_commitment(self.h, ctypes.byref(cm), ctypes.byref(cn))
+        if rc:
+            raise OracleError(rc)
+        return _take(cm, cn.value)
+
+    def open(self, transcript: Transcript):
+        pf, pn = u8p(), ctypes.c_size_t()
+        idx, nq = _idx(self.params)
+        rc = lib().oracle_fri_state_open(self.h, transcript.h, ctypes.byref(pf), ctypes.byref(pn), idx)
+        if rc:
+            raise OracleError(rc)
+        return _take(pf, pn.value), list(idx[:nq])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_fri_state_free(self.h)
+            self.h = None
 
 
 def fri_params_new(blowup, num_queries, final_size, grinding_bits=0, cap_height=0) -> FriParams:
@@ -499,20 +552,24 @@ def fri_params_new(blowup, num_queries, final_size, grinding_bits=0, cap_height=
 FRI_KIND_EXT, FRI_KIND_EVALS, FRI_KIND_EXT_EVALS = 0, 1, 2
 
 
-def fri_commit_open_kind(kind, values, params: FriParams, transcript: Transcript, degree_bound=0):
+def fri_commit_open_kind(kind, values, params: FriParams, transcript: Transcript, degree_bound=0,
+                         return_indices=False):
     """fri::commit_ext (kind 0, Ext4 coefficients as 4 words each),
     commit_evals (1, base LDE values) or commit_ext_evals (2, Ext4 LDE
-    values), then public_commitment + open; returns (commitment, proof)."""
+    values), then public_commitment + open; returns (commitment, proof)
+    and, with return_indices, the sampled indices."""
     words = [int(v) for v in values]
-    count = len(words) if kind == FRI_KIND_EVALS else len(words) // 4
+    count = len(words) if kind in (FRI_KIND_BASE, FRI_KIND_EVALS) else len(words) // 4
     cm, pf = u8p(), u8p()
     cn, pn = ctypes.c_size_t(), ctypes.c_size_t()
+    idx, nq = _idx(params)
     rc = lib().oracle_fri_commit_open_kind(kind, u32_array(words), count, degree_bound, ctypes.byref(params),
                                            transcript.h, ctypes.byref(cm), ctypes.byref(cn), ctypes.byref(pf),
-                                           ctypes.byref(pn))
+                                           ctypes.byref(pn), idx)
     if rc:
         raise OracleError(rc)
-    return _take(cm, cn.value), _take(pf, pn.value)
+    out = (_take(cm, cn.value), _take(pf, pn.value))
+    return out + (list(idx[:nq]),) if return_indices else out
 
 
 def evaluate_on_coset_ext(coeffs4, blowup):
@@ -524,9 +581,15 @@ def evaluate_on_coset_ext(coeffs4, blowup):
     return list(out)
 
 
-def fri_verify(commitment: bytes, proof: bytes, params: FriParams, transcript: Transcript) -> int:
-    return lib().oracle_fri_verify(_buf(commitment), len(commitment), _buf(proof), len(proof), ctypes.byref(params),
-                                   transcript.h)
+def fri_verify(commitment: bytes, proof: bytes, params: FriParams, transcript: Transcript, indices=None) -> int:
+    """Status of fri::verify; on success `indices` (a list, when given)
+    receives the indices it returns (fri.rs:686)."""
+    idx, nq = _idx(params)
+    rc = lib().oracle_fri_verify(_buf(commitment), len(commitment), _buf(proof), len(proof), ctypes.byref(params),
+                                 transcript.h, idx)
+    if rc == 0 and indices is not None:
+        indices[:] = list(idx[:nq])
+    return rc
 
 
 def stage_timers():

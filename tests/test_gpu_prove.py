@@ -93,14 +93,8 @@ def test_fri_bench_shape(ctx):
 
 def _seeded_coeffs(n, seed):
     """c_i = Field::from_u64(splitmix64(&mut st)) (the reference's test idiom)."""
-    import numpy as np
-    from helpers import splitmix64, P
-    out = np.empty(n, dtype=np.uint32)
-    st = seed
-    for i in range(n):
-        st, z_ = splitmix64(st)
-        out[i] = z_ % P
-    return out
+    from helpers import splitmix_fields
+    return splitmix_fields(n, seed)
 
 
 @pytest.mark.parametrize("logp", [12, 16])

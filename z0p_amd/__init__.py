@@ -48,6 +48,37 @@ class Transcript:
         lib().z0p_transcript_challenge_bytes(self._h, label, out)
         return bytes(out)
 
+    def challenge_bb(self, label: bytes) -> int:
+        """Transcript::challenge_bb (transcript.rs:124-129)."""
+        return lib().z0p_transcript_challenge_bb(self._h, label)
+
+    def challenge_ext(self, label: bytes):
+        """Transcript::challenge_ext (transcript.rs:136-141): four canonical words."""
+        out = (ctypes.c_uint32 * 4)()
+        lib().z0p_transcript_challenge_ext(self._h, label, out)
+        return tuple(out)
+
+    def challenge_index(self, label: bytes, domain_size: int):
+        """Transcript::challenge_index (transcript.rs:148-160): None unless the
+        domain is a nonzero power of two."""
+        out = ctypes.c_uint64()
+        rc = lib().z0p_transcript_challenge_index(self._h, label, domain_size, ctypes.byref(out))
+        if rc == 3:
+            return None
+        check(rc)
+        return out.value
+
+    def grind(self, label: bytes, difficulty_bits: int, ctx=None) -> int:
+        """Transcript::grind (transcript.rs:206-228) on the GPU of ctx."""
+        ctx = ctx or default_context()
+        out = ctypes.c_uint64()
+        check(lib().z0p_transcript_grind(ctx.handle, self._h, label, difficulty_bits, ctypes.byref(out)))
+        return out.value
+
+    def verify_grind(self, label: bytes, nonce: int, difficulty_bits: int) -> bool:
+        """Transcript::verify_grind (transcript.rs:235-246)."""
+        return lib().z0p_transcript_verify_grind(self._h, label, nonce, difficulty_bits) == 1
+
 
 class ChaCha20:
     """ChaCha20 blinding stream (src/rng.rs:82-168): ``ChaCha20(key, nonce)``

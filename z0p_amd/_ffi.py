@@ -11,6 +11,7 @@ LIB_PATH = pathlib.Path(__file__).resolve().parent / "libz0p_amd.so"
 
 u8p = ctypes.POINTER(ctypes.c_uint8)
 u32p = ctypes.POINTER(ctypes.c_uint32)
+u64p = ctypes.POINTER(ctypes.c_uint64)
 u64 = ctypes.c_uint64
 u32 = ctypes.c_uint32
 sz = ctypes.c_size_t
@@ -110,6 +111,11 @@ SIGNATURES = {
     "z0p_transcript_free": (None, [vp]),
     "z0p_transcript_absorb": (None, [vp, ctypes.c_char_p, u8p, sz]),
     "z0p_transcript_challenge_bytes": (None, [vp, ctypes.c_char_p, u8p]),
+    "z0p_transcript_challenge_bb": (u32, [vp, ctypes.c_char_p]),
+    "z0p_transcript_challenge_ext": (None, [vp, ctypes.c_char_p, u32p]),
+    "z0p_transcript_challenge_index": (ctypes.c_int, [vp, ctypes.c_char_p, u64, ctypes.POINTER(u64)]),
+    "z0p_transcript_grind": (ctypes.c_int, [vp, vp, ctypes.c_char_p, u32, ctypes.POINTER(u64)]),
+    "z0p_transcript_verify_grind": (ctypes.c_int, [vp, ctypes.c_char_p, u64, u32]),
     "z0p_air_params_new": (ctypes.c_int, [u64, u64, u64, u64, u32, u32, u32, ctypes.POINTER(AirParamsC)]),
     "z0p_air_params_for_security": (ctypes.c_int, [u64, u32, u32, u32, u32, ctypes.POINTER(AirParamsC)]),
     "z0p_fri_params_new": (ctypes.c_int, [u64, u64, u64, ctypes.POINTER(FriParamsC)]),
@@ -154,16 +160,25 @@ SIGNATURES = {
     "z0p_fold_acc_finalize": (ctypes.c_int, [vp, vp, ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
     "z0p_fold_verify": (ctypes.c_int, [ctypes.POINTER(FoldParamsC), vp, u8p, sz]),
     "z0p_fri_commit_open": (ctypes.c_int, [vp, u32p, sz, ctypes.POINTER(FriParamsC), vp, ctypes.POINTER(u8p),
-                                           ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
-    "z0p_fri_verify": (ctypes.c_int, [u8p, sz, u8p, sz, ctypes.POINTER(FriParamsC), vp]),
+                                           ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz), u64p, sz]),
+    "z0p_fri_verify": (ctypes.c_int, [u8p, sz, u8p, sz, ctypes.POINTER(FriParamsC), vp, u64p, sz]),
     "z0p_fri_commit_open_ext": (ctypes.c_int, [vp, u32p, sz, ctypes.POINTER(FriParamsC), vp, ctypes.POINTER(u8p),
-                                               ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+                                               ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz), u64p,
+                                               sz]),
     "z0p_fri_commit_open_evals": (ctypes.c_int, [vp, u32p, sz, u64, ctypes.POINTER(FriParamsC), vp,
                                                  ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.POINTER(u8p),
-                                                 ctypes.POINTER(sz)]),
+                                                 ctypes.POINTER(sz), u64p, sz]),
     "z0p_fri_commit_open_ext_evals": (ctypes.c_int, [vp, u32p, sz, u64, ctypes.POINTER(FriParamsC), vp,
                                                      ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.POINTER(u8p),
-                                                     ctypes.POINTER(sz)]),
+                                                     ctypes.POINTER(sz), u64p, sz]),
+    "z0p_fri_commit": (ctypes.c_int, [vp, u32p, sz, ctypes.POINTER(FriParamsC), vp, ctypes.POINTER(vp)]),
+    "z0p_fri_commit_ext": (ctypes.c_int, [vp, u32p, sz, ctypes.POINTER(FriParamsC), vp, ctypes.POINTER(vp)]),
+    "z0p_fri_commit_evals": (ctypes.c_int, [vp, u32p, sz, u64, ctypes.POINTER(FriParamsC), vp, ctypes.POINTER(vp)]),
+    "z0p_fri_commit_ext_evals": (ctypes.c_int, [vp, u32p, sz, u64, ctypes.POINTER(FriParamsC), vp,
+                                                ctypes.POINTER(vp)]),
+    "z0p_fri_state_commitment": (ctypes.c_int, [vp, ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+    "z0p_fri_open": (ctypes.c_int, [vp, vp, ctypes.POINTER(u8p), ctypes.POINTER(sz), u64p, sz]),
+    "z0p_fri_state_free": (None, [vp]),
     "z0p_ctx_set_hbm_budget": (ctypes.c_int, [vp, u64]),
     "z0p_ctx_set_waves": (ctypes.c_int, [vp, u32]),
     "z0p_air_verify_batch": (ctypes.c_int, [ctypes.POINTER(AirParamsC), ctypes.POINTER(vp), ctypes.POINTER(u8p),
@@ -192,6 +207,7 @@ SIGNATURES = {
     "z0p_test_fold": (ctypes.c_int, [vp, u32p, sz, sz, sz, u32p, u32p, u8p]),
     "z0p_test_grind": (ctypes.c_int, [vp, u8p, sz, u32, ctypes.POINTER(u64)]),
     "z0p_test_copy": (ctypes.c_int, [vp, sz, ctypes.POINTER(ctypes.c_float)]),
+    "z0p_test_set_gate_spin": (ctypes.c_int, [vp, u32]),
 }
 
 _lib = None

@@ -176,6 +176,44 @@ void z0p_transcript_challenge_bytes(z0p_transcript* t, const char* label, uint8_
     if (t && label && out) t->t.squeeze(label, out);
 }
 
+uint32_t z0p_transcript_challenge_bb(z0p_transcript* t, const char* label) {
+    if (!t || !label) return 0;
+    uint8_t d[32];
+    t->t.squeeze(label, d);
+    return uint32_t(z0p::load_le64(d) % z0p::P);
+}
+
+void z0p_transcript_challenge_ext(z0p_transcript* t, const char* label, uint32_t out[4]) {
+    if (!t || !label || !out) return;
+    const z0p::E4 e = t->t.challenge_ext(label);
+    for (int i = 0; i < 4; ++i) out[i] = e.v[i];
+}
+
+int z0p_transcript_challenge_index(z0p_transcript* t, const char* label, uint64_t domain_size, uint64_t* out) {
+    if (!t || !label || !out) return Z0P_ERR_INVALID_ARGUMENT;
+    if (!z0p::pow2(domain_size)) return Z0P_ERR_INVALID_DOMAIN_SIZE;
+    *out = t->t.challenge_index(label, domain_size);
+    return Z0P_OK;
+}
+
+int z0p_transcript_grind(z0p_ctx* ctx, z0p_transcript* t, const char* label, uint32_t difficulty_bits,
+                         uint64_t* nonce) {
+    if (!ctx || !t || !label || !nonce) return Z0P_ERR_INVALID_ARGUMENT;
+    return guarded([&] { *nonce = z0p::transcript_grind(*ctx->c, t->t, label, difficulty_bits); });
+}
+
+int z0p_transcript_verify_grind(z0p_transcript* t, const char* label, uint64_t nonce, uint32_t difficulty_bits) {
+    if (!t || !label || difficulty_bits > 256) return 0;
+    const std::vector<uint8_t> head = t->t.pow_head(label);
+    uint8_t d[32];
+    z0p::pow_digest(head, nonce, d);
+    if (z0p::leading_zero_bits(d) < difficulty_bits) return 0;
+    uint8_t nb[8];
+    z0p::store_le64(nb, nonce);
+    t->t.absorb(label, nb, 8);
+    return 1;
+}
+
 int z0p_air_params_new(uint64_t n_steps, uint64_t blowup, uint64_t num_queries, uint64_t final_size, uint32_t a0,
                        uint32_t b0, uint32_t b_last, z0p_air_params* out) {
     if (!out) return Z0P_ERR_INVALID_ARGUMENT;
@@ -613,59 +651,141 @@ int z0p_air_verify_batch(const z0p_air_params* params, z0p_transcript* const* tr
 }
 
 namespace {
+bool indices_ok(const uint64_t* indices, size_t cap, const z0p_fri_params* params) {
+    return !indices || cap >= params->num_queries;
+}
+
+void put_indices(const std::vector<uint64_t>& q, uint64_t* indices) {
+    if (indices) std::copy(q.begin(), q.end(), indices);
+}
+
 int fri_entry(z0p_ctx* ctx, const uint32_t* src, size_t count, int kind, uint64_t degree_bound,
               const z0p_fri_params* params, z0p_transcript* transcript, uint8_t** commitment, size_t* commitment_len,
-              uint8_t** proof, size_t* proof_len) {
+              uint8_t** proof, size_t* proof_len, uint64_t* indices, size_t indices_cap) {
     if (!ctx || !params || !transcript || !commitment || !commitment_len || !proof || !proof_len ||
-        (!src && count))
+        (!src && count) || !indices_ok(indices, indices_cap, params))
         return Z0P_ERR_INVALID_ARGUMENT;
     int rc = Z0P_OK;
     int g = guarded([&] {
         z0p::Bytes c, p;
-        z0p::fri_commit_open(*ctx->c, src, count, kind, degree_bound, *params, transcript->t, c, p);
+        std::vector<uint64_t> q;
+        z0p::fri_commit_open(*ctx->c, src, count, kind, degree_bound, *params, transcript->t, c, p, q);
         rc = emit(c, commitment, commitment_len);
         if (rc == Z0P_OK) {
             rc = emit(p, proof, proof_len);
             if (rc != Z0P_OK) std::free(*commitment);
         }
+        if (rc == Z0P_OK) put_indices(q, indices);
     });
     return g != Z0P_OK ? g : rc;
+}
+
+int fri_commit_entry(z0p_ctx* ctx, const uint32_t* src, size_t count, int kind, uint64_t degree_bound,
+                     const z0p_fri_params* params, z0p_transcript* transcript, z0p_fri_state** out) {
+    if (!ctx || !params || !transcript || !out || (!src && count)) return Z0P_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto* h = new (std::nothrow) z0p_fri_state;
+    if (!h) return Z0P_ERR_OUT_OF_MEMORY;
+    int g = guarded([&] { h->s = z0p::fri_commit(*ctx->c, src, count, kind, degree_bound, *params, transcript->t); });
+    if (g != Z0P_OK) {
+        delete h;
+        return g;
+    }
+    *out = h;
+    return Z0P_OK;
 }
 }  // namespace
 
 int z0p_fri_commit_open(z0p_ctx* ctx, const uint32_t* coeffs, size_t n, const z0p_fri_params* params,
                         z0p_transcript* transcript, uint8_t** commitment, size_t* commitment_len, uint8_t** proof,
-                        size_t* proof_len) {
+                        size_t* proof_len, uint64_t* indices, size_t indices_cap) {
     return fri_entry(ctx, coeffs, n, z0p::FRI_COEFFS, 0, params, transcript, commitment, commitment_len, proof,
-                     proof_len);
+                     proof_len, indices, indices_cap);
 }
 
 int z0p_fri_commit_open_ext(z0p_ctx* ctx, const uint32_t* coeffs4, size_t n, const z0p_fri_params* params,
                             z0p_transcript* transcript, uint8_t** commitment, size_t* commitment_len, uint8_t** proof,
-                            size_t* proof_len) {
+                            size_t* proof_len, uint64_t* indices, size_t indices_cap) {
     return fri_entry(ctx, coeffs4, n, z0p::FRI_COEFFS_EXT, 0, params, transcript, commitment, commitment_len, proof,
-                     proof_len);
+                     proof_len, indices, indices_cap);
 }
 
 int z0p_fri_commit_open_evals(z0p_ctx* ctx, const uint32_t* evals, size_t m, uint64_t degree_bound,
                               const z0p_fri_params* params, z0p_transcript* transcript, uint8_t** commitment,
-                              size_t* commitment_len, uint8_t** proof, size_t* proof_len) {
+                              size_t* commitment_len, uint8_t** proof, size_t* proof_len, uint64_t* indices,
+                              size_t indices_cap) {
     return fri_entry(ctx, evals, m, z0p::FRI_EVALS, degree_bound, params, transcript, commitment, commitment_len,
-                     proof, proof_len);
+                     proof, proof_len, indices, indices_cap);
 }
 
 int z0p_fri_commit_open_ext_evals(z0p_ctx* ctx, const uint32_t* evals4, size_t m, uint64_t degree_bound,
                                   const z0p_fri_params* params, z0p_transcript* transcript, uint8_t** commitment,
-                                  size_t* commitment_len, uint8_t** proof, size_t* proof_len) {
+                                  size_t* commitment_len, uint8_t** proof, size_t* proof_len, uint64_t* indices,
+                                  size_t indices_cap) {
     return fri_entry(ctx, evals4, m, z0p::FRI_EVALS_EXT, degree_bound, params, transcript, commitment, commitment_len,
-                     proof, proof_len);
+                     proof, proof_len, indices, indices_cap);
 }
 
 int z0p_fri_verify(const uint8_t* commitment, size_t commitment_len, const uint8_t* proof, size_t proof_len,
-                   const z0p_fri_params* params, z0p_transcript* transcript) {
-    if (!params || !transcript || (!commitment && commitment_len) || (!proof && proof_len))
+                   const z0p_fri_params* params, z0p_transcript* transcript, uint64_t* indices, size_t indices_cap) {
+    if (!params || !transcript || (!commitment && commitment_len) || (!proof && proof_len) ||
+        !indices_ok(indices, indices_cap, params))
         return Z0P_ERR_INVALID_ARGUMENT;
-    return guarded([&] { z0p::fri_verify(commitment, commitment_len, proof, proof_len, *params, transcript->t); });
+    return guarded([&] {
+        std::vector<uint64_t> q = z0p::fri_verify(commitment, commitment_len, proof, proof_len, *params, transcript->t);
+        put_indices(q, indices);
+    });
+}
+
+int z0p_fri_commit(z0p_ctx* ctx, const uint32_t* coeffs, size_t n, const z0p_fri_params* params,
+                   z0p_transcript* transcript, z0p_fri_state** out) {
+    return fri_commit_entry(ctx, coeffs, n, z0p::FRI_COEFFS, 0, params, transcript, out);
+}
+
+int z0p_fri_commit_ext(z0p_ctx* ctx, const uint32_t* coeffs4, size_t n, const z0p_fri_params* params,
+                       z0p_transcript* transcript, z0p_fri_state** out) {
+    return fri_commit_entry(ctx, coeffs4, n, z0p::FRI_COEFFS_EXT, 0, params, transcript, out);
+}
+
+int z0p_fri_commit_evals(z0p_ctx* ctx, const uint32_t* evals, size_t m, uint64_t degree_bound,
+                         const z0p_fri_params* params, z0p_transcript* transcript, z0p_fri_state** out) {
+    return fri_commit_entry(ctx, evals, m, z0p::FRI_EVALS, degree_bound, params, transcript, out);
+}
+
+int z0p_fri_commit_ext_evals(z0p_ctx* ctx, const uint32_t* evals4, size_t m, uint64_t degree_bound,
+                             const z0p_fri_params* params, z0p_transcript* transcript, z0p_fri_state** out) {
+    return fri_commit_entry(ctx, evals4, m, z0p::FRI_EVALS_EXT, degree_bound, params, transcript, out);
+}
+
+int z0p_fri_state_commitment(const z0p_fri_state* state, uint8_t** commitment, size_t* commitment_len) {
+    if (!state || !state->s || !commitment || !commitment_len) return Z0P_ERR_INVALID_ARGUMENT;
+    const z0p::Bytes& b = z0p::fri_state_commitment(*state->s);
+    auto* p = static_cast<uint8_t*>(std::malloc(std::max<size_t>(b.size(), 1)));
+    if (!p) return Z0P_ERR_OUT_OF_MEMORY;
+    std::memcpy(p, b.data(), b.size());
+    *commitment = p;
+    *commitment_len = b.size();
+    return Z0P_OK;
+}
+
+int z0p_fri_open(z0p_fri_state* state, z0p_transcript* transcript, uint8_t** proof, size_t* proof_len,
+                 uint64_t* indices, size_t indices_cap) {
+    if (!state || !state->s || !transcript || !proof || !proof_len) return Z0P_ERR_INVALID_ARGUMENT;
+    if (indices && indices_cap < z0p::fri_state_queries(*state->s)) return Z0P_ERR_INVALID_ARGUMENT;
+    int rc = Z0P_OK;
+    int g = guarded([&] {
+        std::vector<uint64_t> q;
+        z0p::Bytes bytes = z0p::fri_open(*state->s, transcript->t, q);
+        rc = emit(bytes, proof, proof_len);
+        if (rc == Z0P_OK) put_indices(q, indices);
+    });
+    return g != Z0P_OK ? g : rc;
+}
+
+void z0p_fri_state_free(z0p_fri_state* state) {
+    if (!state) return;
+    z0p::fri_state_free(state->s);
+    delete state;
 }
 
 }  // extern "C"

@@ -289,6 +289,13 @@ struct Ctx {
     uint8_t* mailbox_dev = nullptr;
     uint64_t gate = 0;
     bool pipeline_single = true;  // pre-enqueue the phases of a lone proof
+    // Polls of host memory a pre-enqueued phase's gate makes before it
+    // gives up (a few seconds); the gate then reports a timeout.
+    uint32_t gate_spin = 1u << 22;
+
+    // Device buffer of z0p_transcript_grind: the grind head and the running
+    // minimum nonce (allocated on its first call).
+    uint8_t* grind_buf = nullptr;
 
     // Scratch arena for the kernel-level diagnostic hooks.
     uint8_t* scratch = nullptr;

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 
 #include "device_common.hpp"
 #include "profiler.hpp"
@@ -1464,13 +1465,51 @@ __global__ __launch_bounds__(SMALL_WG) void k_small_commit(SmallCommitJob inl, c
 }
 
 // ---------------------------------------------------------------------------
-// Proof-of-work scan. The 55-byte preimage head||nonce is one BLAKE3 block
-// (START|END|ROOT). A job's workgroups walk its chunks in increasing order
-// and stop once a passing nonce below their next chunk is known, so the
-// reported minimum is the exact smallest passing nonce in [0, limit): every
-// chunk at or below it is scanned.
+// Proof-of-work scan over head || u64 nonce (transcript.rs:182-351). A job's
+// workgroups walk its chunks of [start, limit) in increasing order and stop
+// once a passing nonce below their next chunk is known, so the reported
+// minimum is the exact smallest passing nonce in the range: every chunk at
+// or below it is scanned.
 constexpr int GRIND_PER_THREAD = 4;
-constexpr uint32_t POW_HEAD_LEN = 47;  // 32-byte history digest + u64 len + "fri-pow"
+
+// Nonce scan with the nonce starting in tail word W (byte offset 4W + s/8);
+// the word is a template argument so the message stays in registers.
+template <int W>
+__device__ __forceinline__ void grind_scan(const GrindJob& J, uint32_t local) {
+    uint32_t tw[32], cv[8];
+#pragma unroll
+    for (int w = 0; w < 32; ++w) tw[w] = gc(J.head->tail)[w];
+#pragma unroll
+    for (int w = 0; w < 8; ++w) cv[w] = gc(J.head->cv)[w];
+    const uint32_t s = 8u * (J.nonce_off & 3u);
+    const bool two = J.len1 != 0;
+    const uint64_t chunk = uint64_t(WG) * GRIND_PER_THREAD;
+    const uint64_t stride = uint64_t(J.nblocks) * chunk;
+    for (uint64_t start = J.start + uint64_t(local) * chunk; start < J.limit; start += stride) {
+        if (__hip_atomic_load(J.result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < start) return;
+        for (int k = 0; k < GRIND_PER_THREAD; ++k) {
+            const uint64_t nonce = start + uint64_t(k) * WG + threadIdx.x;
+            if (nonce >= J.limit) break;
+            const uint32_t lo = uint32_t(nonce), hi = uint32_t(nonce >> 32);
+            // The nonce's 8 LE bytes spread over words W..W+2.
+            const uint32_t c0 = lo << s;
+            const uint32_t c1 = s ? (lo >> (32u - s)) | (hi << s) : hi;
+            const uint32_t c2 = s ? hi >> (32u - s) : 0u;
+            uint32_t m[32];
+#pragma unroll
+            for (int w = 0; w < 32; ++w) m[w] = tw[w];
+            m[W] |= c0;
+            m[W + 1] |= c1;
+            if (W + 2 < 32) m[W + 2] |= c2;
+            uint32_t d[8];
+            compress8(cv, m, 0, 0, J.len0, J.flags0, d);
+            if (W >= 14 && two) compress8(d, m + 16, 0, 0, J.len1, b3::CHUNK_END | b3::ROOT, d);
+            const uint32_t be = __builtin_bswap32(d[0]);
+            const uint32_t lz = be == 0 ? 32u : uint32_t(__builtin_clz(be));
+            if (lz >= J.bits) atomicMin(J.result, (unsigned long long)nonce);
+        }
+    }
+}
 
 __global__ void k_grind_init(GrindJob inl, const GrindJob* tab, uint32_t njobs) {
     uint32_t i = blockIdx.x * WG + threadIdx.x;
@@ -1484,31 +1523,24 @@ __global__ void k_grind_init(GrindJob inl, const GrindJob* tab, uint32_t njobs) 
 __global__ __launch_bounds__(WG) void k_grind(GrindJob inl, const GrindJob* tab, uint32_t njobs) {
     uint32_t local;
     const GrindJob J = job_of(inl, tab, njobs, local);
-    uint32_t hw[16];
-#pragma unroll
-    for (int w = 0; w < 16; ++w) hw[w] = gc(J.hw)[w];
-    const uint64_t chunk = uint64_t(WG) * GRIND_PER_THREAD;
-    const uint64_t stride = uint64_t(J.nblocks) * chunk;
-    for (uint64_t start = uint64_t(local) * chunk; start < J.limit; start += stride) {
-        if (__hip_atomic_load(J.result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < start) return;
-        for (int k = 0; k < GRIND_PER_THREAD; ++k) {
-            uint64_t nonce = start + uint64_t(k) * WG + threadIdx.x;
-            if (nonce >= J.limit) break;
-            uint32_t m[16];
-#pragma unroll
-            for (int w = 0; w < 16; ++w) m[w] = hw[w];
-            // nonce bytes occupy preimage bytes 47..54: word 11 byte 3, words 12-13
-            uint32_t lo = uint32_t(nonce), hi = uint32_t(nonce >> 32);
-            m[11] |= lo << 24;
-            m[12] = (lo >> 8) | (hi << 24);
-            m[13] = hi >> 8;
-            uint32_t iv[8], d[8];
-            b3_iv(iv);
-            compress8(iv, m, 0, 0, POW_HEAD_LEN + 8, b3::CHUNK_START | b3::CHUNK_END | b3::ROOT, d);
-            uint32_t be = __builtin_bswap32(d[0]);
-            uint32_t lz = be == 0 ? 32u : uint32_t(__builtin_clz(be));
-            if (lz >= J.bits) atomicMin(J.result, (unsigned long long)nonce);
-        }
+    switch (J.nonce_off >> 2) {
+        case 0: grind_scan<0>(J, local); break;
+        case 1: grind_scan<1>(J, local); break;
+        case 2: grind_scan<2>(J, local); break;
+        case 3: grind_scan<3>(J, local); break;
+        case 4: grind_scan<4>(J, local); break;
+        case 5: grind_scan<5>(J, local); break;
+        case 6: grind_scan<6>(J, local); break;
+        case 7: grind_scan<7>(J, local); break;
+        case 8: grind_scan<8>(J, local); break;
+        case 9: grind_scan<9>(J, local); break;
+        case 10: grind_scan<10>(J, local); break;
+        case 11: grind_scan<11>(J, local); break;
+        case 12: grind_scan<12>(J, local); break;
+        case 13: grind_scan<13>(J, local); break;
+        case 14: grind_scan<14>(J, local); break;
+        case 15: grind_scan<15>(J, local); break;
+        default: break;
     }
 }
 
@@ -1561,8 +1593,11 @@ __global__ __launch_bounds__(COLLECT_WG_MAX) void k_collect(CollectItem inl, con
         if (!open) __hip_atomic_store(gate.timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     __syncthreads();
+    // A gate that gave up hands zeros to the next phase instead of whatever
+    // the mailbox holds: the openings then read live = 0 and skip, and the
+    // other phases compute in-bounds garbage that the host discards.
     for (uint32_t k = threadIdx.x; k < gate.words; k += blockDim.x)
-        gate.dst[k] = __hip_atomic_load(gate.src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        gate.dst[k] = open ? __hip_atomic_load(gate.src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) : 0u;
 }
 
 // Plain streaming copy: the calibration reference for FETCH/WRITE_SIZE.
@@ -1859,6 +1894,28 @@ void launch_grind(const Launch& L, std::vector<GrindJob> jobs) {
     }
     ProfScope ps(L.prof, "grind", L.stream);
     hipLaunchKernelGGL(k_grind, dim3(g), dim3(WG), 0, L.stream, jobs[0], table, n);
+}
+
+bool grind_prepare(const uint8_t* head, size_t head_len, GrindHead& h, GrindJob& j) {
+    const size_t total = head_len + 8;
+    if (total > GRIND_MAX_PREIMAGE) return false;
+    const size_t first = head_len / 64;            // block the nonce starts in
+    const size_t blocks = (total + 63) / 64;       // blocks of head || nonce
+    b3_iv(h.cv);
+    for (size_t b = 0; b < first; ++b) {
+        uint32_t m[16];
+        for (int k = 0; k < 16; ++k) m[k] = load_le32(head + 64 * b + 4 * k);
+        compress8(h.cv, m, 0, 0, 64, b == 0 ? b3::CHUNK_START : 0u, h.cv);
+    }
+    uint8_t tail[128] = {0};
+    std::memcpy(tail, head + 64 * first, head_len - 64 * first);
+    for (int k = 0; k < 32; ++k) h.tail[k] = load_le32(tail + 4 * k);
+    const bool two = blocks - first == 2;
+    j.nonce_off = uint32_t(head_len - 64 * first);
+    j.len0 = two ? 64u : uint32_t(total - 64 * first);
+    j.flags0 = (first == 0 ? b3::CHUNK_START : 0u) | (two ? 0u : b3::CHUNK_END | b3::ROOT);
+    j.len1 = two ? uint32_t(total - 64 * (blocks - 1)) : 0u;
+    return true;
 }
 
 void launch_bitrev(const Launch& L, std::vector<BitrevJob> jobs) {

@@ -315,14 +315,31 @@ struct SmallCommitJob {
 };
 void launch_small_commit(const Launch& L, std::vector<SmallCommitJob> jobs);
 
+// Proof-of-work preimage head || u64 nonce (one BLAKE3 chunk, at most 1024
+// bytes): the blocks before the one the nonce starts in do not depend on the
+// nonce, so the host compresses them into `cv`; the device compresses the
+// one or two tail blocks per nonce.
+struct GrindHead {
+    uint32_t cv[8];     // chaining value entering the first tail block
+    uint32_t tail[32];  // tail blocks as LE words, the nonce bytes zero
+};
+constexpr uint32_t GRIND_MAX_PREIMAGE = 1024;
+
 struct GrindJob {
-    const uint32_t* hw;  // device memory: 47-byte head as 16 LE words, zero-padded
+    const GrindHead* head;  // device memory
     uint32_t bits;
     uint32_t nblocks;
-    uint64_t limit;
+    uint64_t start, limit;  // nonces scanned: [start, limit)
     unsigned long long* result;
+    uint32_t nonce_off;     // byte offset of the nonce in the tail (< 64)
+    uint32_t len0, flags0;  // first tail block: byte length and flags
+    uint32_t len1;          // second tail block's byte length (0: one tail block)
     uint32_t block_begin;
 };
+
+// Head (BLAKE3(history) || u64 len || label) of `head_len` bytes: fills `h`
+// and the job's tail fields; false when head || nonce exceeds one chunk.
+bool grind_prepare(const uint8_t* head, size_t head_len, GrindHead& h, GrindJob& j);
 
 // ---- launchers ---------------------------------------------------------------
 void launch_fib_coeffs(const Launch& L, std::vector<FibCoeffJob> jobs);

@@ -168,6 +168,7 @@ Ctx::~Ctx() {
     if (flag_host) (void)hipHostFree(flag_host);
     if (mailbox_host) (void)hipHostFree(mailbox_host);
     if (scratch) (void)hipFree(scratch);
+    if (grind_buf) (void)hipFree(grind_buf);
     for (auto& kv : spare) (void)hipFree(kv.second);
     if (tw4096) (void)hipFree(tw4096);
     if (tw4096_inv) (void)hipFree(tw4096_inv);
@@ -509,9 +510,15 @@ struct Proof {
     TreeGeom tg;                     // AIR trace tree
     std::vector<uint64_t> lens;      // FRI layer d length, d = 0..folds
     std::vector<TreeGeom> geoms;     // FRI layer d tree, d < folds
-    // device memory: one pool block
+    // device memory: one pool block, or a block the proof owns (own_mem:
+    // a FRI prover state that outlives its commit call)
     uint8_t* mem = nullptr;
     size_t mem_bytes = 0;
+    bool own_mem = false;
+    // fri::commit alone (fri.rs:408-549): stop once "fri-final" is absorbed,
+    // keeping the layers and trees for a later fri::open
+    bool split_commit = false;
+    bool resume = false;  // fri::open of a committed state: admitted as it is
     size_t stage_need = 0;  // words of staging any one iteration may use
     uint32_t *col = nullptr, *coeff = nullptr, *scratch = nullptr, *A = nullptr, *Bc = nullptr, *trace_tree = nullptr;
     uint32_t *colB = nullptr, *coeffB = nullptr, *mask_dev = nullptr;
@@ -528,7 +535,8 @@ struct Proof {
     std::vector<std::vector<Digest>> caps;
     std::vector<E4> final_values;
     FoldConsts fold{};
-    uint32_t pow_hw[16] = {0};
+    GrindHead pow{};     // proof-of-work head (device layout)
+    GrindJob pow_job{};  // its tail fields: fixed by the label length
     int window = 0;
     uint64_t nonce = 0;
     std::vector<uint64_t> queries;
@@ -1214,9 +1222,19 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
         }
         case PH_GRIND: {
             GrindJob g{};
-            g.hw = static_cast<const uint32_t*>(cst ? cst : c.ring.push(P.pow_hw, sizeof(P.pow_hw)));
+            if (P.pow_job.len0 == 0) {
+                // The tail layout depends only on the head's length (47 bytes
+                // for "fri-pow"), so a phase enqueued ahead knows it already.
+                const uint8_t zeros[47] = {0};
+                GrindHead unused;
+                require(grind_prepare(zeros, sizeof(zeros), unused, P.pow_job), Z0P_ERR_DEVICE);
+            }
+            g = P.pow_job;
+            g.head = static_cast<const GrindHead*>(cst ? cst : c.ring.push(&P.pow, sizeof(P.pow)));
             g.bits = P.fp.grinding_bits;
             g.nblocks = uint32_t(std::min(2048, std::max(16, c.sched.grind_blocks)));  // one proof; batches re-spread
+            // Window w scans [2^(20+4w), 2^(24+4w)) after the windows before it.
+            g.start = P.window == 0 ? 0 : uint64_t(1) << std::min<uint64_t>(63, 20 + 4 * uint64_t(P.window));
             g.limit = uint64_t(1) << std::min<uint64_t>(63, 24 + 4 * uint64_t(P.window));
             g.result = P.grind_res;
             ch.push_back(g);
@@ -1283,6 +1301,9 @@ void commit_layer(Proof& P, const uint32_t* w, size_t d) {
     P.phase = PH_FOLD;
 }
 
+void begin_open(Proof& P);
+void write_fri_commitment_body(Writer& w, const Proof& P);
+
 void sample_queries(Proof& P) {
     P.queries.clear();
     for (uint64_t q = 0; q < P.fp.num_queries; ++q) P.queries.push_back(P.tr->challenge_index("fri-query", P.m));
@@ -1300,12 +1321,25 @@ void finish_final(Proof& P, const uint32_t* w) {
     for (uint64_t i = 0; i < len; ++i)
         for (int c = 0; c < 4; ++c) store_le32(fin.data() + 16 * i + 4 * c, P.final_values[i].v[c]);
     P.tr->absorb("fri-final", fin);
+    if (P.split_commit) {
+        // fri::commit ends here (fri.rs:629-632); fri::open resumes later.
+        Writer wc;
+        wc.byte(WIRE_VERSION);
+        write_fri_commitment_body(wc, P);
+        P.commitment = wc.finish();
+        P.phase = PH_DONE;
+        return;
+    }
+    begin_open(P);
+}
+
+// fri::open (fri.rs:645-666): the proof-of-work on the device when
+// grinding, then the query indices.
+void begin_open(Proof& P) {
     if (P.fp.grinding_bits > 0) {
         // Proof-of-work (transcript.rs:206-228) on the device.
         std::vector<uint8_t> head = P.tr->pow_head("fri-pow");
-        require(head.size() == 47, Z0P_ERR_DEVICE);
-        std::memset(P.pow_hw, 0, sizeof(P.pow_hw));
-        for (size_t i = 0; i < head.size(); ++i) P.pow_hw[i >> 2] |= uint32_t(head[i]) << (8 * (i & 3));
+        require(head.size() == 47 && grind_prepare(head.data(), head.size(), P.pow, P.pow_job), Z0P_ERR_DEVICE);
         P.window = 0;
         P.phase = PH_GRIND;
     } else {
@@ -1519,7 +1553,7 @@ Engine::~Engine() {
     // Proof memory goes back to the pool once no launch can still use it.
     if (!flight_.empty()) (void)hipStreamSynchronize(c_.stream);
     for (Proof* p : active_)
-        if (p->mem) c_.pool.release(p->mem, p->mem_bytes);
+        if (p->mem && !p->own_mem) c_.pool.release(p->mem, p->mem_bytes);
 }
 
 void Engine::submit(Proof* p) {
@@ -1528,8 +1562,10 @@ void Engine::submit(Proof* p) {
 }
 
 void Engine::finish(Proof* p, int status) {
-    if (p->mem) c_.pool.release(p->mem, p->mem_bytes);
-    p->mem = nullptr;
+    if (p->mem && !p->own_mem) {
+        c_.pool.release(p->mem, p->mem_bytes);
+        p->mem = nullptr;
+    }
     p->status = status;
     p->phase = PH_DONE;
     p->t_done = now_ms();
@@ -1539,6 +1575,15 @@ void Engine::finish(Proof* p, int status) {
 void Engine::admit() {
     while (!queue_.empty() && active_.size() < max_active_) {
         Proof* p = queue_.front();
+        if (p->own_mem) {
+            // Memory of its own: no pool block and no budget share. A
+            // resumed state keeps its device data and transcript position.
+            queue_.pop_front();
+            p->t_admit = now_ms();
+            if (!p->resume) start(c_, *p);
+            active_.push_back(p);
+            continue;
+        }
         uint8_t* mem = nullptr;
         if (budget_ && c_.pool.in_use() != 0 && c_.pool.in_use() + p->mem_bytes > budget_) break;
         if (!c_.pool.alloc(p->mem_bytes, mem)) {
@@ -1719,17 +1764,17 @@ void Engine::pipeline(Proof& P) {
     std::vector<Ph> phs{{PH_FRONT, 0, 0}};
     if (P.air || P.row) phs.push_back({PH_COMPOSE, 0, 0});
     for (size_t d = 0; d < P.folds; ++d) phs.push_back({PH_FOLD, d, 0});
-    if (P.fp.grinding_bits > 0) phs.push_back({PH_GRIND, 0, 0});
+    if (P.fp.grinding_bits > 0 && !P.split_commit) phs.push_back({PH_GRIND, 0, 0});
     constexpr size_t SLOT = Ctx::MAILBOX_SLOT;
     // The openings too, when their query list fits the remaining slots; they
     // run only if the first proof-of-work window finds the nonce.
     const size_t open_slots = (8 + 8 * size_t(P.fp.num_queries) + SLOT - 1) / SLOT;
-    const bool with_open = phs.size() + open_slots <= Ctx::MAILBOX_SLOTS;
+    const bool with_open = !P.split_commit && phs.size() + open_slots <= Ctx::MAILBOX_SLOTS;
     if (with_open) phs.push_back({PH_OPEN, 0, 0});
     require(phs.size() <= Ctx::MAILBOX_SLOTS, Z0P_ERR_DEVICE);
     static_assert(sizeof(ComposeConsts) <= SLOT && sizeof(RangeConsts) <= SLOT && sizeof(TableConsts) <= SLOT &&
                       sizeof(FoldConsts) <= SLOT &&
-                      64 <= SLOT,
+                      sizeof(GrindHead) <= SLOT,
                   "mailbox slot");
     c_.reserve_stage(P.stage_need * phs.size());
     std::vector<std::vector<Step>> chains(phs.size());
@@ -1763,7 +1808,7 @@ void Engine::pipeline(Proof& P) {
                 gate.src = reinterpret_cast<const uint32_t*>(c_.mailbox_dev + Ctx::MAILBOX_HEAD + (k + 1) * SLOT);
                 gate.dst = reinterpret_cast<uint32_t*>(P.consts + (k + 1) * SLOT);
                 // only the next phase's constants cross PCIe
-                size_t bytes = 64;  // the proof-of-work head
+                size_t bytes = sizeof(GrindHead);  // the proof-of-work head
                 switch (phs[k + 1].ph) {
                     case PH_COMPOSE:
                         bytes = P.row == ROW_RANGE ? sizeof(RangeConsts)
@@ -1775,7 +1820,7 @@ void Engine::pipeline(Proof& P) {
                     default: break;
                 }
                 gate.words = uint32_t((bytes + 3) / 4);
-                gate.spin_limit = 1u << 22;
+                gate.spin_limit = c_.gate_spin;
                 gate.timeout = reinterpret_cast<uint32_t*>(c_.mailbox_dev + 8);
             }
             const std::vector<const std::vector<Step>*> one{&chains[k]};
@@ -1804,12 +1849,14 @@ void Engine::pipeline(Proof& P) {
                 if (nx.ph == PH_OPEN) {
                     // Nonce found: hand over the queries; otherwise cancel the
                     // openings and continue the search stepwise.
+                    // The query list lands before `live` is published, so a
+                    // slot that reads live = 1 always holds this proof's list.
                     const uint64_t live = P.phase == PH_OPEN ? 1 : 0;
-                    std::memcpy(slot, &live, 8);
                     if (live) {
                         require(P.queries.size() == P.fp.num_queries, Z0P_ERR_DEVICE);
                         std::memcpy(slot + 8, P.queries.data(), 8 * P.queries.size());
                     }
+                    __atomic_store_n(reinterpret_cast<uint64_t*>(slot), live, __ATOMIC_RELEASE);
                     cancelled = !live;
                 } else {
                     require(P.phase == nx.ph && (nx.ph != PH_FOLD || P.depth == nx.depth), Z0P_ERR_DEVICE);
@@ -1817,7 +1864,7 @@ void Engine::pipeline(Proof& P) {
                     else if (nx.ph == PH_COMPOSE && P.row == ROW_TABLE) std::memcpy(slot, &P.tc, sizeof(P.tc));
                     else if (nx.ph == PH_COMPOSE) std::memcpy(slot, &P.cc, sizeof(P.cc));
                     else if (nx.ph == PH_FOLD) std::memcpy(slot, &P.fold, sizeof(P.fold));
-                    else std::memcpy(slot, P.pow_hw, sizeof(P.pow_hw));
+                    else std::memcpy(slot, &P.pow, sizeof(P.pow));
                 }
                 __atomic_store_n(gate, g0 + k + 1, __ATOMIC_RELEASE);
             }
@@ -1939,6 +1986,7 @@ Proof* proof_new_fold_fri(Ctx& c, const FoldStepDesc* steps, uint32_t nsteps, ui
 const std::vector<uint64_t>& proof_queries(const Proof* p) { return p->queries; }
 void proof_free(Proof* p) { delete p; }
 int proof_status(const Proof* p) { return p->status; }
+bool proof_done(const Proof* p) { return p->phase == PH_DONE && p->status == Z0P_OK; }
 Bytes& proof_bytes(Proof* p) { return p->proof; }
 Bytes& proof_commitment(Proof* p) { return p->commitment; }
 size_t proof_mem_bytes(const Proof* p) { return p->mem_bytes; }
@@ -2002,7 +2050,8 @@ Bytes table_prove_zk(Ctx& ctx, const z0p_table_params& p, const uint32_t* table,
 }
 
 void fri_commit_open(Ctx& ctx, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,
-                     const z0p_fri_params& fp, Transcript& tr, Bytes& commitment, Bytes& proof) {
+                     const z0p_fri_params& fp, Transcript& tr, Bytes& commitment, Bytes& proof,
+                     std::vector<uint64_t>& indices) {
     std::unique_ptr<Proof> P(proof_new_fri(ctx, src, count, kind, degree_bound, fp, &tr));
     Engine e(ctx, 0, 1);
     e.set_pipelined(true);
@@ -2011,6 +2060,109 @@ void fri_commit_open(Ctx& ctx, const uint32_t* src, uint64_t count, int kind, ui
     if (P->status != Z0P_OK) throw Status(P->status);
     commitment = std::move(P->commitment);
     proof = std::move(P->proof);
+    indices = P->queries;
+}
+
+// ---------------------------------------------------------------------------
+// fri::commit / fri::open as two calls (fri.rs:408-666): the state keeps its
+// layers and trees in device memory of its own between them.
+
+FriState* fri_commit(Ctx& ctx, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,
+                     const z0p_fri_params& fp, Transcript& tr) {
+    std::unique_ptr<Proof> P(proof_new_fri(ctx, src, count, kind, degree_bound, fp, &tr));
+    P->split_commit = true;
+    hip_check(hipMalloc(&P->mem, P->mem_bytes));
+    P->own_mem = true;
+    try {
+        run_proof(ctx, P.get());
+    } catch (...) {
+        (void)hipStreamSynchronize(ctx.stream);
+        (void)hipFree(P->mem);
+        P->mem = nullptr;
+        throw;
+    }
+    P->tr = nullptr;
+    auto* s = new FriState;
+    s->ctx = &ctx;
+    s->proof = P.release();
+    return s;
+}
+
+const Bytes& fri_state_commitment(const FriState& s) { return s.proof->commitment; }
+uint64_t fri_state_queries(const FriState& s) { return s.proof->fp.num_queries; }
+
+Bytes fri_open(FriState& s, Transcript& tr, std::vector<uint64_t>& indices) {
+    Proof& P = *s.proof;
+    P.tr = &tr;
+    P.resume = true;
+    P.status = Z0P_OK;
+    P.proof = Bytes();
+    P.queries.clear();
+    begin_open(P);
+    Engine e(*s.ctx, 0, 1);
+    e.submit(&P);
+    e.run();
+    P.tr = nullptr;
+    if (P.status != Z0P_OK) throw Status(P.status);
+    indices = P.queries;
+    return std::move(P.proof);
+}
+
+void fri_state_free(FriState* s) {
+    if (!s) return;
+    if (s->proof) {
+        (void)hipStreamSynchronize(s->ctx->stream);
+        if (s->proof->mem) (void)hipFree(s->proof->mem);
+        delete s->proof;
+    }
+    delete s;
+}
+
+// ---------------------------------------------------------------------------
+// Transcript::grind (transcript.rs:206-228) on the device.
+
+uint64_t grind_search(Ctx& c, uint8_t* dev, const GrindHead& h, GrindJob j, uint32_t bits) {
+    GrindHead* dh = reinterpret_cast<GrindHead*>(dev);
+    auto* res = reinterpret_cast<unsigned long long*>(dev + GRIND_BUF_RESULT);
+    hip_check(hipMemcpyAsync(dh, &h, sizeof(h), hipMemcpyHostToDevice, c.stream));
+    j.head = dh;
+    j.bits = bits;
+    j.nblocks = uint32_t(std::min(2048, std::max(16, c.sched.grind_blocks)));
+    j.result = res;
+    uint64_t found = ~uint64_t(0);
+    for (uint64_t window = 0; found == ~uint64_t(0); ++window) {
+        // Window w scans [2^(20+4w), 2^(24+4w)) after the windows before it.
+        j.start = window == 0 ? 0 : uint64_t(1) << std::min<uint64_t>(63, 20 + 4 * window);
+        j.limit = uint64_t(1) << std::min<uint64_t>(63, 24 + 4 * window);
+        const Launch L = c.launch();
+        launch_grind(L, {j});
+        hip_check(hipMemcpyAsync(&found, res, 8, hipMemcpyDeviceToHost, c.stream));
+        hip_check(hipStreamSynchronize(c.stream));
+        if (c.prof.enabled) c.prof.resolve();
+    }
+    return found;
+}
+
+uint64_t transcript_grind(Ctx& c, Transcript& tr, const char* label, uint32_t bits, uint8_t* dev) {
+    uint8_t nb[8];
+    if (bits > 32) {
+        // The protocol caps difficulty at 32: fail closed with nonce 0.
+        store_le64(nb, 0);
+        tr.absorb(label, nb, 8);
+        return 0;
+    }
+    std::vector<uint8_t> head = tr.pow_head(label);
+    GrindHead h;
+    GrindJob j{};
+    require(grind_prepare(head.data(), head.size(), h, j), Z0P_ERR_MALFORMED_INPUT);
+    if (!dev) {
+        if (!c.grind_buf) hip_check(hipMalloc(&c.grind_buf, GRIND_BUF_BYTES));
+        dev = c.grind_buf;
+    }
+    const uint64_t nonce = grind_search(c, dev, h, j, bits);
+    store_le64(nb, nonce);
+    tr.absorb(label, nb, 8);
+    return nonce;
 }
 
 }  // namespace z0p

@@ -91,6 +91,8 @@ Proof* proof_new_fold_fri(Ctx& c, const FoldStepDesc* steps, uint32_t nsteps, ui
 const std::vector<uint64_t>& proof_queries(const Proof* p);
 void proof_free(Proof* p);
 int proof_status(const Proof* p);
+// True once the proof finished its last phase (its bytes are ready).
+bool proof_done(const Proof* p);
 Bytes& proof_bytes(Proof* p);
 Bytes& proof_commitment(Proof* p);
 size_t proof_mem_bytes(const Proof* p);
@@ -158,11 +160,39 @@ Bytes air_prove(Ctx& ctx, const z0p_air_params& p, Transcript& tr);
 Bytes air_prove_with_trace(Ctx& ctx, const z0p_air_params& p, const uint32_t* col_a, uint64_t len_a,
                                           const uint32_t* col_b, uint64_t len_b, Transcript& tr);
 Bytes air_prove_zk(Ctx& ctx, const z0p_air_params& p, uint64_t n_rand, ChaCha20& rng, Transcript& tr);
+// fri::commit* + public_commitment + open; `indices` receives the sampled
+// leaf indices fri::open returns (fri.rs:645-666).
 void fri_commit_open(Ctx& ctx, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,
-                     const z0p_fri_params& fp, Transcript& tr,
-                     Bytes& commitment, Bytes& proof);
+                     const z0p_fri_params& fp, Transcript& tr, Bytes& commitment, Bytes& proof,
+                     std::vector<uint64_t>& indices);
 // One proof alone through the scheduler (latency mode); throws its status.
 void run_proof(Ctx& ctx, Proof* p);
+
+// fri::commit* (fri.rs:408-549) and fri::open (fri.rs:645-666) as separate
+// calls: the state holds its ctx and a proof whose device memory (layers and
+// trees) is its own until fri_state_free.
+struct FriState {
+    Ctx* ctx = nullptr;
+    Proof* proof = nullptr;
+};
+FriState* fri_commit(Ctx& ctx, const uint32_t* src, uint64_t count, int kind, uint64_t degree_bound,
+                     const z0p_fri_params& fp, Transcript& tr);
+// Commitment::to_bytes of fri::public_commitment (fri.rs:863-880).
+const Bytes& fri_state_commitment(const FriState& s);
+uint64_t fri_state_queries(const FriState& s);
+// Proof::to_bytes; `indices` receives the sampled leaf indices.
+Bytes fri_open(FriState& s, Transcript& tr, std::vector<uint64_t>& indices);
+void fri_state_free(FriState* s);
+
+// Transcript::grind (transcript.rs:206-228): the smallest nonce, searched on
+// the device, then absorbed under `label`. `dev` is a device buffer of
+// GRIND_BUF_BYTES (the head, then the running minimum at GRIND_BUF_RESULT);
+// null uses the ctx's own buffer. Throws MalformedInput when head || nonce
+// exceeds one BLAKE3 chunk (labels above 976 bytes).
+constexpr size_t GRIND_BUF_RESULT = 192, GRIND_BUF_BYTES = 256;
+uint64_t transcript_grind(Ctx& c, Transcript& tr, const char* label, uint32_t bits, uint8_t* dev = nullptr);
+// The search alone: the smallest passing nonce of the prepared head.
+uint64_t grind_search(Ctx& c, uint8_t* dev, const GrindHead& h, GrindJob j, uint32_t bits);
 Bytes range_prove(Ctx& ctx, const z0p_range_params& p, Transcript& tr);
 Bytes range_prove_with_columns(Ctx& ctx, const z0p_range_params& p, const uint32_t* cols, uint64_t ncols,
                                uint64_t len, Transcript& tr);
@@ -182,8 +212,9 @@ Bytes fold_acc_finalize(FoldAcc* a, Transcript& tr);
 
 // Host verifier (air.rs:607-752, fri.rs:686-859); throws Status on rejection.
 void air_verify(const z0p_air_params& p, Transcript& tr, const uint8_t* proof, size_t len);
-void fri_verify(const uint8_t* cm, size_t cm_len, const uint8_t* pf, size_t pf_len, const z0p_fri_params& fp,
-                Transcript& tr);
+// Returns the sampled query indices (fri.rs:686).
+std::vector<uint64_t> fri_verify(const uint8_t* cm, size_t cm_len, const uint8_t* pf, size_t pf_len,
+                                 const z0p_fri_params& fp, Transcript& tr);
 void range_verify(const z0p_range_params& p, Transcript& tr, const uint8_t* proof, size_t len);
 void fold_verify(const z0p_fold_params& p, Transcript& tr, const uint8_t* proof, size_t len);
 void table_verify(const z0p_table_params& p, const uint32_t* table, uint64_t table_len, Transcript& tr,
@@ -211,4 +242,7 @@ struct z0p_chacha20 {
 };
 struct z0p_fold_acc {
     z0p::FoldAcc* a = nullptr;
+};
+struct z0p_fri_state {
+    z0p::FriState* s = nullptr;
 };

@@ -215,17 +215,32 @@ int z0p_air_prove_batch(z0p_ctx* ctx, const z0p_air_params* params, z0p_transcri
         z0p::Engine e(c, budget, DEFAULT_MAX_ACTIVE);
         for (auto& p : ps)
             if (p) e.submit(p.get());
-        e.run();
+        // A device error that ends the batch has already given every
+        // unfinished instance its status; instances that finished before it
+        // still hand over their proofs, so statuses[i] == 0 always comes
+        // with proofs[i].
+        int ended = Z0P_OK;
+        try {
+            e.run();
+        } catch (const z0p::Status& s) {
+            ended = s.code;
+        } catch (...) {
+            ended = Z0P_ERR_DEVICE;
+        }
         for (size_t i = 0; i < count; ++i) {
             if (!ps[i]) continue;
             statuses[i] = z0p::proof_status(ps[i].get());
-            if (statuses[i] != Z0P_OK) continue;
-            proofs[i] = hand_over(z0p::proof_bytes(ps[i].get()), &proof_lens[i]);
-            if (!proofs[i]) {
-                statuses[i] = Z0P_ERR_OUT_OF_MEMORY;
-                proof_lens[i] = 0;
+            if (statuses[i] == Z0P_OK && z0p::proof_done(ps[i].get())) {
+                proofs[i] = hand_over(z0p::proof_bytes(ps[i].get()), &proof_lens[i]);
+                if (!proofs[i]) {
+                    statuses[i] = Z0P_ERR_OUT_OF_MEMORY;
+                    proof_lens[i] = 0;
+                }
+            } else if (statuses[i] == Z0P_OK) {
+                statuses[i] = ended;
             }
         }
+        if (ended != Z0P_OK) throw z0p::Status(ended);
     });
 }
 

@@ -2,6 +2,7 @@
 // one device stage on host data so a parity failure can be pinned to one
 // kernel. Natural-order inputs are permuted to the device's bit-reversed
 // layout on the host and back.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -294,21 +295,28 @@ int z0p_test_copy(z0p_ctx* ctx, size_t bytes, float* ms) {
     });
 }
 
+int z0p_test_set_gate_spin(z0p_ctx* ctx, uint32_t polls) {
+    if (!ctx) return Z0P_ERR_INVALID_ARGUMENT;
+    ctx->c->gate_spin = polls;
+    return Z0P_OK;
+}
+
 int z0p_test_grind(z0p_ctx* ctx, const uint8_t* head, size_t head_len, uint32_t bits, uint64_t* nonce) {
     if (!ctx || !head || !nonce) return Z0P_ERR_INVALID_ARGUMENT;
     return guarded([&] {
-        require(head_len == 47 && bits <= 32, Z0P_ERR_MALFORMED_INPUT);
+        require(bits <= 32, Z0P_ERR_MALFORMED_INPUT);
         Ctx& c = *ctx->c;
-        c.reserve_scratch(1 << 20);
+        GrindHead h;
         GrindJob g{};
-        uint32_t hw[16] = {0};
-        for (size_t i = 0; i < head_len; ++i) hw[i >> 2] |= uint32_t(head[i]) << (8 * (i & 3));
-        g.hw = to_dev(c, hw, 16);
+        require(grind_prepare(head, head_len, h, g), Z0P_ERR_MALFORMED_INPUT);
+        c.reserve_scratch(1 << 20);
+        g.head = to_dev(c, &h, 1);
         g.bits = bits;
         g.nblocks = 2048;
         g.result = c.scratch_alloc<unsigned long long>(1);
         uint64_t found = ~uint64_t(0);
         for (uint64_t window = 0; found == ~uint64_t(0); ++window) {
+            g.start = window == 0 ? 0 : uint64_t(1) << std::min<uint64_t>(63, 20 + 4 * window);
             g.limit = uint64_t(1) << std::min<uint64_t>(63, 24 + 4 * window);
             run_chain(c, {g});
             to_host(c, &found, reinterpret_cast<const uint64_t*>(g.result), 1);

@@ -371,8 +371,8 @@ bool verify_capped_row(const Digest& entry, const std::vector<uint8_t>& row, uin
 
 }  // namespace
 
-void fri_verify(const uint8_t* cmb, size_t cm_len, const uint8_t* pfb, size_t pf_len, const z0p_fri_params& fp,
-                Transcript& tr) {
+std::vector<uint64_t> fri_verify(const uint8_t* cmb, size_t cm_len, const uint8_t* pfb, size_t pf_len,
+                                 const z0p_fri_params& fp, Transcript& tr) {
     Reader rc(cmb, cm_len);
     rc.version();
     CommitmentV cm = read_commitment_body(rc);
@@ -381,7 +381,7 @@ void fri_verify(const uint8_t* cmb, size_t cm_len, const uint8_t* pfb, size_t pf
     rp.version();
     FriProofV pf = read_fri_proof_body(rp);
     rp.end();
-    fri_verify_impl(cm, pf, fp, tr);
+    return fri_verify_impl(cm, pf, fp, tr);
 }
 
 void air_verify(const z0p_air_params& p, Transcript& tr, const uint8_t* bytes, size_t len) {
@@ -672,19 +672,26 @@ void fold_verify(const z0p_fold_params& p, Transcript& tr, const uint8_t* bytes,
         std::vector<StepOpeningV> openings;
     };
     constexpr uint32_t MAX_STEPS = 1u << 20;  // codec.rs:34
+    // Vectors grow one decoded item at a time: memory stays proportional to
+    // the bytes actually read, never to an untrusted count (codec.rs:10-13).
     Reader r(bytes, len);
     r.version();
-    std::vector<StepV> steps(r.count(MAX_STEPS));
-    for (auto& st : steps) {
+    std::vector<StepV> steps;
+    const uint32_t nsteps = r.count(MAX_STEPS);
+    for (uint32_t k = 0; k < nsteps; ++k) {
+        StepV st;
         for (auto& c : st.claims) c = r.field();
         st.cap = read_cap(r);
-        st.openings.resize(r.count(MAX_QUERIES));
-        for (auto& o : st.openings) {
+        const uint32_t nops = r.count(MAX_QUERIES);
+        for (uint32_t i = 0; i < nops; ++i) {
+            StepOpeningV o;
             o.index = r.u64();
             for (auto& v : o.cols) v = r.field();
             o.path = read_path(r);
             o.shift_path = read_path(r);
+            st.openings.push_back(std::move(o));
         }
+        steps.push_back(std::move(st));
     }
     CommitmentV cm = read_commitment_body(r);
     FriProofV pf = read_fri_proof_body(r);
