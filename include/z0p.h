@@ -120,6 +120,18 @@ int z0p_transcript_challenge_index(z0p_transcript* t, const char* label, uint64_
  * Z0P_ERR_MALFORMED_INPUT. */
 int z0p_transcript_grind(z0p_ctx* ctx, z0p_transcript* t, const char* label, uint32_t difficulty_bits,
                          uint64_t* nonce);
+/* z0p_transcript_grind that allocates nothing: no hipMalloc, hipFree or
+ * other allocating routine runs, and no profiling events are recorded. It
+ * has no counterpart in the reference; it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation). The grind head and the running minimum nonce live
+ * in one static 256-byte device array per GPU, shared by every call on that
+ * GPU and keeping the last call's contents (the minimum is reset by a kernel
+ * before each search window), so calls that overlap on one GPU race, from
+ * any ctx. Same limits and results as z0p_transcript_grind: labels up to
+ * 976 bytes, else Z0P_ERR_MALFORMED_INPUT. */
+int z0p_transcript_grind_noalloc(z0p_ctx* ctx, z0p_transcript* t, const char* label, uint32_t difficulty_bits,
+                                 uint64_t* nonce);
 /* Transcript::verify_grind (transcript.rs:235-246): 1 when the nonce meets the
  * difficulty (absorbed as grind absorbs it), 0 otherwise (nothing absorbed;
  * also for difficulty_bits > 256 or a null argument). Host only. */

@@ -115,6 +115,7 @@ SIGNATURES = {
     "z0p_transcript_challenge_ext": (None, [vp, ctypes.c_char_p, u32p]),
     "z0p_transcript_challenge_index": (ctypes.c_int, [vp, ctypes.c_char_p, u64, ctypes.POINTER(u64)]),
     "z0p_transcript_grind": (ctypes.c_int, [vp, vp, ctypes.c_char_p, u32, ctypes.POINTER(u64)]),
+    "z0p_transcript_grind_noalloc": (ctypes.c_int, [vp, vp, ctypes.c_char_p, u32, ctypes.POINTER(u64)]),
     "z0p_transcript_verify_grind": (ctypes.c_int, [vp, ctypes.c_char_p, u64, u32]),
     "z0p_air_params_new": (ctypes.c_int, [u64, u64, u64, u64, u32, u32, u32, ctypes.POINTER(AirParamsC)]),
     "z0p_air_params_for_security": (ctypes.c_int, [u64, u32, u32, u32, u32, ctypes.POINTER(AirParamsC)]),

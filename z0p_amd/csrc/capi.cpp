@@ -202,6 +202,14 @@ int z0p_transcript_grind(z0p_ctx* ctx, z0p_transcript* t, const char* label, uin
     return guarded([&] { *nonce = z0p::transcript_grind(*ctx->c, t->t, label, difficulty_bits); });
 }
 
+int z0p_transcript_grind_noalloc(z0p_ctx* ctx, z0p_transcript* t, const char* label, uint32_t difficulty_bits,
+                                 uint64_t* nonce) {
+    if (!ctx || !t || !label || !nonce) return Z0P_ERR_INVALID_ARGUMENT;
+    uint8_t* dev = z0p::noalloc_scratch(ctx->c->device);
+    if (!dev) return Z0P_ERR_DEVICE;  // registered when the ctx was created
+    return guarded([&] { *nonce = z0p::transcript_grind(*ctx->c, t->t, label, difficulty_bits, dev); });
+}
+
 int z0p_transcript_verify_grind(z0p_transcript* t, const char* label, uint64_t nonce, uint32_t difficulty_bits) {
     if (!t || !label || difficulty_bits > 256) return 0;
     const std::vector<uint8_t> head = t->t.pow_head(label);

@@ -15,9 +15,11 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "device_common.hpp"
 #include "profiler.hpp"
+#include "status.hpp"
 
 namespace z0p {
 
@@ -1894,6 +1896,35 @@ void launch_grind(const Launch& L, std::vector<GrindJob> jobs) {
     }
     ProfScope ps(L.prof, "grind", L.stream);
     hipLaunchKernelGGL(k_grind, dim3(g), dim3(WG), 0, L.stream, jobs[0], table, n);
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free grind's device memory: one grind head and the running
+// minimum (GRIND_BUF_BYTES), a static array of which every device holds its
+// own copy until the process ends. Host code never passes the array's name
+// to a kernel or a copy: it uses the address hipGetSymbolAddress returns on
+// each device, looked up once per device when a context is created there.
+static __device__ __align__(256) uint8_t scratch[GRIND_BUF_BYTES];
+
+namespace {
+std::mutex noalloc_mu;
+uint8_t* noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+void noalloc_scratch_init(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) throw Status(Z0P_ERR_NO_DEVICE);
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (noalloc_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    require(e == hipSuccess && p != nullptr, Z0P_ERR_DEVICE);
+    noalloc_ptr[device] = static_cast<uint8_t*>(p);
+}
+
+uint8_t* noalloc_scratch(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    return noalloc_ptr[device];
 }
 
 bool grind_prepare(const uint8_t* head, size_t head_len, GrindHead& h, GrindJob& j) {

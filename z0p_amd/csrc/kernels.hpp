@@ -341,6 +341,18 @@ struct GrindJob {
 // and the job's tail fields; false when head || nonce exceeds one chunk.
 bool grind_prepare(const uint8_t* head, size_t head_len, GrindHead& h, GrindJob& j);
 
+// Device buffer of a transcript grind: the GrindHead, then the running
+// minimum nonce at GRIND_BUF_RESULT.
+constexpr size_t GRIND_BUF_RESULT = 192, GRIND_BUF_BYTES = 256;
+static_assert(sizeof(GrindHead) <= GRIND_BUF_RESULT, "grind buffer layout");
+
+// The allocation-free grind's static device array (GRIND_BUF_BYTES per
+// device): noalloc_scratch_init looks up its address on `device` (which must
+// be current) once per device; noalloc_scratch returns it (null before).
+constexpr int NOALLOC_MAX_DEVICES = 64;
+void noalloc_scratch_init(int device);
+uint8_t* noalloc_scratch(int device);
+
 // ---- launchers ---------------------------------------------------------------
 void launch_fib_coeffs(const Launch& L, std::vector<FibCoeffJob> jobs);
 void launch_ntt(const Launch& L, std::vector<NttJob> jobs);

@@ -180,6 +180,7 @@ Ctx* ctx_create(int device) {
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0 || device < 0 || device >= count)
         throw Status(Z0P_ERR_NO_DEVICE);
     hip_check(hipSetDevice(device));
+    noalloc_scratch_init(device);
     auto c = std::make_unique<Ctx>();
     c->device = device;
     hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -2121,7 +2122,7 @@ void fri_state_free(FriState* s) {
 // ---------------------------------------------------------------------------
 // Transcript::grind (transcript.rs:206-228) on the device.
 
-uint64_t grind_search(Ctx& c, uint8_t* dev, const GrindHead& h, GrindJob j, uint32_t bits) {
+uint64_t grind_search(Ctx& c, uint8_t* dev, const GrindHead& h, GrindJob j, uint32_t bits, bool profile) {
     GrindHead* dh = reinterpret_cast<GrindHead*>(dev);
     auto* res = reinterpret_cast<unsigned long long*>(dev + GRIND_BUF_RESULT);
     hip_check(hipMemcpyAsync(dh, &h, sizeof(h), hipMemcpyHostToDevice, c.stream));
@@ -2134,11 +2135,12 @@ uint64_t grind_search(Ctx& c, uint8_t* dev, const GrindHead& h, GrindJob j, uint
         // Window w scans [2^(20+4w), 2^(24+4w)) after the windows before it.
         j.start = window == 0 ? 0 : uint64_t(1) << std::min<uint64_t>(63, 20 + 4 * window);
         j.limit = uint64_t(1) << std::min<uint64_t>(63, 24 + 4 * window);
-        const Launch L = c.launch();
+        Launch L = c.launch();
+        if (!profile) L.prof = nullptr;  // no event pairs
         launch_grind(L, {j});
         hip_check(hipMemcpyAsync(&found, res, 8, hipMemcpyDeviceToHost, c.stream));
         hip_check(hipStreamSynchronize(c.stream));
-        if (c.prof.enabled) c.prof.resolve();
+        if (profile && c.prof.enabled) c.prof.resolve();
     }
     return found;
 }
@@ -2155,11 +2157,12 @@ uint64_t transcript_grind(Ctx& c, Transcript& tr, const char* label, uint32_t bi
     GrindHead h;
     GrindJob j{};
     require(grind_prepare(head.data(), head.size(), h, j), Z0P_ERR_MALFORMED_INPUT);
-    if (!dev) {
+    const bool own = dev == nullptr;
+    if (own) {
         if (!c.grind_buf) hip_check(hipMalloc(&c.grind_buf, GRIND_BUF_BYTES));
         dev = c.grind_buf;
     }
-    const uint64_t nonce = grind_search(c, dev, h, j, bits);
+    const uint64_t nonce = grind_search(c, dev, h, j, bits, own);
     store_le64(nb, nonce);
     tr.absorb(label, nb, 8);
     return nonce;

@@ -188,11 +188,11 @@ void fri_state_free(FriState* s);
 // the device, then absorbed under `label`. `dev` is a device buffer of
 // GRIND_BUF_BYTES (the head, then the running minimum at GRIND_BUF_RESULT);
 // null uses the ctx's own buffer. Throws MalformedInput when head || nonce
-// exceeds one BLAKE3 chunk (labels above 976 bytes).
-constexpr size_t GRIND_BUF_RESULT = 192, GRIND_BUF_BYTES = 256;
+// exceeds one BLAKE3 chunk (labels above 976 bytes). With `dev` given the
+// call allocates nothing and records no profiling events.
 uint64_t transcript_grind(Ctx& c, Transcript& tr, const char* label, uint32_t bits, uint8_t* dev = nullptr);
 // The search alone: the smallest passing nonce of the prepared head.
-uint64_t grind_search(Ctx& c, uint8_t* dev, const GrindHead& h, GrindJob j, uint32_t bits);
+uint64_t grind_search(Ctx& c, uint8_t* dev, const GrindHead& h, GrindJob j, uint32_t bits, bool profile = true);
 Bytes range_prove(Ctx& ctx, const z0p_range_params& p, Transcript& tr);
 Bytes range_prove_with_columns(Ctx& ctx, const z0p_range_params& p, const uint32_t* cols, uint64_t ncols,
                                uint64_t len, Transcript& tr);
