@@ -192,16 +192,20 @@ def served(z, budget_gib, duration_s, schedule, single_schedule=None, seed=77):
             out.append(seeded_instance(z, rng.randint(10, 20), rng.getrandbits(63), b"serve/%s/%d" % (tag.encode(), i)))
         return out
 
+    tally = {"requests": 0, "verified": 0}
+
     def run(srv, reqs, arrivals):
         lat = []
+        ids = {}
+        proofs = {}
 
         def producer():
             t0 = time.perf_counter()
-            for (p, l), at in zip(reqs, arrivals):
+            for i, ((p, l), at) in enumerate(zip(reqs, arrivals)):
                 d = t0 + at - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
-                srv.submit(p, z.Transcript(l))
+                ids[srv.submit(p, z.Transcript(l))] = i
         th = threading.Thread(target=producer)
         t0 = time.perf_counter()
         th.start()
@@ -209,11 +213,18 @@ def served(z, budget_gib, duration_s, schedule, single_schedule=None, seed=77):
             r = srv.poll(1000)
             if r is None:
                 continue
-            _, proof, lt, _ = r
+            rid, proof, lt, _ = r
             assert isinstance(proof, bytes)
+            proofs[rid] = proof
             lat.append(lt)
         t1 = time.perf_counter()
         th.join()
+        # Every served proof is checked by the host verifier (untimed).
+        order = sorted(proofs, key=lambda rid: ids[rid])
+        ok = z.air.verify_batch([reqs[ids[rid]][0] for rid in order], [proofs[rid] for rid in order],
+                                [z.Transcript(reqs[ids[rid]][1]) for rid in order])
+        tally["requests"] += len(reqs)
+        tally["verified"] += sum(1 for v in ok if v is None)
         lat.sort()
         return t1 - t0, lat
 
@@ -258,32 +269,75 @@ def served(z, budget_gib, duration_s, schedule, single_schedule=None, seed=77):
         "saturated_proofs_per_s": round(sat, 1),
         "one_at_a_time_proofs_per_s": round(single, 1),
         "saturated_vs_one_at_a_time": round(sat / single, 2),
+        "requests": tally["requests"],
+        "verified": tally["verified"],
         "schedule": schedule,
         "levels": levels,
     }
 
 
+def host_cpu():
+    """CPU model, logical CPUs visible to this process and physical cores."""
+    model, phys = None, set()
+    try:
+        cur = {}
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if ":" not in line:
+                    if cur.get("core id") is not None:
+                        phys.add((cur.get("physical id"), cur.get("core id")))
+                    cur = {}
+                    continue
+                k, v = [x.strip() for x in line.split(":", 1)]
+                cur[k] = v
+                if k == "model name" and model is None:
+                    model = v
+        if cur.get("core id") is not None:
+            phys.add((cur.get("physical id"), cur.get("core id")))
+    except OSError:
+        pass
+    visible = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"model": model, "logical_cpus_visible": visible, "logical_cpus_host": os.cpu_count(),
+            "physical_cores_host": len(phys) or None}
+
+
 def cpu_baseline(n, repeats):
+    """The oracle (C++ restatement of the reference prover) on the host: all
+    visible threads (one warm-up, median of `repeats`, per-stage times) and
+    one thread (one run)."""
     import oracle_ffi as o
-    threads = os.cpu_count() or 1
-    o.set_threads(threads)
+    cpu = host_cpu()
+    threads = cpu["logical_cpus_visible"]
     bl = o.fib_b_last(1, 1, n)
     p = o.air_params_for_security(n, 1, 1, bl)
+    o.set_threads(threads)
     o.air_prove(p, o.Transcript(LABEL))  # warm-up
-    times = []
+    times, stages = [], []
     for _ in range(repeats):
+        o.clear_stage_timers()
         t0 = time.perf_counter()
         o.air_prove(p, o.Transcript(LABEL))
         times.append(time.perf_counter() - t0)
-    times.sort()
-    med = times[len(times) // 2]
+        stages.append(o.stage_timers())
+    order = sorted(range(repeats), key=lambda i: times[i])
+    med_i = order[len(order) // 2]
+    med = times[med_i]
+    o.set_threads(1)
+    t0 = time.perf_counter()
+    o.air_prove(p, o.Transcript(LABEL))
+    t_one = time.perf_counter() - t0
+    o.set_threads(threads)
     return {
         "value": round(med * 1e6 / n, 4),
         "unit": "us/step",
         "cores": threads,
         "kind": "port",
-        "sample": "oracle (C++ restatement of the reference prover) AIR N=%d prod, %d threads, median of %d after 1 warm-up, %.3f s/proof"
-        % (n, threads, repeats, med),
+        "sample": "oracle (C++ restatement of the reference prover) AIR N=%d prod on %d threads: median of %d "
+                  "after 1 warm-up, %.3f s/proof; on 1 thread: one run, %.3f s/proof" % (n, threads, repeats, med,
+                                                                                          t_one),
+        "one_thread": {"value": round(t_one * 1e6 / n, 4), "unit": "us/step", "cores": 1, "s_per_proof": round(t_one, 3)},
+        "stage_ms": {k: round(v, 2) for k, v in sorted(stages[med_i].items())},
+        "cpu": cpu,
     }
 
 

@@ -84,6 +84,9 @@ int z0p_ctx_create(int device, z0p_ctx** out);
 void z0p_ctx_destroy(z0p_ctx* ctx);
 /* Per-kernel HIP-event timing on the ctx stream (off by default). */
 int z0p_ctx_set_profiling(z0p_ctx* ctx, int enabled);
+/* Times only launches named `kernel` (as z0p_ctx_kernel_stats names them)
+ * while profiling is on; NULL or "" times every launch. */
+int z0p_ctx_profile_only(z0p_ctx* ctx, const char* kernel);
 /* "name=launches,total_ms;..." for the launches since the last reset; returns
  * the length needed including the terminating NUL. */
 size_t z0p_ctx_kernel_stats(z0p_ctx* ctx, char* buf, size_t cap);

@@ -144,7 +144,10 @@ class Context:
     def handle(self):
         return self._h
 
-    def set_profiling(self, enabled: bool):
+    def set_profiling(self, enabled: bool, only: str = None):
+        """Per-launch HIP events on the context stream; `only` restricts
+        them to launches of one kernel name."""
+        check(lib().z0p_ctx_profile_only(self._h, only.encode() if only else None))
         check(lib().z0p_ctx_set_profiling(self._h, 1 if enabled else 0))
 
     def kernel_stats(self):

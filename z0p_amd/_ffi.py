@@ -102,6 +102,7 @@ SIGNATURES = {
     "z0p_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(vp)]),
     "z0p_ctx_destroy": (None, [vp]),
     "z0p_ctx_set_profiling": (ctypes.c_int, [vp, ctypes.c_int]),
+    "z0p_ctx_profile_only": (ctypes.c_int, [vp, ctypes.c_char_p]),
     "z0p_ctx_kernel_stats": (sz, [vp, ctypes.c_char_p, sz]),
     "z0p_ctx_reset_stats": (None, [vp]),
     "z0p_status_string": (ctypes.c_char_p, [ctypes.c_int]),

@@ -63,6 +63,12 @@ int z0p_ctx_set_profiling(z0p_ctx* ctx, int enabled) {
     return Z0P_OK;
 }
 
+int z0p_ctx_profile_only(z0p_ctx* ctx, const char* kernel) {
+    if (!ctx) return Z0P_ERR_INVALID_ARGUMENT;
+    ctx->c->prof.only = kernel ? kernel : "";
+    return Z0P_OK;
+}
+
 size_t z0p_ctx_kernel_stats(z0p_ctx* ctx, char* buf, size_t cap) {
     if (!ctx) return 0;
     std::string s;

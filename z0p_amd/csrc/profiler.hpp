@@ -12,6 +12,7 @@ namespace z0p {
 
 struct Profiler {
     bool enabled = false;
+    std::string only;  // when not empty: time launches of this name only
     struct Pending {
         const char* name;
         hipEvent_t start, stop;
@@ -73,6 +74,7 @@ struct ProfScope {
     hipStream_t s;
     Profiler::Pending ev{nullptr, nullptr, nullptr};
     ProfScope(Profiler* prof, const char* name, hipStream_t stream) : p(prof && prof->enabled ? prof : nullptr), s(stream) {
+        if (p && !p->only.empty() && p->only != name) p = nullptr;
         if (!p) return;
         ev.name = name;
         ev.start = p->take();
