@@ -13,8 +13,9 @@ from z0p_amd import _ffi
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = dict(tail_max_log=17, tail_group_max_log=8, tail_spread_log=0, levels_fuse=2, small_max_log=10,
-               grind_blocks=4096, fold_fuse_max_log=0, compose_fuse=1, trace_fuse=1)
+DEFAULT = dict(tail_max_log=17, tail_group_max_log=7, tail_spread_log=0, levels_fuse=2, small_max_log=10,
+               grind_blocks=4096, fold_fuse_max_log=0, compose_fuse=1, trace_fuse=1, subtree_levels=0,
+               compose_subtree=0)
 
 SCHEDULES = [
     dict(DEFAULT, tail_group_max_log=9, tail_spread_log=8),
@@ -25,6 +26,11 @@ SCHEDULES = [
     dict(DEFAULT, small_max_log=8, grind_blocks=1 << 20),
     dict(DEFAULT, fold_fuse_max_log=16, compose_fuse=0, trace_fuse=0),
     dict(DEFAULT, fold_fuse_max_log=27, small_max_log=0),
+    dict(DEFAULT, subtree_levels=8),
+    dict(DEFAULT, subtree_levels=3, tail_max_log=10),
+    dict(DEFAULT, subtree_levels=8, tail_max_log=4, fold_fuse_max_log=22, small_max_log=0, compose_subtree=8),
+    dict(DEFAULT, compose_subtree=2, tail_max_log=13),
+    dict(DEFAULT, subtree_levels=1, compose_fuse=0, trace_fuse=0, tail_group_max_log=3),
 ]
 
 
@@ -39,9 +45,10 @@ def test_default_schedule(sched_ctx):
     assert sched_ctx.get_schedule() == DEFAULT
 
 
-@pytest.mark.parametrize("sched", SCHEDULES, ids=lambda s: "g%d_s%d_t%d_f%d_sm%d_gb%d_ff%d" % (
+@pytest.mark.parametrize("sched", SCHEDULES, ids=lambda s: "g%d_s%d_t%d_f%d_sm%d_gb%d_ff%d_c%d%d_st%d_cs%d" % (
     s["tail_group_max_log"], s["tail_spread_log"], s["tail_max_log"], s["levels_fuse"], s["small_max_log"],
-    s["grind_blocks"], s["fold_fuse_max_log"]))
+    s["grind_blocks"], s["fold_fuse_max_log"], s["compose_fuse"], s["trace_fuse"], s["subtree_levels"],
+    s["compose_subtree"]))
 def test_proofs_identical_under_schedule(sched_ctx, sched):
     ctx = sched_ctx
     ctx.set_schedule(sched)
@@ -67,11 +74,16 @@ def test_proofs_identical_under_schedule(sched_ctx, sched):
     assert (cm, pf) == (ocm, opf)
 
 
-@pytest.mark.parametrize("leaves", [2, 64, 1 << 11, 1 << 16])
-@pytest.mark.parametrize("tail", [(2, 0, 4), (9, 0, 17), (9, 8, 17), (6, 6, 10)])
+@pytest.mark.parametrize("leaves", [2, 4, 64, 1 << 9, 1 << 11, 1 << 16, 1 << 19])
+@pytest.mark.parametrize("tail", [(2, 0, 4, 8), (9, 0, 17, 8), (9, 8, 17, 0), (6, 6, 10, 8), (8, 0, 2, 8),
+                                  (8, 0, 12, 2)])
 def test_merkle_levels_under_tail_shapes(sched_ctx, leaves, tail):
+    """Every level of a 64-byte-leaf tree: leaves with their workgroups'
+    subtrees (up to 9 levels), node-level jobs reducing 10 levels per
+    workgroup, and quad-lane tails, in every split the knobs allow."""
     ctx = sched_ctx
-    ctx.set_schedule(dict(DEFAULT, tail_group_max_log=tail[0], tail_spread_log=tail[1], tail_max_log=tail[2]))
+    ctx.set_schedule(dict(DEFAULT, tail_group_max_log=tail[0], tail_spread_log=tail[1], tail_max_log=tail[2],
+                          subtree_levels=tail[3]))
     rng = np.random.default_rng(leaves + tail[0])
     flat = rng.integers(0, 256, leaves * 64, dtype=np.uint8).tobytes()
     out = (np.zeros(32 * (2 * leaves - 1), dtype=np.uint8))
@@ -86,7 +98,8 @@ def test_set_schedule_rejects_out_of_range(sched_ctx):
     for bad in [dict(DEFAULT, tail_group_max_log=10), dict(DEFAULT, levels_fuse=3), dict(DEFAULT, small_max_log=11),
                 dict(DEFAULT, small_max_log=2), dict(DEFAULT, grind_blocks=8), dict(DEFAULT, tail_max_log=1),
                 dict(DEFAULT, fold_fuse_max_log=2), dict(DEFAULT, fold_fuse_max_log=28), dict(DEFAULT, compose_fuse=2),
-                dict(DEFAULT, trace_fuse=-1)]:
+                dict(DEFAULT, trace_fuse=-1), dict(DEFAULT, subtree_levels=9), dict(DEFAULT, subtree_levels=-1),
+                dict(DEFAULT, compose_subtree=9)]:
         with pytest.raises(z.Z0pError) as e:
             sched_ctx.set_schedule(bad)
         assert e.value.name == "InvalidArgument"

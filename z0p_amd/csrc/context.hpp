@@ -40,6 +40,7 @@ struct GeometryTables {
     const uint32_t* zvan = nullptr;  // [B] s_j^n - 1, Montgomery
     const uint32_t* xn = nullptr;    // [n] omega_n^brev(q), Montgomery
     const uint32_t* finv = nullptr;  // [m/4] Omega_m^-brev(i), Montgomery
+    PointInverses pinv;              // built on first use by an AIR proof (Ctx::point_inverses)
     ~GeometryTables() {
         for (void* p : owned) (void)hipFree(p);
     }
@@ -226,7 +227,7 @@ class HostPool {
 // (z0p_ctx_set_schedule) or measured by the autotuner (tuner.cpp).
 struct Schedule {
     int tail_max_log = 17;       // levels of at most 2^k digests go to the quad-lane tail
-    int tail_group_max_log = 8;  // tail workgroup: at most 2^k digests (2^(k+1) threads)
+    int tail_group_max_log = 7;  // tail workgroup: at most 2^k digests (2^(k+1) threads)
     int tail_spread_log = 0;     // smaller groups so a level spans about 2^k workgroups
     int levels_fuse = 2;         // wide levels per launch (1 or 2)
     int small_max_log = 10;      // fold + commit in one workgroup for layers of at most 2^k values (0: off)
@@ -234,6 +235,8 @@ struct Schedule {
     int fold_fuse_max_log = 0;   // fold + next layer's leaves in one kernel for layers of at most 2^k values (0: off)
     int compose_fuse = 1;        // composition + FRI layer 0's leaves in one kernel (0: separate kernels)
     int trace_fuse = 1;          // linked column + trace leaves in one kernel (0: separate kernels)
+    int subtree_levels = 0;      // tree levels above level 1 reduced in LDS per leaf workgroup (0: off)
+    int compose_subtree = 0;     // the same inside the composition kernel (its own limit; 0: off)
 
     // Digests per tail workgroup (log2) for a level of `count` digests.
     int tail_group_log(uint64_t count) const {
@@ -250,7 +253,9 @@ struct Schedule {
                (levels_fuse == 1 || levels_fuse == 2) && small_max_log >= 0 && small_max_log <= 10 &&
                (small_max_log == 0 || small_max_log >= 4) && grind_blocks >= 16 && grind_blocks <= (1 << 20) &&
                (fold_fuse_max_log == 0 || (fold_fuse_max_log >= 3 && fold_fuse_max_log <= 27)) &&
-               (compose_fuse == 0 || compose_fuse == 1) && (trace_fuse == 0 || trace_fuse == 1);
+               (compose_fuse == 0 || compose_fuse == 1) && (trace_fuse == 0 || trace_fuse == 1) &&
+               subtree_levels >= 0 && subtree_levels <= SUBTREE_MAX_LEVELS && compose_subtree >= 0 &&
+               compose_subtree <= SUBTREE_MAX_LEVELS;
     }
 };
 
@@ -413,6 +418,8 @@ struct Ctx {
     }
 
     const GeometryTables& geometry(int logn, int logb);
+    // The geometry's per-point inverses (PointInverses), built on first use.
+    const PointInverses& point_inverses(int logn, int logb);
 
     ~Ctx();
 };

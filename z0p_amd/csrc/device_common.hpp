@@ -163,6 +163,10 @@ __device__ __forceinline__ uint32_t point_m(PowTable w_m, const uint32_t* shift_
     return mont_mul(pow_lookup(w_m, t << logb), gc(shift_m)[j]);
 }
 
+// Word offset of level l in a level-contiguous tree whose level 0 has L
+// digests (l <= log2 L): 8 * (L + L/2 + ... + L/2^(l-1)).
+__device__ __forceinline__ uint64_t tree_level_words(uint64_t L, int l) { return 16 * (L - (L >> l)); }
+
 __device__ __forceinline__ void store_digest(uint32_t* dst, const uint32_t d[8]) {
     st4(dst, 0, u32x4{d[0], d[1], d[2], d[3]});
     st4(dst, 1, u32x4{d[4], d[5], d[6], d[7]});

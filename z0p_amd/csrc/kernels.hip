@@ -584,54 +584,96 @@ constexpr int ELT = 16;
 constexpr int CELT = 8;
 
 // B[i] = A[i + B] + coeff*(x^n - 1)/(x - w_last) (air.rs:806-844);
-// x^n - 1 = s_j^n - 1 is constant per coset (table zvan_m).
+// x^n - 1 = s_j^n - 1 is constant per coset (table zvan_m), 1/(x - w_last)
+// comes from the geometry's point table.
 __global__ __launch_bounds__(WG) void k_linked(LinkedJob inl, const LinkedJob* tab, uint32_t njobs) {
     uint32_t local;
     const LinkedJob J = job_of(inl, tab, njobs, local);
     const uint32_t m = 1u << (J.logn + J.logb);
     const uint32_t base = local * (WG * ELT) + threadIdx.x;
     const gcptr A = gc(J.a);
-    uint32_t den[ELT], an[ELT];
-    // The next-row loads are scattered inside the coset block: issue them
-    // first so their latency overlaps the batched inversion.
+    uint32_t an[ELT], iw[ELT];
+    // The next-row loads are scattered inside the coset block: all in
+    // flight before the arithmetic.
 #pragma unroll
     for (int e = 0; e < ELT; ++e) {
         const uint32_t p = base + e * WG;
         if (p < m) {
             const PosInfo pi = pos_info(p, J.logn, J.logb);
             an[e] = A[pi.pnext];
-            const uint32_t x = mont_mul(gc(J.xn)[p & ((1u << J.logn) - 1)], gc(J.shift_m)[pi.j]);
-            den[e] = sub(x, J.w_last_m);
-        } else {
-            an[e] = 0;
-            den[e] = R1;
+            iw[e] = gc(J.iw)[p];
         }
     }
-    batch_inv_m<ELT>(den);
 #pragma unroll
     for (int e = 0; e < ELT; ++e) {
         const uint32_t p = base + e * WG;
         if (p < m) {
             const uint32_t j = brev_bits(p >> J.logn, J.logb);
-            gm(J.b)[p] = add(an[e], mont_mul(den[e], mont_mul(J.coeff, gc(J.zvan_m)[j])));
+            gm(J.b)[p] = add(an[e], mont_mul(iw[e], mont_mul(J.coeff, gc(J.zvan_m)[j])));
         }
     }
 }
 
-// Composition value per point (air.rs:847-929), written as FRI layer 0:
+// Per-point inverses of a geometry (PointInverses): 8 positions per thread,
+// strided by the workgroup, with one batched inversion of (x - 1)(x - w).
+__global__ __launch_bounds__(WG) void k_point_inv(PointInvJob J) {
+    const uint32_t m = 1u << (J.logn + J.logb);
+    const uint32_t base = blockIdx.x * (WG * CELT) + threadIdx.x;
+    const uint32_t wl1 = sub(J.w_last_m, R1);
+    uint32_t d1[CELT], pr[CELT];
+#pragma unroll
+    for (int e = 0; e < CELT; ++e) {
+        const uint32_t p = base + e * WG;
+        if (p < m) {
+            const uint32_t j = brev_bits(p >> J.logn, J.logb);
+            const uint32_t x = mont_mul(gc(J.xn)[p & ((1u << J.logn) - 1)], gc(J.shift_m)[j]);
+            d1[e] = sub(x, R1);
+            pr[e] = mont_mul(d1[e], sub(d1[e], wl1));
+        } else {
+            d1[e] = pr[e] = R1;
+        }
+    }
+    batch_inv_m<CELT>(pr);
+#pragma unroll
+    for (int e = 0; e < CELT; ++e) {
+        const uint32_t p = base + e * WG;
+        if (p >= m) continue;
+        const uint32_t j = brev_bits(p >> J.logn, J.logb);
+        const uint32_t dw = sub(d1[e], wl1);  // x - w_last
+        gm(J.i1)[p] = mont_mul(dw, pr[e]);
+        gm(J.iw)[p] = mont_mul(d1[e], pr[e]);
+        gm(J.iz)[p] = mont_mul(dw, gc(J.zinv_m)[j]);
+    }
+}
+
+// One composition value (air.rs:847-929) from the trace values at the row
+// and the next row and the point's inverses:
 //   a0*(A'-B)/Z + a1*(B'-A-B)/Z + a2*(A-a0)/(x-1) + a3*(B-b0)/(x-1) + a4*(B-bl)/(x-w)
-// with 1/Z = (x-w)/(x^n-1) and x^n - 1 constant per coset.
+__device__ __forceinline__ void compose_value(const ComposeConsts& C, uint32_t va, uint32_t vb, uint32_t na,
+                                              uint32_t nb, uint32_t i1, uint32_t iw, uint32_t iz, uint32_t (&o)[4]) {
+    const uint32_t s0 = mont_mul(sub(na, vb), iz);
+    const uint32_t s1 = mont_mul(sub(nb, add(va, vb)), iz);
+    const uint32_t s2 = mont_mul(sub(va, C.a0), i1);
+    const uint32_t s3 = mont_mul(sub(vb, C.b0), i1);
+    const uint32_t s4 = mont_mul(sub(vb, C.b_last), iw);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t p01 = mont_mul2(s0, C.alpha_m[0][k], s1, C.alpha_m[1][k]);
+        const uint32_t p23 = mont_mul2(s2, C.alpha_m[2][k], s3, C.alpha_m[3][k]);
+        o[k] = add(add(p01, p23), mont_mul(s4, C.alpha_m[4][k]));
+    }
+}
+
+// Composition per point (air.rs:847-929), written as FRI layer 0 (every
+// load of an element in flight before its arithmetic).
 __global__ __launch_bounds__(WG) void k_compose(ComposeJob inl, const ComposeJob* tab, uint32_t njobs) {
     uint32_t local;
     const ComposeJob J = job_of(inl, tab, njobs, local);
     const uint32_t m = 1u << (J.logn + J.logb);
     const uint32_t base = local * (WG * CELT) + threadIdx.x;
     const gcptr A = gc(J.a), Bv = gc(J.b);
-    // x - w_last = (x - 1) - (w_last - 1): only x - 1 is kept per element.
     const ComposeConsts C = *J.c;
-    const uint32_t wl1 = sub(C.w_last_m, R1);
-    uint32_t d1[CELT], pr[CELT], na[CELT], nb[CELT];
-    // Scattered next-row loads first, overlapping the batched inversion.
+    uint32_t va[CELT], vb[CELT], na[CELT], nb[CELT], i1[CELT], iw[CELT], iz[CELT];
 #pragma unroll
     for (int e = 0; e < CELT; ++e) {
         const uint32_t p = base + e * WG;
@@ -639,39 +681,21 @@ __global__ __launch_bounds__(WG) void k_compose(ComposeJob inl, const ComposeJob
             const PosInfo pi = pos_info(p, J.logn, J.logb);
             na[e] = A[pi.pnext];
             nb[e] = Bv[pi.pnext];
-            const uint32_t x = mont_mul(gc(J.xn)[p & ((1u << J.logn) - 1)], gc(J.shift_m)[pi.j]);
-            d1[e] = sub(x, R1);
-            pr[e] = mont_mul(d1[e], sub(d1[e], wl1));
-        } else {
-            na[e] = nb[e] = 0;
-            d1[e] = pr[e] = R1;
+            va[e] = A[p];
+            vb[e] = Bv[p];
+            i1[e] = gc(J.pinv.i1)[p];
+            iw[e] = gc(J.pinv.iw)[p];
+            iz[e] = gc(J.pinv.iz)[p];
         }
     }
-    batch_inv_m<CELT>(pr);
-    static_for<0, CELT>([&](auto ec) {
-        constexpr int e = decltype(ec)::value;
-        const uint32_t p = base + e * WG;
-        if (p >= m) return;
-        const uint32_t j = brev_bits(p >> J.logn, J.logb);
-        const uint32_t dw = sub(d1[e], wl1);
-        const uint32_t i1 = mont_mul(dw, pr[e]);
-        const uint32_t iw = mont_mul(d1[e], pr[e]);
-        const uint32_t iz = mont_mul(dw, gc(J.zinv_m)[j]);
-        const uint32_t va = A[p], vb = Bv[p];
-        const uint32_t s0 = mont_mul(sub(na[e], vb), iz);
-        const uint32_t s1 = mont_mul(sub(nb[e], add(va, vb)), iz);
-        const uint32_t s2 = mont_mul(sub(va, C.a0), i1);
-        const uint32_t s3 = mont_mul(sub(vb, C.b0), i1);
-        const uint32_t s4 = mont_mul(sub(vb, C.b_last), iw);
-        uint32_t o[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t p01 = mont_mul2(s0, C.alpha_m[0][k], s1, C.alpha_m[1][k]);
-            const uint32_t p23 = mont_mul2(s2, C.alpha_m[2][k], s3, C.alpha_m[3][k]);
-            o[k] = add(add(p01, p23), mont_mul(s4, C.alpha_m[4][k]));
-        }
+    for (int e = 0; e < CELT; ++e) {
+        const uint32_t p = base + e * WG;
+        if (p >= m) continue;
+        uint32_t o[4];
+        compose_value(C, va[e], vb[e], na[e], nb[e], i1[e], iw[e], iz[e], o);
         st4(J.layer0, p, u32x4{o[0], o[1], o[2], o[3]});
-    });
+    }
 }
 
 // Range-check composition per point (range.rs:767-823), as FRI layer 0.
@@ -844,9 +868,23 @@ __global__ __launch_bounds__(WG) void k_fold_combine(FoldCombineJob inl, const F
 }
 
 // ---------------------------------------------------------------------------
+// Subtree reduction in LDS (defined after the quad-lane BLAKE3 below): a
+// workgroup's `cnt` digests of level `lvl` (cnt <= SUB_CNT, a power of two)
+// sit in `a` as a[k * SUB_CNT + i] = word k of digest i, the workgroup's
+// first digest being digest `first` of its level; `b` is laid out the same
+// way (8 * SUB_CNT words). Builds levels lvl + 1 .. lvl + nlev (cnt >> nlev
+// >= 1) of the level-contiguous tree whose level 0 at `tree` has `leaves`
+// digests, writing every node. Every thread of the workgroup must call it.
+constexpr int SUB_CNT = WG;
+constexpr int SUB_LDS_WORDS = 2 * 8 * SUB_CNT;
+__device__ void subtree_reduce(uint32_t* a, uint32_t* b, int cnt, int nlev, uint32_t* tree, uint64_t leaves, int lvl,
+                               uint64_t first);
+
+// ---------------------------------------------------------------------------
 // Merkle leaves: thread g hashes leaves 2g and 2g+1 and, when asked, their
-// parent (one node of level 1). No LDS: wide levels run at full occupancy.
-// Each leaf kind has its own fully unrolled body (static register indexing).
+// parent (one node of level 1); contiguous kinds then reduce their
+// workgroup's 256 parents further in LDS (sub_levels). Each leaf kind has
+// its own fully unrolled body (static register indexing).
 template <int KIND>
 __device__ __forceinline__ void leaf_message(const LeavesJob& J, uint64_t leaf, uint32_t (&m)[16]) {
     if (KIND == LEAF_BASE) {
@@ -896,9 +934,30 @@ __device__ __forceinline__ uint64_t trace_pair_of(const LeavesJob& J, uint64_t t
     return brev_bits(uint32_t(tau), logn - logs);
 }
 
+__device__ __forceinline__ void load_digest(const uint32_t* src, uint32_t d[8]) {
+    const u32x4 x = ld4(src, 0), y = ld4(src, 1);
+    d[0] = x.x; d[1] = x.y; d[2] = x.z; d[3] = x.w;
+    d[4] = y.x; d[5] = y.y; d[6] = y.z; d[7] = y.w;
+}
+
+// The level-1 digest this thread produced (thread t of the workgroup owns
+// level-1 digest first + t) goes to LDS, then the workgroup reduces
+// sub_levels more levels. Called by every thread of the workgroup.
+__device__ __forceinline__ void subtree_after_parent(uint32_t* sa, uint32_t* sb, const uint32_t (&p)[8], bool live,
+                                                     uint32_t* tree, uint64_t leaves, uint64_t first, int sub_levels) {
+    if (live) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sa[k * SUB_CNT + threadIdx.x] = p[k];
+    }
+    __syncthreads();
+    const uint64_t parents = leaves >> 1;
+    const int cnt = int(parents - first < uint64_t(SUB_CNT) ? parents - first : uint64_t(SUB_CNT));
+    subtree_reduce(sa, sb, cnt, sub_levels, tree, leaves, 1, first);
+}
+
 template <int KIND>
-__device__ __forceinline__ void leaves_body(const LeavesJob& J, uint64_t tau) {
-    uint32_t m[16], d0[8], d1[8];
+__device__ __forceinline__ void leaves_body(const LeavesJob& J, uint64_t tau, uint32_t* sa, uint32_t* sb) {
+    uint32_t m[16], d0[8], d1[8], p[8];
     if (J.leaves == 1) {
         if (tau != 0) return;
         leaf_message<KIND>(J, 0, m);
@@ -907,19 +966,33 @@ __device__ __forceinline__ void leaves_body(const LeavesJob& J, uint64_t tau) {
         return;
     }
     const uint64_t pairs = J.leaves >> 1;
-    if (tau >= pairs) return;
-    const uint64_t g = KIND == LEAF_TRACE ? trace_pair_of(J, tau) : tau;
-    leaf_message<KIND>(J, 2 * g, m);
-    leaf64_words(m, d0);
-    leaf_message<KIND>(J, 2 * g + 1, m);
-    leaf64_words(m, d1);
-    store_digest(J.levels + (2 * g) * 8, d0);
-    store_digest(J.levels + (2 * g + 1) * 8, d1);
-    if (J.with_parent) {
-        uint32_t p[8];
-        node_words(d0, d1, p);
-        store_digest(J.levels + (J.leaves + g) * 8, p);
+    const bool live = tau < pairs;
+    if (!live && J.sub_levels == 0) return;
+    if (live) {
+        const uint64_t g = KIND == LEAF_TRACE ? trace_pair_of(J, tau) : tau;
+        if (KIND == LEAF_NODES) {
+            uint32_t s0[8], s1[8];
+            load_digest(J.src + (4 * g) * 8, s0);
+            load_digest(J.src + (4 * g + 1) * 8, s1);
+            node_words(s0, s1, d0);
+            load_digest(J.src + (4 * g + 2) * 8, s0);
+            load_digest(J.src + (4 * g + 3) * 8, s1);
+            node_words(s0, s1, d1);
+        } else {
+            leaf_message<KIND>(J, 2 * g, m);
+            leaf64_words(m, d0);
+            leaf_message<KIND>(J, 2 * g + 1, m);
+            leaf64_words(m, d1);
+        }
+        store_digest(J.levels + (2 * g) * 8, d0);
+        store_digest(J.levels + (2 * g + 1) * 8, d1);
+        if (J.with_parent) {
+            node_words(d0, d1, p);
+            store_digest(J.levels + (J.leaves + g) * 8, p);
+        }
     }
+    if (KIND != LEAF_TRACE && J.sub_levels > 0)
+        subtree_after_parent(sa, sb, p, live, J.levels, J.leaves, tau - threadIdx.x, J.sub_levels);
 }
 
 // Trace leaves with the linked column computed in the same pass: a thread's
@@ -934,7 +1007,7 @@ __device__ __forceinline__ void leaves_linked(const LeavesJob& J, uint64_t tau) 
     const int logn = J.logn, logb = J.logb;
     const uint32_t bmask = (1u << logb) - 1, nmask = (1u << logn) - 1;
     const gcptr A = gc(J.src);
-    uint32_t a[16], an[16], den[16], pos[16], jj[16];
+    uint32_t a[16], an[16], iw[16], pos[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const uint64_t i = g * 16 + r;
@@ -942,17 +1015,15 @@ __device__ __forceinline__ void leaves_linked(const LeavesJob& J, uint64_t tau) 
         const uint32_t t = uint32_t(i >> logb);
         const uint32_t blk = brev_bits(j, logb) << logn;
         pos[r] = blk | brev_bits(t, logn);
-        jj[r] = j;
         a[r] = A[pos[r]];
         an[r] = A[blk | brev_bits((t + 1) & nmask, logn)];
-        const uint32_t x = mont_mul(gc(J.xn)[pos[r] & nmask], gc(J.shift_m)[j]);
-        den[r] = sub(x, J.w_last_m);
+        iw[r] = gc(J.iw)[pos[r]];
     }
-    batch_inv_m<16>(den);
     uint32_t m0[16], m1[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        const uint32_t b = add(an[r], mont_mul(den[r], mont_mul(J.coeff, gc(J.zvan_m)[jj[r]])));
+        const uint32_t j = uint32_t(g * 16 + r) & bmask;
+        const uint32_t b = add(an[r], mont_mul(iw[r], mont_mul(J.coeff, gc(J.zvan_m)[j])));
         gm(J.b_out)[pos[r]] = b;
         if (r < 8) {
             m0[2 * r] = a[r];
@@ -1027,21 +1098,31 @@ __device__ __forceinline__ void leaves_rows(const LeavesJob& J, uint64_t tau) {
     }
 }
 
+// Three kernels, so the light leaf kinds do not inherit the register
+// footprint (and occupancy) of the trace rows with the linked column.
 __global__ __launch_bounds__(WG) void k_leaves(LeavesJob inl, const LeavesJob* tab, uint32_t njobs) {
+    __shared__ uint32_t sub[SUB_LDS_WORDS];
     uint32_t local;
     const LeavesJob J = job_of(inl, tab, njobs, local);
     const uint64_t tau = uint64_t(local) * WG + threadIdx.x;
-    if (J.kind == LEAF_TRACE_LINKED) leaves_linked(J, tau);
-    else if (J.kind == LEAF_ROWS) leaves_rows(J, tau);
-    else if (J.kind == LEAF_TRACE) leaves_body<LEAF_TRACE>(J, tau);
-    else if (J.kind == LEAF_BASE) leaves_body<LEAF_BASE>(J, tau);
-    else leaves_body<LEAF_EXT>(J, tau);
+    uint32_t* sa = sub;
+    uint32_t* sb = sub + 8 * SUB_CNT;
+    if (J.kind == LEAF_TRACE) leaves_body<LEAF_TRACE>(J, tau, sa, sb);
+    else if (J.kind == LEAF_BASE) leaves_body<LEAF_BASE>(J, tau, sa, sb);
+    else if (J.kind == LEAF_NODES) leaves_body<LEAF_NODES>(J, tau, sa, sb);
+    else leaves_body<LEAF_EXT>(J, tau, sa, sb);
 }
 
-__device__ __forceinline__ void load_digest(const uint32_t* src, uint32_t d[8]) {
-    const u32x4 x = ld4(src, 0), y = ld4(src, 1);
-    d[0] = x.x; d[1] = x.y; d[2] = x.z; d[3] = x.w;
-    d[4] = y.x; d[5] = y.y; d[6] = y.z; d[7] = y.w;
+__global__ __launch_bounds__(WG) void k_leaves_linked(LeavesJob inl, const LeavesJob* tab, uint32_t njobs) {
+    uint32_t local;
+    const LeavesJob J = job_of(inl, tab, njobs, local);
+    leaves_linked(J, uint64_t(local) * WG + threadIdx.x);
+}
+
+__global__ __launch_bounds__(WG) void k_leaves_rows(LeavesJob inl, const LeavesJob* tab, uint32_t njobs) {
+    uint32_t local;
+    const LeavesJob J = job_of(inl, tab, njobs, local);
+    leaves_rows(J, uint64_t(local) * WG + threadIdx.x);
 }
 
 // Wide levels: thread g reduces digests 2g, 2g+1 (fuse = 1) or 4g..4g+3
@@ -1082,14 +1163,15 @@ __global__ __launch_bounds__(WG) void k_levels(LevelsJob inl, const LevelsJob* t
 // across lanes.
 constexpr int FUSE_VALUES = 8 * WG;
 constexpr int FUSE_LDS = FUSE_VALUES * 4 + FUSE_VALUES / 8;
+static_assert(FUSE_LDS >= SUB_LDS_WORDS, "staged values' LDS holds the subtree buffers");
 __device__ __forceinline__ int fuse_at(int q, int k) { return q * 4 + k + (q >> 3); }
 
 // Thread t hashes staged values 8t..8t+7 (leaves 2t, 2t+1 of the workgroup's
 // first leaf `leaf0`) and, with_parent, their parent into the next level.
-__device__ __forceinline__ void hash_staged(const uint32_t* lds, uint32_t* tree, uint64_t leaves, uint64_t leaf0,
-                                            int with_parent, int nvals) {
+__device__ __forceinline__ bool hash_staged(const uint32_t* lds, uint32_t* tree, uint64_t leaves, uint64_t leaf0,
+                                            int with_parent, int nvals, uint32_t (&par)[8]) {
     const int t = threadIdx.x;
-    if (8 * t >= nvals) return;
+    if (8 * t >= nvals) return false;
     uint32_t m0[16], m1[16];
 #pragma unroll
     for (int w = 0; w < 16; ++w) {
@@ -1103,10 +1185,19 @@ __device__ __forceinline__ void hash_staged(const uint32_t* lds, uint32_t* tree,
     store_digest(tree + leaf * 8, dg0);
     store_digest(tree + (leaf + 1) * 8, dg1);
     if (with_parent) {
-        uint32_t par[8];
         node_words(dg0, dg1, par);
         store_digest(tree + (leaves + leaf / 2) * 8, par);
     }
+    return true;
+}
+
+// After hash_staged: the staged values are no longer needed, so the same
+// LDS holds the subtree buffers (a barrier first: other threads may still
+// read their staged values).
+__device__ __forceinline__ void staged_subtree(uint32_t* lds, const uint32_t (&par)[8], bool live, uint32_t* tree,
+                                               uint64_t leaves, uint64_t leaf0, int sub_levels) {
+    __syncthreads();
+    subtree_after_parent(lds, lds + 8 * SUB_CNT, par, live, tree, leaves, leaf0 / 2, sub_levels);
 }
 
 // Composition fused with layer 0's Merkle leaves: the workgroup computes
@@ -1121,8 +1212,7 @@ __global__ __launch_bounds__(WG) void k_compose_leaves(ComposeJob inl, const Com
     const uint32_t base = first + threadIdx.x;
     const gcptr A = gc(J.a), Bv = gc(J.b);
     const ComposeConsts C = *J.c;
-    const uint32_t wl1 = sub(C.w_last_m, R1);
-    uint32_t d1[CELT], pr[CELT], na[CELT], nb[CELT];
+    uint32_t va[CELT], vb[CELT], na[CELT], nb[CELT], i1[CELT], iw[CELT], iz[CELT];
 #pragma unroll
     for (int e = 0; e < CELT; ++e) {
         const uint32_t p = base + e * WG;
@@ -1130,43 +1220,28 @@ __global__ __launch_bounds__(WG) void k_compose_leaves(ComposeJob inl, const Com
             const PosInfo pi = pos_info(p, J.logn, J.logb);
             na[e] = A[pi.pnext];
             nb[e] = Bv[pi.pnext];
-            const uint32_t x = mont_mul(gc(J.xn)[p & ((1u << J.logn) - 1)], gc(J.shift_m)[pi.j]);
-            d1[e] = sub(x, R1);
-            pr[e] = mont_mul(d1[e], sub(d1[e], wl1));
-        } else {
-            na[e] = nb[e] = 0;
-            d1[e] = pr[e] = R1;
+            va[e] = A[p];
+            vb[e] = Bv[p];
+            i1[e] = gc(J.pinv.i1)[p];
+            iw[e] = gc(J.pinv.iw)[p];
+            iz[e] = gc(J.pinv.iz)[p];
         }
     }
-    batch_inv_m<CELT>(pr);
     static_for<0, CELT>([&](auto ec) {
         constexpr int e = decltype(ec)::value;
         const uint32_t p = base + e * WG;
         if (p >= m) return;
-        const uint32_t j = brev_bits(p >> J.logn, J.logb);
-        const uint32_t dw = sub(d1[e], wl1);
-        const uint32_t i1 = mont_mul(dw, pr[e]);
-        const uint32_t iw = mont_mul(d1[e], pr[e]);
-        const uint32_t iz = mont_mul(dw, gc(J.zinv_m)[j]);
-        const uint32_t va = A[p], vb = Bv[p];
-        const uint32_t s0 = mont_mul(sub(na[e], vb), iz);
-        const uint32_t s1 = mont_mul(sub(nb[e], add(va, vb)), iz);
-        const uint32_t s2 = mont_mul(sub(va, C.a0), i1);
-        const uint32_t s3 = mont_mul(sub(vb, C.b0), i1);
-        const uint32_t s4 = mont_mul(sub(vb, C.b_last), iw);
         uint32_t o[4];
+        compose_value(C, va[e], vb[e], na[e], nb[e], i1[e], iw[e], iz[e], o);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t p01 = mont_mul2(s0, C.alpha_m[0][k], s1, C.alpha_m[1][k]);
-            const uint32_t p23 = mont_mul2(s2, C.alpha_m[2][k], s3, C.alpha_m[3][k]);
-            o[k] = add(add(p01, p23), mont_mul(s4, C.alpha_m[4][k]));
-            lds[fuse_at(int(threadIdx.x) + e * WG, k)] = o[k];
-        }
+        for (int k = 0; k < 4; ++k) lds[fuse_at(int(threadIdx.x) + e * WG, k)] = o[k];
         st4(J.layer0, p, u32x4{o[0], o[1], o[2], o[3]});
     });
     __syncthreads();
     const int nvals = int(m - first < uint32_t(FUSE_VALUES) ? m - first : uint32_t(FUSE_VALUES));
-    hash_staged(lds, J.tree, m >> 2, first >> 2, J.with_parent, nvals);
+    uint32_t par[8];
+    const bool live = hash_staged(lds, J.tree, m >> 2, first >> 2, J.with_parent, nvals, par);
+    if (J.sub_levels > 0) staged_subtree(lds, par, live, J.tree, m >> 2, first >> 2, J.sub_levels);
 }
 
 // ---- quad-lane BLAKE3 for the narrow tree levels ------------------------
@@ -1264,6 +1339,56 @@ __device__ __forceinline__ void quad_node(const uint32_t (&l)[8], const uint32_t
     quad_compress(ha, hb, m2, q, 1, b3::CHUNK_END | b3::ROOT);
     oa = ha;
     ob = hb;
+}
+
+__device__ void subtree_reduce(uint32_t* a, uint32_t* b, int cnt, int nlev, uint32_t* tree, uint64_t leaves, int lvl,
+                               uint64_t first) {
+    const int t = threadIdx.x;
+    uint32_t* src = a;
+    uint32_t* dst = b;
+    uint64_t f = first;
+    for (int r = 1; r <= nlev; ++r) {
+        const int nodes = cnt >> r;
+        f >>= 1;
+        uint32_t* out = tree + tree_level_words(leaves, lvl + r) + f * 8;
+        if (nodes > WG / 8) {
+            // Wide: one lane per node (the fewest instructions per node).
+            for (int i = t; i < nodes; i += WG) {
+                uint32_t L[8], R[8], o[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    L[k] = src[k * SUB_CNT + 2 * i];
+                    R[k] = src[k * SUB_CNT + 2 * i + 1];
+                }
+                node_words(L, R, o);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) dst[k * SUB_CNT + i] = o[k];
+                store_digest(out + uint64_t(i) * 8, o);
+            }
+        } else {
+            // Narrow: a quad of lanes per node cuts the dependent chain.
+            const int g = t >> 2;
+            const uint32_t q = uint32_t(t & 3);
+            if (g < nodes) {  // quad-uniform
+                uint32_t L[8], R[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    L[k] = src[k * SUB_CNT + 2 * g];
+                    R[k] = src[k * SUB_CNT + 2 * g + 1];
+                }
+                uint32_t oa, ob;
+                quad_node(L, R, q, oa, ob);
+                dst[q * SUB_CNT + g] = oa;
+                dst[(4 + q) * SUB_CNT + g] = ob;
+                gm(out)[uint64_t(g) * 8 + q] = oa;
+                gm(out)[uint64_t(g) * 8 + 4 + q] = ob;
+            }
+        }
+        __syncthreads();
+        uint32_t* tmp = src;
+        src = dst;
+        dst = tmp;
+    }
 }
 
 // Narrow levels: up to group_log levels per 2^group_log-digest group in LDS,
@@ -1382,7 +1507,9 @@ __global__ __launch_bounds__(WG) void k_fold_leaves(FoldJob inl, const FoldJob* 
     }
     __syncthreads();
     const int nvals = int(outs - first < uint64_t(FUSE_VALUES) ? outs - first : uint64_t(FUSE_VALUES));
-    hash_staged(lds, J.tree, outs >> 2, first >> 2, J.with_parent, nvals);
+    uint32_t par[8];
+    const bool live = hash_staged(lds, J.tree, outs >> 2, first >> 2, J.with_parent, nvals, par);
+    if (J.sub_levels > 0) staged_subtree(lds, par, live, J.tree, outs >> 2, first >> 2, J.sub_levels);
 }
 
 // ---------------------------------------------------------------------------
@@ -1787,6 +1914,12 @@ void launch_linked(const Launch& L, std::vector<LinkedJob> jobs) {
     run(L, "linked_b", k_linked, jobs, g, 0);
 }
 
+void launch_point_inv(const Launch& L, const PointInvJob& job) {
+    if (L.dry) return;
+    const uint64_t m = uint64_t(1) << (job.logn + job.logb);
+    hipLaunchKernelGGL(k_point_inv, dim3(grid_for(m, WG * CELT)), dim3(WG), 0, L.stream, job);
+}
+
 void launch_compose(const Launch& L, std::vector<ComposeJob> all) {
     std::vector<ComposeJob> plain, fused;
     for (const ComposeJob& j : all) (j.tree ? fused : plain).push_back(j);
@@ -1825,16 +1958,22 @@ void launch_fold_combine(const Launch& L, std::vector<FoldCombineJob> jobs) {
     run(L, "fold_combine", k_fold_combine, jobs, g, 0);
 }
 
-void launch_leaves(const Launch& L, std::vector<LeavesJob> jobs) {
-    bool trace = false, rows = false;
-    for (const LeavesJob& j : jobs) {
-        trace = trace || j.kind == LEAF_TRACE || j.kind == LEAF_TRACE_LINKED;
-        rows = rows || j.kind == LEAF_ROWS;
+void launch_leaves(const Launch& L, std::vector<LeavesJob> all) {
+    std::vector<LeavesJob> linked, rows, plain;
+    bool trace = false, nodes_only = true;
+    for (const LeavesJob& j : all) {
+        if (j.kind == LEAF_TRACE_LINKED) linked.push_back(j);
+        else if (j.kind == LEAF_ROWS) rows.push_back(j);
+        else plain.push_back(j);
+        trace = trace || j.kind == LEAF_TRACE;
+        nodes_only = nodes_only && j.kind == LEAF_NODES;
     }
-    unsigned g = assign_blocks(jobs, [](const LeavesJob& j) {
-        return j.leaves == 1 ? 1u : grid_for(j.leaves / 2, WG);
-    });
-    run(L, rows ? "row_leaves" : trace ? "trace_leaves" : "fri_leaves", k_leaves, jobs, g, 0);
+    auto blocks = [](const LeavesJob& j) { return j.leaves == 1 ? 1u : grid_for(j.leaves / 2, WG); };
+    if (!linked.empty()) run(L, "trace_leaves", k_leaves_linked, linked, assign_blocks(linked, blocks), 0);
+    if (!rows.empty()) run(L, "row_leaves", k_leaves_rows, rows, assign_blocks(rows, blocks), 0);
+    if (!plain.empty())
+        run(L, nodes_only ? "tree_nodes" : trace ? "trace_leaves" : "fri_leaves", k_leaves, plain,
+            assign_blocks(plain, blocks), 0);
 }
 
 void launch_levels(const Launch& L, std::vector<LevelsJob> jobs) {

@@ -97,14 +97,32 @@ struct NttJob {
     uint32_t block_begin;
 };
 
+// Per-point inverses of an AIR geometry (trace length n, m LDE points),
+// Montgomery form, in the bit-reversed LDE order, for x_p the point at
+// position p and w_last = omega_n^(n-1):
+//   i1[p] = 1/(x_p - 1), iw[p] = 1/(x_p - w_last), iz[p] = (x_p - w_last)/(x_p^n - 1).
+// They depend on the domain only, like the twiddle tables.
+struct PointInverses {
+    const uint32_t* i1 = nullptr;
+    const uint32_t* iw = nullptr;
+    const uint32_t* iz = nullptr;
+};
+struct PointInvJob {
+    uint32_t *i1, *iw, *iz;
+    const uint32_t* xn;
+    const uint32_t* shift_m;
+    const uint32_t* zinv_m;
+    uint32_t w_last_m;
+    int logn, logb;
+};
+void launch_point_inv(const Launch& L, const PointInvJob& job);
+
 struct LinkedJob {
     const uint32_t* a;
     uint32_t* b;
-    const uint32_t* xn;  // [n] omega_n^brev(q), Montgomery (point = shift_j * xn[q])
-    const uint32_t* shift_m;
+    const uint32_t* iw;      // [m] 1/(x_p - w_last), Montgomery (PointInverses)
     const uint32_t* zvan_m;  // [B] s_j^n - 1, Montgomery
     uint32_t coeff;          // (b_last - a0) w_last / n, canonical
-    uint32_t w_last_m;
     int logn, logb;
     uint32_t block_begin;
 };
@@ -118,9 +136,7 @@ struct ComposeJob {
     const uint32_t* a;
     const uint32_t* b;
     uint32_t* layer0;
-    const uint32_t* xn;  // [n] omega_n^brev(q), Montgomery
-    const uint32_t* shift_m;
-    const uint32_t* zinv_m;  // [B] 1/(s_j^n - 1)
+    PointInverses pinv;      // 1/(x - 1), 1/(x - w_last), 1/Z per position
     const ComposeConsts* c;  // device memory: the alphas come from the transcript
     int logn, logb;
     uint32_t block_begin;
@@ -129,6 +145,7 @@ struct ComposeJob {
     // thread, so the values are hashed while still in registers.
     uint32_t* tree = nullptr;
     int with_parent = 0;
+    int sub_levels = 0;  // levels above level 1 reduced in LDS per workgroup
 };
 
 // Range-check composition (range.rs:767-823), written as FRI layer 0:
@@ -219,7 +236,23 @@ void launch_fold_open(const Launch& L, const FoldOpenJob& job);
 // LEAF_ROWS: one leaf per LDE row of ncols bit-reversed columns (col_stride
 // words apart), BLAKE3(0x00 || row words LE) at the row's natural index
 // (merkle.rs:62-79 with hash.rs:352-382): the range AIR's joint row tree.
-enum LeafKind : int { LEAF_TRACE = 0, LEAF_EXT = 1, LEAF_BASE = 2, LEAF_RAW = 3, LEAF_TRACE_LINKED = 4, LEAF_ROWS = 5 };
+// LEAF_NODES: the "leaves" are the nodes of a tree level above level h: leaf
+// i = node(src digest 2i, src digest 2i + 1), src being level h; `levels`
+// then points at level h + 1 and `leaves` counts its nodes.
+enum LeafKind : int {
+    LEAF_TRACE = 0,
+    LEAF_EXT = 1,
+    LEAF_BASE = 2,
+    LEAF_RAW = 3,
+    LEAF_TRACE_LINKED = 4,
+    LEAF_ROWS = 5,
+    LEAF_NODES = 6
+};
+// Kinds whose workgroups own contiguous leaves (subtree fusion possible).
+constexpr bool leaf_contiguous(int kind) { return kind == LEAF_EXT || kind == LEAF_BASE || kind == LEAF_RAW || kind == LEAF_NODES; }
+// A workgroup reduces at most this many levels above level 1 in LDS (its 256
+// level-1 digests down to one).
+constexpr int SUBTREE_MAX_LEVELS = 8;
 struct LeavesJob {
     const uint32_t* src;
     const uint32_t* src_b;  // trace column B
@@ -229,13 +262,14 @@ struct LeavesJob {
     int with_parent;        // also write level 1 (needs leaves >= 2)
     int logn, logb;         // trace geometry
     uint32_t block_begin;
+    // Contiguous kinds with with_parent: levels 2 .. 1 + sub_levels too, each
+    // workgroup reducing its 256 level-1 digests in LDS (0: none).
+    int sub_levels = 0;
     // LEAF_TRACE_LINKED (leaves >= 2): B = A[next row] + coeff (x^n - 1)/(x - w_last).
     uint32_t* b_out = nullptr;
-    const uint32_t* xn = nullptr;       // [n] omega_n^brev(q), Montgomery
-    const uint32_t* shift_m = nullptr;  // [B] s_j, Montgomery
+    const uint32_t* iw = nullptr;       // [m] 1/(x_p - w_last), Montgomery (PointInverses)
     const uint32_t* zvan_m = nullptr;   // [B] s_j^n - 1, Montgomery
     uint32_t coeff = 0;                 // (b_last - a0) w_last / n, canonical
-    uint32_t w_last_m = 0;
     // LEAF_ROWS: src holds ncols columns of col_stride words each.
     int ncols = 0;
     uint64_t col_stride = 0;
@@ -297,6 +331,7 @@ struct FoldJob {
     // (parents into level 1 when with_parent), eight outputs per thread.
     uint32_t* tree = nullptr;
     int with_parent = 0;
+    int sub_levels = 0;  // levels above level 1 reduced in LDS per workgroup
 };
 
 // Fold of layer d into a small layer d+1 plus its whole Merkle commitment

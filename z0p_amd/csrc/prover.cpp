@@ -160,6 +160,34 @@ const GeometryTables& Ctx::geometry(int logn, int logb) {
     return *raw;
 }
 
+const PointInverses& Ctx::point_inverses(int logn, int logb) {
+    const GeometryTables& g = geometry(logn, logb);
+    GeometryTables& t = *tables.at(std::make_pair(logn, logb));
+    if (t.pinv.i1) return t.pinv;
+    const uint64_t m = uint64_t(1) << (logn + logb);
+    void *i1 = nullptr, *iw = nullptr, *iz = nullptr;
+    hip_check(hipMalloc(&i1, m * 4));
+    t.owned.push_back(i1);
+    hip_check(hipMalloc(&iw, m * 4));
+    t.owned.push_back(iw);
+    hip_check(hipMalloc(&iz, m * 4));
+    t.owned.push_back(iz);
+    PointInvJob j{};
+    j.i1 = static_cast<uint32_t*>(i1);
+    j.iw = static_cast<uint32_t*>(iw);
+    j.iz = static_cast<uint32_t*>(iz);
+    j.xn = g.xn;
+    j.shift_m = g.coset.shift;
+    j.zinv_m = g.zinv;
+    j.w_last_m = to_mont(pow_c(root_of_unity(uint32_t(logn)), (uint64_t(1) << logn) - 1));
+    j.logn = logn;
+    j.logb = logb;
+    launch_point_inv(launch(), j);
+    hip_check(hipStreamSynchronize(stream));
+    t.pinv = PointInverses{j.i1, j.iw, j.iz};
+    return t.pinv;
+}
+
 Ctx::~Ctx() {
     tables.clear();
     if (stream) (void)hipStreamSynchronize(stream);
@@ -273,12 +301,35 @@ TreeGeom tree_geom(uint64_t leaves, uint64_t cap_height) {
     return g;
 }
 
+// Levels a leaf (or node) workgroup reduces in LDS above the level holding
+// its parents: its share of that level is min(256, parents) digests, and the
+// reduction stops at the cap level (`remaining` levels left) and at the
+// schedule's limit.
+int subtree_levels_for(uint64_t parents, int remaining, const Schedule& s) {
+    int k = 0;
+    while (k < SUBTREE_MAX_LEVELS && (uint64_t(2) << k) <= std::min<uint64_t>(parents, 256)) ++k;
+    return std::max(0, std::min({k, remaining, s.subtree_levels}));
+}
+
 void tree_levels(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, int have, const Schedule& s) {
     while (have < g.target) {
         const uint64_t count = g.leaves >> have;
         const int remaining = g.target - have;
         uint32_t* lvl = tree + g.level_offset_words(have);
-        if (count > (uint64_t(1) << s.tail_max_log)) {
+        if (count > (uint64_t(1) << s.tail_max_log) && s.subtree_levels > 0 && count >= 8) {
+            // Wide levels in throughput mode: each workgroup turns 1024
+            // digests into two levels of nodes, then reduces its 256 nodes of
+            // the second level further in LDS.
+            LeavesJob nj{};
+            nj.kind = LEAF_NODES;
+            nj.src = lvl;
+            nj.levels = tree + g.level_offset_words(have + 1);
+            nj.leaves = count / 2;
+            nj.with_parent = remaining >= 2 ? 1 : 0;
+            nj.sub_levels = nj.with_parent ? subtree_levels_for(count / 4, remaining - 2, s) : 0;
+            chain.push_back(nj);
+            have += 1 + nj.with_parent + nj.sub_levels;
+        } else if (count > (uint64_t(1) << s.tail_max_log)) {
             const int fuse = std::min(s.levels_fuse, remaining);
             chain.push_back(LevelsJob{lvl, count, fuse, 0});
             have += fuse;
@@ -295,8 +346,11 @@ void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, Lea
     leaves.levels = tree;
     leaves.leaves = g.leaves;
     leaves.with_parent = g.target >= 1 ? 1 : 0;
+    leaves.sub_levels = leaves.with_parent && leaf_contiguous(leaves.kind)
+                            ? subtree_levels_for(g.leaves / 2, g.target - 1, s)
+                            : 0;
     chain.push_back(leaves);
-    tree_levels(chain, tree, g, leaves.with_parent, s);
+    tree_levels(chain, tree, g, leaves.with_parent + leaves.sub_levels, s);
 }
 
 namespace {
@@ -508,6 +562,7 @@ struct Proof {
     int logn = 0, logb = 0;
     size_t folds = 0;
     const GeometryTables* gt = nullptr;
+    const PointInverses* pinv = nullptr;  // AIR: the geometry's per-point inverses
     TreeGeom tg;                     // AIR trace tree
     std::vector<uint64_t> lens;      // FRI layer d length, d = 0..folds
     std::vector<TreeGeom> geoms;     // FRI layer d tree, d < folds
@@ -638,6 +693,7 @@ void init_air_shape(Ctx& c, Proof& P, const z0p_air_params& p, Transcript* tr, u
     P.w_last = pow_c(root_of_unity(uint32_t(P.logn)), P.n - 1);
     P.coeff_c = mul(mul(sub(actual_b_last, P.a0), P.w_last), inv_c(uint32_t(P.n % P_MOD)));
     P.gt = &c.geometry(P.logn, P.logb);
+    P.pinv = &c.point_inverses(P.logn, P.logb);
     P.mem_bytes = layout(P, nullptr);
 }
 
@@ -1043,20 +1099,16 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                     // The linked column computed by the trace-leaves kernel.
                     t.kind = LEAF_TRACE_LINKED;
                     t.b_out = P.Bc;
-                    t.xn = gt.xn;
-                    t.shift_m = gt.coset.shift;
+                    t.iw = P.pinv->iw;
                     t.zvan_m = gt.zvan;
                     t.coeff = P.coeff_c;
-                    t.w_last_m = to_mont(P.w_last);
                 } else {
                     LinkedJob lj{};
                     lj.a = P.A;
                     lj.b = P.Bc;
-                    lj.xn = gt.xn;
-                    lj.shift_m = gt.coset.shift;
+                    lj.iw = P.pinv->iw;
                     lj.zvan_m = gt.zvan;
                     lj.coeff = P.coeff_c;
-                    lj.w_last_m = to_mont(P.w_last);
                     lj.logn = P.logn;
                     lj.logb = P.logb;
                     ch.push_back(lj);
@@ -1158,9 +1210,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
             cj.a = P.A;
             cj.b = P.Bc;
             cj.layer0 = P.vals[0];
-            cj.xn = gt.xn;
-            cj.shift_m = gt.coset.shift;
-            cj.zinv_m = gt.zinv;
+            cj.pinv = *P.pinv;
             cj.c = static_cast<const ComposeConsts*>(cst ? cst : c.ring.push(&P.cc, sizeof(P.cc)));
             cj.logn = P.logn;
             cj.logb = P.logb;
@@ -1168,8 +1218,12 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 // Layer 0's leaves (and their parents) hashed by the composition kernel.
                 cj.tree = P.trees[0];
                 cj.with_parent = P.geoms[0].target >= 1 ? 1 : 0;
+                Schedule cs = c.sched;
+                cs.subtree_levels = c.sched.compose_subtree;
+                cj.sub_levels = cj.with_parent ? subtree_levels_for(P.geoms[0].leaves / 2, P.geoms[0].target - 1, cs)
+                                               : 0;
                 ch.push_back(cj);
-                tree_levels(ch, P.trees[0], P.geoms[0], cj.with_parent, c.sched);
+                tree_levels(ch, P.trees[0], P.geoms[0], cj.with_parent + cj.sub_levels, c.sched);
             } else {
                 ch.push_back(cj);
                 tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_EXT, P.vals[0]), c.sched);
@@ -1211,8 +1265,11 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 // Layer d+1's leaves (and parents) hashed by the fold kernel.
                 fj.tree = P.trees[d + 1];
                 fj.with_parent = P.geoms[d + 1].target >= 1 ? 1 : 0;
+                fj.sub_levels = fj.with_parent ? subtree_levels_for(P.geoms[d + 1].leaves / 2,
+                                                                    P.geoms[d + 1].target - 1, c.sched)
+                                               : 0;
                 ch.push_back(fj);
-                tree_levels(ch, P.trees[d + 1], P.geoms[d + 1], fj.with_parent, c.sched);
+                tree_levels(ch, P.trees[d + 1], P.geoms[d + 1], fj.with_parent + fj.sub_levels, c.sched);
                 collect_cap(c, P, P.trees[d + 1], P.geoms[d + 1], items, used);
             } else {
                 ch.push_back(fj);

@@ -173,11 +173,9 @@ int z0p_test_linked(z0p_ctx* ctx, const uint32_t* a, size_t n, size_t blowup, ui
         LinkedJob lj{};
         lj.a = da;
         lj.b = db;
-        lj.xn = gt.xn;
-        lj.shift_m = gt.coset.shift;
+        lj.iw = c.point_inverses(logn, logb).iw;
         lj.zvan_m = gt.zvan;
         lj.coeff = coeff;
-        lj.w_last_m = to_mont(w_last);
         lj.logn = logn;
         lj.logb = logb;
         run_chain(c, {lj});
@@ -194,7 +192,6 @@ int z0p_test_compose(z0p_ctx* ctx, const uint32_t* a, const uint32_t* b, size_t 
         Ctx& c = *ctx->c;
         int logn = int(ilog2(n)), logb = int(ilog2(blowup));
         size_t m = n * blowup;
-        const GeometryTables& gt = c.geometry(logn, logb);
         std::vector<uint32_t> abr(m), bbr(m), o(4 * m);
         for (size_t i = 0; i < m; ++i) {
             uint64_t p = br_pos(i, logn, logb);
@@ -216,9 +213,7 @@ int z0p_test_compose(z0p_ctx* ctx, const uint32_t* a, const uint32_t* b, size_t 
         cj.a = da;
         cj.b = db;
         cj.layer0 = dl;
-        cj.xn = gt.xn;
-        cj.shift_m = gt.coset.shift;
-        cj.zinv_m = gt.zinv;
+        cj.pinv = c.point_inverses(logn, logb);
         cj.c = to_dev(c, &cc, 1);
         cj.logn = logn;
         cj.logb = logb;

@@ -73,6 +73,8 @@ TuneResult autotune(Ctx& c, const std::vector<z0p_air_params>& work, unsigned re
         {"fold_fuse_max_log", &Schedule::fold_fuse_max_log, {0, 12, 14, 16, 18, 22}},
         {"compose_fuse", &Schedule::compose_fuse, {0, 1}},
         {"trace_fuse", &Schedule::trace_fuse, {0, 1}},
+        {"subtree_levels", &Schedule::subtree_levels, {0, 2, 4, 6, 8}},
+        {"compose_subtree", &Schedule::compose_subtree, {0, 1, 2, 4}},
     };
     // A candidate must beat the incumbent by this fraction to replace it, so
     // run-to-run noise does not move the schedule.
