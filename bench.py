@@ -25,6 +25,9 @@ P = 2013265921
 MASK64 = (1 << 64) - 1
 LABEL = b"bench/air-scale"
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: 256 CUs, one wave64 VALU instruction per CU per clock
+# (4 SIMDs x 16 lanes), 2.4 GHz peak engine clock.
+VALU_PEAK_GINST = 256 * 2.4
 # The reference's published figure for this metric: 2^20 steps proved in
 # 1,733.55 ms on 16 CPU cores (tools/baseline.json:108-112), 1.653 us/step.
 PUBLISHED_US_PER_STEP = 1.653
@@ -37,8 +40,10 @@ def tree_work(m, layers, cap_height, sched):
     hashes the next layer's leaves and parents for layers of at most
     2^fold_fuse_max_log values; a layer of at most 2^small_max_log values is
     folded and its whole tree built by one kernel (both "fold"). The rest is
-    the "merkle" family. Bytes count the leaf input once and every digest
-    written once; a leaf is one compression, a node two."""
+    the "merkle" family. Bytes count every digest written once and the
+    64-byte leaf input once where the hashing kernel reads it from memory (a
+    fused kernel hashes values it has just computed, so reads nothing more);
+    a leaf is one compression, a node two."""
     cap = 1 << cap_height
     work = {}
 
@@ -51,7 +56,8 @@ def tree_work(m, layers, cap_height, sched):
         # `fam` hashes the leaves (and their parents, or the whole tree);
         # the levels left over run in the "merkle" family.
         top = min(cap, leaves)  # digests at the cap level
-        add(fam, leaves, (64 + 32) * leaves)  # 64-byte leaves
+        fused = fam != "merkle"
+        add(fam, leaves, (32 if fused else 64 + 32) * leaves)
         nodes = leaves - top
         if parents_too and leaves > top:
             add(fam, leaves, 32 * (leaves // 2))
@@ -91,12 +97,25 @@ def algorithmic_bytes(n, m, layers, cap_height, sched):
     return alg, {fam: comp for fam, (comp, _) in work.items()}
 
 
+def kernel_alg_bytes(m, sched):
+    """Algorithmic HBM bytes of one launch of the kernel that dominates the
+    proof, from the per-unit figures of the path (u32 words, Ext4 = 16 bytes,
+    digests = 32 bytes; every array read or written once): the composition
+    reads A and B (8 bytes per LDE point) and writes layer 0 (16 bytes per
+    point); fused with the first FRI tree (compose_leaves) it also writes
+    the m/4 quad-leaf digests and their m/8 parents, 36 bytes per point."""
+    if sched.get("compose_fuse", 1):
+        return {"compose_leaves": 8 * m + 16 * m + 32 * (m // 4) + 32 * (m // 8)}
+    return {"compose": 8 * m + 16 * m}
+
+
 # Kernel launch names -> families. Tree level kernels are shared by the
 # trace and FRI trees and are reported as one "merkle" family beside them.
 FAMILY = {
     "intt": "intt", "lde": "lde", "linked_b": "linked_b", "compose": "compose", "compose_leaves": "compose",
     "fold_leaves": "fold",
     "trace_leaves": "merkle", "fri_leaves": "merkle", "tree_levels": "merkle", "tree_tail": "merkle",
+    "tree_nodes": "merkle",
     "fold": "fold", "small_commit": "fold",
 }
 
@@ -438,7 +457,93 @@ def widened(z, ctx, reps, with_cpu):
     return out
 
 
+def rocprof_summary(d, proofs, copy_mib):
+    """Per-kernel summary of the rocprofv3 runs under `d`: `trace/` (kernel
+    trace), `pmcA/` (SQ_WAVES SQ_INSTS_VALU SQ_ACTIVE_INST_VALU
+    SQ_THREAD_CYCLES_VALU SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR
+    SQ_LDS_BANK_CONFLICT SQ_BUSY_CYCLES GRBM_GUI_ACTIVE), `fetch/`
+    (FETCH_SIZE) and `write/` (WRITE_SIZE), each from one bench run that
+    proves `proofs` times and copies `copy_mib` MiB twice with k_copy.
+    FETCH_SIZE / WRITE_SIZE are KiB per dispatch, scaled so that k_copy
+    reads and writes exactly its buffer. Returns the JSON object the bench's
+    roofline reads (keyed by kernel symbol) and a markdown table."""
+    import collections
+    import csv
+    import re
+
+    def short(name):
+        mt = re.search(r"(k_\w+|__amd_rocclr_\w+)", name)
+        return mt.group(1) if mt else name[:40]
+
+    durs = collections.defaultdict(list)
+    with open(os.path.join(d, "trace", "run_kernel_trace.csv")) as fh:
+        for r in csv.DictReader(fh):
+            durs[short(r["Kernel_Name"])].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    sums = collections.defaultdict(lambda: collections.defaultdict(float))
+    disp = collections.defaultdict(lambda: collections.defaultdict(set))
+    for sub in ("pmcA", "fetch", "write"):
+        with open(os.path.join(d, sub, "run_counter_collection.csv")) as fh:
+            for r in csv.DictReader(fh):
+                k = short(r["Kernel_Name"])
+                sums[k][r["Counter_Name"]] += float(r["Counter_Value"])
+                disp[k][sub].add(int(r["Dispatch_Id"]))
+    copy_kib = copy_mib * 1024
+
+    def per(k, c, sub):
+        n = len(disp[k][sub])
+        return sums[k].get(c, 0.0) / n if n else 0.0
+
+    fscale = copy_kib / per("k_copy", "FETCH_SIZE", "fetch")
+    wscale = copy_kib / per("k_copy", "WRITE_SIZE", "write")
+    kernels = {}
+    for k, ds in durs.items():
+        avg_ns = sum(ds) / len(ds)
+        valu = per(k, "SQ_INSTS_VALU", "pmcA")
+        grbm = per(k, "GRBM_GUI_ACTIVE", "pmcA")  # summed over the 8 XCDs
+        active = per(k, "SQ_ACTIVE_INST_VALU", "pmcA")
+        kernels[k] = {
+            "dispatches_per_proof": round(len(ds) / proofs, 2),
+            "avg_us": round(avg_ns / 1e3, 3),
+            "us_per_proof": round(sum(ds) / 1e3 / proofs, 2),
+            "valu_insts_per_launch": round(valu),
+            "valu_issue_frac": round(valu / (256 * grbm / 8), 4) if grbm else None,
+            "valu_lane_util": round(per(k, "SQ_THREAD_CYCLES_VALU", "pmcA") / (64 * active), 4) if active else None,
+            "waves_per_launch": round(per(k, "SQ_WAVES", "pmcA")),
+            "lds_bank_conflict_cycles_per_launch": round(per(k, "SQ_LDS_BANK_CONFLICT", "pmcA")),
+            "fetch_bytes_per_launch": round(per(k, "FETCH_SIZE", "fetch") * fscale * 1024),
+            "write_bytes_per_launch": round(per(k, "WRITE_SIZE", "write") * wscale * 1024),
+        }
+    order = sorted(kernels, key=lambda k: -kernels[k]["us_per_proof"])
+    lines = ["| kernel | launches / proof | avg us | us / proof | VALU Minst / launch | VALU issue | "
+             "lane util | fetch MiB / launch | write MiB / launch |",
+             "|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    for k in order:
+        v = kernels[k]
+        f = lambda x: "-" if x is None else "%.3f" % x
+        lines.append("| %s | %g | %.2f | %.1f | %.2f | %s | %s | %.2f | %.2f |" % (
+            k, v["dispatches_per_proof"], v["avg_us"], v["us_per_proof"], v["valu_insts_per_launch"] / 1e6,
+            f(v["valu_issue_frac"]), f(v["valu_lane_util"]), v["fetch_bytes_per_launch"] / 2**20,
+            v["write_bytes_per_launch"] / 2**20))
+    obj = {"fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4), "proofs": proofs,
+           "kernels": {k: kernels[k] for k in order}}
+    return obj, "\n".join(lines)
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--rocprof-summary":
+        sp = argparse.ArgumentParser()
+        sp.add_argument("--rocprof-summary", required=True, metavar="DIR")
+        sp.add_argument("--proofs", type=int, default=5)
+        sp.add_argument("--copy-mib", type=int, default=256)
+        sp.add_argument("--json", default=None, help="write the per-kernel JSON here")
+        a = sp.parse_args()
+        obj, table = rocprof_summary(a.rocprof_summary, a.proofs, a.copy_mib)
+        if a.json:
+            with open(a.json, "w") as fh:
+                json.dump(obj, fh, indent=1)
+        print("fetch x%.4f, write x%.4f" % (obj["fetch_scale"], obj["write_scale"]))
+        print(table)
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -480,22 +585,30 @@ def main():
         single_tune = tune(ctx, [params], args.tune_reps)
     for _ in range(args.warmup):
         proof = z.air.prove(params, z.Transcript(LABEL), ctx)
-    # Timed region: plain proofs, no per-kernel events (each call returns
-    # the serialized proof, so the device is idle at both ends). The
-    # scheduler's host intervals are always recorded.
-    ctx.reset_stats()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        proof = z.air.prove(params, z.Transcript(LABEL), ctx)
-    t1 = time.perf_counter()
-    host = {k.lstrip("~"): round(v[1] / args.steps, 4) for k, v in ctx.kernel_stats().items() if k.startswith("~")}
-    # Per-kernel breakdown: separate proofs with HIP events around every launch.
+    # Per-kernel breakdown (untimed proofs with HIP events around every
+    # launch); the kernel with the most device time per proof is the
+    # dominant one.
     ctx.set_profiling(True)
     ctx.reset_stats()
     for _ in range(args.prof_steps):
         z.air.prove(params, z.Transcript(LABEL), ctx)
     ctx.set_profiling(False)
     stats = ctx.kernel_stats()
+    kern_only = {k: v for k, v in stats.items() if not k.startswith("~") and k != "collect"}
+    dom_kernel = max(kern_only, key=lambda k: kern_only[k][1]) if kern_only else None
+    # Timed region: plain proofs (each call returns the serialized proof, so
+    # the device is idle at both ends), with HIP events on the context stream
+    # around the dominant kernel's launches only. The scheduler's host
+    # intervals are always recorded.
+    ctx.set_profiling(bool(dom_kernel), only=dom_kernel)
+    ctx.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        proof = z.air.prove(params, z.Transcript(LABEL), ctx)
+    t1 = time.perf_counter()
+    timed = ctx.kernel_stats()
+    ctx.set_profiling(False)
+    host = {k.lstrip("~"): round(v[1] / args.steps, 4) for k, v in timed.items() if k.startswith("~")}
     z.air.verify(params, proof, z.Transcript(LABEL))  # outside the timed region, as the reference bench
 
     copy = None
@@ -534,31 +647,38 @@ def main():
             families[f]["gcompress_per_s"] = round(comp / (fam_time[f] * 1e-3) / 1e9, 2)
     for f, v in families.items():
         v["hbm_frac"] = round(v["achieved_gbs"] / HBM_PEAK_GBS, 4)
-    # Measured HBM traffic per proof (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-    # calibrated on a plain copy kernel), committed with the profile.
-    traffic = {}
-    tpath = os.path.join(ROOT, "profiles", "air_2p20_traffic.json")
-    if n == N_STEPS and os.path.exists(tpath):
-        with open(tpath) as fh:
-            for f, v in json.load(fh)["families"].items():
-                traffic[f] = v["fetch_bytes_per_proof"] + v["write_bytes_per_proof"]
-    dom = max(fam_time, key=fam_time.get) if fam_time else None
+    # The dominant kernel against its rooflines: algorithmic bytes per launch
+    # over the launch's event-timed duration in the timed region (HBM), and
+    # its VALU instructions per launch from the committed counter profile
+    # over the same duration (VALU issue: one wave64 instruction per CU per
+    # clock at 2.4 GHz on 256 CUs).
     roofline = None
-    if dom:
-        t_ms = fam_time[dom]
-        achieved = alg[dom] / (t_ms * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic.get(dom),
-                    "traffic_unit": "HBM bytes per proof (profiles/air_2p20_traffic.json)" if dom in traffic else None,
-                    "ms_per_proof": round(t_ms, 4), "alg_bytes": alg[dom]}
-        if dom == "merkle":
-            # BLAKE3 hashing is integer-VALU work: 12 VALU ops per G function,
-            # 56 per compression, 672 ops; MI355X issues 256 CUs x 4 SIMDs x
-            # 16 lanes per clock at 2.4 GHz.
-            peak_gc = 256 * 4 * 16 * 2.4e9 / 672 / 1e9
-            got = families["merkle"]["gcompress_per_s"]
-            roofline["valu"] = {"achieved": got, "peak": round(peak_gc, 1), "unit": "Gcompress/s",
-                                "frac": round(got / peak_gc, 4)}
+    counters = {}
+    cpath = os.path.join(ROOT, "profiles", "air_2p20_counters.json")
+    if n == N_STEPS and os.path.exists(cpath):
+        with open(cpath) as fh:
+            counters = json.load(fh)
+    if dom_kernel and dom_kernel in timed:
+        launches, tot_ms = timed[dom_kernel]
+        avg_s = tot_ms / launches * 1e-3
+        alg = kernel_alg_bytes(m, ctx.get_schedule()).get(dom_kernel)
+        achieved = alg / avg_s / 1e9 if alg else None
+        roofline = {"bound": "hbm", "kernel": dom_kernel, "achieved": round(achieved, 1) if achieved else None,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                    "traffic": None, "alg_bytes": alg, "avg_launch_us": round(avg_s * 1e6, 2),
+                    "launches_timed": launches}
+        ck = counters.get("kernels", {}).get("k_" + dom_kernel)  # launch name -> kernel symbol
+        if ck:
+            roofline["traffic"] = ck["fetch_bytes_per_launch"] + ck["write_bytes_per_launch"]
+            roofline["traffic_source"] = "profiles/air_2p20_counters.json (rocprofv3 FETCH_SIZE, WRITE_SIZE)"
+            peak = VALU_PEAK_GINST
+            got = ck["valu_insts_per_launch"] / avg_s / 1e9
+            roofline["valu"] = {"achieved": round(got, 1), "peak": peak,
+                                "unit": "G VALU wave64 instructions/s", "frac": round(got / peak, 4),
+                                "insts_per_launch": ck["valu_insts_per_launch"],
+                                "profile_avg_launch_us": ck["avg_us"],
+                                "source": "profiles/air_2p20_counters.json (rocprofv3 SQ_INSTS_VALU)"}
 
     single_sched = ctx.get_schedule()
     batch = batched(z, ctx, args.batch, 0 if args.schedule else max(1, args.tune_reps // 2)) if args.batch else None

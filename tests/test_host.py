@@ -274,15 +274,18 @@ def test_fibonacci_interpolant_closed_form():
 @pytest.mark.parametrize("logn,cap_height", [(20, 4), (10, 4), (14, 0)])
 def test_bench_tree_work_is_schedule_invariant(logn, cap_height):
     # The bench attributes each tree's hashing to the kernel family that runs
-    # it under the schedule's fusions; the proof's totals must not move.
+    # it under the schedule's fusions; the proof's compressions must not
+    # move, and its bytes only by the 64-byte leaf inputs a fused kernel
+    # hashes without reading them back.
     import bench
     n, m = 1 << logn, 1 << (logn + 4)
     layers, L = [], m
     while L > 16:
         layers.append(L)
         L //= 4
-    base = dict(tail_max_log=17, tail_group_max_log=8, tail_spread_log=0, levels_fuse=2, small_max_log=0,
-                grind_blocks=4096, fold_fuse_max_log=0, compose_fuse=0, trace_fuse=1)
+    base = dict(tail_max_log=17, tail_group_max_log=7, tail_spread_log=0, levels_fuse=2, small_max_log=0,
+                grind_blocks=4096, fold_fuse_max_log=0, compose_fuse=0, trace_fuse=1, subtree_levels=0,
+                compose_subtree=0)
     # unfused: every tree in the merkle family, leaf + 2 per node up to the cap
     cap = 1 << cap_height
     want = sum(k + 2 * (k - min(cap, k)) for k in [m // 8] + [x // 4 for x in layers])
@@ -292,8 +295,23 @@ def test_bench_tree_work_is_schedule_invariant(logn, cap_height):
                   dict(base, compose_fuse=1, small_max_log=8, fold_fuse_max_log=22)]:
         alg, comp = bench.algorithmic_bytes(n, m, layers, cap_height, sched)
         assert sum(comp.values()) == want
-        tree_bytes = lambda a: sum(a.values()) - sum(alg0.values())
-        assert tree_bytes(alg) == 0, sched
+        fused_leaves = 0
+        for d, L in enumerate(layers):
+            if d == 0:
+                fused_leaves += L // 4 if sched["compose_fuse"] else 0
+            elif (sched["small_max_log"] and L <= 1 << sched["small_max_log"]) or \
+                    (sched["fold_fuse_max_log"] and L <= 1 << sched["fold_fuse_max_log"]):
+                fused_leaves += L // 4
+        assert sum(alg.values()) - sum(alg0.values()) == -64 * fused_leaves, sched
+
+
+def test_bench_dominant_kernel_bytes():
+    # compose_leaves: A and B in, Ext4 layer 0 out, quad-leaf digests and
+    # their parents out: 36 bytes per LDE point.
+    import bench
+    m = 1 << 24
+    assert bench.kernel_alg_bytes(m, {"compose_fuse": 1}) == {"compose_leaves": 36 * m}
+    assert bench.kernel_alg_bytes(m, {"compose_fuse": 0}) == {"compose": 24 * m}
 
 
 @pytest.mark.parametrize("args", [(8, 16, 12_345, 12), (1 << 10, 8, 200, 100), (1 << 14, 24, 0xABCDEF, 100),
