@@ -224,6 +224,24 @@ int z0p_fri_commit_open_ext_evals(z0p_ctx* ctx, const uint32_t* evals4, size_t m
 int z0p_fri_verify(const uint8_t* commitment, size_t commitment_len, const uint8_t* proof, size_t proof_len,
                    const z0p_fri_params* params, z0p_transcript* transcript, uint64_t* indices, size_t indices_cap);
 
+/* Generic byte-leaf Merkle trees (merkle.rs), host only. Paths are `steps`
+ * entries of 33 bytes: the sibling digest, then the direction byte (1: the
+ * running node is the left child, 0: the right; any other value fails), as
+ * in the wire format's paths (codec.rs:87-99).
+ * merkle::hash_leaf (merkle.rs:16-18): BLAKE3(0x00 || data). */
+void z0p_merkle_hash_leaf(const uint8_t* data, size_t len, uint8_t out[32]);
+/* merkle::hash_node (merkle.rs:24-26): BLAKE3(0x01 || left || right). */
+void z0p_merkle_hash_node(const uint8_t left[32], const uint8_t right[32], uint8_t out[32]);
+/* MerkleTree::verify (merkle.rs:165-189): 1 when the path opens the leaf
+ * data at `index` under `root` (index < 2^steps, direction bits equal to
+ * index's bits), else 0. */
+int z0p_merkle_verify(const uint8_t root[32], const uint8_t* leaf, size_t leaf_len, uint64_t index,
+                      const uint8_t* path, size_t steps);
+/* MerkleTree::verify_digest (merkle.rs:196-216): the same for a leaf given
+ * by its digest. */
+int z0p_merkle_verify_digest(const uint8_t root[32], const uint8_t leaf_digest[32], uint64_t index,
+                             const uint8_t* path, size_t steps);
+
 /* fri::ProverState (fri.rs:276-294): the committed layers and their trees,
  * kept in device memory that the state owns, so fri::open can run later
  * (and more than once) over the same commitment. A state uses its ctx: calls

@@ -267,4 +267,4 @@ def default_context() -> Context:
     return _default
 
 
-from . import air, fold, fri, range, table  # noqa: E402
+from . import air, fold, fri, merkle, range, table  # noqa: E402

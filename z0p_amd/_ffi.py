@@ -5,6 +5,7 @@ The library is built in-tree (``__graft_entry__.build()`` or
 no code path can silently fall back to a CPU implementation.
 """
 import ctypes
+import os
 import pathlib
 
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "libz0p_amd.so"
@@ -164,6 +165,10 @@ SIGNATURES = {
     "z0p_fri_commit_open": (ctypes.c_int, [vp, u32p, sz, ctypes.POINTER(FriParamsC), vp, ctypes.POINTER(u8p),
                                            ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz), u64p, sz]),
     "z0p_fri_verify": (ctypes.c_int, [u8p, sz, u8p, sz, ctypes.POINTER(FriParamsC), vp, u64p, sz]),
+    "z0p_merkle_hash_leaf": (None, [u8p, sz, u8p]),
+    "z0p_merkle_hash_node": (None, [u8p, u8p, u8p]),
+    "z0p_merkle_verify": (ctypes.c_int, [u8p, u8p, sz, u64, u8p, sz]),
+    "z0p_merkle_verify_digest": (ctypes.c_int, [u8p, u8p, u64, u8p, sz]),
     "z0p_fri_commit_open_ext": (ctypes.c_int, [vp, u32p, sz, ctypes.POINTER(FriParamsC), vp, ctypes.POINTER(u8p),
                                                ctypes.POINTER(sz), ctypes.POINTER(u8p), ctypes.POINTER(sz), u64p,
                                                sz]),
@@ -219,10 +224,19 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
-    if not LIB_PATH.exists():
-        raise ImportError("libz0p_amd.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
-    L = ctypes.CDLL(str(LIB_PATH))
+    host_only = os.environ.get("Z0P_HOST_LIBRARY")
+    if host_only:
+        # A host-only build (the verifier and host entries under a
+        # sanitizer, `make -C z0p_amd/csrc sanitize`): it exports no device
+        # entry, so calls that need one fail with AttributeError.
+        L = ctypes.CDLL(host_only)
+    else:
+        if not LIB_PATH.exists():
+            raise ImportError("libz0p_amd.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(str(LIB_PATH))
     for name, (res, args) in SIGNATURES.items():
+        if host_only and not hasattr(L, name):
+            continue
         fn = getattr(L, name)
         fn.restype = res
         fn.argtypes = args

@@ -37,23 +37,6 @@ uint32_t brev_host(uint64_t x, int bits) {
 
 namespace {
 
-// (F(k), F(k+1)) canonical by fast doubling.
-void fib_pair(uint64_t k, uint32_t& f, uint32_t& g) {
-    f = 0;
-    g = 1;
-    for (int b = 63; b >= 0; --b) {
-        uint32_t f2 = mul(f, sub(dbl(g), f));
-        uint32_t g2 = add(mul(f, f), mul(g, g));
-        if ((k >> b) & 1) {
-            f = g2;
-            g = add(f2, g2);
-        } else {
-            f = f2;
-            g = g2;
-        }
-    }
-}
-
 template <typename T>
 T* upload(std::vector<void*>& owned, const std::vector<T>& host) {
     void* p = nullptr;
@@ -80,13 +63,6 @@ double now_ms() {
 }
 
 }  // namespace
-
-uint32_t fib_b_last(uint32_t a0, uint32_t b0, uint64_t n) {
-    // B[n-1] = F(n-1) a0 + F(n) b0
-    uint32_t f, g;
-    fib_pair(n - 1, f, g);
-    return add(mul(f, a0), mul(g, b0));
-}
 
 const GeometryTables& Ctx::geometry(int logn, int logb) {
     auto key = std::make_pair(logn, logb);

@@ -8,8 +8,7 @@
 #include <vector>
 
 #include "context.hpp"
-#include "rng.hpp"
-#include "transcript.hpp"
+#include "host.hpp"
 
 namespace z0p {
 
@@ -52,9 +51,6 @@ void tree_levels(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, in
 void launch_chains(const Launch& L, const std::vector<const std::vector<Step>*>& chains,
                    const std::vector<CollectItem>& collect, uint64_t* flag = nullptr, uint64_t seq = 0,
                    const HostGate& gate = HostGate{});
-
-// air::build_trace(a0, b0, n).1[n-1] by fast doubling (air.rs:341-354).
-uint32_t fib_b_last(uint32_t a0, uint32_t b0, uint64_t n);
 
 // One proof request and its state (defined in prover.cpp).
 struct Proof;
@@ -210,35 +206,12 @@ void fold_acc_add_segment(FoldAcc* a, const uint32_t* col_a, uint64_t len_a, con
                           Transcript& tr);
 Bytes fold_acc_finalize(FoldAcc* a, Transcript& tr);
 
-// Host verifier (air.rs:607-752, fri.rs:686-859); throws Status on rejection.
-void air_verify(const z0p_air_params& p, Transcript& tr, const uint8_t* proof, size_t len);
-// Returns the sampled query indices (fri.rs:686).
-std::vector<uint64_t> fri_verify(const uint8_t* cm, size_t cm_len, const uint8_t* pf, size_t pf_len,
-                                 const z0p_fri_params& fp, Transcript& tr);
-void range_verify(const z0p_range_params& p, Transcript& tr, const uint8_t* proof, size_t len);
-void fold_verify(const z0p_fold_params& p, Transcript& tr, const uint8_t* proof, size_t len);
-void table_verify(const z0p_table_params& p, const uint32_t* table, uint64_t table_len, Transcript& tr,
-                  const uint8_t* proof, size_t len);
-// table.rs:357-363: the saturating bound |T| (n + n_rand - 1) - n.
-inline uint64_t membership_bound(uint64_t table_len, uint64_t n, uint64_t n_rand) {
-    const uint64_t rows = n_rand > UINT64_MAX - n ? UINT64_MAX : n + n_rand;
-    const unsigned __int128 prod = (unsigned __int128)table_len * (rows == 0 ? 0 : rows - 1);
-    const uint64_t sat = prod > UINT64_MAX ? UINT64_MAX : uint64_t(prod);
-    return sat < n ? 0 : sat - n;
-}
-
 }  // namespace z0p
 
-// Opaque handles of the C ABI.
-struct z0p_transcript {
-    z0p::Transcript t;
-};
+// Opaque handles of the C ABI that hold device state.
 struct z0p_ctx {
     z0p::Ctx* c;
     std::string tune_log;  // trials of the last z0p_ctx_autotune
-};
-struct z0p_chacha20 {
-    z0p::ChaCha20* r = nullptr;
 };
 struct z0p_fold_acc {
     z0p::FoldAcc* a = nullptr;

@@ -9,8 +9,8 @@
 #include <cstring>
 
 #include "blake3.hpp"
+#include "host.hpp"
 #include "params.hpp"
-#include "prover.hpp"
 
 namespace z0p {
 
@@ -783,6 +783,42 @@ void fold_verify(const z0p_fold_params& p, Transcript& tr, const uint8_t* bytes,
         require(!pf.queries[q].empty(), Z0P_ERR_FRI_VERIFICATION_FAILED);
         require(e4_eq(pf.queries[q][0].values[leaf % 4], combined), Z0P_ERR_FRI_VERIFICATION_FAILED);
     }
+}
+
+// Generic byte-leaf Merkle trees (merkle.rs:16-30, 165-235).
+void merkle_hash_leaf(const uint8_t* data, size_t len, uint8_t out[32]) {
+    Blake3 h;
+    const uint8_t prefix = 0x00;
+    h.update(&prefix, 1);
+    h.update(data, len);
+    h.digest(out);
+}
+
+void merkle_hash_node(const uint8_t left[32], const uint8_t right[32], uint8_t out[32]) {
+    Digest l, r;
+    std::memcpy(l.data(), left, 32);
+    std::memcpy(r.data(), right, 32);
+    const Digest d = hash_node(l, r);
+    std::memcpy(out, d.data(), 32);
+}
+
+// `path` holds `steps` entries of 33 bytes, sibling digest then the
+// direction byte (1: the running node is the left child), as on the wire.
+bool merkle_verify_digest(const uint8_t root[32], const uint8_t leaf_digest[32], uint64_t index, const uint8_t* path,
+                          size_t steps) {
+    if (steps >= 64 || (index >> steps) != 0) return false;
+    Digest cur;
+    std::memcpy(cur.data(), leaf_digest, 32);
+    for (size_t l = 0; l < steps; ++l) {
+        const uint8_t* st = path + 33 * l;
+        if (st[32] > 1) return false;
+        const bool is_left = st[32] == 1;
+        if (is_left != (((index >> l) & 1) == 0)) return false;
+        Digest sib;
+        std::memcpy(sib.data(), st, 32);
+        cur = is_left ? hash_node(cur, sib) : hash_node(sib, cur);
+    }
+    return std::memcmp(cur.data(), root, 32) == 0;
 }
 
 }  // namespace z0p
