@@ -112,7 +112,8 @@ def kernel_alg_bytes(m, sched):
 # Kernel launch names -> families. Tree level kernels are shared by the
 # trace and FRI trees and are reported as one "merkle" family beside them.
 FAMILY = {
-    "intt": "intt", "lde": "lde", "linked_b": "linked_b", "compose": "compose", "compose_leaves": "compose",
+    "intt": "intt", "lde": "lde", "lde_rows": "lde", "lde_cols": "lde", "linked_b": "linked_b", "compose": "compose",
+    "compose_leaves": "compose",
     "fold_leaves": "fold",
     "trace_leaves": "merkle", "fri_leaves": "merkle", "tree_levels": "merkle", "tree_tail": "merkle",
     "tree_nodes": "merkle",

@@ -418,40 +418,45 @@ __device__ void ntt_intt_rows(const NttJob& J, uint32_t* lds, uint32_t u) {
 
 // The coefficient row is already bit-reversed, so a DIT transform consumes
 // it in place; the row table is stored bit-reversed to match.
+// Rows pass of the coset LDE for 4096-point rows (k_lde_rows): thread q
+// loads coefficients 16q .. 16q + 15 of row R (four 16-byte loads; the row
+// is bit-reversed, so this is exactly the DIT's first-pass register layout)
+// with their row-table scales, runs the DIT, and writes
+// y[u] = x * s_j^c * w_n^(u c) for u = q + 256 t, c = brev(R).
+template <bool MASK>
+__device__ void lde_rows_4096(const NttJob& J, uint32_t* lds, uint32_t R, uint32_t j) {
+    const uint32_t q = threadIdx.x;
+    const int logn = J.log1 + 12;
+    const uint32_t c = brev_bits(R, J.log1);
+    const uint32_t* rt = J.tab_a + size_t(j) * N4K + 16 * q;
+    const uint32_t* in = J.in + size_t(R) * N4K + 16 * q;
+    uint32_t x[16];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const u32x4 a = ld4(in, v), sc = ld4(rt, v);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t C = 16 * q + 4 * v + k;
+            x[4 * v + k] = mont_mul(lde_coeff<MASK>(J, a[k], brev_bits(R * uint32_t(N4K) + C, logn), j), sc[k]);
+        }
+    }
+    dit4096(x, lds, gc(J.tw4096));
+    // y[u] = x * s_j^c * w_n^(u c), u = q + 256 t: geometric in t.
+    uint32_t f = mont_mul(pow_lookup(J.pt, (q * c) << J.logb), gc(J.tab_b)[(size_t(j) << J.log1) + c]);
+    const uint32_t step = pow_lookup(J.pt, (256u * c) << J.logb);
+    const gptr yo = gm(J.out) + (size_t(j) << logn) + size_t(c) * N4K;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        yo[q + 256 * t] = mont_mul(x[t], f);
+        f = mont_mul(f, step);
+    }
+}
+
+// Rows of any other length (k_ntt): DIT in LDS from the bit-reversed row.
 template <bool MASK>
 __device__ void ntt_lde_rows(const NttJob& J, uint32_t* lds, uint32_t R, uint32_t j) {
     const int n1 = 1 << J.log1, n2 = 1 << J.log2;
     const int logn = J.log1 + J.log2;
-    if (J.log2 == 12) {
-        const uint32_t q = threadIdx.x;
-        const uint32_t c = brev_bits(R, J.log1);
-        const gcptr rt = gc(J.tab_a) + size_t(j) * n2;
-        const gcptr in = gc(J.in) + size_t(R) * n2;
-        // Coalesced load + row-table scale into LDS, then DIT from the
-        // bit-reversed positions.
-        uint32_t x[16];
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            const int C = int(q) + 256 * t;
-            x[t] = mont_mul(lde_coeff<MASK>(J, in[C], brev_bits(R * uint32_t(n2) + uint32_t(C), logn), j), rt[C]);
-        }
-#pragma unroll
-        for (int t = 0; t < 16; ++t) lds[pad16(int(q) + 256 * t)] = x[t];
-        __syncthreads();
-        lds_get<2>(lds, x, q);
-        __syncthreads();  // pass-2 reads complete before dit4096 overwrites
-        dit4096(x, lds, gc(J.tw4096));
-        // y[u] = x * s_j^c * w_n^(u c), u = q + 256 t: geometric in t.
-        uint32_t f = mont_mul(pow_lookup(J.pt, (q * c) << J.logb), gc(J.tab_b)[size_t(j) * n1 + c]);
-        const uint32_t step = pow_lookup(J.pt, (256u * c) << J.logb);
-        const gptr yo = gm(J.out) + (size_t(j) << logn) + size_t(c) * n2;
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            yo[q + 256 * t] = mont_mul(x[t], f);
-            f = mont_mul(f, step);
-        }
-        return;
-    }
     uint32_t* s = lds;
     uint32_t* tw = lds + n2;
     load_tw(tw, J.tw4096, J.log2);
@@ -470,35 +475,41 @@ __device__ void ntt_lde_rows(const NttJob& J, uint32_t* lds, uint32_t R, uint32_
     }
 }
 
+// Columns pass of the coset LDE (k_lde_cols) for 2^4..2^12-point columns:
+// a 4096-element tile of C = 4096 / n1 columns through LDS (pitch C + 1),
+// the DIF transform in registers, bit-reversed rows out.
+__device__ void lde_cols_tile(const NttJob& J, uint32_t* lds, uint32_t bx, uint32_t j) {
+    const int n1 = 1 << J.log1, n2 = 1 << J.log2;
+    const int logn = J.log1 + J.log2;
+    const int L = J.log1, logC = 12 - L, C = 1 << logC, pitch = C + 1;
+    const int u0 = int(bx) * C;
+    const int tid = threadIdx.x;
+    const gcptr yj = gc(J.in) + (size_t(j) << logn);
+    uint32_t v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int e = tid + 256 * k, r = e >> logC, c = e & (C - 1);
+        v[k] = yj[size_t(r) * n2 + u0 + c];
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int e = tid + 256 * k, r = e >> logC, c = e & (C - 1);
+        lds[r * pitch + c] = v[k];
+    }
+    __syncthreads();
+    cols_ntt<false>(lds, L, logC, gc(J.tw4096));
+    const gptr oj = gm(J.out) + (size_t(brev_bits(j, J.logb)) << logn);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int e = tid + 256 * k, qq = e & (n1 - 1), uu = e >> L;
+        oj[size_t(brev_bits(uint32_t(u0 + uu), J.log2)) * n1 + qq] = lds[qq * pitch + uu];
+    }
+}
+
+// Columns of other lengths (k_ntt).
 __device__ void ntt_lde_cols(const NttJob& J, uint32_t* lds, uint32_t bx, uint32_t j) {
     const int n1 = 1 << J.log1, n2 = 1 << J.log2, cols = J.cols;
     const int logn = J.log1 + J.log2;
-    if (cols_in_regs(J.log1)) {
-        const int L = J.log1, logC = 12 - L, C = 1 << logC, pitch = C + 1;
-        const int u0 = int(bx) * C;
-        const int tid = threadIdx.x;
-        const gcptr yj = gc(J.in) + (size_t(j) << logn);
-        uint32_t v[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const int e = tid + 256 * k, r = e >> logC, c = e & (C - 1);
-            v[k] = yj[size_t(r) * n2 + u0 + c];
-        }
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const int e = tid + 256 * k, r = e >> logC, c = e & (C - 1);
-            lds[r * pitch + c] = v[k];
-        }
-        __syncthreads();
-        cols_ntt<false>(lds, L, logC, gc(J.tw4096));
-        const gptr oj = gm(J.out) + (size_t(brev_bits(j, J.logb)) << logn);
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const int e = tid + 256 * k, qq = e & (n1 - 1), uu = e >> L;
-            oj[size_t(brev_bits(uint32_t(u0 + uu), J.log2)) * n1 + qq] = lds[qq * pitch + uu];
-        }
-        return;
-    }
     const int st = n1 + 1;
     uint32_t* s = lds;
     uint32_t* tw = lds + cols * st;
@@ -555,6 +566,28 @@ __global__ __launch_bounds__(WG) void k_ntt(NttJob inl, const NttJob* tab, uint3
         default:
             break;
     }
+}
+
+// The two passes of the 2^16..2^24-point coset LDEs (4096-point rows, 2^4..
+// 2^12-point columns) in kernels of their own: register use and LDS are
+// those of one path, not of every NTT kind.
+__global__ __launch_bounds__(WG) void k_lde_rows(NttJob inl, const NttJob* tab, uint32_t njobs) {
+    extern __shared__ uint32_t lds[];
+    uint32_t local;
+    const NttJob J = job_of(inl, tab, njobs, local);
+    const uint32_t n1 = 1u << J.log1;
+    if (J.mask)
+        lde_rows_4096<true>(J, lds, local % n1, local / n1);
+    else
+        lde_rows_4096<false>(J, lds, local % n1, local / n1);
+}
+
+__global__ __launch_bounds__(WG) void k_lde_cols(NttJob inl, const NttJob* tab, uint32_t njobs) {
+    extern __shared__ uint32_t lds[];
+    uint32_t local;
+    const NttJob J = job_of(inl, tab, njobs, local);
+    const uint32_t per = (1u << J.log2) / uint32_t(J.cols);
+    lde_cols_tile(J, lds, local % per, local / per);
 }
 
 __global__ __launch_bounds__(WG) void k_bitrev(BitrevJob inl, const BitrevJob* tab, uint32_t njobs) {
@@ -1795,7 +1828,7 @@ size_t ntt_lds(const NttJob& j) {
         case NTT_INTT_SMALL:
         case NTT_LDE_SMALL: return j.logn == 12 ? size_t(N4K_LDS) * 4 : (size_t(1) << j.logn) * 6;
         case NTT_INTT_ROWS:
-        case NTT_LDE_ROWS: return j.log2 == 12 ? size_t(N4K_LDS) * 4 : (size_t(1) << j.log2) * 6;
+        case NTT_LDE_ROWS: return (size_t(1) << j.log2) * 6;  // rows of 4096 run in k_lde_rows
         case NTT_INTT_COLS:
         case NTT_LDE_COLS:
             if (j.log1 >= 4 && j.log1 <= 12) return ((size_t(1) << j.log1) + N4K) * 4;  // pitch C + 1
@@ -1896,15 +1929,26 @@ void launch_fib_coeffs(const Launch& L, std::vector<FibCoeffJob> jobs) {
     run(L, "fib_coeffs", k_fib_coeffs, jobs, g, 0);
 }
 
+bool lde_rows_job(const NttJob& j) { return j.kind == NTT_LDE_ROWS && j.log2 == 12; }
+bool lde_cols_job(const NttJob& j) { return j.kind == NTT_LDE_COLS && j.log2 == 12 && j.log1 >= 4 && j.log1 <= 12; }
+
 void launch_ntt(const Launch& L, std::vector<NttJob> jobs) {
+    std::vector<NttJob> rows, cols, rest;
+    for (const NttJob& j : jobs) (lde_rows_job(j) ? rows : lde_cols_job(j) ? cols : rest).push_back(j);
     size_t lds = 0;
     bool intt = false;
-    for (const NttJob& j : jobs) {
+    for (const NttJob& j : rest) {
         lds = std::max(lds, ntt_lds(j));
         intt = intt || j.kind <= NTT_INTT_ROWS;
     }
-    unsigned g = assign_blocks(jobs, [](const NttJob& j) { return ntt_blocks(j); });
-    run(L, intt ? "intt" : "lde", k_ntt, jobs, g, lds);
+    unsigned g = assign_blocks(rest, [](const NttJob& j) { return ntt_blocks(j); });
+    run(L, intt ? "intt" : "lde", k_ntt, rest, g, lds);
+    g = assign_blocks(rows, [](const NttJob& j) { return ntt_blocks(j); });
+    run(L, "lde_rows", k_lde_rows, rows, g, size_t(N4K_LDS) * 4);
+    lds = 0;
+    for (const NttJob& j : cols) lds = std::max(lds, ntt_lds(j));
+    g = assign_blocks(cols, [](const NttJob& j) { return ntt_blocks(j); });
+    run(L, "lde_cols", k_lde_cols, cols, g, lds);
 }
 
 void launch_linked(const Launch& L, std::vector<LinkedJob> jobs) {
