@@ -237,6 +237,7 @@ struct Schedule {
     int trace_fuse = 1;          // linked column + trace leaves in one kernel (0: separate kernels)
     int subtree_levels = 0;      // tree levels above level 1 reduced in LDS per leaf workgroup (0: off)
     int compose_subtree = 0;     // the same inside the composition kernel (its own limit; 0: off)
+    int graph = 0;               // each iteration's launches captured into one HIP graph (0: direct launches)
 
     // Digests per tail workgroup (log2) for a level of `count` digests.
     int tail_group_log(uint64_t count) const {
@@ -255,7 +256,7 @@ struct Schedule {
                (fold_fuse_max_log == 0 || (fold_fuse_max_log >= 3 && fold_fuse_max_log <= 27)) &&
                (compose_fuse == 0 || compose_fuse == 1) && (trace_fuse == 0 || trace_fuse == 1) &&
                subtree_levels >= 0 && subtree_levels <= SUBTREE_MAX_LEVELS && compose_subtree >= 0 &&
-               compose_subtree <= SUBTREE_MAX_LEVELS;
+               compose_subtree <= SUBTREE_MAX_LEVELS && (graph == 0 || graph == 1);
     }
 };
 
@@ -314,7 +315,10 @@ struct Ctx {
     Launch launch() { return Launch{stream, &prof, &ring, false}; }
 
     // Runs a launch sequence dry (tables only), uploads the tables with one
-    // copy, then replays it for real.
+    // copy, then replays it for real: as direct launches, or (schedule
+    // `graph`, profiling off) captured into a HIP graph that is instantiated
+    // and launched once. The sequence holds kernel launches only (no
+    // memset or copy nodes).
     template <typename F>
     void enqueue(F&& launches) {
         const size_t mark = ring.mark();
@@ -323,8 +327,36 @@ struct Ctx {
         launches(static_cast<const Launch&>(dry));
         ring.flush(stream);
         ring.replay_from(mark);
-        launches(static_cast<const Launch&>(launch()));
+        if (sched.graph && !prof.enabled) {
+            hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            hipGraph_t g = nullptr;
+            try {
+                launches(static_cast<const Launch&>(launch()));
+            } catch (...) {
+                (void)hipStreamEndCapture(stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                ring.end_replay();
+                throw;
+            }
+            hip_check(hipStreamEndCapture(stream, &g));
+            hipGraphExec_t x = nullptr;
+            const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            hip_check(e);
+            graph_execs.push_back(x);
+            hip_check(hipGraphLaunch(x, stream));
+        } else {
+            launches(static_cast<const Launch&>(launch()));
+        }
         ring.end_replay();
+    }
+
+    // Executable graphs launched since the stream last drained; destroyed
+    // once the work they launched has completed.
+    std::vector<hipGraphExec_t> graph_execs;
+    void release_graphs() {
+        for (hipGraphExec_t x : graph_execs) (void)hipGraphExecDestroy(x);
+        graph_execs.clear();
     }
 
     void swap_stage() {
@@ -369,7 +401,10 @@ struct Ctx {
             hip_check(hipStreamSynchronize(stream));
             prof.resolve();
         }
-        if (reset_ring) ring.reset();
+        if (reset_ring) {
+            ring.reset();
+            release_graphs();  // nothing else is queued: every graph has run
+        }
     }
 
     void reserve_scratch(size_t bytes) {
@@ -396,6 +431,7 @@ struct Ctx {
         hip_check(hipStreamSynchronize(stream));
         if (prof.enabled) prof.resolve();
         ring.reset();
+        release_graphs();
     }
 
     // Device buffers released by IVC accumulators, kept by size for the

@@ -167,6 +167,7 @@ const PointInverses& Ctx::point_inverses(int logn, int logb) {
 Ctx::~Ctx() {
     tables.clear();
     if (stream) (void)hipStreamSynchronize(stream);
+    release_graphs();
     if (stage_host) (void)hipHostFree(stage_host);
     if (alt_stage_host) (void)hipHostFree(alt_stage_host);
     if (flag_host) (void)hipHostFree(flag_host);
@@ -1724,6 +1725,7 @@ bool Engine::iterate() {
     if (flight_.empty()) {
         if (active_.empty()) return !queue_.empty();
         c_.ring.reset();  // every launch has completed
+        c_.release_graphs();
         if (pipelined_ && c_.pipeline_single && active_.size() == 1 && queue_.empty() &&
             active_[0]->phase == PH_FRONT) {
             Proof* p = active_[0];

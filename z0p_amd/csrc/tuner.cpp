@@ -75,6 +75,7 @@ TuneResult autotune(Ctx& c, const std::vector<z0p_air_params>& work, unsigned re
         {"trace_fuse", &Schedule::trace_fuse, {0, 1}},
         {"subtree_levels", &Schedule::subtree_levels, {0, 2, 4, 6, 8}},
         {"compose_subtree", &Schedule::compose_subtree, {0, 1, 2, 4}},
+        {"graph", &Schedule::graph, {0, 1}},
     };
     // A candidate must beat the incumbent by this fraction to replace it, so
     // run-to-run noise does not move the schedule.
