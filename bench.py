@@ -173,6 +173,13 @@ def batched(z, ctx, count, tune_reps, seed=2024):
     ok = z.air.verify_batch([p for p, _ in inst], got, [z.Transcript(l) for _, l in inst])
     t_verify = time.perf_counter() - t0
     assert all(v is None for v in ok)
+    # The same, with the Merkle path hashing on the GPU (after one warm-up
+    # call of the same size, which grows the pinned staging once).
+    z.air.verify_batch_device([p for p, _ in inst], got, [z.Transcript(l) for _, l in inst], ctx)
+    t0 = time.perf_counter()
+    ok = z.air.verify_batch_device([p for p, _ in inst], got, [z.Transcript(l) for _, l in inst], ctx)
+    t_verify_dev = time.perf_counter() - t0
+    assert all(v is None for v in ok)
     batch_sched = ctx.get_schedule()
     ctx.set_schedule(single_sched)
     t0 = time.perf_counter()
@@ -190,6 +197,7 @@ def batched(z, ctx, count, tune_reps, seed=2024):
         "one_at_a_time_proofs_per_s": round(count / t_single, 1),
         "speedup_vs_one_at_a_time": round(t_single / t_batch, 2),
         "verify_proofs_per_s": round(count / t_verify, 1),
+        "verify_device_proofs_per_s": round(count / t_verify_dev, 1),
         "verify_threads": os.cpu_count(),
         "autotune": tuned,
         "schedule": batch_sched,
@@ -607,10 +615,25 @@ def main():
     for _ in range(args.steps):
         proof = z.air.prove(params, z.Transcript(LABEL), ctx)
     t1 = time.perf_counter()
+    ms = (t1 - t0) * 1000.0 / args.steps
     timed = ctx.kernel_stats()
     ctx.set_profiling(False)
     host = {k.lstrip("~"): round(v[1] / args.steps, 4) for k, v in timed.items() if k.startswith("~")}
     z.air.verify(params, proof, z.Transcript(LABEL))  # outside the timed region, as the reference bench
+    # Batched verification throughput on copies of this proof: host threads
+    # alone, and with the Merkle path hashing on the GPU.
+    vcount = 128
+    t0 = time.perf_counter()
+    vok = z.air.verify_batch([params] * vcount, [proof] * vcount, [z.Transcript(LABEL) for _ in range(vcount)])
+    t_vc = time.perf_counter() - t0
+    z.air.verify_batch_device([params] * vcount, [proof] * vcount, [z.Transcript(LABEL) for _ in range(vcount)], ctx)
+    t0 = time.perf_counter()
+    vok_d = z.air.verify_batch_device([params] * vcount, [proof] * vcount,
+                                      [z.Transcript(LABEL) for _ in range(vcount)], ctx)
+    t_vd = time.perf_counter() - t0
+    assert all(v is None for v in vok) and all(v is None for v in vok_d)
+    verify_single = {"proofs": vcount, "host_threads": os.cpu_count(),
+                     "host_proofs_per_s": round(vcount / t_vc, 1), "device_paths_proofs_per_s": round(vcount / t_vd, 1)}
 
     copy = None
     if args.copy_mib:
@@ -623,7 +646,6 @@ def main():
         copy = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": round(cms.value, 4),
                 "gbs_rw": round(2 * nbytes / (cms.value * 1e-3) / 1e9, 1)}
 
-    ms = (t1 - t0) * 1000.0 / args.steps
     m = n * params.blowup
     layers = []
     L = m
@@ -708,6 +730,7 @@ def main():
                    "proof_bytes": len(proof), "proofs_per_s": round(1000.0 / ms, 2)},
         "kernels_ms_per_proof": {k: round(v[1] / per, 4) for k, v in sorted(stats.items()) if not k.startswith("~")},
         "host_ms_per_proof": host,
+        "verify_batch": verify_single,
         "schedule": single_sched,
         "autotune": single_tune,
         "families": families,

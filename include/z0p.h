@@ -414,6 +414,18 @@ int z0p_air_prove_batch(z0p_ctx* ctx, const z0p_air_params* params, z0p_transcri
  * own z0p_air_verify would; null or shared transcripts are rejected with
  * Z0P_ERR_INVALID_ARGUMENT). statuses[i] receives instance i's status.
  * threads 0: one per hardware thread. */
+/* z0p_air_verify_batch with the Merkle path hashing on ctx's device: host
+ * threads parse each proof and replay its transcript with every other check,
+ * then one kernel per chunk of 64 proofs walks all their authentication
+ * paths (one per lane). statuses[i] equals what z0p_air_verify returns for
+ * instance i (a failed path check recorded before a host-side failure is
+ * reported as Z0P_ERR_BAD_MERKLE_PATH, as the serial order would). On a
+ * rejected instance the transcript may be advanced past the failing check.
+ * The batched verification service of the survey's ranked extensions over
+ * air::verify (air.rs:607-752, fri.rs:686-859). */
+int z0p_air_verify_batch_device(z0p_ctx* ctx, const z0p_air_params* params, z0p_transcript* const* transcripts,
+                                const uint8_t* const* proofs, const size_t* proof_lens, size_t count,
+                                uint32_t threads, int* statuses);
 int z0p_air_verify_batch(const z0p_air_params* params, z0p_transcript* const* transcripts,
                          const uint8_t* const* proofs, const size_t* proof_lens, size_t count, uint32_t threads,
                          int* statuses);

@@ -190,6 +190,9 @@ SIGNATURES = {
     "z0p_ctx_set_waves": (ctypes.c_int, [vp, u32]),
     "z0p_air_verify_batch": (ctypes.c_int, [ctypes.POINTER(AirParamsC), ctypes.POINTER(vp), ctypes.POINTER(u8p),
                                             ctypes.POINTER(sz), sz, u32, ctypes.POINTER(ctypes.c_int)]),
+    "z0p_air_verify_batch_device": (ctypes.c_int, [vp, ctypes.POINTER(AirParamsC), ctypes.POINTER(vp),
+                                                   ctypes.POINTER(u8p), ctypes.POINTER(sz), sz, u32,
+                                                   ctypes.POINTER(ctypes.c_int)]),
     "z0p_ctx_get_schedule": (ctypes.c_int, [vp, ctypes.POINTER(ScheduleC)]),
     "z0p_ctx_set_schedule": (ctypes.c_int, [vp, ctypes.POINTER(ScheduleC)]),
     "z0p_ctx_autotune": (ctypes.c_int, [vp, ctypes.POINTER(AirParamsC), sz, u32, ctypes.POINTER(ScheduleC),

@@ -114,21 +114,39 @@ def verify(params: Params, proof: bytes, transcript) -> None:
     check(lib().z0p_air_verify(ctypes.byref(params), transcript.handle, buf(proof), len(proof)))
 
 
-def verify_batch(params_list, proofs, transcripts, threads=0):
-    """air::verify for every (params, proof, transcript) on host threads;
-    returns a list of None (accepted) or Z0pError per instance."""
+def _batch_args(params_list, proofs, transcripts):
     count = len(params_list)
     if len(proofs) != count or len(transcripts) != count:
         raise ValueError("one proof and one transcript per instance")
     ps = (Params * max(count, 1))(*params_list)
     ts = (ctypes.c_void_p * max(count, 1))(*[t.handle for t in transcripts])
-    keep = [(ctypes.c_uint8 * max(len(b), 1)).from_buffer_copy(b or b"\0") for b in proofs]
-    pp = (u8p * max(count, 1))(*[ctypes.cast(k, u8p) for k in keep])
+    # Pointers into the bytes objects themselves (immutable, kept alive by
+    # `keep`): no copy of the proofs.
+    keep = [bytes(b) if b else b"\0" for b in proofs]
+    pp = (u8p * max(count, 1))(*[ctypes.cast(ctypes.c_char_p(k), u8p) for k in keep])
     lens = (ctypes.c_size_t * max(count, 1))(*[len(b) for b in proofs])
     sts = (ctypes.c_int * max(count, 1))()
+    return count, ps, ts, keep, pp, lens, sts
+
+
+def verify_batch_device(params_list, proofs, transcripts, ctx=None, threads=0):
+    """air::verify for every instance, the Merkle path hashing batched on the
+    GPU of ctx and the rest on host threads; returns a list of None
+    (accepted) or Z0pError per instance, as verify_batch does."""
+    from . import default_context
+    ctx = ctx or default_context()
+    count, ps, ts, keep, pp, lens, sts = _batch_args(params_list, proofs, transcripts)
+    check(lib().z0p_air_verify_batch_device(ctx.handle, ps, ts, pp, lens, count, threads, sts))
+    return [None if sts[i] == 0 else Z0pError(sts[i]) for i in range(count)]
+
+
+def verify_batch(params_list, proofs, transcripts, threads=0):
+    """air::verify for every (params, proof, transcript) on host threads;
+    returns a list of None (accepted) or Z0pError per instance."""
+    count, ps, ts, keep, pp, lens, sts = _batch_args(params_list, proofs, transcripts)
     check(lib().z0p_air_verify_batch(ps, ts, pp, lens, count, threads, sts))
     return [None if sts[i] == 0 else Z0pError(sts[i]) for i in range(count)]
 
 
 __all__ = ["Params", "params_new", "for_security", "b_last", "prove", "prove_with_trace", "prove_zk", "prove_batch",
-           "verify", "verify_batch", "Z0pError"]
+           "verify", "verify_batch", "verify_batch_device", "Z0pError"]

@@ -400,4 +400,20 @@ void z0p_fri_state_free(z0p_fri_state* state) {
     delete state;
 }
 
+int z0p_air_verify_batch_device(z0p_ctx* ctx, const z0p_air_params* params, z0p_transcript* const* transcripts,
+                                const uint8_t* const* proofs, const size_t* proof_lens, size_t count,
+                                uint32_t threads, int* statuses) {
+    if (!ctx) return Z0P_ERR_INVALID_ARGUMENT;
+    if (count && (!params || !transcripts || !proofs || !proof_lens || !statuses)) return Z0P_ERR_INVALID_ARGUMENT;
+    std::set<const z0p_transcript*> seen;
+    for (size_t i = 0; i < count; ++i)
+        if (!transcripts[i] || !seen.insert(transcripts[i]).second || (!proofs[i] && proof_lens[i]))
+            return Z0P_ERR_INVALID_ARGUMENT;
+    std::vector<z0p::Transcript*> trs(count);
+    for (size_t i = 0; i < count; ++i) trs[i] = &transcripts[i]->t;
+    return guarded([&] {
+        z0p::air_verify_batch_device(*ctx->c, params, trs.data(), proofs, proof_lens, count, threads, statuses);
+    });
+}
+
 }  // extern "C"

@@ -162,6 +162,34 @@ class DescRing final : public TableSink {
     bool replay_ = false;
 };
 
+// A pinned host buffer and a device buffer of the same size, grown on
+// demand and kept (the batched verification's staging).
+struct PinnedPair {
+    uint8_t* host = nullptr;
+    uint8_t* dev = nullptr;
+    size_t cap = 0;
+    void reserve(size_t bytes) {
+        if (bytes <= cap) return;
+        release();
+        const size_t want = std::max<size_t>(bytes + bytes / 4, size_t(1) << 16);
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&host), want, hipHostMallocDefault));
+        const hipError_t e = hipMalloc(reinterpret_cast<void**>(&dev), want);
+        if (e != hipSuccess) {
+            (void)hipHostFree(host);
+            host = nullptr;
+            hip_check(e);
+        }
+        cap = want;
+    }
+    void release() {
+        if (host) (void)hipHostFree(host);
+        if (dev) (void)hipFree(dev);
+        host = dev = nullptr;
+        cap = 0;
+    }
+    ~PinnedPair() { release(); }
+};
+
 // Persistent host worker threads for the per-proof transcript steps of a
 // scheduler iteration (each proof owns its transcript, so they are
 // independent). run() also works on the calling thread.
@@ -311,6 +339,16 @@ struct Ctx {
     uint32_t* tw4096_inv = nullptr;  // [2048] omega_4096^{-e}, Montgomery
     std::map<std::pair<int, int>, std::unique_ptr<GeometryTables>> tables;
     std::unique_ptr<HostPool> host_pool;  // created on first use by a batch
+    HostPool& host_workers() {
+        if (!host_pool) {
+            const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+            host_pool.reset(new HostPool(std::min(hw, 16u) - 1));
+        }
+        return *host_pool;
+    }
+    // Batched verification staging: two slots of (checks, siblings, flags).
+    PinnedPair verify_stage[2][3];
+    hipEvent_t verify_done[2] = {nullptr, nullptr};
 
     Launch launch() { return Launch{stream, &prof, &ring, false}; }
 

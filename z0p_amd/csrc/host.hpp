@@ -32,6 +32,29 @@ inline uint64_t membership_bound(uint64_t table_len, uint64_t n, uint64_t n_rand
     return sat < n ? 0 : sat - n;
 }
 
+// Merkle path checks of a batched verification that run on the device: a
+// 64-byte leaf hashed up `steps` levels (bit l of `dirs` set: the running
+// node is the left child at level l, siblings[sib + l] the other child)
+// must reach `expected`. Words are little-endian, as on the wire.
+struct PathCheck {
+    uint32_t leaf[16];
+    uint32_t expected[8];
+    uint32_t steps;
+    uint32_t sib;
+    uint64_t dirs;
+};
+struct PathChecks {
+    std::vector<PathCheck> checks;
+    std::vector<uint32_t> siblings;  // 8 words per digest
+};
+// air::verify with its Merkle path checks appended to `defer` instead of
+// run (every other check runs here, in the reference's order). The proof is
+// accepted iff this returns and every recorded check holds; if this throws,
+// a recorded check that fails came first, so the proof's status is then
+// BadMerklePath, else the thrown status.
+void air_verify_deferred(const z0p_air_params& p, Transcript& tr, const uint8_t* proof, size_t len,
+                         PathChecks& defer);
+
 // Generic Merkle trees over byte leaves (merkle.rs:16-30, 165-235):
 // leaf H(0x00 || data), node H(0x01 || left || right).
 void merkle_hash_leaf(const uint8_t* data, size_t len, uint8_t out[32]);

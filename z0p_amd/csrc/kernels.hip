@@ -1277,6 +1277,37 @@ __global__ __launch_bounds__(WG) void k_compose_leaves(ComposeJob inl, const Com
     if (J.sub_levels > 0) staged_subtree(lds, par, live, J.tree, m >> 2, first >> 2, J.sub_levels);
 }
 
+// Batched verification: lane i walks one Merkle path from its 64-byte leaf
+// (one path per lane; paths are at most 63 levels, checked by the host).
+__global__ __launch_bounds__(WG) void k_verify_paths(const uint32_t* checks, const uint32_t* siblings,
+                                                     uint32_t count, uint8_t* ok) {
+    const uint32_t i = blockIdx.x * WG + threadIdx.x;
+    if (i >= count) return;
+    const gcptr c = gc(checks) + size_t(i) * PATH_CHECK_WORDS;
+    uint32_t m[16], h[8];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) m[k] = c[k];
+    leaf64_words(m, h);
+    const uint32_t steps = c[24], first = c[25];
+    const uint64_t dirs = uint64_t(c[26]) | (uint64_t(c[27]) << 32);
+    for (uint32_t l = 0; l < steps; ++l) {
+        const gcptr sp = gc(siblings) + (size_t(first) + l) * 8;
+        uint32_t sw[8], o[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sw[k] = sp[k];
+        if ((dirs >> l) & 1)
+            node_words(h, sw, o);
+        else
+            node_words(sw, h, o);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) h[k] = o[k];
+    }
+    bool eq = true;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) eq = eq && h[k] == c[16 + k];
+    ok[i] = eq ? 1 : 0;
+}
+
 // ---- quad-lane BLAKE3 for the narrow tree levels ------------------------
 // Four lanes of a quad compute one compression: lane q holds state column q
 // (v[q], v[4+q], v[8+q], v[12+q]) and runs the column G on it; for the
@@ -2163,6 +2194,12 @@ void launch_pow_brev(const Launch& L, PowTable pt, uint32_t* out, int bits, int 
     if (L.dry) return;
     hipLaunchKernelGGL(k_pow_brev, dim3(grid_for(uint64_t(1) << bits, WG)), dim3(WG), 0, L.stream, pt, out, bits,
                        shift);
+}
+
+void launch_verify_paths(hipStream_t s, const uint32_t* checks, const uint32_t* siblings, uint32_t count,
+                         uint8_t* ok) {
+    if (count == 0) return;
+    hipLaunchKernelGGL(k_verify_paths, dim3((count + WG - 1) / WG), dim3(WG), 0, s, checks, siblings, count, ok);
 }
 
 void launch_copy(const Launch& L, const uint32_t* in, uint32_t* out, uint64_t words) {

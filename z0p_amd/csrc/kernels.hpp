@@ -449,6 +449,14 @@ void launch_collect(const Launch& L, std::vector<CollectItem> items, uint64_t* f
                     const HostGate& gate);
 void launch_copy(const Launch& L, const uint32_t* in, uint32_t* out, uint64_t words);
 
+// Batched verification's Merkle path checks: `count` records of
+// PATH_CHECK_WORDS words (16 leaf words, 8 expected digest words, steps,
+// first sibling, direction bits low and high), siblings as 8 words each;
+// ok[i] = 1 when record i's path reaches its expected digest, else 0.
+constexpr int PATH_CHECK_WORDS = 28;
+void launch_verify_paths(hipStream_t s, const uint32_t* checks, const uint32_t* siblings, uint32_t count,
+                         uint8_t* ok);
+
 // out[i] = w^(brev_bits(i) << shift) for i < 2^bits, w's powers from `pt`
 // (Montgomery form): the bit-reversed point tables of a geometry.
 void launch_pow_brev(const Launch& L, PowTable pt, uint32_t* out, int bits, int shift);

@@ -168,6 +168,8 @@ Ctx::~Ctx() {
     tables.clear();
     if (stream) (void)hipStreamSynchronize(stream);
     release_graphs();
+    for (hipEvent_t& e : verify_done)
+        if (e) (void)hipEventDestroy(e);
     if (stage_host) (void)hipHostFree(stage_host);
     if (alt_stage_host) (void)hipHostFree(alt_stage_host);
     if (flag_host) (void)hipHostFree(flag_host);
@@ -1696,11 +1698,7 @@ void Engine::complete(const std::vector<Proof*>& ws, uint64_t seq, bool reset_ri
         }
     };
     if (ws.size() >= 4) {
-        if (!c_.host_pool) {
-            const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-            c_.host_pool.reset(new HostPool(std::min(hw, 16u) - 1));
-        }
-        c_.host_pool->run(ws.size(), step);
+        c_.host_workers().run(ws.size(), step);
     } else {
         for (size_t i = 0; i < ws.size(); ++i) step(i);
     }

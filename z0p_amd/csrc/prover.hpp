@@ -197,6 +197,13 @@ Bytes table_prove(Ctx& ctx, const z0p_table_params& p, const uint32_t* table, ui
 Bytes table_prove_zk(Ctx& ctx, const z0p_table_params& p, const uint32_t* table, uint64_t table_len, uint64_t n_rand,
                      ChaCha20& rng, Transcript& tr);
 
+// Batched air::verify with the Merkle path hashing on the device
+// (verify_batch.cpp); statuses[i] is what air_verify would return for
+// instance i.
+void air_verify_batch_device(Ctx& c, const z0p_air_params* params, Transcript* const* trs,
+                             const uint8_t* const* proofs, const size_t* lens, size_t count, unsigned threads,
+                             int* statuses);
+
 // IVC accumulator (fold.rs:361-570; fold.cpp).
 struct FoldAcc;
 FoldAcc* fold_acc_new(Ctx& c, const z0p_fold_params& p);

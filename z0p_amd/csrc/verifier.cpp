@@ -171,13 +171,50 @@ uint64_t brev64(uint64_t x, uint64_t bits) {
     return r;
 }
 
-// fri.rs:1108-1115
-uint32_t point_at(uint64_t lde_len, uint64_t depth, uint64_t j) {
-    uint64_t log_m = ilog2(lde_len);
-    uint64_t bits = log_m - 2 * depth;
-    uint32_t omega = root_of_unity(uint32_t(log_m));
-    uint32_t base = mul(GENERATOR, pow_c(omega, brev64(j, bits)));
-    return pow_c(base, uint64_t(1) << (2 * depth));
+// The points of point_at (fri.rs:1108-1115) without a power chain per call: for layer
+// depth d of an LDE of 2^log_m points, point_at(m, d, j) =
+// g^(4^d) * omega_m^(brev(j, log_m - 2d) * 4^d), with omega_m^e read from
+// three 512-entry tables (e < 2^27) and g^(4^d) kept per depth.
+struct DomainPoints {
+    uint64_t log_m;
+    uint32_t t0[512], t1[512], t2[512];
+    std::vector<uint32_t> g4;  // g^(4^d)
+    explicit DomainPoints(uint64_t lm) : log_m(lm) {
+        const uint32_t w = root_of_unity(uint32_t(lm));
+        const uint32_t w9 = pow_c(w, uint64_t(1) << 9), w18 = pow_c(w, uint64_t(1) << 18);
+        t0[0] = t1[0] = t2[0] = 1;
+        for (int i = 1; i < 512; ++i) {
+            t0[i] = mul(t0[i - 1], w);
+            t1[i] = mul(t1[i - 1], w9);
+            t2[i] = mul(t2[i - 1], w18);
+        }
+        uint32_t g = GENERATOR;
+        for (uint64_t d = 0; 2 * d <= lm; ++d) {
+            g4.push_back(g);
+            g = mul(mul(g, g), mul(g, g));
+        }
+    }
+    uint32_t at(uint64_t depth, uint64_t j) const {
+        const uint64_t e = brev64(j, log_m - 2 * depth) << (2 * depth);
+        return mul(g4[depth], mul(t0[e & 511], mul(t1[(e >> 9) & 511], t2[e >> 18])));
+    }
+};
+
+// In-place inverses of nonzero values (one field inversion for the batch).
+void batch_inverse(std::vector<uint32_t>& v) {
+    if (v.empty()) return;
+    std::vector<uint32_t> pre(v.size());
+    uint32_t acc = 1;
+    for (size_t i = 0; i < v.size(); ++i) {
+        pre[i] = acc;
+        acc = mul(acc, v[i]);
+    }
+    uint32_t inv = inv_c(acc);
+    for (size_t i = v.size(); i-- > 0;) {
+        const uint32_t vi = v[i];
+        v[i] = mul(inv, pre[i]);
+        inv = mul(inv, vi);
+    }
 }
 
 // fri.rs:1311-1331 with canonical arithmetic.
@@ -224,8 +261,25 @@ void absorb_fri_dims(Transcript& tr, uint64_t m, uint64_t poly_len, uint64_t bou
 }
 
 // fri::verify (fri.rs:686-859); returns the query indices.
+// Records a 64-byte-leaf path check for the device (directions already
+// checked against the position by the caller).
+void defer_path(PathChecks& d, const uint8_t leaf[64], const Path& path, const Digest& expected) {
+    PathCheck c{};
+    std::memcpy(c.leaf, leaf, 64);
+    std::memcpy(c.expected, expected.data(), 32);
+    c.steps = uint32_t(path.size());
+    c.sib = uint32_t(d.siblings.size() / 8);
+    for (size_t l = 0; l < path.size(); ++l) {
+        if (path[l].second) c.dirs |= uint64_t(1) << l;
+        uint32_t w[8];
+        words_of_digest(path[l].first, w);
+        d.siblings.insert(d.siblings.end(), w, w + 8);
+    }
+    d.checks.push_back(c);
+}
+
 std::vector<uint64_t> fri_verify_impl(const CommitmentV& cm, const FriProofV& pf, const z0p_fri_params& fp,
-                                      Transcript& tr) {
+                                      Transcript& tr, PathChecks* defer = nullptr) {
     validate_fri(fp);
     require(pf.queries.size() == fp.num_queries, Z0P_ERR_FRI_VERIFICATION_FAILED);
     require(pf.final_poly.size() == fp.final_size, Z0P_ERR_FRI_VERIFICATION_FAILED);
@@ -276,12 +330,27 @@ std::vector<uint64_t> fri_verify_impl(const CommitmentV& cm, const FriProofV& pf
     const uint32_t unit = root_of_unity(2);
     const uint64_t residual = cm.degree_bound >> (2 * folds);
     require(residual + 1 < fp.final_size, Z0P_ERR_FRI_VERIFICATION_FAILED);
+    const DomainPoints dom(ilog2(cm.lde_len));
     std::vector<uint32_t> fpts;
-    for (uint64_t j = 0; j < fp.final_size; ++j) fpts.push_back(point_at(cm.lde_len, folds, j));
+    for (uint64_t j = 0; j < fp.final_size; ++j) fpts.push_back(dom.at(folds, j));
     require(check_degree_ext(fpts, pf.final_poly, residual), Z0P_ERR_FRI_VERIFICATION_FAILED);
     require(layer_cap(pf.final_poly, fp.cap_height) == cm.caps.back(), Z0P_ERR_FRI_VERIFICATION_FAILED);
     std::vector<uint64_t> idx;
     for (size_t q = 0; q < pf.queries.size(); ++q) idx.push_back(tr.challenge_index("fri-query", cm.lde_len));
+    // 1 / x for every (query, depth) whose base index is in range, inverted
+    // as one batch; the loop below reads them in the same order.
+    std::vector<uint32_t> xinv;
+    for (size_t q = 0; q < pf.queries.size(); ++q) {
+        uint64_t index = idx[q];
+        for (uint64_t d = 0; d < std::min<uint64_t>(pf.queries[q].size(), folds); ++d) {
+            const uint64_t base = index / 4 * 4;
+            if (base + 3 >= (cm.lde_len >> (2 * d))) break;
+            xinv.push_back(dom.at(d, base));
+            index /= 4;
+        }
+    }
+    batch_inverse(xinv);
+    size_t xi = 0;
     for (size_t q = 0; q < pf.queries.size(); ++q) {
         const auto& folds_w = pf.queries[q];
         require(folds_w.size() == folds, Z0P_ERR_FRI_VERIFICATION_FAILED);
@@ -298,16 +367,28 @@ std::vector<uint64_t> fri_verify_impl(const CommitmentV& cm, const FriProofV& pf
             uint64_t qi = base / 4;
             uint64_t e = qi >> (depth_bits - drop);
             require(e < cap.size(), Z0P_ERR_BAD_MERKLE_PATH);
-            uint32_t x = point_at(cm.lde_len, d, base);
-            Digest node = quad_leaf_digest(w.values);
-            uint64_t pos = qi;
-            for (auto& st : w.path) {
-                require(st.second == ((pos & 1) == 0), Z0P_ERR_BAD_MERKLE_PATH);
-                node = st.second ? hash_node(node, st.first) : hash_node(st.first, node);
-                pos >>= 1;
+            const uint32_t x_inv = xinv[xi++];
+            if (defer) {
+                uint64_t pos = qi;
+                for (auto& st : w.path) {
+                    require(st.second == ((pos & 1) == 0), Z0P_ERR_BAD_MERKLE_PATH);
+                    pos >>= 1;
+                }
+                uint8_t leaf[64];
+                for (int k = 0; k < 4; ++k)
+                    for (int c = 0; c < 4; ++c) store_le32(leaf + 16 * k + 4 * c, w.values[k].v[c]);
+                defer_path(*defer, leaf, w.path, cap[e]);
+            } else {
+                Digest node = quad_leaf_digest(w.values);
+                uint64_t pos = qi;
+                for (auto& st : w.path) {
+                    require(st.second == ((pos & 1) == 0), Z0P_ERR_BAD_MERKLE_PATH);
+                    node = st.second ? hash_node(node, st.first) : hash_node(st.first, node);
+                    pos >>= 1;
+                }
+                require(node == cap[e], Z0P_ERR_BAD_MERKLE_PATH);
             }
-            require(node == cap[e], Z0P_ERR_BAD_MERKLE_PATH);
-            E4 parent = fold_values(w.values, inv_c(x), fc[d].data(), unit);
+            E4 parent = fold_values(w.values, x_inv, fc[d].data(), unit);
             if (d + 1 == folds_w.size()) {
                 uint64_t fi = index / 4;
                 require(fi < pf.final_poly.size() && e4_eq(pf.final_poly[fi], parent), Z0P_ERR_FRI_VERIFICATION_FAILED);
@@ -384,7 +465,15 @@ std::vector<uint64_t> fri_verify(const uint8_t* cmb, size_t cm_len, const uint8_
     return fri_verify_impl(cm, pf, fp, tr);
 }
 
-void air_verify(const z0p_air_params& p, Transcript& tr, const uint8_t* bytes, size_t len) {
+namespace {
+
+// The checks of verify_capped64 that come before the hashing.
+bool capped64_shape_ok(uint64_t index, uint64_t total_depth, const Path& path) {
+    return total_depth < 64 && path.size() <= total_depth && (index >> total_depth) == 0 &&
+           directions_match(index, path);
+}
+
+void air_verify_impl(const z0p_air_params& p, Transcript& tr, const uint8_t* bytes, size_t len, PathChecks* defer) {
     // Proof::from_bytes (air.rs:289-327)
     Reader r(bytes, len);
     r.version();
@@ -441,7 +530,7 @@ void air_verify(const z0p_air_params& p, Transcript& tr, const uint8_t* bytes, s
     E4 al[5];
     for (auto& a : al) a = tr.challenge_ext("fib/alpha");
     const z0p_fri_params fp = air_fri_params(p);
-    std::vector<uint64_t> idx = fri_verify_impl(cm, pf, fp, tr);
+    std::vector<uint64_t> idx = fri_verify_impl(cm, pf, fp, tr, defer);
     require(pow2(m), Z0P_ERR_FRI_VERIFICATION_FAILED);
     const uint32_t w_last = pow_c(root_of_unity(ilog2(n)), n - 1);
     const uint64_t log_m = ilog2(m);
@@ -463,20 +552,41 @@ void air_verify(const z0p_air_params& p, Transcript& tr, const uint8_t* bytes, s
         require(load_le32(o.leaf + ro) == o.a && load_le32(o.leaf + ro + 4) == o.b, Z0P_ERR_BAD_MERKLE_PATH);
         uint64_t e = (index / 8) >> (depth - drop);
         require(e < column_cap.size(), Z0P_ERR_BAD_MERKLE_PATH);
-        require(verify_capped64(column_cap[e], o.leaf, index / 8, depth, o.path), Z0P_ERR_BAD_MERKLE_PATH);
+        if (defer) {
+            require(capped64_shape_ok(index / 8, depth, o.path), Z0P_ERR_BAD_MERKLE_PATH);
+            defer_path(*defer, o.leaf, o.path, column_cap[e]);
+        } else {
+            require(verify_capped64(column_cap[e], o.leaf, index / 8, depth, o.path), Z0P_ERR_BAD_MERKLE_PATH);
+        }
         uint64_t shifted = (index + offset) % m;
         uint64_t so = (shifted & 7) * 8;
         require(load_le32(o.shift_leaf + so) == o.an && load_le32(o.shift_leaf + so + 4) == o.bn,
                 Z0P_ERR_BAD_MERKLE_PATH);
         uint64_t se = (shifted / 8) >> (depth - drop);
         require(se < column_cap.size(), Z0P_ERR_BAD_MERKLE_PATH);
-        require(verify_capped64(column_cap[se], o.shift_leaf, shifted / 8, depth, o.shift_path),
-                Z0P_ERR_BAD_MERKLE_PATH);
+        if (defer) {
+            require(capped64_shape_ok(shifted / 8, depth, o.shift_path), Z0P_ERR_BAD_MERKLE_PATH);
+            defer_path(*defer, o.shift_leaf, o.shift_path, column_cap[se]);
+        } else {
+            require(verify_capped64(column_cap[se], o.shift_leaf, shifted / 8, depth, o.shift_path),
+                    Z0P_ERR_BAD_MERKLE_PATH);
+        }
         uint32_t x = mul(GENERATOR, pow_c(omega_m, index));
         E4 expected = composition_at(o.a, o.b, o.an, o.bn, x, w_last, a0, b0, b_last, n, al);
         require(!pf.queries[q].empty(), Z0P_ERR_FRI_VERIFICATION_FAILED);
         require(e4_eq(pf.queries[q][0].values[leaf % 4], expected), Z0P_ERR_FRI_VERIFICATION_FAILED);
     }
+}
+
+}  // namespace
+
+void air_verify(const z0p_air_params& p, Transcript& tr, const uint8_t* bytes, size_t len) {
+    air_verify_impl(p, tr, bytes, len, nullptr);
+}
+
+void air_verify_deferred(const z0p_air_params& p, Transcript& tr, const uint8_t* bytes, size_t len,
+                         PathChecks& defer) {
+    air_verify_impl(p, tr, bytes, len, &defer);
 }
 
 void range_verify(const z0p_range_params& p, Transcript& tr, const uint8_t* bytes, size_t len) {
