@@ -369,7 +369,33 @@ def cpu_baseline(n, repeats):
     }
 
 
-def widened(z, ctx, reps, with_cpu):
+def fib_columns(n, a0=1, b0=1):
+    """build_trace(a0, b0, n) as two uint32 arrays (air.rs:341-354): A[i+1] =
+    B[i], B[i+1] = A[i] + B[i], computed in blocks by the Fibonacci matrix
+    powers so that 2^20 rows take milliseconds."""
+    import numpy as np
+    a = np.zeros(n, dtype=np.uint64)
+    b = np.zeros(n, dtype=np.uint64)
+    a[0], b[0] = a0 % P, b0 % P
+    blk = min(n, 1024)
+    for i in range(1, blk):
+        a[i] = b[i - 1]
+        b[i] = (a[i - 1] + b[i - 1]) % P
+    # (A[i+k], B[i+k]) = (F(k-1) A[i] + F(k) B[i], F(k) A[i] + F(k+1) B[i])
+    fib = [0, 1]
+    while len(fib) < blk + 2:
+        fib.append((fib[-1] + fib[-2]) % P)
+    fkm1, fk, fk1 = np.uint64(fib[blk - 1]), np.uint64(fib[blk]), np.uint64(fib[blk + 1])
+    p64 = np.uint64(P)
+    for s in range(blk, n, blk):
+        e = min(s + blk, n)
+        pa, pb = a[s - blk:e - blk], b[s - blk:e - blk]
+        a[s:e] = ((fkm1 * pa) % p64 + (fk * pb) % p64) % p64
+        b[s:e] = ((fk * pa) % p64 + (fk1 * pb) % p64) % p64
+    return a.astype(np.uint32), b.astype(np.uint32)
+
+
+def widened(z, ctx, reps, with_cpu, only=None):
     """The other provers on the same device path, each at a production-size
     workload: GPU time per proof (median of `reps` after one warm-up) and,
     unless skipped, the oracle's time on the host for the same proof (one
@@ -444,9 +470,18 @@ def widened(z, ctx, reps, with_cpu):
         return acc.finalize(t)
 
     cases.append(("fold", "fold::Accumulator, 4 segments of 2^16 rows, production shape", fold_gpu, fold_cpu))
+    # The two-column AIR from caller columns: two INTTs and two LDEs at 2^20.
+    wa, wb = fib_columns(n)
+    wp = z.air.for_security(n, 1, 1, int(wb[-1]))
+    cases.append(("air_with_trace", "air::prove_with_trace, N=2^20 columns from the caller (2 INTTs, 2 LDEs)",
+                  lambda: z.air.prove_with_trace(wp, wa, wb, z.Transcript(b"w/trace"), ctx),
+                  lambda: o.air_prove_with_trace(o.air_params_for_security(n, 1, 1, int(wb[-1])),
+                                                 list(map(int, wa)), list(map(int, wb)), o.Transcript(b"w/trace"))))
 
     out = {}
     for name, workload, gpu, cpu in cases:
+        if only and name != only:
+            continue
         got = gpu()  # warm-up
         times = []
         for _ in range(reps):
@@ -570,7 +605,9 @@ def main():
     ap.add_argument("--schedule", default=None,
                     help="JSON kernel schedule to use instead of tuning (z0p_schedule fields)")
     ap.add_argument("--widened", type=int, default=3,
-                    help="timed proofs per widened workload (ZK AIR, FRI ext, range, table, fold); 0 skips them")
+                    help="timed proofs per widened workload (ZK AIR, FRI ext, range, table, fold, AIR with "
+                         "caller columns); 0 skips them")
+    ap.add_argument("--widened-only", default=None, help="run only this widened workload (by its JSON key)")
     ap.add_argument("--copy-mib", type=int, default=256,
                     help="size of the plain copy kernel run after the proofs (counter calibration and "
                          "achievable-bandwidth reference); 0 disables")
@@ -708,7 +745,7 @@ def main():
     serve_sched = batch["schedule"] if batch else single_sched
     serve = served(z, args.serve_budget_gib, args.serve_s, serve_sched, single_sched) if args.serve_s > 0 else None
     ctx.set_schedule(single_sched)
-    wide = widened(z, ctx, args.widened, not args.no_cpu) if args.widened > 0 else None
+    wide = widened(z, ctx, args.widened, not args.no_cpu, args.widened_only) if args.widened > 0 else None
 
     out = {
         "metric": METRIC,
