@@ -266,6 +266,7 @@ struct Schedule {
     int subtree_levels = 0;      // tree levels above level 1 reduced in LDS per leaf workgroup (0: off)
     int compose_subtree = 0;     // the same inside the composition kernel (its own limit; 0: off)
     int graph = 0;               // each iteration's launches captured into one HIP graph (0: direct launches)
+    int lde_split = 1;           // LDE row and column passes in kernels of their own (0: inside k_ntt)
 
     // Digests per tail workgroup (log2) for a level of `count` digests.
     int tail_group_log(uint64_t count) const {
@@ -284,7 +285,8 @@ struct Schedule {
                (fold_fuse_max_log == 0 || (fold_fuse_max_log >= 3 && fold_fuse_max_log <= 27)) &&
                (compose_fuse == 0 || compose_fuse == 1) && (trace_fuse == 0 || trace_fuse == 1) &&
                subtree_levels >= 0 && subtree_levels <= SUBTREE_MAX_LEVELS && compose_subtree >= 0 &&
-               compose_subtree <= SUBTREE_MAX_LEVELS && (graph == 0 || graph == 1);
+               compose_subtree <= SUBTREE_MAX_LEVELS && (graph == 0 || graph == 1) &&
+               (lde_split == 0 || lde_split == 1);
     }
 };
 
@@ -350,7 +352,7 @@ struct Ctx {
     PinnedPair verify_stage[2][3];
     hipEvent_t verify_done[2] = {nullptr, nullptr};
 
-    Launch launch() { return Launch{stream, &prof, &ring, false}; }
+    Launch launch() { return Launch{stream, &prof, &ring, false, sched.lde_split != 0}; }
 
     // Runs a launch sequence dry (tables only), uploads the tables with one
     // copy, then replays it for real: as direct launches, or (schedule

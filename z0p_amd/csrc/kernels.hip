@@ -552,15 +552,24 @@ __global__ __launch_bounds__(WG) void k_ntt(NttJob inl, const NttJob* tab, uint3
             break;
         case NTT_LDE_ROWS: {
             const uint32_t n1 = 1u << J.log1;
-            if (J.mask)
+            if (J.log2 == 12) {
+                if (J.mask)
+                    lde_rows_4096<true>(J, lds, local % n1, local / n1);
+                else
+                    lde_rows_4096<false>(J, lds, local % n1, local / n1);
+            } else if (J.mask) {
                 ntt_lde_rows<true>(J, lds, local % n1, local / n1);
-            else
+            } else {
                 ntt_lde_rows<false>(J, lds, local % n1, local / n1);
+            }
             break;
         }
         case NTT_LDE_COLS: {
             const uint32_t per = (1u << J.log2) / uint32_t(J.cols);
-            ntt_lde_cols(J, lds, local % per, local / per);
+            if (cols_in_regs(J.log1) && J.log2 == 12)
+                lde_cols_tile(J, lds, local % per, local / per);
+            else
+                ntt_lde_cols(J, lds, local % per, local / per);
             break;
         }
         default:
@@ -1859,7 +1868,7 @@ size_t ntt_lds(const NttJob& j) {
         case NTT_INTT_SMALL:
         case NTT_LDE_SMALL: return j.logn == 12 ? size_t(N4K_LDS) * 4 : (size_t(1) << j.logn) * 6;
         case NTT_INTT_ROWS:
-        case NTT_LDE_ROWS: return (size_t(1) << j.log2) * 6;  // rows of 4096 run in k_lde_rows
+        case NTT_LDE_ROWS: return j.log2 == 12 ? size_t(N4K_LDS) * 4 : (size_t(1) << j.log2) * 6;
         case NTT_INTT_COLS:
         case NTT_LDE_COLS:
             if (j.log1 >= 4 && j.log1 <= 12) return ((size_t(1) << j.log1) + N4K) * 4;  // pitch C + 1
@@ -1965,7 +1974,8 @@ bool lde_cols_job(const NttJob& j) { return j.kind == NTT_LDE_COLS && j.log2 == 
 
 void launch_ntt(const Launch& L, std::vector<NttJob> jobs) {
     std::vector<NttJob> rows, cols, rest;
-    for (const NttJob& j : jobs) (lde_rows_job(j) ? rows : lde_cols_job(j) ? cols : rest).push_back(j);
+    for (const NttJob& j : jobs)
+        (!L.lde_split ? rest : lde_rows_job(j) ? rows : lde_cols_job(j) ? cols : rest).push_back(j);
     size_t lds = 0;
     bool intt = false;
     for (const NttJob& j : rest) {

@@ -33,6 +33,7 @@ struct Launch {
     Profiler* prof;   // may be null
     TableSink* sink;  // required for launches with more than one job
     bool dry = false;
+    bool lde_split = true;  // 2^16..2^24-point LDE passes in kernels of their own (else in k_ntt)
 };
 
 // Two-level power tables for omega_m^{+-e}, e < m:
