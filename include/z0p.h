@@ -397,7 +397,7 @@ int z0p_ctx_set_hbm_budget(z0p_ctx* ctx, uint64_t bytes);
  * alternating waves once at least `min_active` are active (0: never), so the
  * host's transcript steps for one wave overlap the other wave's kernels. The
  * proofs are the same either way. Default 8. A server started on the ctx
- * uses the setting in every lane. */
+ * uses the setting in both of its lanes. */
 int z0p_ctx_set_waves(z0p_ctx* ctx, uint32_t min_active);
 /* Runs air::prove for every params[i] with transcripts[i] (each advanced as
  * that single call would advance it). The instances share every kernel
@@ -463,20 +463,17 @@ size_t z0p_ctx_autotune_log(z0p_ctx* ctx, char* buf, size_t cap);
 
 /* ---- serving (no reference counterpart) ----
  * An asynchronous request loop that takes over the ctx until
- * z0p_server_destroy. Requests below 2^16 steps, of 2^16..2^18 steps and of
- * 2^19 steps and more run in three lanes, each a scheduler on its own host
- * thread and HIP stream (the larger lanes on contexts the server creates on
- * the same device with the ctx's kernel schedule), so short requests are not
- * held behind long ones.
+ * z0p_server_destroy. Requests below 2^16 steps and the larger ones run in two
+ * lanes, each a scheduler on its own host thread and HIP stream (the large
+ * lane on a second context the server creates on the same device with the
+ * ctx's kernel schedule), so short requests are not held behind long ones.
  * Requests arriving while others run are admitted into their lane's running
  * batch as device memory and slots free up, so their device work overlaps;
  * a request alone in its lane runs all its phases back to back. Every result
  * is byte-identical to the proof z0p_air_prove returns for the same request. */
 typedef struct z0p_server z0p_server;
 /* hbm_budget 0: half of the free device memory; the small lane gets half of
- * the budget, at most 1 GiB, the medium lane a quarter of the rest, at most
- * 8 GiB (below 512 MiB the medium requests share the large lane), the large
- * lane the remainder. max_active 0: 256 per lane. */
+ * the budget, at most 1 GiB. max_active 0: 256 per lane. */
 int z0p_server_create(z0p_ctx* ctx, uint64_t hbm_budget, uint32_t max_active, z0p_server** out);
 /* Queues one AIR proof. The transcript is copied: the request proves over
  * its own copy and the caller's transcript is left unchanged. */

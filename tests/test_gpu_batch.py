@@ -187,7 +187,7 @@ def test_batch_waves_on_and_off(ctx, waves):
 
 
 def test_server_two_waves_match_alone(ctx):
-    """Every lane alternating two waves from 2 active proofs: every result is
+    """Both lanes alternating two waves from 2 active proofs: every result is
     the proof the request gets alone."""
     inst = _instances(24, 8, 17, 41)
     params = [z.air.for_security(n, a0, b0, bl) for n, a0, b0, bl, _ in inst]

@@ -65,21 +65,20 @@ struct ServerRequest {
     }
 };
 
-// Requests go to lanes by size: below 2^16 steps, 2^16..2^18, and 2^19 up.
-// Each lane is one scheduler on its own stream (and its own context), run by
-// its own host thread: short proofs keep cycling through their phases while
-// long ones occupy the other lanes, and the streams' kernels share the
-// device. Each lane batches proofs of similar phase lengths, so an iteration
-// of the medium lane does not wait for a 2^20 proof's phase.
-constexpr int LANES = 3;
-constexpr uint64_t LANE_MIN_LOG[LANES] = {0, 16, 19};
+// Requests of at least 2^LARGE_LOG steps go to the large-proof lane, the rest
+// to the small-proof lane. Each lane is one scheduler on its own stream (and
+// its own context), run by its own host thread: short proofs keep cycling
+// through their phases while long ones occupy the other lane, and the two
+// streams' kernels share the device.
+constexpr uint64_t LARGE_LOG = 16;
+constexpr int LANES = 2;
 
 struct z0p_server;
 
 struct Lane {
     z0p_server* s = nullptr;
     z0p::Ctx* c = nullptr;
-    bool owned = false;  // created by the server (every lane but the first)
+    bool owned = false;  // created by the server (the large lane)
     size_t budget = 0;
     std::unique_ptr<z0p::Engine> engine;
     std::thread worker;
@@ -92,12 +91,6 @@ struct Lane {
 struct z0p_server {
     size_t max_active = DEFAULT_MAX_ACTIVE;
     Lane lanes[LANES];
-    bool medium = true;  // false: the 2^16..2^18 requests share the large lane
-    int lane_of(uint64_t n_steps) const {
-        int l = 0;
-        while (l + 1 < LANES && n_steps >= (uint64_t(1) << LANE_MIN_LOG[l + 1])) ++l;
-        return l == 1 && !medium ? 2 : l;
-    }
     std::mutex mu;
     std::condition_variable cv_done;
     bool stop = false;
@@ -263,32 +256,18 @@ int z0p_server_create(z0p_ctx* ctx, uint64_t hbm_budget, uint32_t max_active, z0
             hbm_budget = free_b / 2;
         }
         s->max_active = max_active ? max_active : DEFAULT_MAX_ACTIVE;
-        // The caller's context serves the small lane; the others get contexts
-        // of their own (stream, pool, tables) with the caller's kernel schedule
-        // and wave setting. Budget shares: the small lane half of the budget,
-        // at most 1 GiB (a 2^15-step proof needs ~32 MiB); the medium lane a
-        // quarter of the rest, at most 8 GiB (a 2^18-step proof needs ~256
-        // MiB); the large lane the remainder.
-        s->lanes[0].c = small_ctx;
-        for (int l = 1; l < LANES; ++l) {
-            Lane& ln = s->lanes[l];
-            ln.c = z0p::ctx_create(small_ctx->device);
-            ln.owned = true;
-            ln.c->sched = small_ctx->sched;
-            ln.c->waves_min = small_ctx->waves_min;
-        }
-        const size_t total = size_t(hbm_budget);
-        s->lanes[0].budget = std::min<size_t>(total / 2, size_t(1) << 30);
-        const size_t rest = total - s->lanes[0].budget;
-        s->lanes[1].budget = std::min<size_t>(rest / 4, size_t(8) << 30);
-        // The medium lane needs room for a couple of its largest requests (a
-        // 2^18-step production proof holds ~270 MB); below 512 MiB its
-        // requests share the large lane instead.
-        if (s->lanes[1].budget < (size_t(512) << 20)) {
-            s->medium = false;
-            s->lanes[1].budget = 0;
-        }
-        s->lanes[2].budget = rest - s->lanes[1].budget;
+        // The large lane gets its own context (stream, pool, tables) with the
+        // caller's kernel schedule and wave setting. The small lane's share of the budget is
+        // half of it, at most 1 GiB (a 2^15-step proof needs ~32 MiB).
+        Lane& small = s->lanes[0];
+        Lane& large = s->lanes[1];
+        small.c = small_ctx;
+        large.c = z0p::ctx_create(small_ctx->device);
+        large.owned = true;
+        large.c->sched = small_ctx->sched;
+        large.c->waves_min = small_ctx->waves_min;
+        small.budget = std::min<size_t>(size_t(hbm_budget) / 2, size_t(1) << 30);
+        large.budget = size_t(hbm_budget) - small.budget;
         for (Lane& l : s->lanes) {
             l.s = s.get();
             l.engine.reset(new z0p::Engine(*l.c, l.budget, s->max_active));
@@ -315,7 +294,7 @@ int z0p_server_submit_air(z0p_server* s, const z0p_air_params* params, const z0p
         r->tr = transcript->t;
         r->params = *params;
         r->submit_ms = now_ms();
-        Lane& lane = s->lanes[s->lane_of(params->n_steps)];
+        Lane& lane = s->lanes[params->n_steps >= (uint64_t(1) << LARGE_LOG) ? 1 : 0];
         std::lock_guard<std::mutex> lk(s->mu);
         r->id = s->next_id++;
         *request_id = r->id;
