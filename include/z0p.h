@@ -446,6 +446,8 @@ typedef struct z0p_schedule {
     int32_t compose_subtree;    /* the same inside the composition kernel (0 = off, 1..8) */
     int32_t graph;              /* launch each scheduler iteration as one captured HIP graph (0 or 1) */
     int32_t lde_split;          /* LDE row and column passes in kernels of their own, else in the shared NTT kernel (0 or 1) */
+    int32_t grind_pool;         /* workgroups of the pooled proof-of-work search when several proofs grind in one launch
+                                 * (0 = each proof on workgroups of its own; 0..8192) */
 } z0p_schedule;
 int z0p_ctx_get_schedule(z0p_ctx* ctx, z0p_schedule* out);
 /* Z0P_ERR_INVALID_ARGUMENT for a value outside the ranges above. */

@@ -94,6 +94,67 @@ def test_batch_rejects_shared_transcript(ctx):
     assert e.value.name == "InvalidArgument"
 
 
+def test_batch_pooled_grind_across_windows(ctx):
+    """Several 26-bit proofs-of-work in one launch go through the pooled
+    search (every wave takes chunks of any unfinished proof); their smallest
+    nonces lie beyond the first window of 2^24 nonces for some of them, so
+    the search also continues window by window. Each proof must equal its
+    single-instance proof (whose 26-bit grind is pinned to the oracle) and
+    the per-proof workgroup search of the same batch."""
+    n = 64
+    inst = [(2, 3, b"g26a"), (5, 8, b"g26b"), (1, 1, b"g26c"), (7, 9, b"g26d")]
+    params = []
+    for a0, b0, _ in inst:
+        p = z.air.params_new(n, 8, 6, 8, a0, b0, o.fib_b_last(a0, b0, n))
+        p.grinding_bits = 26
+        params.append(p)
+    saved = ctx.get_schedule()
+    try:
+        ctx.set_schedule(dict(saved, grind_pool=1024))
+        print("pooled 26-bit batch", flush=True)
+        pooled = z.air.prove_batch(params, [z.Transcript(lab) for *_, lab in inst], ctx)
+        ctx.set_schedule(dict(saved, grind_pool=0))
+        print("per-proof 26-bit batch", flush=True)
+        split = z.air.prove_batch(params, [z.Transcript(lab) for *_, lab in inst], ctx)
+    finally:
+        ctx.set_schedule(saved)
+    assert pooled == split
+    for p, (_, _, lab), g in zip(params, inst, pooled):
+        assert g == z.air.prove(p, z.Transcript(lab), ctx), lab
+    # The first instance's smallest nonce (103,769,303) is pinned by the
+    # single-proof test against the oracle.
+    assert pooled[0] == z.air.prove(params[0], z.Transcript(b"g26a"), ctx)
+
+
+def test_batch_and_server_zero_copy_views(ctx):
+    """zero_copy hands the C ABI's buffers over as read-only memoryviews:
+    equal to the bytes results, and released when dropped."""
+    import gc
+    inst = _instances(6, 6, 12, 5)
+    params = [z.air.for_security(n, a0, b0, bl) for n, a0, b0, bl, _ in inst]
+    want = z.air.prove_batch(params, [z.Transcript(lab) for *_, lab in inst], ctx)
+    views = z.air.prove_batch(params, [z.Transcript(lab) for *_, lab in inst], ctx, zero_copy=True)
+    for v, w in zip(views, want):
+        assert isinstance(v, memoryview) and v.readonly
+        assert v == w and bytes(v) == w
+    assert z.air.verify_batch(params, views, [z.Transcript(lab) for *_, lab in inst]) == [None] * len(inst)
+    del views
+    gc.collect()
+    sctx = z.Context(0)
+    with z.Server(sctx, hbm_budget=1 << 30) as srv:
+        ids = {srv.submit(p, z.Transcript(lab)): i for i, (p, (*_, lab)) in enumerate(zip(params, inst))}
+        got = {}
+        deadline = time.time() + 60
+        while len(got) < len(inst) and time.time() < deadline:
+            r = srv.poll(200, zero_copy=True)
+            if r is not None:
+                got[r[0]] = r[1]
+        assert len(got) == len(inst)
+        for rid, v in got.items():
+            assert isinstance(v, memoryview) and v == want[ids[rid]]
+    sctx.close()
+
+
 def _serve(requests, budget, gaps_s, waves=None):
     """Submit from a producer thread with the given inter-arrival gaps, poll
     from this thread; returns {request index: (proof, latency, queue)}."""

@@ -233,6 +233,37 @@ def test_verify_batch_matches_single_verifies():
     assert z.air.verify_batch([], [], []) == []
 
 
+def test_zero_copy_view_releases_buffer():
+    # z0p_free releases what malloc returned, so a malloc'd buffer stands in
+    # for one the library hands over: the view reads it without a copy, is
+    # read-only, compares equal to bytes, and frees it once the last view of
+    # it (slices included) is gone.
+    import ctypes
+    import gc
+    import weakref
+    from z0p_amd import _ffi
+    libc = ctypes.CDLL(None)
+    libc.malloc.restype = ctypes.c_void_p
+    libc.malloc.argtypes = [ctypes.c_size_t]
+    addr = libc.malloc(4096)
+    ctypes.memmove(addr, bytes(range(256)) * 16, 4096)
+    v = _ffi.view(ctypes.cast(addr, _ffi.u8p), 4096)
+    assert v.readonly and len(v) == 4096 and v == bytes(range(256)) * 16
+    with pytest.raises(TypeError):
+        v[0] = 1
+    part = v[256:512]
+    assert bytes(part) == bytes(range(256))
+    owner = v.obj
+    alive = weakref.ref(owner)
+    del v, owner
+    gc.collect()
+    assert alive() is not None  # the slice still holds the buffer
+    del part
+    gc.collect()
+    assert alive() is None
+    assert _ffi.view(ctypes.cast(libc.malloc(8), _ffi.u8p), 0) == b""
+
+
 def test_fibonacci_interpolant_closed_form():
     # The device writes the Fibonacci columns' coefficients in closed form
     # (Binet: A[i] = c1 phi^i + c2 psi^i, phi, psi the roots of x^2 = x + 1,
@@ -285,7 +316,7 @@ def test_bench_tree_work_is_schedule_invariant(logn, cap_height):
         L //= 4
     base = dict(tail_max_log=17, tail_group_max_log=7, tail_spread_log=0, levels_fuse=2, small_max_log=0,
                 grind_blocks=4096, fold_fuse_max_log=0, compose_fuse=0, trace_fuse=1, subtree_levels=0,
-                compose_subtree=0, graph=0, lde_split=1)
+                compose_subtree=0, graph=0, lde_split=1, grind_pool=1024)
     # unfused: every tree in the merkle family, leaf + 2 per node up to the cap
     cap = 1 << cap_height
     want = sum(k + 2 * (k - min(cap, k)) for k in [m // 8] + [x // 4 for x in layers])

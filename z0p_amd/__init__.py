@@ -223,8 +223,10 @@ class Server:
         check(lib().z0p_server_submit_air(self._h, ctypes.byref(params), transcript.handle, ctypes.byref(rid)))
         return rid.value
 
-    def poll(self, timeout_ms: int = 0):
-        from ._ffi import u8p
+    def poll(self, timeout_ms: int = 0, zero_copy: bool = False):
+        """With ``zero_copy`` the proof is a read-only memoryview of the
+        buffer the C ABI hands over instead of a ``bytes`` copy."""
+        from ._ffi import u8p, view
         rid = ctypes.c_uint64()
         st = ctypes.c_int()
         out = u8p()
@@ -236,7 +238,7 @@ class Server:
         if rc == 1:
             return None
         check(rc)
-        res = take(out, n.value) if st.value == 0 else Z0pError(st.value)
+        res = (view if zero_copy else take)(out, n.value) if st.value == 0 else Z0pError(st.value)
         return rid.value, res, lat.value, q.value
 
     def pending(self) -> int:

@@ -84,7 +84,8 @@ class FriParamsC(ctypes.Structure):
 class ScheduleC(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int32) for name in (
         "tail_max_log", "tail_group_max_log", "tail_spread_log", "levels_fuse", "small_max_log", "grind_blocks",
-        "fold_fuse_max_log", "compose_fuse", "trace_fuse", "subtree_levels", "compose_subtree", "graph", "lde_split")]
+        "fold_fuse_max_log", "compose_fuse", "trace_fuse", "subtree_levels", "compose_subtree", "graph", "lde_split",
+        "grind_pool")]
 
     def to_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -266,3 +267,17 @@ def take(ptr, n):
     out = ctypes.string_at(ptr, n)
     lib().z0p_free(ptr)
     return out
+
+
+def view(ptr, n):
+    """Zero-copy, read-only ``memoryview`` of a buffer the library handed
+    over; the buffer is released (z0p_free) once the view and every view
+    derived from it are gone. Compares equal to ``bytes`` of the same content."""
+    import weakref
+    addr = ctypes.cast(ptr, ctypes.c_void_p).value
+    if n == 0 or not addr:
+        lib().z0p_free(ptr)
+        return memoryview(b"")
+    arr = (ctypes.c_uint8 * n).from_address(addr)
+    weakref.finalize(arr, lib().z0p_free, ctypes.c_void_p(addr))
+    return memoryview(arr).cast("B").toreadonly()

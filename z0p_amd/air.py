@@ -1,7 +1,7 @@
 """Fibonacci AIR prover/verifier (reference: src/air.rs)."""
 import ctypes
 
-from ._ffi import AirParamsC, Z0pError, buf, check, lib, take, u8p
+from ._ffi import AirParamsC, Z0pError, buf, check, lib, take, u8p, view
 
 Params = AirParamsC
 
@@ -86,11 +86,13 @@ def prove_zk(params: Params, n_rand: int, rng, transcript, ctx=None) -> bytes:
     return take(out, n.value)
 
 
-def prove_batch(params_list, transcripts, ctx=None):
+def prove_batch(params_list, transcripts, ctx=None, zero_copy=False):
     """``air.prove`` for every (params, transcript) pair in one batch: the
     instances share every kernel launch and are admitted as device memory
     allows. Returns a list of proof bytes, or of Z0pError for instances that
     failed; each transcript is advanced as its single call would advance it.
+    With ``zero_copy`` the proofs are read-only memoryviews of the buffers the
+    C ABI hands over (no copy into ``bytes``).
     """
     from . import default_context
     ctx = ctx or default_context()
@@ -105,7 +107,7 @@ def prove_batch(params_list, transcripts, ctx=None):
     check(lib().z0p_air_prove_batch(ctx.handle, ps, ts, count, outs, lens, sts))
     result = []
     for i in range(count):
-        result.append(take(outs[i], lens[i]) if sts[i] == 0 else Z0pError(sts[i]))
+        result.append((view if zero_copy else take)(outs[i], lens[i]) if sts[i] == 0 else Z0pError(sts[i]))
     return result
 
 

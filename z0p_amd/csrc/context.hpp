@@ -267,6 +267,7 @@ struct Schedule {
     int compose_subtree = 0;     // the same inside the composition kernel (its own limit; 0: off)
     int graph = 0;               // each iteration's launches captured into one HIP graph (0: direct launches)
     int lde_split = 1;           // LDE row and column passes in kernels of their own (0: inside k_ntt)
+    int grind_pool = 1024;       // workgroups of the pooled search when several proofs grind (0: per-proof workgroups)
 
     // Digests per tail workgroup (log2) for a level of `count` digests.
     int tail_group_log(uint64_t count) const {
@@ -286,7 +287,7 @@ struct Schedule {
                (compose_fuse == 0 || compose_fuse == 1) && (trace_fuse == 0 || trace_fuse == 1) &&
                subtree_levels >= 0 && subtree_levels <= SUBTREE_MAX_LEVELS && compose_subtree >= 0 &&
                compose_subtree <= SUBTREE_MAX_LEVELS && (graph == 0 || graph == 1) &&
-               (lde_split == 0 || lde_split == 1);
+               (lde_split == 0 || lde_split == 1) && grind_pool >= 0 && grind_pool <= 8192;
     }
 };
 
@@ -352,7 +353,7 @@ struct Ctx {
     PinnedPair verify_stage[2][3];
     hipEvent_t verify_done[2] = {nullptr, nullptr};
 
-    Launch launch() { return Launch{stream, &prof, &ring, false, sched.lde_split != 0}; }
+    Launch launch() { return Launch{stream, &prof, &ring, false, sched.lde_split != 0, uint32_t(sched.grind_pool)}; }
 
     // Runs a launch sequence dry (tables only), uploads the tables with one
     // copy, then replays it for real: as direct launches, or (schedule

@@ -1674,15 +1674,44 @@ __global__ __launch_bounds__(SMALL_WG) void k_small_commit(SmallCommitJob inl, c
 // or below it is scanned.
 constexpr int GRIND_PER_THREAD = 4;
 
-// Nonce scan with the nonce starting in tail word W (byte offset 4W + s/8);
-// the word is a template argument so the message stays in registers.
+// One nonce of a job whose nonce starts in tail word W (byte offset 4W + s/8;
+// the word is a template argument so the message stays in registers): its
+// digest is compared with the difficulty and a pass lowers the job's minimum.
 template <int W>
-__device__ __forceinline__ void grind_scan(const GrindJob& J, uint32_t local) {
-    uint32_t tw[32], cv[8];
+__device__ __forceinline__ void grind_try(const GrindJob& J, const uint32_t (&tw)[32], const uint32_t (&cv)[8],
+                                          uint32_t s, bool two, uint64_t nonce) {
+    const uint32_t lo = uint32_t(nonce), hi = uint32_t(nonce >> 32);
+    // The nonce's 8 LE bytes spread over words W..W+2.
+    const uint32_t c0 = lo << s;
+    const uint32_t c1 = s ? (lo >> (32u - s)) | (hi << s) : hi;
+    const uint32_t c2 = s ? hi >> (32u - s) : 0u;
+    uint32_t m[32];
+#pragma unroll
+    for (int w = 0; w < 32; ++w) m[w] = tw[w];
+    m[W] |= c0;
+    m[W + 1] |= c1;
+    if (W + 2 < 32) m[W + 2] |= c2;
+    uint32_t d[8];
+    compress8(cv, m, 0, 0, J.len0, J.flags0, d);
+    if (W >= 14 && two) compress8(d, m + 16, 0, 0, J.len1, b3::CHUNK_END | b3::ROOT, d);
+    const uint32_t be = __builtin_bswap32(d[0]);
+    const uint32_t lz = be == 0 ? 32u : uint32_t(__builtin_clz(be));
+    if (lz >= J.bits) atomicMin(J.result, (unsigned long long)nonce);
+}
+
+template <int W>
+__device__ __forceinline__ void grind_load_head(const GrindJob& J, uint32_t (&tw)[32], uint32_t (&cv)[8]) {
 #pragma unroll
     for (int w = 0; w < 32; ++w) tw[w] = gc(J.head->tail)[w];
 #pragma unroll
     for (int w = 0; w < 8; ++w) cv[w] = gc(J.head->cv)[w];
+}
+
+// Nonce scan of one job by its own workgroups (strided chunks).
+template <int W>
+__device__ __forceinline__ void grind_scan(const GrindJob& J, uint32_t local) {
+    uint32_t tw[32], cv[8];
+    grind_load_head<W>(J, tw, cv);
     const uint32_t s = 8u * (J.nonce_off & 3u);
     const bool two = J.len1 != 0;
     const uint64_t chunk = uint64_t(WG) * GRIND_PER_THREAD;
@@ -1692,33 +1721,121 @@ __device__ __forceinline__ void grind_scan(const GrindJob& J, uint32_t local) {
         for (int k = 0; k < GRIND_PER_THREAD; ++k) {
             const uint64_t nonce = start + uint64_t(k) * WG + threadIdx.x;
             if (nonce >= J.limit) break;
-            const uint32_t lo = uint32_t(nonce), hi = uint32_t(nonce >> 32);
-            // The nonce's 8 LE bytes spread over words W..W+2.
-            const uint32_t c0 = lo << s;
-            const uint32_t c1 = s ? (lo >> (32u - s)) | (hi << s) : hi;
-            const uint32_t c2 = s ? hi >> (32u - s) : 0u;
-            uint32_t m[32];
-#pragma unroll
-            for (int w = 0; w < 32; ++w) m[w] = tw[w];
-            m[W] |= c0;
-            m[W + 1] |= c1;
-            if (W + 2 < 32) m[W + 2] |= c2;
-            uint32_t d[8];
-            compress8(cv, m, 0, 0, J.len0, J.flags0, d);
-            if (W >= 14 && two) compress8(d, m + 16, 0, 0, J.len1, b3::CHUNK_END | b3::ROOT, d);
-            const uint32_t be = __builtin_bswap32(d[0]);
-            const uint32_t lz = be == 0 ? 32u : uint32_t(__builtin_clz(be));
-            if (lz >= J.bits) atomicMin(J.result, (unsigned long long)nonce);
+            grind_try<W>(J, tw, cv, s, two, nonce);
         }
     }
 }
 
+// Pooled search over the jobs of one launch (batches): every workgroup takes
+// GRIND_POOL_CHUNK-nonce chunks of its current job, in increasing order, from
+// the job's counter (the u64 after its running minimum) and scans them. A job
+// with no chunk left below its minimum or its limit is finished, and the
+// workgroup moves to another job, so no job's tail holds the device while
+// workgroups idle. Every chunk that starts below a job's final minimum is
+// handed out and scanned, so the minimum is exactly the smallest passing
+// nonce. A chunk is 16 nonces per thread: one counter update per 4096 nonces
+// keeps the device-scope atomics on a job's counter rare.
+constexpr int GRIND_POOL_PER_THREAD = 16;
+constexpr uint64_t GRIND_POOL_CHUNK = uint64_t(WG) * GRIND_POOL_PER_THREAD;
+
+__device__ __forceinline__ unsigned long long* grind_counter(const GrindJob& J) { return J.result + 1; }
+
+template <int W>
+__device__ __forceinline__ void grind_pool_job(const GrindJob& J, unsigned long long* slot) {
+    uint32_t tw[32], cv[8];
+    grind_load_head<W>(J, tw, cv);
+    const uint32_t s = 8u * (J.nonce_off & 3u);
+    const bool two = J.len1 != 0;
+    for (;;) {
+        __syncthreads();  // everyone has read the previous chunk's start
+        if (threadIdx.x == 0) {
+            // The first thread takes the chunk and decides for the workgroup.
+            const unsigned long long c =
+                __hip_atomic_fetch_add(grind_counter(J), 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t st = J.start + uint64_t(c) * GRIND_POOL_CHUNK;
+            const bool done =
+                st >= J.limit || st >= __hip_atomic_load(J.result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            *slot = done ? ~0ull : st;
+        }
+        __syncthreads();
+        const uint64_t start = *slot;
+        if (start == ~0ull) return;  // workgroup-uniform
+        for (int k = 0; k < GRIND_POOL_PER_THREAD; ++k) {
+            const uint64_t nonce = start + uint64_t(k) * WG + threadIdx.x;
+            if (nonce < J.limit) grind_try<W>(J, tw, cv, s, two, nonce);
+        }
+    }
+}
+
+// True while the job still has a chunk that starts below its minimum.
+__device__ __forceinline__ bool grind_job_live(const GrindJob& J) {
+    const uint64_t next = __hip_atomic_load(grind_counter(J), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t start = J.start + next * GRIND_POOL_CHUNK;
+    return start < J.limit && start < __hip_atomic_load(J.result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Running minimum to "none", pooled chunk counter to 0.
 __global__ void k_grind_init(GrindJob inl, const GrindJob* tab, uint32_t njobs) {
     uint32_t i = blockIdx.x * WG + threadIdx.x;
+    unsigned long long* r = nullptr;
     if (njobs == 0) {
-        if (i == 0) *inl.result = ~0ull;
+        if (i == 0) r = inl.result;
     } else if (i < njobs) {
-        *tab[i].result = ~0ull;
+        r = tab[i].result;
+    }
+    if (r) {
+        r[0] = ~0ull;
+        r[1] = 0;
+    }
+}
+
+// Workgroup w starts on job (w mod njobs); a workgroup whose job is finished
+// has its first wave look at the next jobs in order (64 at a time) and takes
+// the first live one, and it exits when none is left.
+__global__ __launch_bounds__(WG) void k_grind_pool(const GrindJob* tab, uint32_t njobs) {
+    __shared__ unsigned long long slot;
+    __shared__ uint32_t next_job;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t j = blockIdx.x % njobs;
+    for (;;) {
+        const GrindJob J = tab[j];
+        switch (J.nonce_off >> 2) {
+            case 0: grind_pool_job<0>(J, &slot); break;
+            case 1: grind_pool_job<1>(J, &slot); break;
+            case 2: grind_pool_job<2>(J, &slot); break;
+            case 3: grind_pool_job<3>(J, &slot); break;
+            case 4: grind_pool_job<4>(J, &slot); break;
+            case 5: grind_pool_job<5>(J, &slot); break;
+            case 6: grind_pool_job<6>(J, &slot); break;
+            case 7: grind_pool_job<7>(J, &slot); break;
+            case 8: grind_pool_job<8>(J, &slot); break;
+            case 9: grind_pool_job<9>(J, &slot); break;
+            case 10: grind_pool_job<10>(J, &slot); break;
+            case 11: grind_pool_job<11>(J, &slot); break;
+            case 12: grind_pool_job<12>(J, &slot); break;
+            case 13: grind_pool_job<13>(J, &slot); break;
+            case 14: grind_pool_job<14>(J, &slot); break;
+            case 15: grind_pool_job<15>(J, &slot); break;
+            default: return;  // unreachable: the nonce starts in the first 64 tail bytes
+        }
+        // Offsets 1..njobs from j (j itself last); njobs means none is live.
+        if (threadIdx.x < 64) {
+            uint32_t pick = njobs;
+            for (uint32_t base = 1; base <= njobs; base += 64) {
+                const uint32_t k = base + lane;
+                const bool live = k <= njobs && grind_job_live(tab[(j + k) % njobs]);
+                const unsigned long long mask = __ballot(live);
+                if (mask) {
+                    pick = (j + base + uint32_t(__builtin_ctzll(mask))) % njobs;
+                    break;
+                }
+            }
+            if (threadIdx.x == 0) next_job = pick;
+        }
+        __syncthreads();
+        j = next_job;
+        __syncthreads();  // next_job is read by everyone before it is written again
+        if (j == njobs) return;
     }
 }
 
@@ -1746,6 +1863,7 @@ __global__ __launch_bounds__(WG) void k_grind(GrindJob inl, const GrindJob* tab,
     }
 }
 
+// ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
 constexpr int COLLECT_WG_MAX = 1024;
 constexpr uint32_t COLLECT_LDS_ITEMS = 8192;
@@ -2119,7 +2237,12 @@ void launch_grind(const Launch& L, std::vector<GrindJob> jobs) {
                            jobs[0], table, n);
     }
     ProfScope ps(L.prof, "grind", L.stream);
-    hipLaunchKernelGGL(k_grind, dim3(g), dim3(WG), 0, L.stream, jobs[0], table, n);
+    if (n >= 2 && L.grind_pool > 0) {
+        // Several proofs grinding: one pool of workgroups over all of them.
+        hipLaunchKernelGGL(k_grind_pool, dim3(L.grind_pool), dim3(WG), 0, L.stream, table, n);
+    } else {
+        hipLaunchKernelGGL(k_grind, dim3(g), dim3(WG), 0, L.stream, jobs[0], table, n);
+    }
 }
 
 // ---------------------------------------------------------------------------

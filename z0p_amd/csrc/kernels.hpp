@@ -34,6 +34,7 @@ struct Launch {
     TableSink* sink;  // required for launches with more than one job
     bool dry = false;
     bool lde_split = true;  // 2^16..2^24-point LDE passes in kernels of their own (else in k_ntt)
+    uint32_t grind_pool = 0;  // workgroups of the pooled search for launches of several grinds (0: per-job workgroups)
 };
 
 // Two-level power tables for omega_m^{+-e}, e < m:
@@ -366,7 +367,7 @@ struct GrindJob {
     uint32_t bits;
     uint32_t nblocks;
     uint64_t start, limit;  // nonces scanned: [start, limit)
-    unsigned long long* result;
+    unsigned long long* result;  // [0] running minimum, [1] pooled chunk counter
     uint32_t nonce_off;     // byte offset of the nonce in the tail (< 64)
     uint32_t len0, flags0;  // first tail block: byte length and flags
     uint32_t len1;          // second tail block's byte length (0: one tail block)
@@ -378,9 +379,10 @@ struct GrindJob {
 bool grind_prepare(const uint8_t* head, size_t head_len, GrindHead& h, GrindJob& j);
 
 // Device buffer of a transcript grind: the GrindHead, then the running
-// minimum nonce at GRIND_BUF_RESULT.
+// minimum nonce at GRIND_BUF_RESULT and the pooled search's chunk counter
+// after it (GrindJob::result points at both).
 constexpr size_t GRIND_BUF_RESULT = 192, GRIND_BUF_BYTES = 256;
-static_assert(sizeof(GrindHead) <= GRIND_BUF_RESULT, "grind buffer layout");
+static_assert(sizeof(GrindHead) <= GRIND_BUF_RESULT && GRIND_BUF_RESULT + 16 <= GRIND_BUF_BYTES, "grind buffer layout");
 
 // The allocation-free grind's static device array (GRIND_BUF_BYTES per
 // device): noalloc_scratch_init looks up its address on `device` (which must

@@ -623,7 +623,7 @@ size_t layout(Proof& P, uint8_t* base) {
         P.vals[d] = take(d == 0 && P.base0() ? P.m : P.lens[d] * 4);
         if (d < P.folds) P.trees[d] = take(P.geoms[d].words());
     }
-    P.grind_res = reinterpret_cast<unsigned long long*>(take(2));
+    P.grind_res = reinterpret_cast<unsigned long long*>(take(4));  // running minimum, pooled chunk counter
     P.consts = reinterpret_cast<uint8_t*>(take(Ctx::MAILBOX_SLOTS * Ctx::MAILBOX_SLOT / 4));
     return off;
 }

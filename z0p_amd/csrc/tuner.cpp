@@ -77,6 +77,7 @@ TuneResult autotune(Ctx& c, const std::vector<z0p_air_params>& work, unsigned re
         {"compose_subtree", &Schedule::compose_subtree, {0, 1, 2, 4}},
         {"graph", &Schedule::graph, {0, 1}},
         {"lde_split", &Schedule::lde_split, {0, 1}},
+        {"grind_pool", &Schedule::grind_pool, {0, 512, 1024, 2048}},
     };
     // A candidate must beat the incumbent by this fraction to replace it, so
     // run-to-run noise does not move the schedule.
