@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 
 #include "device_common.hpp"
 #include "profiler.hpp"
@@ -204,9 +205,12 @@ __device__ void intt4096_row(gcptr in, gptr out, gcptr tw, uint32_t ninv_m, uint
 // groups of G = 2^R elements i = blk * S * G + base + S * t' for a pass of R
 // stages with distances h = S * 2^k; twiddle exponent j * 2048 / h in
 // w_4096 units with j = base + S * (t' mod 2^k).
-template <bool DIT, int R>
-__device__ __forceinline__ void cols_pass(uint32_t* s, int lS, int logC, gcptr tw) {
+// lS and logC are ints, or std::integral_constant when the caller knows them
+// at compile time (then every index folds to a per-thread base + constants).
+template <bool DIT, int R, typename LS, typename LOGC>
+__device__ __forceinline__ void cols_pass(uint32_t* s, LS lS_, LOGC logC_, gcptr tw) {
     constexpr int G = 1 << R, NG = 16 / G;
+    const int lS = int(lS_), logC = int(logC_);
     const int tid = threadIdx.x;
     const int C = 1 << logC, pitch = C + 1;
     const int c = tid & (C - 1), ti = tid >> logC;
@@ -272,6 +276,32 @@ __device__ __forceinline__ void cols_ntt(uint32_t* s, int L, int logC, gcptr tw)
             cols_partial<false>(s, r0, logC, tw);
             __syncthreads();
         }
+    }
+}
+
+// The DIF column transform for a column length 2^L known at compile time
+// (the LDE's columns pass): the same passes as cols_ntt<false>.
+template <int V>
+using ic = std::integral_constant<int, V>;
+
+template <int L>
+__device__ __forceinline__ void cols_dif_t(uint32_t* s, gcptr tw) {
+    constexpr int r0 = L & 3, LOGC = 12 - L;
+    if constexpr (L - 4 >= r0) {
+        cols_pass<false, 4>(s, ic<L - 4>{}, ic<LOGC>{}, tw);
+        __syncthreads();
+    }
+    if constexpr (L - 8 >= r0) {
+        cols_pass<false, 4>(s, ic<L - 8>{}, ic<LOGC>{}, tw);
+        __syncthreads();
+    }
+    if constexpr (L - 12 >= r0) {
+        cols_pass<false, 4>(s, ic<L - 12>{}, ic<LOGC>{}, tw);
+        __syncthreads();
+    }
+    if constexpr (r0 != 0) {
+        cols_pass<false, r0>(s, ic<0>{}, ic<LOGC>{}, tw);
+        __syncthreads();
     }
 }
 
@@ -478,10 +508,11 @@ __device__ void ntt_lde_rows(const NttJob& J, uint32_t* lds, uint32_t R, uint32_
 // Columns pass of the coset LDE (k_lde_cols) for 2^4..2^12-point columns:
 // a 4096-element tile of C = 4096 / n1 columns through LDS (pitch C + 1),
 // the DIF transform in registers, bit-reversed rows out.
-__device__ void lde_cols_tile(const NttJob& J, uint32_t* lds, uint32_t bx, uint32_t j) {
-    const int n1 = 1 << J.log1, n2 = 1 << J.log2;
-    const int logn = J.log1 + J.log2;
-    const int L = J.log1, logC = 12 - L, C = 1 << logC, pitch = C + 1;
+template <int L>
+__device__ __forceinline__ void lde_cols_tile_t(const NttJob& J, uint32_t* lds, uint32_t bx, uint32_t j) {
+    constexpr int n1 = 1 << L, n2 = N4K;
+    constexpr int logn = L + 12;
+    constexpr int logC = 12 - L, C = 1 << logC, pitch = C + 1;
     const int u0 = int(bx) * C;
     const int tid = threadIdx.x;
     const gcptr yj = gc(J.in) + (size_t(j) << logn);
@@ -497,12 +528,29 @@ __device__ void lde_cols_tile(const NttJob& J, uint32_t* lds, uint32_t bx, uint3
         lds[r * pitch + c] = v[k];
     }
     __syncthreads();
-    cols_ntt<false>(lds, L, logC, gc(J.tw4096));
+    cols_dif_t<L>(lds, gc(J.tw4096));
     const gptr oj = gm(J.out) + (size_t(brev_bits(j, J.logb)) << logn);
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         const int e = tid + 256 * k, qq = e & (n1 - 1), uu = e >> L;
-        oj[size_t(brev_bits(uint32_t(u0 + uu), J.log2)) * n1 + qq] = lds[qq * pitch + uu];
+        oj[size_t(brev_bits(uint32_t(u0 + uu), 12)) * n1 + qq] = lds[qq * pitch + uu];
+    }
+}
+
+// Columns pass for 2^4..2^12-point columns of 4096-point rows, one
+// instantiation per column length.
+__device__ __forceinline__ void lde_cols_tile(const NttJob& J, uint32_t* lds, uint32_t bx, uint32_t j) {
+    switch (J.log1) {
+        case 4: lde_cols_tile_t<4>(J, lds, bx, j); break;
+        case 5: lde_cols_tile_t<5>(J, lds, bx, j); break;
+        case 6: lde_cols_tile_t<6>(J, lds, bx, j); break;
+        case 7: lde_cols_tile_t<7>(J, lds, bx, j); break;
+        case 8: lde_cols_tile_t<8>(J, lds, bx, j); break;
+        case 9: lde_cols_tile_t<9>(J, lds, bx, j); break;
+        case 10: lde_cols_tile_t<10>(J, lds, bx, j); break;
+        case 11: lde_cols_tile_t<11>(J, lds, bx, j); break;
+        case 12: lde_cols_tile_t<12>(J, lds, bx, j); break;
+        default: break;
     }
 }
 

@@ -1815,15 +1815,15 @@ void Engine::pipeline(Proof& P) {
     std::vector<std::vector<CollectItem>> items(phs.size());
     size_t used = 0;
     const double t_build = now_ms();
-    for (size_t k = 0; k < phs.size(); ++k) {
-        P.phase = phs[k].ph;
-        P.depth = phs[k].depth;
-        P.window = 0;
-        build(c_, P, chains[k], items[k], used, k == 0 ? nullptr : P.consts + k * SLOT);
-        phs[k].stage_off = P.stage_off;
-    }
-    P.phase = PH_FRONT;
-    P.depth = 0;
+    auto build_phases = [&](size_t k0, size_t k1) {
+        for (size_t k = k0; k < k1; ++k) {
+            P.phase = phs[k].ph;
+            P.depth = phs[k].depth;
+            P.window = 0;
+            build(c_, P, chains[k], items[k], used, k == 0 ? nullptr : P.consts + k * SLOT);
+            phs[k].stage_off = P.stage_off;
+        }
+    };
     uint64_t* gate = reinterpret_cast<uint64_t*>(c_.mailbox_host);
     uint32_t* timeout = reinterpret_cast<uint32_t*>(c_.mailbox_host + 8);
     __atomic_store_n(timeout, 0u, __ATOMIC_RELEASE);
@@ -1832,8 +1832,8 @@ void Engine::pipeline(Proof& P) {
     // reading an earlier proof's queries.
     if (with_open) std::memset(c_.mailbox_host + Ctx::MAILBOX_HEAD + (phs.size() - 1) * SLOT, 0, 8);
     const uint64_t g0 = c_.gate, g_end = g0 + phs.size(), seq0 = c_.seq;
-    c_.enqueue([&](const Launch& L) {
-        for (size_t k = 0; k < phs.size(); ++k) {
+    auto launch_phases = [&](const Launch& L, size_t k0, size_t k1) {
+        for (size_t k = k0; k < k1; ++k) {
             // Phase k's collect also gates phase k + 1 (its constants).
             HostGate gate{};
             if (k + 1 < phs.size()) {
@@ -1860,7 +1860,27 @@ void Engine::pipeline(Proof& P) {
             const std::vector<const std::vector<Step>*> one{&chains[k]};
             launch_chains(L, one, items[k], c_.flag_dev, seq0 + 1 + k, gate);
         }
-    });
+    };
+    // The first phase goes to the device before the later ones are built:
+    // the device works on it while the host builds and launches the rest.
+    auto drain = [&] {
+        // Open every remaining gate so the enqueued work drains.
+        __atomic_store_n(gate, g_end, __ATOMIC_RELEASE);
+        (void)hipStreamSynchronize(c_.stream);
+        c_.gate = g_end;
+        c_.ring.reset();
+    };
+    build_phases(0, 1);
+    c_.enqueue([&](const Launch& L) { launch_phases(L, 0, 1); });
+    try {
+        build_phases(1, phs.size());
+        P.phase = PH_FRONT;
+        P.depth = 0;
+        if (phs.size() > 1) c_.enqueue([&](const Launch& L) { launch_phases(L, 1, phs.size()); });
+    } catch (...) {
+        drain();
+        throw;
+    }
     c_.seq = seq0 + phs.size();
     c_.prof.host("~host_build_launch", now_ms() - t_build);
     try {
@@ -1911,11 +1931,7 @@ void Engine::pipeline(Proof& P) {
             c_.prof.resolve();
         }
     } catch (...) {
-        // Open every remaining gate so the enqueued work drains, then fail.
-        __atomic_store_n(gate, g_end, __ATOMIC_RELEASE);
-        (void)hipStreamSynchronize(c_.stream);
-        c_.gate = g_end;
-        c_.ring.reset();
+        drain();
         throw;
     }
     c_.gate = g_end;
