@@ -38,8 +38,9 @@ def tree_work(m, layers, cap_height, sched):
     per kernel family under the schedule's fusions. The composition kernel
     hashes layer 0's leaves and their parents (compose_fuse); a fold kernel
     hashes the next layer's leaves and parents for layers of at most
-    2^fold_fuse_max_log values; a layer of at most 2^small_max_log values is
-    folded and its whole tree built by one kernel (both "fold"). The rest is
+    2^fold_fuse_max_log values; a layer of 2^8..2^fold_commit_max_log values,
+    or of at most 2^small_max_log values, is folded and its whole tree built
+    by one kernel (all "fold"). The rest is
     the "merkle" family. Bytes count every digest written once and the
     64-byte leaf input once where the hashing kernel reads it from memory (a
     fused kernel hashes values it has just computed, so reads nothing more);
@@ -69,6 +70,8 @@ def tree_work(m, layers, cap_height, sched):
         if d == 0:
             fused = bool(sched["compose_fuse"])
             tree(L // 4, "compose" if fused else "merkle", parents_too=fused)
+        elif sched.get("fold_commit_max_log", 0) and (1 << 8) <= L <= 1 << sched["fold_commit_max_log"]:
+            tree(L // 4, "fold", whole=True)
         elif sched["small_max_log"] > 0 and L <= 1 << sched["small_max_log"]:
             tree(L // 4, "fold", whole=True)
         elif sched["fold_fuse_max_log"] > 0 and L <= 1 << sched["fold_fuse_max_log"]:
@@ -114,8 +117,8 @@ def kernel_alg_bytes(m, sched):
 FAMILY = {
     "intt": "intt", "lde": "lde", "lde_rows": "lde", "lde_cols": "lde", "linked_b": "linked_b", "compose": "compose",
     "compose_leaves": "compose",
-    "fold_leaves": "fold",
-    "trace_leaves": "merkle", "fri_leaves": "merkle", "tree_levels": "merkle", "tree_tail": "merkle",
+    "fold_leaves": "fold", "fold_commit": "fold",
+    "trace_leaves": "merkle", "fri_leaves": "merkle", "tree_levels": "merkle", "tree_tail": "merkle", "tree_top": "merkle",
     "tree_nodes": "merkle",
     "fold": "fold", "small_commit": "fold",
 }

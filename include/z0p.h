@@ -448,6 +448,9 @@ typedef struct z0p_schedule {
     int32_t lde_split;          /* LDE row and column passes in kernels of their own, else in the shared NTT kernel (0 or 1) */
     int32_t grind_pool;         /* workgroups of the pooled proof-of-work search when several proofs grind in one launch
                                  * (0 = each proof on workgroups of its own; 0..8192) */
+    int32_t fold_commit_max_log; /* FRI layers of 2^8..2^k values committed in one launch: fold, leaves and the
+                                  * whole tree, its upper levels by the last workgroup of each group (0 = off, 8..27) */
+    int32_t tree_top;           /* a proof tree's narrow levels in one staged launch instead of tail launches (0 or 1) */
 } z0p_schedule;
 int z0p_ctx_get_schedule(z0p_ctx* ctx, z0p_schedule* out);
 /* Z0P_ERR_INVALID_ARGUMENT for a value outside the ranges above. */

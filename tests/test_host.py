@@ -316,14 +316,16 @@ def test_bench_tree_work_is_schedule_invariant(logn, cap_height):
         L //= 4
     base = dict(tail_max_log=17, tail_group_max_log=7, tail_spread_log=0, levels_fuse=2, small_max_log=0,
                 grind_blocks=4096, fold_fuse_max_log=0, compose_fuse=0, trace_fuse=1, subtree_levels=0,
-                compose_subtree=0, graph=0, lde_split=1, grind_pool=1024)
+                compose_subtree=0, graph=0, lde_split=1, grind_pool=1024, fold_commit_max_log=0, tree_top=0)
     # unfused: every tree in the merkle family, leaf + 2 per node up to the cap
     cap = 1 << cap_height
     want = sum(k + 2 * (k - min(cap, k)) for k in [m // 8] + [x // 4 for x in layers])
     alg0, comp0 = bench.algorithmic_bytes(n, m, layers, cap_height, base)
     assert comp0 == {"merkle": want}
     for sched in [dict(base, compose_fuse=1), dict(base, small_max_log=10), dict(base, fold_fuse_max_log=18),
-                  dict(base, compose_fuse=1, small_max_log=8, fold_fuse_max_log=22)]:
+                  dict(base, compose_fuse=1, small_max_log=8, fold_fuse_max_log=22),
+                  dict(base, fold_commit_max_log=20), dict(base, small_max_log=10, fold_commit_max_log=16,
+                                                            fold_fuse_max_log=22)]:
         alg, comp = bench.algorithmic_bytes(n, m, layers, cap_height, sched)
         assert sum(comp.values()) == want
         fused_leaves = 0
@@ -331,6 +333,7 @@ def test_bench_tree_work_is_schedule_invariant(logn, cap_height):
             if d == 0:
                 fused_leaves += L // 4 if sched["compose_fuse"] else 0
             elif (sched["small_max_log"] and L <= 1 << sched["small_max_log"]) or \
+                    (sched["fold_commit_max_log"] and (1 << 8) <= L <= 1 << sched["fold_commit_max_log"]) or \
                     (sched["fold_fuse_max_log"] and L <= 1 << sched["fold_fuse_max_log"]):
                 fused_leaves += L // 4
         assert sum(alg.values()) - sum(alg0.values()) == -64 * fused_leaves, sched

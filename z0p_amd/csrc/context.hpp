@@ -268,6 +268,8 @@ struct Schedule {
     int graph = 0;               // each iteration's launches captured into one HIP graph (0: direct launches)
     int lde_split = 1;           // LDE row and column passes in kernels of their own (0: inside k_ntt)
     int grind_pool = 1024;       // workgroups of the pooled search when several proofs grind (0: per-proof workgroups)
+    int fold_commit_max_log = 18;  // FRI layers of 2^8..2^k values committed in one launch (fold, leaves, tree; 0: off)
+    int tree_top = 0;              // a proof tree's tail levels in one staged launch instead of tail launches (0: off)
 
     // Digests per tail workgroup (log2) for a level of `count` digests.
     int tail_group_log(uint64_t count) const {
@@ -287,7 +289,9 @@ struct Schedule {
                (compose_fuse == 0 || compose_fuse == 1) && (trace_fuse == 0 || trace_fuse == 1) &&
                subtree_levels >= 0 && subtree_levels <= SUBTREE_MAX_LEVELS && compose_subtree >= 0 &&
                compose_subtree <= SUBTREE_MAX_LEVELS && (graph == 0 || graph == 1) &&
-               (lde_split == 0 || lde_split == 1) && grind_pool >= 0 && grind_pool <= 8192;
+               (lde_split == 0 || lde_split == 1) && grind_pool >= 0 && grind_pool <= 8192 &&
+               (fold_commit_max_log == 0 || (fold_commit_max_log >= 8 && fold_commit_max_log <= 27)) &&
+               (tree_top == 0 || tree_top == 1);
     }
 };
 

@@ -1633,6 +1633,162 @@ __global__ __launch_bounds__(WG) void k_fold_leaves(FoldJob inl, const FoldJob* 
     if (J.sub_levels > 0) staged_subtree(lds, par, live, J.tree, outs >> 2, first >> 2, J.sub_levels);
 }
 
+// `nlev` levels above `count` digests held in LDS as cur[k][i] (word k of
+// digest i; the first is digest `first` of level `lvl` of the tree whose
+// level 0 has `leaves` digests), one quad of lanes per node, every level
+// written to the tree; the digests of the last level end in cur again when
+// nlev is even, else in nxt. Called by every thread of the workgroup.
+__device__ __forceinline__ void quad_levels(uint32_t (*cur)[64], uint32_t (*nxt)[64], int count, int nlev,
+                                            uint32_t* tree, uint64_t leaves, int lvl, uint64_t first) {
+    const uint32_t q = threadIdx.x & 3u;
+    for (int l = 0; l < nlev; ++l) {
+        const int pc = count >> 1;
+        uint32_t* out = tree + tree_level_words(leaves, lvl + l + 1) + (first >> (l + 1)) * 8;
+        for (int g = int(threadIdx.x >> 2); g < pc; g += WG / 4) {  // quad-uniform
+            uint32_t L[8], R[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                L[k] = cur[k][2 * g];
+                R[k] = cur[k][2 * g + 1];
+            }
+            uint32_t oa, ob;
+            quad_node(L, R, q, oa, ob);
+            nxt[q][g] = oa;
+            nxt[4 + q][g] = ob;
+            gm(out)[uint64_t(g) * 8 + q] = oa;
+            gm(out)[uint64_t(g) * 8 + 4 + q] = ob;
+        }
+        __syncthreads();
+        auto tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+        count = pc;
+    }
+}
+
+// Levels lvl+1 .. target of a tree reduced in stages of six levels: this
+// workgroup holds digest `idx` of level lvl (count digests there) in LDS
+// (word k at (in_b ? b : a)[k][0]); it publishes it, and the last workgroup
+// to arrive of each group of 64 loads the group's digests and reduces them
+// up to six levels, until the target level. `ctr` holds one zeroed counter
+// per group per stage (tree_stage_counters) and is left zero. The digest a
+// stage hands on is published with agent-scope atomic stores (sc1), waited
+// for (s_waitcnt: acknowledged) before the same wave's counter update, and
+// read with agent-scope atomic loads (sc1) after the counter says all have
+// arrived: no workgroup writes back or invalidates whole caches, which
+// thousands of device-scope fences would (buffer_wbl2 per workgroup
+// measured ~3x slower on a 4096-workgroup layer). Called by every thread of
+// the workgroup.
+__device__ void tree_stages(uint32_t (*a)[64], uint32_t (*b)[64], bool in_b, uint32_t* tree, uint64_t leaves,
+                            int lvl, uint64_t idx, uint64_t count, int target, uint32_t* ctr) {
+    __shared__ uint32_t last;
+    const uint32_t tid = threadIdx.x;
+    while (lvl < target) {
+        // Threads 0..7 and 0 are in wave 0: the counter update follows the
+        // completed stores.
+        const uint64_t group = idx >> FOLD_COMMIT_GROUP_LOG;
+        const uint64_t gsize = count - group * 64 < 64 ? count - group * 64 : 64;
+        uint32_t* dig = tree + tree_level_words(leaves, lvl) + idx * 8;
+        if (tid < 8) __hip_atomic_store(dig + tid, (in_b ? b : a)[tid][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // The stores complete (acknowledged at agent scope: they are sc1
+        // stores) before the counter update below, in the same wave.
+        if (tid < 64) __builtin_amdgcn_s_waitcnt(0);
+        if (tid == 0)
+            last = __hip_atomic_fetch_add(ctr + group, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                           uint32_t(gsize - 1)
+                       ? 1u
+                       : 0u;
+        __syncthreads();
+        if (!last) return;  // workgroup-uniform
+        uint32_t* src = tree + tree_level_words(leaves, lvl) + group * 64 * 8;
+        for (int w = int(tid); w < int(gsize) * 8; w += WG)
+            a[w & 7][w >> 3] = __hip_atomic_load(src + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int nl = target - lvl < FOLD_COMMIT_GROUP_LOG ? target - lvl : FOLD_COMMIT_GROUP_LOG;
+        quad_levels(a, b, int(gsize), nl, tree, leaves, lvl, group * 64);
+        in_b = (nl & 1) != 0;
+        if (tid == 0) ctr[group] = 0;  // left zero for the next use
+        ctr += (count + 63) / 64;
+        lvl += nl;
+        idx = group;
+        count >>= nl;
+    }
+}
+
+// Levels above `lvl` of a tree up to its target, in one launch: workgroup w
+// loads digests 64w.. of level lvl (written by an earlier launch), reduces
+// them up to six levels in LDS, then the staged top (tree_stages). Replaces
+// the quad-lane tail launches of a tree's narrow levels.
+__global__ __launch_bounds__(WG) void k_tree_top(TreeTopJob inl, const TreeTopJob* tab, uint32_t njobs) {
+    __shared__ uint32_t bufA[8][64], bufB[8][64];
+    uint32_t local;
+    const TreeTopJob J = job_of(inl, tab, njobs, local);
+    const uint64_t count = J.leaves >> J.lvl;
+    const uint64_t first = uint64_t(local) * 64;
+    const uint64_t here = count - first < 64 ? count - first : 64;
+    const gcptr src = gc(J.tree + tree_level_words(J.leaves, J.lvl)) + first * 8;
+    for (int w = int(threadIdx.x); w < int(here) * 8; w += WG) bufA[w & 7][w >> 3] = src[w];
+    __syncthreads();
+    int own = J.target - J.lvl < FOLD_COMMIT_GROUP_LOG ? J.target - J.lvl : FOLD_COMMIT_GROUP_LOG;
+    quad_levels(bufA, bufB, int(here), own, J.tree, J.leaves, J.lvl, first);
+    tree_stages(bufA, bufB, own & 1, J.tree, J.leaves, J.lvl + own, local, count >> own, J.target, J.counters);
+}
+
+// Lane q of a quad gets the value of lane K of its quad.
+template <int K>
+__device__ __forceinline__ uint32_t quad_bcast(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_mov_dpp(int(v), K * 0x55, 0xF, 0xF, true));
+}
+
+// One FRI layer's whole commitment in one launch (a FoldJob with counters):
+// thread t folds output t of its workgroup's 256 (fri.rs:1273-1331); each quad
+// of lanes then hashes its four outputs as one 64-byte leaf (lane q holds
+// output q; DPP broadcasts give every lane the 16 message words), and the
+// workgroup reduces its 64 leaves by six levels in LDS, a quad per node. The
+// levels above are reduced in stages: the last workgroup to finish of each
+// group of 64 (device-scope counter) takes the group's 64 digests six levels
+// further, up to the cap level. The groups at one stage are independent, so
+// wide trees keep many workgroups busy; a narrow layer costs one launch and
+// ~1 us per level instead of a fold, a leaves and several tree launches.
+// The digest a stage hands to the next one is published with agent-scope
+// atomic stores, completed (workgroup-scope release: the stores are
+// acknowledged) before the same wave's counter update, and read with
+// agent-scope atomic loads: no workgroup writes back or invalidates whole
+// caches, which thousands of device-scope fences would.
+__global__ __launch_bounds__(WG) void k_fold_commit(FoldJob inl, const FoldJob* tab, uint32_t njobs) {
+    __shared__ uint32_t bufA[8][64], bufB[8][64];
+    __shared__ uint32_t last;
+    uint32_t local;
+    const FoldJob J = job_of(inl, tab, njobs, local);
+    const uint64_t outs = uint64_t(1) << (J.log_len - 2);
+    const uint64_t leaves = outs >> 2;
+    const uint32_t tid = threadIdx.x, q = tid & 3u, g = tid >> 2;
+    const uint64_t i = uint64_t(local) * WG + tid;  // outs is a multiple of 256
+    const E4 v = fold_one(J.layer, J.base_field, J.finv, *J.c, i);
+    st4(J.next, i, u32x4{v.v[0], v.v[1], v.v[2], v.v[3]});
+    uint32_t m[16];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        m[c] = quad_bcast<0>(v.v[c]);
+        m[4 + c] = quad_bcast<1>(v.v[c]);
+        m[8 + c] = quad_bcast<2>(v.v[c]);
+        m[12 + c] = quad_bcast<3>(v.v[c]);
+    }
+    const bool q_odd = (q & 1) != 0, q_hi = (q & 2) != 0;
+    uint32_t ha = sel4(q_odd, q_hi, b3::IV0, b3::IV1, b3::IV2, b3::IV3);
+    uint32_t hb = sel4(q_odd, q_hi, b3::IV4, b3::IV5, b3::IV6, b3::IV7);
+    quad_compress(ha, hb, m, q, 64, b3::CHUNK_START | b3::CHUNK_END | b3::ROOT);  // hash.rs:388-397
+    const uint64_t leaf = uint64_t(local) * 64 + g;
+    gm(J.tree)[leaf * 8 + q] = ha;
+    gm(J.tree)[leaf * 8 + 4 + q] = hb;
+    bufA[q][g] = ha;
+    bufA[4 + q][g] = hb;
+    __syncthreads();
+    const int own = J.target < FOLD_COMMIT_GROUP_LOG ? J.target : FOLD_COMMIT_GROUP_LOG;
+    quad_levels(bufA, bufB, 64, own, J.tree, leaves, 0, uint64_t(local) * 64);
+    tree_stages(bufA, bufB, own & 1, J.tree, leaves, own, local, leaves >> own, J.target, J.counters);
+}
+
 // ---------------------------------------------------------------------------
 // A small FRI layer in one workgroup: fold layer d into layer d+1 (at most
 // SMALL_MAX_VALUES values), hash its quad leaves, and build its tree up to
@@ -1968,6 +2124,13 @@ __global__ __launch_bounds__(COLLECT_WG_MAX) void k_collect(CollectItem inl, con
         gate.dst[k] = open ? __hip_atomic_load(gate.src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) : 0u;
 }
 
+__global__ __launch_bounds__(WG) void k_zero(ZeroJob inl, const ZeroJob* tab, uint32_t njobs) {
+    uint32_t local;
+    const ZeroJob J = job_of(inl, tab, njobs, local);
+    const uint64_t i = uint64_t(local) * WG + threadIdx.x;
+    if (i < J.words) J.dst[i] = 0;
+}
+
 // Plain streaming copy: the calibration reference for FETCH/WRITE_SIZE.
 __global__ void k_copy(const uint4* __restrict__ in, uint4* __restrict__ out, uint64_t n16) {
     uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x;
@@ -2233,6 +2396,16 @@ void launch_levels(const Launch& L, std::vector<LevelsJob> jobs) {
 }
 
 // One launch per group size: the workgroup is two threads per digest.
+void launch_zero(const Launch& L, std::vector<ZeroJob> jobs) {
+    unsigned g = assign_blocks(jobs, [](const ZeroJob& j) { return grid_for(j.words, WG); });
+    run(L, "zero", k_zero, jobs, g, 0);
+}
+
+void launch_tree_top(const Launch& L, std::vector<TreeTopJob> jobs) {
+    unsigned g = assign_blocks(jobs, [](const TreeTopJob& j) { return grid_for(j.leaves >> j.lvl, 64); });
+    run(L, "tree_top", k_tree_top, jobs, g, 0);
+}
+
 void launch_tail(const Launch& L, std::vector<TailJob> all) {
     for (int gl = TAIL_GROUP_MIN_LOG; gl <= TAIL_GROUP_MAX_LOG; ++gl) {
         std::vector<TailJob> jobs;
@@ -2259,8 +2432,12 @@ void launch_small_commit(const Launch& L, std::vector<SmallCommitJob> jobs) {
 }
 
 void launch_fold(const Launch& L, std::vector<FoldJob> all) {
-    std::vector<FoldJob> plain, fused;
-    for (const FoldJob& j : all) (j.tree ? fused : plain).push_back(j);
+    std::vector<FoldJob> plain, fused, commit;
+    for (const FoldJob& j : all) (j.counters ? commit : j.tree ? fused : plain).push_back(j);
+    if (!commit.empty()) {
+        unsigned g = assign_blocks(commit, [](const FoldJob& j) { return grid_for(uint64_t(1) << (j.log_len - 2), WG); });
+        run(L, "fold_commit", k_fold_commit, commit, g, 0);
+    }
     if (!plain.empty()) {
         unsigned g = assign_blocks(plain, [](const FoldJob& j) { return grid_for(uint64_t(1) << (j.log_len - 2), WG); });
         run(L, "fold", k_fold, plain, g, 0);

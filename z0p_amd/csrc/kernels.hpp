@@ -334,6 +334,53 @@ struct FoldJob {
     uint32_t* tree = nullptr;
     int with_parent = 0;
     int sub_levels = 0;  // levels above level 1 reduced in LDS per workgroup
+    // Non-null (with tree; also when no counter is needed): the whole
+    // commitment of layer d+1 in one launch
+    // (k_fold_commit): one output per thread, the quad leaves, six levels per
+    // workgroup, then levels above by the last workgroup of each group of 64
+    // (FOLD_COMMIT_GROUP) up to level `target`, the cap level. The counters
+    // (tree_stage_counters(leaves, 0, target) of them) are zero on entry and
+    // are left zero.
+    uint32_t* counters = nullptr;
+    int target = 0;
+};
+constexpr int FOLD_COMMIT_GROUP_LOG = 6;  // digests a workgroup reduces per stage: 2^6
+// Counters a staged tree top uses from level `lvl` (count digests there, 64
+// per workgroup, each reducing up to six levels) to level `target`: one per
+// group of 64 digests at each stage above the workgroups' own levels. A
+// one-launch layer commit starts at its leaves (lvl 0).
+inline uint64_t tree_stage_counters(uint64_t count, int lvl, int target) {
+    uint64_t total = 0;
+    int own = target - lvl < FOLD_COMMIT_GROUP_LOG ? target - lvl : FOLD_COMMIT_GROUP_LOG;
+    lvl += own;
+    count >>= own;
+    while (lvl < target) {
+        total += (count + 63) / 64;
+        const int nl = target - lvl < FOLD_COMMIT_GROUP_LOG ? target - lvl : FOLD_COMMIT_GROUP_LOG;
+        lvl += nl;
+        count >>= nl;
+    }
+    return total;
+}
+
+// Clears `words` words of device memory (a proof's staged-top counters,
+// before its first tree): a kernel, not a memset, so that captured launch
+// sequences hold kernels only.
+struct ZeroJob {
+    uint32_t* dst;
+    uint64_t words;
+    uint32_t block_begin;
+};
+
+// Tree levels above `lvl` up to `target` in one launch (k_tree_top): 64
+// digests of level lvl per workgroup, then the staged top; the counters
+// (tree_stage_counters of them) are zero on entry and are left zero.
+struct TreeTopJob {
+    uint32_t* tree;  // level-contiguous, level 0 has `leaves` digests
+    uint64_t leaves;
+    int lvl, target;
+    uint32_t* counters;
+    uint32_t block_begin;
 };
 
 // Fold of layer d into a small layer d+1 plus its whole Merkle commitment
@@ -399,6 +446,8 @@ void launch_compose(const Launch& L, std::vector<ComposeJob> jobs);
 void launch_leaves(const Launch& L, std::vector<LeavesJob> jobs);
 void launch_levels(const Launch& L, std::vector<LevelsJob> jobs);
 void launch_tail(const Launch& L, std::vector<TailJob> jobs);
+void launch_tree_top(const Launch& L, std::vector<TreeTopJob> jobs);
+void launch_zero(const Launch& L, std::vector<ZeroJob> jobs);
 void launch_fold(const Launch& L, std::vector<FoldJob> jobs);
 // Initialises every job's result to ~0, then searches.
 void launch_grind(const Launch& L, std::vector<GrindJob> jobs);

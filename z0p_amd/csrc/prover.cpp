@@ -290,7 +290,8 @@ int subtree_levels_for(uint64_t parents, int remaining, const Schedule& s) {
     return std::max(0, std::min({k, remaining, s.subtree_levels}));
 }
 
-void tree_levels(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, int have, const Schedule& s) {
+void tree_levels(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, int have, const Schedule& s,
+                 uint32_t* counters) {
     while (have < g.target) {
         const uint64_t count = g.leaves >> have;
         const int remaining = g.target - have;
@@ -312,6 +313,10 @@ void tree_levels(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, in
             const int fuse = std::min(s.levels_fuse, remaining);
             chain.push_back(LevelsJob{lvl, count, fuse, 0});
             have += fuse;
+        } else if (counters && s.tree_top) {
+            // Every narrow level in one staged launch.
+            chain.push_back(TreeTopJob{tree, g.leaves, have, g.target, counters, 0});
+            have = g.target;
         } else {
             const int gl = s.tail_group_log(count);
             const int k = std::min(gl, remaining);
@@ -321,7 +326,8 @@ void tree_levels(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, in
     }
 }
 
-void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves, const Schedule& s) {
+void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves, const Schedule& s,
+                uint32_t* counters) {
     leaves.levels = tree;
     leaves.leaves = g.leaves;
     leaves.with_parent = g.target >= 1 ? 1 : 0;
@@ -329,13 +335,14 @@ void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, Lea
                             ? subtree_levels_for(g.leaves / 2, g.target - 1, s)
                             : 0;
     chain.push_back(leaves);
-    tree_levels(chain, tree, g, leaves.with_parent + leaves.sub_levels, s);
+    tree_levels(chain, tree, g, leaves.with_parent + leaves.sub_levels, s, counters);
 }
 
 namespace {
 
 // Per-round buckets: step r of every chain, grouped by kernel type.
 struct Buckets {
+    std::vector<ZeroJob> zero;
     std::vector<FibCoeffJob> fib;
     std::vector<BitrevJob> bitrev;
     std::vector<InterleaveJob> interleave;
@@ -348,11 +355,13 @@ struct Buckets {
     std::vector<LeavesJob> leaves;
     std::vector<LevelsJob> levels;
     std::vector<TailJob> tail;
+    std::vector<TreeTopJob> top;
     std::vector<FoldJob> fold;
     std::vector<SmallCommitJob> small;
     std::vector<GrindJob> grind;
     std::vector<OpenJob> open;
 
+    std::vector<ZeroJob>& of(const ZeroJob&) { return zero; }
     std::vector<FibCoeffJob>& of(const FibCoeffJob&) { return fib; }
     std::vector<BitrevJob>& of(const BitrevJob&) { return bitrev; }
     std::vector<InterleaveJob>& of(const InterleaveJob&) { return interleave; }
@@ -365,6 +374,7 @@ struct Buckets {
     std::vector<LeavesJob>& of(const LeavesJob&) { return leaves; }
     std::vector<LevelsJob>& of(const LevelsJob&) { return levels; }
     std::vector<TailJob>& of(const TailJob&) { return tail; }
+    std::vector<TreeTopJob>& of(const TreeTopJob&) { return top; }
     std::vector<FoldJob>& of(const FoldJob&) { return fold; }
     std::vector<SmallCommitJob>& of(const SmallCommitJob&) { return small; }
     std::vector<GrindJob>& of(const GrindJob&) { return grind; }
@@ -374,6 +384,7 @@ struct Buckets {
         std::visit([this](const auto& j) { of(j).push_back(j); }, s);
     }
     void launch(const Launch& L) const {
+        if (!zero.empty()) launch_zero(L, zero);
         if (!fib.empty()) launch_fib_coeffs(L, fib);
         if (!bitrev.empty()) launch_bitrev(L, bitrev);
         if (!ntt.empty()) launch_ntt(L, ntt);
@@ -386,6 +397,7 @@ struct Buckets {
         if (!leaves.empty()) launch_leaves(L, leaves);
         if (!levels.empty()) launch_levels(L, levels);
         if (!tail.empty()) launch_tail(L, tail);
+        if (!top.empty()) launch_tree_top(L, top);
         if (!fold.empty()) launch_fold(L, fold);
         if (!small.empty()) launch_small_commit(L, small);
         if (!grind.empty()) launch_grind(L, grind);
@@ -560,6 +572,10 @@ struct Proof {
     uint32_t* planar = nullptr;  // FRI_COEFFS_EXT: the four coordinate LDEs
     std::vector<uint32_t*> vals, trees;
     unsigned long long* grind_res = nullptr;
+    uint32_t* counters = nullptr;  // staged tree-top counters (cleared at the start of FRONT)
+    std::vector<uint64_t> counter_off;  // per tree: offset of its counters
+    uint32_t* tree_counters(size_t d) { return counters + counter_off[d]; }  // d = folds: the trace tree
+    uint64_t ncounters = 0;
     uint8_t* consts = nullptr;  // per-phase transcript constants (pre-enqueued phases)
     uint32_t wpq = 0, fri_wpq = 0;  // opening words per query (all / FRI part)
     // AIR constants
@@ -624,6 +640,17 @@ size_t layout(Proof& P, uint8_t* base) {
         if (d < P.folds) P.trees[d] = take(P.geoms[d].words());
     }
     P.grind_res = reinterpret_cast<unsigned long long*>(take(4));  // running minimum, pooled chunk counter
+    // Staged-top counters: FRI layer d's tree at counter_off[d], the trace
+    // (or row) tree at counter_off[folds].
+    P.counter_off.assign(P.folds + 1, 0);
+    P.ncounters = 0;
+    for (size_t d = 0; d < P.folds; ++d) {
+        P.counter_off[d] = P.ncounters;
+        P.ncounters += tree_stage_counters(P.geoms[d].leaves, 0, P.geoms[d].target);
+    }
+    P.counter_off[P.folds] = P.ncounters;
+    if (P.tg.leaves) P.ncounters += tree_stage_counters(P.tg.leaves, 0, P.tg.target);
+    P.counters = take(std::max<uint64_t>(P.ncounters, 1));
     P.consts = reinterpret_cast<uint8_t*>(take(Ctx::MAILBOX_SLOTS * Ctx::MAILBOX_SLOT / 4));
     return off;
 }
@@ -985,6 +1012,14 @@ void collect_cap(Ctx& c, Proof& P, uint32_t* tree, const TreeGeom& g, std::vecto
     items.push_back(CollectItem{tree + g.level_offset_words(g.target), dst, words, 0});
 }
 
+// Layer k (> 0, with a tree) committed in one launch with its fold
+// (k_fold_commit): 2^8..2^fold_commit_max_log values (whole workgroups of
+// 256 outputs); it takes precedence over the one-workgroup small commit.
+bool fold_commit_layer(const Proof& P, size_t k, const Schedule& s) {
+    return k < P.folds && s.fold_commit_max_log > 0 && P.lens[k] >= (uint64_t(1) << 8) &&
+           P.lens[k] <= (uint64_t(1) << s.fold_commit_max_log) && P.geoms[k].target >= 1;
+}
+
 LeavesJob leaves_of(int kind, const uint32_t* src) {
     LeavesJob j{};
     j.kind = kind;
@@ -998,6 +1033,9 @@ LeavesJob leaves_of(int kind, const uint32_t* src) {
 void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& items, size_t& used,
            const void* cst = nullptr) {
     const GeometryTables& gt = *P.gt;
+    // The staged tree tops find their counters zero: FRONT clears them first
+    // (its own trees use them), and every use leaves them zero.
+    if (P.phase == PH_FRONT && P.counters) ch.push_back(ZeroJob{P.counters, std::max<uint64_t>(P.ncounters, 1), 0});
     switch (P.phase) {
         case PH_FRONT: {
             if (P.row) {
@@ -1026,7 +1064,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 t.col_stride = P.m;
                 t.logn = P.logn;
                 t.logb = P.logb;
-                tree_steps(ch, P.trace_tree, P.tg, t, c.sched);
+                tree_steps(ch, P.trace_tree, P.tg, t, c.sched, P.tree_counters(P.folds));
                 collect_cap(c, P, P.trace_tree, P.tg, items, used);
             } else if (P.air && P.two_col) {
                 if (!P.host_a) {
@@ -1061,7 +1099,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 t.src_b = P.Bc;
                 t.logn = P.logn;
                 t.logb = P.logb;
-                tree_steps(ch, P.trace_tree, P.tg, t, c.sched);
+                tree_steps(ch, P.trace_tree, P.tg, t, c.sched, P.tree_counters(P.folds));
                 collect_cap(c, P, P.trace_tree, P.tg, items, used);
             } else if (P.air) {
                 // interpolate(build_trace(a0, b0, n).0) (air.rs:368-369) in closed form.
@@ -1092,7 +1130,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                     lj.logb = P.logb;
                     ch.push_back(lj);
                 }
-                tree_steps(ch, P.trace_tree, P.tg, t, c.sched);
+                tree_steps(ch, P.trace_tree, P.tg, t, c.sched, P.tree_counters(P.folds));
                 collect_cap(c, P, P.trace_tree, P.tg, items, used);
             } else {
                 switch (P.fri_in) {
@@ -1146,7 +1184,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                     }
                 }
                 tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(P.base0() ? LEAF_BASE : LEAF_EXT, P.vals[0]),
-                           c.sched);
+                           c.sched, P.tree_counters(0));
                 collect_cap(c, P, P.trees[0], P.geoms[0], items, used);
             }
             break;
@@ -1165,7 +1203,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 tj.logn = P.logn;
                 tj.logb = P.logb;
                 ch.push_back(tj);
-                tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_EXT, P.vals[0]), c.sched);
+                tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_EXT, P.vals[0]), c.sched, P.tree_counters(0));
                 collect_cap(c, P, P.trees[0], P.geoms[0], items, used);
                 break;
             }
@@ -1181,7 +1219,7 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 rj.logn = P.logn;
                 rj.logb = P.logb;
                 ch.push_back(rj);
-                tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_EXT, P.vals[0]), c.sched);
+                tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_EXT, P.vals[0]), c.sched, P.tree_counters(0));
                 collect_cap(c, P, P.trees[0], P.geoms[0], items, used);
                 break;
             }
@@ -1202,10 +1240,10 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 cj.sub_levels = cj.with_parent ? subtree_levels_for(P.geoms[0].leaves / 2, P.geoms[0].target - 1, cs)
                                                : 0;
                 ch.push_back(cj);
-                tree_levels(ch, P.trees[0], P.geoms[0], cj.with_parent + cj.sub_levels, c.sched);
+                tree_levels(ch, P.trees[0], P.geoms[0], cj.with_parent + cj.sub_levels, c.sched, P.tree_counters(0));
             } else {
                 ch.push_back(cj);
-                tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_EXT, P.vals[0]), c.sched);
+                tree_steps(ch, P.trees[0], P.geoms[0], leaves_of(LEAF_EXT, P.vals[0]), c.sched, P.tree_counters(0));
             }
             collect_cap(c, P, P.trees[0], P.geoms[0], items, used);
             break;
@@ -1213,6 +1251,22 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
         case PH_FOLD: {
             const size_t d = P.depth;
             const FoldConsts* fc = static_cast<const FoldConsts*>(cst ? cst : c.ring.push(&P.fold, sizeof(P.fold)));
+            FoldJob fj{};
+            fj.layer = P.vals[d];
+            fj.next = P.vals[d + 1];
+            fj.finv = gt.finv;
+            fj.c = fc;
+            fj.log_len = int(ilog2(P.lens[d]));
+            fj.base_field = d == 0 && P.base0() ? 1 : 0;
+            if (fold_commit_layer(P, d + 1, c.sched)) {
+                // Layer d+1's whole commitment in this launch (k_fold_commit).
+                fj.tree = P.trees[d + 1];
+                fj.counters = P.tree_counters(d + 1);
+                fj.target = P.geoms[d + 1].target;
+                ch.push_back(fj);
+                collect_cap(c, P, P.trees[d + 1], P.geoms[d + 1], items, used);
+                break;
+            }
             if (d + 1 < P.folds && c.sched.small_max_log > 0 && P.lens[d + 1] <= (uint64_t(1) << c.sched.small_max_log)) {
                 // Small next layer: fold, leaves and tree in one workgroup.
                 SmallCommitJob sj{};
@@ -1228,13 +1282,6 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                 collect_cap(c, P, P.trees[d + 1], P.geoms[d + 1], items, used);
                 break;
             }
-            FoldJob fj{};
-            fj.layer = P.vals[d];
-            fj.next = P.vals[d + 1];
-            fj.finv = gt.finv;
-            fj.c = fc;
-            fj.log_len = int(ilog2(P.lens[d]));
-            fj.base_field = d == 0 && P.base0() ? 1 : 0;
             if (d + 1 == P.folds) {
                 ch.push_back(fj);
                 const uint32_t words = uint32_t(P.lens[d + 1] * 4);
@@ -1248,11 +1295,13 @@ void build(Ctx& c, Proof& P, std::vector<Step>& ch, std::vector<CollectItem>& it
                                                                     P.geoms[d + 1].target - 1, c.sched)
                                                : 0;
                 ch.push_back(fj);
-                tree_levels(ch, P.trees[d + 1], P.geoms[d + 1], fj.with_parent + fj.sub_levels, c.sched);
+                tree_levels(ch, P.trees[d + 1], P.geoms[d + 1], fj.with_parent + fj.sub_levels, c.sched,
+                            P.tree_counters(d + 1));
                 collect_cap(c, P, P.trees[d + 1], P.geoms[d + 1], items, used);
             } else {
                 ch.push_back(fj);
-                tree_steps(ch, P.trees[d + 1], P.geoms[d + 1], leaves_of(LEAF_EXT, P.vals[d + 1]), c.sched);
+                tree_steps(ch, P.trees[d + 1], P.geoms[d + 1], leaves_of(LEAF_EXT, P.vals[d + 1]), c.sched,
+                           P.tree_counters(d + 1));
                 collect_cap(c, P, P.trees[d + 1], P.geoms[d + 1], items, used);
             }
             break;

@@ -36,14 +36,18 @@ TreeGeom tree_geom(uint64_t leaves, uint64_t cap_height);
 
 // One kernel step of one proof; a proof's steps of one iteration form a
 // chain that must run in order.
-using Step = std::variant<FibCoeffJob, BitrevJob, InterleaveJob, NttJob, LinkedJob, ComposeJob, RangeComposeJob,
-                          TableComposeJob, FoldCombineJob, LeavesJob, LevelsJob, TailJob, FoldJob, SmallCommitJob,
-                          GrindJob, OpenJob>;
+using Step = std::variant<ZeroJob, FibCoeffJob, BitrevJob, InterleaveJob, NttJob, LinkedJob, ComposeJob, RangeComposeJob,
+                          TableComposeJob, FoldCombineJob, LeavesJob, LevelsJob, TailJob, TreeTopJob, FoldJob,
+                          SmallCommitJob, GrindJob, OpenJob>;
 
-// Leaves, then wide levels, then LDS tail launches up to the cap level.
-void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves, const Schedule& s);
+// Leaves, then wide levels, then the narrow levels up to the cap level: LDS
+// tail launches, or with `counters` (tree_stage_counters(leaves, 0, target)
+// zeroed words) and the schedule's tree_top, one staged launch.
+void tree_steps(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, LeavesJob leaves, const Schedule& s,
+                uint32_t* counters = nullptr);
 // The tree levels above `have` (levels 0..have already written) up to the cap.
-void tree_levels(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, int have, const Schedule& s);
+void tree_levels(std::vector<Step>& chain, uint32_t* tree, const TreeGeom& g, int have, const Schedule& s,
+                 uint32_t* counters = nullptr);
 
 // Launches step r of every chain for r = 0, 1, ... (one launch per kernel
 // type per r), then the collect items, which end by storing `seq` to

@@ -38,6 +38,14 @@ void to_host(Ctx& c, T* host, const T* dev, size_t count) {
     c.sync();
 }
 
+// Zeroed staged-top counters for a tree of geometry g (scratch memory).
+uint32_t* zeroed_counters(Ctx& c, const TreeGeom& g) {
+    const uint64_t n = std::max<uint64_t>(1, tree_stage_counters(g.leaves, 0, g.target));
+    uint32_t* d = c.scratch_alloc<uint32_t>(n);
+    hip_check(hipMemsetAsync(d, 0, n * 4, c.stream));
+    return d;
+}
+
 // Runs one chain of steps (a single job per launch: no tables).
 void run_chain(Ctx& c, const std::vector<Step>& chain) {
     std::vector<const std::vector<Step>*> chains{&chain};
@@ -115,7 +123,7 @@ int z0p_test_merkle64(z0p_ctx* ctx, const uint8_t* flat, size_t leaves, uint8_t*
         lj.kind = LEAF_RAW;
         lj.src = din;
         std::vector<Step> chain;
-        tree_steps(chain, tree, g, lj, c.sched);
+        tree_steps(chain, tree, g, lj, c.sched, zeroed_counters(c, g));
         run_chain(c, chain);
         to_host(c, reinterpret_cast<uint32_t*>(levels), tree, g.words());
     });
@@ -148,7 +156,7 @@ int z0p_test_trace_tree(z0p_ctx* ctx, const uint32_t* a, const uint32_t* b, size
         lj.logn = logn;
         lj.logb = logb;
         std::vector<Step> chain;
-        tree_steps(chain, tree, g, lj, c.sched);
+        tree_steps(chain, tree, g, lj, c.sched, zeroed_counters(c, g));
         run_chain(c, chain);
         to_host(c, reinterpret_cast<uint32_t*>(levels), tree, g.words());
     });
