@@ -1097,35 +1097,35 @@ __device__ __forceinline__ void leaves_linked(const LeavesJob& J, uint64_t tau) 
     const int logn = J.logn, logb = J.logb;
     const uint32_t bmask = (1u << logb) - 1, nmask = (1u << logn) - 1;
     const gcptr A = gc(J.src);
-    uint32_t a[16], an[16], iw[16], pos[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const uint64_t i = g * 16 + r;
-        const uint32_t j = uint32_t(i) & bmask;
-        const uint32_t t = uint32_t(i >> logb);
-        const uint32_t blk = brev_bits(j, logb) << logn;
-        pos[r] = blk | brev_bits(t, logn);
-        a[r] = A[pos[r]];
-        an[r] = A[blk | brev_bits((t + 1) & nmask, logn)];
-        iw[r] = gc(J.iw)[pos[r]];
-    }
-    uint32_t m0[16], m1[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const uint32_t j = uint32_t(g * 16 + r) & bmask;
-        const uint32_t b = add(an[r], mont_mul(iw[r], mont_mul(J.coeff, gc(J.zvan_m)[j])));
-        gm(J.b_out)[pos[r]] = b;
-        if (r < 8) {
-            m0[2 * r] = a[r];
-            m0[2 * r + 1] = b;
-        } else {
-            m1[2 * (r - 8)] = a[r];
-            m1[2 * (r - 8) + 1] = b;
-        }
-    }
+    // One leaf (8 rows) at a time: its loads, its B, its hash; fewer values
+    // live at once than with all 16 rows loaded up front, so more waves fit.
     uint32_t d0[8], d1[8];
-    leaf64_words(m0, d0);
-    leaf64_words(m1, d1);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        uint32_t a[8], an[8], iw[8], pos[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const uint64_t i = g * 16 + 8 * h + r;
+            const uint32_t j = uint32_t(i) & bmask;
+            const uint32_t t = uint32_t(i >> logb);
+            const uint32_t blk = brev_bits(j, logb) << logn;
+            pos[r] = blk | brev_bits(t, logn);
+            a[r] = A[pos[r]];
+            an[r] = A[blk | brev_bits((t + 1) & nmask, logn)];
+            iw[r] = gc(J.iw)[pos[r]];
+        }
+        uint32_t m[16];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const uint32_t j = uint32_t(g * 16 + 8 * h + r) & bmask;
+            const uint32_t b = add(an[r], mont_mul(iw[r], mont_mul(J.coeff, gc(J.zvan_m)[j])));
+            gm(J.b_out)[pos[r]] = b;
+            m[2 * r] = a[r];
+            m[2 * r + 1] = b;
+        }
+        leaf64_words(m, h == 0 ? d0 : d1);
+        __builtin_amdgcn_sched_barrier(0);  // the next leaf's loads stay after this hash
+    }
     store_digest(J.levels + (2 * g) * 8, d0);
     store_digest(J.levels + (2 * g + 1) * 8, d1);
     if (J.with_parent) {
@@ -1694,7 +1694,8 @@ __device__ void tree_stages(uint32_t (*a)[64], uint32_t (*b)[64], bool in_b, uin
         // stores) before the counter update below, in the same wave.
         if (tid < 64) __builtin_amdgcn_s_waitcnt(0);
         if (tid == 0)
-            last = __hip_atomic_fetch_add(ctr + group, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+            last = __hip_atomic_fetch_add(ctr + group * STAGE_COUNTER_WORDS, 1u, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT) ==
                            uint32_t(gsize - 1)
                        ? 1u
                        : 0u;
@@ -1707,8 +1708,8 @@ __device__ void tree_stages(uint32_t (*a)[64], uint32_t (*b)[64], bool in_b, uin
         const int nl = target - lvl < FOLD_COMMIT_GROUP_LOG ? target - lvl : FOLD_COMMIT_GROUP_LOG;
         quad_levels(a, b, int(gsize), nl, tree, leaves, lvl, group * 64);
         in_b = (nl & 1) != 0;
-        if (tid == 0) ctr[group] = 0;  // left zero for the next use
-        ctr += (count + 63) / 64;
+        if (tid == 0) ctr[group * STAGE_COUNTER_WORDS] = 0;  // left zero for the next use
+        ctr += (count + 63) / 64 * STAGE_COUNTER_WORDS;
         lvl += nl;
         idx = group;
         count >>= nl;

@@ -345,10 +345,14 @@ struct FoldJob {
     int target = 0;
 };
 constexpr int FOLD_COMMIT_GROUP_LOG = 6;  // digests a workgroup reduces per stage: 2^6
-// Counters a staged tree top uses from level `lvl` (count digests there, 64
-// per workgroup, each reducing up to six levels) to level `target`: one per
-// group of 64 digests at each stage above the workgroups' own levels. A
-// one-launch layer commit starts at its leaves (lvl 0).
+// Words per stage counter (1: adjacent; padding each counter to 256 bytes
+// measured no faster).
+constexpr uint32_t STAGE_COUNTER_WORDS = 1;
+// Words of counters a staged tree top uses from level `lvl` (count digests
+// there, 64 per workgroup, each reducing up to six levels) to level
+// `target`: one counter per group of 64 digests at each stage above the
+// workgroups' own levels. A one-launch layer commit starts at its leaves
+// (lvl 0).
 inline uint64_t tree_stage_counters(uint64_t count, int lvl, int target) {
     uint64_t total = 0;
     int own = target - lvl < FOLD_COMMIT_GROUP_LOG ? target - lvl : FOLD_COMMIT_GROUP_LOG;
@@ -360,7 +364,7 @@ inline uint64_t tree_stage_counters(uint64_t count, int lvl, int target) {
         lvl += nl;
         count >>= nl;
     }
-    return total;
+    return total * STAGE_COUNTER_WORDS;
 }
 
 // Clears `words` words of device memory (a proof's staged-top counters,
