@@ -41,6 +41,12 @@ Z0P_HD uint32_t sub(uint32_t a, uint32_t b) {
     return d < e ? d : e;
 }
 Z0P_HD uint32_t neg(uint32_t a) { return a == 0 ? 0 : P - a; }
+// Unreduced forms for an operand that only a Montgomery product consumes:
+// for a, b < p they lie in [0, 2p), and a Montgomery product of such a value
+// with a canonical constant is still in range (below 2p^2 < p 2^32) and
+// canonical. One and two instructions instead of three.
+Z0P_HD uint32_t sub_lazy(uint32_t a, uint32_t b) { return a + (P - b); }
+Z0P_HD uint32_t add_lazy(uint32_t a, uint32_t b) { return a + b; }
 Z0P_HD uint32_t halve(uint32_t a) { return (a + ((a & 1u) ? P : 0u)) >> 1; }
 Z0P_HD uint32_t dbl(uint32_t a) { return add(a, a); }
 
@@ -55,7 +61,7 @@ Z0P_HD uint32_t redc(uint64_t t) {
     return r < s ? r : s;
 }
 
-// Montgomery multiply: a*b*2^{-32} mod p for a, b < p; output < p.
+// Montgomery multiply: a*b*2^{-32} mod p for a < 2p, b < p; output < p.
 Z0P_HD uint32_t mont_mul(uint32_t a, uint32_t b) { return redc(uint64_t(a) * b); }
 
 // a*b + c*d reduced once (all < p: the sum is below 2p^2 < p 2^32).

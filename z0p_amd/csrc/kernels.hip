@@ -85,6 +85,9 @@ constexpr int N4K_LDS = N4K + N4K / 16;
 
 __device__ __forceinline__ int pad16(int i) { return i + (i >> 4); }
 
+// DIT: b may be unreduced (< 2p), a must be canonical; outputs canonical.
+// DIF: inputs and outputs canonical (the difference the product consumes
+// stays unreduced).
 template <bool DIT>
 __device__ __forceinline__ void bfly(uint32_t& a, uint32_t& b, uint32_t w) {
     if (DIT) {
@@ -94,43 +97,86 @@ __device__ __forceinline__ void bfly(uint32_t& a, uint32_t& b, uint32_t w) {
     } else {
         const uint32_t u = a;
         a = add(u, b);
-        b = mont_mul(sub(u, b), w);
+        b = mont_mul(sub_lazy(u, b), w);
+    }
+}
+
+// In a DIT pass of the register radix-16 transform, the output at register
+// index i of the stage of distance D is the multiplied operand of the next
+// stage (distance 2D, same pass) when bit 2D of i is set; it may then stay
+// unreduced, unless that next butterfly has the trivial twiddle (pass 2 with
+// i & (2D - 1) == 0), which adds it as it is. Operands that are added next
+// (bit 2D clear) and the last stage's outputs stay canonical.
+template <int P, int D>
+constexpr bool dit_lazy_out(int i) {
+    return D < 8 && (i & (2 * D)) != 0 && !(P == 2 && (i & (2 * D - 1)) == 0);
+}
+
+// LDS twiddle tables of a 4096-point row (twl, TWL_WORDS): the 2048 powers
+// w^e (pass 0 reads w^((q + 256 r) 8/D): consecutive lanes read words 8/D
+// apart), then for each stage D of pass 1 the 16D powers w^(k 128/D) (pass 1
+// reads entry (q mod 16) + 16 r: consecutive words, or one word).
+constexpr int TWL_WORDS = 2048 + 16 * (1 + 2 + 4 + 8);
+__device__ __forceinline__ constexpr int twl1_base(int D) { return 2048 + 16 * (D - 1); }
+
+__device__ __forceinline__ void load_twl(uint32_t* twl, gcptr tw) {
+    for (int i = int(threadIdx.x); i < TWL_WORDS; i += WG) {
+        int e = i;
+        if (i >= 2048) {
+            const int j = i - 2048;
+            const int D = j < 16 ? 1 : j < 48 ? 2 : j < 112 ? 4 : 8;
+            e = (j - 16 * (D - 1)) * (128 / D);
+        }
+        twl[i] = tw[e];
     }
 }
 
 // One stage of register distance D within pass P; q is the thread index.
-template <bool DIT, int P, int D>
-__device__ __forceinline__ void stage16(uint32_t (&x)[16], gcptr tw, uint32_t q) {
+// With twl (LDS tables, load_twl) passes 0 and 1 read their twiddles there;
+// pass 2's are the same for every thread and come from tw.
+template <bool DIT, int P, int D, bool TWL = false>
+__device__ __forceinline__ void stage16(uint32_t (&x)[16], gcptr tw, uint32_t q, const uint32_t* twl = nullptr) {
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
         if (t & D) continue;
         const uint32_t r = uint32_t(t & (D - 1));
-        if (P == 2 && r == 0) {  // twiddle w^0 = 1
+        if (P == 2 && r == 0) {  // twiddle w^0 = 1: both inputs canonical here
             const uint32_t u = x[t];
-            x[t] = add(u, x[t + D]);
-            x[t + D] = sub(u, x[t + D]);
+            if (DIT) {
+                x[t] = dit_lazy_out<P, D>(t) ? add_lazy(u, x[t + D]) : add(u, x[t + D]);
+                x[t + D] = dit_lazy_out<P, D>(t + D) ? sub_lazy(u, x[t + D]) : sub(u, x[t + D]);
+            } else {
+                x[t] = add(u, x[t + D]);
+                x[t + D] = sub(u, x[t + D]);
+            }
             continue;
         }
-        uint32_t e;
-        if (P == 0) e = (q + 256u * r) * (8u / D);
-        else if (P == 1) e = ((q & 15u) + 16u * r) * (128u / D);
-        else e = r * (2048u / D);
-        bfly<DIT>(x[t], x[t + D], tw[e]);
+        uint32_t w;
+        if (P == 0) w = TWL ? twl[(q + 256u * r) * (8u / D)] : tw[(q + 256u * r) * (8u / D)];
+        else if (P == 1) w = TWL ? twl[twl1_base(D) + (q & 15u) + 16u * r] : tw[((q & 15u) + 16u * r) * (128u / D)];
+        else w = tw[r * (2048u / D)];
+        if (DIT) {
+            const uint32_t v = mont_mul(x[t + D], w);
+            x[t + D] = dit_lazy_out<P, D>(t + D) ? sub_lazy(x[t], v) : sub(x[t], v);
+            x[t] = dit_lazy_out<P, D>(t) ? add_lazy(x[t], v) : add(x[t], v);
+        } else {
+            bfly<false>(x[t], x[t + D], w);
+        }
     }
 }
 
-template <bool DIT, int P>
-__device__ __forceinline__ void pass16(uint32_t (&x)[16], gcptr tw, uint32_t q) {
+template <bool DIT, int P, bool TWL = false>
+__device__ __forceinline__ void pass16(uint32_t (&x)[16], gcptr tw, uint32_t q, const uint32_t* twl = nullptr) {
     if (DIT) {
-        stage16<DIT, P, 1>(x, tw, q);
-        stage16<DIT, P, 2>(x, tw, q);
-        stage16<DIT, P, 4>(x, tw, q);
-        stage16<DIT, P, 8>(x, tw, q);
+        stage16<DIT, P, 1, TWL>(x, tw, q, twl);
+        stage16<DIT, P, 2, TWL>(x, tw, q, twl);
+        stage16<DIT, P, 4, TWL>(x, tw, q, twl);
+        stage16<DIT, P, 8, TWL>(x, tw, q, twl);
     } else {
-        stage16<DIT, P, 8>(x, tw, q);
-        stage16<DIT, P, 4>(x, tw, q);
-        stage16<DIT, P, 2>(x, tw, q);
-        stage16<DIT, P, 1>(x, tw, q);
+        stage16<DIT, P, 8, TWL>(x, tw, q, twl);
+        stage16<DIT, P, 4, TWL>(x, tw, q, twl);
+        stage16<DIT, P, 2, TWL>(x, tw, q, twl);
+        stage16<DIT, P, 1, TWL>(x, tw, q, twl);
     }
 }
 
@@ -168,18 +214,20 @@ __device__ __forceinline__ void dif4096(uint32_t (&x)[16], uint32_t* s, gcptr tw
 }
 
 // DIT over x in pass-2 layout (bit-reversed input positions 16 q + t);
-// returns with x in pass-0 layout (natural i = q + 256 t).
-__device__ __forceinline__ void dit4096(uint32_t (&x)[16], uint32_t* s, gcptr tw) {
+// returns with x in pass-0 layout (natural i = q + 256 t). With twl, the LDS
+// twiddle tables (load_twl, complete by the first barrier) serve passes 1, 0.
+template <bool TWL = false>
+__device__ __forceinline__ void dit4096(uint32_t (&x)[16], uint32_t* s, gcptr tw, const uint32_t* twl = nullptr) {
     const uint32_t q = threadIdx.x;
     pass16<true, 2>(x, tw, q);
     lds_put<2>(s, x, q);
     __syncthreads();
     lds_get<1>(s, x, q);
-    pass16<true, 1>(x, tw, q);
+    pass16<true, 1, TWL>(x, tw, q, twl);
     lds_put<1>(s, x, q);
     __syncthreads();
     lds_get<0>(s, x, q);
-    pass16<true, 0>(x, tw, q);
+    pass16<true, 0, TWL>(x, tw, q, twl);
 }
 
 // Inverse of one 4096-point row (natural evaluations at `in`), scaled by
@@ -456,6 +504,8 @@ __device__ void ntt_intt_rows(const NttJob& J, uint32_t* lds, uint32_t u) {
 template <bool MASK>
 __device__ void lde_rows_4096(const NttJob& J, uint32_t* lds, uint32_t R, uint32_t j) {
     const uint32_t q = threadIdx.x;
+    uint32_t* twl = lds + N4K_LDS;  // the launch sizes LDS for N4K_LDS + TWL_WORDS words
+    load_twl(twl, gc(J.tw4096));
     const int logn = J.log1 + 12;
     const uint32_t c = brev_bits(R, J.log1);
     const uint32_t* rt = J.tab_a + size_t(j) * N4K + 16 * q;
@@ -470,7 +520,7 @@ __device__ void lde_rows_4096(const NttJob& J, uint32_t* lds, uint32_t R, uint32
             x[4 * v + k] = mont_mul(lde_coeff<MASK>(J, a[k], brev_bits(R * uint32_t(N4K) + C, logn), j), sc[k]);
         }
     }
-    dit4096(x, lds, gc(J.tw4096));
+    dit4096<true>(x, lds, gc(J.tw4096), twl);
     // y[u] = x * s_j^c * w_n^(u c), u = q + 256 t: geometric in t.
     uint32_t f = mont_mul(pow_lookup(J.pt, (q * c) << J.logb), gc(J.tab_b)[(size_t(j) << J.log1) + c]);
     const uint32_t step = pow_lookup(J.pt, (256u * c) << J.logb);
@@ -639,7 +689,7 @@ __global__ __launch_bounds__(WG) void k_lde_rows(NttJob inl, const NttJob* tab, 
         lde_rows_4096<false>(J, lds, local % n1, local / n1);
 }
 
-__global__ __launch_bounds__(WG) void k_lde_cols(NttJob inl, const NttJob* tab, uint32_t njobs) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) void k_lde_cols(NttJob inl, const NttJob* tab, uint32_t njobs) {
     extern __shared__ uint32_t lds[];
     uint32_t local;
     const NttJob J = job_of(inl, tab, njobs, local);
@@ -2198,7 +2248,7 @@ size_t ntt_lds(const NttJob& j) {
         case NTT_INTT_SMALL:
         case NTT_LDE_SMALL: return j.logn == 12 ? size_t(N4K_LDS) * 4 : (size_t(1) << j.logn) * 6;
         case NTT_INTT_ROWS:
-        case NTT_LDE_ROWS: return j.log2 == 12 ? size_t(N4K_LDS) * 4 : (size_t(1) << j.log2) * 6;
+        case NTT_LDE_ROWS: return j.log2 == 12 ? size_t(N4K_LDS + TWL_WORDS) * 4 : (size_t(1) << j.log2) * 6;
         case NTT_INTT_COLS:
         case NTT_LDE_COLS:
             if (j.log1 >= 4 && j.log1 <= 12) return ((size_t(1) << j.log1) + N4K) * 4;  // pitch C + 1
@@ -2315,7 +2365,7 @@ void launch_ntt(const Launch& L, std::vector<NttJob> jobs) {
     unsigned g = assign_blocks(rest, [](const NttJob& j) { return ntt_blocks(j); });
     run(L, intt ? "intt" : "lde", k_ntt, rest, g, lds);
     g = assign_blocks(rows, [](const NttJob& j) { return ntt_blocks(j); });
-    run(L, "lde_rows", k_lde_rows, rows, g, size_t(N4K_LDS) * 4);
+    run(L, "lde_rows", k_lde_rows, rows, g, size_t(N4K_LDS + TWL_WORDS) * 4);
     lds = 0;
     for (const NttJob& j : cols) lds = std::max(lds, ntt_lds(j));
     g = assign_blocks(cols, [](const NttJob& j) { return ntt_blocks(j); });
