@@ -451,6 +451,8 @@ typedef struct z0p_schedule {
     int32_t fold_commit_max_log; /* FRI layers of 2^8..2^k values committed in one launch: fold, leaves and the
                                   * whole tree, its upper levels by the last workgroup of each group (0 = off, 8..27) */
     int32_t tree_top;           /* a proof tree's narrow levels in one staged launch instead of tail launches (0 or 1) */
+    int32_t wave_streams;       /* the two waves of a batch or server lane on two HIP streams, so that one wave's
+                                 * kernels run beside the other's (0 = one stream; 0 or 1) */
 } z0p_schedule;
 int z0p_ctx_get_schedule(z0p_ctx* ctx, z0p_schedule* out);
 /* Z0P_ERR_INVALID_ARGUMENT for a value outside the ranges above. */

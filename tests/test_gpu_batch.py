@@ -255,3 +255,43 @@ def test_server_two_waves_match_alone(ctx):
     alone = [z.air.prove(p, z.Transcript(lab), ctx) for p, (*_, lab) in zip(params, inst)]
     out = _serve([(p, lab) for p, (*_, lab) in zip(params, inst)], 0, [0.0] * len(inst), waves=2)
     assert [out[i][0] for i in range(len(inst))] == alone
+
+
+def test_wave_streams_match_alone(ctx):
+    """The two waves on two streams (schedule wave_streams): kernels of both
+    waves run side by side, proofs are admitted on the idle stream, and
+    every batch and served result is the proof the request gets alone, also
+    under a budget that admits proofs while a wave is in flight."""
+    inst = _instances(16, 8, 17, 53)
+    params = [z.air.for_security(n, a0, b0, bl) for n, a0, b0, bl, _ in inst]
+    alone = [z.air.prove(p, z.Transcript(lab), ctx) for p, (*_, lab) in zip(params, inst)]
+    saved = ctx.get_schedule()
+    ctx.set_waves(2)
+    try:
+        ctx.set_schedule(dict(saved, wave_streams=1))
+        print("two streams, free budget", flush=True)
+        got = z.air.prove_batch(params, [z.Transcript(lab) for *_, lab in inst], ctx)
+        ctx.set_hbm_budget(48 << 20)
+        print("two streams, tight budget", flush=True)
+        got_tight = z.air.prove_batch(params, [z.Transcript(lab) for *_, lab in inst], ctx)
+    finally:
+        ctx.set_hbm_budget(0)
+        ctx.set_waves(8)
+        ctx.set_schedule(saved)
+    assert got == alone
+    assert got_tight == alone
+    print("server, two streams", flush=True)
+    sctx = z.Context(0)
+    sctx.set_schedule(dict(sctx.get_schedule(), wave_streams=1))
+    sctx.set_waves(2)
+    out = {}
+    # 1 GiB: the large lane's 512 MiB holds a few 2^17-step proofs at once.
+    with z.Server(sctx, hbm_budget=1 << 30) as srv:
+        ids = {srv.submit(p, z.Transcript(lab)): i for i, (p, (*_, lab)) in enumerate(zip(params, inst))}
+        deadline = time.time() + 120
+        while len(out) < len(inst) and time.time() < deadline:
+            r = srv.poll(200)
+            if r is not None:
+                out[ids[r[0]]] = r[1]
+    sctx.close()
+    assert [out.get(i) for i in range(len(inst))] == alone

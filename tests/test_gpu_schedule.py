@@ -15,7 +15,8 @@ pytestmark = pytest.mark.gpu
 
 DEFAULT = dict(tail_max_log=17, tail_group_max_log=7, tail_spread_log=0, levels_fuse=2, small_max_log=10,
                grind_blocks=4096, fold_fuse_max_log=0, compose_fuse=1, trace_fuse=1, subtree_levels=0,
-               compose_subtree=0, graph=0, lde_split=1, grind_pool=1024, fold_commit_max_log=18, tree_top=0)
+               compose_subtree=0, graph=0, lde_split=1, grind_pool=1024, fold_commit_max_log=18, tree_top=0,
+               wave_streams=0)
 
 SCHEDULES = [
     dict(DEFAULT, tail_group_max_log=9, tail_spread_log=8),
@@ -44,6 +45,8 @@ SCHEDULES = [
     dict(DEFAULT, tree_top=1),
     dict(DEFAULT, tree_top=1, tail_max_log=4, subtree_levels=3),
     dict(DEFAULT, tree_top=1, tail_max_log=24, fold_commit_max_log=0, small_max_log=0),
+    dict(DEFAULT, wave_streams=1),
+    dict(DEFAULT, wave_streams=1, graph=1, grind_pool=0),
 ]
 
 
@@ -61,7 +64,8 @@ def test_default_schedule(sched_ctx):
 @pytest.mark.parametrize("sched", SCHEDULES, ids=lambda s: "g%d_s%d_t%d_f%d_sm%d_gb%d_ff%d_c%d%d_st%d_cs%d_h%d" % (
     s["tail_group_max_log"], s["tail_spread_log"], s["tail_max_log"], s["levels_fuse"], s["small_max_log"],
     s["grind_blocks"], s["fold_fuse_max_log"], s["compose_fuse"], s["trace_fuse"], s["subtree_levels"],
-    s["compose_subtree"], s["graph"] + 2 * s["lde_split"]) + "_p%d_fc%d_tt%d" % (s["grind_pool"], s["fold_commit_max_log"], s["tree_top"]))
+    s["compose_subtree"], s["graph"] + 2 * s["lde_split"]) + "_p%d_fc%d_tt%d_ws%d" % (s["grind_pool"], s["fold_commit_max_log"], s["tree_top"],
+                                                     s["wave_streams"]))
 def test_proofs_identical_under_schedule(sched_ctx, sched):
     ctx = sched_ctx
     ctx.set_schedule(sched)
@@ -114,7 +118,8 @@ def test_set_schedule_rejects_out_of_range(sched_ctx):
                 dict(DEFAULT, trace_fuse=-1), dict(DEFAULT, subtree_levels=9), dict(DEFAULT, subtree_levels=-1),
                 dict(DEFAULT, compose_subtree=9), dict(DEFAULT, graph=2), dict(DEFAULT, lde_split=2),
                 dict(DEFAULT, grind_pool=-1), dict(DEFAULT, grind_pool=8193), dict(DEFAULT, fold_commit_max_log=7),
-                dict(DEFAULT, fold_commit_max_log=28), dict(DEFAULT, tree_top=2)]:
+                dict(DEFAULT, fold_commit_max_log=28), dict(DEFAULT, tree_top=2), dict(DEFAULT, wave_streams=2),
+                dict(DEFAULT, wave_streams=-1)]:
         with pytest.raises(z.Z0pError) as e:
             sched_ctx.set_schedule(bad)
         assert e.value.name == "InvalidArgument"

@@ -316,7 +316,8 @@ def test_bench_tree_work_is_schedule_invariant(logn, cap_height):
         L //= 4
     base = dict(tail_max_log=17, tail_group_max_log=7, tail_spread_log=0, levels_fuse=2, small_max_log=0,
                 grind_blocks=4096, fold_fuse_max_log=0, compose_fuse=0, trace_fuse=1, subtree_levels=0,
-                compose_subtree=0, graph=0, lde_split=1, grind_pool=1024, fold_commit_max_log=0, tree_top=0)
+                compose_subtree=0, graph=0, lde_split=1, grind_pool=1024, fold_commit_max_log=0, tree_top=0,
+                wave_streams=0)
     # unfused: every tree in the merkle family, leaf + 2 per node up to the cap
     cap = 1 << cap_height
     want = sum(k + 2 * (k - min(cap, k)) for k in [m // 8] + [x // 4 for x in layers])

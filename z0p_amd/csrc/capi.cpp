@@ -81,7 +81,8 @@ namespace {
 z0p_schedule to_c(const z0p::Schedule& s) {
     return z0p_schedule{s.tail_max_log, s.tail_group_max_log, s.tail_spread_log, s.levels_fuse, s.small_max_log,
                         s.grind_blocks, s.fold_fuse_max_log, s.compose_fuse, s.trace_fuse, s.subtree_levels,
-                        s.compose_subtree, s.graph, s.lde_split, s.grind_pool, s.fold_commit_max_log, s.tree_top};
+                        s.compose_subtree, s.graph, s.lde_split, s.grind_pool, s.fold_commit_max_log, s.tree_top,
+                        s.wave_streams};
 }
 }  // namespace
 
@@ -110,6 +111,7 @@ int z0p_ctx_set_schedule(z0p_ctx* ctx, const z0p_schedule* in) {
     s.grind_pool = in->grind_pool;
     s.fold_commit_max_log = in->fold_commit_max_log;
     s.tree_top = in->tree_top;
+    s.wave_streams = in->wave_streams;
     if (!s.valid()) return Z0P_ERR_INVALID_ARGUMENT;
     ctx->c->sched = s;
     return Z0P_OK;

@@ -270,6 +270,7 @@ struct Schedule {
     int grind_pool = 1024;       // workgroups of the pooled search when several proofs grind (0: per-proof workgroups)
     int fold_commit_max_log = 18;  // FRI layers of 2^8..2^k values committed in one launch (fold, leaves, tree; 0: off)
     int tree_top = 0;              // a proof tree's tail levels in one staged launch instead of tail launches (0: off)
+    int wave_streams = 0;          // the scheduler's two waves on two streams, so their kernels overlap (0: one stream)
 
     // Digests per tail workgroup (log2) for a level of `count` digests.
     int tail_group_log(uint64_t count) const {
@@ -291,7 +292,7 @@ struct Schedule {
                compose_subtree <= SUBTREE_MAX_LEVELS && (graph == 0 || graph == 1) &&
                (lde_split == 0 || lde_split == 1) && grind_pool >= 0 && grind_pool <= 8192 &&
                (fold_commit_max_log == 0 || (fold_commit_max_log >= 8 && fold_commit_max_log <= 27)) &&
-               (tree_top == 0 || tree_top == 1);
+               (tree_top == 0 || tree_top == 1) && (wave_streams == 0 || wave_streams == 1);
     }
 };
 
@@ -311,7 +312,7 @@ struct Ctx {
     size_t stage_words = 0;
     // A second staging buffer: the scheduler alternates two waves of proofs,
     // and the wave on the device writes one buffer while the host reads the
-    // other wave's words from the other (swap_stage exchanges them).
+    // other wave's words from the other (swap_wave exchanges them).
     uint32_t* alt_stage_dev = nullptr;
     uint32_t* alt_stage_host = nullptr;
     size_t alt_stage_words = 0;
@@ -322,6 +323,17 @@ struct Ctx {
     uint64_t* flag_host = nullptr;
     uint64_t* flag_dev = nullptr;
     uint64_t seq = 0;
+    // The other wave's stream, completion flag and admission event (schedule
+    // `wave_streams`): swap_wave exchanges them with `stream`, `flag_*` and
+    // `setup_done` together with the staging buffers, so each wave's
+    // launches, flag and staging stay on one side. `setup_done` is recorded
+    // on a side's stream after the device work of admitting proofs there
+    // (uploads), before that side's next wave; a wave launched on the other
+    // side waits for it.
+    hipStream_t alt_stream = nullptr;
+    uint64_t* alt_flag_host = nullptr;
+    uint64_t* alt_flag_dev = nullptr;
+    hipEvent_t setup_done = nullptr, alt_setup_done = nullptr;
     // Mailbox for pre-enqueued phases (pinned host memory, device-mapped):
     // word 0 = gate counter the device polls, word 1 = gate timeout flag,
     // then MAILBOX_SLOTS constant slots of MAILBOX_SLOT bytes.
@@ -404,11 +416,22 @@ struct Ctx {
         graph_execs.clear();
     }
 
-    void swap_stage() {
+    // Switches to the other wave's staging buffer and, with `streams`, also
+    // to its stream, flag and admission event.
+    void swap_wave(bool streams) {
         std::swap(stage_dev, alt_stage_dev);
         std::swap(stage_host, alt_stage_host);
         std::swap(stage_words, alt_stage_words);
+        if (streams) {
+            std::swap(stream, alt_stream);
+            std::swap(flag_host, alt_flag_host);
+            std::swap(flag_dev, alt_flag_dev);
+            std::swap(setup_done, alt_setup_done);
+        }
     }
+    // Waves on two streams: only without per-launch profiling, whose event
+    // resolution synchronises one stream at a time.
+    bool wave_streams() const { return sched.wave_streams != 0 && !prof.enabled; }
 
     void reserve_stage(size_t words) {
         if (words <= stage_words) return;
@@ -473,6 +496,7 @@ struct Ctx {
     }
 
     void sync() {
+        hip_check(hipStreamSynchronize(alt_stream));
         hip_check(hipStreamSynchronize(stream));
         if (prof.enabled) prof.resolve();
         ring.reset();

@@ -1808,7 +1808,6 @@ __device__ __forceinline__ uint32_t quad_bcast(uint32_t v) {
 // caches, which thousands of device-scope fences would.
 __global__ __launch_bounds__(WG) void k_fold_commit(FoldJob inl, const FoldJob* tab, uint32_t njobs) {
     __shared__ uint32_t bufA[8][64], bufB[8][64];
-    __shared__ uint32_t last;
     uint32_t local;
     const FoldJob J = job_of(inl, tab, njobs, local);
     const uint64_t outs = uint64_t(1) << (J.log_len - 2);

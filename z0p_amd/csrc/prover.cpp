@@ -166,6 +166,7 @@ const PointInverses& Ctx::point_inverses(int logn, int logb) {
 
 Ctx::~Ctx() {
     tables.clear();
+    if (alt_stream) (void)hipStreamSynchronize(alt_stream);
     if (stream) (void)hipStreamSynchronize(stream);
     release_graphs();
     for (hipEvent_t& e : verify_done)
@@ -173,12 +174,16 @@ Ctx::~Ctx() {
     if (stage_host) (void)hipHostFree(stage_host);
     if (alt_stage_host) (void)hipHostFree(alt_stage_host);
     if (flag_host) (void)hipHostFree(flag_host);
+    if (alt_flag_host) (void)hipHostFree(alt_flag_host);
+    if (setup_done) (void)hipEventDestroy(setup_done);
+    if (alt_setup_done) (void)hipEventDestroy(alt_setup_done);
     if (mailbox_host) (void)hipHostFree(mailbox_host);
     if (scratch) (void)hipFree(scratch);
     if (grind_buf) (void)hipFree(grind_buf);
     for (auto& kv : spare) (void)hipFree(kv.second);
     if (tw4096) (void)hipFree(tw4096);
     if (tw4096_inv) (void)hipFree(tw4096_inv);
+    if (alt_stream) (void)hipStreamDestroy(alt_stream);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -191,6 +196,11 @@ Ctx* ctx_create(int device) {
     auto c = std::make_unique<Ctx>();
     c->device = device;
     hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    hip_check(hipStreamCreateWithFlags(&c->alt_stream, hipStreamNonBlocking));
+    hip_check(hipEventCreateWithFlags(&c->setup_done, hipEventDisableTiming));
+    hip_check(hipEventCreateWithFlags(&c->alt_setup_done, hipEventDisableTiming));
+    hip_check(hipEventRecord(c->setup_done, c->stream));
+    hip_check(hipEventRecord(c->alt_setup_done, c->alt_stream));
     const uint32_t w = root_of_unity(12), wi = inv_c(w);
     auto fwd = geometric_m(w, 2048), inv = geometric_m(wi, 2048);
     hip_check(hipMalloc(&c->tw4096, 2048 * 4));
@@ -204,6 +214,10 @@ Ctx* ctx_create(int device) {
     hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->flag_host), 256, hipHostMallocMapped | hipHostMallocCoherent));
     hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->flag_dev), c->flag_host, 0));
     *c->flag_host = 0;
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->alt_flag_host), 256,
+                            hipHostMallocMapped | hipHostMallocCoherent));
+    hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->alt_flag_dev), c->alt_flag_host, 0));
+    *c->alt_flag_host = 0;
     const size_t mb = Ctx::MAILBOX_HEAD + Ctx::MAILBOX_SLOTS * Ctx::MAILBOX_SLOT;
     hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->mailbox_host), mb, hipHostMallocMapped | hipHostMallocCoherent));
     hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->mailbox_dev), c->mailbox_host, 0));
@@ -1637,7 +1651,10 @@ size_t auto_budget(Ctx& c, size_t want) {
 
 Engine::~Engine() {
     // Proof memory goes back to the pool once no launch can still use it.
-    if (!flight_.empty()) (void)hipStreamSynchronize(c_.stream);
+    if (!flight_.empty()) {
+        (void)hipStreamSynchronize(c_.alt_stream);
+        (void)hipStreamSynchronize(c_.stream);
+    }
     for (Proof* p : active_)
         if (p->mem && !p->own_mem) c_.pool.release(p->mem, p->mem_bytes);
 }
@@ -1768,7 +1785,15 @@ void Engine::complete(const std::vector<Proof*>& ws, uint64_t seq, bool reset_ri
 }
 
 bool Engine::iterate() {
+    // Two waves on two streams: the stream of the wave launched before the
+    // one in flight is idle (that wave has completed), so proofs are
+    // admitted there, ahead of the next wave on that stream, and never
+    // behind the wave in flight.
+    const bool streams = c_.wave_streams();
+    const bool on_idle = streams && !flight_.empty();
+    if (on_idle) c_.swap_wave(true);
     admit();
+    if (streams) hip_check(hipEventRecord(c_.setup_done, c_.stream));
     if (flight_.empty()) {
         if (active_.empty()) return !queue_.empty();
         c_.ring.reset();  // every launch has completed
@@ -1816,18 +1841,22 @@ bool Engine::iterate() {
     }
     uint64_t seq = 0;
     if (!ws.empty()) {
-        c_.swap_stage();
+        if (!on_idle) c_.swap_wave(streams);
         const size_t half = c_.ring.capacity() / 2;
         ring_half_ ^= 1;
         c_.ring.reset_to(ring_half_ * half, (ring_half_ + 1) * half);
+        // Proofs admitted on the other stream (and not launched there) have
+        // their uploads ordered before this wave.
+        if (streams) hip_check(hipStreamWaitEvent(c_.stream, c_.alt_setup_done, 0));
         seq = launch(ws);
     }
     c_.prof.host("~host_build_launch", now_ms() - t_build);
     if (!flight_.empty()) {
-        // The earlier wave staged its words in the other buffer.
-        if (!ws.empty()) c_.swap_stage();
+        // The earlier wave staged its words in the other buffer (and, with
+        // two streams, ran on the other stream).
+        if (!ws.empty() || on_idle) c_.swap_wave(streams);
         complete(flight_, flight_seq_, false);
-        if (!ws.empty()) c_.swap_stage();
+        if (!ws.empty()) c_.swap_wave(streams);
     }
     flight_ = std::move(ws);
     flight_seq_ = seq;
@@ -1989,6 +2018,7 @@ void Engine::pipeline(Proof& P) {
 void Engine::fail_all(int status) {
     // Nothing enqueued (a wave in flight, a launch cut short) may still use
     // the memory the proofs hand back.
+    (void)hipStreamSynchronize(c_.alt_stream);
     (void)hipStreamSynchronize(c_.stream);
     flight_.clear();
     for (Proof* p : active_) {

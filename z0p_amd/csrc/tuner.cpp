@@ -80,6 +80,7 @@ TuneResult autotune(Ctx& c, const std::vector<z0p_air_params>& work, unsigned re
         {"grind_pool", &Schedule::grind_pool, {0, 512, 1024, 2048}},
         {"fold_commit_max_log", &Schedule::fold_commit_max_log, {0, 12, 14, 16, 18, 20, 22}},
         {"tree_top", &Schedule::tree_top, {0, 1}},
+        {"wave_streams", &Schedule::wave_streams, {0, 1}},
     };
     // A candidate must beat the incumbent by this fraction to replace it, so
     // run-to-run noise does not move the schedule.
