@@ -85,7 +85,7 @@ class ScheduleC(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int32) for name in (
         "tail_max_log", "tail_group_max_log", "tail_spread_log", "levels_fuse", "small_max_log", "grind_blocks",
         "fold_fuse_max_log", "compose_fuse", "trace_fuse", "subtree_levels", "compose_subtree", "graph", "lde_split",
-        "grind_pool", "fold_commit_max_log", "tree_top")]
+        "grind_pool", "fold_commit_max_log", "tree_top", "wave_streams")]
 
     def to_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
