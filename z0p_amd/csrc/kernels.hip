@@ -1621,21 +1621,31 @@ __global__ __launch_bounds__(2 * TAIL_MAX_SUB) void k_tail(TailJob inl, const Ta
 // Arity-4 fold of a bit-reversed layer (fri.rs:1273-1331). Quad i sits at
 // (x, -x, j0 x, -j0 x), x the point of position 4i;
 // x^{-1} = 31^{-4^d} * Omega_m^{-(brev(i) << 2d)}.
-// Output i of the fold of `layer` (length 2^log_len).
-__device__ __forceinline__ E4 fold_one(const uint32_t* layer, int base_field, const uint32_t* finv,
-                                        const FoldConsts& C, uint64_t i) {
-    const uint32_t inv_x = mont_mul(gc(finv)[i], C.ginv_m);
+// The operands of output i of the fold of `layer` (length 2^log_len): its
+// quad and its entry of the inverse-point table.
+struct FoldIn {
     E4 q[4];
-    if (base_field) {
+    uint32_t fi;
+};
+template <bool BASE>
+__device__ __forceinline__ FoldIn fold_load(const uint32_t* layer, const uint32_t* finv, uint64_t i) {
+    FoldIn r;
+    r.fi = gc(finv)[i];
+    if (BASE) {
         const u32x4 v = ld4(layer, i);
-        q[0] = e4_base(v.x); q[1] = e4_base(v.y); q[2] = e4_base(v.z); q[3] = e4_base(v.w);
+        r.q[0] = e4_base(v.x); r.q[1] = e4_base(v.y); r.q[2] = e4_base(v.z); r.q[3] = e4_base(v.w);
     } else {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const u32x4 v = ld4(layer, i * 4 + k);
-            q[k] = E4{{v.x, v.y, v.z, v.w}};
+            r.q[k] = E4{{v.x, v.y, v.z, v.w}};
         }
     }
+    return r;
+}
+__device__ __forceinline__ E4 fold_compute(const FoldIn& in, const FoldConsts& C) {
+    const uint32_t inv_x = mont_mul(in.fi, C.ginv_m);
+    const E4* q = in.q;
     const E4 s01 = e4_add(q[0], q[1]), s23 = e4_add(q[2], q[3]);
     const E4 dodd = e4_sub(q[0], q[1]);
     const E4 dskew = e4_scale_m(e4_sub(q[2], q[3]), C.unit_m);
@@ -1645,20 +1655,66 @@ __device__ __forceinline__ E4 fold_one(const uint32_t* layer, int base_field, co
     acc = e4_add(acc, e4_mul_mat(O, C.cm[0]));
     return e4_add(e4_scale_m(M, C.inv4_m), e4_scale_m(acc, inv_x));
 }
+__device__ __forceinline__ E4 fold_one(const uint32_t* layer, int base_field, const uint32_t* finv,
+                                        const FoldConsts& C, uint64_t i) {
+    return base_field ? fold_compute(fold_load<true>(layer, finv, i), C)
+                      : fold_compute(fold_load<false>(layer, finv, i), C);
+}
+
+// Two outputs per thread, i and i + WG (i < quads): both outputs' loads are
+// issued before the first one's arithmetic. The second's index is clamped
+// into the layer (tiny layers), so no branch separates the loads from the
+// arithmetic, and its result is stored only when it exists.
+constexpr int FOLD_PER = 2;
+template <bool BASE>
+__device__ __forceinline__ void fold_two(const FoldJob& J, const FoldConsts& C, uint64_t i, uint64_t quads) {
+    const uint64_t i2 = i + WG;
+    const FoldIn a = fold_load<BASE>(J.layer, J.finv, i);
+    const FoldIn b = fold_load<BASE>(J.layer, J.finv, i2 < quads ? i2 : i);
+    const E4 x = fold_compute(a, C);
+    st4(J.next, i, u32x4{x.v[0], x.v[1], x.v[2], x.v[3]});
+    const E4 y = fold_compute(b, C);
+    if (i2 < quads) st4(J.next, i2, u32x4{y.v[0], y.v[1], y.v[2], y.v[3]});
+}
 
 __global__ __launch_bounds__(WG) void k_fold(FoldJob inl, const FoldJob* tab, uint32_t njobs) {
     uint32_t local;
     const FoldJob J = job_of(inl, tab, njobs, local);
     const uint64_t quads = uint64_t(1) << (J.log_len - 2);
-    const uint64_t i = uint64_t(local) * WG + threadIdx.x;
+    const uint64_t i = uint64_t(local) * (WG * FOLD_PER) + threadIdx.x;
     if (i >= quads) return;
-    const E4 out = fold_one(J.layer, J.base_field, J.finv, *J.c, i);
-    st4(J.next, i, u32x4{out.v[0], out.v[1], out.v[2], out.v[3]});
+    const FoldConsts C = *J.c;
+    if (J.base_field) fold_two<true>(J, C, i, quads);
+    else fold_two<false>(J, C, i, quads);
 }
 
 // Fold fused with the next layer's Merkle leaves: the workgroup folds 2048
 // consecutive outputs as k_fold does (coalesced quads), stages them in LDS,
-// then hashes them as 512 quad leaves of layer d+1 (and their parents).
+// then hashes them as 512 quad leaves of layer d+1 (and their parents). The
+// next output's loads are issued before each output's arithmetic (indices
+// clamped into the layer, results kept only for outputs that exist).
+template <bool BASE>
+__device__ __forceinline__ void fold_stage(const FoldJob& J, const FoldConsts& C, uint32_t* lds, uint64_t first,
+                                           uint64_t outs) {
+    const uint64_t i0 = first + threadIdx.x;
+    FoldIn cur = fold_load<BASE>(J.layer, J.finv, i0 < outs ? i0 : outs - 1);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const uint64_t i = i0 + uint64_t(e) * WG;
+        FoldIn nxt;
+        if (e < 7) {
+            const uint64_t in = i + WG;
+            nxt = fold_load<BASE>(J.layer, J.finv, in < outs ? in : outs - 1);
+        }
+        const E4 v = fold_compute(cur, C);
+        if (i < outs) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) lds[fuse_at(int(threadIdx.x) + e * WG, k)] = v.v[k];
+            st4(J.next, i, u32x4{v.v[0], v.v[1], v.v[2], v.v[3]});
+        }
+        if (e < 7) cur = nxt;
+    }
+}
 __global__ __launch_bounds__(WG) void k_fold_leaves(FoldJob inl, const FoldJob* tab, uint32_t njobs) {
     __shared__ uint32_t lds[FUSE_LDS];
     uint32_t local;
@@ -1666,16 +1722,8 @@ __global__ __launch_bounds__(WG) void k_fold_leaves(FoldJob inl, const FoldJob* 
     const uint64_t outs = uint64_t(1) << (J.log_len - 2);
     const uint64_t first = uint64_t(local) * FUSE_VALUES;
     const FoldConsts C = *J.c;
-#pragma unroll 2
-    for (int e = 0; e < 8; ++e) {
-        const uint64_t i = first + threadIdx.x + uint64_t(e) * WG;
-        if (i < outs) {
-            const E4 v = fold_one(J.layer, J.base_field, J.finv, C, i);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) lds[fuse_at(int(threadIdx.x) + e * WG, k)] = v.v[k];
-            st4(J.next, i, u32x4{v.v[0], v.v[1], v.v[2], v.v[3]});
-        }
-    }
+    if (J.base_field) fold_stage<true>(J, C, lds, first, outs);
+    else fold_stage<false>(J, C, lds, first, outs);
     __syncthreads();
     const int nvals = int(outs - first < uint64_t(FUSE_VALUES) ? outs - first : uint64_t(FUSE_VALUES));
     uint32_t par[8];
@@ -2489,7 +2537,9 @@ void launch_fold(const Launch& L, std::vector<FoldJob> all) {
         run(L, "fold_commit", k_fold_commit, commit, g, 0);
     }
     if (!plain.empty()) {
-        unsigned g = assign_blocks(plain, [](const FoldJob& j) { return grid_for(uint64_t(1) << (j.log_len - 2), WG); });
+        unsigned g = assign_blocks(plain, [](const FoldJob& j) {
+            return grid_for(uint64_t(1) << (j.log_len - 2), WG * FOLD_PER);
+        });
         run(L, "fold", k_fold, plain, g, 0);
     }
     if (!fused.empty()) {

@@ -1620,10 +1620,13 @@ void host_step(Ctx& c, Proof& P) {
             }
             break;
         }
-        case PH_OPEN:
+        case PH_OPEN: {
+            const double t0 = now_ms();
             serialize(P, w);
+            c.prof.host("~host_serialize", now_ms() - t0);
             P.phase = PH_DONE;
             break;
+        }
         case PH_DONE:
             break;
     }
