@@ -694,7 +694,18 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) void k_
     uint32_t local;
     const NttJob J = job_of(inl, tab, njobs, local);
     const uint32_t per = (1u << J.log2) / uint32_t(J.cols);
-    lde_cols_tile(J, lds, local % per, local / per);
+    // A tile reads 4 * 2^(12 - log1) bytes of each row: from log1 = 8 on,
+    // G = 2^(log1 - 7) neighbouring tiles share every 128-byte line. Blocks
+    // go to the 8 XCDs round-robin, so the G tiles of a line are given to
+    // blocks 8 apart (one XCD, one L2): the line is fetched once. A
+    // bijection on each group of 8G blocks (per is a multiple of 8G).
+    uint32_t t = local;
+    if (J.log1 > 7) {
+        const int gl = J.log1 - 7;
+        const uint32_t grp = 8u << gl, r = t & (grp - 1);
+        t = (t & ~(grp - 1)) | ((r & 7u) << gl) | (r >> 3);
+    }
+    lde_cols_tile(J, lds, t % per, t / per);
 }
 
 __global__ __launch_bounds__(WG) void k_bitrev(BitrevJob inl, const BitrevJob* tab, uint32_t njobs) {
@@ -1110,19 +1121,21 @@ __device__ __forceinline__ void leaves_body(const LeavesJob& J, uint64_t tau, ui
     if (!live && J.sub_levels == 0) return;
     if (live) {
         const uint64_t g = KIND == LEAF_TRACE ? trace_pair_of(J, tau) : tau;
+        // Both halves' loads are issued before the first compression.
         if (KIND == LEAF_NODES) {
-            uint32_t s0[8], s1[8];
+            uint32_t s0[8], s1[8], s2[8], s3[8];
             load_digest(J.src + (4 * g) * 8, s0);
             load_digest(J.src + (4 * g + 1) * 8, s1);
+            load_digest(J.src + (4 * g + 2) * 8, s2);
+            load_digest(J.src + (4 * g + 3) * 8, s3);
             node_words(s0, s1, d0);
-            load_digest(J.src + (4 * g + 2) * 8, s0);
-            load_digest(J.src + (4 * g + 3) * 8, s1);
-            node_words(s0, s1, d1);
+            node_words(s2, s3, d1);
         } else {
+            uint32_t m1[16];
             leaf_message<KIND>(J, 2 * g, m);
+            leaf_message<KIND>(J, 2 * g + 1, m1);
             leaf64_words(m, d0);
-            leaf_message<KIND>(J, 2 * g + 1, m);
-            leaf64_words(m, d1);
+            leaf64_words(m1, d1);
         }
         store_digest(J.levels + (2 * g) * 8, d0);
         store_digest(J.levels + (2 * g + 1) * 8, d1);
@@ -1276,13 +1289,14 @@ __global__ __launch_bounds__(WG) void k_levels(LevelsJob inl, const LevelsJob* t
     uint32_t a[8], b[8];
     if (J.fuse == 2) {
         if (g >= count / 4) return;
-        uint32_t p0[8], p1[8], top[8];
+        // All four digests' loads before the first compression.
+        uint32_t c[8], d[8], p0[8], p1[8], top[8];
         load_digest(J.level + (4 * g) * 8, a);
         load_digest(J.level + (4 * g + 1) * 8, b);
+        load_digest(J.level + (4 * g + 2) * 8, c);
+        load_digest(J.level + (4 * g + 3) * 8, d);
         node_words(a, b, p0);
-        load_digest(J.level + (4 * g + 2) * 8, a);
-        load_digest(J.level + (4 * g + 3) * 8, b);
-        node_words(a, b, p1);
+        node_words(c, d, p1);
         store_digest(next + (2 * g) * 8, p0);
         store_digest(next + (2 * g + 1) * 8, p1);
         node_words(p0, p1, top);
