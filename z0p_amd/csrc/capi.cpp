@@ -62,6 +62,12 @@ int z0p_ctx_profile_only(z0p_ctx* ctx, const char* kernel) {
 
 size_t z0p_ctx_kernel_stats(z0p_ctx* ctx, char* buf, size_t cap) {
     if (!ctx) return 0;
+    // Launches still timing: wait for them, so the figures are complete.
+    if (!ctx->c->prof.pending.empty()) {
+        (void)hipStreamSynchronize(ctx->c->alt_stream);
+        (void)hipStreamSynchronize(ctx->c->stream);
+        ctx->c->prof.resolve();
+    }
     std::string s;
     for (auto& kv : ctx->c->prof.stats)
         s += kv.first + "=" + std::to_string(kv.second.launches) + "," + std::to_string(kv.second.total_ms) + ";";
@@ -74,7 +80,13 @@ size_t z0p_ctx_kernel_stats(z0p_ctx* ctx, char* buf, size_t cap) {
 }
 
 void z0p_ctx_reset_stats(z0p_ctx* ctx) {
-    if (ctx) ctx->c->prof.stats.clear();
+    if (!ctx) return;
+    if (!ctx->c->prof.pending.empty()) {
+        (void)hipStreamSynchronize(ctx->c->alt_stream);
+        (void)hipStreamSynchronize(ctx->c->stream);
+        ctx->c->prof.resolve();
+    }
+    ctx->c->prof.stats.clear();
 }
 
 namespace {

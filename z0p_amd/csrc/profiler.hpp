@@ -1,6 +1,7 @@
 // Per-kernel timing with HIP events recorded on the launching stream.
 // When enabled, every launcher brackets its kernel with an event pair; the
-// host resolves elapsed times after the next stream synchronisation.
+// host resolves elapsed times after the next stream synchronisation, or
+// without waiting for the pairs that have completed (resolve_ready).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -46,8 +47,17 @@ struct Profiler {
     }
 
     // Call after the stream has been synchronised.
-    void resolve() {
-        for (auto& p : pending) {
+    void resolve() { resolve_pairs(false); }
+    // The pairs whose stop event has completed; the others stay pending.
+    void resolve_ready() { resolve_pairs(true); }
+    void resolve_pairs(bool ready_only) {
+        size_t keep = 0;
+        for (size_t i = 0; i < pending.size(); ++i) {
+            Pending& p = pending[i];
+            if (ready_only && p.stop && hipEventQuery(p.stop) != hipSuccess) {
+                pending[keep++] = p;
+                continue;
+            }
             float ms = 0.f;
             if (p.start && p.stop && hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
                 Stat& s = stats[p.name];
@@ -57,7 +67,7 @@ struct Profiler {
             if (p.start) pool.push_back(p.start);
             if (p.stop) pool.push_back(p.stop);
         }
-        pending.clear();
+        pending.resize(keep);
     }
 
     ~Profiler() {

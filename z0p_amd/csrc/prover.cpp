@@ -2006,11 +2006,9 @@ void Engine::pipeline(Proof& P) {
             c_.prof.host("~host_wait_device", t_host - t_wait);
             c_.prof.host("~host_transcript", now_ms() - t_host);
         }
-        if (c_.prof.enabled) {
-            // Every gate is open by now: resolve the per-launch events.
-            hip_check(hipStreamSynchronize(c_.stream));
-            c_.prof.resolve();
-        }
+        // The per-launch events that have completed; the rest resolve when
+        // the statistics are read (no wait on the proof's critical path).
+        if (c_.prof.enabled) c_.prof.resolve_ready();
     } catch (...) {
         drain();
         throw;
