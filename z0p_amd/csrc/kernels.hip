@@ -802,11 +802,11 @@ __global__ __launch_bounds__(WG) void k_point_inv(PointInvJob J) {
 //   a0*(A'-B)/Z + a1*(B'-A-B)/Z + a2*(A-a0)/(x-1) + a3*(B-b0)/(x-1) + a4*(B-bl)/(x-w)
 __device__ __forceinline__ void compose_value(const ComposeConsts& C, uint32_t va, uint32_t vb, uint32_t na,
                                               uint32_t nb, uint32_t i1, uint32_t iw, uint32_t iz, uint32_t (&o)[4]) {
-    const uint32_t s0 = mont_mul(sub(na, vb), iz);
-    const uint32_t s1 = mont_mul(sub(nb, add(va, vb)), iz);
-    const uint32_t s2 = mont_mul(sub(va, C.a0), i1);
-    const uint32_t s3 = mont_mul(sub(vb, C.b0), i1);
-    const uint32_t s4 = mont_mul(sub(vb, C.b_last), iw);
+    const uint32_t s0 = mont_mul(sub_lazy(na, vb), iz);
+    const uint32_t s1 = mont_mul(sub_lazy(nb, add(va, vb)), iz);
+    const uint32_t s2 = mont_mul(sub_lazy(va, C.a0), i1);
+    const uint32_t s3 = mont_mul(sub_lazy(vb, C.b0), i1);
+    const uint32_t s4 = mont_mul(sub_lazy(vb, C.b_last), iw);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const uint32_t p01 = mont_mul2(s0, C.alpha_m[0][k], s1, C.alpha_m[1][k]);
@@ -1357,20 +1357,19 @@ __device__ __forceinline__ void staged_subtree(uint32_t* lds, const uint32_t (&p
 // Composition fused with layer 0's Merkle leaves: the workgroup computes
 // 2048 consecutive layer positions as k_compose does (coalesced), stages
 // them in LDS, then hashes them as 512 quad leaves (and their parents).
-__global__ __launch_bounds__(WG) void k_compose_leaves(ComposeJob inl, const ComposeJob* tab, uint32_t njobs) {
-    __shared__ uint32_t lds[FUSE_LDS];
-    uint32_t local;
-    const ComposeJob J = job_of(inl, tab, njobs, local);
-    const uint32_t m = 1u << (J.logn + J.logb);
-    const uint32_t first = local * FUSE_VALUES;
-    const uint32_t base = first + threadIdx.x;
+// The composition of this workgroup's 2048 positions (CELT per thread),
+// staged in LDS and written as layer 0. FULL: every position exists (all
+// but a layer's last, partial workgroup), so no per-lane guard splits the
+// loads from the arithmetic.
+template <bool FULL>
+__device__ __forceinline__ void compose_stage(const ComposeJob& J, const ComposeConsts& C, uint32_t* lds, uint32_t base,
+                                              uint32_t m) {
     const gcptr A = gc(J.a), Bv = gc(J.b);
-    const ComposeConsts C = *J.c;
     uint32_t va[CELT], vb[CELT], na[CELT], nb[CELT], i1[CELT], iw[CELT], iz[CELT];
 #pragma unroll
     for (int e = 0; e < CELT; ++e) {
         const uint32_t p = base + e * WG;
-        if (p < m) {
+        if (FULL || p < m) {
             const PosInfo pi = pos_info(p, J.logn, J.logb);
             na[e] = A[pi.pnext];
             nb[e] = Bv[pi.pnext];
@@ -1384,13 +1383,24 @@ __global__ __launch_bounds__(WG) void k_compose_leaves(ComposeJob inl, const Com
     static_for<0, CELT>([&](auto ec) {
         constexpr int e = decltype(ec)::value;
         const uint32_t p = base + e * WG;
-        if (p >= m) return;
+        if (!FULL && p >= m) return;
         uint32_t o[4];
         compose_value(C, va[e], vb[e], na[e], nb[e], i1[e], iw[e], iz[e], o);
 #pragma unroll
         for (int k = 0; k < 4; ++k) lds[fuse_at(int(threadIdx.x) + e * WG, k)] = o[k];
         st4(J.layer0, p, u32x4{o[0], o[1], o[2], o[3]});
     });
+}
+__global__ __launch_bounds__(WG) void k_compose_leaves(ComposeJob inl, const ComposeJob* tab, uint32_t njobs) {
+    __shared__ uint32_t lds[FUSE_LDS];
+    uint32_t local;
+    const ComposeJob J = job_of(inl, tab, njobs, local);
+    const uint32_t m = 1u << (J.logn + J.logb);
+    const uint32_t first = local * FUSE_VALUES;
+    const uint32_t base = first + threadIdx.x;
+    const ComposeConsts C = *J.c;
+    if (first + uint32_t(FUSE_VALUES) <= m) compose_stage<true>(J, C, lds, base, m);
+    else compose_stage<false>(J, C, lds, base, m);
     __syncthreads();
     const int nvals = int(m - first < uint32_t(FUSE_VALUES) ? m - first : uint32_t(FUSE_VALUES));
     uint32_t par[8];
@@ -1707,7 +1717,7 @@ __global__ __launch_bounds__(WG) void k_fold(FoldJob inl, const FoldJob* tab, ui
 // then hashes them as 512 quad leaves of layer d+1 (and their parents). The
 // next output's loads are issued before each output's arithmetic (indices
 // clamped into the layer, results kept only for outputs that exist).
-template <bool BASE>
+template <bool BASE, bool FULL>
 __device__ __forceinline__ void fold_stage(const FoldJob& J, const FoldConsts& C, uint32_t* lds, uint64_t first,
                                            uint64_t outs) {
     const uint64_t i0 = first + threadIdx.x;
@@ -1721,7 +1731,7 @@ __device__ __forceinline__ void fold_stage(const FoldJob& J, const FoldConsts& C
             nxt = fold_load<BASE>(J.layer, J.finv, in < outs ? in : outs - 1);
         }
         const E4 v = fold_compute(cur, C);
-        if (i < outs) {
+        if (FULL || i < outs) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) lds[fuse_at(int(threadIdx.x) + e * WG, k)] = v.v[k];
             st4(J.next, i, u32x4{v.v[0], v.v[1], v.v[2], v.v[3]});
@@ -1736,8 +1746,15 @@ __global__ __launch_bounds__(WG) void k_fold_leaves(FoldJob inl, const FoldJob* 
     const uint64_t outs = uint64_t(1) << (J.log_len - 2);
     const uint64_t first = uint64_t(local) * FUSE_VALUES;
     const FoldConsts C = *J.c;
-    if (J.base_field) fold_stage<true>(J, C, lds, first, outs);
-    else fold_stage<false>(J, C, lds, first, outs);
+    // FULL: all 2048 outputs of this workgroup exist (every workgroup but a
+    // small layer's last).
+    if (first + FUSE_VALUES <= outs) {
+        if (J.base_field) fold_stage<true, true>(J, C, lds, first, outs);
+        else fold_stage<false, true>(J, C, lds, first, outs);
+    } else {
+        if (J.base_field) fold_stage<true, false>(J, C, lds, first, outs);
+        else fold_stage<false, false>(J, C, lds, first, outs);
+    }
     __syncthreads();
     const int nvals = int(outs - first < uint64_t(FUSE_VALUES) ? outs - first : uint64_t(FUSE_VALUES));
     uint32_t par[8];
