@@ -211,12 +211,18 @@ def batched(z, ctx, count, tune_reps, seed=2024):
     }
 
 
-def served(z, budget_gib, duration_s, schedule, single_schedule=None, seed=77):
+def served(z, budget_gib, duration_s, schedule, single_schedule=None, seed=77, tune_reps=0, wide_gib=0):
     """Open-loop Poisson arrivals of AIR proofs with log2 steps uniform in
     10..20 against a fixed HBM budget: saturated throughput first (all
     requests at once after one untimed saturated round, median of five
-    rounds), then three offered loads below it. The one-at-a-time comparison
-    runs under the single-proof schedule, as the batched comparison does."""
+    rounds), then three offered loads below it. With tune_reps, the kernel
+    schedule is first tuned (from `schedule`) on one batch of 64 requests of
+    the same mix, drawn from a seed of its own. The one-at-a-time comparison
+    runs under the single-proof schedule, as the batched comparison does.
+    With wide_gib, the same server is measured again under that larger
+    budget on the same saturated request sets (median of five rounds), and
+    p50/p99 at 0.8x of it: more proofs in flight amortise each iteration's
+    latency-bound launches, at the price of latency."""
     import random
     import threading
     rng = random.Random(seed)
@@ -273,10 +279,18 @@ def served(z, budget_gib, duration_s, schedule, single_schedule=None, seed=77):
     sctx = z.Context(0)
     if schedule:
         sctx.set_schedule(schedule)
+    tuned = None
+    if tune_reps > 0:
+        trng = random.Random(seed + 1)
+        sample = [seeded_instance(z, trng.randint(10, 20), trng.getrandbits(63), b"serve/tune/%d" % i)
+                  for i in range(64)]
+        tuned = tune(sctx, [p for p, _ in sample], tune_reps)
+    serve_schedule = sctx.get_schedule()
     levels = []
     with z.Server(sctx, hbm_budget=budget_gib << 30) as srv:
         run(srv, [seeded_instance(z, k, k, b"serve/warm/%d" % k) for k in range(10, 21)], [0.0] * 11)
         sat_n = 400
+        sat_state = rng.getstate()  # the wider budget replays these request sets
         run(srv, make(sat_n, "satwarm"), [0.0] * sat_n)
         sat = statistics.median([sat_n / run(srv, make(sat_n, "sat%d" % k), [0.0] * sat_n)[0] for k in range(5)])
         for frac in (0.5, 0.8, 0.95):
@@ -300,6 +314,30 @@ def served(z, budget_gib, duration_s, schedule, single_schedule=None, seed=77):
             z.air.prove(p, z.Transcript(l), sctx)
         rates.append(len(sample) / (time.perf_counter() - t0))
     single = statistics.median(rates)
+    wide = None
+    if wide_gib > 0:
+        wrng = random.Random()
+        wrng.setstate(sat_state)
+
+        def wmake(count, tag):
+            return [seeded_instance(z, wrng.randint(10, 20), wrng.getrandbits(63),
+                                    b"serve/%s/%d" % (tag.encode(), i)) for i in range(count)]
+        sctx.set_schedule(serve_schedule)
+        with z.Server(sctx, hbm_budget=wide_gib << 30) as srv:
+            run(srv, [seeded_instance(z, k, k, b"serve/warm/%d" % k) for k in range(10, 21)], [0.0] * 11)
+            run(srv, wmake(sat_n, "satwarm"), [0.0] * sat_n)
+            wsat = statistics.median([sat_n / run(srv, wmake(sat_n, "sat%d" % k), [0.0] * sat_n)[0] for k in range(5)])
+            qps = 0.8 * wsat
+            count = max(50, min(4000, int(qps * duration_s)))
+            t, arr = 0.0, []
+            for _ in range(count):
+                t += wrng.expovariate(qps)
+                arr.append(t)
+            wall, lat = run(srv, wmake(count, "q80"), arr)
+        wide = {"hbm_budget_gib": wide_gib, "saturated_proofs_per_s": round(wsat, 1),
+                "saturated_vs_one_at_a_time": round(wsat / single, 2),
+                "load_0p8": {"offered_qps": round(qps, 1), "requests": count, "proofs_per_s": round(count / wall, 1),
+                             "p50_ms": pct(lat, 0.50), "p99_ms": pct(lat, 0.99), "max_ms": round(lat[-1], 3)}}
     sctx.close()
     return {
         "workload": "Poisson AIR requests, steps 2^10..2^20 log-uniform, production schedule, HBM budget %d GiB"
@@ -309,8 +347,10 @@ def served(z, budget_gib, duration_s, schedule, single_schedule=None, seed=77):
         "saturated_vs_one_at_a_time": round(sat / single, 2),
         "requests": tally["requests"],
         "verified": tally["verified"],
-        "schedule": schedule,
+        "schedule": serve_schedule,
+        "autotune": tuned,
         "levels": levels,
+        "wider_budget": wide,
     }
 
 
@@ -609,6 +649,8 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="instances in the batched workload; 0 skips it")
     ap.add_argument("--serve-s", type=float, default=2.0, help="seconds of arrivals per served load; 0 skips it")
     ap.add_argument("--serve-budget-gib", type=int, default=32)
+    ap.add_argument("--serve-wide-gib", type=int, default=128,
+                    help="also measure the server under this larger HBM budget; 0 skips it")
     ap.add_argument("--tune-reps", type=int, default=5,
                     help="autotune the kernel schedule on each workload first (timed runs per candidate); "
                          "0 keeps the default schedule")
@@ -753,7 +795,9 @@ def main():
     single_sched = ctx.get_schedule()
     batch = batched(z, ctx, args.batch, 0 if args.schedule else max(1, args.tune_reps // 2)) if args.batch else None
     serve_sched = batch["schedule"] if batch else single_sched
-    serve = served(z, args.serve_budget_gib, args.serve_s, serve_sched, single_sched) if args.serve_s > 0 else None
+    serve = (served(z, args.serve_budget_gib, args.serve_s, serve_sched, single_sched,
+                    tune_reps=0 if args.schedule else max(1, args.tune_reps // 2), wide_gib=args.serve_wide_gib)
+             if args.serve_s > 0 else None)
     ctx.set_schedule(single_sched)
     wide = widened(z, ctx, args.widened, not args.no_cpu, args.widened_only) if args.widened > 0 else None
 
