@@ -217,7 +217,8 @@ def served(z, budget_gib, duration_s, schedule, single_schedule=None, seed=77, t
     requests at once after one untimed saturated round, median of five
     rounds), then three offered loads below it. With tune_reps, the kernel
     schedule is first tuned (from `schedule`) on one batch of 64 requests of
-    the same mix, drawn from a seed of its own. The one-at-a-time comparison
+    the same mix, drawn from a seed of its own, and the large lane's
+    schedule (from that one) on 32 requests of 2^16..2^20 steps. The one-at-a-time comparison
     runs under the single-proof schedule, as the batched comparison does.
     With wide_gib, the same server is measured again under that larger
     budget on the same saturated request sets (median of five rounds), and
@@ -286,8 +287,19 @@ def served(z, budget_gib, duration_s, schedule, single_schedule=None, seed=77, t
                   for i in range(64)]
         tuned = tune(sctx, [p for p, _ in sample], tune_reps)
     serve_schedule = sctx.get_schedule()
+    # The large lane (2^16 steps and up) carries most of the device time, so
+    # it gets a schedule of its own, tuned (from the served one) on a batch
+    # of 32 requests of its sizes drawn from a seed of its own.
+    large_sched, large_tuned = None, None
+    if tune_reps > 0:
+        lrng = random.Random(seed + 2)
+        lsample = [seeded_instance(z, lrng.randint(16, 20), lrng.getrandbits(63), b"serve/tune-large/%d" % i)
+                   for i in range(32)]
+        large_tuned = tune(sctx, [p for p, _ in lsample], tune_reps)
+        large_sched = sctx.get_schedule()
+        sctx.set_schedule(serve_schedule)
     levels = []
-    with z.Server(sctx, hbm_budget=budget_gib << 30) as srv:
+    with z.Server(sctx, hbm_budget=budget_gib << 30, large_schedule=large_sched) as srv:
         run(srv, [seeded_instance(z, k, k, b"serve/warm/%d" % k) for k in range(10, 21)], [0.0] * 11)
         sat_n = 400
         sat_state = rng.getstate()  # the wider budget replays these request sets
@@ -323,7 +335,7 @@ def served(z, budget_gib, duration_s, schedule, single_schedule=None, seed=77, t
             return [seeded_instance(z, wrng.randint(10, 20), wrng.getrandbits(63),
                                     b"serve/%s/%d" % (tag.encode(), i)) for i in range(count)]
         sctx.set_schedule(serve_schedule)
-        with z.Server(sctx, hbm_budget=wide_gib << 30) as srv:
+        with z.Server(sctx, hbm_budget=wide_gib << 30, large_schedule=large_sched) as srv:
             run(srv, [seeded_instance(z, k, k, b"serve/warm/%d" % k) for k in range(10, 21)], [0.0] * 11)
             run(srv, wmake(sat_n, "satwarm"), [0.0] * sat_n)
             wsat = statistics.median([sat_n / run(srv, wmake(sat_n, "sat%d" % k), [0.0] * sat_n)[0] for k in range(5)])
@@ -349,6 +361,8 @@ def served(z, budget_gib, duration_s, schedule, single_schedule=None, seed=77, t
         "verified": tally["verified"],
         "schedule": serve_schedule,
         "autotune": tuned,
+        "large_lane_schedule": large_sched,
+        "large_lane_autotune": large_tuned,
         "levels": levels,
         "wider_budget": wide,
     }

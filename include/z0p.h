@@ -482,6 +482,13 @@ typedef struct z0p_server z0p_server;
 /* hbm_budget 0: half of the free device memory; the small lane gets half of
  * the budget, at most 1 GiB. max_active 0: 256 per lane. */
 int z0p_server_create(z0p_ctx* ctx, uint64_t hbm_budget, uint32_t max_active, z0p_server** out);
+/* z0p_server_create with a kernel schedule of its own for the large lane
+ * (NULL: the ctx's, as z0p_server_create): the two lanes see different proof
+ * sizes, and a schedule tuned on 2^16..2^20-step batches runs the large lane
+ * faster than one tuned on the whole mix. Z0P_ERR_INVALID_ARGUMENT when a
+ * field is out of range. */
+int z0p_server_create_lanes(z0p_ctx* ctx, const z0p_schedule* large_schedule, uint64_t hbm_budget,
+                            uint32_t max_active, z0p_server** out);
 /* Queues one AIR proof. The transcript is copied: the request proves over
  * its own copy and the caller's transcript is left unchanged. */
 int z0p_server_submit_air(z0p_server* s, const z0p_air_params* params, const z0p_transcript* transcript,

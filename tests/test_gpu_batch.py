@@ -155,14 +155,14 @@ def test_batch_and_server_zero_copy_views(ctx):
     sctx.close()
 
 
-def _serve(requests, budget, gaps_s, waves=None):
+def _serve(requests, budget, gaps_s, waves=None, large_schedule=None):
     """Submit from a producer thread with the given inter-arrival gaps, poll
     from this thread; returns {request index: (proof, latency, queue)}."""
     sctx = z.Context(0)
     if waves is not None:
         sctx.set_waves(waves)
     out = {}
-    with z.Server(sctx, hbm_budget=budget) as srv:
+    with z.Server(sctx, hbm_budget=budget, large_schedule=large_schedule) as srv:
         ids = {}
 
         def producer():
@@ -295,3 +295,30 @@ def test_wave_streams_match_alone(ctx):
                 out[ids[r[0]]] = r[1]
     sctx.close()
     assert [out.get(i) for i in range(len(inst))] == alone
+
+
+def test_server_large_lane_schedule(ctx):
+    """A large lane under a schedule of its own (every fusion and launch knob
+    away from the default) returns each request's proof byte-identical to
+    z0p_air_prove under the context's schedule; an out-of-range large-lane
+    schedule is refused at creation."""
+    inst = _instances(14, 10, 18, 33)
+    params = [z.air.for_security(n, a0, b0, bl) for n, a0, b0, bl, _ in inst]
+    alone = [z.air.prove(p, z.Transcript(lab), ctx) for p, (*_, lab) in zip(params, inst)]
+    large = dict(ctx.get_schedule())
+    large.update(tail_max_log=12, tail_group_max_log=7, fold_fuse_max_log=22, subtree_levels=2, compose_subtree=1,
+                 levels_fuse=1, fold_commit_max_log=0, wave_streams=1, grind_pool=512)
+    assert large != ctx.get_schedule()
+    rng = random.Random(9)
+    gaps = [rng.expovariate(2000.0) for _ in inst]
+    print("serving with a large-lane schedule", sorted(p.n_steps.bit_length() - 1 for p in params), flush=True)
+    out = _serve([(p, lab) for p, (*_, lab) in zip(params, inst)], 4 << 30, gaps, large_schedule=large)
+    assert sorted(out) == list(range(len(inst)))
+    assert any(p.n_steps >= 1 << 16 for p in params) and any(p.n_steps < 1 << 16 for p in params)
+    for i, a in enumerate(alone):
+        assert out[i][0] == a, params[i].n_steps
+    bad = dict(large, levels_fuse=3)
+    sctx = z.Context(0)
+    with pytest.raises(z.Z0pError):
+        z.Server(sctx, hbm_budget=1 << 30, large_schedule=bad)
+    sctx.close()

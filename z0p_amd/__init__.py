@@ -210,11 +210,15 @@ class Server:
     requests arriving while others run join the running batch as device
     memory and slots free up. ``submit`` returns a request id; ``poll``
     returns ``(id, proof bytes or Z0pError, latency_ms, queue_ms)`` or None.
+    ``large_schedule`` (a schedule dict) runs the lane of 2^16 steps and up
+    under a schedule of its own; by default both lanes use the context's.
     """
 
-    def __init__(self, ctx: "Context", hbm_budget: int = 0, max_active: int = 0):
+    def __init__(self, ctx: "Context", hbm_budget: int = 0, max_active: int = 0, large_schedule: dict = None):
+        from ._ffi import ScheduleC
         h = ctypes.c_void_p()
-        check(lib().z0p_server_create(ctx.handle, hbm_budget, max_active, ctypes.byref(h)))
+        ls = ctypes.byref(ScheduleC.from_dict(large_schedule)) if large_schedule else None
+        check(lib().z0p_server_create_lanes(ctx.handle, ls, hbm_budget, max_active, ctypes.byref(h)))
         self._h = h
         self._ctx = ctx  # keep the context alive while the server runs
 

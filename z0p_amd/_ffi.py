@@ -202,6 +202,7 @@ SIGNATURES = {
     "z0p_air_prove_batch": (ctypes.c_int, [vp, ctypes.POINTER(AirParamsC), ctypes.POINTER(vp), sz,
                                            ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_int)]),
     "z0p_server_create": (ctypes.c_int, [vp, u64, u32, ctypes.POINTER(vp)]),
+    "z0p_server_create_lanes": (ctypes.c_int, [vp, ctypes.POINTER(ScheduleC), u64, u32, ctypes.POINTER(vp)]),
     "z0p_server_submit_air": (ctypes.c_int, [vp, ctypes.POINTER(AirParamsC), vp, ctypes.POINTER(u64)]),
     "z0p_server_poll": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_double),

@@ -13,7 +13,6 @@
 #include "params.hpp"
 #include "prover.hpp"
 
-
 namespace {
 
 using z0p::capi::guarded;
@@ -31,6 +30,27 @@ int emit(z0p::Bytes& v, uint8_t** out, size_t* len) {
 }
 
 }  // namespace
+
+bool z0p::schedule_from_c(const z0p_schedule& in, Schedule& s) {
+    s.tail_max_log = in.tail_max_log;
+    s.tail_group_max_log = in.tail_group_max_log;
+    s.tail_spread_log = in.tail_spread_log;
+    s.levels_fuse = in.levels_fuse;
+    s.small_max_log = in.small_max_log;
+    s.grind_blocks = in.grind_blocks;
+    s.fold_fuse_max_log = in.fold_fuse_max_log;
+    s.compose_fuse = in.compose_fuse;
+    s.trace_fuse = in.trace_fuse;
+    s.subtree_levels = in.subtree_levels;
+    s.compose_subtree = in.compose_subtree;
+    s.graph = in.graph;
+    s.lde_split = in.lde_split;
+    s.grind_pool = in.grind_pool;
+    s.fold_commit_max_log = in.fold_commit_max_log;
+    s.tree_top = in.tree_top;
+    s.wave_streams = in.wave_streams;
+    return s.valid();
+}
 
 extern "C" {
 
@@ -107,24 +127,7 @@ int z0p_ctx_get_schedule(z0p_ctx* ctx, z0p_schedule* out) {
 int z0p_ctx_set_schedule(z0p_ctx* ctx, const z0p_schedule* in) {
     if (!ctx || !in) return Z0P_ERR_INVALID_ARGUMENT;
     z0p::Schedule s;
-    s.tail_max_log = in->tail_max_log;
-    s.tail_group_max_log = in->tail_group_max_log;
-    s.tail_spread_log = in->tail_spread_log;
-    s.levels_fuse = in->levels_fuse;
-    s.small_max_log = in->small_max_log;
-    s.grind_blocks = in->grind_blocks;
-    s.fold_fuse_max_log = in->fold_fuse_max_log;
-    s.compose_fuse = in->compose_fuse;
-    s.trace_fuse = in->trace_fuse;
-    s.subtree_levels = in->subtree_levels;
-    s.compose_subtree = in->compose_subtree;
-    s.graph = in->graph;
-    s.lde_split = in->lde_split;
-    s.grind_pool = in->grind_pool;
-    s.fold_commit_max_log = in->fold_commit_max_log;
-    s.tree_top = in->tree_top;
-    s.wave_streams = in->wave_streams;
-    if (!s.valid()) return Z0P_ERR_INVALID_ARGUMENT;
+    if (!z0p::schedule_from_c(*in, s)) return Z0P_ERR_INVALID_ARGUMENT;
     ctx->c->sched = s;
     return Z0P_OK;
 }

@@ -155,6 +155,9 @@ struct TuneResult {
     std::string log;
 };
 TuneResult autotune(Ctx& c, const std::vector<z0p_air_params>& work, unsigned reps);
+// The C ABI's schedule struct as the scheduler's; false when a field is out
+// of range (capi.cpp).
+bool schedule_from_c(const z0p_schedule& in, Schedule& out);
 
 Bytes air_prove(Ctx& ctx, const z0p_air_params& p, Transcript& tr);
 Bytes air_prove_with_trace(Ctx& ctx, const z0p_air_params& p, const uint32_t* col_a, uint64_t len_a,

@@ -245,7 +245,14 @@ int z0p_air_prove_batch(z0p_ctx* ctx, const z0p_air_params* params, z0p_transcri
 }
 
 int z0p_server_create(z0p_ctx* ctx, uint64_t hbm_budget, uint32_t max_active, z0p_server** out) {
+    return z0p_server_create_lanes(ctx, nullptr, hbm_budget, max_active, out);
+}
+
+int z0p_server_create_lanes(z0p_ctx* ctx, const z0p_schedule* large_schedule, uint64_t hbm_budget,
+                            uint32_t max_active, z0p_server** out) {
     if (!ctx || !out) return Z0P_ERR_INVALID_ARGUMENT;
+    z0p::Schedule large_sched = ctx->c->sched;
+    if (large_schedule && !z0p::schedule_from_c(*large_schedule, large_sched)) return Z0P_ERR_INVALID_ARGUMENT;
     return guarded([&] {
         auto s = std::make_unique<z0p_server>();
         z0p::Ctx* small_ctx = ctx->c;
@@ -257,14 +264,16 @@ int z0p_server_create(z0p_ctx* ctx, uint64_t hbm_budget, uint32_t max_active, z0
         }
         s->max_active = max_active ? max_active : DEFAULT_MAX_ACTIVE;
         // The large lane gets its own context (stream, pool, tables) with the
-        // caller's kernel schedule and wave setting. The small lane's share of the budget is
-        // half of it, at most 1 GiB (a 2^15-step proof needs ~32 MiB).
+        // caller's wave setting and kernel schedule, or the schedule given for
+        // it (set before its scheduler thread starts). The small lane's share
+        // of the budget is half of it, at most 1 GiB (a 2^15-step proof needs
+        // ~32 MiB).
         Lane& small = s->lanes[0];
         Lane& large = s->lanes[1];
         small.c = small_ctx;
         large.c = z0p::ctx_create(small_ctx->device);
         large.owned = true;
-        large.c->sched = small_ctx->sched;
+        large.c->sched = large_sched;
         large.c->waves_min = small_ctx->waves_min;
         small.budget = std::min<size_t>(size_t(hbm_budget) / 2, size_t(1) << 30);
         large.budget = size_t(hbm_budget) - small.budget;
