@@ -152,15 +152,22 @@ def tune(ctx, params_list, reps):
             "trials": log.count(";"), "tune_s": round(time.perf_counter() - t0, 2), "log": log}
 
 
-def batched(z, ctx, count, tune_reps, seed=2024):
-    """256 independent proofs, log2 steps drawn uniformly from 10..18, proved
-    as one batch; beside it the same instances one at a time (under the
-    context's single-proof schedule)."""
+def batch_instances(z, count, seed=2024):
+    """The batched workload: `count` instances, log2 steps drawn uniformly
+    from 10..18 by splitmix64 from the seed."""
     st = seed
     inst = []
     for i in range(count):
         st, r = splitmix64(st)
         inst.append(seeded_instance(z, 10 + r % 9, seed * 7919 + i, b"batch/%d" % i))
+    return inst
+
+
+def batched(z, ctx, count, tune_reps, seed=2024):
+    """256 independent proofs, log2 steps drawn uniformly from 10..18, proved
+    as one batch; beside it the same instances one at a time (under the
+    context's single-proof schedule)."""
+    inst = batch_instances(z, count, seed)
     steps = sum(p.n_steps for p, _ in inst)
     single_sched = ctx.get_schedule()
     tuned = tune(ctx, [p for p, _ in inst], tune_reps)
@@ -637,6 +644,39 @@ def rocprof_summary(d, proofs, copy_mib):
     return obj, "\n".join(lines)
 
 
+def copy_kernel(ctx, mib):
+    """The plain copy kernel twice on `mib` MiB (the counters' calibration and
+    an achievable-bandwidth reference); the second run's figures."""
+    import ctypes
+    from z0p_amd import _ffi
+    cms = ctypes.c_float()
+    nbytes = mib << 20
+    _ffi.check(_ffi.lib().z0p_test_copy(ctx.handle, nbytes, ctypes.byref(cms)))
+    _ffi.check(_ffi.lib().z0p_test_copy(ctx.handle, nbytes, ctypes.byref(cms)))
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": round(cms.value, 4),
+            "gbs_rw": round(2 * nbytes / (cms.value * 1e-3) / 1e9, 1)}
+
+
+def profile_batch(z, args):
+    """--profile-batch: the batched workload alone, for a kernel profile of
+    the batch (per-proof figures = run totals / (instances x REPS))."""
+    ctx = z.Context(0)
+    if args.schedule:
+        ctx.set_schedule(json.loads(args.schedule))
+    inst = batch_instances(z, args.batch)
+    z.air.prove_batch([p for p, _ in inst], [z.Transcript(l) for _, l in inst], ctx, zero_copy=True)
+    t0 = time.perf_counter()
+    for _ in range(args.profile_batch):
+        got = z.air.prove_batch([p for p, _ in inst], [z.Transcript(l) for _, l in inst], ctx, zero_copy=True)
+        assert all(isinstance(g, memoryview) for g in got)
+    dt = (time.perf_counter() - t0) / args.profile_batch
+    copy = copy_kernel(ctx, args.copy_mib) if args.copy_mib else None
+    print(json.dumps({"profile_batch": {"instances": len(inst), "reps": args.profile_batch,
+                                        "ms_per_batch": round(dt * 1e3, 3), "schedule": ctx.get_schedule(),
+                                        "copy": copy}}))
+    ctx.close()
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--rocprof-summary":
         sp = argparse.ArgumentParser()
@@ -677,6 +717,10 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=256,
                     help="size of the plain copy kernel run after the proofs (counter calibration and "
                          "achievable-bandwidth reference); 0 disables")
+    ap.add_argument("--profile-batch", type=int, default=0, metavar="REPS",
+                    help="profiling run only: the batched workload (--batch instances) REPS times after one "
+                         "warm-up batch, under --schedule or the default schedule, then the copy kernel twice; "
+                         "prints one line and exits")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: proofs are independent units and this build runs on a single "
@@ -685,6 +729,9 @@ def main():
 
     import z0p_amd as z
 
+    if args.profile_batch:
+        profile_batch(z, args)
+        return
     n = args.n
     bl = z.air.b_last(1, 1, n)
     params = z.air.for_security(n, 1, 1, bl)
@@ -738,16 +785,7 @@ def main():
     verify_single = {"proofs": vcount, "host_threads": os.cpu_count(),
                      "host_proofs_per_s": round(vcount / t_vc, 1), "device_paths_proofs_per_s": round(vcount / t_vd, 1)}
 
-    copy = None
-    if args.copy_mib:
-        import ctypes
-        from z0p_amd import _ffi
-        cms = ctypes.c_float()
-        nbytes = args.copy_mib << 20
-        _ffi.check(_ffi.lib().z0p_test_copy(ctx.handle, nbytes, ctypes.byref(cms)))
-        _ffi.check(_ffi.lib().z0p_test_copy(ctx.handle, nbytes, ctypes.byref(cms)))
-        copy = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": round(cms.value, 4),
-                "gbs_rw": round(2 * nbytes / (cms.value * 1e-3) / 1e9, 1)}
+    copy = copy_kernel(ctx, args.copy_mib) if args.copy_mib else None
 
     m = n * params.blowup
     layers = []
